@@ -1,0 +1,164 @@
+/* C ABI of the gfx950 KDK Schroedinger-Poisson library (libqjamd.so).
+ *
+ * The reference (pmocz/quantum-jax) has no FFI: its operator surface is a set
+ * of module-level Python functions over jnp arrays that read module globals.
+ * Each entry point below replaces one of them (file:line cited); the globals
+ * become the plan (grid, box, precision) and explicit scalar arguments.
+ *
+ * Conventions
+ *  - Arrays are device pointers in C order (nx, ny, nz), z contiguous.
+ *    "complex" = interleaved (re, im) of the plan's real type; "real" = the
+ *    plan's real type (float for QJ_C64, double for QJ_C128).  Particle
+ *    positions, velocities, masses and accelerations are always double,
+ *    shape (np, 3) row-major.
+ *  - The caller owns every buffer; the plan owns only twiddle/phase tables
+ *    and a few bytes of reduction scratch.  Operators that need temporary
+ *    fields take a caller-provided workspace of qj_workspace_size() bytes.
+ *  - Every call is asynchronous on `stream` (a hipStream_t, NULL = default
+ *    stream) and performs no host synchronisation.  Scalars the reference
+ *    returns to the host (a_max, dt_s) are written to a caller-owned device
+ *    array of QJ_DIAG_COUNT doubles (`diag`).
+ *  - Return value 0 on success, negative on error (QJ_E*); the message of the
+ *    last error on this host thread is returned by qj_last_error().  Nothing
+ *    aborts.  A plan must not be used concurrently from two streams.
+ *  - Grid sizes: each of nx, ny, nz a power of two in [8, 512].
+ */
+#ifndef QJ_API_H
+#define QJ_API_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QJ_C64 0
+#define QJ_C128 1
+
+#define QJ_OK 0
+#define QJ_EARG (-1)
+#define QJ_EUNSUPPORTED (-2)
+#define QJ_EHIP (-3)
+
+/* slots of the device diagnostics array (doubles) */
+#define QJ_DIAG_AMAX 0 /* running max |a_grid|        quantum-stars.py:260,327 */
+#define QJ_DIAG_DTS 1  /* min |dx / vel|              quantum-stars.py:328     */
+#define QJ_DIAG_MASS 2 /* FP64 sum of the last Poisson source over cells     */
+#define QJ_DIAG_NAN 3  /* 1.0 once a non-finite source value was seen        */
+#define QJ_DIAG_COUNT 8
+
+typedef struct qj_plan qj_plan;
+
+/* Mesh, spectral setup and precision: replaces the module globals of
+ * quantum-stars.py:72-75,131-149 (nx,ny,nz, Lx,Ly,Lz, kx,ky,kz,k_sq) and the
+ * x64 switch of :69-70.  k is generated inside the kernels (FFT order,
+ * Nyquist bin negative as in :142-148). */
+int qj_plan_create(qj_plan** plan, int nx, int ny, int nz, double Lx, double Ly, double Lz,
+                   int dtype);
+int qj_plan_destroy(qj_plan* plan);
+size_t qj_workspace_size(const qj_plan* plan);
+const char* qj_last_error(void);
+int qj_version(void);
+
+/* get_potential(rho) -> V; quantum-stars.py:189-193, quantum-fluid.py:171-175.
+ * dc_mode 0: the k = 0 bin follows the reference (divided by 1, not zeroed),
+ * evaluated from an FP64 sum of rho; dc_mode 1: zero-mean V. */
+int qj_get_potential(qj_plan* plan, const void* rho, double rho_bar, double four_pi_G,
+                     double b_coeff, int dc_mode, void* V, void* work, void* stream);
+
+/* a_d = -Re IFFT(1j k_d V_hat) on the grid (3 real planes ax|ay|az) and V;
+ * max |a| is folded into diag[QJ_DIAG_AMAX].  quantum-stars.py:254-260,
+ * quantum-fluid.py:268-272. */
+int qj_acceleration_grid(qj_plan* plan, const void* rho, double rho_bar, double four_pi_G,
+                         int dc_mode, void* V, void* a3, double* diag, void* work, void* stream);
+
+/* psi <- exp(-1j coef V) psi; coef = m_per_hbar*dt/2. quantum-stars.py:301,318 */
+int qj_kick(qj_plan* plan, void* psi, const void* V, double coef, void* stream);
+
+/* psi <- IFFT(exp(-1j dt k^2/(2 m_per_hbar)) FFT psi); quantum-stars.py:307-309 */
+int qj_drift(qj_plan* plan, void* psi, double dt, double m_per_hbar, void* stream);
+
+/* get_cic_indices_and_weights(pos); quantum-stars.py:200-209 (FP64, bit-exact
+ * indices; i/ip1 int32 (np,3), weights double (np,3)). */
+int qj_cic_indices(qj_plan* plan, const double* pos, long long np, int* i, int* ip1, double* w_i,
+                   double* w_ip1, void* stream);
+
+/* bin_particles(pos, m) / bin_stars(pos); quantum-bh.py:223-258,
+ * quantum-stars.py:212-247.  mass may be NULL (then every particle has
+ * m_scalar).  accumulate = 0 clears rho first. */
+int qj_bin_particles(qj_plan* plan, const double* pos, const double* mass, double m_scalar,
+                     long long np, void* rho, int accumulate, void* stream);
+
+/* 8-corner CIC interpolation of the planes a3 = ax|ay|az to the particles;
+ * quantum-stars.py:262-287.  acc is double (np, 3). */
+int qj_interp(qj_plan* plan, const double* pos, long long np, const void* a3, double* acc,
+              void* stream);
+
+/* get_acceleration(pos, rho) -> (acc, a_max); quantum-stars.py:250-288.
+ * a3 (3 real planes) receives the grid acceleration. */
+int qj_get_acceleration(qj_plan* plan, const double* pos, long long np, const void* rho,
+                        double rho_bar, double four_pi_G, int dc_mode, double* acc, void* a3,
+                        double* diag, void* work, void* stream);
+
+/* apply_grav_accel(rho, vx, vy, vz, dt); quantum-fluid.py:267-278 (in place
+ * on vx, vy, vz; a3 is 3 real planes of scratch that receive the grid
+ * acceleration). */
+int qj_apply_grav_accel(qj_plan* plan, const void* rho, void* vx, void* vy, void* vz, double dt,
+                        double rho_bar, double four_pi_G, int dc_mode, void* a3, double* diag,
+                        void* work, void* stream);
+
+/* solve_hydro(rho, vx, vy, vz, dt); quantum-fluid.py:281-349 (cubic cells;
+ * out-of-place: the *_o buffers must not alias the inputs). */
+int qj_solve_hydro(qj_plan* plan, const void* rho, const void* vx, const void* vy, const void* vz,
+                   double dt, double cs, void* rho_o, void* vx_o, void* vy_o, void* vz_o,
+                   void* stream);
+
+/* ---- fused stepping: compute_step + update + fori_loop ----------------
+ * quantum-stars.py:295-351,454 (stars), quantum-fluid.py:414-459 (gas) and
+ * the particle-free step.  The step kind follows the state: stars when
+ * np > 0, gas when gas[0] != NULL, plain otherwise.
+ *
+ * The second half-kick of a step and the first of the next one use the same
+ * potential (|psi|^2, star positions and gas density do not change between
+ * them), so the field is kept with a pending kick: after qj_kdk_steps the
+ * psi (and gas velocity) buffers lack the last half kick, which
+ * qj_kdk_finalize applies.  Star velocities are always complete.
+ */
+typedef struct {
+  void* psi;        /* complex N                                              */
+  void* rho_s;      /* real N, star density scratch (must start zeroed)       */
+  void* gas[4];     /* rho, vx, vy, vz: real N each (gas step) or NULL        */
+  void* gas_tmp[4]; /* second buffers for the out-of-place gas update         */
+  double* pos;      /* (np,3)                                                 */
+  double* vel;      /* (np,3)                                                 */
+  const double* mass; /* (np) or NULL                                         */
+  double m_scalar;  /* star mass when mass == NULL                            */
+  long long np;
+  void* grid;       /* 4 real planes V|ax|ay|az                               */
+  void* brho;       /* real N, b*rho for self-interaction; NULL when b == 0   */
+  double* diag;     /* QJ_DIAG_COUNT doubles                                  */
+  /* host-side bookkeeping, initialise to 0 */
+  int grid_valid;   /* grid holds V and a of the current state                */
+  int gas_swapped;  /* the live gas fields are gas_tmp (after an odd count)   */
+  double pending;   /* time of the owed kick of psi and gas velocity          */
+} qj_state;
+
+typedef struct {
+  double dt;
+  double m_per_hbar;
+  double four_pi_G;
+  double rho_bar;
+  double b_coeff;
+  double cs;
+  int dc_mode;
+} qj_params;
+
+int qj_kdk_steps(qj_plan* plan, qj_state* state, const qj_params* prm, int nsteps, void* work,
+                 void* stream);
+int qj_kdk_finalize(qj_plan* plan, qj_state* state, const qj_params* prm, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QJ_API_H */

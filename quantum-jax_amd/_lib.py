@@ -1,0 +1,99 @@
+"""ctypes binding of libqjamd.so (the C ABI declared in include/qj_api.h).
+
+The library is required: there is no fallback.  If it has not been built,
+``lib()`` raises with instructions instead of silently running anything else.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libqjamd.so")
+
+QJ_C64 = 0
+QJ_C128 = 1
+DIAG_AMAX, DIAG_DTS, DIAG_MASS, DIAG_NAN, DIAG_COUNT = 0, 1, 2, 3, 8
+
+vp = C.c_void_p
+dp = C.POINTER(C.c_double)
+i64 = C.c_longlong
+f64 = C.c_double
+i32 = C.c_int
+
+
+class QjState(C.Structure):
+    """Mirror of ``qj_state``."""
+    _fields_ = [
+        ("psi", vp), ("rho_s", vp), ("gas", vp * 4), ("gas_tmp", vp * 4),
+        ("pos", vp), ("vel", vp), ("mass", vp), ("m_scalar", f64), ("np", i64),
+        ("grid", vp), ("brho", vp), ("diag", vp),
+        ("grid_valid", i32), ("gas_swapped", i32), ("pending", f64),
+    ]
+
+
+class QjParams(C.Structure):
+    """Mirror of ``qj_params``."""
+    _fields_ = [
+        ("dt", f64), ("m_per_hbar", f64), ("four_pi_G", f64), ("rho_bar", f64),
+        ("b_coeff", f64), ("cs", f64), ("dc_mode", i32),
+    ]
+
+
+# name -> (restype, argtypes); every entry point returns int status unless noted
+SIGNATURES = {
+    "qj_plan_create": (i32, [C.POINTER(vp), i32, i32, i32, f64, f64, f64, i32]),
+    "qj_plan_destroy": (i32, [vp]),
+    "qj_workspace_size": (C.c_size_t, [vp]),
+    "qj_last_error": (C.c_char_p, []),
+    "qj_version": (i32, []),
+    "qj_get_potential": (i32, [vp, vp, f64, f64, f64, i32, vp, vp, vp]),
+    "qj_acceleration_grid": (i32, [vp, vp, f64, f64, i32, vp, vp, vp, vp, vp]),
+    "qj_kick": (i32, [vp, vp, vp, f64, vp]),
+    "qj_drift": (i32, [vp, vp, f64, f64, vp]),
+    "qj_cic_indices": (i32, [vp, vp, i64, vp, vp, vp, vp, vp]),
+    "qj_bin_particles": (i32, [vp, vp, vp, f64, i64, vp, i32, vp]),
+    "qj_interp": (i32, [vp, vp, i64, vp, vp, vp]),
+    "qj_get_acceleration": (i32, [vp, vp, i64, vp, f64, f64, i32, vp, vp, vp, vp, vp]),
+    "qj_apply_grav_accel": (i32, [vp, vp, vp, vp, vp, f64, f64, f64, i32, vp, vp, vp, vp]),
+    "qj_solve_hydro": (i32, [vp, vp, vp, vp, vp, f64, f64, vp, vp, vp, vp, vp]),
+    "qj_kdk_steps": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), i32, vp, vp]),
+    "qj_kdk_finalize": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), vp]),
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+class QjError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libqjamd.so (once).  Raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise QjError(
+                f"{LIB_PATH} is missing: build the gfx950 library first "
+                "(python quantum-jax_amd/build.py or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        msg = lib().qj_last_error().decode(errors="replace")
+        kind = {-1: "bad argument", -2: "unsupported", -3: "HIP error"}.get(rc, str(rc))
+        raise QjError(f"{what}: {kind}: {msg}")

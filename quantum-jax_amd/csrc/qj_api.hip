@@ -1,0 +1,509 @@
+// C-ABI layer: plan management, argument checking and the composition of
+// the kernels into the reference operators and the fused KDK step.
+#include <math.h>
+#include <string.h>
+
+#include <string>
+
+#include "../../include/qj_api.h"
+#include "qj_common.h"
+#include "qj_kernels.h"
+
+using namespace qj;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int hipfail(hipError_t e, const char* where) {
+  g_err = std::string(where) + ": " + hipGetErrorString(e);
+  return QJ_EHIP;
+}
+
+#define QJ_CHECK(expr, where)                    \
+  do {                                           \
+    hipError_t e_ = (expr);                      \
+    if (e_ != hipSuccess) return hipfail(e_, where); \
+  } while (0)
+
+bool pow2_ok(int n) { return n >= 8 && n <= 512 && (n & (n - 1)) == 0; }
+
+template <class T>
+__global__ void k_fill(T* a, long long n, T v) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) a[i] = v;
+}
+
+template <class T>
+hipError_t fill(void* a, long long n, T v, hipStream_t s) {
+  long long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  k_fill<T><<<(int)blocks, 256, 0, s>>>((T*)a, n, v);
+  return hipGetLastError();
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace
+
+struct qj_plan {
+  Dims d;
+  int dtype;
+  int device;
+  size_t rsize;  // bytes of one real
+  void* tw[3] = {nullptr, nullptr, nullptr};
+  void* drift[3] = {nullptr, nullptr, nullptr};
+  double drift_key_dt = NAN, drift_key_mph = NAN;
+  double* partials = nullptr;
+  unsigned int* counter = nullptr;
+  int nblk_src = 0;
+
+  long long ncell() const { return (long long)d.nx * d.ny * d.nz; }
+  long long nhalf() const { return (long long)(d.nx / 2 + 1) * d.ny * d.nz; }
+};
+
+namespace {
+
+struct Work {
+  void* H;
+  void* Uz;
+  void* Uy;
+  void* R;
+};
+
+Work carve(const qj_plan* p, void* work) {
+  char* w = (char*)work;
+  const size_t hb = align256(p->nhalf() * 2 * p->rsize);
+  Work k;
+  k.H = w;
+  k.Uz = w + hb;
+  k.Uy = w + 2 * hb;
+  k.R = w + 3 * hb;
+  return k;
+}
+
+template <class T>
+SpecTables<T> tables(const qj_plan* p) {
+  SpecTables<T> t;
+  t.tw_x = p->tw[0];
+  t.tw_y = p->tw[1];
+  t.tw_z = p->tw[2];
+  t.drift_x = p->drift[0];
+  t.drift_y = p->drift[1];
+  t.drift_z = p->drift[2];
+  return t;
+}
+
+int check_plan(const qj_plan* p) {
+  if (!p) return fail(QJ_EARG, "null plan");
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev != p->device)
+    return fail(QJ_EARG, "current HIP device differs from the plan's device");
+  return QJ_OK;
+}
+
+template <class T>
+int ensure_drift(qj_plan* p, double dt, double mph, hipStream_t s) {
+  if (p->drift_key_dt == dt && p->drift_key_mph == mph) return QJ_OK;
+  const double c = dt / (2.0 * mph);  // quantum-stars.py:308: dt * k^2 / m_per_hbar / 2
+  const int n[3] = {p->d.nx, p->d.ny, p->d.nz};
+  const double L[3] = {p->d.Lx, p->d.Ly, p->d.Lz};
+  for (int a = 0; a < 3; ++a)
+    QJ_CHECK(launch_drift_table<T>(p->drift[a], n[a], L[a], c, s), "drift table");
+  p->drift_key_dt = dt;
+  p->drift_key_mph = mph;
+  return QJ_OK;
+}
+
+// Poisson chain after the source pass: H -> grid (V [, a]).
+template <class T>
+int poisson_tail(qj_plan* p, const Work& w, int grad, double four_pi_G, double rho_bar,
+                 int dc_mode, void* V, void* ax, void* ay, void* az, const void* addend,
+                 double* diag, hipStream_t s) {
+  SpecTables<T> tb = tables<T>(p);
+  QJ_CHECK(launch_fft_y_half<T>(p->d, tb, w.H, s), "fft_y");
+  QJ_CHECK(launch_poisson_z<T>(p->d, tb, grad, w.H, w.Uz, four_pi_G, rho_bar, dc_mode, diag, s),
+           "poisson_z");
+  QJ_CHECK(launch_poisson_y<T>(p->d, tb, grad, w.H, w.Uz, w.Uy, s), "poisson_y");
+  QJ_CHECK(launch_poisson_x<T>(p->d, tb, grad, w.H, w.Uy, w.Uz, V, ax, ay, az, addend, diag, s),
+           "poisson_x");
+  return QJ_OK;
+}
+
+template <class T>
+void* plane(void* base, const qj_plan* p, int k) {
+  return (char*)base + (size_t)k * p->ncell() * sizeof(T);
+}
+
+// ---------------------------------------------------------------------------
+template <class T>
+int get_potential_t(qj_plan* p, const void* rho, double rho_bar, double four_pi_G, double b,
+                    int dc_mode, void* V, void* work, hipStream_t s) {
+  Work w = carve(p, work);
+  SpecTables<T> tb = tables<T>(p);
+  void* brho = (b != 0.0) ? w.R : nullptr;
+  // diag scratch: the source mass lands in the plan-owned partials' tail
+  double* diag = p->partials + p->nblk_src;
+  QJ_CHECK(launch_source_x<T>(p->d, tb, 0, nullptr, (void*)rho, 0, brho, b, rho_bar, w.H,
+                              p->partials, p->counter, diag, s),
+           "source_x");
+  return poisson_tail<T>(p, w, 0, four_pi_G, rho_bar, dc_mode, V, nullptr, nullptr, nullptr, brho,
+                         diag, s);
+}
+
+template <class T>
+int accel_grid_t(qj_plan* p, const void* rho, double rho_bar, double four_pi_G, int dc_mode,
+                 void* V, void* a3, double* diag, void* work, hipStream_t s) {
+  Work w = carve(p, work);
+  SpecTables<T> tb = tables<T>(p);
+  // the source mass and max|a| land in the caller's diag, else in plan scratch
+  double* dg = diag ? diag : p->partials + p->nblk_src;
+  QJ_CHECK(launch_source_x<T>(p->d, tb, 0, nullptr, (void*)rho, 0, nullptr, 0.0, rho_bar, w.H,
+                              p->partials, p->counter, dg, s),
+           "source_x");
+  void* Vout = V ? V : w.R;  // the pass always writes V
+  return poisson_tail<T>(p, w, 1, four_pi_G, rho_bar, dc_mode, Vout, plane<T>(a3, p, 0),
+                         plane<T>(a3, p, 1), plane<T>(a3, p, 2), nullptr, dg, s);
+}
+
+}  // namespace
+
+// ===========================================================================
+// exported entry points
+// ===========================================================================
+extern "C" {
+
+const char* qj_last_error(void) { return g_err.c_str(); }
+
+int qj_version(void) { return 1; }
+
+int qj_plan_create(qj_plan** out, int nx, int ny, int nz, double Lx, double Ly, double Lz,
+                   int dtype) {
+  if (!out) return fail(QJ_EARG, "null output pointer");
+  *out = nullptr;
+  if (!pow2_ok(nx) || !pow2_ok(ny) || !pow2_ok(nz))
+    return fail(QJ_EUNSUPPORTED, "grid sizes must be powers of two in [8, 512]");
+  if (!(Lx > 0) || !(Ly > 0) || !(Lz > 0)) return fail(QJ_EARG, "box lengths must be positive");
+  if (dtype != QJ_C64 && dtype != QJ_C128) return fail(QJ_EARG, "dtype must be QJ_C64 or QJ_C128");
+  qj_plan* p = new qj_plan();
+  p->d = Dims{nx, ny, nz, Lx, Ly, Lz};
+  p->dtype = dtype;
+  p->rsize = dtype == QJ_C64 ? sizeof(float) : sizeof(double);
+  hipError_t e = hipGetDevice(&p->device);
+  const int n[3] = {nx, ny, nz};
+  for (int a = 0; a < 3 && e == hipSuccess; ++a) {
+    e = hipMalloc(&p->tw[a], (size_t)n[a] * 2 * p->rsize);
+    if (e == hipSuccess) e = hipMalloc(&p->drift[a], (size_t)n[a] * 2 * p->rsize);
+  }
+  p->nblk_src = source_x_blocks(p->d);
+  if (e == hipSuccess)
+    e = hipMalloc((void**)&p->partials, (size_t)(p->nblk_src + kDiagCount) * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&p->counter, 256);
+  if (e == hipSuccess) e = hipMemset(p->counter, 0, 256);
+  if (e == hipSuccess)
+    e = hipMemset(p->partials, 0, (size_t)(p->nblk_src + kDiagCount) * sizeof(double));
+  for (int a = 0; a < 3 && e == hipSuccess; ++a) {
+    e = dtype == QJ_C64 ? launch_twiddles<float>(p->tw[a], n[a], nullptr)
+                        : launch_twiddles<double>(p->tw[a], n[a], nullptr);
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    qj_plan_destroy(p);
+    return hipfail(e, "qj_plan_create");
+  }
+  *out = p;
+  return QJ_OK;
+}
+
+int qj_plan_destroy(qj_plan* p) {
+  if (!p) return QJ_OK;
+  for (int a = 0; a < 3; ++a) {
+    if (p->tw[a]) (void)hipFree(p->tw[a]);
+    if (p->drift[a]) (void)hipFree(p->drift[a]);
+  }
+  if (p->partials) (void)hipFree(p->partials);
+  if (p->counter) (void)hipFree(p->counter);
+  delete p;
+  return QJ_OK;
+}
+
+size_t qj_workspace_size(const qj_plan* p) {
+  if (!p) return 0;
+  return 3 * align256(p->nhalf() * 2 * p->rsize) + align256(p->ncell() * p->rsize);
+}
+
+#define QJ_PRECISION(p, call_f, call_d) ((p)->dtype == QJ_C64 ? (call_f) : (call_d))
+
+int qj_get_potential(qj_plan* p, const void* rho, double rho_bar, double four_pi_G,
+                     double b_coeff, int dc_mode, void* V, void* work, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!rho || !V || !work) return fail(QJ_EARG, "qj_get_potential: null buffer");
+  hipStream_t s = (hipStream_t)stream;
+  return QJ_PRECISION(p, get_potential_t<float>(p, rho, rho_bar, four_pi_G, b_coeff, dc_mode, V, work, s),
+                      get_potential_t<double>(p, rho, rho_bar, four_pi_G, b_coeff, dc_mode, V, work, s));
+}
+
+int qj_acceleration_grid(qj_plan* p, const void* rho, double rho_bar, double four_pi_G,
+                         int dc_mode, void* V, void* a3, double* diag, void* work, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!rho || !a3 || !work) return fail(QJ_EARG, "qj_acceleration_grid: null buffer");
+  hipStream_t s = (hipStream_t)stream;
+  return QJ_PRECISION(p, accel_grid_t<float>(p, rho, rho_bar, four_pi_G, dc_mode, V, a3, diag, work, s),
+                      accel_grid_t<double>(p, rho, rho_bar, four_pi_G, dc_mode, V, a3, diag, work, s));
+}
+
+int qj_kick(qj_plan* p, void* psi, const void* V, double coef, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!psi || !V) return fail(QJ_EARG, "qj_kick: null buffer");
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = QJ_PRECISION(p, launch_kick<float>(psi, V, nullptr, coef, p->ncell(), s),
+                              launch_kick<double>(psi, V, nullptr, coef, p->ncell(), s));
+  return e == hipSuccess ? QJ_OK : hipfail(e, "qj_kick");
+}
+
+int qj_drift(qj_plan* p, void* psi, double dt, double m_per_hbar, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!psi) return fail(QJ_EARG, "qj_drift: null buffer");
+  if (!(m_per_hbar != 0.0)) return fail(QJ_EARG, "qj_drift: m_per_hbar must be non-zero");
+  hipStream_t s = (hipStream_t)stream;
+  if (p->dtype == QJ_C64) {
+    if (int rc = ensure_drift<float>(p, dt, m_per_hbar, s)) return rc;
+    SpecTables<float> tb = tables<float>(p);
+    for (int a = 2; a >= 0; --a)
+      QJ_CHECK(launch_drift_axis<float>(p->d, tb, a, psi, nullptr, nullptr, 0.0, 0, s), "drift");
+  } else {
+    if (int rc = ensure_drift<double>(p, dt, m_per_hbar, s)) return rc;
+    SpecTables<double> tb = tables<double>(p);
+    for (int a = 2; a >= 0; --a)
+      QJ_CHECK(launch_drift_axis<double>(p->d, tb, a, psi, nullptr, nullptr, 0.0, 0, s), "drift");
+  }
+  return QJ_OK;
+}
+
+int qj_cic_indices(qj_plan* p, const double* pos, long long np, int* i, int* ip1, double* w_i,
+                   double* w_ip1, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (np < 0 || (np > 0 && (!pos || !i || !ip1 || !w_i || !w_ip1)))
+    return fail(QJ_EARG, "qj_cic_indices: bad arguments");
+  hipError_t e = launch_cic_indices<float>(p->d, pos, np, i, ip1, w_i, w_ip1, (hipStream_t)stream);
+  return e == hipSuccess ? QJ_OK : hipfail(e, "qj_cic_indices");
+}
+
+int qj_bin_particles(qj_plan* p, const double* pos, const double* mass, double m_scalar,
+                     long long np, void* rho, int accumulate, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!rho || np < 0 || (np > 0 && !pos)) return fail(QJ_EARG, "qj_bin_particles: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = hipSuccess;
+  if (!accumulate)
+    e = QJ_PRECISION(p, fill<float>(rho, p->ncell(), 0.0f, s), fill<double>(rho, p->ncell(), 0.0, s));
+  if (e == hipSuccess)
+    e = QJ_PRECISION(p, launch_deposit<float>(p->d, pos, mass, m_scalar, np, rho, s),
+                     launch_deposit<double>(p->d, pos, mass, m_scalar, np, rho, s));
+  return e == hipSuccess ? QJ_OK : hipfail(e, "qj_bin_particles");
+}
+
+int qj_interp(qj_plan* p, const double* pos, long long np, const void* a3, double* acc,
+              void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (np < 0 || (np > 0 && (!pos || !a3 || !acc))) return fail(QJ_EARG, "qj_interp: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e;
+  if (p->dtype == QJ_C64)
+    e = launch_interp<float>(p->d, pos, np, plane<float>((void*)a3, p, 0),
+                             plane<float>((void*)a3, p, 1), plane<float>((void*)a3, p, 2), acc, s);
+  else
+    e = launch_interp<double>(p->d, pos, np, plane<double>((void*)a3, p, 0),
+                              plane<double>((void*)a3, p, 1), plane<double>((void*)a3, p, 2), acc,
+                              s);
+  return e == hipSuccess ? QJ_OK : hipfail(e, "qj_interp");
+}
+
+int qj_get_acceleration(qj_plan* p, const double* pos, long long np, const void* rho,
+                        double rho_bar, double four_pi_G, int dc_mode, double* acc, void* a3,
+                        double* diag, void* work, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!rho || !a3 || !work || np < 0 || (np > 0 && (!pos || !acc)))
+    return fail(QJ_EARG, "qj_get_acceleration: bad arguments");
+  int rc = qj_acceleration_grid(p, rho, rho_bar, four_pi_G, dc_mode, nullptr, a3, diag, work, stream);
+  if (rc) return rc;
+  return qj_interp(p, pos, np, a3, acc, stream);
+}
+
+int qj_apply_grav_accel(qj_plan* p, const void* rho, void* vx, void* vy, void* vz, double dt,
+                        double rho_bar, double four_pi_G, int dc_mode, void* a3, double* diag,
+                        void* work, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!rho || !vx || !vy || !vz || !a3 || !work)
+    return fail(QJ_EARG, "qj_apply_grav_accel: null buffer");
+  int rc = qj_acceleration_grid(p, rho, rho_bar, four_pi_G, dc_mode, nullptr, a3, diag, work, stream);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e;
+  if (p->dtype == QJ_C64)
+    e = launch_gas_kick<float>(p->ncell(), vx, vy, vz, plane<float>(a3, p, 0),
+                               plane<float>(a3, p, 1), plane<float>(a3, p, 2), dt, s);
+  else
+    e = launch_gas_kick<double>(p->ncell(), vx, vy, vz, plane<double>(a3, p, 0),
+                                plane<double>(a3, p, 1), plane<double>(a3, p, 2), dt, s);
+  return e == hipSuccess ? QJ_OK : hipfail(e, "qj_apply_grav_accel");
+}
+
+int qj_solve_hydro(qj_plan* p, const void* rho, const void* vx, const void* vy, const void* vz,
+                   double dt, double cs, void* rho_o, void* vx_o, void* vy_o, void* vz_o,
+                   void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!rho || !vx || !vy || !vz || !rho_o || !vx_o || !vy_o || !vz_o)
+    return fail(QJ_EARG, "qj_solve_hydro: null buffer");
+  if (p->d.nx != p->d.ny || p->d.nx != p->d.nz || p->d.Lx != p->d.Ly || p->d.Lx != p->d.Lz)
+    return fail(QJ_EUNSUPPORTED, "qj_solve_hydro: the gas update needs a cubic grid and box");
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = QJ_PRECISION(
+      p,
+      launch_hydro<float>(p->d, rho, vx, vy, vz, nullptr, nullptr, nullptr, 0.0, dt, cs, rho_o,
+                          vx_o, vy_o, vz_o, s),
+      launch_hydro<double>(p->d, rho, vx, vy, vz, nullptr, nullptr, nullptr, 0.0, dt, cs, rho_o,
+                           vx_o, vy_o, vz_o, s));
+  return e == hipSuccess ? QJ_OK : hipfail(e, "qj_solve_hydro");
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// fused KDK stepping
+// ===========================================================================
+namespace {
+
+template <class T>
+int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void* work,
+                hipStream_t s) {
+  Work w = carve(p, work);
+  SpecTables<T> tb = tables<T>(p);
+  if (int rc = ensure_drift<T>(p, pr->dt, pr->m_per_hbar, s)) return rc;
+  const bool stars = st->np > 0;
+  const bool gas = st->gas[0] != nullptr;
+  const bool grad = stars || gas;
+  void* V = plane<T>(st->grid, p, 0);
+  void* ax = grad ? plane<T>(st->grid, p, 1) : nullptr;
+  void* ay = grad ? plane<T>(st->grid, p, 2) : nullptr;
+  void* az = grad ? plane<T>(st->grid, p, 3) : nullptr;
+  void* brho = (pr->b_coeff != 0.0) ? st->brho : nullptr;
+  const double dt = pr->dt;
+  auto live_gas = [&](int k) { return st->gas_swapped ? st->gas_tmp[k] : st->gas[k]; };
+  auto spare_gas = [&](int k) { return st->gas_swapped ? st->gas[k] : st->gas_tmp[k]; };
+  void* src = nullptr;
+
+  for (int step = 0; step < nsteps; ++step) {
+    if (!st->grid_valid) {
+      // potential (and acceleration) of the current state
+      if (stars)
+        QJ_CHECK(launch_deposit<T>(p->d, st->pos, st->mass, st->m_scalar, st->np, st->rho_s, s),
+                 "deposit");
+      src = stars ? st->rho_s : (gas ? live_gas(0) : nullptr);
+      QJ_CHECK(launch_source_x<T>(p->d, tb, 0, st->psi, src, stars ? 1 : 0, brho, pr->b_coeff,
+                                  pr->rho_bar, w.H, p->partials, p->counter, st->diag, s),
+               "source_x");
+      if (int rc = poisson_tail<T>(p, w, grad, pr->four_pi_G, pr->rho_bar, pr->dc_mode, V, ax, ay,
+                                   az, nullptr, st->diag, s))
+        return rc;
+      st->grid_valid = 1;
+    }
+    const double kick_t = st->pending + 0.5 * dt;  // owed half kick + this step's first
+    // stars: half kick (cached a), drift, deposit at the new positions
+    if (stars)
+      QJ_CHECK(launch_star_kick_drift_deposit<T>(p->d, st->pos, st->vel, st->np, ax, ay, az,
+                                                 0.5 * dt, dt, st->mass, st->m_scalar, st->rho_s,
+                                                 s),
+               "star kick/drift/deposit");
+    // gas: owed + first half kick of the velocity, then the MUSCL update
+    if (gas) {
+      QJ_CHECK(launch_hydro<T>(p->d, live_gas(0), live_gas(1), live_gas(2), live_gas(3), ax, ay, az,
+                               kick_t, dt, pr->cs, spare_gas(0), spare_gas(1), spare_gas(2),
+                               spare_gas(3), s),
+               "hydro");
+      st->gas_swapped ^= 1;
+    }
+    // psi: kick folded into the z drift sweep, then y and x (+ Poisson source)
+    QJ_CHECK(launch_drift_axis<T>(p->d, tb, 2, st->psi, V, brho, pr->m_per_hbar * kick_t, 1, s),
+             "drift z");
+    QJ_CHECK(launch_drift_axis<T>(p->d, tb, 1, st->psi, nullptr, nullptr, 0.0, 0, s), "drift y");
+    src = stars ? st->rho_s : (gas ? live_gas(0) : nullptr);
+    QJ_CHECK(launch_source_x<T>(p->d, tb, 1, st->psi, src, stars ? 1 : 0, brho, pr->b_coeff,
+                                pr->rho_bar, w.H, p->partials, p->counter, st->diag, s),
+             "drift x + source");
+    if (int rc = poisson_tail<T>(p, w, grad, pr->four_pi_G, pr->rho_bar, pr->dc_mode, V, ax, ay, az,
+                                 nullptr, st->diag, s))
+      return rc;
+    // stars: second half kick with the new acceleration; dt_s
+    if (stars) {
+      QJ_CHECK(fill<double>(st->diag + kDiagDtS, 1, __builtin_inf(), s), "dt_s reset");
+      QJ_CHECK(launch_star_kick<T>(p->d, st->pos, st->vel, st->np, ax, ay, az, 0.5 * dt, st->diag,
+                                   s),
+               "star kick");
+    }
+    st->pending = 0.5 * dt;
+  }
+  return QJ_OK;
+}
+
+template <class T>
+int kdk_finalize_t(qj_plan* p, qj_state* st, const qj_params* pr, hipStream_t s) {
+  if (st->pending == 0.0) return QJ_OK;
+  void* V = plane<T>(st->grid, p, 0);
+  void* brho = (pr->b_coeff != 0.0) ? st->brho : nullptr;
+  QJ_CHECK(launch_kick<T>(st->psi, V, brho, pr->m_per_hbar * st->pending, p->ncell(), s), "kick");
+  if (st->gas[0]) {
+    void* g1 = st->gas_swapped ? st->gas_tmp[1] : st->gas[1];
+    void* g2 = st->gas_swapped ? st->gas_tmp[2] : st->gas[2];
+    void* g3 = st->gas_swapped ? st->gas_tmp[3] : st->gas[3];
+    QJ_CHECK(launch_gas_kick<T>(p->ncell(), g1, g2, g3, plane<T>(st->grid, p, 1),
+                                plane<T>(st->grid, p, 2), plane<T>(st->grid, p, 3), st->pending, s),
+             "gas kick");
+  }
+  st->pending = 0.0;
+  return QJ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qj_kdk_steps(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void* work,
+                 void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!st || !pr || !work || nsteps < 0) return fail(QJ_EARG, "qj_kdk_steps: bad arguments");
+  if (!st->psi || !st->grid || !st->diag) return fail(QJ_EARG, "qj_kdk_steps: null state buffer");
+  if (st->np > 0 && (!st->pos || !st->vel || !st->rho_s))
+    return fail(QJ_EARG, "qj_kdk_steps: stars need pos, vel and rho_s");
+  if (st->gas[0] && (!st->gas[1] || !st->gas[2] || !st->gas[3] || !st->gas_tmp[0] ||
+                     !st->gas_tmp[1] || !st->gas_tmp[2] || !st->gas_tmp[3]))
+    return fail(QJ_EARG, "qj_kdk_steps: gas needs rho, vx, vy, vz and their second buffers");
+  if (st->gas[0] && st->np > 0)
+    return fail(QJ_EUNSUPPORTED, "qj_kdk_steps: stars and gas in one state");
+  if (st->gas[0] && (p->d.nx != p->d.ny || p->d.nx != p->d.nz || p->d.Lx != p->d.Ly ||
+                     p->d.Lx != p->d.Lz))
+    return fail(QJ_EUNSUPPORTED, "qj_kdk_steps: the gas update needs a cubic grid and box");
+  if (pr->b_coeff != 0.0 && !st->brho)
+    return fail(QJ_EARG, "qj_kdk_steps: self-interaction needs the brho buffer");
+  if (!(pr->m_per_hbar != 0.0)) return fail(QJ_EARG, "qj_kdk_steps: m_per_hbar must be non-zero");
+  hipStream_t s = (hipStream_t)stream;
+  return p->dtype == QJ_C64 ? kdk_steps_t<float>(p, st, pr, nsteps, work, s)
+                            : kdk_steps_t<double>(p, st, pr, nsteps, work, s);
+}
+
+int qj_kdk_finalize(qj_plan* p, qj_state* st, const qj_params* pr, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!st || !pr) return fail(QJ_EARG, "qj_kdk_finalize: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  return p->dtype == QJ_C64 ? kdk_finalize_t<float>(p, st, pr, s)
+                            : kdk_finalize_t<double>(p, st, pr, s);
+}
+
+}  // extern "C"

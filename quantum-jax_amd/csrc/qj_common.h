@@ -1,0 +1,157 @@
+// Common device helpers for the gfx950 KDK Schroedinger-Poisson kernels:
+// a plain complex type, compile-time DFT twiddles, FP64 phase range
+// reduction and small reduction utilities.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <type_traits>
+
+namespace qj {
+
+// ---------------------------------------------------------------------------
+// complex value (interleaved re/im, 8 B for float, 16 B for double)
+// ---------------------------------------------------------------------------
+template <class T>
+struct alignas(2 * sizeof(T)) cpx {
+  T r, i;
+};
+
+template <class T>
+__device__ __forceinline__ cpx<T> operator+(cpx<T> a, cpx<T> b) { return {a.r + b.r, a.i + b.i}; }
+template <class T>
+__device__ __forceinline__ cpx<T> operator-(cpx<T> a, cpx<T> b) { return {a.r - b.r, a.i - b.i}; }
+template <class T>
+__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
+  return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r};
+}
+template <class T>
+__device__ __forceinline__ cpx<T> cmulc(cpx<T> a, cpx<T> b) {  // a * conj(b)
+  return {a.r * b.r + a.i * b.i, a.i * b.r - a.r * b.i};
+}
+template <class T>
+__device__ __forceinline__ cpx<T> conjg(cpx<T> a) { return {a.r, -a.i}; }
+template <class T>
+__device__ __forceinline__ cpx<T> scale(cpx<T> a, T s) { return {a.r * s, a.i * s}; }
+
+// ---------------------------------------------------------------------------
+// compile-time loop
+// ---------------------------------------------------------------------------
+template <int I, int E, class F>
+__device__ __forceinline__ void sfor(F&& f) {
+  if constexpr (I < E) {
+    f(std::integral_constant<int, I>{});
+    sfor<I + 1, E>(f);
+  }
+}
+
+// cos(pi*k/32), k = 0..16 (the first quadrant of the 64-point circle).
+constexpr double kCosQ[17] = {
+    1.0, 0.9951847266721969, 0.9807852804032304, 0.9569403357322088,
+    0.9238795325112867, 0.881921264348355, 0.8314696123025452, 0.773010453362737,
+    0.7071067811865476, 0.6343932841636455, 0.5555702330196023, 0.4713967368259978,
+    0.38268343236508984, 0.29028467725446233, 0.19509032201612833, 0.09801714032956077,
+    0.0};
+
+// cos(2*pi*k/64) for any integer k, from the quadrant table by symmetry.
+constexpr double cos64(int k) {
+  k = ((k % 64) + 64) % 64;
+  if (k <= 16) return kCosQ[k];
+  if (k <= 32) return -kCosQ[32 - k];
+  if (k <= 48) return -kCosQ[k - 32];
+  return kCosQ[64 - k];
+}
+constexpr double sin64(int k) { return cos64(k - 16); }
+
+// v * exp(DIR * 2*pi*i*M/E), M and E compile-time (E divides 64).
+// DIR = -1 is the forward (numpy) sign convention.
+template <int M, int E, int DIR, class T>
+__device__ __forceinline__ cpx<T> twc(cpx<T> v) {
+  constexpr int m = ((M % E) + E) % E;
+  if constexpr (m == 0) {
+    return v;
+  } else if constexpr (2 * m == E) {
+    return {-v.r, -v.i};
+  } else if constexpr (4 * m == E) {  // exp(DIR*i*pi/2) = DIR*i
+    if constexpr (DIR < 0) return {v.i, -v.r};
+    else return {-v.i, v.r};
+  } else if constexpr (4 * m == 3 * E) {  // exp(DIR*i*3pi/2) = -DIR*i
+    if constexpr (DIR < 0) return {-v.i, v.r};
+    else return {v.i, -v.r};
+  } else {
+    static_assert(64 % E == 0, "register DFT size must divide 64");
+    constexpr T c = (T)cos64(m * (64 / E));
+    constexpr T s = (T)(DIR * sin64(m * (64 / E)));
+    return {v.r * c - v.i * s, v.r * s + v.i * c};
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 phase handling.  Cody-Waite reduction of an FP64 phase to [-pi, pi]
+// with a two-part 2*pi constant and FMA (the product n*2pi_hi is formed
+// exactly inside the fused multiply-add), then sine/cosine in T.
+// ---------------------------------------------------------------------------
+constexpr double kTwoPiHi = 6.283185307179586232e+00;   // nearest double to 2*pi
+constexpr double kTwoPiLo = 2.449293598294706e-16;     // 2*pi - kTwoPiHi
+constexpr double kInvTwoPi = 0.15915494309189535;
+
+__device__ __forceinline__ double reduce_2pi(double th) {
+  double n = rint(th * kInvTwoPi);
+  double r = fma(-n, kTwoPiHi, th);
+  return fma(-n, kTwoPiLo, r);
+}
+
+// exp(-i*theta) with an FP64 argument
+template <class T>
+__device__ __forceinline__ cpx<T> expmi(double th) {
+  double r = reduce_2pi(th);
+  if constexpr (std::is_same<T, float>::value) {
+    float s, c;
+    sincosf((float)r, &s, &c);
+    return {c, -s};
+  } else {
+    double s, c;
+    sincos(r, &s, &c);
+    return {c, -s};
+  }
+}
+
+// ---------------------------------------------------------------------------
+// atomics on non-negative IEEE values (bit patterns order like the values)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void atomic_max_nonneg(float* addr, float v) {
+  atomicMax(reinterpret_cast<unsigned int*>(addr), __float_as_uint(v));
+}
+__device__ __forceinline__ void atomic_max_nonneg(double* addr, double v) {
+  atomicMax(reinterpret_cast<unsigned long long*>(addr),
+            (unsigned long long)__double_as_longlong(v));
+}
+__device__ __forceinline__ void atomic_min_nonneg(double* addr, double v) {
+  atomicMin(reinterpret_cast<unsigned long long*>(addr),
+            (unsigned long long)__double_as_longlong(v));
+}
+
+// wavefront (64-lane) reductions
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+template <class T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Slots of the device-side diagnostics word array (doubles).
+enum DiagSlot {
+  kDiagAMax = 0,     // running max |a_grid| (quantum-stars.py:260,327)
+  kDiagDtS = 1,      // min |dx / vel| (quantum-stars.py:328)
+  kDiagMass = 2,     // FP64 sum of the last Poisson source over cells
+  kDiagNaN = 3,      // non-zero once a NaN/Inf was seen in the source
+  kDiagCount = 8
+};
+
+}  // namespace qj

@@ -1,0 +1,160 @@
+// Register/LDS pencil FFT for gfx950.
+//
+// An N-point pencil (N a power of two, 8..512) is spread over TT = N/E
+// threads; thread j holds the E elements x[j + TT*m], m = 0..E-1.  The
+// transform is a two-stage Stockham decomposition N = E * TT:
+//   stage 1: an E-point DFT in registers over m (stride TT), twiddle
+//            W_N^(j*k1);
+//   exchange through LDS (one write, one read, block barrier);
+//   stage 2: E/TT independent TT-point DFTs in registers.
+// Output element k1 + E*k2 lands in thread (k1 % TT), register
+// (k1 / TT) + (E/TT)*k2, i.e. in exactly the input distribution
+// "thread j holds k = j + TT*m".  A forward transform, a diagonal k-space
+// multiply and an inverse transform therefore chain in registers with no
+// extra data movement.
+#pragma once
+
+#include "qj_common.h"
+
+namespace qj {
+
+// Elements per thread for an N-point pencil.
+template <int N>
+struct PencilShape {
+  static_assert((N & (N - 1)) == 0 && N >= 8 && N <= 512, "pencil length must be 2^k in [8,512]");
+  static constexpr int E = (N <= 16) ? N : (N <= 256 ? 16 : 32);
+  static constexpr int TT = N / E;        // threads per pencil
+  static constexpr int kThreads = 256;    // workgroup size
+  static constexpr int C = kThreads / TT; // pencils per workgroup
+  static constexpr int P = TT + 1;        // padded LDS row (k1-major)
+  static constexpr int kLds = (TT > 1) ? C * E * P : 1;  // complex elements
+};
+
+// In-register E-point DFT, natural order in and out (radix-2/4 four-step
+// recursion with compile-time twiddles).
+template <int E, int DIR, class T>
+__device__ __forceinline__ void dft(cpx<T>* v) {
+  if constexpr (E == 1) {
+    return;
+  } else if constexpr (E == 2) {
+    cpx<T> a = v[0], b = v[1];
+    v[0] = a + b;
+    v[1] = a - b;
+  } else if constexpr (E == 4) {
+    cpx<T> t0 = v[0] + v[2], t1 = v[0] - v[2];
+    cpx<T> t2 = v[1] + v[3];
+    cpx<T> t3 = twc<1, 4, DIR>(v[1] - v[3]);
+    v[0] = t0 + t2;
+    v[2] = t0 - t2;
+    v[1] = t1 + t3;
+    v[3] = t1 - t3;
+  } else {
+    constexpr int A = (E >= 16) ? 4 : 2;
+    constexpr int B = E / A;
+    cpx<T> y[B][A];
+    // step 1: A-point DFTs over x[B*a + b], then twiddle W_E^(b*ka)
+    sfor<0, B>([&](auto bc) {
+      constexpr int b = decltype(bc)::value;
+      cpx<T> t[A];
+      sfor<0, A>([&](auto ac) {
+        constexpr int a = decltype(ac)::value;
+        t[a] = v[B * a + b];
+      });
+      dft<A, DIR, T>(t);
+      sfor<0, A>([&](auto kc) {
+        constexpr int ka = decltype(kc)::value;
+        y[b][ka] = twc<b * ka, E, DIR>(t[ka]);
+      });
+    });
+    // step 2: B-point DFTs over b -> X[ka + A*kb]
+    sfor<0, A>([&](auto kc) {
+      constexpr int ka = decltype(kc)::value;
+      cpx<T> t[B];
+      sfor<0, B>([&](auto bc) {
+        constexpr int b = decltype(bc)::value;
+        t[b] = y[b][ka];
+      });
+      dft<B, DIR, T>(t);
+      sfor<0, B>([&](auto kb) {
+        constexpr int k = decltype(kb)::value;
+        v[ka + A * k] = t[k];
+      });
+    });
+  }
+}
+
+// N-point pencil transform over the TT threads that hold one pencil.
+//   v    : the thread's E elements (x[j + TT*m])
+//   j    : thread rank inside the pencil, 0..TT-1
+//   lds  : this pencil's private LDS slice (E*P complex elements)
+//   tw   : W_N^p = exp(-2*pi*i*p/N), p = 0..N-1 (conjugated for DIR = +1)
+// Every thread of the workgroup must call this (it contains barriers).
+template <int N, int DIR, class T>
+__device__ __forceinline__ void pencil_fft(cpx<T>* v, int j, cpx<T>* lds,
+                                           const cpx<T>* __restrict__ tw) {
+  using S = PencilShape<N>;
+  constexpr int E = S::E, TT = S::TT, P = S::P;
+  dft<E, DIR, T>(v);
+  if constexpr (TT > 1) {
+#pragma unroll
+    for (int k1 = 1; k1 < E; ++k1) {
+      cpx<T> w = tw[(j * k1) & (N - 1)];
+      v[k1] = (DIR < 0) ? cmul(v[k1], w) : cmulc(v[k1], w);
+    }
+    __syncthreads();  // previous readers of this LDS slice are done
+#pragma unroll
+    for (int k1 = 0; k1 < E; ++k1) lds[k1 * P + j] = v[k1];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < E / TT; ++q) {
+      cpx<T> u[TT];
+      const int k1 = j + TT * q;
+#pragma unroll
+      for (int b = 0; b < TT; ++b) u[b] = lds[k1 * P + b];
+      dft<TT, DIR, T>(u);
+#pragma unroll
+      for (int kb = 0; kb < TT; ++kb) v[q + (E / TT) * kb] = u[kb];
+    }
+  }
+}
+
+// Signed frequency index of bin k of an n-point transform, Nyquist negative
+// (quantum-stars.py:142-148: ifftshift(arange(-n/2, n/2))).
+__device__ __forceinline__ int freq_index(int k, int n) { return (k < n / 2) ? k : k - n; }
+
+// Pass geometry: pencil q starts at
+//   base(q) = (q >> inner_log2) * s_outer + (q & (2^inner_log2 - 1)) * s_inner
+// and its elements are s_elem apart.
+struct PassGeom {
+  long long s_outer;
+  long long s_inner;
+  long long s_elem;
+  int inner_log2;
+  int ncols;
+};
+
+__device__ __forceinline__ long long pencil_base(const PassGeom& g, int q) {
+  return (long long)(q >> g.inner_log2) * g.s_outer +
+         (long long)(q & ((1 << g.inner_log2) - 1)) * g.s_inner;
+}
+
+// Thread -> (pencil slot, rank) mapping inside a 256-thread workgroup.
+// STRIDED: pencil slot fastest across lanes (neighbouring pencils are
+// neighbouring addresses: x and y passes).  CONTIGUOUS: rank fastest
+// (a pencil's elements are neighbouring addresses: z passes).
+template <int N, bool CONTIG>
+struct Lane {
+  int col, j;
+  __device__ __forceinline__ Lane(int tid) {
+    using S = PencilShape<N>;
+    if constexpr (CONTIG) {
+      j = tid % S::TT;
+      col = tid / S::TT;
+    } else {
+      col = tid % S::C;
+      j = tid / S::C;
+    }
+  }
+};
+
+}  // namespace qj

@@ -1,0 +1,101 @@
+// Internal host-side launcher interface between the C-ABI layer and the
+// kernel translation units.  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace qj {
+
+struct Dims {
+  int nx, ny, nz;
+  double Lx, Ly, Lz;
+};
+
+// Per-plan device tables the spectral passes read.
+template <class T>
+struct SpecTables {
+  const void* tw_x;  // W_nx^p, p < nx   (cpx<T>)
+  const void* tw_y;
+  const void* tw_z;
+  void* drift_x;     // exp(-i dt k^2/(2 mph)) / n per axis (cpx<T>)
+  void* drift_y;
+  void* drift_z;
+};
+
+// Poisson source: rho = |psi|^2 + src (either may be null); source_x also
+// performs the x drift sweep on psi when drift != 0.
+template <class T>
+hipError_t launch_twiddles(void* tw, int n, hipStream_t s);
+template <class T>
+hipError_t launch_drift_table(void* tab, int n, double L, double c, hipStream_t s);
+
+template <class T>
+hipError_t launch_kick(void* psi, const void* V, const void* Vb, double coef, long long n,
+                       hipStream_t s);
+
+template <class T>
+hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, void* psi,
+                             const void* V, const void* Vb, double kick_coef, int kick,
+                             hipStream_t s);
+
+template <class T>
+hipError_t launch_source_x(const Dims& d, const SpecTables<T>& tb, int drift, void* psi,
+                           void* src, int clear_src, void* brho, double b_coeff, double rho_bar,
+                           void* H, double* partials, unsigned int* counter, double* diag,
+                           hipStream_t s);
+
+template <class T>
+hipError_t launch_fft_y_half(const Dims& d, const SpecTables<T>& tb, void* H, hipStream_t s);
+
+template <class T>
+hipError_t launch_poisson_z(const Dims& d, const SpecTables<T>& tb, int grad, void* H, void* Uz,
+                            double four_pi_G, double rho_bar, int dc_mode, const double* diag,
+                            hipStream_t s);
+
+template <class T>
+hipError_t launch_poisson_y(const Dims& d, const SpecTables<T>& tb, int grad, void* H, void* Uz,
+                            void* Uy, hipStream_t s);
+
+template <class T>
+hipError_t launch_poisson_x(const Dims& d, const SpecTables<T>& tb, int grad, const void* H,
+                            const void* Uy, const void* Uzz, void* V, void* ax, void* ay,
+                            void* az, const void* addend, double* diag, hipStream_t s);
+
+// Number of workgroups source_x launches (size of the FP64 partials array).
+int source_x_blocks(const Dims& d);
+
+// ---- particles ----------------------------------------------------------
+template <class T>
+hipError_t launch_cic_indices(const Dims& d, const double* pos, long long np, int* i, int* ip1,
+                              double* w_i, double* w_ip1, hipStream_t s);
+template <class T>
+hipError_t launch_deposit(const Dims& d, const double* pos, const double* mass, double m_scalar,
+                          long long np, void* rho, hipStream_t s);
+template <class T>
+hipError_t launch_interp(const Dims& d, const double* pos, long long np, const void* ax,
+                         const void* ay, const void* az, double* acc, hipStream_t s);
+// vel += interp(a)*h1; pos = mod(pos + vel*dt, L); deposit(pos) into rho
+template <class T>
+hipError_t launch_star_kick_drift_deposit(const Dims& d, double* pos, double* vel, long long np,
+                                          const void* ax, const void* ay, const void* az,
+                                          double h1, double dt, const double* mass,
+                                          double m_scalar, void* rho, hipStream_t s);
+// vel += interp(a)*h2; diag[dt_s] = min |dx/vel|
+template <class T>
+hipError_t launch_star_kick(const Dims& d, const double* pos, double* vel, long long np,
+                            const void* ax, const void* ay, const void* az, double h2,
+                            double* diag, hipStream_t s);
+
+// ---- gas ----------------------------------------------------------------
+// Applies v += a*kick first when kick != 0 (a may be null when kick == 0).
+template <class T>
+hipError_t launch_hydro(const Dims& d, const void* rho, const void* vx, const void* vy,
+                        const void* vz, const void* ax, const void* ay, const void* az,
+                        double kick, double dt, double cs, void* rho_o, void* vx_o, void* vy_o,
+                        void* vz_o, hipStream_t s);
+template <class T>
+hipError_t launch_gas_kick(long long n, void* vx, void* vy, void* vz, const void* ax,
+                           const void* ay, const void* az, double h, hipStream_t s);
+
+}  // namespace qj

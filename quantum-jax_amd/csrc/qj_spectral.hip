@@ -1,0 +1,688 @@
+// Spectral pass kernels of the KDK step: kinetic drift sweeps, the Poisson
+// source pass, the half-spectrum Poisson passes with fused spectral
+// gradients, and the pointwise potential kick.
+//
+// Field layout in HBM: C-ordered (nx, ny, nz), z contiguous, complex values
+// interleaved.  The Poisson solve works on a half spectrum along x,
+// H[kx][y][z] with kx = 0..nx/2 (real input => Hermitian along x).
+//
+// Sweep plan of one spectral operator:
+//   drift  : IFFT_a * diag(D_a) * FFT_a for a = z, y, x.  exp(-i dt |k|^2/(2 mph))
+//            factorises over the axes, so each axis is one read+write sweep.
+//   Poisson: x (rho -> half spectrum, fused with the x drift sweep) ->
+//            y (FFT) -> z (FFT, -4 pi G/k^2, IFFT; dV/dz spectrum) ->
+//            y (IFFT; dV/dy spectrum) -> x (Hermitian-completed IFFT to
+//            V, a_x, a_y, a_z and max |a|).
+#include "qj_fft.h"
+#include "qj_kernels.h"
+
+namespace qj {
+
+namespace {
+
+__host__ __device__ inline int ilog2(int v) {
+  int r = 0;
+  while ((1 << r) < v) ++r;
+  return r;
+}
+
+template <class T>
+__device__ __forceinline__ cpx<T> ld(const cpx<T>* p, int i) { return p[i]; }
+
+// ---------------------------------------------------------------------------
+// tables
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void k_twiddles(cpx<T>* tw, int n) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  double s, c;
+  sincospi(2.0 * (double)p / (double)n, &s, &c);
+  tw[p] = {(T)c, (T)(-s)};
+}
+
+// D[k] = exp(-i * c * kphys^2) / n, kphys = 2 pi/L * freq(k); c = dt/(2 mph).
+// quantum-stars.py:308 (one axis of the separable kinetic phase).
+template <class T>
+__global__ void k_drift_table(cpx<T>* tab, int n, double L, double c) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double kp = 2.0 * M_PI / L * (double)freq_index(k, n);
+  double th = reduce_2pi(c * kp * kp);
+  double s, co;
+  sincos(th, &s, &co);
+  double inv = 1.0 / (double)n;
+  tab[k] = {(T)(co * inv), (T)(-s * inv)};
+}
+
+// ---------------------------------------------------------------------------
+// pointwise kick: psi <- exp(-i coef (V + Vb)) psi  (quantum-stars.py:301)
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_kick(cpx<T>* __restrict__ psi, const T* __restrict__ V,
+                                              const T* __restrict__ Vb, double coef,
+                                              long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    double v = (double)V[i];
+    if (Vb) v += (double)Vb[i];
+    psi[i] = cmul(psi[i], expmi<T>(coef * v));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// drift sweep along one axis, optionally preceded by the potential kick
+// ---------------------------------------------------------------------------
+template <class T, int N, bool CONTIG, bool KICK>
+__global__ __launch_bounds__(256) void k_drift(cpx<T>* __restrict__ psi, PassGeom g,
+                                               const cpx<T>* __restrict__ D,
+                                               const cpx<T>* __restrict__ tw,
+                                               const T* __restrict__ V,
+                                               const T* __restrict__ Vb, double kick_coef) {
+  using S = PencilShape<N>;
+  __shared__ cpx<T> lds[S::kLds];
+  Lane<N, CONTIG> ln(threadIdx.x);
+  const int q = blockIdx.x * S::C + ln.col;
+  const bool act = q < g.ncols;
+  const int base = act ? (int)pencil_base(g, q) : 0;
+  const int se = (int)g.s_elem;
+  cpx<T> v[S::E];
+#pragma unroll
+  for (int m = 0; m < S::E; ++m) {
+    const int idx = base + (ln.j + S::TT * m) * se;
+    v[m] = act ? psi[idx] : cpx<T>{0, 0};
+    if constexpr (KICK) {
+      if (act) {
+        double th = (double)V[idx];
+        if (Vb) th += (double)Vb[idx];
+        v[m] = cmul(v[m], expmi<T>(kick_coef * th));
+      }
+    }
+  }
+  cpx<T>* my = lds + ln.col * S::E * S::P;
+  pencil_fft<N, -1, T>(v, ln.j, my, tw);
+#pragma unroll
+  for (int m = 0; m < S::E; ++m) v[m] = cmul(v[m], D[ln.j + S::TT * m]);
+  pencil_fft<N, +1, T>(v, ln.j, my, tw);
+  if (act) {
+#pragma unroll
+    for (int m = 0; m < S::E; ++m) psi[base + (ln.j + S::TT * m) * se] = v[m];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// block-level FP64 sum with a deterministic last-block finish
+// ---------------------------------------------------------------------------
+__device__ void block_sum_finish(double local, bool bad, double* partials, unsigned int* counter,
+                                 double* diag) {
+  __shared__ double red[4];
+  __shared__ int flag[4];
+  __shared__ bool last;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double s = wave_sum(local);
+  int anybad = __any(bad);
+  if (lane == 0) {
+    red[wv] = s;
+    flag[wv] = anybad;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bs = (red[0] + red[1]) + (red[2] + red[3]);
+    partials[blockIdx.x] = bs;
+    if (flag[0] | flag[1] | flag[2] | flag[3])
+      atomicExch(reinterpret_cast<unsigned long long*>(&diag[kDiagNaN]),
+                 (unsigned long long)__double_as_longlong(1.0));
+    __threadfence();
+    unsigned int t = atomicAdd(counter, 1u);
+    last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (last) {
+    __threadfence();
+    // Neumaier-compensated sum of the block partials
+    double sum = 0.0, comp = 0.0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) {
+      double x = __builtin_nontemporal_load(&partials[i]);
+      double t = sum + x;
+      comp += (fabs(sum) >= fabs(x)) ? (sum - t) + x : (x - t) + sum;
+      sum = t;
+    }
+    __shared__ double hs[256], hc[256];
+    hs[threadIdx.x] = sum;
+    hc[threadIdx.x] = comp;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double S = 0.0, Cc = 0.0;
+      for (int i = 0; i < (int)blockDim.x; ++i) {
+        double x = hs[i];
+        double t = S + x;
+        Cc += (fabs(S) >= fabs(x)) ? (S - t) + x : (x - t) + S;
+        S = t;
+        Cc += hc[i];
+      }
+      diag[kDiagMass] = S + Cc;
+      *counter = 0u;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Poisson source pass along x (optionally fused with the x drift sweep):
+//   rho = |psi|^2 + src; FP64 mass; forward x-FFT of (rho - rho_bar);
+//   store kx = 0..nx/2 into H.
+// ---------------------------------------------------------------------------
+template <class T, int N, bool DRIFT>
+__global__ __launch_bounds__(256) void k_source_x(
+    cpx<T>* __restrict__ psi, T* __restrict__ src, int clear_src, T* __restrict__ brho,
+    double b_coeff, double rho_bar, PassGeom g, const cpx<T>* __restrict__ D,
+    const cpx<T>* __restrict__ tw, cpx<T>* __restrict__ H, double* __restrict__ partials,
+    unsigned int* __restrict__ counter, double* __restrict__ diag) {
+  using S = PencilShape<N>;
+  __shared__ cpx<T> lds[S::kLds];
+  Lane<N, false> ln(threadIdx.x);
+  const int q = blockIdx.x * S::C + ln.col;
+  const bool act = q < g.ncols;
+  const int base = act ? (int)pencil_base(g, q) : 0;
+  const int se = (int)g.s_elem;
+  cpx<T>* my = lds + ln.col * S::E * S::P;
+  cpx<T> v[S::E];
+  T rho[S::E];
+#pragma unroll
+  for (int m = 0; m < S::E; ++m) rho[m] = (T)0;
+  if (psi) {
+#pragma unroll
+    for (int m = 0; m < S::E; ++m)
+      v[m] = act ? psi[base + (ln.j + S::TT * m) * se] : cpx<T>{0, 0};
+    if constexpr (DRIFT) {
+      pencil_fft<N, -1, T>(v, ln.j, my, tw);
+#pragma unroll
+      for (int m = 0; m < S::E; ++m) v[m] = cmul(v[m], D[ln.j + S::TT * m]);
+      pencil_fft<N, +1, T>(v, ln.j, my, tw);
+      if (act) {
+#pragma unroll
+        for (int m = 0; m < S::E; ++m) psi[base + (ln.j + S::TT * m) * se] = v[m];
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < S::E; ++m) rho[m] = v[m].r * v[m].r + v[m].i * v[m].i;
+  }
+  double local = 0.0;
+  bool bad = false;
+#pragma unroll
+  for (int m = 0; m < S::E; ++m) {
+    const int idx = base + (ln.j + S::TT * m) * se;
+    if (act) {
+      if (src) {
+        rho[m] += src[idx];
+        if (clear_src) src[idx] = (T)0;
+      }
+      if (brho) brho[idx] = (T)(b_coeff * (double)rho[m]);
+      local += (double)rho[m];
+      bad |= !isfinite(rho[m]);
+    }
+    v[m] = {act ? (T)((double)rho[m] - rho_bar) : (T)0, (T)0};
+  }
+  pencil_fft<N, -1, T>(v, ln.j, my, tw);
+  if (act) {
+#pragma unroll
+    for (int m = 0; m < S::E; ++m) {
+      const int k = ln.j + S::TT * m;
+      if (k <= N / 2) H[base + k * se] = v[m];
+    }
+  }
+  block_sum_finish(local, bad, partials, counter, diag);
+}
+
+// ---------------------------------------------------------------------------
+// generic in-place complex FFT sweep (forward)
+// ---------------------------------------------------------------------------
+template <class T, int N, bool CONTIG, int DIR>
+__global__ __launch_bounds__(256) void k_fft(cpx<T>* __restrict__ a, PassGeom g,
+                                             const cpx<T>* __restrict__ tw) {
+  using S = PencilShape<N>;
+  __shared__ cpx<T> lds[S::kLds];
+  Lane<N, CONTIG> ln(threadIdx.x);
+  const int q = blockIdx.x * S::C + ln.col;
+  const bool act = q < g.ncols;
+  const int base = act ? (int)pencil_base(g, q) : 0;
+  const int se = (int)g.s_elem;
+  cpx<T> v[S::E];
+#pragma unroll
+  for (int m = 0; m < S::E; ++m) v[m] = act ? a[base + (ln.j + S::TT * m) * se] : cpx<T>{0, 0};
+  pencil_fft<N, DIR, T>(v, ln.j, lds + ln.col * S::E * S::P, tw);
+  if (act) {
+#pragma unroll
+    for (int m = 0; m < S::E; ++m) a[base + (ln.j + S::TT * m) * se] = v[m];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Poisson z pass on the half spectrum: FFT_z, V_hat = -4piG rho_hat/k^2
+// (quantum-stars.py:191, k = 0 divided by 1 with the FP64 source sum),
+// IFFT_z of V_hat and, with GRAD, of i kz V_hat (Nyquist derivative 0).
+// ---------------------------------------------------------------------------
+template <class T, int N, bool GRAD>
+__global__ __launch_bounds__(256) void k_poisson_z(cpx<T>* __restrict__ H, cpx<T>* __restrict__ Uz,
+                                                   PassGeom g, const cpx<T>* __restrict__ tw,
+                                                   int ny_log2, int nx, int ny, double kfx,
+                                                   double kfy, double kfz, double four_pi_G,
+                                                   double rho_bar, double ntot, int dc_mode,
+                                                   const double* __restrict__ diag) {
+  using S = PencilShape<N>;
+  __shared__ cpx<T> lds[S::kLds];
+  Lane<N, true> ln(threadIdx.x);
+  const int q = blockIdx.x * S::C + ln.col;
+  const bool act = q < g.ncols;
+  const int base = act ? (int)pencil_base(g, q) : 0;
+  cpx<T>* my = lds + ln.col * S::E * S::P;
+  cpx<T> v[S::E];
+#pragma unroll
+  for (int m = 0; m < S::E; ++m) v[m] = act ? H[base + ln.j + S::TT * m] : cpx<T>{0, 0};
+  pencil_fft<N, -1, T>(v, ln.j, my, tw);
+  const int x = q >> ny_log2, y = q & (ny - 1);
+  const double kx = kfx * (double)freq_index(x, nx);
+  const double ky = kfy * (double)freq_index(y, ny);
+  const double kxy2 = kx * kx + ky * ky;
+  const double inv_ntot = 1.0 / ntot;
+  cpx<T> w[GRAD ? S::E : 1];
+#pragma unroll
+  for (int m = 0; m < S::E; ++m) {
+    const int kzi = ln.j + S::TT * m;
+    const double kz = kfz * (double)freq_index(kzi, N);
+    const double k2 = kxy2 + kz * kz;
+    const T Gk = (T)(-four_pi_G * inv_ntot / (k2 == 0.0 ? 1.0 : k2));
+    v[m] = scale(v[m], Gk);
+    if (x == 0 && y == 0 && kzi == 0) {
+      double dc = (dc_mode == 0) ? -four_pi_G * (diag[kDiagMass] - ntot * rho_bar) * inv_ntot : 0.0;
+      v[m] = {(T)dc, (T)0};
+    }
+    if constexpr (GRAD) {
+      const T kzt = (kzi == N / 2) ? (T)0 : (T)kz;
+      w[m] = {-kzt * v[m].i, kzt * v[m].r};
+    }
+  }
+  pencil_fft<N, +1, T>(v, ln.j, my, tw);
+  if (act) {
+#pragma unroll
+    for (int m = 0; m < S::E; ++m) H[base + ln.j + S::TT * m] = v[m];
+  }
+  if constexpr (GRAD) {
+    pencil_fft<N, +1, T>(w, ln.j, my, tw);
+    if (act) {
+#pragma unroll
+      for (int m = 0; m < S::E; ++m) Uz[base + ln.j + S::TT * m] = w[m];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Poisson y pass (inverse): U -> IFFT_y U (in place); with GRAD also
+// IFFT_y(i ky U) -> Uy and IFFT_y Uz (in place).
+// ---------------------------------------------------------------------------
+template <class T, int N, bool GRAD>
+__global__ __launch_bounds__(256) void k_poisson_y(cpx<T>* __restrict__ H, cpx<T>* __restrict__ Uz,
+                                                   cpx<T>* __restrict__ Uy, PassGeom g,
+                                                   const cpx<T>* __restrict__ tw, double kfy) {
+  using S = PencilShape<N>;
+  __shared__ cpx<T> lds[S::kLds];
+  Lane<N, false> ln(threadIdx.x);
+  const int q = blockIdx.x * S::C + ln.col;
+  const bool act = q < g.ncols;
+  const int base = act ? (int)pencil_base(g, q) : 0;
+  const int se = (int)g.s_elem;
+  cpx<T>* my = lds + ln.col * S::E * S::P;
+  cpx<T> v[S::E];
+#pragma unroll
+  for (int m = 0; m < S::E; ++m) v[m] = act ? H[base + (ln.j + S::TT * m) * se] : cpx<T>{0, 0};
+  cpx<T> w[GRAD ? S::E : 1];
+  if constexpr (GRAD) {
+#pragma unroll
+    for (int m = 0; m < S::E; ++m) {
+      const int k = ln.j + S::TT * m;
+      const T ky = (k == N / 2) ? (T)0 : (T)(kfy * (double)freq_index(k, N));
+      w[m] = {-ky * v[m].i, ky * v[m].r};
+    }
+  }
+  pencil_fft<N, +1, T>(v, ln.j, my, tw);
+  if (act) {
+#pragma unroll
+    for (int m = 0; m < S::E; ++m) H[base + (ln.j + S::TT * m) * se] = v[m];
+  }
+  if constexpr (GRAD) {
+    pencil_fft<N, +1, T>(w, ln.j, my, tw);
+    if (act) {
+#pragma unroll
+      for (int m = 0; m < S::E; ++m) Uy[base + (ln.j + S::TT * m) * se] = w[m];
+    }
+#pragma unroll
+    for (int m = 0; m < S::E; ++m) v[m] = act ? Uz[base + (ln.j + S::TT * m) * se] : cpx<T>{0, 0};
+    pencil_fft<N, +1, T>(v, ln.j, my, tw);
+    if (act) {
+#pragma unroll
+      for (int m = 0; m < S::E; ++m) Uz[base + (ln.j + S::TT * m) * se] = v[m];
+    }
+  }
+}
+
+// Hermitian completion along x: bin k of the full spectrum from the half.
+template <int N, class T>
+__device__ __forceinline__ cpx<T> load_herm(const cpx<T>* __restrict__ a, int base, int se, int k) {
+  if (k <= N / 2) return a[base + k * se];
+  cpx<T> c = a[base + (N - k) * se];
+  return {c.r, -c.i};
+}
+
+// ---------------------------------------------------------------------------
+// Poisson x pass (inverse, Hermitian-completed): V = IFFT_x(UU);
+// with GRAD: a_x + i a_y = -IFFT_x(i kx UU + i Uy), a_z = -IFFT_x(Uzz); max|a|.
+// quantum-stars.py:192 and :256-260.
+// ---------------------------------------------------------------------------
+template <class T, int N, bool GRAD>
+__global__ __launch_bounds__(256) void k_poisson_x(const cpx<T>* __restrict__ H,
+                                                   const cpx<T>* __restrict__ Uy,
+                                                   const cpx<T>* __restrict__ Uzz, PassGeom g,
+                                                   const cpx<T>* __restrict__ tw, double kfx,
+                                                   T* __restrict__ V, T* __restrict__ ax,
+                                                   T* __restrict__ ay, T* __restrict__ az,
+                                                   const T* __restrict__ addend,
+                                                   double* __restrict__ diag) {
+  using S = PencilShape<N>;
+  __shared__ cpx<T> lds[S::kLds];
+  Lane<N, false> ln(threadIdx.x);
+  const int q = blockIdx.x * S::C + ln.col;
+  const bool act = q < g.ncols;
+  const int base = act ? (int)pencil_base(g, q) : 0;
+  const int se = (int)g.s_elem;
+  cpx<T>* my = lds + ln.col * S::E * S::P;
+  cpx<T> v[S::E];
+#pragma unroll
+  for (int m = 0; m < S::E; ++m)
+    v[m] = act ? load_herm<N>(H, base, se, ln.j + S::TT * m) : cpx<T>{0, 0};
+  cpx<T> w[GRAD ? S::E : 1];
+  if constexpr (GRAD) {
+#pragma unroll
+    for (int m = 0; m < S::E; ++m) {
+      const int k = ln.j + S::TT * m;
+      const T kx = (k == N / 2) ? (T)0 : (T)(kfx * (double)freq_index(k, N));
+      w[m] = {-kx * v[m].i, kx * v[m].r};
+    }
+  }
+  pencil_fft<N, +1, T>(v, ln.j, my, tw);
+  if (act) {
+#pragma unroll
+    for (int m = 0; m < S::E; ++m) {
+      const int idx = base + (ln.j + S::TT * m) * se;
+      V[idx] = addend ? v[m].r + addend[idx] : v[m].r;
+    }
+  }
+  if constexpr (GRAD) {
+#pragma unroll
+    for (int m = 0; m < S::E; ++m) {
+      cpx<T> u = act ? load_herm<N>(Uy, base, se, ln.j + S::TT * m) : cpx<T>{0, 0};
+      w[m] = {w[m].r - u.i, w[m].i + u.r};  // + i*Uy
+    }
+    pencil_fft<N, +1, T>(w, ln.j, my, tw);
+    T amax = (T)0;
+    if (act) {
+#pragma unroll
+      for (int m = 0; m < S::E; ++m) {
+        const int idx = base + (ln.j + S::TT * m) * se;
+        const T a0 = -w[m].r, a1 = -w[m].i;
+        ax[idx] = a0;
+        ay[idx] = a1;
+        const T m0 = fabs(a0), m1 = fabs(a1);
+        amax = (m0 > amax || m0 != m0) ? m0 : amax;
+        amax = (m1 > amax || m1 != m1) ? m1 : amax;
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < S::E; ++m)
+      v[m] = act ? load_herm<N>(Uzz, base, se, ln.j + S::TT * m) : cpx<T>{0, 0};
+    pencil_fft<N, +1, T>(v, ln.j, my, tw);
+    if (act) {
+#pragma unroll
+      for (int m = 0; m < S::E; ++m) {
+        const int idx = base + (ln.j + S::TT * m) * se;
+        const T a2 = -v[m].r;
+        az[idx] = a2;
+        const T m2 = fabs(a2);
+        amax = (m2 > amax || m2 != m2) ? m2 : amax;
+      }
+    }
+    // wave max (NaN-propagating), one atomic per wave
+    double am = (double)amax;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      double other = __shfl_xor(am, o, 64);
+      am = (other > am || other != other) ? other : am;
+    }
+    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&diag[kDiagAMax], am);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+template <int N>
+inline int nblocks(int ncols) {
+  return (ncols + PencilShape<N>::C - 1) / PencilShape<N>::C;
+}
+
+#define QJ_DISPATCH_N(n, ...)                      \
+  switch (n) {                                     \
+    case 8: { constexpr int N_ = 8; __VA_ARGS__; } break;     \
+    case 16: { constexpr int N_ = 16; __VA_ARGS__; } break;   \
+    case 32: { constexpr int N_ = 32; __VA_ARGS__; } break;   \
+    case 64: { constexpr int N_ = 64; __VA_ARGS__; } break;   \
+    case 128: { constexpr int N_ = 128; __VA_ARGS__; } break; \
+    case 256: { constexpr int N_ = 256; __VA_ARGS__; } break; \
+    case 512: { constexpr int N_ = 512; __VA_ARGS__; } break; \
+    default: return hipErrorInvalidValue;          \
+  }
+
+// pencil geometries over a full (nxh, ny, nz) field
+inline PassGeom geom_x(const Dims& d) {  // pencils along x, q = y*nz + z
+  return PassGeom{0, 1, (long long)d.ny * d.nz, ilog2(d.ny * d.nz), d.ny * d.nz};
+}
+inline PassGeom geom_y(const Dims& d, int nxh) {  // pencils along y, q = x*nz + z
+  return PassGeom{(long long)d.ny * d.nz, 1, d.nz, ilog2(d.nz), nxh * d.nz};
+}
+inline PassGeom geom_z(const Dims& d, int nxh) {  // pencils along z, q = x*ny + y
+  return PassGeom{0, d.nz, 1, 30, nxh * d.ny};
+}
+
+}  // namespace
+
+template <class T>
+hipError_t launch_twiddles(void* tw, int n, hipStream_t s) {
+  k_twiddles<T><<<(n + 255) / 256, 256, 0, s>>>((cpx<T>*)tw, n);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_drift_table(void* tab, int n, double L, double c, hipStream_t s) {
+  k_drift_table<T><<<(n + 255) / 256, 256, 0, s>>>((cpx<T>*)tab, n, L, c);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_kick(void* psi, const void* V, const void* Vb, double coef, long long n,
+                       hipStream_t s) {
+  long long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  k_kick<T><<<(int)blocks, 256, 0, s>>>((cpx<T>*)psi, (const T*)V, (const T*)Vb, coef, n);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, void* psi,
+                             const void* V, const void* Vb, double kick_coef, int kick,
+                             hipStream_t s) {
+  cpx<T>* p = (cpx<T>*)psi;
+  const T* v = (const T*)V;
+  const T* vb = (const T*)Vb;
+  if (axis == 2) {
+    PassGeom g = geom_z(d, d.nx);
+    const cpx<T>* D = (const cpx<T>*)tb.drift_z;
+    const cpx<T>* tw = (const cpx<T>*)tb.tw_z;
+    QJ_DISPATCH_N(d.nz, {
+      if (kick)
+        k_drift<T, N_, true, true><<<nblocks<N_>(g.ncols), 256, 0, s>>>(p, g, D, tw, v, vb, kick_coef);
+      else
+        k_drift<T, N_, true, false><<<nblocks<N_>(g.ncols), 256, 0, s>>>(p, g, D, tw, v, vb, kick_coef);
+    });
+  } else if (axis == 1) {
+    PassGeom g = geom_y(d, d.nx);
+    const cpx<T>* D = (const cpx<T>*)tb.drift_y;
+    const cpx<T>* tw = (const cpx<T>*)tb.tw_y;
+    QJ_DISPATCH_N(d.ny, {
+      if (kick)
+        k_drift<T, N_, false, true><<<nblocks<N_>(g.ncols), 256, 0, s>>>(p, g, D, tw, v, vb, kick_coef);
+      else
+        k_drift<T, N_, false, false><<<nblocks<N_>(g.ncols), 256, 0, s>>>(p, g, D, tw, v, vb, kick_coef);
+    });
+  } else {
+    PassGeom g = geom_x(d);
+    const cpx<T>* D = (const cpx<T>*)tb.drift_x;
+    const cpx<T>* tw = (const cpx<T>*)tb.tw_x;
+    QJ_DISPATCH_N(d.nx, {
+      if (kick)
+        k_drift<T, N_, false, true><<<nblocks<N_>(g.ncols), 256, 0, s>>>(p, g, D, tw, v, vb, kick_coef);
+      else
+        k_drift<T, N_, false, false><<<nblocks<N_>(g.ncols), 256, 0, s>>>(p, g, D, tw, v, vb, kick_coef);
+    });
+  }
+  return hipGetLastError();
+}
+
+int source_x_blocks(const Dims& d) {
+  const int ncols = d.ny * d.nz;
+  int C = 256;
+  switch (d.nx) {
+    case 8: C = PencilShape<8>::C; break;
+    case 16: C = PencilShape<16>::C; break;
+    case 32: C = PencilShape<32>::C; break;
+    case 64: C = PencilShape<64>::C; break;
+    case 128: C = PencilShape<128>::C; break;
+    case 256: C = PencilShape<256>::C; break;
+    case 512: C = PencilShape<512>::C; break;
+    default: break;
+  }
+  return (ncols + C - 1) / C;
+}
+
+template <class T>
+hipError_t launch_source_x(const Dims& d, const SpecTables<T>& tb, int drift, void* psi,
+                           void* src, int clear_src, void* brho, double b_coeff, double rho_bar,
+                           void* H, double* partials, unsigned int* counter, double* diag,
+                           hipStream_t s) {
+  PassGeom g = geom_x(d);
+  const cpx<T>* D = (const cpx<T>*)tb.drift_x;
+  const cpx<T>* tw = (const cpx<T>*)tb.tw_x;
+  QJ_DISPATCH_N(d.nx, {
+    if (drift)
+      k_source_x<T, N_, true><<<nblocks<N_>(g.ncols), 256, 0, s>>>(
+          (cpx<T>*)psi, (T*)src, clear_src, (T*)brho, b_coeff, rho_bar, g, D, tw, (cpx<T>*)H,
+          partials, counter, diag);
+    else
+      k_source_x<T, N_, false><<<nblocks<N_>(g.ncols), 256, 0, s>>>(
+          (cpx<T>*)psi, (T*)src, clear_src, (T*)brho, b_coeff, rho_bar, g, D, tw, (cpx<T>*)H,
+          partials, counter, diag);
+  });
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_fft_y_half(const Dims& d, const SpecTables<T>& tb, void* H, hipStream_t s) {
+  PassGeom g = geom_y(d, d.nx / 2 + 1);
+  const cpx<T>* tw = (const cpx<T>*)tb.tw_y;
+  QJ_DISPATCH_N(d.ny, {
+    k_fft<T, N_, false, -1><<<nblocks<N_>(g.ncols), 256, 0, s>>>((cpx<T>*)H, g, tw);
+  });
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_poisson_z(const Dims& d, const SpecTables<T>& tb, int grad, void* H, void* Uz,
+                            double four_pi_G, double rho_bar, int dc_mode, const double* diag,
+                            hipStream_t s) {
+  PassGeom g = geom_z(d, d.nx / 2 + 1);
+  const cpx<T>* tw = (const cpx<T>*)tb.tw_z;
+  const double kfx = 2.0 * M_PI / d.Lx, kfy = 2.0 * M_PI / d.Ly, kfz = 2.0 * M_PI / d.Lz;
+  const double ntot = (double)d.nx * d.ny * d.nz;
+  const int nyl = ilog2(d.ny);
+  QJ_DISPATCH_N(d.nz, {
+    if (grad)
+      k_poisson_z<T, N_, true><<<nblocks<N_>(g.ncols), 256, 0, s>>>(
+          (cpx<T>*)H, (cpx<T>*)Uz, g, tw, nyl, d.nx, d.ny, kfx, kfy, kfz, four_pi_G, rho_bar,
+          ntot, dc_mode, diag);
+    else
+      k_poisson_z<T, N_, false><<<nblocks<N_>(g.ncols), 256, 0, s>>>(
+          (cpx<T>*)H, (cpx<T>*)Uz, g, tw, nyl, d.nx, d.ny, kfx, kfy, kfz, four_pi_G, rho_bar,
+          ntot, dc_mode, diag);
+  });
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_poisson_y(const Dims& d, const SpecTables<T>& tb, int grad, void* H, void* Uz,
+                            void* Uy, hipStream_t s) {
+  PassGeom g = geom_y(d, d.nx / 2 + 1);
+  const cpx<T>* tw = (const cpx<T>*)tb.tw_y;
+  const double kfy = 2.0 * M_PI / d.Ly;
+  QJ_DISPATCH_N(d.ny, {
+    if (grad)
+      k_poisson_y<T, N_, true><<<nblocks<N_>(g.ncols), 256, 0, s>>>(
+          (cpx<T>*)H, (cpx<T>*)Uz, (cpx<T>*)Uy, g, tw, kfy);
+    else
+      k_poisson_y<T, N_, false><<<nblocks<N_>(g.ncols), 256, 0, s>>>(
+          (cpx<T>*)H, (cpx<T>*)Uz, (cpx<T>*)Uy, g, tw, kfy);
+  });
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_poisson_x(const Dims& d, const SpecTables<T>& tb, int grad, const void* H,
+                            const void* Uy, const void* Uzz, void* V, void* ax, void* ay,
+                            void* az, const void* addend, double* diag, hipStream_t s) {
+  PassGeom g = geom_x(d);
+  const cpx<T>* tw = (const cpx<T>*)tb.tw_x;
+  const double kfx = 2.0 * M_PI / d.Lx;
+  QJ_DISPATCH_N(d.nx, {
+    if (grad)
+      k_poisson_x<T, N_, true><<<nblocks<N_>(g.ncols), 256, 0, s>>>(
+          (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)ax,
+          (T*)ay, (T*)az, (const T*)addend, diag);
+    else
+      k_poisson_x<T, N_, false><<<nblocks<N_>(g.ncols), 256, 0, s>>>(
+          (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)ax,
+          (T*)ay, (T*)az, (const T*)addend, diag);
+  });
+  return hipGetLastError();
+}
+
+#define QJ_INST(T)                                                                              \
+  template hipError_t launch_twiddles<T>(void*, int, hipStream_t);                              \
+  template hipError_t launch_drift_table<T>(void*, int, double, double, hipStream_t);           \
+  template hipError_t launch_kick<T>(void*, const void*, const void*, double, long long,        \
+                                     hipStream_t);                                              \
+  template hipError_t launch_drift_axis<T>(const Dims&, const SpecTables<T>&, int, void*,        \
+                                           const void*, const void*, double, int, hipStream_t); \
+  template hipError_t launch_source_x<T>(const Dims&, const SpecTables<T>&, int, void*, void*,  \
+                                         int, void*, double, double, void*, double*,            \
+                                         unsigned int*, double*, hipStream_t);                  \
+  template hipError_t launch_fft_y_half<T>(const Dims&, const SpecTables<T>&, void*,            \
+                                           hipStream_t);                                        \
+  template hipError_t launch_poisson_z<T>(const Dims&, const SpecTables<T>&, int, void*, void*, \
+                                          double, double, int, const double*, hipStream_t);     \
+  template hipError_t launch_poisson_y<T>(const Dims&, const SpecTables<T>&, int, void*, void*, \
+                                          void*, hipStream_t);                                  \
+  template hipError_t launch_poisson_x<T>(const Dims&, const SpecTables<T>&, int, const void*,  \
+                                          const void*, const void*, void*, void*, void*, void*, \
+                                          const void*, double*, hipStream_t);
+
+QJ_INST(float)
+QJ_INST(double)
+
+}  // namespace qj

@@ -1,0 +1,163 @@
+"""The KDK step driver: compute_step + update + fori_loop of the reference.
+
+quantum-stars.py:295-351,454 (stars), quantum-fluid.py:414-459,551 (gas) and
+the particle-free step.  ``KDKSimulation.step(n)`` runs n steps on the
+device with no host synchronisation; reading ``psi`` (or ``state()``)
+applies the owed half kick first, so what a caller sees is always the
+reference's end-of-step state.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _lib
+from .core import Physics, Plan, ptr
+
+
+class KDKSimulation:
+    """State of one box plus the fused native stepper.
+
+    psi   : complex (nx,ny,nz)
+    pos, vel : float64 (np,3) star positions/velocities (stars step)
+    mass  : float (common star mass) or float64 (np,) per-particle masses
+    gas   : (rho, vx, vy, vz) real fields (gas step)
+    """
+
+    def __init__(self, plan: Plan, phys: Physics, dt: float, psi: torch.Tensor,
+                 pos: torch.Tensor | None = None, vel: torch.Tensor | None = None,
+                 mass=0.0, gas=None, t: float = 0.0):
+        self.plan, self.phys, self.dt = plan, phys, float(dt)
+        self._psi = plan.field(psi, "psi", complex_=True).clone()
+        dev, rdt = plan.device, plan.real_dtype
+        self.t = float(t)
+        self.np = 0 if pos is None else int(pos.shape[0])
+        if (pos is None) != (vel is None):
+            raise ValueError("pos and vel go together")
+        if self.np and gas is not None:
+            raise ValueError("stars and gas in one simulation are not supported")
+        self.pos = self.vel = None
+        self.mass_t = None
+        self.m_scalar = 0.0
+        self.rho_s = None
+        if self.np:
+            self.pos = plan.particles(pos, "pos").clone()
+            self.vel = plan.particles(vel, "vel").clone()
+            if isinstance(mass, torch.Tensor):
+                self.mass_t = plan.particles(mass, "mass", cols=0).clone()
+            else:
+                self.m_scalar = float(mass)
+            self.rho_s = torch.zeros(plan.shape, dtype=rdt, device=dev)
+        self.gas = None
+        self._gas_tmp = None
+        if gas is not None:
+            self.gas = [plan.field(g, n).clone() for g, n in zip(gas, ("rho", "vx", "vy", "vz"))]
+            self._gas_tmp = [torch.empty_like(g) for g in self.gas]
+        self.grid = torch.empty((4,) + plan.shape, dtype=rdt, device=dev)
+        self.diag = plan.new_diag()
+        self.brho = torch.empty(plan.shape, dtype=rdt, device=dev) if phys.b_coeff else None
+        st = _lib.QjState()
+        st.psi = self._psi.data_ptr()
+        st.rho_s = ptr(self.rho_s).value
+        if self.gas is not None:
+            for k in range(4):
+                st.gas[k] = self.gas[k].data_ptr()
+                st.gas_tmp[k] = self._gas_tmp[k].data_ptr()
+        st.pos = ptr(self.pos).value
+        st.vel = ptr(self.vel).value
+        st.mass = ptr(self.mass_t).value
+        st.m_scalar = self.m_scalar
+        st.np = self.np
+        st.grid = self.grid.data_ptr()
+        st.brho = ptr(self.brho).value
+        st.diag = self.diag.data_ptr()
+        st.grid_valid = 0
+        st.gas_swapped = 0
+        st.pending = 0.0
+        self._st = st
+        self._params = _lib.QjParams(self.dt, phys.m_per_hbar, phys.four_pi_G, phys.rho_bar,
+                                     phys.b_coeff, phys.cs, phys.dc_mode)
+
+    # -- stepping ------------------------------------------------------------
+    def set_dt(self, dt: float):
+        self.dt = float(dt)
+        self._params.dt = self.dt
+
+    def step(self, n: int = 1):
+        """Advance n KDK steps (asynchronous on the current stream)."""
+        if n <= 0:
+            return self
+        with torch.cuda.device(self.plan.device):
+            _lib.check(_lib.lib().qj_kdk_steps(self.plan.handle, C.byref(self._st),
+                                               C.byref(self._params), int(n),
+                                               ptr(self.plan.workspace()), self.plan.stream()),
+                       "qj_kdk_steps")
+        self.t += n * self.dt
+        return self
+
+    def finalize(self):
+        """Apply the owed half kick (idempotent)."""
+        with torch.cuda.device(self.plan.device):
+            _lib.check(_lib.lib().qj_kdk_finalize(self.plan.handle, C.byref(self._st),
+                                                  C.byref(self._params), self.plan.stream()),
+                       "qj_kdk_finalize")
+        return self
+
+    # -- state access ------------------------------------------------------
+    @property
+    def psi(self) -> torch.Tensor:
+        self.finalize()
+        return self._psi
+
+    def gas_fields(self):
+        if self.gas is None:
+            return None
+        self.finalize()
+        return tuple(self._gas_tmp if self._st.gas_swapped else self.gas)
+
+    @property
+    def V(self) -> torch.Tensor:
+        """Potential of the current state (computed on the first step)."""
+        return self.grid[0]
+
+    @property
+    def a_grid(self) -> torch.Tensor:
+        return self.grid[1:]
+
+    def a_max(self) -> float:
+        return float(self.diag[_lib.DIAG_AMAX].item())
+
+    def dt_s(self) -> float:
+        return float(self.diag[_lib.DIAG_DTS].item())
+
+    def nan_seen(self) -> bool:
+        return bool(self.diag[_lib.DIAG_NAN].item() != 0.0)
+
+    def state(self) -> dict:
+        """The reference's state dict (quantum-stars.py:434-440) after finalize."""
+        d = {"psi": self.psi, "t": self.t}
+        if self.np:
+            d.update(pos=self.pos, vel=self.vel, a_max=self.a_max(), dt_s=self.dt_s())
+        if self.gas is not None:
+            d.update(zip(("rho", "vx", "vy", "vz"), self.gas_fields()))
+        return d
+
+
+def compute_step(psi, pos, vel, t, a_max, dt_s, *, plan: Plan, phys: Physics, dt: float,
+                 m_s=0.0):
+    """One stars step with the reference signature (quantum-stars.py:295-330)."""
+    sim = KDKSimulation(plan, phys, dt, psi, pos, vel, mass=m_s, t=t)
+    sim.diag[_lib.DIAG_AMAX] = float(a_max)
+    sim.step(1)
+    st = sim.state()
+    return st["psi"], st["pos"], st["vel"], st["t"], st["a_max"], st["dt_s"]
+
+
+def compute_step_gas(psi, rho, vx, vy, vz, t, *, plan: Plan, phys: Physics, dt: float):
+    """One gas step with the reference signature (quantum-fluid.py:414-438)."""
+    sim = KDKSimulation(plan, phys, dt, psi, gas=(rho, vx, vy, vz), t=t)
+    sim.step(1)
+    st = sim.state()
+    return st["psi"], st["rho"], st["vx"], st["vy"], st["vz"], st["t"]

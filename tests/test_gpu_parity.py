@@ -1,0 +1,299 @@
+"""HIP path vs the FP64 oracle on the same seeded inputs (needs a GPU).
+
+Tolerances: relative L2 error ``rel(a, b) = |a-b|/|b|``.  complex128/FP64
+plans must agree to ~1e-12 (FFT roundoff); complex64/FP32 plans to a few
+1e-7 per operator and 1e-5 after a few steps.  CIC indices are compared
+bit-exactly.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+C64, C128 = torch.complex64, torch.complex128
+TOL1 = {C64: 2e-6, C128: 1e-12}        # one operator
+TOLS = {C64: 2e-5, C128: 1e-11}        # a few steps
+
+
+def rel(a, b):
+    a = a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
+    b = b.detach().cpu().numpy() if isinstance(b, torch.Tensor) else np.asarray(b)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def dev(x, dtype):
+    return torch.as_tensor(np.ascontiguousarray(x)).to("cuda", dtype)
+
+
+def plan_for(qj, g, dtype):
+    return qj.Plan(g.nx, g.ny, g.nz, L=(g.Lx, g.Ly, g.Lz), dtype=dtype)
+
+
+def rdt(dtype):
+    return torch.float32 if dtype == C64 else torch.float64
+
+
+SHAPES = [O.Grid(32, 32, 32), O.Grid(64, 16, 32, 1.0, 0.5, 2.0), O.Grid(16, 128, 8, 2.0, 1.0, 1.0)]
+
+
+def _psi(g, seed=1):
+    return O.filtered_gaussian_psi(g, seed, sigma_k=8 * np.pi)
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+@pytest.mark.parametrize("g", SHAPES, ids=lambda g: f"{g.nx}x{g.ny}x{g.nz}")
+def test_drift_matches_oracle(qj, cuda, dtype, g):
+    psi = _psi(g)
+    ph = O.Physics(m_per_hbar=1.3)
+    plan = plan_for(qj, g, dtype)
+    for dt in (1e-3, -2.5e-3):
+        out = qj.drift(dev(psi, dtype), dt, plan, qj.Physics(m_per_hbar=1.3))
+        assert rel(out, O.drift(psi, dt, g, ph)) < TOL1[dtype]
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+def test_drift_large_phase_range_reduction(qj, cuda, dtype):
+    # at 256^2-class wavenumbers the kinetic phase reaches hundreds of radians
+    g = O.Grid(8, 8, 256)
+    psi = _psi(g, 2)
+    ph = O.Physics()
+    plan = plan_for(qj, g, dtype)
+    out = qj.drift(dev(psi, dtype), 5e-4, plan, qj.Physics())
+    assert rel(out, O.drift(psi, 5e-4, g, ph)) < TOL1[dtype]
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+def test_kick_matches_oracle(qj, cuda, dtype):
+    g = SHAPES[0]
+    rng = np.random.default_rng(3)
+    psi = _psi(g)
+    V = rng.standard_normal(g.shape)
+    plan = plan_for(qj, g, dtype)
+    out = qj.kick(dev(psi, dtype), dev(V, rdt(dtype)), 0.37, plan)
+    Vr = dev(V, rdt(dtype)).double().cpu().numpy()   # compare on the rounded input
+    assert rel(out, O.kick(psi, Vr, 0.37)) < TOL1[dtype]
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+@pytest.mark.parametrize("g", SHAPES, ids=lambda g: f"{g.nx}x{g.ny}x{g.nz}")
+@pytest.mark.parametrize("b", [0.0, -0.3])
+def test_potential_matches_oracle(qj, cuda, dtype, g, b):
+    psi = _psi(g, 4)
+    rho = np.abs(psi) ** 2
+    ph = O.Physics(four_pi_G=2.0, rho_bar=1.3, b_coeff=b)
+    plan = plan_for(qj, g, dtype)
+    V = qj.get_potential(dev(rho, rdt(dtype)), plan,
+                         qj.Physics(four_pi_G=2.0, rho_bar=1.3, b_coeff=b))
+    ref = O.get_potential(rho, g, ph)
+    assert rel(V, ref) < 4 * TOL1[dtype]
+    # the k=0 rule: mean V = -4 pi G (mean rho - rho_bar) (+ b mean rho)
+    assert abs(V.double().mean().item() - ref.mean()) < (1e-6 if dtype == C64 else 1e-12)
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+def test_potential_zero_mean_mode(qj, cuda, dtype):
+    g = SHAPES[0]
+    rho = np.abs(_psi(g, 5)) ** 2
+    plan = plan_for(qj, g, dtype)
+    V = qj.get_potential(dev(rho, rdt(dtype)), plan, qj.Physics(rho_bar=7.0, dc_mode=1))
+    ref = O.get_potential(rho, g, O.Physics(rho_bar=rho.mean()))
+    assert rel(V, ref) < 4 * TOL1[dtype]
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+@pytest.mark.parametrize("g", SHAPES, ids=lambda g: f"{g.nx}x{g.ny}x{g.nz}")
+def test_acceleration_grid_matches_oracle(qj, cuda, dtype, g):
+    rho = np.abs(_psi(g, 6)) ** 2
+    ph = O.Physics(four_pi_G=1.0, rho_bar=1.0)
+    plan = plan_for(qj, g, dtype)
+    diag = plan.new_diag()
+    V, a = qj.acceleration_grid(dev(rho, rdt(dtype)), plan, qj.Physics(), diag=diag)
+    ref = O.acceleration_grid(rho, g, ph)
+    assert rel(a, ref) < 4 * TOL1[dtype]
+    assert rel(V, O.get_potential(rho, g, ph)) < 4 * TOL1[dtype]
+    amax = diag[0].item()
+    assert abs(amax - np.abs(ref).max()) <= (1e-5 if dtype == C64 else 1e-12) * np.abs(ref).max()
+
+
+def _adversarial_positions(g, n=20000, seed=7):
+    rng = np.random.default_rng(seed)
+    L = np.array([g.Lx, g.Ly, g.Lz])
+    d = np.array([g.dx, g.dy, g.dz])
+    pos = rng.random((n, 3)) * L
+    k = rng.integers(-2, np.array(g.shape) + 2, size=(n, 3))
+    # faces, centres and their FP64 neighbours, the box edges, outside values
+    faces = k * d
+    pos[: n // 5] = faces[: n // 5]
+    pos[n // 5: 2 * n // 5] = (k * d + 0.5 * d)[n // 5: 2 * n // 5]
+    pos[2 * n // 5: 3 * n // 5] = np.nextafter(faces, faces + 1)[2 * n // 5: 3 * n // 5]
+    pos[3 * n // 5: 3 * n // 5 + 500] = np.nextafter(faces, faces - 1)[3 * n // 5: 3 * n // 5 + 500]
+    pos[-6:] = [[0.0, 0.0, 0.0], L, -L, [1e-17, -1e-17, L[2] - 1e-17],
+                [-0.0, 5 * L[1], -3.3 * L[2]], L * (1 - 2 ** -52)]
+    return pos
+
+
+@pytest.mark.parametrize("g", [O.Grid(64, 64, 64), O.Grid(32, 16, 8, 1.0, 3.0, 0.7)],
+                         ids=["64", "aniso"])
+def test_cic_indices_bit_exact(qj, cuda, g):
+    pos = _adversarial_positions(g)
+    plan = plan_for(qj, g, C64)
+    i, ip1, w_i, w_ip1 = qj.get_cic_indices_and_weights(dev(pos, torch.float64), plan)
+    ri, rip1, rw_i, rw_ip1 = O.get_cic_indices_and_weights(pos, g)
+    assert np.array_equal(i.cpu().numpy(), ri)
+    assert np.array_equal(ip1.cpu().numpy(), rip1)
+    assert np.array_equal(w_i.cpu().numpy(), rw_i)       # bitwise FP64
+    assert np.array_equal(w_ip1.cpu().numpy(), rw_ip1)
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+def test_deposit_matches_oracle(qj, cuda, dtype):
+    g = O.Grid(32, 16, 64, 1.0, 0.5, 2.0)
+    rng = np.random.default_rng(8)
+    pos = rng.random((5000, 3)) * np.array([g.Lx, g.Ly, g.Lz])
+    m = rng.random(5000)
+    plan = plan_for(qj, g, dtype)
+    rho = qj.bin_particles(dev(pos, torch.float64), dev(m, torch.float64), plan)
+    ref = O.bin_particles(pos, m, g)
+    assert rel(rho, ref) < TOL1[dtype]
+    rho_s = qj.bin_stars(dev(pos, torch.float64), 0.25, plan)
+    assert rel(rho_s, O.bin_stars(pos, 0.25, g)) < TOL1[dtype]
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+def test_interp_and_get_acceleration_match_oracle(qj, cuda, dtype):
+    g = O.Grid(32, 32, 32)
+    rng = np.random.default_rng(9)
+    pos = rng.random((3000, 3))
+    rho = np.abs(_psi(g, 10)) ** 2 + O.bin_stars(pos, 1e-4, g)
+    plan = plan_for(qj, g, dtype)
+    acc, a_max = qj.get_acceleration(dev(pos, torch.float64), dev(rho, rdt(dtype)), plan,
+                                     qj.Physics())
+    racc, ramax = O.get_acceleration(pos, rho, g, O.Physics())
+    assert rel(acc, racc) < 4 * TOL1[dtype]
+    assert abs(a_max.item() - ramax) <= 1e-5 * ramax
+    f = rng.standard_normal((3,) + g.shape)
+    out = qj.interp(dev(pos, torch.float64), dev(f, rdt(dtype)), plan)
+    fr = dev(f, rdt(dtype)).double().cpu().numpy()
+    assert rel(out, O.interp_grid(fr, pos, g)) < 1e-14
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+def test_hydro_matches_oracle(qj, cuda, dtype):
+    g = O.Grid(32, 32, 32)
+    X, Y, Z = O.cell_centres(g)
+    rng = np.random.default_rng(11)
+    rho = 0.2 * (1 + 0.3 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y)) \
+        + 0.01 * rng.random(g.shape)
+    vx = 0.2 * np.cos(2 * np.pi * Z) + 0.02 * rng.standard_normal(g.shape)
+    vy = 0.1 * np.sin(4 * np.pi * X)
+    vz = -0.1 * np.cos(2 * np.pi * Y) + 0.02 * rng.standard_normal(g.shape)
+    plan = plan_for(qj, g, dtype)
+    r = rdt(dtype)
+    ins = [dev(a, r) for a in (rho, vx, vy, vz)]
+    insr = [a.double().cpu().numpy() for a in ins]
+    out = qj.solve_hydro(*ins, 1e-3, plan, qj.Physics(cs=0.3))
+    ref = O.solve_hydro(*insr, 1e-3, g.dx, g.vol, 0.3)
+    for o, rf in zip(out, ref):
+        assert rel(o, rf) < (1e-6 if dtype == C64 else 1e-13)
+
+
+def _stars_case(g, n_p, seed=12):
+    psi = _psi(g, seed)
+    pos, vel, m_s = O.star_ic(g, n_p, seed + 1000, sigma_v=0.2)
+    return psi, pos, vel, m_s
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+def test_stars_steps_match_oracle(qj, cuda, dtype):
+    g = O.Grid(32, 32, 32)
+    psi, pos, vel, m_s = _stars_case(g, 2000)
+    ph = O.Physics(rho_bar=1.1)
+    dt, nsteps = 1e-3, 4
+    plan = plan_for(qj, g, dtype)
+    sim = qj.KDKSimulation(plan, qj.Physics(rho_bar=1.1), dt, dev(psi, dtype),
+                           dev(pos, torch.float64), dev(vel, torch.float64), mass=m_s)
+    sim.step(nsteps)
+    st = sim.state()
+    s = (psi, pos, vel, 0.0, 0.0, np.inf)
+    for _ in range(nsteps):
+        s = O.compute_step_stars(*s, dt, m_s, g, ph)
+    assert rel(st["psi"], s[0]) < TOLS[dtype]
+    dpos = (st["pos"].cpu().numpy() - s[1] + 0.5) % 1.0 - 0.5
+    assert np.max(np.abs(dpos)) < (1e-9 if dtype == C64 else 1e-13)
+    assert rel(st["vel"], s[2]) < (1e-5 if dtype == C64 else 1e-12)
+    assert abs(st["a_max"] - s[4]) < 1e-5 * s[4]
+    assert abs(st["dt_s"] - s[5]) < 1e-5 * s[5]
+
+
+def test_stars_steps_per_particle_mass(qj, cuda):
+    g = O.Grid(32, 32, 32)
+    psi, pos, vel, m_s = _stars_case(g, 1000, 13)
+    m = np.random.default_rng(1).random(1000) * 2 * m_s
+    ph = O.Physics(rho_bar=1.1)
+    plan = plan_for(qj, g, C128)
+    sim = qj.KDKSimulation(plan, qj.Physics(rho_bar=1.1), 1e-3, dev(psi, C128),
+                           dev(pos, torch.float64), dev(vel, torch.float64),
+                           mass=dev(m, torch.float64))
+    sim.step(2)
+    s = (psi, pos, vel, 0.0, 0.0, np.inf)
+    for _ in range(2):
+        s = O.compute_step_stars(*s, 1e-3, m, g, ph)
+    assert rel(sim.psi, s[0]) < 1e-11
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+def test_plain_steps_match_oracle(qj, cuda, dtype):
+    # the particle-free configuration (64^3, dt = 1e-3) over a few steps
+    g = O.Grid(64, 64, 64)
+    psi = O.filtered_gaussian_psi(g, 17)
+    ph = O.Physics(rho_bar=1.0)
+    plan = plan_for(qj, g, dtype)
+    sim = qj.KDKSimulation(plan, qj.Physics(rho_bar=1.0), 1e-3, dev(psi, dtype))
+    sim.step(5)
+    ref = psi
+    for _ in range(5):
+        ref = O.compute_step_plain(ref, 1e-3, g, ph)
+    assert rel(sim.psi, ref) < TOLS[dtype]
+    assert rel(sim.V, O.get_potential(np.abs(ref) ** 2, g, ph)) < 4 * TOLS[dtype]
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+def test_gas_steps_match_oracle(qj, cuda, dtype):
+    g = O.Grid(32, 32, 32)
+    psi = _psi(g, 14)
+    rho, vx, vy, vz = O.gas_ic(g, 14, sigma_k=8 * np.pi)
+    rng = np.random.default_rng(2)
+    vx = 0.05 * rng.standard_normal(g.shape)
+    rb = float(np.mean(np.abs(psi) ** 2 + rho))
+    ph = O.Physics(rho_bar=rb, cs=0.1)
+    plan = plan_for(qj, g, dtype)
+    r = rdt(dtype)
+    gas = [dev(a, r) for a in (rho, vx, vy, vz)]
+    gasr = [a.double().cpu().numpy() for a in gas]
+    sim = qj.KDKSimulation(plan, qj.Physics(rho_bar=rb, cs=0.1), 1e-3, dev(psi, dtype), gas=gas)
+    sim.step(3)
+    st = sim.state()
+    s = (psi, *gasr, 0.0)
+    for _ in range(3):
+        s = O.compute_step_gas(*s, 1e-3, g, ph)
+    assert rel(st["psi"], s[0]) < TOLS[dtype]
+    for k, name in enumerate(("rho", "vx", "vy", "vz")):
+        assert rel(st[name], s[1 + k]) < (5e-5 if dtype == C64 else 1e-11), name
+
+
+def test_plain_torch_comparator_matches_oracle(cuda):
+    import importlib
+    plain = importlib.import_module("quantum-jax_amd.plain")
+    g = O.Grid(32, 32, 32)
+    psi, pos, vel, m_s = _stars_case(g, 500, 15)
+    ph = O.Physics(rho_bar=1.1)
+    p = plain.PlainKDK(g.shape, dtype=C128, rho_bar=1.1)
+    out = p.step_stars(dev(psi, C128), dev(pos, torch.float64), dev(vel, torch.float64), m_s,
+                       1e-3, torch.zeros((), dtype=torch.float64, device="cuda"))
+    ref = O.compute_step_stars(psi, pos, vel, 0.0, 0.0, np.inf, 1e-3, m_s, g, ph)
+    assert rel(out[0], ref[0]) < 1e-12
+    assert rel(out[2], ref[2]) < 1e-12

@@ -1,0 +1,380 @@
+"""Pins of the FP64 oracle against closed forms, invariants and brute force.
+
+The reference (Python/JAX) cannot be imported here and ships no tests or
+golden vectors, so each oracle function is pinned by facts that a plausible
+transcription mistake (a dropped term, a wrong sign, a transposed axis, a
+wrong Nyquist convention) would violate.
+"""
+
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+G = O.Grid(16, 16, 16)
+PH = O.Physics(m_per_hbar=1.0, four_pi_G=1.0, rho_bar=1.0)
+
+
+def centres(g):
+    return O.cell_centres(g)
+
+
+def test_kvec_fft_order_nyquist_negative():
+    k = O.kvec(8, 1.0) / (2 * np.pi)
+    assert np.array_equal(k, [0, 1, 2, 3, -4, -3, -2, -1])
+    # matches numpy's frequency order except the Nyquist sign
+    f = np.fft.fftfreq(8, d=1 / 8)
+    assert np.array_equal(k[:4], f[:4]) and k[4] == -4
+
+
+@pytest.mark.parametrize("m", [(1, 0, 0), (0, 2, 0), (0, 0, 3), (1, 2, 3), (8, 0, 1)])
+def test_potential_single_mode_closed_form(m):
+    g = O.Grid(16, 16, 16, 1.0, 2.0, 0.5)
+    X, Y, Z = centres(g)
+    kv = np.array([2 * np.pi * m[0] / g.Lx, 2 * np.pi * m[1] / g.Ly, 2 * np.pi * m[2] / g.Lz])
+    phase = kv[0] * X + kv[1] * Y + kv[2] * Z
+    A = 0.3
+    ph = O.Physics(four_pi_G=2.5, rho_bar=1.7)
+    rho = ph.rho_bar + A * np.cos(phase)
+    V = O.get_potential(rho, g, ph)
+    k2 = float(kv @ kv)
+    assert np.max(np.abs(V - (-ph.four_pi_G * A * np.cos(phase) / k2))) < 1e-12
+
+
+def test_potential_k0_rule_is_divided_by_one():
+    # quantum-stars.py:191 divides the k=0 bin by 1: mean V = -4piG (mean rho - rho_bar)
+    ph = O.Physics(four_pi_G=3.0, rho_bar=1.0)
+    rho = np.full(G.shape, 1.25)
+    V = O.get_potential(rho, G, ph)
+    assert np.allclose(V, -3.0 * 0.25, rtol=0, atol=1e-14)
+
+
+def test_potential_laplacian():
+    rng = np.random.default_rng(3)
+    rho = 1.0 + rng.random(G.shape)
+    V = O.get_potential(rho, G, PH)
+    m = O.make_grid(G)
+    lap = np.real(np.fft.ifftn(-m.k_sq * np.fft.fftn(V)))
+    assert np.max(np.abs(lap - PH.four_pi_G * (rho - rho.mean()))) < 1e-11
+
+
+def test_potential_self_interaction_term():
+    rng = np.random.default_rng(4)
+    rho = 1.0 + rng.random(G.shape)
+    ph_b = O.Physics(b_coeff=-0.7)
+    assert np.allclose(O.get_potential(rho, G, ph_b), O.get_potential(rho, G, PH) - 0.7 * rho)
+
+
+def test_kick_invariants():
+    rng = np.random.default_rng(5)
+    psi = rng.standard_normal(G.shape) + 1j * rng.standard_normal(G.shape)
+    V = rng.standard_normal(G.shape)
+    out = O.kick(psi, V, 0.37)
+    assert np.allclose(np.abs(out), np.abs(psi), rtol=1e-15, atol=0)
+    out = O.kick(psi, np.full(G.shape, 2.0), 0.25)
+    assert np.allclose(out, psi * np.exp(-0.5j))
+
+
+@pytest.mark.parametrize("m", [(1, 0, 0), (3, -2, 5), (8, 8, 8)])
+def test_drift_plane_wave_closed_form(m):
+    g = O.Grid(16, 16, 16, 1.0, 0.5, 2.0)
+    X, Y, Z = centres(g)
+    kv = np.array([2 * np.pi * m[0] / g.Lx, 2 * np.pi * m[1] / g.Ly, 2 * np.pi * m[2] / g.Lz])
+    psi = np.exp(1j * (kv[0] * X + kv[1] * Y + kv[2] * Z))
+    ph = O.Physics(m_per_hbar=1.3)
+    dt = 1e-3
+    out = O.drift(psi, dt, g, ph)
+    want = np.exp(-1j * dt * float(kv @ kv) / (2 * ph.m_per_hbar)) * psi
+    assert np.max(np.abs(out - want)) < 1e-12
+
+
+def test_drift_unitary_and_reversible():
+    rng = np.random.default_rng(6)
+    psi = rng.standard_normal(G.shape) + 1j * rng.standard_normal(G.shape)
+    out = O.drift(psi, 3e-3, G, PH)
+    assert abs(np.sum(np.abs(out) ** 2) / np.sum(np.abs(psi) ** 2) - 1) < 1e-13
+    back = O.drift(out, -3e-3, G, PH)
+    assert np.max(np.abs(back - psi)) < 1e-12
+
+
+def _exact_index(x, dx):
+    # floor((x - dx/2)/dx) in exact rational arithmetic
+    q = (Fraction(x) - Fraction(dx) / 2) / Fraction(dx)
+    return q.numerator // q.denominator
+
+
+def test_cic_indices_match_exact_floor_away_from_faces():
+    g = O.Grid(32, 16, 8, 1.0, 3.0, 0.7)
+    rng = np.random.default_rng(7)
+    pos = rng.random((4000, 3)) * np.array([g.Lx, g.Ly, g.Lz])
+    i, ip1, w_i, w_ip1 = O.get_cic_indices_and_weights(pos, g)
+    dxs = (g.dx, g.dy, g.dz)
+    ns = g.shape
+    for p in range(0, 4000, 7):
+        for a in range(3):
+            e = _exact_index(pos[p, a], dxs[a])
+            frac = float((Fraction(pos[p, a]) - Fraction(dxs[a]) / 2) / Fraction(dxs[a]) - e)
+            if 1e-9 < frac < 1 - 1e-9:
+                assert i[p, a] == e % ns[a]
+                assert ip1[p, a] == (e + 1) % ns[a]
+                assert abs(w_ip1[p, a] - frac) < 1e-9
+                assert abs(w_i[p, a] - (1 - frac)) < 1e-9
+    assert np.allclose(w_i + w_ip1, 1.0, atol=1e-12)
+
+
+def test_cic_special_positions():
+    g = O.Grid(64, 64, 64)
+    dx = g.dx
+    # exactly on cell centres (dx is a power of two: all exact)
+    pos = np.array([[(5 + 0.5) * dx, (0 + 0.5) * dx, (63 + 0.5) * dx]])
+    i, ip1, w_i, w_ip1 = O.get_cic_indices_and_weights(pos, g)
+    assert i.tolist() == [[5, 0, 63]] and ip1.tolist() == [[6, 1, 0]]
+    assert w_i.tolist() == [[1.0, 1.0, 1.0]] and w_ip1.tolist() == [[0.0, 0.0, 0.0]]
+    # on faces and at the box edges: x=0 wraps to the last cell
+    pos = np.array([[0.0, 1.0, 3 * dx]])
+    i, ip1, w_i, w_ip1 = O.get_cic_indices_and_weights(pos, g)
+    assert i.tolist() == [[63, 63, 2]] and ip1.tolist() == [[0, 0, 3]]
+    assert np.allclose(w_i, 0.5) and np.allclose(w_ip1, 0.5)
+    # negative positions (before the floored wrap) and far outside the box
+    pos = np.array([[-0.3 * dx, -2.6, 7.25]])
+    i, ip1, _, _ = O.get_cic_indices_and_weights(pos, g)
+    assert i[0, 0] == 63 and ip1[0, 0] == 0
+    assert i[0, 1] == (int(np.floor((-2.6 - 0.5 * dx) / dx)) % 64)
+    assert i[0, 2] == (int(np.floor((7.25 - 0.5 * dx) / dx)) % 64)
+
+
+def test_deposit_conserves_mass_and_centres():
+    g = O.Grid(16, 8, 32, 1.0, 0.5, 2.0)
+    rng = np.random.default_rng(8)
+    pos = rng.random((500, 3)) * np.array([g.Lx, g.Ly, g.Lz])
+    m = rng.random(500)
+    rho = O.bin_particles(pos, m, g)
+    assert abs(rho.sum() * g.vol - m.sum()) < 1e-12 * m.sum()
+    assert rho.min() >= 0
+    # a particle on a cell centre lands in that cell only
+    p = np.array([[(3.5) * g.dx, (2.5) * g.dy, (31.5) * g.dz]])
+    rho = O.bin_particles(p, 2.0, g)
+    assert np.count_nonzero(rho) == 1 and rho[3, 2, 31] == 2.0 / g.vol
+
+
+def test_deposit_interp_transpose_identity():
+    g = O.Grid(16, 16, 16)
+    rng = np.random.default_rng(9)
+    pos = rng.random((300, 3))
+    m = rng.random(300)
+    f = rng.standard_normal(g.shape)
+    lhs = np.sum(O.bin_particles(pos, m, g) * f) * g.vol
+    rhs = np.sum(m * O.interp_grid(f[None], pos, g)[:, 0])
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+def test_interp_constant_and_linear():
+    g = O.Grid(16, 16, 16)
+    rng = np.random.default_rng(10)
+    pos = 0.1 + 0.8 * rng.random((200, 3))     # keep away from the periodic seam
+    X, Y, Z = centres(g)
+    f = np.stack([np.full(g.shape, 2.5), X, 3 * Y - Z])
+    out = O.interp_grid(f, pos, g)
+    assert np.allclose(out[:, 0], 2.5)
+    assert np.allclose(out[:, 1], pos[:, 0], atol=1e-12)
+    assert np.allclose(out[:, 2], 3 * pos[:, 1] - pos[:, 2], atol=1e-12)
+
+
+def test_acceleration_single_mode():
+    g = O.Grid(16, 16, 16)
+    X, Y, Z = centres(g)
+    kx, kz = 2 * np.pi * 2, 2 * np.pi * 1
+    rho = PH.rho_bar + 0.4 * np.cos(kx * X + kz * Z)
+    a = O.acceleration_grid(rho, g, PH)
+    k2 = kx ** 2 + kz ** 2
+    # V = -G4 A cos(phase)/k2, a = -grad V = -G4 A k sin(phase)/k2
+    s = np.sin(kx * X + kz * Z)
+    assert np.allclose(a[0], -PH.four_pi_G * 0.4 * kx * s / k2, atol=1e-12)
+    assert np.allclose(a[1], 0, atol=1e-12)
+    assert np.allclose(a[2], -PH.four_pi_G * 0.4 * kz * s / k2, atol=1e-12)
+
+
+def test_acceleration_nyquist_does_not_matter():
+    # Re IFFT(1j k V_hat) is unchanged by the Nyquist sign for real V
+    g = O.Grid(8, 8, 8)
+    rng = np.random.default_rng(11)
+    rho = 1 + rng.random(g.shape)
+    a = O.acceleration_grid(rho, g, PH)
+    m = O.make_grid(g)
+    kx = m.kx.copy()
+    kx[4] = -kx[4]
+    V_hat = -np.fft.fftn(PH.four_pi_G * (rho - PH.rho_bar)) / (m.k_sq + (m.k_sq == 0))
+    ax2 = -np.real(np.fft.ifftn(1j * kx * V_hat))
+    assert np.allclose(a[0], ax2, atol=1e-13)
+
+
+def _stars_setup(g, n_p=64, seed=12):
+    psi = O.filtered_gaussian_psi(g, seed, sigma_k=8 * np.pi)
+    pos, vel, m_s = O.star_ic(g, n_p, seed + 1000, sigma_v=0.3)
+    return psi, pos, vel, m_s
+
+
+def test_stars_step_time_reversible():
+    g = O.Grid(16, 16, 16)
+    ph = O.Physics(rho_bar=1.1)
+    psi, pos, vel, m_s = _stars_setup(g)
+    dt = 2e-3
+    s1 = O.compute_step_stars(psi, pos, vel, 0.0, 0.0, np.inf, dt, m_s, g, ph)
+    s0 = O.compute_step_stars(s1[0], s1[1], s1[2], 0.0, 0.0, np.inf, -dt, m_s, g, ph)
+    assert np.max(np.abs(s0[0] - psi)) < 1e-11
+    dpos = (s0[1] - pos + 0.5) % 1.0 - 0.5
+    assert np.max(np.abs(dpos)) < 1e-12
+    assert np.max(np.abs(s0[2] - vel)) < 1e-11
+
+
+def test_stars_step_conserves_mass_and_tracks_diagnostics():
+    g = O.Grid(16, 16, 16)
+    ph = O.Physics(rho_bar=1.1)
+    psi, pos, vel, m_s = _stars_setup(g)
+    M0 = O.mass_total(psi, g, np.full(len(pos), m_s))
+    out = O.compute_step_stars(psi, pos, vel, 0.0, 0.0, np.inf, 1e-3, m_s, g, ph)
+    assert abs(O.mass_total(out[0], g, np.full(len(pos), m_s)) / M0 - 1) < 1e-13
+    assert out[4] > 0 and np.isclose(out[5], np.min(np.abs(g.dx / out[2])))
+    assert np.all((out[1] >= 0) & (out[1] <= 1))
+
+
+def test_plain_step_second_order():
+    g = O.Grid(16, 16, 16)
+    ph = O.Physics(rho_bar=1.0, four_pi_G=300.0)
+    psi0 = O.filtered_gaussian_psi(g, 21, sigma_k=2 * np.pi)
+    T = 0.02
+
+    def run(n):
+        p = psi0
+        for _ in range(n):
+            p = O.compute_step_plain(p, T / n, g, ph)
+        return p
+
+    a, b, c = run(4), run(8), run(16)
+    r = np.linalg.norm(a - b) / np.linalg.norm(b - c)
+    assert 3.5 < r < 4.5
+
+
+def test_plain_step_equals_stars_step_without_stars():
+    g = O.Grid(16, 16, 16)
+    ph = O.Physics(rho_bar=1.0)
+    psi = O.filtered_gaussian_psi(g, 22)
+    a = O.compute_step_plain(psi, 1e-3, g, ph)
+    z = np.zeros((0, 3))
+    b = O.compute_step_stars(psi, z, z, 0.0, 0.0, np.inf, 1e-3, 0.0, g, ph)[0]
+    assert np.array_equal(a, b)
+
+
+# ---------------------------------------------------------------- hydro ----
+
+def _gas(g, seed=30, amp=0.2):
+    rng = np.random.default_rng(seed)
+    X, Y, Z = centres(g)
+    rho = 1.0 + amp * np.sin(2 * np.pi * X) * np.cos(4 * np.pi * Y) + 0.05 * rng.random(g.shape)
+    vx = 0.1 * np.cos(2 * np.pi * Z) + 0.02 * rng.standard_normal(g.shape)
+    vy = 0.05 * np.sin(2 * np.pi * X)
+    vz = -0.07 * np.cos(2 * np.pi * Y) + 0.02 * rng.standard_normal(g.shape)
+    return rho, vx, vy, vz
+
+
+def test_hydro_uniform_state_is_stationary():
+    g = O.Grid(8, 8, 8)
+    rho = np.full(g.shape, 0.7)
+    v = np.full(g.shape, 0.3)
+    out = O.solve_hydro(rho, v, 0 * v, -v, 1e-3, g.dx, g.vol, 0.1)
+    assert np.allclose(out[0], 0.7, rtol=1e-14) and np.allclose(out[1], 0.3, rtol=1e-13)
+    assert np.allclose(out[2], 0, atol=1e-14) and np.allclose(out[3], -0.3, rtol=1e-13)
+
+
+def test_hydro_conserves_mass_and_momentum():
+    g = O.Grid(16, 16, 16)
+    rho, vx, vy, vz = _gas(g)
+    out = O.solve_hydro(rho, vx, vy, vz, 2e-3, g.dx, g.vol, 0.5)
+    M0, M1 = rho.sum(), out[0].sum()
+    assert abs(M1 / M0 - 1) < 1e-14
+    for a, b in ((vx, out[1]), (vy, out[2]), (vz, out[3])):
+        assert abs((rho * a).sum() - (out[0] * b).sum()) < 1e-12 * M0
+
+
+def test_limiter_is_minmod():
+    rng = np.random.default_rng(31)
+    f = rng.standard_normal((16, 16, 16))
+    dx = 0.1
+    grads = O.get_gradient(f, dx)
+    lim = O.slope_limiter(f, dx, *grads)
+    for a in range(3):
+        back = (f - np.roll(f, 1, axis=a)) / dx
+        fwd = (np.roll(f, -1, axis=a) - f) / dx
+        mm = np.where(back * fwd > 0, np.sign(back) * np.minimum(np.abs(back), np.abs(fwd)), 0)
+        assert np.allclose(lim[a], mm, rtol=1e-9, atol=1e-9)
+
+
+@pytest.mark.parametrize("perm", [(1, 0, 2), (2, 1, 0), (0, 2, 1)])
+def test_hydro_axis_permutation_symmetry(perm):
+    # pins the rotated argument order of the Y and Z fluxes
+    g = O.Grid(8, 8, 8)
+    rho, vx, vy, vz = _gas(g, 32)
+    v = [vx, vy, vz]
+    out = O.solve_hydro(rho, vx, vy, vz, 1e-3, g.dx, g.vol, 0.3)
+    tr = lambda a: np.transpose(a, perm)
+    vp = [tr(v[perm[0]]), tr(v[perm[1]]), tr(v[perm[2]])]
+    outp = O.solve_hydro(tr(rho), *vp, 1e-3, g.dx, g.vol, 0.3)
+    assert np.allclose(outp[0], tr(out[0]), atol=1e-14)
+    for d in range(3):
+        assert np.allclose(outp[1 + d], tr(out[1 + perm[d]]), atol=1e-13)
+
+
+def test_hydro_mirror_symmetry():
+    g = O.Grid(8, 8, 8)
+    rho, vx, vy, vz = _gas(g, 33)
+    out = O.solve_hydro(rho, vx, vy, vz, 1e-3, g.dx, g.vol, 0.3)
+    # reflect x: cell i -> n-1-i, vx -> -vx
+    fl = lambda a: a[::-1, :, :].copy()
+    o2 = O.solve_hydro(fl(rho), -fl(vx), fl(vy), fl(vz), 1e-3, g.dx, g.vol, 0.3)
+    assert np.allclose(o2[0], fl(out[0]), atol=1e-14)
+    assert np.allclose(o2[1], -fl(out[1]), atol=1e-13)
+
+
+def test_hydro_sound_wave_speed():
+    n = 64
+    g = O.Grid(n, 4, 4)
+    # a 1-D right-moving acoustic wave along x: the phase of the k=1 mode
+    # advances by k cs t
+    cs = 1.0
+    X = (np.arange(n) + 0.5) / n
+    eps = 1e-4
+    rho = (1 + eps * np.sin(2 * np.pi * X))[:, None, None] * np.ones(g.shape)
+    vx = (eps * cs * np.sin(2 * np.pi * X))[:, None, None] * np.ones(g.shape)
+    z = np.zeros(g.shape)
+    dt = 0.2 * g.dx / cs
+    nsteps = 40
+    for _ in range(nsteps):
+        rho, vx, _, _ = O.solve_hydro(rho, vx, z, z, dt, g.dx, g.dx ** 3, cs)
+    ph0 = np.angle(np.fft.fft(np.sin(2 * np.pi * X))[1])
+    ph1 = np.angle(np.fft.fft(rho[:, 0, 0] - 1)[1])
+    shift = -(ph1 - ph0) / (2 * np.pi)
+    assert abs(shift - cs * dt * nsteps) < 0.02 * cs * dt * nsteps
+
+
+def test_gas_step_conserves_mass():
+    g = O.Grid(16, 16, 16)
+    ph = O.Physics(rho_bar=1.2, cs=0.5)
+    psi = O.filtered_gaussian_psi(g, 34, sigma_k=8 * np.pi)
+    rho, vx, vy, vz = _gas(g, 35)
+    rho = 0.2 * rho
+    M0 = O.mass_total(psi, g, rho_gas=rho)
+    out = O.compute_step_gas(psi, rho, vx, vy, vz, 0.0, 1e-3, g, ph)
+    assert abs(O.mass_total(out[0], g, rho_gas=out[1]) / M0 - 1) < 1e-13
+
+
+def test_ic_definitions():
+    g = O.Grid(16, 16, 16)
+    psi = O.filtered_gaussian_psi(g, 17)
+    assert abs(np.mean(np.abs(psi) ** 2) - 1) < 1e-14
+    pos, vel, m_s = O.star_ic(g, 1000, 1017)
+    assert pos.min() >= 0 and pos.max() < 1 and abs(m_s * 1000 - 0.1) < 1e-15
+    assert abs(vel.std() - 0.05) < 0.005
+    rho, *_ = O.gas_ic(g, 17)
+    assert rho.min() >= 0.01 and abs(rho.mean() - 0.2) < 1e-12
