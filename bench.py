@@ -1,0 +1,331 @@
+"""Benchmark of the KDK Schroedinger-Poisson step on one MI355X.
+
+Workload (the headline configuration): a 256^3 complex64 psi (seeded
+filtered Gaussian, mean |psi|^2 = 1) plus 2^18 equal-mass FP64 stars
+(uniform positions, N(0, 0.05^2) velocities, total mass 0.1, rho_bar = 1.1),
+dt = 5e-4.  One step = one full kick-drift-kick step of psi and the stars.
+
+Prints ONE JSON line: cell-updates/s of the fused gfx950 path, the roofline
+of its dominant kernel (per-stage HIP-event timing), the plain complex128
+PyTorch path on the same GPU, accuracy against the FP64 (complex128) path
+after the same number of steps, and the CPU restatement of the reference
+path (FP64 NumPy/SciPy, timed on a bounded sample on this host).
+
+Usage: python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--n 256]
+                       [--stars 262144] [--no-cpu] [--no-plain] [--no-accuracy]
+"""
+
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def stage_bytes(name, nx, ny, nz, c, r, n_p, stars):
+    """Algorithmic HBM bytes of one launch of a stage (read + write once)."""
+    N = nx * ny * nz
+    Nh = (nx // 2 + 1) * ny * nz
+    src = 2 * r if stars else r          # stars: read rho_s and clear it
+    table = {
+        "drift_z_kick": N * (2 * c + r),
+        "drift_z": 2 * N * c, "drift_y": 2 * N * c, "drift_x": 2 * N * c,
+        "drift_x_source": 2 * N * c + N * src + Nh * c,
+        "source_x": N * c + N * src + Nh * c,
+        "fft_y_half": 2 * Nh * c,
+        "poisson_z": 3 * Nh * c,
+        "poisson_y": 5 * Nh * c,
+        "poisson_x": 3 * Nh * c + 4 * N * r,
+        "deposit": n_p * (24 + 8 * r),
+        "star_kick_drift_deposit": n_p * (96 + 24 * r + 8 * r),
+        "star_kick": n_p * (72 + 24 * r),
+        "kick": N * (2 * c + r),
+    }
+    return table.get(name)
+
+
+def stream_copy_gbs(dev):
+    n = 1 << 28  # 1 GiB of float32
+    a = torch.empty(n, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    for _ in range(3):
+        b.copy_(a)
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    s.record()
+    reps = 20
+    for _ in range(reps):
+        b.copy_(a)
+    e.record()
+    torch.cuda.synchronize()
+    ms = s.elapsed_time(e) / reps
+    del a, b
+    return 2 * n * 4 / (ms * 1e-3) / 1e9
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--stars", type=int, default=1 << 18)
+    ap.add_argument("--dt", type=float, default=5e-4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: every box in this setup has one GPU and the "
+              "multi-GPU decomposition is a design (see DESIGN.md), so --gpus must be 1.",
+              file=sys.stderr)
+        sys.exit(2)
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        sys.exit(2)
+
+    qj = importlib.import_module("quantum-jax_amd")
+    from importlib import import_module
+    lib_mod = import_module("quantum-jax_amd._lib")
+    diag_mod = import_module("quantum-jax_amd.diagnostics")
+    dev = torch.device("cuda", 0)
+    n, n_p, dt = args.n, args.stars, args.dt
+    N = n ** 3
+    phys = qj.Physics(rho_bar=1.1)
+
+    log(f"IC: {n}^3 psi seed 17, {n_p} stars seed 1017")
+    psi0 = qj.ic.filtered_gaussian_psi((n, n, n), 17)
+    pos0, vel0, m_s = qj.ic.star_ic(n_p, 1017)
+    psi_d = torch.as_tensor(psi0).to(dev, torch.complex64)
+    pos_d = torch.as_tensor(pos0).to(dev)
+    vel_d = torch.as_tensor(vel0).to(dev)
+
+    plan = qj.Plan(n, dtype=torch.complex64, device=dev)
+    sim = qj.KDKSimulation(plan, phys, dt, psi_d, pos_d, vel_d, mass=m_s)
+
+    # ---- warmup + timed region ------------------------------------------
+    log(f"warmup {args.warmup} steps")
+    sim.step(args.warmup)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    log(f"timed {args.steps} steps")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record()
+    sim.step(args.steps)
+    sim.finalize()
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_total = ev0.elapsed_time(ev1)
+    ms_step = ms_total / args.steps
+    value = N * args.steps / (ms_total * 1e-3)
+    log(f"{ms_step:.4f} ms/step, {value:.4e} cell-updates/s (host wall {wall:.3f}s)")
+
+    # ---- per-stage timing pass (HIP events around every launch) ---------
+    L = lib_mod.lib()
+    nst = L.qj_stage_count()
+    names = [L.qj_stage_name(k).decode() for k in range(nst)]
+    L.qj_timing_enable(plan.handle, 1)
+    tsteps = max(10, min(50, args.steps))
+    sim.step(tsteps)
+    import ctypes
+    ms_arr = (ctypes.c_double * nst)()
+    cnt_arr = (ctypes.c_longlong * nst)()
+    lib_mod.check(L.qj_timing_read(plan.handle, ms_arr, cnt_arr, nst), "timing_read")
+    L.qj_timing_enable(plan.handle, 0)
+    c_b, r_b = 8, 4
+    stages = {}
+    for k in range(nst):
+        if cnt_arr[k]:
+            avg = ms_arr[k] / cnt_arr[k]
+            b = stage_bytes(names[k], n, n, n, c_b, r_b, n_p, True)
+            stages[names[k]] = {
+                "launches_per_step": round(cnt_arr[k] / tsteps, 3),
+                "avg_ms": round(avg, 5),
+                "bytes_per_launch": b,
+                "gbs": round(b / (avg * 1e-3) / 1e9, 1) if b else None,
+            }
+    dom = max(stages, key=lambda s: ms_arr[names.index(s)])
+    dstat = stages[dom]
+    achieved = dstat["gbs"]
+    log(f"dominant kernel {dom}: {dstat['avg_ms']} ms, {achieved} GB/s")
+    step_bytes = sum(v["bytes_per_launch"] * v["launches_per_step"] for v in stages.values()
+                     if v["bytes_per_launch"])
+    out = {
+        "metric": "KDK cell-updates/s at 256^3 (c64 vs plain c128, same GPU) + adjoint step cost",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic (seeded filtered-Gaussian psi, uniform FP64 stars)",
+        "config": {"workload": f"stars {n}^3 c64 psi + {n_p} FP64 stars, dt={dt}",
+                   "grid": [n, n, n], "stars": n_p, "dt": dt, "parallelism": "none (1 GPU)"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "step_bytes_algorithmic": int(step_bytes),
+        "step_gbs_effective": round(step_bytes / (ms_step * 1e-3) / 1e9, 1),
+        "schedule_s_fraction": round(227.4 * value / 8.0e12, 4) if n == 256 else None,
+        "stages": stages,
+    }
+    del sim
+    torch.cuda.empty_cache()
+
+    out["stream_copy_gbs"] = round(stream_copy_gbs(dev), 1)
+
+    # ---- accuracy vs the FP64 (complex128) path ----------------------------
+    if not args.no_accuracy:
+        log("accuracy: c64 and c128 fused paths from the same IC")
+        nacc = args.steps
+        res = {}
+        for tag, cd in (("c64", torch.complex64), ("c128", torch.complex128)):
+            pl = qj.Plan(n, dtype=cd, device=dev)
+            s = qj.KDKSimulation(pl, phys, dt, torch.as_tensor(psi0).to(dev, cd),
+                                 torch.as_tensor(pos0).to(dev), torch.as_tensor(vel0).to(dev),
+                                 mass=m_s)
+            E0 = diag_mod.energy(pl, phys, s.psi, s.pos, s.vel, m_s)
+            M0 = diag_mod.mass(pl, s.psi, torch.full((n_p,), m_s, dtype=torch.float64,
+                                                     device=dev))
+            s.step(nacc)
+            st = s.state()
+            E1 = diag_mod.energy(pl, phys, st["psi"], st["pos"], st["vel"], m_s)
+            M1 = diag_mod.mass(pl, st["psi"], torch.full((n_p,), m_s, dtype=torch.float64,
+                                                         device=dev))
+            res[tag] = dict(psi=st["psi"].to(torch.complex128), pos=st["pos"].clone(),
+                            E0=E0, E1=E1, M0=M0, M1=M1)
+            del s, pl
+            torch.cuda.empty_cache()
+        a, b = res["c64"], res["c128"]
+        err = ((a["psi"] - b["psi"]).norm() / b["psi"].norm()).item()
+        dpos = ((a["pos"] - b["pos"] + 0.5) % 1.0 - 0.5).abs().max().item()
+        out["accuracy"] = {
+            "steps": nacc,
+            "psi_rel_l2_vs_fp64": err,
+            "star_pos_max_abs_vs_fp64": dpos,
+            "mass_drift_c64": abs(a["M1"] - a["M0"]) / a["M0"],
+            "mass_drift_c128": abs(b["M1"] - b["M0"]) / b["M0"],
+            "energy_drift_c64": abs(a["E1"] - a["E0"]) / abs(a["E0"]),
+            "energy_drift_c128": abs(b["E1"] - b["E0"]) / abs(b["E0"]),
+            "fp64_reference": "the complex128 build of the same fused kernels, pinned to the "
+                              "FP64 NumPy oracle at 1e-11 in tests/test_gpu_parity.py",
+        }
+        log(f"accuracy: {out['accuracy']}")
+        del res, a, b
+        torch.cuda.empty_cache()
+
+    # ---- HIP complex128 and the plain complex128 PyTorch path ----------------
+    if not args.no_plain:
+        k128 = max(5, min(20, args.steps // 10))
+        pl = qj.Plan(n, dtype=torch.complex128, device=dev)
+        s = qj.KDKSimulation(pl, phys, dt, torch.as_tensor(psi0).to(dev, torch.complex128),
+                             torch.as_tensor(pos0).to(dev), torch.as_tensor(vel0).to(dev),
+                             mass=m_s)
+        s.step(2)
+        torch.cuda.synchronize()
+        ev0.record()
+        s.step(k128)
+        s.finalize()
+        ev1.record()
+        torch.cuda.synchronize()
+        hip128 = N * k128 / (ev0.elapsed_time(ev1) * 1e-3)
+        del s, pl
+        torch.cuda.empty_cache()
+        plain = importlib.import_module("quantum-jax_amd.plain")
+        pk = plain.PlainKDK((n, n, n), dtype=torch.complex128, device=dev, rho_bar=1.1)
+        p = torch.as_tensor(psi0).to(dev, torch.complex128)
+        x = torch.as_tensor(pos0).to(dev)
+        v = torch.as_tensor(vel0).to(dev)
+        amax = torch.zeros((), dtype=torch.float64, device=dev)
+        p, x, v, amax, _ = pk.step_stars(p, x, v, m_s, dt, amax)
+        kp = 3
+        torch.cuda.synchronize()
+        ev0.record()
+        for _ in range(kp):
+            p, x, v, amax, _ = pk.step_stars(p, x, v, m_s, dt, amax)
+        ev1.record()
+        torch.cuda.synchronize()
+        plain128 = N * kp / (ev0.elapsed_time(ev1) * 1e-3)
+        out["plain_c128_same_gpu"] = {
+            "value": plain128, "unit": "cell-updates/s", "steps": kp,
+            "what": "reference operator sequence in unfused torch ops (torch.fft, index_add_)",
+        }
+        out["hip_c128"] = {"value": hip128, "unit": "cell-updates/s", "steps": k128}
+        out["speedup_vs_plain_c128"] = value / plain128
+        log(f"plain c128 {plain128:.3e}, hip c128 {hip128:.3e}, speedup {value / plain128:.1f}x")
+        del p, x, v, pk
+        torch.cuda.empty_cache()
+
+    # ---- CPU restatement of the reference path (bounded sample) -------------
+    if not args.no_cpu:
+        import oracle as O
+        thr = cpu_threads()
+        O.set_workers(thr)
+        g = O.Grid(n, n, n)
+        ph = O.Physics(rho_bar=1.1)
+        st = (psi0, pos0, vel0, 0.0, 0.0, np.inf)
+        log(f"cpu baseline: oracle step at {n}^3, {thr} threads")
+        st = O.compute_step_stars(*st, dt, m_s, g, ph)   # warm-up
+        times = []
+        for _ in range(2):
+            t = time.perf_counter()
+            st = O.compute_step_stars(*st, dt, m_s, g, ph)
+            times.append(time.perf_counter() - t)
+        tmed = float(np.median(times))
+        out["cpu_baseline"] = {
+            "value": N / tmed, "unit": "cell-updates/s", "cores": thr, "kind": "port",
+            "sample": f"1 warm-up + {len(times)} timed steps of the same {n}^3 + {n_p}-star "
+                      f"workload, FP64 NumPy/SciPy restatement of the reference operator order "
+                      f"(unmerged kicks, 14 FFTs/step, scipy.fft workers={thr}); "
+                      f"median {tmed:.2f} s/step; CPU: {cpu_model()}",
+        }
+        log(f"cpu: {tmed:.2f} s/step")
+
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

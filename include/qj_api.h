@@ -157,6 +157,16 @@ int qj_kdk_steps(qj_plan* plan, qj_state* state, const qj_params* prm, int nstep
                  void* stream);
 int qj_kdk_finalize(qj_plan* plan, qj_state* state, const qj_params* prm, void* stream);
 
+/* ---- stage timing (tracing hook; no counterpart in the reference, which
+ * only wall-clocks its chunk loop at quantum-stars.py:451,467) ----------
+ * When enabled, every kernel launch of the plan is bracketed by a pair of
+ * HIP events on its stream; qj_timing_read synchronises on them and returns
+ * (then clears) the summed milliseconds and launch count per stage. */
+int qj_stage_count(void);
+const char* qj_stage_name(int stage);
+int qj_timing_enable(qj_plan* plan, int enable);
+int qj_timing_read(qj_plan* plan, double* ms, long long* launches, int nstages);
+
 #ifdef __cplusplus
 }
 #endif

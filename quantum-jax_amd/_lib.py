@@ -61,6 +61,10 @@ SIGNATURES = {
     "qj_solve_hydro": (i32, [vp, vp, vp, vp, vp, f64, f64, vp, vp, vp, vp, vp]),
     "qj_kdk_steps": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), i32, vp, vp]),
     "qj_kdk_finalize": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), vp]),
+    "qj_stage_count": (i32, []),
+    "qj_stage_name": (C.c_char_p, [i32]),
+    "qj_timing_enable": (i32, [vp, i32]),
+    "qj_timing_read": (i32, [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong), i32]),
 }
 
 _lock = threading.Lock()

@@ -4,6 +4,7 @@
 #include <string.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/qj_api.h"
 #include "qj_common.h"
@@ -51,6 +52,17 @@ hipError_t fill(void* a, long long n, T v, hipStream_t s) {
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// Kernel stages that the optional event timer tells apart.
+enum Stage {
+  kStageDriftZKick = 0, kStageDriftZ, kStageDriftY, kStageDriftX, kStageDriftXSource,
+  kStageSourceX, kStageFftY, kStagePoissonZ, kStagePoissonY, kStagePoissonX, kStageDeposit,
+  kStageStarKDD, kStageStarKick, kStageHydro, kStageGasKick, kStageKick, kStageCount
+};
+const char* kStageNames[kStageCount] = {
+    "drift_z_kick", "drift_z", "drift_y", "drift_x", "drift_x_source", "source_x", "fft_y_half",
+    "poisson_z", "poisson_y", "poisson_x", "deposit", "star_kick_drift_deposit", "star_kick",
+    "hydro", "gas_kick", "kick"};
+
 }  // namespace
 
 struct qj_plan {
@@ -64,12 +76,61 @@ struct qj_plan {
   double* partials = nullptr;
   unsigned int* counter = nullptr;
   int nblk_src = 0;
+  // optional per-stage event timing (tracing hook of the bench harness)
+  int timing = 0;
+  struct Rec {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  double stage_ms[kStageCount] = {};
+  long long stage_n[kStageCount] = {};
+
+  hipEvent_t take() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
 
   long long ncell() const { return (long long)d.nx * d.ny * d.nz; }
   long long nhalf() const { return (long long)(d.nx / 2 + 1) * d.ny * d.nz; }
 };
 
 namespace {
+
+// Records a (begin, end) event pair around one launch when timing is on.
+struct StageTimer {
+  qj_plan* p;
+  int id;
+  hipStream_t s;
+  hipEvent_t a = nullptr;
+  StageTimer(qj_plan* p_, int id_, hipStream_t s_) : p(p_), id(id_), s(s_) {
+    if (p->timing) {
+      a = p->take();
+      if (a) (void)hipEventRecord(a, s);
+    }
+  }
+  ~StageTimer() {
+    if (!a) return;
+    hipEvent_t b = p->take();
+    if (!b) return;
+    (void)hipEventRecord(b, s);
+    p->recs.push_back({id, a, b});
+  }
+};
+
+#define QJ_STAGE(id, expr, where)                      \
+  do {                                                 \
+    StageTimer st_(p, id, s);                          \
+    hipError_t e_ = (expr);                            \
+    if (e_ != hipSuccess) return hipfail(e_, where);   \
+  } while (0)
 
 struct Work {
   void* H;
@@ -128,11 +189,11 @@ int poisson_tail(qj_plan* p, const Work& w, int grad, double four_pi_G, double r
                  int dc_mode, void* V, void* ax, void* ay, void* az, const void* addend,
                  double* diag, hipStream_t s) {
   SpecTables<T> tb = tables<T>(p);
-  QJ_CHECK(launch_fft_y_half<T>(p->d, tb, w.H, s), "fft_y");
-  QJ_CHECK(launch_poisson_z<T>(p->d, tb, grad, w.H, w.Uz, four_pi_G, rho_bar, dc_mode, diag, s),
+  QJ_STAGE(kStageFftY, launch_fft_y_half<T>(p->d, tb, w.H, s), "fft_y");
+  QJ_STAGE(kStagePoissonZ, launch_poisson_z<T>(p->d, tb, grad, w.H, w.Uz, four_pi_G, rho_bar, dc_mode, diag, s),
            "poisson_z");
-  QJ_CHECK(launch_poisson_y<T>(p->d, tb, grad, w.H, w.Uz, w.Uy, s), "poisson_y");
-  QJ_CHECK(launch_poisson_x<T>(p->d, tb, grad, w.H, w.Uy, w.Uz, V, ax, ay, az, addend, diag, s),
+  QJ_STAGE(kStagePoissonY, launch_poisson_y<T>(p->d, tb, grad, w.H, w.Uz, w.Uy, s), "poisson_y");
+  QJ_STAGE(kStagePoissonX, launch_poisson_x<T>(p->d, tb, grad, w.H, w.Uy, w.Uz, V, ax, ay, az, addend, diag, s),
            "poisson_x");
   return QJ_OK;
 }
@@ -151,7 +212,7 @@ int get_potential_t(qj_plan* p, const void* rho, double rho_bar, double four_pi_
   void* brho = (b != 0.0) ? w.R : nullptr;
   // diag scratch: the source mass lands in the plan-owned partials' tail
   double* diag = p->partials + p->nblk_src;
-  QJ_CHECK(launch_source_x<T>(p->d, tb, 0, nullptr, (void*)rho, 0, brho, b, rho_bar, w.H,
+  QJ_STAGE(kStageSourceX, launch_source_x<T>(p->d, tb, 0, nullptr, (void*)rho, 0, brho, b, rho_bar, w.H,
                               p->partials, p->counter, diag, s),
            "source_x");
   return poisson_tail<T>(p, w, 0, four_pi_G, rho_bar, dc_mode, V, nullptr, nullptr, nullptr, brho,
@@ -165,7 +226,7 @@ int accel_grid_t(qj_plan* p, const void* rho, double rho_bar, double four_pi_G, 
   SpecTables<T> tb = tables<T>(p);
   // the source mass and max|a| land in the caller's diag, else in plan scratch
   double* dg = diag ? diag : p->partials + p->nblk_src;
-  QJ_CHECK(launch_source_x<T>(p->d, tb, 0, nullptr, (void*)rho, 0, nullptr, 0.0, rho_bar, w.H,
+  QJ_STAGE(kStageSourceX, launch_source_x<T>(p->d, tb, 0, nullptr, (void*)rho, 0, nullptr, 0.0, rho_bar, w.H,
                               p->partials, p->counter, dg, s),
            "source_x");
   void* Vout = V ? V : w.R;  // the pass always writes V
@@ -224,6 +285,11 @@ int qj_plan_create(qj_plan** out, int nx, int ny, int nz, double Lx, double Ly, 
 
 int qj_plan_destroy(qj_plan* p) {
   if (!p) return QJ_OK;
+  for (auto& r : p->recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (hipEvent_t e : p->pool) (void)hipEventDestroy(e);
   for (int a = 0; a < 3; ++a) {
     if (p->tw[a]) (void)hipFree(p->tw[a]);
     if (p->drift[a]) (void)hipFree(p->drift[a]);
@@ -231,6 +297,41 @@ int qj_plan_destroy(qj_plan* p) {
   if (p->partials) (void)hipFree(p->partials);
   if (p->counter) (void)hipFree(p->counter);
   delete p;
+  return QJ_OK;
+}
+
+int qj_stage_count(void) { return kStageCount; }
+
+const char* qj_stage_name(int stage) {
+  return (stage >= 0 && stage < kStageCount) ? kStageNames[stage] : "";
+}
+
+int qj_timing_enable(qj_plan* p, int enable) {
+  if (!p) return fail(QJ_EARG, "null plan");
+  p->timing = enable ? 1 : 0;
+  return QJ_OK;
+}
+
+int qj_timing_read(qj_plan* p, double* ms, long long* launches, int nstages) {
+  if (!p || nstages < 0 || (nstages > 0 && (!ms || !launches)))
+    return fail(QJ_EARG, "qj_timing_read: bad arguments");
+  for (auto& r : p->recs) {
+    float t = 0.f;
+    hipError_t e = hipEventSynchronize(r.b);
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, r.a, r.b);
+    if (e != hipSuccess) return hipfail(e, "qj_timing_read");
+    p->stage_ms[r.stage] += t;
+    p->stage_n[r.stage] += 1;
+    p->pool.push_back(r.a);
+    p->pool.push_back(r.b);
+  }
+  p->recs.clear();
+  for (int k = 0; k < nstages && k < kStageCount; ++k) {
+    ms[k] = p->stage_ms[k];
+    launches[k] = p->stage_n[k];
+    p->stage_ms[k] = 0.0;
+    p->stage_n[k] = 0;
+  }
   return QJ_OK;
 }
 
@@ -277,12 +378,12 @@ int qj_drift(qj_plan* p, void* psi, double dt, double m_per_hbar, void* stream) 
     if (int rc = ensure_drift<float>(p, dt, m_per_hbar, s)) return rc;
     SpecTables<float> tb = tables<float>(p);
     for (int a = 2; a >= 0; --a)
-      QJ_CHECK(launch_drift_axis<float>(p->d, tb, a, psi, nullptr, nullptr, 0.0, 0, s), "drift");
+      QJ_STAGE(kStageDriftX - a, launch_drift_axis<float>(p->d, tb, a, psi, nullptr, nullptr, 0.0, 0, s), "drift");
   } else {
     if (int rc = ensure_drift<double>(p, dt, m_per_hbar, s)) return rc;
     SpecTables<double> tb = tables<double>(p);
     for (int a = 2; a >= 0; --a)
-      QJ_CHECK(launch_drift_axis<double>(p->d, tb, a, psi, nullptr, nullptr, 0.0, 0, s), "drift");
+      QJ_STAGE(kStageDriftX - a, launch_drift_axis<double>(p->d, tb, a, psi, nullptr, nullptr, 0.0, 0, s), "drift");
   }
   return QJ_OK;
 }
@@ -404,10 +505,10 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
     if (!st->grid_valid) {
       // potential (and acceleration) of the current state
       if (stars)
-        QJ_CHECK(launch_deposit<T>(p->d, st->pos, st->mass, st->m_scalar, st->np, st->rho_s, s),
+        QJ_STAGE(kStageDeposit, launch_deposit<T>(p->d, st->pos, st->mass, st->m_scalar, st->np, st->rho_s, s),
                  "deposit");
       src = stars ? st->rho_s : (gas ? live_gas(0) : nullptr);
-      QJ_CHECK(launch_source_x<T>(p->d, tb, 0, st->psi, src, stars ? 1 : 0, brho, pr->b_coeff,
+      QJ_STAGE(kStageSourceX, launch_source_x<T>(p->d, tb, 0, st->psi, src, stars ? 1 : 0, brho, pr->b_coeff,
                                   pr->rho_bar, w.H, p->partials, p->counter, st->diag, s),
                "source_x");
       if (int rc = poisson_tail<T>(p, w, grad, pr->four_pi_G, pr->rho_bar, pr->dc_mode, V, ax, ay,
@@ -418,24 +519,24 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
     const double kick_t = st->pending + 0.5 * dt;  // owed half kick + this step's first
     // stars: half kick (cached a), drift, deposit at the new positions
     if (stars)
-      QJ_CHECK(launch_star_kick_drift_deposit<T>(p->d, st->pos, st->vel, st->np, ax, ay, az,
+      QJ_STAGE(kStageStarKDD, launch_star_kick_drift_deposit<T>(p->d, st->pos, st->vel, st->np, ax, ay, az,
                                                  0.5 * dt, dt, st->mass, st->m_scalar, st->rho_s,
                                                  s),
                "star kick/drift/deposit");
     // gas: owed + first half kick of the velocity, then the MUSCL update
     if (gas) {
-      QJ_CHECK(launch_hydro<T>(p->d, live_gas(0), live_gas(1), live_gas(2), live_gas(3), ax, ay, az,
+      QJ_STAGE(kStageHydro, launch_hydro<T>(p->d, live_gas(0), live_gas(1), live_gas(2), live_gas(3), ax, ay, az,
                                kick_t, dt, pr->cs, spare_gas(0), spare_gas(1), spare_gas(2),
                                spare_gas(3), s),
                "hydro");
       st->gas_swapped ^= 1;
     }
     // psi: kick folded into the z drift sweep, then y and x (+ Poisson source)
-    QJ_CHECK(launch_drift_axis<T>(p->d, tb, 2, st->psi, V, brho, pr->m_per_hbar * kick_t, 1, s),
+    QJ_STAGE(kStageDriftZKick, launch_drift_axis<T>(p->d, tb, 2, st->psi, V, brho, pr->m_per_hbar * kick_t, 1, s),
              "drift z");
-    QJ_CHECK(launch_drift_axis<T>(p->d, tb, 1, st->psi, nullptr, nullptr, 0.0, 0, s), "drift y");
+    QJ_STAGE(kStageDriftY, launch_drift_axis<T>(p->d, tb, 1, st->psi, nullptr, nullptr, 0.0, 0, s), "drift y");
     src = stars ? st->rho_s : (gas ? live_gas(0) : nullptr);
-    QJ_CHECK(launch_source_x<T>(p->d, tb, 1, st->psi, src, stars ? 1 : 0, brho, pr->b_coeff,
+    QJ_STAGE(kStageDriftXSource, launch_source_x<T>(p->d, tb, 1, st->psi, src, stars ? 1 : 0, brho, pr->b_coeff,
                                 pr->rho_bar, w.H, p->partials, p->counter, st->diag, s),
              "drift x + source");
     if (int rc = poisson_tail<T>(p, w, grad, pr->four_pi_G, pr->rho_bar, pr->dc_mode, V, ax, ay, az,
@@ -444,7 +545,7 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
     // stars: second half kick with the new acceleration; dt_s
     if (stars) {
       QJ_CHECK(fill<double>(st->diag + kDiagDtS, 1, __builtin_inf(), s), "dt_s reset");
-      QJ_CHECK(launch_star_kick<T>(p->d, st->pos, st->vel, st->np, ax, ay, az, 0.5 * dt, st->diag,
+      QJ_STAGE(kStageStarKick, launch_star_kick<T>(p->d, st->pos, st->vel, st->np, ax, ay, az, 0.5 * dt, st->diag,
                                    s),
                "star kick");
     }
@@ -458,12 +559,12 @@ int kdk_finalize_t(qj_plan* p, qj_state* st, const qj_params* pr, hipStream_t s)
   if (st->pending == 0.0) return QJ_OK;
   void* V = plane<T>(st->grid, p, 0);
   void* brho = (pr->b_coeff != 0.0) ? st->brho : nullptr;
-  QJ_CHECK(launch_kick<T>(st->psi, V, brho, pr->m_per_hbar * st->pending, p->ncell(), s), "kick");
+  QJ_STAGE(kStageKick, launch_kick<T>(st->psi, V, brho, pr->m_per_hbar * st->pending, p->ncell(), s), "kick");
   if (st->gas[0]) {
     void* g1 = st->gas_swapped ? st->gas_tmp[1] : st->gas[1];
     void* g2 = st->gas_swapped ? st->gas_tmp[2] : st->gas[2];
     void* g3 = st->gas_swapped ? st->gas_tmp[3] : st->gas[3];
-    QJ_CHECK(launch_gas_kick<T>(p->ncell(), g1, g2, g3, plane<T>(st->grid, p, 1),
+    QJ_STAGE(kStageGasKick, launch_gas_kick<T>(p->ncell(), g1, g2, g3, plane<T>(st->grid, p, 1),
                                 plane<T>(st->grid, p, 2), plane<T>(st->grid, p, 3), st->pending, s),
              "gas kick");
   }

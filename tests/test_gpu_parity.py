@@ -258,7 +258,15 @@ def test_plain_steps_match_oracle(qj, cuda, dtype):
     for _ in range(5):
         ref = O.compute_step_plain(ref, 1e-3, g, ph)
     assert rel(sim.psi, ref) < TOLS[dtype]
-    assert rel(sim.V, O.get_potential(np.abs(ref) ** 2, g, ph)) < 4 * TOLS[dtype]
+    # V's mean is -4 pi G (mean rho - rho_bar) (k=0 rule): FP32 mass drift of
+    # the sweeps (~1e-7 per sweep) shows up there, so compare the fluctuating
+    # part against the oracle and the mean against the device's own mass.
+    V = sim.V.double().cpu().numpy()
+    Vr = O.get_potential(np.abs(ref) ** 2, g, ph)
+    assert rel(V - V.mean(), Vr - Vr.mean()) < 4 * TOLS[dtype]
+    rho_dev = (sim.psi.abs() ** 2).double().mean().item()
+    assert abs(V.mean() + (rho_dev - 1.0)) < (1e-7 if dtype == C64 else 1e-13)
+    assert abs(rho_dev - 1.0) < (5e-6 if dtype == C64 else 1e-13)
 
 
 @pytest.mark.parametrize("dtype", [C64, C128])
