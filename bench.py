@@ -20,7 +20,6 @@ from __future__ import annotations
 import argparse
 import importlib
 import json
-import math
 import os
 import platform
 import sys
@@ -70,10 +69,10 @@ def stage_bytes(name, nx, ny, nz, c, r, n_p, stars):
         "fft_y_half": 2 * Nh * c,
         "poisson_z": 3 * Nh * c,
         "poisson_y": 5 * Nh * c,
-        "poisson_x": 3 * Nh * c + 4 * N * r,
+        "poisson_x": 3 * Nh * c + 5 * N * r,      # V plane + (a_x, a_y, a_z, V) cells
         "deposit": n_p * (24 + 8 * r),
-        "star_kick_drift_deposit": n_p * (96 + 24 * r + 8 * r),
-        "star_kick": n_p * (72 + 24 * r),
+        "star_step": n_p * (96 + 8 * 4 * r + 8 * r),
+        "star_kick": n_p * (72 + 8 * 4 * r),
         "kick": N * (2 * c + r),
     }
     return table.get(name)

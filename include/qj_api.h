@@ -66,11 +66,13 @@ int qj_version(void);
 int qj_get_potential(qj_plan* plan, const void* rho, double rho_bar, double four_pi_G,
                      double b_coeff, int dc_mode, void* V, void* work, void* stream);
 
-/* a_d = -Re IFFT(1j k_d V_hat) on the grid (3 real planes ax|ay|az) and V;
- * max |a| is folded into diag[QJ_DIAG_AMAX].  quantum-stars.py:254-260,
- * quantum-fluid.py:268-272. */
+/* a_d = -Re IFFT(1j k_d V_hat) on the grid and V; quantum-stars.py:254-260,
+ * quantum-fluid.py:268-272.  a4 is (nx, ny, nz, 4) reals holding
+ * (a_x, a_y, a_z, V) per cell (the reference's a_grid, stacked on the last
+ * axis, plus V); V (nullable) is the plain (nx, ny, nz) potential.  max |a|
+ * is folded into diag[QJ_DIAG_AMAX] (diag nullable). */
 int qj_acceleration_grid(qj_plan* plan, const void* rho, double rho_bar, double four_pi_G,
-                         int dc_mode, void* V, void* a3, double* diag, void* work, void* stream);
+                         int dc_mode, void* V, void* a4, double* diag, void* work, void* stream);
 
 /* psi <- exp(-1j coef V) psi; coef = m_per_hbar*dt/2. quantum-stars.py:301,318 */
 int qj_kick(qj_plan* plan, void* psi, const void* V, double coef, void* stream);
@@ -89,22 +91,22 @@ int qj_cic_indices(qj_plan* plan, const double* pos, long long np, int* i, int* 
 int qj_bin_particles(qj_plan* plan, const double* pos, const double* mass, double m_scalar,
                      long long np, void* rho, int accumulate, void* stream);
 
-/* 8-corner CIC interpolation of the planes a3 = ax|ay|az to the particles;
- * quantum-stars.py:262-287.  acc is double (np, 3). */
-int qj_interp(qj_plan* plan, const double* pos, long long np, const void* a3, double* acc,
+/* 8-corner CIC interpolation of the (a_x, a_y, a_z) components of a4 to the
+ * particles; quantum-stars.py:262-287.  acc is double (np, 3). */
+int qj_interp(qj_plan* plan, const double* pos, long long np, const void* a4, double* acc,
               void* stream);
 
 /* get_acceleration(pos, rho) -> (acc, a_max); quantum-stars.py:250-288.
- * a3 (3 real planes) receives the grid acceleration. */
+ * a4 ((nx,ny,nz,4) reals) receives the grid acceleration and V. */
 int qj_get_acceleration(qj_plan* plan, const double* pos, long long np, const void* rho,
-                        double rho_bar, double four_pi_G, int dc_mode, double* acc, void* a3,
+                        double rho_bar, double four_pi_G, int dc_mode, double* acc, void* a4,
                         double* diag, void* work, void* stream);
 
 /* apply_grav_accel(rho, vx, vy, vz, dt); quantum-fluid.py:267-278 (in place
- * on vx, vy, vz; a3 is 3 real planes of scratch that receive the grid
+ * on vx, vy, vz; a4 ((nx,ny,nz,4) reals of scratch) receives the grid
  * acceleration). */
 int qj_apply_grav_accel(qj_plan* plan, const void* rho, void* vx, void* vy, void* vz, double dt,
-                        double rho_bar, double four_pi_G, int dc_mode, void* a3, double* diag,
+                        double rho_bar, double four_pi_G, int dc_mode, void* a4, double* diag,
                         void* work, void* stream);
 
 /* solve_hydro(rho, vx, vy, vz, dt); quantum-fluid.py:281-349 (cubic cells;
@@ -120,9 +122,9 @@ int qj_solve_hydro(qj_plan* plan, const void* rho, const void* vx, const void* v
  *
  * The second half-kick of a step and the first of the next one use the same
  * potential (|psi|^2, star positions and gas density do not change between
- * them), so the field is kept with a pending kick: after qj_kdk_steps the
- * psi (and gas velocity) buffers lack the last half kick, which
- * qj_kdk_finalize applies.  Star velocities are always complete.
+ * them), so the state is kept with pending kicks: after qj_kdk_steps the
+ * psi, gas-velocity and star-velocity buffers lack the last closing half
+ * kick, which qj_kdk_finalize applies (and then sets diag[QJ_DIAG_DTS]).
  */
 typedef struct {
   void* psi;        /* complex N                                              */
@@ -134,13 +136,14 @@ typedef struct {
   const double* mass; /* (np) or NULL                                         */
   double m_scalar;  /* star mass when mass == NULL                            */
   long long np;
-  void* grid;       /* 4 real planes V|ax|ay|az                               */
+  void* grid;       /* V (N reals) then A4 (N x 4 reals: a_x, a_y, a_z, V)     */
   void* brho;       /* real N, b*rho for self-interaction; NULL when b == 0   */
   double* diag;     /* QJ_DIAG_COUNT doubles                                  */
   /* host-side bookkeeping, initialise to 0 */
   int grid_valid;   /* grid holds V and a of the current state                */
   int gas_swapped;  /* the live gas fields are gas_tmp (after an odd count)   */
   double pending;   /* time of the owed kick of psi and gas velocity          */
+  double pending_star; /* time of the owed closing kick of the star velocities */
 } qj_state;
 
 typedef struct {

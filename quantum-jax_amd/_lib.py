@@ -30,7 +30,7 @@ class QjState(C.Structure):
         ("psi", vp), ("rho_s", vp), ("gas", vp * 4), ("gas_tmp", vp * 4),
         ("pos", vp), ("vel", vp), ("mass", vp), ("m_scalar", f64), ("np", i64),
         ("grid", vp), ("brho", vp), ("diag", vp),
-        ("grid_valid", i32), ("gas_swapped", i32), ("pending", f64),
+        ("grid_valid", i32), ("gas_swapped", i32), ("pending", f64), ("pending_star", f64),
     ]
 
 

@@ -56,11 +56,11 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 enum Stage {
   kStageDriftZKick = 0, kStageDriftZ, kStageDriftY, kStageDriftX, kStageDriftXSource,
   kStageSourceX, kStageFftY, kStagePoissonZ, kStagePoissonY, kStagePoissonX, kStageDeposit,
-  kStageStarKDD, kStageStarKick, kStageHydro, kStageGasKick, kStageKick, kStageCount
+  kStageStarStep, kStageStarKick, kStageHydro, kStageGasKick, kStageKick, kStageCount
 };
 const char* kStageNames[kStageCount] = {
     "drift_z_kick", "drift_z", "drift_y", "drift_x", "drift_x_source", "source_x", "fft_y_half",
-    "poisson_z", "poisson_y", "poisson_x", "deposit", "star_kick_drift_deposit", "star_kick",
+    "poisson_z", "poisson_y", "poisson_x", "deposit", "star_step", "star_kick",
     "hydro", "gas_kick", "kick"};
 
 }  // namespace
@@ -186,18 +186,19 @@ int ensure_drift(qj_plan* p, double dt, double mph, hipStream_t s) {
 // Poisson chain after the source pass: H -> grid (V [, a]).
 template <class T>
 int poisson_tail(qj_plan* p, const Work& w, int grad, double four_pi_G, double rho_bar,
-                 int dc_mode, void* V, void* ax, void* ay, void* az, const void* addend,
+                 int dc_mode, void* V, void* A4, const void* addend,
                  double* diag, hipStream_t s) {
   SpecTables<T> tb = tables<T>(p);
   QJ_STAGE(kStageFftY, launch_fft_y_half<T>(p->d, tb, w.H, s), "fft_y");
   QJ_STAGE(kStagePoissonZ, launch_poisson_z<T>(p->d, tb, grad, w.H, w.Uz, four_pi_G, rho_bar, dc_mode, diag, s),
            "poisson_z");
   QJ_STAGE(kStagePoissonY, launch_poisson_y<T>(p->d, tb, grad, w.H, w.Uz, w.Uy, s), "poisson_y");
-  QJ_STAGE(kStagePoissonX, launch_poisson_x<T>(p->d, tb, grad, w.H, w.Uy, w.Uz, V, ax, ay, az, addend, diag, s),
+  QJ_STAGE(kStagePoissonX, launch_poisson_x<T>(p->d, tb, grad, w.H, w.Uy, w.Uz, V, A4, addend, diag, s),
            "poisson_x");
   return QJ_OK;
 }
 
+// Real plane k of a buffer of N-real planes (the grid is V then A4).
 template <class T>
 void* plane(void* base, const qj_plan* p, int k) {
   return (char*)base + (size_t)k * p->ncell() * sizeof(T);
@@ -215,13 +216,12 @@ int get_potential_t(qj_plan* p, const void* rho, double rho_bar, double four_pi_
   QJ_STAGE(kStageSourceX, launch_source_x<T>(p->d, tb, 0, nullptr, (void*)rho, 0, brho, b, rho_bar, w.H,
                               p->partials, p->counter, diag, s),
            "source_x");
-  return poisson_tail<T>(p, w, 0, four_pi_G, rho_bar, dc_mode, V, nullptr, nullptr, nullptr, brho,
-                         diag, s);
+  return poisson_tail<T>(p, w, 0, four_pi_G, rho_bar, dc_mode, V, nullptr, brho, diag, s);
 }
 
 template <class T>
 int accel_grid_t(qj_plan* p, const void* rho, double rho_bar, double four_pi_G, int dc_mode,
-                 void* V, void* a3, double* diag, void* work, hipStream_t s) {
+                 void* V, void* a4, double* diag, void* work, hipStream_t s) {
   Work w = carve(p, work);
   SpecTables<T> tb = tables<T>(p);
   // the source mass and max|a| land in the caller's diag, else in plan scratch
@@ -230,8 +230,7 @@ int accel_grid_t(qj_plan* p, const void* rho, double rho_bar, double four_pi_G, 
                               p->partials, p->counter, dg, s),
            "source_x");
   void* Vout = V ? V : w.R;  // the pass always writes V
-  return poisson_tail<T>(p, w, 1, four_pi_G, rho_bar, dc_mode, Vout, plane<T>(a3, p, 0),
-                         plane<T>(a3, p, 1), plane<T>(a3, p, 2), nullptr, dg, s);
+  return poisson_tail<T>(p, w, 1, four_pi_G, rho_bar, dc_mode, Vout, a4, nullptr, dg, s);
 }
 
 }  // namespace
@@ -352,12 +351,12 @@ int qj_get_potential(qj_plan* p, const void* rho, double rho_bar, double four_pi
 }
 
 int qj_acceleration_grid(qj_plan* p, const void* rho, double rho_bar, double four_pi_G,
-                         int dc_mode, void* V, void* a3, double* diag, void* work, void* stream) {
+                         int dc_mode, void* V, void* a4, double* diag, void* work, void* stream) {
   if (int rc = check_plan(p)) return rc;
-  if (!rho || !a3 || !work) return fail(QJ_EARG, "qj_acceleration_grid: null buffer");
+  if (!rho || !a4 || !work) return fail(QJ_EARG, "qj_acceleration_grid: null buffer");
   hipStream_t s = (hipStream_t)stream;
-  return QJ_PRECISION(p, accel_grid_t<float>(p, rho, rho_bar, four_pi_G, dc_mode, V, a3, diag, work, s),
-                      accel_grid_t<double>(p, rho, rho_bar, four_pi_G, dc_mode, V, a3, diag, work, s));
+  return QJ_PRECISION(p, accel_grid_t<float>(p, rho, rho_bar, four_pi_G, dc_mode, V, a4, diag, work, s),
+                      accel_grid_t<double>(p, rho, rho_bar, four_pi_G, dc_mode, V, a4, diag, work, s));
 }
 
 int qj_kick(qj_plan* p, void* psi, const void* V, double coef, void* stream) {
@@ -411,49 +410,44 @@ int qj_bin_particles(qj_plan* p, const double* pos, const double* mass, double m
   return e == hipSuccess ? QJ_OK : hipfail(e, "qj_bin_particles");
 }
 
-int qj_interp(qj_plan* p, const double* pos, long long np, const void* a3, double* acc,
+int qj_interp(qj_plan* p, const double* pos, long long np, const void* a4, double* acc,
               void* stream) {
   if (int rc = check_plan(p)) return rc;
-  if (np < 0 || (np > 0 && (!pos || !a3 || !acc))) return fail(QJ_EARG, "qj_interp: bad arguments");
+  if (np < 0 || (np > 0 && (!pos || !a4 || !acc))) return fail(QJ_EARG, "qj_interp: bad arguments");
   hipStream_t s = (hipStream_t)stream;
   hipError_t e;
   if (p->dtype == QJ_C64)
-    e = launch_interp<float>(p->d, pos, np, plane<float>((void*)a3, p, 0),
-                             plane<float>((void*)a3, p, 1), plane<float>((void*)a3, p, 2), acc, s);
+    e = launch_interp<float>(p->d, pos, np, a4, acc, s);
   else
-    e = launch_interp<double>(p->d, pos, np, plane<double>((void*)a3, p, 0),
-                              plane<double>((void*)a3, p, 1), plane<double>((void*)a3, p, 2), acc,
-                              s);
+    e = launch_interp<double>(p->d, pos, np, a4, acc, s);
   return e == hipSuccess ? QJ_OK : hipfail(e, "qj_interp");
 }
 
 int qj_get_acceleration(qj_plan* p, const double* pos, long long np, const void* rho,
-                        double rho_bar, double four_pi_G, int dc_mode, double* acc, void* a3,
+                        double rho_bar, double four_pi_G, int dc_mode, double* acc, void* a4,
                         double* diag, void* work, void* stream) {
   if (int rc = check_plan(p)) return rc;
-  if (!rho || !a3 || !work || np < 0 || (np > 0 && (!pos || !acc)))
+  if (!rho || !a4 || !work || np < 0 || (np > 0 && (!pos || !acc)))
     return fail(QJ_EARG, "qj_get_acceleration: bad arguments");
-  int rc = qj_acceleration_grid(p, rho, rho_bar, four_pi_G, dc_mode, nullptr, a3, diag, work, stream);
+  int rc = qj_acceleration_grid(p, rho, rho_bar, four_pi_G, dc_mode, nullptr, a4, diag, work, stream);
   if (rc) return rc;
-  return qj_interp(p, pos, np, a3, acc, stream);
+  return qj_interp(p, pos, np, a4, acc, stream);
 }
 
 int qj_apply_grav_accel(qj_plan* p, const void* rho, void* vx, void* vy, void* vz, double dt,
-                        double rho_bar, double four_pi_G, int dc_mode, void* a3, double* diag,
+                        double rho_bar, double four_pi_G, int dc_mode, void* a4, double* diag,
                         void* work, void* stream) {
   if (int rc = check_plan(p)) return rc;
-  if (!rho || !vx || !vy || !vz || !a3 || !work)
+  if (!rho || !vx || !vy || !vz || !a4 || !work)
     return fail(QJ_EARG, "qj_apply_grav_accel: null buffer");
-  int rc = qj_acceleration_grid(p, rho, rho_bar, four_pi_G, dc_mode, nullptr, a3, diag, work, stream);
+  int rc = qj_acceleration_grid(p, rho, rho_bar, four_pi_G, dc_mode, nullptr, a4, diag, work, stream);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   hipError_t e;
   if (p->dtype == QJ_C64)
-    e = launch_gas_kick<float>(p->ncell(), vx, vy, vz, plane<float>(a3, p, 0),
-                               plane<float>(a3, p, 1), plane<float>(a3, p, 2), dt, s);
+    e = launch_gas_kick<float>(p->ncell(), vx, vy, vz, a4, dt, s);
   else
-    e = launch_gas_kick<double>(p->ncell(), vx, vy, vz, plane<double>(a3, p, 0),
-                                plane<double>(a3, p, 1), plane<double>(a3, p, 2), dt, s);
+    e = launch_gas_kick<double>(p->ncell(), vx, vy, vz, a4, dt, s);
   return e == hipSuccess ? QJ_OK : hipfail(e, "qj_apply_grav_accel");
 }
 
@@ -468,9 +462,9 @@ int qj_solve_hydro(qj_plan* p, const void* rho, const void* vx, const void* vy, 
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = QJ_PRECISION(
       p,
-      launch_hydro<float>(p->d, rho, vx, vy, vz, nullptr, nullptr, nullptr, 0.0, dt, cs, rho_o,
+      launch_hydro<float>(p->d, rho, vx, vy, vz, nullptr, 0.0, dt, cs, rho_o,
                           vx_o, vy_o, vz_o, s),
-      launch_hydro<double>(p->d, rho, vx, vy, vz, nullptr, nullptr, nullptr, 0.0, dt, cs, rho_o,
+      launch_hydro<double>(p->d, rho, vx, vy, vz, nullptr, 0.0, dt, cs, rho_o,
                            vx_o, vy_o, vz_o, s));
   return e == hipSuccess ? QJ_OK : hipfail(e, "qj_solve_hydro");
 }
@@ -491,10 +485,8 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
   const bool stars = st->np > 0;
   const bool gas = st->gas[0] != nullptr;
   const bool grad = stars || gas;
-  void* V = plane<T>(st->grid, p, 0);
-  void* ax = grad ? plane<T>(st->grid, p, 1) : nullptr;
-  void* ay = grad ? plane<T>(st->grid, p, 2) : nullptr;
-  void* az = grad ? plane<T>(st->grid, p, 3) : nullptr;
+  void* V = st->grid;
+  void* A4 = grad ? plane<T>(st->grid, p, 1) : nullptr;
   void* brho = (pr->b_coeff != 0.0) ? st->brho : nullptr;
   const double dt = pr->dt;
   auto live_gas = [&](int k) { return st->gas_swapped ? st->gas_tmp[k] : st->gas[k]; };
@@ -511,64 +503,73 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
       QJ_STAGE(kStageSourceX, launch_source_x<T>(p->d, tb, 0, st->psi, src, stars ? 1 : 0, brho, pr->b_coeff,
                                   pr->rho_bar, w.H, p->partials, p->counter, st->diag, s),
                "source_x");
-      if (int rc = poisson_tail<T>(p, w, grad, pr->four_pi_G, pr->rho_bar, pr->dc_mode, V, ax, ay,
-                                   az, nullptr, st->diag, s))
+      if (int rc = poisson_tail<T>(p, w, grad, pr->four_pi_G, pr->rho_bar, pr->dc_mode, V, A4,
+                                   nullptr, st->diag, s))
         return rc;
       st->grid_valid = 1;
     }
     const double kick_t = st->pending + 0.5 * dt;  // owed half kick + this step's first
-    // stars: half kick (cached a), drift, deposit at the new positions
+    // stars: owed closing kick of the previous step and this step's opening
+    // half kick (same grid, same positions), drift, deposit at the new positions
     if (stars)
-      QJ_STAGE(kStageStarKDD, launch_star_kick_drift_deposit<T>(p->d, st->pos, st->vel, st->np, ax, ay, az,
-                                                 0.5 * dt, dt, st->mass, st->m_scalar, st->rho_s,
-                                                 s),
-               "star kick/drift/deposit");
+      QJ_STAGE(kStageStarStep,
+               launch_star_step<T>(p->d, st->pos, st->vel, st->np, A4, st->pending_star, 0.5 * dt,
+                                   dt, st->mass, st->m_scalar, st->rho_s, s),
+               "star step");
     // gas: owed + first half kick of the velocity, then the MUSCL update
     if (gas) {
-      QJ_STAGE(kStageHydro, launch_hydro<T>(p->d, live_gas(0), live_gas(1), live_gas(2), live_gas(3), ax, ay, az,
-                               kick_t, dt, pr->cs, spare_gas(0), spare_gas(1), spare_gas(2),
-                               spare_gas(3), s),
+      QJ_STAGE(kStageHydro,
+               launch_hydro<T>(p->d, live_gas(0), live_gas(1), live_gas(2), live_gas(3), A4, kick_t,
+                               dt, pr->cs, spare_gas(0), spare_gas(1), spare_gas(2), spare_gas(3),
+                               s),
                "hydro");
       st->gas_swapped ^= 1;
     }
     // psi: kick folded into the z drift sweep, then y and x (+ Poisson source)
-    QJ_STAGE(kStageDriftZKick, launch_drift_axis<T>(p->d, tb, 2, st->psi, V, brho, pr->m_per_hbar * kick_t, 1, s),
+    QJ_STAGE(kStageDriftZKick,
+             launch_drift_axis<T>(p->d, tb, 2, st->psi, V, brho, pr->m_per_hbar * kick_t, 1, s),
              "drift z");
-    QJ_STAGE(kStageDriftY, launch_drift_axis<T>(p->d, tb, 1, st->psi, nullptr, nullptr, 0.0, 0, s), "drift y");
+    QJ_STAGE(kStageDriftY, launch_drift_axis<T>(p->d, tb, 1, st->psi, nullptr, nullptr, 0.0, 0, s),
+             "drift y");
     src = stars ? st->rho_s : (gas ? live_gas(0) : nullptr);
-    QJ_STAGE(kStageDriftXSource, launch_source_x<T>(p->d, tb, 1, st->psi, src, stars ? 1 : 0, brho, pr->b_coeff,
+    QJ_STAGE(kStageDriftXSource,
+             launch_source_x<T>(p->d, tb, 1, st->psi, src, stars ? 1 : 0, brho, pr->b_coeff,
                                 pr->rho_bar, w.H, p->partials, p->counter, st->diag, s),
              "drift x + source");
-    if (int rc = poisson_tail<T>(p, w, grad, pr->four_pi_G, pr->rho_bar, pr->dc_mode, V, ax, ay, az,
+    if (int rc = poisson_tail<T>(p, w, grad, pr->four_pi_G, pr->rho_bar, pr->dc_mode, V, A4,
                                  nullptr, st->diag, s))
       return rc;
-    // stars: second half kick with the new acceleration; dt_s
-    if (stars) {
-      QJ_CHECK(fill<double>(st->diag + kDiagDtS, 1, __builtin_inf(), s), "dt_s reset");
-      QJ_STAGE(kStageStarKick, launch_star_kick<T>(p->d, st->pos, st->vel, st->np, ax, ay, az, 0.5 * dt, st->diag,
-                                   s),
-               "star kick");
-    }
+    // the closing half kicks (psi, gas and stars) are owed to the next step
     st->pending = 0.5 * dt;
+    if (stars) st->pending_star = 0.5 * dt;
   }
   return QJ_OK;
 }
 
 template <class T>
 int kdk_finalize_t(qj_plan* p, qj_state* st, const qj_params* pr, hipStream_t s) {
-  if (st->pending == 0.0) return QJ_OK;
-  void* V = plane<T>(st->grid, p, 0);
-  void* brho = (pr->b_coeff != 0.0) ? st->brho : nullptr;
-  QJ_STAGE(kStageKick, launch_kick<T>(st->psi, V, brho, pr->m_per_hbar * st->pending, p->ncell(), s), "kick");
-  if (st->gas[0]) {
-    void* g1 = st->gas_swapped ? st->gas_tmp[1] : st->gas[1];
-    void* g2 = st->gas_swapped ? st->gas_tmp[2] : st->gas[2];
-    void* g3 = st->gas_swapped ? st->gas_tmp[3] : st->gas[3];
-    QJ_STAGE(kStageGasKick, launch_gas_kick<T>(p->ncell(), g1, g2, g3, plane<T>(st->grid, p, 1),
-                                plane<T>(st->grid, p, 2), plane<T>(st->grid, p, 3), st->pending, s),
-             "gas kick");
+  void* V = st->grid;
+  void* A4 = plane<T>(st->grid, p, 1);
+  if (st->pending != 0.0) {
+    void* brho = (pr->b_coeff != 0.0) ? st->brho : nullptr;
+    QJ_STAGE(kStageKick,
+             launch_kick<T>(st->psi, V, brho, pr->m_per_hbar * st->pending, p->ncell(), s), "kick");
+    if (st->gas[0]) {
+      void* g1 = st->gas_swapped ? st->gas_tmp[1] : st->gas[1];
+      void* g2 = st->gas_swapped ? st->gas_tmp[2] : st->gas[2];
+      void* g3 = st->gas_swapped ? st->gas_tmp[3] : st->gas[3];
+      QJ_STAGE(kStageGasKick, launch_gas_kick<T>(p->ncell(), g1, g2, g3, A4, st->pending, s),
+               "gas kick");
+    }
+    st->pending = 0.0;
   }
-  st->pending = 0.0;
+  if (st->pending_star != 0.0 && st->np > 0) {
+    QJ_CHECK(fill<double>(st->diag + kDiagDtS, 1, __builtin_inf(), s), "dt_s reset");
+    QJ_STAGE(kStageStarKick,
+             launch_star_kick<T>(p->d, st->pos, st->vel, st->np, A4, st->pending_star, st->diag, s),
+             "star kick");
+    st->pending_star = 0.0;
+  }
   return QJ_OK;
 }
 

@@ -28,6 +28,12 @@ struct PencilShape {
   static constexpr int C = kThreads / TT; // pencils per workgroup
   static constexpr int P = TT + 1;        // padded LDS row (k1-major)
   static constexpr int kLds = (TT > 1) ? C * E * P : 1;  // complex elements
+  // Column-fastest layout for the strided lane mapping (neighbouring lanes
+  // hold neighbouring pencils): element (k1, j) of pencil col lives at
+  // k1*KS + j*C + col.  The 16-element pad per k1 block puts the pencil
+  // groups a wave reads at once into disjoint LDS banks.
+  static constexpr int KS = TT * C + 16;
+  static constexpr int kLdsStrided = (TT > 1) ? E * KS : 1;
 };
 
 // In-register E-point DFT, natural order in and out (radix-2/4 four-step
@@ -86,12 +92,14 @@ __device__ __forceinline__ void dft(cpx<T>* v) {
 // N-point pencil transform over the TT threads that hold one pencil.
 //   v    : the thread's E elements (x[j + TT*m])
 //   j    : thread rank inside the pencil, 0..TT-1
-//   lds  : this pencil's private LDS slice (E*P complex elements)
+//   lds  : CONTIG: this pencil's private slice (E*P complex elements);
+//          strided: the workgroup's shared exchange buffer, and col is the
+//          pencil's slot (column-fastest layout, see PencilShape::KS)
 //   tw   : W_N^p = exp(-2*pi*i*p/N), p = 0..N-1 (conjugated for DIR = +1)
 // Every thread of the workgroup must call this (it contains barriers).
-template <int N, int DIR, class T>
+template <int N, int DIR, class T, bool CONTIG = true>
 __device__ __forceinline__ void pencil_fft(cpx<T>* v, int j, cpx<T>* lds,
-                                           const cpx<T>* __restrict__ tw) {
+                                           const cpx<T>* __restrict__ tw, int col = 0) {
   using S = PencilShape<N>;
   constexpr int E = S::E, TT = S::TT, P = S::P;
   dft<E, DIR, T>(v);
@@ -101,20 +109,117 @@ __device__ __forceinline__ void pencil_fft(cpx<T>* v, int j, cpx<T>* lds,
       cpx<T> w = tw[(j * k1) & (N - 1)];
       v[k1] = (DIR < 0) ? cmul(v[k1], w) : cmulc(v[k1], w);
     }
-    __syncthreads();  // previous readers of this LDS slice are done
+    __syncthreads();  // previous readers of this LDS buffer are done
+    if constexpr (CONTIG) {
 #pragma unroll
-    for (int k1 = 0; k1 < E; ++k1) lds[k1 * P + j] = v[k1];
+      for (int k1 = 0; k1 < E; ++k1) lds[k1 * P + j] = v[k1];
+    } else {
+#pragma unroll
+      for (int k1 = 0; k1 < E; ++k1) lds[k1 * S::KS + j * S::C + col] = v[k1];
+    }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < E / TT; ++q) {
       cpx<T> u[TT];
       const int k1 = j + TT * q;
 #pragma unroll
-      for (int b = 0; b < TT; ++b) u[b] = lds[k1 * P + b];
+      for (int b = 0; b < TT; ++b)
+        u[b] = CONTIG ? lds[k1 * P + b] : lds[k1 * S::KS + b * S::C + col];
       dft<TT, DIR, T>(u);
 #pragma unroll
       for (int kb = 0; kb < TT; ++kb) v[q + (E / TT) * kb] = u[kb];
     }
+  }
+}
+
+// Several pencils per lane in the strided layout: the VW pencils of a lane
+// are neighbours in memory (q0, q0+1), so one row of them is one 16-byte
+// access for c64, and the pencils share the stage twiddles and the two
+// block barriers of the exchange.  All VW pencils use the column-fastest
+// LDS layout, one kLdsStrided block each.
+template <int N, int DIR, class T, int VW>
+__device__ __forceinline__ void pencil_fft_strided(cpx<T> (*v)[PencilShape<N>::E], int j, int col,
+                                                   cpx<T>* lds, const cpx<T>* __restrict__ tw) {
+  using S = PencilShape<N>;
+  constexpr int E = S::E, TT = S::TT, C = S::C, KS = S::KS, L = S::kLdsStrided;
+#pragma unroll
+  for (int w = 0; w < VW; ++w) dft<E, DIR, T>(v[w]);
+  if constexpr (TT > 1) {
+#pragma unroll
+    for (int k1 = 1; k1 < E; ++k1) {
+      const cpx<T> t = tw[(j * k1) & (N - 1)];
+#pragma unroll
+      for (int w = 0; w < VW; ++w) v[w][k1] = (DIR < 0) ? cmul(v[w][k1], t) : cmulc(v[w][k1], t);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < VW; ++w)
+#pragma unroll
+      for (int k1 = 0; k1 < E; ++k1) lds[w * L + k1 * KS + j * C + col] = v[w][k1];
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < VW; ++w) {
+#pragma unroll
+      for (int q = 0; q < E / TT; ++q) {
+        cpx<T> u[TT];
+        const int k1 = j + TT * q;
+#pragma unroll
+        for (int b = 0; b < TT; ++b) u[b] = lds[w * L + k1 * KS + b * C + col];
+        dft<TT, DIR, T>(u);
+#pragma unroll
+        for (int kb = 0; kb < TT; ++kb) v[w][q + (E / TT) * kb] = u[kb];
+      }
+    }
+  }
+}
+
+// Pencils per lane of the strided sweeps: two for c64 up to 256 points
+// (the LDS of two column-fastest blocks still fits four workgroups per CU
+// next to each other's registers), one otherwise.
+template <class T, int N>
+struct StridedVW {
+  static constexpr int value = (sizeof(T) == 4 && N <= 256) ? 2 : 1;
+};
+
+// VW neighbouring complex (or real) values starting at element i.
+template <int VW, class T>
+__device__ __forceinline__ void ldc(const cpx<T>* __restrict__ p, int i, cpx<T>* o) {
+  if constexpr (VW == 2 && sizeof(T) == 4) {
+    const float4 t = *reinterpret_cast<const float4*>(p + i);
+    o[0] = {t.x, t.y};
+    o[1] = {t.z, t.w};
+  } else {
+#pragma unroll
+    for (int w = 0; w < VW; ++w) o[w] = p[i + w];
+  }
+}
+template <int VW, class T>
+__device__ __forceinline__ void stc(cpx<T>* __restrict__ p, int i, const cpx<T>* o) {
+  if constexpr (VW == 2 && sizeof(T) == 4) {
+    *reinterpret_cast<float4*>(p + i) = make_float4(o[0].r, o[0].i, o[1].r, o[1].i);
+  } else {
+#pragma unroll
+    for (int w = 0; w < VW; ++w) p[i + w] = o[w];
+  }
+}
+template <int VW, class T>
+__device__ __forceinline__ void ldr(const T* __restrict__ p, int i, T* o) {
+  if constexpr (VW == 2 && sizeof(T) == 4) {
+    const float2 t = *reinterpret_cast<const float2*>(p + i);
+    o[0] = t.x;
+    o[1] = t.y;
+  } else {
+#pragma unroll
+    for (int w = 0; w < VW; ++w) o[w] = p[i + w];
+  }
+}
+template <int VW, class T>
+__device__ __forceinline__ void str(T* __restrict__ p, int i, const T* o) {
+  if constexpr (VW == 2 && sizeof(T) == 4) {
+    *reinterpret_cast<float2*>(p + i) = make_float2(o[0], o[1]);
+  } else {
+#pragma unroll
+    for (int w = 0; w < VW; ++w) p[i + w] = o[w];
   }
 }
 

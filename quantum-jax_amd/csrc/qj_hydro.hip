@@ -175,39 +175,35 @@ __global__ __launch_bounds__(256) void k_hydro(HGeom<T> g, const T* __restrict__
 template <class T>
 __global__ __launch_bounds__(256) void k_gas_kick(long long n, T* __restrict__ vx,
                                                   T* __restrict__ vy, T* __restrict__ vz,
-                                                  const T* __restrict__ ax,
-                                                  const T* __restrict__ ay,
-                                                  const T* __restrict__ az, T h) {
+                                                  const T* __restrict__ A4, T h) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long stride = (long long)gridDim.x * blockDim.x;
   for (; i < n; i += stride) {  // quantum-fluid.py:274-276
-    vx[i] += ax[i] * h;
-    vy[i] += ay[i] * h;
-    vz[i] += az[i] * h;
+    vx[i] += A4[4 * i] * h;
+    vy[i] += A4[4 * i + 1] * h;
+    vz[i] += A4[4 * i + 2] * h;
   }
 }
 
 }  // namespace
 
 template <class T>
-hipError_t launch_gas_kick(long long n, void* vx, void* vy, void* vz, const void* ax,
-                           const void* ay, const void* az, double h, hipStream_t s) {
+hipError_t launch_gas_kick(long long n, void* vx, void* vy, void* vz, const void* A4, double h,
+                           hipStream_t s) {
   long long blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  k_gas_kick<T><<<(int)blocks, 256, 0, s>>>(n, (T*)vx, (T*)vy, (T*)vz, (const T*)ax,
-                                            (const T*)ay, (const T*)az, (T)h);
+  k_gas_kick<T><<<(int)blocks, 256, 0, s>>>(n, (T*)vx, (T*)vy, (T*)vz, (const T*)A4, (T)h);
   return hipGetLastError();
 }
 
 template <class T>
 hipError_t launch_hydro(const Dims& d, const void* rho, const void* vx, const void* vy,
-                        const void* vz, const void* ax, const void* ay, const void* az,
-                        double kick, double dt, double cs, void* rho_o, void* vx_o, void* vy_o,
-                        void* vz_o, hipStream_t s) {
+                        const void* vz, const void* A4, double kick, double dt, double cs,
+                        void* rho_o, void* vx_o, void* vy_o, void* vz_o, hipStream_t s) {
   if (kick != 0.0) {
     // pending gravitational kick of the gas velocity (applied in place)
     hipError_t e = launch_gas_kick<T>((long long)d.nx * d.ny * d.nz, (void*)vx, (void*)vy,
-                                      (void*)vz, ax, ay, az, kick, s);
+                                      (void*)vz, A4, kick, s);
     if (e != hipSuccess) return e;
   }
   HGeom<T> g;
@@ -227,16 +223,14 @@ hipError_t launch_hydro(const Dims& d, const void* rho, const void* vx, const vo
 }
 
 template hipError_t launch_hydro<float>(const Dims&, const void*, const void*, const void*,
-                                        const void*, const void*, const void*, const void*,
-                                        double, double, double, void*, void*, void*, void*,
-                                        hipStream_t);
+                                        const void*, const void*, double, double, double, void*,
+                                        void*, void*, void*, hipStream_t);
 template hipError_t launch_hydro<double>(const Dims&, const void*, const void*, const void*,
-                                         const void*, const void*, const void*, const void*,
-                                         double, double, double, void*, void*, void*, void*,
-                                         hipStream_t);
-template hipError_t launch_gas_kick<float>(long long, void*, void*, void*, const void*,
-                                           const void*, const void*, double, hipStream_t);
-template hipError_t launch_gas_kick<double>(long long, void*, void*, void*, const void*,
-                                            const void*, const void*, double, hipStream_t);
+                                         const void*, const void*, double, double, double, void*,
+                                         void*, void*, void*, hipStream_t);
+template hipError_t launch_gas_kick<float>(long long, void*, void*, void*, const void*, double,
+                                           hipStream_t);
+template hipError_t launch_gas_kick<double>(long long, void*, void*, void*, const void*, double,
+                                            hipStream_t);
 
 }  // namespace qj

@@ -57,15 +57,16 @@ template <class T>
 hipError_t launch_poisson_y(const Dims& d, const SpecTables<T>& tb, int grad, void* H, void* Uz,
                             void* Uy, hipStream_t s);
 
+// Writes V and, with grad, the cell-interleaved grid A4 = (a_x, a_y, a_z, V).
 template <class T>
 hipError_t launch_poisson_x(const Dims& d, const SpecTables<T>& tb, int grad, const void* H,
-                            const void* Uy, const void* Uzz, void* V, void* ax, void* ay,
-                            void* az, const void* addend, double* diag, hipStream_t s);
+                            const void* Uy, const void* Uzz, void* V, void* A4,
+                            const void* addend, double* diag, hipStream_t s);
 
 // Number of workgroups source_x launches (size of the FP64 partials array).
 int source_x_blocks(const Dims& d);
 
-// ---- particles ----------------------------------------------------------
+// ---- particles (the grid acceleration is A4 = (a_x, a_y, a_z, V) per cell)
 template <class T>
 hipError_t launch_cic_indices(const Dims& d, const double* pos, long long np, int* i, int* ip1,
                               double* w_i, double* w_ip1, hipStream_t s);
@@ -73,29 +74,28 @@ template <class T>
 hipError_t launch_deposit(const Dims& d, const double* pos, const double* mass, double m_scalar,
                           long long np, void* rho, hipStream_t s);
 template <class T>
-hipError_t launch_interp(const Dims& d, const double* pos, long long np, const void* ax,
-                         const void* ay, const void* az, double* acc, hipStream_t s);
-// vel += interp(a)*h1; pos = mod(pos + vel*dt, L); deposit(pos) into rho
+hipError_t launch_interp(const Dims& d, const double* pos, long long np, const void* A4,
+                         double* acc, hipStream_t s);
+// One star step up to the deposit: acc = interp(A4, pos);
+// vel += acc*h_owed (the previous step's closing half kick); vel += acc*h1;
+// pos = mod(pos + vel*dt, L); deposit(pos) into rho.
 template <class T>
-hipError_t launch_star_kick_drift_deposit(const Dims& d, double* pos, double* vel, long long np,
-                                          const void* ax, const void* ay, const void* az,
-                                          double h1, double dt, const double* mass,
-                                          double m_scalar, void* rho, hipStream_t s);
-// vel += interp(a)*h2; diag[dt_s] = min |dx/vel|
+hipError_t launch_star_step(const Dims& d, double* pos, double* vel, long long np, const void* A4,
+                            double h_owed, double h1, double dt, const double* mass,
+                            double m_scalar, void* rho, hipStream_t s);
+// vel += interp(A4)*h; diag[dt_s] = min |dx/vel| (diag[dt_s] must hold +inf)
 template <class T>
 hipError_t launch_star_kick(const Dims& d, const double* pos, double* vel, long long np,
-                            const void* ax, const void* ay, const void* az, double h2,
-                            double* diag, hipStream_t s);
+                            const void* A4, double h, double* diag, hipStream_t s);
 
 // ---- gas ----------------------------------------------------------------
-// Applies v += a*kick first when kick != 0 (a may be null when kick == 0).
+// Applies v += a*kick first when kick != 0 (A4 may be null when kick == 0).
 template <class T>
 hipError_t launch_hydro(const Dims& d, const void* rho, const void* vx, const void* vy,
-                        const void* vz, const void* ax, const void* ay, const void* az,
-                        double kick, double dt, double cs, void* rho_o, void* vx_o, void* vy_o,
-                        void* vz_o, hipStream_t s);
+                        const void* vz, const void* A4, double kick, double dt, double cs,
+                        void* rho_o, void* vx_o, void* vy_o, void* vz_o, hipStream_t s);
 template <class T>
-hipError_t launch_gas_kick(long long n, void* vx, void* vy, void* vz, const void* ax,
-                           const void* ay, const void* az, double h, hipStream_t s);
+hipError_t launch_gas_kick(long long n, void* vx, void* vy, void* vz, const void* A4, double h,
+                           hipStream_t s);
 
 }  // namespace qj

@@ -83,22 +83,47 @@ __device__ __forceinline__ void deposit_one(const CicGeom& g, const Cic& c, doub
   }
 }
 
+// 4 reals of the interleaved grid cell idx (a_x, a_y, a_z, V).
 template <class T>
-__device__ __forceinline__ void interp_one(const CicGeom& g, const Cic& c, const T* ax, const T* ay,
-                                           const T* az, double acc[3]) {
+__device__ __forceinline__ void ld4(const T* __restrict__ A4, int idx, double o[4]) {
+  if constexpr (sizeof(T) == 4) {
+    const float4 t = *reinterpret_cast<const float4*>(A4 + 4 * idx);
+    o[0] = t.x;
+    o[1] = t.y;
+    o[2] = t.z;
+    o[3] = t.w;
+  } else {
+    const double2 a = *reinterpret_cast<const double2*>(A4 + 4 * idx);
+    const double2 b = *reinterpret_cast<const double2*>(A4 + 4 * idx + 2);
+    o[0] = a.x;
+    o[1] = a.y;
+    o[2] = b.x;
+    o[3] = b.y;
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void interp_one(const CicGeom& g, const Cic& c, const T* A4,
+                                           double acc[3]) {
   acc[0] = acc[1] = acc[2] = 0.0;
   if (!c.ok) {
     acc[0] = acc[1] = acc[2] = __builtin_nan("");
     return;
   }
+  // all 8 corner loads are issued before the accumulation
+  double a[8][4];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int sx = kCorner[k][0], sy = kCorner[k][1], sz = kCorner[k][2];
+    ld4(A4, (c.ix[sx] * g.ny + c.iy[sy]) * g.nz + c.iz[sz], a[k]);
+  }
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int sx = kCorner[k][0], sy = kCorner[k][1], sz = kCorner[k][2];
     const double w = c.wx[sx] * c.wy[sy] * c.wz[sz];
-    const int idx = (c.ix[sx] * g.ny + c.iy[sy]) * g.nz + c.iz[sz];
-    acc[0] += w * (double)ax[idx];
-    acc[1] += w * (double)ay[idx];
-    acc[2] += w * (double)az[idx];
+    acc[0] += w * a[k][0];
+    acc[1] += w * a[k][1];
+    acc[2] += w * a[k][2];
   }
 }
 
@@ -146,34 +171,37 @@ __global__ void k_deposit(CicGeom g, const double* __restrict__ pos,
 
 template <class T>
 __global__ void k_interp(CicGeom g, const double* __restrict__ pos, long long np,
-                         const T* __restrict__ ax, const T* __restrict__ ay,
-                         const T* __restrict__ az, double* __restrict__ acc) {
+                         const T* __restrict__ A4, double* __restrict__ acc) {
   long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   Cic c = cic(g, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]);
   double a[3];
-  interp_one(g, c, ax, ay, az, a);
+  interp_one(g, c, A4, a);
   acc[3 * p] = a[0];
   acc[3 * p + 1] = a[1];
   acc[3 * p + 2] = a[2];
 }
 
-// First half of a star step: half kick with the cached grid acceleration,
-// drift with floored periodic wrap, deposit at the new position.
+// One star step up to the deposit: the closing half kick of the previous
+// step (h_owed, same grid and positions), this step's opening half kick,
+// the drift with floored periodic wrap, and the deposit at the new position.
 template <class T>
-__global__ void k_star_kdd(CicGeom g, double* __restrict__ pos, double* __restrict__ vel,
-                           long long np, const T* __restrict__ ax, const T* __restrict__ ay,
-                           const T* __restrict__ az, double h1, double dt,
-                           const double* __restrict__ mass, double m_scalar,
-                           T* __restrict__ rho) {
+__global__ void k_star_step(CicGeom g, double* __restrict__ pos, double* __restrict__ vel,
+                            long long np, const T* __restrict__ A4, double h_owed, double h1,
+                            double dt, const double* __restrict__ mass, double m_scalar,
+                            T* __restrict__ rho) {
   long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   double x[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
   double v[3] = {vel[3 * p], vel[3 * p + 1], vel[3 * p + 2]};
-  if (h1 != 0.0) {
+  if (h_owed != 0.0 || h1 != 0.0) {
     Cic c = cic(g, x[0], x[1], x[2]);
     double a[3];
-    interp_one(g, c, ax, ay, az, a);
+    interp_one(g, c, A4, a);
+    if (h_owed != 0.0) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) v[d] = v[d] + a[d] * h_owed;  // quantum-stars.py:321
+    }
 #pragma unroll
     for (int d = 0; d < 3; ++d) v[d] = v[d] + a[d] * h1;  // quantum-stars.py:304
   }
@@ -192,20 +220,20 @@ __global__ void k_star_kdd(CicGeom g, double* __restrict__ pos, double* __restri
   deposit_one(g, c, (mass ? mass[p] : m_scalar) / g.vol, rho);
 }
 
-// Second half kick and dt_s = min |dx / vel| (quantum-stars.py:321,328).
+// Closing half kick and dt_s = min |dx / vel| (quantum-stars.py:321,328).
 template <class T>
 __global__ void k_star_kick(CicGeom g, const double* __restrict__ pos, double* __restrict__ vel,
-                            long long np, const T* __restrict__ ax, const T* __restrict__ ay,
-                            const T* __restrict__ az, double h2, double* __restrict__ diag) {
+                            long long np, const T* __restrict__ A4, double h,
+                            double* __restrict__ diag) {
   long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   double dts = __builtin_inf();
   if (p < np) {
     Cic c = cic(g, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]);
     double a[3];
-    interp_one(g, c, ax, ay, az, a);
+    interp_one(g, c, A4, a);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      double v = vel[3 * p + d] + a[d] * h2;
+      double v = vel[3 * p + d] + a[d] * h;
       vel[3 * p + d] = v;
       double r = fabs(g.dx / v);
       dts = (r < dts || r != r) ? r : dts;
@@ -240,50 +268,43 @@ hipError_t launch_deposit(const Dims& d, const double* pos, const double* mass, 
 }
 
 template <class T>
-hipError_t launch_interp(const Dims& d, const double* pos, long long np, const void* ax,
-                         const void* ay, const void* az, double* acc, hipStream_t s) {
+hipError_t launch_interp(const Dims& d, const double* pos, long long np, const void* A4,
+                         double* acc, hipStream_t s) {
   if (np <= 0) return hipSuccess;
-  k_interp<T><<<pblocks(np), 256, 0, s>>>(make_geom(d), pos, np, (const T*)ax, (const T*)ay,
-                                          (const T*)az, acc);
+  k_interp<T><<<pblocks(np), 256, 0, s>>>(make_geom(d), pos, np, (const T*)A4, acc);
   return hipGetLastError();
 }
 
 template <class T>
-hipError_t launch_star_kick_drift_deposit(const Dims& d, double* pos, double* vel, long long np,
-                                          const void* ax, const void* ay, const void* az,
-                                          double h1, double dt, const double* mass,
-                                          double m_scalar, void* rho, hipStream_t s) {
+hipError_t launch_star_step(const Dims& d, double* pos, double* vel, long long np, const void* A4,
+                            double h_owed, double h1, double dt, const double* mass,
+                            double m_scalar, void* rho, hipStream_t s) {
   if (np <= 0) return hipSuccess;
-  k_star_kdd<T><<<pblocks(np), 256, 0, s>>>(make_geom(d), pos, vel, np, (const T*)ax,
-                                            (const T*)ay, (const T*)az, h1, dt, mass, m_scalar,
-                                            (T*)rho);
+  k_star_step<T><<<pblocks(np), 256, 0, s>>>(make_geom(d), pos, vel, np, (const T*)A4, h_owed,
+                                             h1, dt, mass, m_scalar, (T*)rho);
   return hipGetLastError();
 }
 
 template <class T>
 hipError_t launch_star_kick(const Dims& d, const double* pos, double* vel, long long np,
-                            const void* ax, const void* ay, const void* az, double h2,
-                            double* diag, hipStream_t s) {
+                            const void* A4, double h, double* diag, hipStream_t s) {
   if (np <= 0) return hipSuccess;
-  k_star_kick<T><<<pblocks(np), 256, 0, s>>>(make_geom(d), pos, vel, np, (const T*)ax,
-                                             (const T*)ay, (const T*)az, h2, diag);
+  k_star_kick<T><<<pblocks(np), 256, 0, s>>>(make_geom(d), pos, vel, np, (const T*)A4, h, diag);
   return hipGetLastError();
 }
 
-#define QJ_PINST(T)                                                                               \
+#define QJ_PINST(T)                                                                              \
   template hipError_t launch_cic_indices<T>(const Dims&, const double*, long long, int*, int*,     \
                                             double*, double*, hipStream_t);                        \
   template hipError_t launch_deposit<T>(const Dims&, const double*, const double*, double,        \
                                         long long, void*, hipStream_t);                            \
   template hipError_t launch_interp<T>(const Dims&, const double*, long long, const void*,         \
-                                       const void*, const void*, double*, hipStream_t);            \
-  template hipError_t launch_star_kick_drift_deposit<T>(const Dims&, double*, double*, long long, \
-                                                        const void*, const void*, const void*,     \
-                                                        double, double, const double*, double,     \
-                                                        void*, hipStream_t);                       \
+                                       double*, hipStream_t);                                      \
+  template hipError_t launch_star_step<T>(const Dims&, double*, double*, long long, const void*, \
+                                          double, double, double, const double*, double, void*,   \
+                                          hipStream_t);                                            \
   template hipError_t launch_star_kick<T>(const Dims&, const double*, double*, long long,          \
-                                          const void*, const void*, const void*, double, double*, \
-                                          hipStream_t);
+                                          const void*, double, double*, hipStream_t);
 
 QJ_PINST(float)
 QJ_PINST(double)

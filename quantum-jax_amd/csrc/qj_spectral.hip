@@ -26,9 +26,6 @@ __host__ __device__ inline int ilog2(int v) {
   return r;
 }
 
-template <class T>
-__device__ __forceinline__ cpx<T> ld(const cpx<T>* p, int i) { return p[i]; }
-
 // ---------------------------------------------------------------------------
 // tables
 // ---------------------------------------------------------------------------
@@ -72,17 +69,17 @@ __global__ __launch_bounds__(256) void k_kick(cpx<T>* __restrict__ psi, const T*
 }
 
 // ---------------------------------------------------------------------------
-// drift sweep along one axis, optionally preceded by the potential kick
+// drift sweep along the contiguous axis, optionally preceded by the kick
 // ---------------------------------------------------------------------------
-template <class T, int N, bool CONTIG, bool KICK>
-__global__ __launch_bounds__(256) void k_drift(cpx<T>* __restrict__ psi, PassGeom g,
-                                               const cpx<T>* __restrict__ D,
-                                               const cpx<T>* __restrict__ tw,
-                                               const T* __restrict__ V,
-                                               const T* __restrict__ Vb, double kick_coef) {
+template <class T, int N, bool KICK>
+__global__ __launch_bounds__(256) void k_drift_c(cpx<T>* __restrict__ psi, PassGeom g,
+                                                 const cpx<T>* __restrict__ D,
+                                                 const cpx<T>* __restrict__ tw,
+                                                 const T* __restrict__ V,
+                                                 const T* __restrict__ Vb, double kick_coef) {
   using S = PencilShape<N>;
   __shared__ cpx<T> lds[S::kLds];
-  Lane<N, CONTIG> ln(threadIdx.x);
+  Lane<N, true> ln(threadIdx.x);
   const int q = blockIdx.x * S::C + ln.col;
   const bool act = q < g.ncols;
   const int base = act ? (int)pencil_base(g, q) : 0;
@@ -109,6 +106,72 @@ __global__ __launch_bounds__(256) void k_drift(cpx<T>* __restrict__ psi, PassGeo
 #pragma unroll
     for (int m = 0; m < S::E; ++m) psi[base + (ln.j + S::TT * m) * se] = v[m];
   }
+}
+
+// Common set-up of a strided sweep: VW neighbouring pencils per lane,
+// pencil slot fastest across lanes.
+#define QJ_STRIDED_SETUP(T, N)                                           \
+  using S = PencilShape<N>;                                              \
+  constexpr int VW = StridedVW<T, N>::value;                             \
+  constexpr int E = S::E;                                                \
+  constexpr int TT = S::TT;                                              \
+  __shared__ cpx<T> lds[S::kLdsStrided * VW];                            \
+  const int col = threadIdx.x % S::C, j = threadIdx.x / S::C;            \
+  const int q0 = (blockIdx.x * S::C + col) * VW;                         \
+  const bool act = q0 < g.ncols;                                         \
+  const int base = act ? (int)pencil_base(g, q0) : 0;                    \
+  const int se = (int)g.s_elem;                                          \
+  (void)E;
+
+template <int VW, int E, class T>
+__device__ __forceinline__ void load_rows(cpx<T> (&v)[VW][E], const cpx<T>* __restrict__ p,
+                                          int base, int se, int j, int TT, bool act) {
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    cpx<T> t[VW];
+    if (act) {
+      ldc<VW>(p, base + (j + TT * m) * se, t);
+    } else {
+#pragma unroll
+      for (int w = 0; w < VW; ++w) t[w] = {0, 0};
+    }
+#pragma unroll
+    for (int w = 0; w < VW; ++w) v[w][m] = t[w];
+  }
+}
+
+template <int VW, int E, class T>
+__device__ __forceinline__ void store_rows(const cpx<T> (&v)[VW][E], cpx<T>* __restrict__ p,
+                                           int base, int se, int j, int TT, bool act) {
+  if (!act) return;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    cpx<T> t[VW];
+#pragma unroll
+    for (int w = 0; w < VW; ++w) t[w] = v[w][m];
+    stc<VW>(p, base + (j + TT * m) * se, t);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// drift sweep along a strided axis (x or y)
+// ---------------------------------------------------------------------------
+template <class T, int N>
+__global__ __launch_bounds__(256) void k_drift_s(cpx<T>* __restrict__ psi, PassGeom g,
+                                                 const cpx<T>* __restrict__ D,
+                                                 const cpx<T>* __restrict__ tw) {
+  QJ_STRIDED_SETUP(T, N)
+  cpx<T> v[VW][E];
+  load_rows<VW, E>(v, psi, base, se, j, TT, act);
+  pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const cpx<T> d = D[j + TT * m];
+#pragma unroll
+    for (int w = 0; w < VW; ++w) v[w][m] = cmul(v[w][m], d);
+  }
+  pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
+  store_rows<VW, E>(v, psi, base, se, j, TT, act);
 }
 
 // ---------------------------------------------------------------------------
@@ -178,83 +241,91 @@ __global__ __launch_bounds__(256) void k_source_x(
     double b_coeff, double rho_bar, PassGeom g, const cpx<T>* __restrict__ D,
     const cpx<T>* __restrict__ tw, cpx<T>* __restrict__ H, double* __restrict__ partials,
     unsigned int* __restrict__ counter, double* __restrict__ diag) {
-  using S = PencilShape<N>;
-  __shared__ cpx<T> lds[S::kLds];
-  Lane<N, false> ln(threadIdx.x);
-  const int q = blockIdx.x * S::C + ln.col;
-  const bool act = q < g.ncols;
-  const int base = act ? (int)pencil_base(g, q) : 0;
-  const int se = (int)g.s_elem;
-  cpx<T>* my = lds + ln.col * S::E * S::P;
-  cpx<T> v[S::E];
-  T rho[S::E];
+  QJ_STRIDED_SETUP(T, N)
+  cpx<T> v[VW][E];
+  T rho[VW][E];
 #pragma unroll
-  for (int m = 0; m < S::E; ++m) rho[m] = (T)0;
+  for (int m = 0; m < E; ++m)
+#pragma unroll
+    for (int w = 0; w < VW; ++w) rho[w][m] = (T)0;
   if (psi) {
-#pragma unroll
-    for (int m = 0; m < S::E; ++m)
-      v[m] = act ? psi[base + (ln.j + S::TT * m) * se] : cpx<T>{0, 0};
+    load_rows<VW, E>(v, psi, base, se, j, TT, act);
     if constexpr (DRIFT) {
-      pencil_fft<N, -1, T>(v, ln.j, my, tw);
+      pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
 #pragma unroll
-      for (int m = 0; m < S::E; ++m) v[m] = cmul(v[m], D[ln.j + S::TT * m]);
-      pencil_fft<N, +1, T>(v, ln.j, my, tw);
-      if (act) {
+      for (int m = 0; m < E; ++m) {
+        const cpx<T> d = D[j + TT * m];
 #pragma unroll
-        for (int m = 0; m < S::E; ++m) psi[base + (ln.j + S::TT * m) * se] = v[m];
+        for (int w = 0; w < VW; ++w) v[w][m] = cmul(v[w][m], d);
       }
+      pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
+      store_rows<VW, E>(v, psi, base, se, j, TT, act);
     }
 #pragma unroll
-    for (int m = 0; m < S::E; ++m) rho[m] = v[m].r * v[m].r + v[m].i * v[m].i;
+    for (int m = 0; m < E; ++m)
+#pragma unroll
+      for (int w = 0; w < VW; ++w) rho[w][m] = v[w][m].r * v[w][m].r + v[w][m].i * v[w][m].i;
   }
   double local = 0.0;
   bool bad = false;
 #pragma unroll
-  for (int m = 0; m < S::E; ++m) {
-    const int idx = base + (ln.j + S::TT * m) * se;
+  for (int m = 0; m < E; ++m) {
+    const int idx = base + (j + TT * m) * se;
     if (act) {
       if (src) {
-        rho[m] += src[idx];
-        if (clear_src) src[idx] = (T)0;
+        T t[VW];
+        ldr<VW>(src, idx, t);
+#pragma unroll
+        for (int w = 0; w < VW; ++w) rho[w][m] += t[w];
+        if (clear_src) {
+          T z[VW];
+#pragma unroll
+          for (int w = 0; w < VW; ++w) z[w] = (T)0;
+          str<VW>(src, idx, z);
+        }
       }
-      if (brho) brho[idx] = (T)(b_coeff * (double)rho[m]);
-      local += (double)rho[m];
-      bad |= !isfinite(rho[m]);
+      if (brho) {
+        T t[VW];
+#pragma unroll
+        for (int w = 0; w < VW; ++w) t[w] = (T)(b_coeff * (double)rho[w][m]);
+        str<VW>(brho, idx, t);
+      }
+#pragma unroll
+      for (int w = 0; w < VW; ++w) {
+        local += (double)rho[w][m];
+        bad |= !isfinite(rho[w][m]);
+      }
     }
-    v[m] = {act ? (T)((double)rho[m] - rho_bar) : (T)0, (T)0};
+#pragma unroll
+    for (int w = 0; w < VW; ++w) v[w][m] = {act ? (T)((double)rho[w][m] - rho_bar) : (T)0, (T)0};
   }
-  pencil_fft<N, -1, T>(v, ln.j, my, tw);
+  pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
   if (act) {
 #pragma unroll
-    for (int m = 0; m < S::E; ++m) {
-      const int k = ln.j + S::TT * m;
-      if (k <= N / 2) H[base + k * se] = v[m];
+    for (int m = 0; m < E; ++m) {
+      const int k = j + TT * m;
+      if (k <= N / 2) {
+        cpx<T> t[VW];
+#pragma unroll
+        for (int w = 0; w < VW; ++w) t[w] = v[w][m];
+        stc<VW>(H, base + k * se, t);
+      }
     }
   }
   block_sum_finish(local, bad, partials, counter, diag);
 }
 
 // ---------------------------------------------------------------------------
-// generic in-place complex FFT sweep (forward)
+// generic in-place complex FFT sweep along a strided axis
 // ---------------------------------------------------------------------------
-template <class T, int N, bool CONTIG, int DIR>
-__global__ __launch_bounds__(256) void k_fft(cpx<T>* __restrict__ a, PassGeom g,
-                                             const cpx<T>* __restrict__ tw) {
-  using S = PencilShape<N>;
-  __shared__ cpx<T> lds[S::kLds];
-  Lane<N, CONTIG> ln(threadIdx.x);
-  const int q = blockIdx.x * S::C + ln.col;
-  const bool act = q < g.ncols;
-  const int base = act ? (int)pencil_base(g, q) : 0;
-  const int se = (int)g.s_elem;
-  cpx<T> v[S::E];
-#pragma unroll
-  for (int m = 0; m < S::E; ++m) v[m] = act ? a[base + (ln.j + S::TT * m) * se] : cpx<T>{0, 0};
-  pencil_fft<N, DIR, T>(v, ln.j, lds + ln.col * S::E * S::P, tw);
-  if (act) {
-#pragma unroll
-    for (int m = 0; m < S::E; ++m) a[base + (ln.j + S::TT * m) * se] = v[m];
-  }
+template <class T, int N, int DIR>
+__global__ __launch_bounds__(256) void k_fft_s(cpx<T>* __restrict__ a, PassGeom g,
+                                               const cpx<T>* __restrict__ tw) {
+  QJ_STRIDED_SETUP(T, N)
+  cpx<T> v[VW][E];
+  load_rows<VW, E>(v, a, base, se, j, TT, act);
+  pencil_fft_strided<N, DIR, T, VW>(v, j, col, lds, tw);
+  store_rows<VW, E>(v, a, base, se, j, TT, act);
 }
 
 // ---------------------------------------------------------------------------
@@ -324,58 +395,58 @@ template <class T, int N, bool GRAD>
 __global__ __launch_bounds__(256) void k_poisson_y(cpx<T>* __restrict__ H, cpx<T>* __restrict__ Uz,
                                                    cpx<T>* __restrict__ Uy, PassGeom g,
                                                    const cpx<T>* __restrict__ tw, double kfy) {
-  using S = PencilShape<N>;
-  __shared__ cpx<T> lds[S::kLds];
-  Lane<N, false> ln(threadIdx.x);
-  const int q = blockIdx.x * S::C + ln.col;
-  const bool act = q < g.ncols;
-  const int base = act ? (int)pencil_base(g, q) : 0;
-  const int se = (int)g.s_elem;
-  cpx<T>* my = lds + ln.col * S::E * S::P;
-  cpx<T> v[S::E];
-#pragma unroll
-  for (int m = 0; m < S::E; ++m) v[m] = act ? H[base + (ln.j + S::TT * m) * se] : cpx<T>{0, 0};
-  cpx<T> w[GRAD ? S::E : 1];
+  QJ_STRIDED_SETUP(T, N)
+  cpx<T> v[VW][E];
+  load_rows<VW, E>(v, H, base, se, j, TT, act);
   if constexpr (GRAD) {
+    cpx<T> w[VW][E];
 #pragma unroll
-    for (int m = 0; m < S::E; ++m) {
-      const int k = ln.j + S::TT * m;
+    for (int m = 0; m < E; ++m) {
+      const int k = j + TT * m;
       const T ky = (k == N / 2) ? (T)0 : (T)(kfy * (double)freq_index(k, N));
-      w[m] = {-ky * v[m].i, ky * v[m].r};
+#pragma unroll
+      for (int x = 0; x < VW; ++x) w[x][m] = {-ky * v[x][m].i, ky * v[x][m].r};
     }
-  }
-  pencil_fft<N, +1, T>(v, ln.j, my, tw);
-  if (act) {
-#pragma unroll
-    for (int m = 0; m < S::E; ++m) H[base + (ln.j + S::TT * m) * se] = v[m];
-  }
-  if constexpr (GRAD) {
-    pencil_fft<N, +1, T>(w, ln.j, my, tw);
-    if (act) {
-#pragma unroll
-      for (int m = 0; m < S::E; ++m) Uy[base + (ln.j + S::TT * m) * se] = w[m];
-    }
-#pragma unroll
-    for (int m = 0; m < S::E; ++m) v[m] = act ? Uz[base + (ln.j + S::TT * m) * se] : cpx<T>{0, 0};
-    pencil_fft<N, +1, T>(v, ln.j, my, tw);
-    if (act) {
-#pragma unroll
-      for (int m = 0; m < S::E; ++m) Uz[base + (ln.j + S::TT * m) * se] = v[m];
-    }
+    pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
+    store_rows<VW, E>(v, H, base, se, j, TT, act);
+    pencil_fft_strided<N, +1, T, VW>(w, j, col, lds, tw);
+    store_rows<VW, E>(w, Uy, base, se, j, TT, act);
+    load_rows<VW, E>(v, Uz, base, se, j, TT, act);
+    pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
+    store_rows<VW, E>(v, Uz, base, se, j, TT, act);
+  } else {
+    pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
+    store_rows<VW, E>(v, H, base, se, j, TT, act);
   }
 }
 
-// Hermitian completion along x: bin k of the full spectrum from the half.
-template <int N, class T>
-__device__ __forceinline__ cpx<T> load_herm(const cpx<T>* __restrict__ a, int base, int se, int k) {
-  if (k <= N / 2) return a[base + k * se];
-  cpx<T> c = a[base + (N - k) * se];
-  return {c.r, -c.i};
+// Rows of the full x spectrum from the half spectrum (Hermitian along x).
+template <int N, int VW, int E, class T>
+__device__ __forceinline__ void load_herm_rows(cpx<T> (&v)[VW][E], const cpx<T>* __restrict__ a,
+                                               int base, int se, int j, int TT, bool act) {
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int k = j + TT * m;
+    cpx<T> t[VW];
+    if (act) {
+      ldc<VW>(a, base + (k <= N / 2 ? k : N - k) * se, t);
+      if (k > N / 2) {
+#pragma unroll
+        for (int w = 0; w < VW; ++w) t[w].i = -t[w].i;
+      }
+    } else {
+#pragma unroll
+      for (int w = 0; w < VW; ++w) t[w] = {0, 0};
+    }
+#pragma unroll
+    for (int w = 0; w < VW; ++w) v[w][m] = t[w];
+  }
 }
 
 // ---------------------------------------------------------------------------
-// Poisson x pass (inverse, Hermitian-completed): V = IFFT_x(UU);
-// with GRAD: a_x + i a_y = -IFFT_x(i kx UU + i Uy), a_z = -IFFT_x(Uzz); max|a|.
+// Poisson x pass (inverse, Hermitian-completed): V = IFFT_x(UU); with GRAD
+// a_x + i a_y = -IFFT_x(i kx UU + i Uy), a_z = -IFFT_x(Uzz), written as the
+// cell-interleaved grid A4 = (a_x, a_y, a_z, V), and max |a|.
 // quantum-stars.py:192 and :256-260.
 // ---------------------------------------------------------------------------
 template <class T, int N, bool GRAD>
@@ -383,71 +454,72 @@ __global__ __launch_bounds__(256) void k_poisson_x(const cpx<T>* __restrict__ H,
                                                    const cpx<T>* __restrict__ Uy,
                                                    const cpx<T>* __restrict__ Uzz, PassGeom g,
                                                    const cpx<T>* __restrict__ tw, double kfx,
-                                                   T* __restrict__ V, T* __restrict__ ax,
-                                                   T* __restrict__ ay, T* __restrict__ az,
+                                                   T* __restrict__ V, T* __restrict__ A4,
                                                    const T* __restrict__ addend,
                                                    double* __restrict__ diag) {
-  using S = PencilShape<N>;
-  __shared__ cpx<T> lds[S::kLds];
-  Lane<N, false> ln(threadIdx.x);
-  const int q = blockIdx.x * S::C + ln.col;
-  const bool act = q < g.ncols;
-  const int base = act ? (int)pencil_base(g, q) : 0;
-  const int se = (int)g.s_elem;
-  cpx<T>* my = lds + ln.col * S::E * S::P;
-  cpx<T> v[S::E];
+  QJ_STRIDED_SETUP(T, N)
+  cpx<T> v[VW][E];
+  load_herm_rows<N, VW, E>(v, H, base, se, j, TT, act);
+  if constexpr (!GRAD) {
+    pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
+    if (act) {
 #pragma unroll
-  for (int m = 0; m < S::E; ++m)
-    v[m] = act ? load_herm<N>(H, base, se, ln.j + S::TT * m) : cpx<T>{0, 0};
-  cpx<T> w[GRAD ? S::E : 1];
-  if constexpr (GRAD) {
+      for (int m = 0; m < E; ++m) {
+        const int idx = base + (j + TT * m) * se;
+        T t[VW];
+        if (addend) ldr<VW>(addend, idx, t);
 #pragma unroll
-    for (int m = 0; m < S::E; ++m) {
-      const int k = ln.j + S::TT * m;
+        for (int w = 0; w < VW; ++w) t[w] = addend ? v[w][m].r + t[w] : v[w][m].r;
+        str<VW>(V, idx, t);
+      }
+    }
+  } else {
+    cpx<T> u[VW][E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int k = j + TT * m;
       const T kx = (k == N / 2) ? (T)0 : (T)(kfx * (double)freq_index(k, N));
-      w[m] = {-kx * v[m].i, kx * v[m].r};
-    }
-  }
-  pencil_fft<N, +1, T>(v, ln.j, my, tw);
-  if (act) {
 #pragma unroll
-    for (int m = 0; m < S::E; ++m) {
-      const int idx = base + (ln.j + S::TT * m) * se;
-      V[idx] = addend ? v[m].r + addend[idx] : v[m].r;
+      for (int w = 0; w < VW; ++w) u[w][m] = {-kx * v[w][m].i, kx * v[w][m].r};
     }
-  }
-  if constexpr (GRAD) {
+    pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
+    T vr[VW][E];
 #pragma unroll
-    for (int m = 0; m < S::E; ++m) {
-      cpx<T> u = act ? load_herm<N>(Uy, base, se, ln.j + S::TT * m) : cpx<T>{0, 0};
-      w[m] = {w[m].r - u.i, w[m].i + u.r};  // + i*Uy
+    for (int m = 0; m < E; ++m) {
+      T t[VW];
+#pragma unroll
+      for (int w = 0; w < VW; ++w) t[w] = vr[w][m] = v[w][m].r;
+      if (act) str<VW>(V, base + (j + TT * m) * se, t);
     }
-    pencil_fft<N, +1, T>(w, ln.j, my, tw);
+    load_herm_rows<N, VW, E>(v, Uy, base, se, j, TT, act);
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+#pragma unroll
+      for (int w = 0; w < VW; ++w) u[w][m] = {u[w][m].r - v[w][m].i, u[w][m].i + v[w][m].r};
+    pencil_fft_strided<N, +1, T, VW>(u, j, col, lds, tw);
+    load_herm_rows<N, VW, E>(v, Uzz, base, se, j, TT, act);
+    pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
     T amax = (T)0;
     if (act) {
 #pragma unroll
-      for (int m = 0; m < S::E; ++m) {
-        const int idx = base + (ln.j + S::TT * m) * se;
-        const T a0 = -w[m].r, a1 = -w[m].i;
-        ax[idx] = a0;
-        ay[idx] = a1;
-        const T m0 = fabs(a0), m1 = fabs(a1);
-        amax = (m0 > amax || m0 != m0) ? m0 : amax;
-        amax = (m1 > amax || m1 != m1) ? m1 : amax;
-      }
-    }
+      for (int m = 0; m < E; ++m) {
+        const int idx = base + (j + TT * m) * se;
 #pragma unroll
-    for (int m = 0; m < S::E; ++m)
-      v[m] = act ? load_herm<N>(Uzz, base, se, ln.j + S::TT * m) : cpx<T>{0, 0};
-    pencil_fft<N, +1, T>(v, ln.j, my, tw);
-    if (act) {
-#pragma unroll
-      for (int m = 0; m < S::E; ++m) {
-        const int idx = base + (ln.j + S::TT * m) * se;
-        const T a2 = -v[m].r;
-        az[idx] = a2;
-        const T m2 = fabs(a2);
-        amax = (m2 > amax || m2 != m2) ? m2 : amax;
+        for (int w = 0; w < VW; ++w) {
+          const T a0 = -u[w][m].r, a1 = -u[w][m].i, a2 = -v[w][m].r;
+          if constexpr (sizeof(T) == 4) {
+            *reinterpret_cast<float4*>(A4 + 4 * (idx + w)) = make_float4(a0, a1, a2, vr[w][m]);
+          } else {
+            A4[4 * (idx + w) + 0] = a0;
+            A4[4 * (idx + w) + 1] = a1;
+            A4[4 * (idx + w) + 2] = a2;
+            A4[4 * (idx + w) + 3] = vr[w][m];
+          }
+          const T m0 = fabs(a0), m1 = fabs(a1), m2 = fabs(a2);
+          amax = (m0 > amax || m0 != m0) ? m0 : amax;
+          amax = (m1 > amax || m1 != m1) ? m1 : amax;
+          amax = (m2 > amax || m2 != m2) ? m2 : amax;
+        }
       }
     }
     // wave max (NaN-propagating), one atomic per wave
@@ -467,6 +539,11 @@ __global__ __launch_bounds__(256) void k_poisson_x(const cpx<T>* __restrict__ H,
 template <int N>
 inline int nblocks(int ncols) {
   return (ncols + PencilShape<N>::C - 1) / PencilShape<N>::C;
+}
+template <class T, int N>
+inline int nblocks_s(int ncols) {
+  const int per = PencilShape<N>::C * StridedVW<T, N>::value;
+  return (ncols + per - 1) / per;
 }
 
 #define QJ_DISPATCH_N(n, ...)                      \
@@ -528,48 +605,42 @@ hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, v
     const cpx<T>* tw = (const cpx<T>*)tb.tw_z;
     QJ_DISPATCH_N(d.nz, {
       if (kick)
-        k_drift<T, N_, true, true><<<nblocks<N_>(g.ncols), 256, 0, s>>>(p, g, D, tw, v, vb, kick_coef);
+        k_drift_c<T, N_, true><<<nblocks<N_>(g.ncols), 256, 0, s>>>(p, g, D, tw, v, vb, kick_coef);
       else
-        k_drift<T, N_, true, false><<<nblocks<N_>(g.ncols), 256, 0, s>>>(p, g, D, tw, v, vb, kick_coef);
-    });
-  } else if (axis == 1) {
-    PassGeom g = geom_y(d, d.nx);
-    const cpx<T>* D = (const cpx<T>*)tb.drift_y;
-    const cpx<T>* tw = (const cpx<T>*)tb.tw_y;
-    QJ_DISPATCH_N(d.ny, {
-      if (kick)
-        k_drift<T, N_, false, true><<<nblocks<N_>(g.ncols), 256, 0, s>>>(p, g, D, tw, v, vb, kick_coef);
-      else
-        k_drift<T, N_, false, false><<<nblocks<N_>(g.ncols), 256, 0, s>>>(p, g, D, tw, v, vb, kick_coef);
+        k_drift_c<T, N_, false><<<nblocks<N_>(g.ncols), 256, 0, s>>>(p, g, D, tw, v, vb, kick_coef);
     });
   } else {
-    PassGeom g = geom_x(d);
-    const cpx<T>* D = (const cpx<T>*)tb.drift_x;
-    const cpx<T>* tw = (const cpx<T>*)tb.tw_x;
-    QJ_DISPATCH_N(d.nx, {
-      if (kick)
-        k_drift<T, N_, false, true><<<nblocks<N_>(g.ncols), 256, 0, s>>>(p, g, D, tw, v, vb, kick_coef);
-      else
-        k_drift<T, N_, false, false><<<nblocks<N_>(g.ncols), 256, 0, s>>>(p, g, D, tw, v, vb, kick_coef);
+    if (kick) return hipErrorInvalidValue;  // the kick is fused into the z sweep only
+    const bool y = axis == 1;
+    PassGeom g = y ? geom_y(d, d.nx) : geom_x(d);
+    const cpx<T>* D = (const cpx<T>*)(y ? tb.drift_y : tb.drift_x);
+    const cpx<T>* tw = (const cpx<T>*)(y ? tb.tw_y : tb.tw_x);
+    QJ_DISPATCH_N(y ? d.ny : d.nx, {
+      k_drift_s<T, N_><<<nblocks_s<T, N_>(g.ncols), 256, 0, s>>>(p, g, D, tw);
     });
   }
   return hipGetLastError();
 }
 
-int source_x_blocks(const Dims& d) {
+template <class T>
+int source_x_blocks_t(const Dims& d) {
   const int ncols = d.ny * d.nz;
-  int C = 256;
   switch (d.nx) {
-    case 8: C = PencilShape<8>::C; break;
-    case 16: C = PencilShape<16>::C; break;
-    case 32: C = PencilShape<32>::C; break;
-    case 64: C = PencilShape<64>::C; break;
-    case 128: C = PencilShape<128>::C; break;
-    case 256: C = PencilShape<256>::C; break;
-    case 512: C = PencilShape<512>::C; break;
-    default: break;
+    case 8: return nblocks_s<T, 8>(ncols);
+    case 16: return nblocks_s<T, 16>(ncols);
+    case 32: return nblocks_s<T, 32>(ncols);
+    case 64: return nblocks_s<T, 64>(ncols);
+    case 128: return nblocks_s<T, 128>(ncols);
+    case 256: return nblocks_s<T, 256>(ncols);
+    case 512: return nblocks_s<T, 512>(ncols);
+    default: return 1;
   }
-  return (ncols + C - 1) / C;
+}
+
+int source_x_blocks(const Dims& d) {
+  // partials are sized for the larger of the two precisions' grids
+  const int a = source_x_blocks_t<float>(d), b = source_x_blocks_t<double>(d);
+  return a > b ? a : b;
 }
 
 template <class T>
@@ -582,11 +653,11 @@ hipError_t launch_source_x(const Dims& d, const SpecTables<T>& tb, int drift, vo
   const cpx<T>* tw = (const cpx<T>*)tb.tw_x;
   QJ_DISPATCH_N(d.nx, {
     if (drift)
-      k_source_x<T, N_, true><<<nblocks<N_>(g.ncols), 256, 0, s>>>(
+      k_source_x<T, N_, true><<<nblocks_s<T, N_>(g.ncols), 256, 0, s>>>(
           (cpx<T>*)psi, (T*)src, clear_src, (T*)brho, b_coeff, rho_bar, g, D, tw, (cpx<T>*)H,
           partials, counter, diag);
     else
-      k_source_x<T, N_, false><<<nblocks<N_>(g.ncols), 256, 0, s>>>(
+      k_source_x<T, N_, false><<<nblocks_s<T, N_>(g.ncols), 256, 0, s>>>(
           (cpx<T>*)psi, (T*)src, clear_src, (T*)brho, b_coeff, rho_bar, g, D, tw, (cpx<T>*)H,
           partials, counter, diag);
   });
@@ -598,7 +669,7 @@ hipError_t launch_fft_y_half(const Dims& d, const SpecTables<T>& tb, void* H, hi
   PassGeom g = geom_y(d, d.nx / 2 + 1);
   const cpx<T>* tw = (const cpx<T>*)tb.tw_y;
   QJ_DISPATCH_N(d.ny, {
-    k_fft<T, N_, false, -1><<<nblocks<N_>(g.ncols), 256, 0, s>>>((cpx<T>*)H, g, tw);
+    k_fft_s<T, N_, -1><<<nblocks_s<T, N_>(g.ncols), 256, 0, s>>>((cpx<T>*)H, g, tw);
   });
   return hipGetLastError();
 }
@@ -633,10 +704,10 @@ hipError_t launch_poisson_y(const Dims& d, const SpecTables<T>& tb, int grad, vo
   const double kfy = 2.0 * M_PI / d.Ly;
   QJ_DISPATCH_N(d.ny, {
     if (grad)
-      k_poisson_y<T, N_, true><<<nblocks<N_>(g.ncols), 256, 0, s>>>(
+      k_poisson_y<T, N_, true><<<nblocks_s<T, N_>(g.ncols), 256, 0, s>>>(
           (cpx<T>*)H, (cpx<T>*)Uz, (cpx<T>*)Uy, g, tw, kfy);
     else
-      k_poisson_y<T, N_, false><<<nblocks<N_>(g.ncols), 256, 0, s>>>(
+      k_poisson_y<T, N_, false><<<nblocks_s<T, N_>(g.ncols), 256, 0, s>>>(
           (cpx<T>*)H, (cpx<T>*)Uz, (cpx<T>*)Uy, g, tw, kfy);
   });
   return hipGetLastError();
@@ -644,20 +715,20 @@ hipError_t launch_poisson_y(const Dims& d, const SpecTables<T>& tb, int grad, vo
 
 template <class T>
 hipError_t launch_poisson_x(const Dims& d, const SpecTables<T>& tb, int grad, const void* H,
-                            const void* Uy, const void* Uzz, void* V, void* ax, void* ay,
-                            void* az, const void* addend, double* diag, hipStream_t s) {
+                            const void* Uy, const void* Uzz, void* V, void* A4,
+                            const void* addend, double* diag, hipStream_t s) {
   PassGeom g = geom_x(d);
   const cpx<T>* tw = (const cpx<T>*)tb.tw_x;
   const double kfx = 2.0 * M_PI / d.Lx;
   QJ_DISPATCH_N(d.nx, {
     if (grad)
-      k_poisson_x<T, N_, true><<<nblocks<N_>(g.ncols), 256, 0, s>>>(
-          (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)ax,
-          (T*)ay, (T*)az, (const T*)addend, diag);
+      k_poisson_x<T, N_, true><<<nblocks_s<T, N_>(g.ncols), 256, 0, s>>>(
+          (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A4,
+          (const T*)addend, diag);
     else
-      k_poisson_x<T, N_, false><<<nblocks<N_>(g.ncols), 256, 0, s>>>(
-          (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)ax,
-          (T*)ay, (T*)az, (const T*)addend, diag);
+      k_poisson_x<T, N_, false><<<nblocks_s<T, N_>(g.ncols), 256, 0, s>>>(
+          (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A4,
+          (const T*)addend, diag);
   });
   return hipGetLastError();
 }
@@ -679,8 +750,8 @@ hipError_t launch_poisson_x(const Dims& d, const SpecTables<T>& tb, int grad, co
   template hipError_t launch_poisson_y<T>(const Dims&, const SpecTables<T>&, int, void*, void*, \
                                           void*, hipStream_t);                                  \
   template hipError_t launch_poisson_x<T>(const Dims&, const SpecTables<T>&, int, const void*,  \
-                                          const void*, const void*, void*, void*, void*, void*, \
-                                          const void*, double*, hipStream_t);
+                                          const void*, const void*, void*, void*, const void*,  \
+                                          double*, hipStream_t);
 
 QJ_INST(float)
 QJ_INST(double)

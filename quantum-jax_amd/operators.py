@@ -34,17 +34,25 @@ def get_potential(rho: torch.Tensor, plan: Plan, phys: Physics) -> torch.Tensor:
     return V
 
 
+def _new_a4(plan: Plan) -> torch.Tensor:
+    return torch.empty(plan.shape + (4,), dtype=plan.real_dtype, device=plan.device)
+
+
 def acceleration_grid(rho: torch.Tensor, plan: Plan, phys: Physics, diag=None):
-    """(V, a) with a[d] = -Re IFFT(1j k_d V_hat); quantum-stars.py:254-259."""
+    """(V, a4): a4[..., d] = -Re IFFT(1j k_d V_hat) for d < 3 and a4[..., 3] = V.
+
+    quantum-stars.py:254-259 (a4[..., :3] is the reference's a_grid, stacked
+    on the last axis).  max |a| is folded into diag[0] when diag is given.
+    """
     plan.field(rho, "rho")
     V = torch.empty_like(rho)
-    a = torch.empty((3,) + plan.shape, dtype=plan.real_dtype, device=plan.device)
+    a4 = _new_a4(plan)
     with torch.cuda.device(plan.device):
         _lib.check(_L().qj_acceleration_grid(plan.handle, ptr(rho), phys.rho_bar,
-                                             phys.four_pi_G, phys.dc_mode, ptr(V), ptr(a),
+                                             phys.four_pi_G, phys.dc_mode, ptr(V), ptr(a4),
                                              ptr(diag), ptr(plan.workspace()), plan.stream()),
                    "acceleration_grid")
-    return V, a
+    return V, a4
 
 
 def kick(psi: torch.Tensor, V: torch.Tensor, coef: float, plan: Plan) -> torch.Tensor:
@@ -114,13 +122,16 @@ def bin_stars(pos: torch.Tensor, m_s: float, plan: Plan) -> torch.Tensor:
     return bin_particles(pos, float(m_s), plan)
 
 
-def interp(pos: torch.Tensor, a: torch.Tensor, plan: Plan) -> torch.Tensor:
-    """8-corner CIC gather of the 3 planes of a; quantum-stars.py:262-287."""
+def interp(pos: torch.Tensor, a4: torch.Tensor, plan: Plan) -> torch.Tensor:
+    """8-corner CIC gather of a4[..., :3] (a4 is (nx,ny,nz,4)); quantum-stars.py:262-287."""
     plan.particles(pos, "pos")
-    plan.field(a, "a", planes=3)
+    if (a4.shape != plan.shape + (4,) or a4.dtype != plan.real_dtype
+            or a4.device != plan.device or not a4.is_contiguous()):
+        raise ValueError(f"a4 must be a contiguous {plan.real_dtype} tensor of shape "
+                         f"{plan.shape + (4,)} on {plan.device}")
     acc = torch.empty((pos.shape[0], 3), dtype=torch.float64, device=plan.device)
     with torch.cuda.device(plan.device):
-        _lib.check(_L().qj_interp(plan.handle, ptr(pos), pos.shape[0], ptr(a), ptr(acc),
+        _lib.check(_L().qj_interp(plan.handle, ptr(pos), pos.shape[0], ptr(a4), ptr(acc),
                                   plan.stream()), "interp")
     return acc
 
@@ -130,7 +141,7 @@ def get_acceleration(pos: torch.Tensor, rho: torch.Tensor, plan: Plan, phys: Phy
     plan.particles(pos, "pos")
     plan.field(rho, "rho")
     diag = plan.new_diag()
-    a = torch.empty((3,) + plan.shape, dtype=plan.real_dtype, device=plan.device)
+    a = _new_a4(plan)
     acc = torch.empty((pos.shape[0], 3), dtype=torch.float64, device=plan.device)
     with torch.cuda.device(plan.device):
         _lib.check(_L().qj_get_acceleration(plan.handle, ptr(pos), pos.shape[0], ptr(rho),
@@ -145,7 +156,7 @@ def apply_grav_accel(rho, vx, vy, vz, dt: float, plan: Plan, phys: Physics):
     """quantum-fluid.py:267-278 -> (vx, vy, vz) (new tensors)."""
     plan.field(rho, "rho")
     out = [plan.field(v, n).clone() for v, n in ((vx, "vx"), (vy, "vy"), (vz, "vz"))]
-    a = torch.empty((3,) + plan.shape, dtype=plan.real_dtype, device=plan.device)
+    a = _new_a4(plan)
     with torch.cuda.device(plan.device):
         _lib.check(_L().qj_apply_grav_accel(plan.handle, ptr(rho), ptr(out[0]), ptr(out[1]),
                                             ptr(out[2]), float(dt), phys.rho_bar,

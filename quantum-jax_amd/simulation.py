@@ -55,7 +55,8 @@ class KDKSimulation:
         if gas is not None:
             self.gas = [plan.field(g, n).clone() for g, n in zip(gas, ("rho", "vx", "vy", "vz"))]
             self._gas_tmp = [torch.empty_like(g) for g in self.gas]
-        self.grid = torch.empty((4,) + plan.shape, dtype=rdt, device=dev)
+        # V (nx,ny,nz) followed by A4 (nx,ny,nz,4) = (a_x, a_y, a_z, V) per cell
+        self.grid = torch.empty(5 * plan.ncell, dtype=rdt, device=dev)
         self.diag = plan.new_diag()
         self.brho = torch.empty(plan.shape, dtype=rdt, device=dev) if phys.b_coeff else None
         st = _lib.QjState()
@@ -76,6 +77,7 @@ class KDKSimulation:
         st.grid_valid = 0
         st.gas_swapped = 0
         st.pending = 0.0
+        st.pending_star = 0.0
         self._st = st
         self._params = _lib.QjParams(self.dt, phys.m_per_hbar, phys.four_pi_G, phys.rho_bar,
                                      phys.b_coeff, phys.cs, phys.dc_mode)
@@ -120,11 +122,17 @@ class KDKSimulation:
     @property
     def V(self) -> torch.Tensor:
         """Potential of the current state (computed on the first step)."""
-        return self.grid[0]
+        return self.grid[: self.plan.ncell].view(self.plan.shape)
+
+    @property
+    def a4(self) -> torch.Tensor:
+        """(nx,ny,nz,4) = (a_x, a_y, a_z, V) of the current state (stars/gas)."""
+        return self.grid[self.plan.ncell:].view(self.plan.shape + (4,))
 
     @property
     def a_grid(self) -> torch.Tensor:
-        return self.grid[1:]
+        """The reference's a_grid, (nx,ny,nz,3) stacked on the last axis."""
+        return self.a4[..., :3]
 
     def a_max(self) -> float:
         return float(self.diag[_lib.DIAG_AMAX].item())
