@@ -158,6 +158,21 @@ def main():
     value = N * args.steps / (ms_total * 1e-3)
     log(f"{ms_step:.4f} ms/step, {value:.4e} cell-updates/s (host wall {wall:.3f}s)")
 
+    # ---- the other x schedule (fused x drift + Poisson source vs split) ---
+    fuse0 = plan.get_option(lib_mod.OPT_FUSE_X_SOURCE)
+    plan.set_option(lib_mod.OPT_FUSE_X_SOURCE, 1 - fuse0)
+    sim.step(3)
+    torch.cuda.synchronize()
+    ev0.record()
+    sim.step(args.steps)
+    sim.finalize()
+    ev1.record()
+    torch.cuda.synchronize()
+    ms_alt = ev0.elapsed_time(ev1) / args.steps
+    log(f"x schedule fuse={1 - fuse0}: {ms_alt:.4f} ms/step (fuse={fuse0}: {ms_step:.4f})")
+    plan.set_option(lib_mod.OPT_FUSE_X_SOURCE, fuse0)
+    schedules = {f"fuse_x_source={fuse0}": ms_step, f"fuse_x_source={1 - fuse0}": ms_alt}
+
     # ---- per-stage timing pass (HIP events around every launch) ---------
     L = lib_mod.lib()
     nst = L.qj_stage_count()
@@ -210,6 +225,7 @@ def main():
         "step_gbs_effective": round(step_bytes / (ms_step * 1e-3) / 1e9, 1),
         "schedule_s_fraction": round(227.4 * value / 8.0e12, 4) if n == 256 else None,
         "stages": stages,
+        "x_schedules_ms_per_step": schedules,
     }
     del sim
     torch.cuda.empty_cache()

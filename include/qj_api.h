@@ -160,6 +160,16 @@ int qj_kdk_steps(qj_plan* plan, qj_state* state, const qj_params* prm, int nstep
                  void* stream);
 int qj_kdk_finalize(qj_plan* plan, qj_state* state, const qj_params* prm, void* stream);
 
+/* ---- schedule options (no counterpart in the reference, whose XLA
+ * compiler picks the fusion) -------------------------------------------
+ * QJ_OPT_FUSE_X_SOURCE: 1 = the x drift sweep and the Poisson source pass
+ * run as one kernel (fewer HBM bytes, more work per workgroup); 0 = two
+ * kernels.  Results are identical; the autotuner picks per grid. */
+#define QJ_OPT_FUSE_X_SOURCE 0
+#define QJ_OPT_COUNT 1
+int qj_plan_set_option(qj_plan* plan, int option, int value);
+int qj_plan_get_option(const qj_plan* plan, int option, int* value);
+
 /* ---- stage timing (tracing hook; no counterpart in the reference, which
  * only wall-clocks its chunk loop at quantum-stars.py:451,467) ----------
  * When enabled, every kernel launch of the plan is bracketed by a pair of

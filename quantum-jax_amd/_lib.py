@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "libqjamd.so")
 QJ_C64 = 0
 QJ_C128 = 1
 DIAG_AMAX, DIAG_DTS, DIAG_MASS, DIAG_NAN, DIAG_COUNT = 0, 1, 2, 3, 8
+OPT_FUSE_X_SOURCE = 0
 
 vp = C.c_void_p
 dp = C.POINTER(C.c_double)
@@ -61,6 +62,8 @@ SIGNATURES = {
     "qj_solve_hydro": (i32, [vp, vp, vp, vp, vp, f64, f64, vp, vp, vp, vp, vp]),
     "qj_kdk_steps": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), i32, vp, vp]),
     "qj_kdk_finalize": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), vp]),
+    "qj_plan_set_option": (i32, [vp, i32, i32]),
+    "qj_plan_get_option": (i32, [vp, i32, C.POINTER(C.c_int)]),
     "qj_stage_count": (i32, []),
     "qj_stage_name": (C.c_char_p, [i32]),
     "qj_timing_enable": (i32, [vp, i32]),

@@ -112,6 +112,17 @@ class Plan:
             self._work = torch.empty(n, dtype=torch.uint8, device=self.device)
         return self._work
 
+    def set_option(self, option: int, value: int):
+        """Schedule option (e.g. _lib.OPT_FUSE_X_SOURCE); results are unchanged."""
+        _lib.check(_lib.lib().qj_plan_set_option(self._handle, int(option), int(value)),
+                   "qj_plan_set_option")
+
+    def get_option(self, option: int) -> int:
+        v = C.c_int()
+        _lib.check(_lib.lib().qj_plan_get_option(self._handle, int(option), C.byref(v)),
+                   "qj_plan_get_option")
+        return v.value
+
     def stream(self):
         return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 

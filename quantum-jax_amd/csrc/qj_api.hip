@@ -76,6 +76,7 @@ struct qj_plan {
   double* partials = nullptr;
   unsigned int* counter = nullptr;
   int nblk_src = 0;
+  int opt[QJ_OPT_COUNT] = {1};  // QJ_OPT_FUSE_X_SOURCE on  // schedule options (qj_plan_set_option)
   // optional per-stage event timing (tracing hook of the bench harness)
   int timing = 0;
   struct Rec {
@@ -296,6 +297,20 @@ int qj_plan_destroy(qj_plan* p) {
   if (p->partials) (void)hipFree(p->partials);
   if (p->counter) (void)hipFree(p->counter);
   delete p;
+  return QJ_OK;
+}
+
+int qj_plan_set_option(qj_plan* p, int option, int value) {
+  if (!p) return fail(QJ_EARG, "null plan");
+  if (option < 0 || option >= QJ_OPT_COUNT) return fail(QJ_EARG, "unknown option");
+  p->opt[option] = value;
+  return QJ_OK;
+}
+
+int qj_plan_get_option(const qj_plan* p, int option, int* value) {
+  if (!p || !value) return fail(QJ_EARG, "null argument");
+  if (option < 0 || option >= QJ_OPT_COUNT) return fail(QJ_EARG, "unknown option");
+  *value = p->opt[option];
   return QJ_OK;
 }
 
@@ -532,10 +547,19 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
     QJ_STAGE(kStageDriftY, launch_drift_axis<T>(p->d, tb, 1, st->psi, nullptr, nullptr, 0.0, 0, s),
              "drift y");
     src = stars ? st->rho_s : (gas ? live_gas(0) : nullptr);
-    QJ_STAGE(kStageDriftXSource,
-             launch_source_x<T>(p->d, tb, 1, st->psi, src, stars ? 1 : 0, brho, pr->b_coeff,
-                                pr->rho_bar, w.H, p->partials, p->counter, st->diag, s),
-             "drift x + source");
+    if (p->opt[QJ_OPT_FUSE_X_SOURCE]) {
+      QJ_STAGE(kStageDriftXSource,
+               launch_source_x<T>(p->d, tb, 1, st->psi, src, stars ? 1 : 0, brho, pr->b_coeff,
+                                  pr->rho_bar, w.H, p->partials, p->counter, st->diag, s),
+               "drift x + source");
+    } else {
+      QJ_STAGE(kStageDriftX,
+               launch_drift_axis<T>(p->d, tb, 0, st->psi, nullptr, nullptr, 0.0, 0, s), "drift x");
+      QJ_STAGE(kStageSourceX,
+               launch_source_x<T>(p->d, tb, 0, st->psi, src, stars ? 1 : 0, brho, pr->b_coeff,
+                                  pr->rho_bar, w.H, p->partials, p->counter, st->diag, s),
+               "source x");
+    }
     if (int rc = poisson_tail<T>(p, w, grad, pr->four_pi_G, pr->rho_bar, pr->dc_mode, V, A4,
                                  nullptr, st->diag, s))
       return rc;

@@ -137,7 +137,7 @@ __device__ __forceinline__ void pencil_fft(cpx<T>* v, int j, cpx<T>* lds,
 // access for c64, and the pencils share the stage twiddles and the two
 // block barriers of the exchange.  All VW pencils use the column-fastest
 // LDS layout, one kLdsStrided block each.
-template <int N, int DIR, class T, int VW>
+template <int N, int DIR, class T, int VW, bool PHASED = false>
 __device__ __forceinline__ void pencil_fft_strided(cpx<T> (*v)[PencilShape<N>::E], int j, int col,
                                                    cpx<T>* lds, const cpx<T>* __restrict__ tw) {
   using S = PencilShape<N>;
@@ -150,6 +150,27 @@ __device__ __forceinline__ void pencil_fft_strided(cpx<T> (*v)[PencilShape<N>::E
       const cpx<T> t = tw[(j * k1) & (N - 1)];
 #pragma unroll
       for (int w = 0; w < VW; ++w) v[w][k1] = (DIR < 0) ? cmul(v[w][k1], t) : cmulc(v[w][k1], t);
+    }
+    if constexpr (PHASED) {
+      // one exchange buffer, the VW pencils take turns (half the LDS)
+#pragma unroll
+      for (int w = 0; w < VW; ++w) {
+        __syncthreads();
+#pragma unroll
+        for (int k1 = 0; k1 < E; ++k1) lds[k1 * KS + j * C + col] = v[w][k1];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < E / TT; ++q) {
+          cpx<T> u[TT];
+          const int k1 = j + TT * q;
+#pragma unroll
+          for (int b = 0; b < TT; ++b) u[b] = lds[k1 * KS + b * C + col];
+          dft<TT, DIR, T>(u);
+#pragma unroll
+          for (int kb = 0; kb < TT; ++kb) v[w][q + (E / TT) * kb] = u[kb];
+        }
+      }
+      return;
     }
     __syncthreads();
 #pragma unroll

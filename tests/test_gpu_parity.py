@@ -308,3 +308,22 @@ def test_plain_torch_comparator_matches_oracle(cuda):
     ref = O.compute_step_stars(psi, pos, vel, 0.0, 0.0, np.inf, 1e-3, m_s, g, ph)
     assert rel(out[0], ref[0]) < 1e-12
     assert rel(out[2], ref[2]) < 1e-12
+
+
+def test_x_schedules_agree_bitwise(qj, cuda):
+    # the fused and the split x sweep compute the same operations
+    _lib = __import__("importlib").import_module("quantum-jax_amd._lib")
+    g = O.Grid(64, 32, 32)
+    psi, pos, vel, m_s = _stars_case(g, 800, 16)
+    out = []
+    for fuse in (1, 0):
+        plan = plan_for(qj, g, C64)
+        plan.set_option(_lib.OPT_FUSE_X_SOURCE, fuse)
+        assert plan.get_option(_lib.OPT_FUSE_X_SOURCE) == fuse
+        sim = qj.KDKSimulation(plan, qj.Physics(rho_bar=1.1), 1e-3, dev(psi, C64),
+                               dev(pos, torch.float64), dev(vel, torch.float64), mass=m_s)
+        sim.step(3)
+        st = sim.state()
+        out.append((st["psi"].clone(), st["pos"].clone(), st["vel"].clone()))
+    for a, b in zip(*out):
+        assert torch.equal(a, b)
