@@ -109,6 +109,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -312,6 +313,57 @@ def main():
         out["speedup_vs_plain_c128"] = value / plain128
         log(f"plain c128 {plain128:.3e}, hip c128 {hip128:.3e}, speedup {value / plain128:.1f}x")
         del p, x, v, pk
+        torch.cuda.empty_cache()
+
+    # ---- adjoint step cost: 128^3 c64 + 2^16 stars, 256 steps, checkpoint 16 --
+    if not args.no_adjoint:
+        ad = importlib.import_module("quantum-jax_amd.adjoint")
+        na, npa, nsa, seg = 128, 1 << 16, 256, 16
+        log(f"adjoint: {na}^3 c64 + {npa} stars, {nsa} steps, checkpoint every {seg}")
+        pa = qj.Plan(na, dtype=torch.complex64, device=dev)
+        psa = torch.as_tensor(qj.ic.filtered_gaussian_psi((na, na, na), 17)).to(dev,
+                                                                                 torch.complex64)
+        x0, v0, msa = qj.ic.star_ic(npa, 1017)
+        xa = torch.as_tensor(x0).to(dev)
+        va = torch.as_tensor(v0).to(dev)
+        ma = torch.full((npa,), msa, dtype=torch.float64, device=dev)
+        tgt = torch.as_tensor(qj.ic.filtered_gaussian_psi((na, na, na), 2017)).abs().pow(2).to(
+            dev, torch.float32)
+
+        def fwd_only():
+            with torch.no_grad():
+                out = ad.kdk_run(psa, xa, va, ma, pa, phys, dt, nsa, seg)
+                return ad.density_loss(out[0], out[1], ma, tgt, pa)
+
+        def fwd_bwd():
+            p_ = psa.clone().requires_grad_(True)
+            m_ = ma.clone().requires_grad_(True)
+            out = ad.kdk_run(p_, xa, va, m_, pa, phys, dt, nsa, seg)
+            L_ = ad.density_loss(out[0], out[1], m_, tgt, pa)
+            L_.backward()
+            return p_.grad, m_.grad
+
+        fwd_only()
+        fwd_bwd()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        fwd_only()
+        torch.cuda.synchronize()
+        t_f = time.perf_counter() - t
+        t = time.perf_counter()
+        gp, gm = fwd_bwd()
+        torch.cuda.synchronize()
+        t_fb = time.perf_counter() - t
+        out["adjoint"] = {
+            "workload": f"{na}^3 c64 + {npa} stars, {nsa} steps, dt={dt}, checkpoint every {seg}, "
+                        "loss sum((|psi|^2 + rho_s - rho_target)^2), grad w.r.t. psi0 and masses",
+            "forward_ms_per_step": 1e3 * t_f / nsa,
+            "forward_plus_adjoint_ms_per_step": 1e3 * t_fb / nsa,
+            "cost_ratio": t_fb / t_f,
+            "grad_finite": bool(torch.isfinite(gp).all().item() and torch.isfinite(gm).all().item()),
+        }
+        log(f"adjoint: {out['adjoint']}")
+        del pa, psa, gp, gm
         torch.cuda.empty_cache()
 
     # ---- CPU restatement of the reference path (bounded sample) -------------

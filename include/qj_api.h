@@ -160,6 +160,45 @@ int qj_kdk_steps(qj_plan* plan, qj_state* state, const qj_params* prm, int nstep
                  void* stream);
 int qj_kdk_finalize(qj_plan* plan, qj_state* state, const qj_params* prm, void* stream);
 
+/* ---- adjoint of the KDK step (no counterpart in the reference, which
+ * never differentiates despite README.md:7) ----------------------------
+ * qj_kdk_record runs nsteps unmerged steps (quantum-stars.py:295-330; the
+ * state must be finalized) and stores, per step, psi at the start (psi0),
+ * psi after the drift (psi2), the grid after the step's Poisson solve
+ * (grid1, 5N reals as in qj_state) and the star positions before and after
+ * the drift (x0, x1); grid0 (nullable) receives the grid at the start.
+ * qj_kdk_adjoint then maps the adjoint of the end state (psi_bar, x_bar,
+ * v_bar; psi_bar = dL/dRe psi + i dL/dIm psi) back through those steps in
+ * place and accumulates m_bar (dL/dm per star).  scratch holds
+ * qj_adjoint_scratch_size() bytes.  Stars and plain steps only. */
+typedef struct {
+  void* psi0;
+  void* psi2;
+  void* grid1;
+  double* x0;
+  double* x1;
+} qj_tape_step;
+
+typedef struct {
+  void* psi_bar;  /* complex N, in/out */
+  double* x_bar;  /* (np,3), in/out   */
+  double* v_bar;  /* (np,3), in/out   */
+  double* m_bar;  /* (np), accumulated */
+  void* scratch;
+} qj_adjoint;
+
+size_t qj_adjoint_scratch_size(const qj_plan* plan);
+int qj_kdk_record(qj_plan* plan, qj_state* state, const qj_params* prm, int nsteps,
+                  qj_tape_step* tape, void* grid0, void* work, void* stream);
+int qj_kdk_adjoint(qj_plan* plan, const qj_state* state, const qj_params* prm, int nsteps,
+                   const qj_tape_step* tape, const void* grid0, qj_adjoint* adj, void* work,
+                   void* stream);
+/* Adjoint of bin_particles: m_bar += interp(rho_bar)/vol, x_bar += the
+ * position gradient (CIC weights are piecewise linear). */
+int qj_deposit_adjoint(qj_plan* plan, const double* pos, const double* mass, double m_scalar,
+                       long long np, const void* rho_bar, double* m_bar, double* x_bar,
+                       void* stream);
+
 /* ---- schedule options (no counterpart in the reference, whose XLA
  * compiler picks the fusion) -------------------------------------------
  * QJ_OPT_FUSE_X_SOURCE: 1 = the x drift sweep and the Poisson source pass

@@ -43,6 +43,16 @@ class QjParams(C.Structure):
     ]
 
 
+class QjTapeStep(C.Structure):
+    """Mirror of ``qj_tape_step``."""
+    _fields_ = [("psi0", vp), ("psi2", vp), ("grid1", vp), ("x0", vp), ("x1", vp)]
+
+
+class QjAdjoint(C.Structure):
+    """Mirror of ``qj_adjoint``."""
+    _fields_ = [("psi_bar", vp), ("x_bar", vp), ("v_bar", vp), ("m_bar", vp), ("scratch", vp)]
+
+
 # name -> (restype, argtypes); every entry point returns int status unless noted
 SIGNATURES = {
     "qj_plan_create": (i32, [C.POINTER(vp), i32, i32, i32, f64, f64, f64, i32]),
@@ -62,6 +72,12 @@ SIGNATURES = {
     "qj_solve_hydro": (i32, [vp, vp, vp, vp, vp, f64, f64, vp, vp, vp, vp, vp]),
     "qj_kdk_steps": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), i32, vp, vp]),
     "qj_kdk_finalize": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), vp]),
+    "qj_adjoint_scratch_size": (C.c_size_t, [vp]),
+    "qj_kdk_record": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), i32,
+                            C.POINTER(QjTapeStep), vp, vp, vp]),
+    "qj_kdk_adjoint": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), i32,
+                             C.POINTER(QjTapeStep), vp, C.POINTER(QjAdjoint), vp, vp]),
+    "qj_deposit_adjoint": (i32, [vp, vp, vp, f64, i64, vp, vp, vp, vp]),
     "qj_plan_set_option": (i32, [vp, i32, i32]),
     "qj_plan_get_option": (i32, [vp, i32, C.POINTER(C.c_int)]),
     "qj_stage_count": (i32, []),

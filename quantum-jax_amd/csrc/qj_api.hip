@@ -56,12 +56,14 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 enum Stage {
   kStageDriftZKick = 0, kStageDriftZ, kStageDriftY, kStageDriftX, kStageDriftXSource,
   kStageSourceX, kStageFftY, kStagePoissonZ, kStagePoissonY, kStagePoissonX, kStageDeposit,
-  kStageStarStep, kStageStarKick, kStageHydro, kStageGasKick, kStageKick, kStageCount
+  kStageStarStep, kStageStarKick, kStageHydro, kStageGasKick, kStageKick, kStageAdjParticles,
+  kStageAdjKick, kStageAdjPoisson, kStageAdjRho, kStageAdjDrift, kStageCount
 };
 const char* kStageNames[kStageCount] = {
     "drift_z_kick", "drift_z", "drift_y", "drift_x", "drift_x_source", "source_x", "fft_y_half",
     "poisson_z", "poisson_y", "poisson_x", "deposit", "star_step", "star_kick",
-    "hydro", "gas_kick", "kick"};
+    "hydro", "gas_kick", "kick", "adj_particles", "adj_kick", "adj_poisson", "adj_rho",
+    "adj_drift"};
 
 }  // namespace
 
@@ -491,6 +493,31 @@ int qj_solve_hydro(qj_plan* p, const void* rho, const void* vx, const void* vy, 
 // ===========================================================================
 namespace {
 
+// V and A4 of the current state (star deposit, source pass, Poisson chain).
+template <class T>
+int initial_grid_t(qj_plan* p, qj_state* st, const qj_params* pr, const Work& w, hipStream_t s) {
+  SpecTables<T> tb = tables<T>(p);
+  const bool stars = st->np > 0;
+  const bool gas = st->gas[0] != nullptr;
+  void* A4 = (stars || gas) ? plane<T>(st->grid, p, 1) : nullptr;
+  void* brho = (pr->b_coeff != 0.0) ? st->brho : nullptr;
+  if (stars)
+    QJ_STAGE(kStageDeposit,
+             launch_deposit<T>(p->d, st->pos, st->mass, st->m_scalar, st->np, st->rho_s, s),
+             "deposit");
+  void* src = stars ? st->rho_s
+                    : (gas ? (st->gas_swapped ? st->gas_tmp[0] : st->gas[0]) : nullptr);
+  QJ_STAGE(kStageSourceX,
+           launch_source_x<T>(p->d, tb, 0, st->psi, src, stars ? 1 : 0, brho, pr->b_coeff,
+                              pr->rho_bar, w.H, p->partials, p->counter, st->diag, s),
+           "source_x");
+  if (int rc = poisson_tail<T>(p, w, (stars || gas) ? 1 : 0, pr->four_pi_G, pr->rho_bar,
+                               pr->dc_mode, st->grid, A4, nullptr, st->diag, s))
+    return rc;
+  st->grid_valid = 1;
+  return QJ_OK;
+}
+
 template <class T>
 int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void* work,
                 hipStream_t s) {
@@ -510,18 +537,7 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
 
   for (int step = 0; step < nsteps; ++step) {
     if (!st->grid_valid) {
-      // potential (and acceleration) of the current state
-      if (stars)
-        QJ_STAGE(kStageDeposit, launch_deposit<T>(p->d, st->pos, st->mass, st->m_scalar, st->np, st->rho_s, s),
-                 "deposit");
-      src = stars ? st->rho_s : (gas ? live_gas(0) : nullptr);
-      QJ_STAGE(kStageSourceX, launch_source_x<T>(p->d, tb, 0, st->psi, src, stars ? 1 : 0, brho, pr->b_coeff,
-                                  pr->rho_bar, w.H, p->partials, p->counter, st->diag, s),
-               "source_x");
-      if (int rc = poisson_tail<T>(p, w, grad, pr->four_pi_G, pr->rho_bar, pr->dc_mode, V, A4,
-                                   nullptr, st->diag, s))
-        return rc;
-      st->grid_valid = 1;
+      if (int rc = initial_grid_t<T>(p, st, pr, w, s)) return rc;
     }
     const double kick_t = st->pending + 0.5 * dt;  // owed half kick + this step's first
     // stars: owed closing kick of the previous step and this step's opening
@@ -597,6 +613,92 @@ int kdk_finalize_t(qj_plan* p, qj_state* st, const qj_params* pr, hipStream_t s)
   return QJ_OK;
 }
 
+// ---------------------------------------------------------------------------
+// recording and adjoint of the (unmerged) KDK step, quantum-stars.py:295-330
+// ---------------------------------------------------------------------------
+size_t grid_bytes(const qj_plan* p) { return 5 * (size_t)p->ncell() * p->rsize; }
+
+template <class T>
+int kdk_record_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, qj_tape_step* tape,
+                 void* grid0, void* work, hipStream_t s) {
+  Work w = carve(p, work);
+  if (int rc = ensure_drift<T>(p, pr->dt, pr->m_per_hbar, s)) return rc;
+  const size_t cb = (size_t)p->ncell() * 2 * sizeof(T);
+  const size_t xb = (size_t)st->np * 3 * sizeof(double);
+  if (!st->grid_valid)
+    if (int rc = initial_grid_t<T>(p, st, pr, w, s)) return rc;
+  if (grid0) QJ_CHECK(hipMemcpyAsync(grid0, st->grid, grid_bytes(p), hipMemcpyDeviceToDevice, s),
+                      "tape grid0");
+  for (int k = 0; k < nsteps; ++k) {
+    qj_tape_step& t = tape[k];
+    QJ_CHECK(hipMemcpyAsync(t.psi0, st->psi, cb, hipMemcpyDeviceToDevice, s), "tape psi0");
+    if (st->np > 0)
+      QJ_CHECK(hipMemcpyAsync(t.x0, st->pos, xb, hipMemcpyDeviceToDevice, s), "tape x0");
+    if (int rc = kdk_steps_t<T>(p, st, pr, 1, work, s)) return rc;
+    QJ_CHECK(hipMemcpyAsync(t.psi2, st->psi, cb, hipMemcpyDeviceToDevice, s), "tape psi2");
+    QJ_CHECK(hipMemcpyAsync(t.grid1, st->grid, grid_bytes(p), hipMemcpyDeviceToDevice, s),
+             "tape grid1");
+    if (st->np > 0)
+      QJ_CHECK(hipMemcpyAsync(t.x1, st->pos, xb, hipMemcpyDeviceToDevice, s), "tape x1");
+    if (int rc = kdk_finalize_t<T>(p, st, pr, s)) return rc;
+  }
+  return QJ_OK;
+}
+
+template <class T>
+int kdk_adjoint_t(qj_plan* p, const qj_state* st, const qj_params* pr, int nsteps,
+                  const qj_tape_step* tape, const void* grid0, qj_adjoint* a, void* work,
+                  hipStream_t s) {
+  Work w = carve(p, work);
+  if (int rc = ensure_drift<T>(p, pr->dt, pr->m_per_hbar, s)) return rc;
+  SpecTables<T> tb = tables<T>(p);
+  const long long n = p->ncell();
+  const double dt = pr->dt, h = 0.5 * dt, c = pr->m_per_hbar * h;
+  const bool stars = st->np > 0;
+  T* Vbar = (T*)a->scratch;
+  T* abar4 = Vbar + n;
+  T* rhobar = abar4 + 4 * n;
+  // one adjoint half step: the closing kick of psi (V, psi_ref) and of the
+  // stars (A4, pos), the density and Poisson adjoints, the deposit adjoint
+  auto half = [&](const void* grid, const void* psi_ref, const double* pos, double dt_xv) -> int {
+    const void* V = grid;
+    const void* A4 = (const char*)grid + (size_t)n * sizeof(T);
+    QJ_CHECK(fill<T>(abar4, 4 * n, (T)0, s), "abar clear");
+    if (stars)
+      QJ_STAGE(kStageAdjParticles,
+               launch_interp_adj<T>(p->d, pos, st->np, A4, h, a->v_bar, dt_xv, a->x_bar, abar4, s),
+               "interp adjoint");
+    QJ_STAGE(kStageAdjKick,
+             launch_kick_adj<T>(a->psi_bar, psi_ref, V, c, Vbar, n, s), "kick adjoint");
+    QJ_STAGE(kStageAdjPoisson,
+             launch_adj_poisson<T>(p->d, tb, Vbar, abar4, pr->four_pi_G, pr->dc_mode, w.H, w.Uz,
+                                   w.Uy, rhobar, s),
+             "adjoint Poisson");
+    QJ_STAGE(kStageAdjRho, launch_rho_adj<T>(a->psi_bar, psi_ref, rhobar, n, s), "rho adjoint");
+    if (stars)
+      QJ_STAGE(kStageAdjParticles,
+               launch_deposit_adj<T>(p->d, pos, st->mass, st->m_scalar, st->np, rhobar, a->m_bar,
+                                     a->x_bar, s),
+               "deposit adjoint");
+    return QJ_OK;
+  };
+  for (int k = nsteps - 1; k >= 0; --k) {
+    const qj_tape_step& t = tape[k];
+    const void* g0 = (k == 0) ? grid0 : tape[k - 1].grid1;
+    // closing half step (grid1 at x1, psi2)
+    if (int rc = half(t.grid1, t.psi2, t.x1, 0.0)) return rc;
+    // drift: psi_bar <- F^H psi_bar (conjugate phase tables = drift of -dt);
+    // pos1 = pos0 + dt vel1 is folded into the next interp adjoint (dt_xv)
+    for (int ax = 0; ax < 3; ++ax)
+      QJ_STAGE(kStageAdjDrift,
+               launch_drift_axis<T>(p->d, tb, ax, a->psi_bar, nullptr, nullptr, 0.0, 0, s, 1),
+               "drift adjoint");
+    // opening half step (grid0 at x0, psi0)
+    if (int rc = half(g0, t.psi0, t.x0, dt)) return rc;
+  }
+  return QJ_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -622,6 +724,59 @@ int qj_kdk_steps(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void
   hipStream_t s = (hipStream_t)stream;
   return p->dtype == QJ_C64 ? kdk_steps_t<float>(p, st, pr, nsteps, work, s)
                             : kdk_steps_t<double>(p, st, pr, nsteps, work, s);
+}
+
+size_t qj_adjoint_scratch_size(const qj_plan* p) {
+  return p ? 6 * (size_t)p->ncell() * p->rsize : 0;
+}
+
+int qj_kdk_record(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, qj_tape_step* tape,
+                  void* grid0, void* work, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!st || !pr || !work || nsteps < 0 || (nsteps > 0 && !tape))
+    return fail(QJ_EARG, "qj_kdk_record: bad arguments");
+  if (st->gas[0]) return fail(QJ_EUNSUPPORTED, "qj_kdk_record: the gas step has no adjoint");
+  if (pr->b_coeff != 0.0)
+    return fail(QJ_EUNSUPPORTED, "qj_kdk_record: self-interaction has no adjoint");
+  if (st->pending != 0.0 || st->pending_star != 0.0)
+    return fail(QJ_EARG, "qj_kdk_record: finalize the state first (owed half kick)");
+  for (int k = 0; k < nsteps; ++k)
+    if (!tape[k].psi0 || !tape[k].psi2 || !tape[k].grid1 ||
+        (st->np > 0 && (!tape[k].x0 || !tape[k].x1)))
+      return fail(QJ_EARG, "qj_kdk_record: null tape buffer");
+  hipStream_t s = (hipStream_t)stream;
+  return p->dtype == QJ_C64 ? kdk_record_t<float>(p, st, pr, nsteps, tape, grid0, work, s)
+                            : kdk_record_t<double>(p, st, pr, nsteps, tape, grid0, work, s);
+}
+
+int qj_kdk_adjoint(qj_plan* p, const qj_state* st, const qj_params* pr, int nsteps,
+                   const qj_tape_step* tape, const void* grid0, qj_adjoint* adj, void* work,
+                   void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!st || !pr || !adj || !work || nsteps < 0 || (nsteps > 0 && (!tape || !grid0)))
+    return fail(QJ_EARG, "qj_kdk_adjoint: bad arguments");
+  if (!adj->psi_bar || !adj->scratch ||
+      (st->np > 0 && (!adj->x_bar || !adj->v_bar || !adj->m_bar)))
+    return fail(QJ_EARG, "qj_kdk_adjoint: null adjoint buffer");
+  if (st->gas[0] || pr->b_coeff != 0.0)
+    return fail(QJ_EUNSUPPORTED, "qj_kdk_adjoint: gas and self-interaction have no adjoint");
+  hipStream_t s = (hipStream_t)stream;
+  return p->dtype == QJ_C64
+             ? kdk_adjoint_t<float>(p, st, pr, nsteps, tape, grid0, adj, work, s)
+             : kdk_adjoint_t<double>(p, st, pr, nsteps, tape, grid0, adj, work, s);
+}
+
+int qj_deposit_adjoint(qj_plan* p, const double* pos, const double* mass, double m_scalar,
+                       long long np, const void* rhobar, double* m_bar, double* x_bar,
+                       void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (np < 0 || (np > 0 && (!pos || !rhobar || !m_bar || !x_bar)))
+    return fail(QJ_EARG, "qj_deposit_adjoint: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e =
+      QJ_PRECISION(p, launch_deposit_adj<float>(p->d, pos, mass, m_scalar, np, rhobar, m_bar, x_bar, s),
+                   launch_deposit_adj<double>(p->d, pos, mass, m_scalar, np, rhobar, m_bar, x_bar, s));
+  return e == hipSuccess ? QJ_OK : hipfail(e, "qj_deposit_adjoint");
 }
 
 int qj_kdk_finalize(qj_plan* p, qj_state* st, const qj_params* pr, void* stream) {

@@ -34,10 +34,11 @@ template <class T>
 hipError_t launch_kick(void* psi, const void* V, const void* Vb, double coef, long long n,
                        hipStream_t s);
 
+// conj_d = 1 applies the conjugate phase table (the drift of -dt).
 template <class T>
 hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, void* psi,
                              const void* V, const void* Vb, double kick_coef, int kick,
-                             hipStream_t s);
+                             hipStream_t s, int conj_d = 0);
 
 template <class T>
 hipError_t launch_source_x(const Dims& d, const SpecTables<T>& tb, int drift, void* psi,
@@ -87,6 +88,37 @@ hipError_t launch_star_step(const Dims& d, double* pos, double* vel, long long n
 template <class T>
 hipError_t launch_star_kick(const Dims& d, const double* pos, double* vel, long long np,
                             const void* A4, double h, double* diag, hipStream_t s);
+
+// ---- adjoints of the particle operations ----------------------------------
+// Adjoint of vel += h*interp(A4, pos), preceded (dt_xv != 0) by that of
+// pos += dt_xv*vel: vbar += dt_xv*xbar; then abar4[c].xyz += w_c*h*vbar
+// (atomics; abar4 is (nx,ny,nz,4), slot 3 unused) and xbar += grad terms.
+template <class T>
+hipError_t launch_interp_adj(const Dims& d, const double* pos, long long np, const void* A4,
+                             double h, double* vbar, double dt_xv, double* xbar, void* abar4,
+                             hipStream_t s);
+// Adjoint of the deposit: mbar += interp(rhobar)/vol, xbar += grad terms.
+template <class T>
+hipError_t launch_deposit_adj(const Dims& d, const double* pos, const double* mass,
+                              double m_scalar, long long np, const void* rhobar, double* mbar,
+                              double* xbar, hipStream_t s);
+
+// ---- adjoint spectral passes -------------------------------------------------
+// psi_bar <- exp(+i coef V) psi_bar; Vbar = coef * Im(conj(psi_bar) psi_ref)
+template <class T>
+hipError_t launch_kick_adj(void* psi_bar, const void* psi_ref, const void* V, double coef,
+                           void* Vbar, long long n, hipStream_t s);
+// psi_bar += 2 rhobar psi_ref
+template <class T>
+hipError_t launch_rho_adj(void* psi_bar, const void* psi_ref, const void* rhobar, long long n,
+                          hipStream_t s);
+// rhobar = P(Vbar + div abar), P the Poisson operator (kernel -4 pi G/k^2 with
+// the k = 0 bin -4 pi G for dc_mode 0, 0 for dc_mode 1), div the spectral
+// divergence with the Nyquist derivative zeroed; abar4 is (nx,ny,nz,4).
+template <class T>
+hipError_t launch_adj_poisson(const Dims& d, const SpecTables<T>& tb, const void* Vbar,
+                              const void* abar4, double four_pi_G, int dc_mode, void* H, void* U2,
+                              void* U3, void* rhobar, hipStream_t s);
 
 // ---- gas ----------------------------------------------------------------
 // Applies v += a*kick first when kick != 0 (A4 may be null when kick == 0).

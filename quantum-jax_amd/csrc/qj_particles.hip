@@ -5,138 +5,11 @@
 // This translation unit is compiled with -ffp-contract=off: the index and
 // weight sequence of quantum-stars.py:200-209 and the leapfrog updates of
 // :304,311-312 must round exactly like separate IEEE operations.
-#include "qj_common.h"
-#include "qj_kernels.h"
+#include "qj_cic.h"
 
 namespace qj {
 
 namespace {
-
-struct CicGeom {
-  int nx, ny, nz;
-  double dx, dy, dz, vol, Lx, Ly, Lz;
-};
-
-CicGeom make_geom(const Dims& d) {
-  CicGeom g;
-  g.nx = d.nx;
-  g.ny = d.ny;
-  g.nz = d.nz;
-  g.dx = d.Lx / d.nx;
-  g.dy = d.Ly / d.ny;
-  g.dz = d.Lz / d.nz;
-  g.vol = g.dx * g.dy * g.dz;  // quantum-stars.py:135
-  g.Lx = d.Lx;
-  g.Ly = d.Ly;
-  g.Lz = d.Lz;
-  return g;
-}
-
-// floored modulo of an integer-valued double (numpy/jnp.mod semantics)
-__device__ __forceinline__ int imod(double fi, int n) {
-  long long ii = (long long)fi;
-  long long r = ii % n;
-  return (int)(r < 0 ? r + n : r);
-}
-
-// One axis of get_cic_indices_and_weights (quantum-stars.py:202-208).
-__device__ __forceinline__ void cic_axis(double x, double dx, int n, int& i0, int& i1, double& w0,
-                                         double& w1, bool& ok) {
-  double fi = floor((x - 0.5 * dx) / dx);
-  double fip1 = fi + 1.0;
-  w0 = ((fip1 + 0.5) * dx - x) / dx;
-  w1 = (x - (fi + 0.5) * dx) / dx;
-  ok = isfinite(fi) && fabs(fi) < 4.0e18;
-  i0 = ok ? imod(fi, n) : 0;
-  i1 = ok ? imod(fip1, n) : 0;
-}
-
-struct Cic {
-  int ix[2], iy[2], iz[2];
-  double wx[2], wy[2], wz[2];
-  bool ok;
-};
-
-__device__ __forceinline__ Cic cic(const CicGeom& g, double x, double y, double z) {
-  Cic c;
-  bool a, b, e;
-  cic_axis(x, g.dx, g.nx, c.ix[0], c.ix[1], c.wx[0], c.wx[1], a);
-  cic_axis(y, g.dy, g.ny, c.iy[0], c.iy[1], c.wy[0], c.wy[1], b);
-  cic_axis(z, g.dz, g.nz, c.iz[0], c.iz[1], c.wz[0], c.wz[1], e);
-  c.ok = a && b && e;
-  return c;
-}
-
-// Corner order of quantum-stars.py:220-243 and :264-287.
-__device__ constexpr int kCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1},
-                                          {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
-
-template <class T>
-__device__ __forceinline__ void deposit_one(const CicGeom& g, const Cic& c, double fac, T* rho) {
-  if (!c.ok) return;
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int sx = kCorner[k][0], sy = kCorner[k][1], sz = kCorner[k][2];
-    const double w = c.wx[sx] * c.wy[sy] * c.wz[sz] * fac;
-    const int idx = (c.ix[sx] * g.ny + c.iy[sy]) * g.nz + c.iz[sz];
-    unsafeAtomicAdd(&rho[idx], (T)w);
-  }
-}
-
-// 4 reals of the interleaved grid cell idx (a_x, a_y, a_z, V).
-template <class T>
-__device__ __forceinline__ void ld4(const T* __restrict__ A4, int idx, double o[4]) {
-  if constexpr (sizeof(T) == 4) {
-    const float4 t = *reinterpret_cast<const float4*>(A4 + 4 * idx);
-    o[0] = t.x;
-    o[1] = t.y;
-    o[2] = t.z;
-    o[3] = t.w;
-  } else {
-    const double2 a = *reinterpret_cast<const double2*>(A4 + 4 * idx);
-    const double2 b = *reinterpret_cast<const double2*>(A4 + 4 * idx + 2);
-    o[0] = a.x;
-    o[1] = a.y;
-    o[2] = b.x;
-    o[3] = b.y;
-  }
-}
-
-template <class T>
-__device__ __forceinline__ void interp_one(const CicGeom& g, const Cic& c, const T* A4,
-                                           double acc[3]) {
-  acc[0] = acc[1] = acc[2] = 0.0;
-  if (!c.ok) {
-    acc[0] = acc[1] = acc[2] = __builtin_nan("");
-    return;
-  }
-  // all 8 corner loads are issued before the accumulation
-  double a[8][4];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int sx = kCorner[k][0], sy = kCorner[k][1], sz = kCorner[k][2];
-    ld4(A4, (c.ix[sx] * g.ny + c.iy[sy]) * g.nz + c.iz[sz], a[k]);
-  }
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int sx = kCorner[k][0], sy = kCorner[k][1], sz = kCorner[k][2];
-    const double w = c.wx[sx] * c.wy[sy] * c.wz[sz];
-    acc[0] += w * a[k][0];
-    acc[1] += w * a[k][1];
-    acc[2] += w * a[k][2];
-  }
-}
-
-// floored modulo with numpy semantics (quantum-stars.py:312 jnp.mod(pos, L))
-__device__ __forceinline__ double fmod_floor(double x, double L) {
-  double r = fmod(x, L);
-  if (r != 0.0) {
-    if ((r < 0.0) != (L < 0.0)) r += L;
-  } else {
-    r = copysign(0.0, L);
-  }
-  return r;
-}
 
 __global__ void k_cic_indices(CicGeom g, const double* __restrict__ pos, long long np,
                               int* __restrict__ i, int* __restrict__ ip1, double* __restrict__ w_i,
@@ -247,6 +120,85 @@ __global__ void k_star_kick(CicGeom g, const double* __restrict__ pos, double* _
   if ((threadIdx.x & 63) == 0 && dts == dts) atomic_min_nonneg(&diag[kDiagDtS], dts);
 }
 
+// d w_c / d x_e for the 8 corners: the per-axis weights are linear in the
+// position with slopes -1/dx (w_i) and +1/dx (w_ip1) inside a cell.
+__device__ __forceinline__ void corner_grad(const CicGeom& g, const Cic& c, int k, double gw[3]) {
+  const int sx = kCorner[k][0], sy = kCorner[k][1], sz = kCorner[k][2];
+  const double dx = (sx ? 1.0 : -1.0) / g.dx, dy = (sy ? 1.0 : -1.0) / g.dy,
+               dz = (sz ? 1.0 : -1.0) / g.dz;
+  gw[0] = dx * c.wy[sy] * c.wz[sz];
+  gw[1] = c.wx[sx] * dy * c.wz[sz];
+  gw[2] = c.wx[sx] * c.wy[sy] * dz;
+}
+
+// Adjoint of vel += h * interp(A4, pos) (and, first, of the drift
+// pos += dt*vel when dt_xv != 0: vbar += dt_xv * xbar).  With
+// acc_bar = h * vbar: abar[c] += w_c acc_bar (the transpose of the gather,
+// into (a_x, a_y, a_z, -) cells) and xbar += sum_c grad w_c (a[c] . acc_bar).
+template <class T>
+__global__ void k_interp_adj(CicGeom g, const double* __restrict__ pos, long long np,
+                             const T* __restrict__ A4, double h, double* __restrict__ vbar,
+                             double dt_xv, double* __restrict__ xbar, T* __restrict__ abar4) {
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  double vb[3], xb[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    xb[d] = xbar[3 * p + d];
+    vb[d] = vbar[3 * p + d];
+    if (dt_xv != 0.0) vb[d] = vb[d] + dt_xv * xb[d];
+    vbar[3 * p + d] = vb[d];
+  }
+  Cic c = cic(g, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]);
+  if (!c.ok) return;
+  const double ab[3] = {h * vb[0], h * vb[1], h * vb[2]};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int sx = kCorner[k][0], sy = kCorner[k][1], sz = kCorner[k][2];
+    const int idx = (c.ix[sx] * g.ny + c.iy[sy]) * g.nz + c.iz[sz];
+    const double w = c.wx[sx] * c.wy[sy] * c.wz[sz];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) unsafeAtomicAdd(&abar4[4 * idx + d], (T)(w * ab[d]));
+    double a[4];
+    ld4(A4, idx, a);
+    const double s = a[0] * ab[0] + a[1] * ab[1] + a[2] * ab[2];
+    double gw[3];
+    corner_grad(g, c, k, gw);
+#pragma unroll
+    for (int e = 0; e < 3; ++e) xb[e] += gw[e] * s;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) xbar[3 * p + d] = xb[d];
+}
+
+// Adjoint of the deposit rho[c] += w_c m / vol: mbar += sum_c w_c rhobar[c] / vol,
+// xbar += (m / vol) sum_c grad w_c rhobar[c].
+template <class T>
+__global__ void k_deposit_adj(CicGeom g, const double* __restrict__ pos,
+                              const double* __restrict__ mass, double m_scalar, long long np,
+                              const T* __restrict__ rhobar, double* __restrict__ mbar,
+                              double* __restrict__ xbar) {
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  Cic c = cic(g, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]);
+  if (!c.ok) return;
+  const double fac = (mass ? mass[p] : m_scalar) / g.vol;
+  double sm = 0.0, xb[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int sx = kCorner[k][0], sy = kCorner[k][1], sz = kCorner[k][2];
+    const double r = (double)rhobar[(c.ix[sx] * g.ny + c.iy[sy]) * g.nz + c.iz[sz]];
+    sm += c.wx[sx] * c.wy[sy] * c.wz[sz] * r;
+    double gw[3];
+    corner_grad(g, c, k, gw);
+#pragma unroll
+    for (int e = 0; e < 3; ++e) xb[e] += gw[e] * r;
+  }
+  mbar[p] += sm / g.vol;
+#pragma unroll
+  for (int e = 0; e < 3; ++e) xbar[3 * p + e] += fac * xb[e];
+}
+
 inline int pblocks(long long np) { return (int)((np + 255) / 256); }
 
 }  // namespace
@@ -293,6 +245,26 @@ hipError_t launch_star_kick(const Dims& d, const double* pos, double* vel, long 
   return hipGetLastError();
 }
 
+template <class T>
+hipError_t launch_interp_adj(const Dims& d, const double* pos, long long np, const void* A4,
+                             double h, double* vbar, double dt_xv, double* xbar, void* abar4,
+                             hipStream_t s) {
+  if (np <= 0) return hipSuccess;
+  k_interp_adj<T><<<pblocks(np), 256, 0, s>>>(make_geom(d), pos, np, (const T*)A4, h, vbar, dt_xv,
+                                              xbar, (T*)abar4);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_deposit_adj(const Dims& d, const double* pos, const double* mass,
+                              double m_scalar, long long np, const void* rhobar, double* mbar,
+                              double* xbar, hipStream_t s) {
+  if (np <= 0) return hipSuccess;
+  k_deposit_adj<T><<<pblocks(np), 256, 0, s>>>(make_geom(d), pos, mass, m_scalar, np,
+                                               (const T*)rhobar, mbar, xbar);
+  return hipGetLastError();
+}
+
 #define QJ_PINST(T)                                                                              \
   template hipError_t launch_cic_indices<T>(const Dims&, const double*, long long, int*, int*,     \
                                             double*, double*, hipStream_t);                        \
@@ -304,7 +276,11 @@ hipError_t launch_star_kick(const Dims& d, const double* pos, double* vel, long 
                                           double, double, double, const double*, double, void*,   \
                                           hipStream_t);                                            \
   template hipError_t launch_star_kick<T>(const Dims&, const double*, double*, long long,          \
-                                          const void*, double, double*, hipStream_t);
+                                          const void*, double, double*, hipStream_t);            \
+  template hipError_t launch_interp_adj<T>(const Dims&, const double*, long long, const void*,     \
+                                           double, double*, double, double*, void*, hipStream_t); \
+  template hipError_t launch_deposit_adj<T>(const Dims&, const double*, const double*, double,     \
+                                            long long, const void*, double*, double*, hipStream_t);
 
 QJ_PINST(float)
 QJ_PINST(double)

@@ -76,7 +76,8 @@ __global__ __launch_bounds__(256) void k_drift_c(cpx<T>* __restrict__ psi, PassG
                                                  const cpx<T>* __restrict__ D,
                                                  const cpx<T>* __restrict__ tw,
                                                  const T* __restrict__ V,
-                                                 const T* __restrict__ Vb, double kick_coef) {
+                                                 const T* __restrict__ Vb, double kick_coef,
+                                                 int conj_d) {
   using S = PencilShape<N>;
   __shared__ cpx<T> lds[S::kLds];
   Lane<N, true> ln(threadIdx.x);
@@ -100,7 +101,10 @@ __global__ __launch_bounds__(256) void k_drift_c(cpx<T>* __restrict__ psi, PassG
   cpx<T>* my = lds + ln.col * S::E * S::P;
   pencil_fft<N, -1, T>(v, ln.j, my, tw);
 #pragma unroll
-  for (int m = 0; m < S::E; ++m) v[m] = cmul(v[m], D[ln.j + S::TT * m]);
+  for (int m = 0; m < S::E; ++m) {
+    const cpx<T> d = D[ln.j + S::TT * m];
+    v[m] = conj_d ? cmulc(v[m], d) : cmul(v[m], d);
+  }
   pencil_fft<N, +1, T>(v, ln.j, my, tw);
   if (act) {
 #pragma unroll
@@ -159,7 +163,7 @@ __device__ __forceinline__ void store_rows(const cpx<T> (&v)[VW][E], cpx<T>* __r
 template <class T, int N>
 __global__ __launch_bounds__(256) void k_drift_s(cpx<T>* __restrict__ psi, PassGeom g,
                                                  const cpx<T>* __restrict__ D,
-                                                 const cpx<T>* __restrict__ tw) {
+                                                 const cpx<T>* __restrict__ tw, int conj_d) {
   QJ_STRIDED_SETUP(T, N)
   cpx<T> v[VW][E];
   load_rows<VW, E>(v, psi, base, se, j, TT, act);
@@ -168,7 +172,7 @@ __global__ __launch_bounds__(256) void k_drift_s(cpx<T>* __restrict__ psi, PassG
   for (int m = 0; m < E; ++m) {
     const cpx<T> d = D[j + TT * m];
 #pragma unroll
-    for (int w = 0; w < VW; ++w) v[w][m] = cmul(v[w][m], d);
+    for (int w = 0; w < VW; ++w) v[w][m] = conj_d ? cmulc(v[w][m], d) : cmul(v[w][m], d);
   }
   pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
   store_rows<VW, E>(v, psi, base, se, j, TT, act);
@@ -533,6 +537,173 @@ __global__ __launch_bounds__(256) void k_poisson_x(const cpx<T>* __restrict__ H,
   }
 }
 
+// ===========================================================================
+// adjoint passes
+// ===========================================================================
+
+// psi_bar <- exp(+i coef V) psi_bar; Vbar = coef * Im(conj(psi_bar) psi_ref).
+// (Adjoint of psi_out = exp(-i coef V) psi_in: the phase factor is unitary
+// and dL/dV = coef Im(conj(psi_bar_out) psi_out) = coef Im(conj(psi_bar_in) psi_in).)
+template <class T>
+__global__ __launch_bounds__(256) void k_kick_adj(cpx<T>* __restrict__ pb,
+                                                  const cpx<T>* __restrict__ pr,
+                                                  const T* __restrict__ V, double coef,
+                                                  T* __restrict__ Vbar, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    const cpx<T> e = expmi<T>(coef * (double)V[i]);  // exp(-i coef V)
+    const cpx<T> b = cmulc(pb[i], e);                 // * exp(+i coef V)
+    pb[i] = b;
+    const cpx<T> r = pr[i];
+    Vbar[i] = (T)(coef * ((double)b.r * r.i - (double)b.i * r.r));
+  }
+}
+
+// psi_bar += 2 rhobar psi_ref  (adjoint of rho = |psi|^2)
+template <class T>
+__global__ __launch_bounds__(256) void k_rho_adj(cpx<T>* __restrict__ pb,
+                                                 const cpx<T>* __restrict__ pr,
+                                                 const T* __restrict__ rb, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    const T f = (T)2 * rb[i];
+    const cpx<T> r = pr[i];
+    cpx<T> b = pb[i];
+    b.r += f * r.r;
+    b.i += f * r.i;
+    pb[i] = b;
+  }
+}
+
+// Adjoint Poisson source along x: half spectra of
+//   T1 = FFT_x(Vbar) + i kx FFT_x(abar_x),  T2 = FFT_x(abar_y),  T3 = FFT_x(abar_z).
+template <class T, int N>
+__global__ __launch_bounds__(256) void k_adj_src_x(const T* __restrict__ Vbar,
+                                                   const T* __restrict__ abar4, PassGeom g,
+                                                   const cpx<T>* __restrict__ tw, double kfx,
+                                                   cpx<T>* __restrict__ T1,
+                                                   cpx<T>* __restrict__ T2,
+                                                   cpx<T>* __restrict__ T3) {
+  QJ_STRIDED_SETUP(T, N)
+  cpx<T> v[VW][E], u[VW][E];
+  auto load_real = [&](cpx<T>(&o)[VW][E], int comp) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int idx = base + (j + TT * m) * se;
+#pragma unroll
+      for (int w = 0; w < VW; ++w) {
+        T x = (T)0;
+        if (act) x = comp < 0 ? Vbar[idx + w] : abar4[4 * (idx + w) + comp];
+        o[w][m] = {x, (T)0};
+      }
+    }
+  };
+  auto store_half = [&](const cpx<T>(&o)[VW][E], cpx<T>* dst) {
+    if (!act) return;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int k = j + TT * m;
+      if (k <= N / 2) {
+        cpx<T> t[VW];
+#pragma unroll
+        for (int w = 0; w < VW; ++w) t[w] = o[w][m];
+        stc<VW>(dst, base + k * se, t);
+      }
+    }
+  };
+  load_real(v, -1);
+  pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
+  load_real(u, 0);
+  pencil_fft_strided<N, -1, T, VW>(u, j, col, lds, tw);
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int k = j + TT * m;
+    const T kx = (k == N / 2) ? (T)0 : (T)(kfx * (double)freq_index(k, N));
+#pragma unroll
+    for (int w = 0; w < VW; ++w) v[w][m] = {v[w][m].r - kx * u[w][m].i, v[w][m].i + kx * u[w][m].r};
+  }
+  store_half(v, T1);
+  load_real(v, 1);
+  pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
+  store_half(v, T2);
+  load_real(v, 2);
+  pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
+  store_half(v, T3);
+}
+
+// Adjoint Poisson y pass (forward FFT): T1 <- FFT_y(T1) + i ky FFT_y(T2),
+// T3 <- FFT_y(T3).
+template <class T, int N>
+__global__ __launch_bounds__(256) void k_adj_y(cpx<T>* __restrict__ T1,
+                                               const cpx<T>* __restrict__ T2,
+                                               cpx<T>* __restrict__ T3, PassGeom g,
+                                               const cpx<T>* __restrict__ tw, double kfy) {
+  QJ_STRIDED_SETUP(T, N)
+  cpx<T> v[VW][E], u[VW][E];
+  load_rows<VW, E>(v, T1, base, se, j, TT, act);
+  pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
+  load_rows<VW, E>(u, T2, base, se, j, TT, act);
+  pencil_fft_strided<N, -1, T, VW>(u, j, col, lds, tw);
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int k = j + TT * m;
+    const T ky = (k == N / 2) ? (T)0 : (T)(kfy * (double)freq_index(k, N));
+#pragma unroll
+    for (int w = 0; w < VW; ++w) v[w][m] = {v[w][m].r - ky * u[w][m].i, v[w][m].i + ky * u[w][m].r};
+  }
+  store_rows<VW, E>(v, T1, base, se, j, TT, act);
+  load_rows<VW, E>(v, T3, base, se, j, TT, act);
+  pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
+  store_rows<VW, E>(v, T3, base, se, j, TT, act);
+}
+
+// Adjoint Poisson z pass: S = FFT_z(T1) + i kz FFT_z(T3); S *= G(k)/ntot;
+// T1 <- IFFT_z(S).
+template <class T, int N>
+__global__ __launch_bounds__(256) void k_adj_z(cpx<T>* __restrict__ T1,
+                                               const cpx<T>* __restrict__ T3, PassGeom g,
+                                               const cpx<T>* __restrict__ tw, int ny_log2,
+                                               int nx, int ny, double kfx, double kfy,
+                                               double kfz, double four_pi_G, double ntot,
+                                               int dc_mode) {
+  using S = PencilShape<N>;
+  __shared__ cpx<T> lds[S::kLds];
+  Lane<N, true> ln(threadIdx.x);
+  const int q = blockIdx.x * S::C + ln.col;
+  const bool act = q < g.ncols;
+  const int base = act ? (int)pencil_base(g, q) : 0;
+  cpx<T>* my = lds + ln.col * S::E * S::P;
+  cpx<T> v[S::E], u[S::E];
+#pragma unroll
+  for (int m = 0; m < S::E; ++m) v[m] = act ? T1[base + ln.j + S::TT * m] : cpx<T>{0, 0};
+  pencil_fft<N, -1, T>(v, ln.j, my, tw);
+#pragma unroll
+  for (int m = 0; m < S::E; ++m) u[m] = act ? T3[base + ln.j + S::TT * m] : cpx<T>{0, 0};
+  pencil_fft<N, -1, T>(u, ln.j, my, tw);
+  const int x = q >> ny_log2, y = q & (ny - 1);
+  const double kx = kfx * (double)freq_index(x, nx);
+  const double ky = kfy * (double)freq_index(y, ny);
+  const double kxy2 = kx * kx + ky * ky;
+#pragma unroll
+  for (int m = 0; m < S::E; ++m) {
+    const int kzi = ln.j + S::TT * m;
+    const double kz = kfz * (double)freq_index(kzi, N);
+    const T kzt = (kzi == N / 2) ? (T)0 : (T)kz;
+    cpx<T> sv = {v[m].r - kzt * u[m].i, v[m].i + kzt * u[m].r};
+    const double k2 = kxy2 + kz * kz;
+    double Gk = -four_pi_G / (k2 == 0.0 ? 1.0 : k2);
+    if (k2 == 0.0 && dc_mode != 0) Gk = 0.0;
+    v[m] = scale(sv, (T)(Gk / ntot));
+  }
+  pencil_fft<N, +1, T>(v, ln.j, my, tw);
+  if (act) {
+#pragma unroll
+    for (int m = 0; m < S::E; ++m) T1[base + ln.j + S::TT * m] = v[m];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
@@ -595,7 +766,7 @@ hipError_t launch_kick(void* psi, const void* V, const void* Vb, double coef, lo
 template <class T>
 hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, void* psi,
                              const void* V, const void* Vb, double kick_coef, int kick,
-                             hipStream_t s) {
+                             hipStream_t s, int conj_d) {
   cpx<T>* p = (cpx<T>*)psi;
   const T* v = (const T*)V;
   const T* vb = (const T*)Vb;
@@ -605,9 +776,11 @@ hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, v
     const cpx<T>* tw = (const cpx<T>*)tb.tw_z;
     QJ_DISPATCH_N(d.nz, {
       if (kick)
-        k_drift_c<T, N_, true><<<nblocks<N_>(g.ncols), 256, 0, s>>>(p, g, D, tw, v, vb, kick_coef);
+        k_drift_c<T, N_, true><<<nblocks<N_>(g.ncols), 256, 0, s>>>(p, g, D, tw, v, vb, kick_coef,
+                                                                    conj_d);
       else
-        k_drift_c<T, N_, false><<<nblocks<N_>(g.ncols), 256, 0, s>>>(p, g, D, tw, v, vb, kick_coef);
+        k_drift_c<T, N_, false><<<nblocks<N_>(g.ncols), 256, 0, s>>>(p, g, D, tw, v, vb, kick_coef,
+                                                                     conj_d);
     });
   } else {
     if (kick) return hipErrorInvalidValue;  // the kick is fused into the z sweep only
@@ -616,7 +789,7 @@ hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, v
     const cpx<T>* D = (const cpx<T>*)(y ? tb.drift_y : tb.drift_x);
     const cpx<T>* tw = (const cpx<T>*)(y ? tb.tw_y : tb.tw_x);
     QJ_DISPATCH_N(y ? d.ny : d.nx, {
-      k_drift_s<T, N_><<<nblocks_s<T, N_>(g.ncols), 256, 0, s>>>(p, g, D, tw);
+      k_drift_s<T, N_><<<nblocks_s<T, N_>(g.ncols), 256, 0, s>>>(p, g, D, tw, conj_d);
     });
   }
   return hipGetLastError();
@@ -733,13 +906,77 @@ hipError_t launch_poisson_x(const Dims& d, const SpecTables<T>& tb, int grad, co
   return hipGetLastError();
 }
 
+template <class T>
+hipError_t launch_kick_adj(void* psi_bar, const void* psi_ref, const void* V, double coef,
+                           void* Vbar, long long n, hipStream_t s) {
+  long long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  k_kick_adj<T><<<(int)blocks, 256, 0, s>>>((cpx<T>*)psi_bar, (const cpx<T>*)psi_ref,
+                                            (const T*)V, coef, (T*)Vbar, n);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_rho_adj(void* psi_bar, const void* psi_ref, const void* rhobar, long long n,
+                          hipStream_t s) {
+  long long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  k_rho_adj<T><<<(int)blocks, 256, 0, s>>>((cpx<T>*)psi_bar, (const cpx<T>*)psi_ref,
+                                           (const T*)rhobar, n);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_adj_poisson(const Dims& d, const SpecTables<T>& tb, const void* Vbar,
+                              const void* abar4, double four_pi_G, int dc_mode, void* H, void* U2,
+                              void* U3, void* rhobar, hipStream_t s) {
+  const double kfx = 2.0 * M_PI / d.Lx, kfy = 2.0 * M_PI / d.Ly, kfz = 2.0 * M_PI / d.Lz;
+  const int nxh = d.nx / 2 + 1;
+  {
+    PassGeom g = geom_x(d);
+    const cpx<T>* tw = (const cpx<T>*)tb.tw_x;
+    QJ_DISPATCH_N(d.nx, {
+      k_adj_src_x<T, N_><<<nblocks_s<T, N_>(g.ncols), 256, 0, s>>>(
+          (const T*)Vbar, (const T*)abar4, g, tw, kfx, (cpx<T>*)H, (cpx<T>*)U2, (cpx<T>*)U3);
+    });
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  {
+    PassGeom g = geom_y(d, nxh);
+    const cpx<T>* tw = (const cpx<T>*)tb.tw_y;
+    QJ_DISPATCH_N(d.ny, {
+      k_adj_y<T, N_><<<nblocks_s<T, N_>(g.ncols), 256, 0, s>>>(
+          (cpx<T>*)H, (const cpx<T>*)U2, (cpx<T>*)U3, g, tw, kfy);
+    });
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  {
+    PassGeom g = geom_z(d, nxh);
+    const cpx<T>* tw = (const cpx<T>*)tb.tw_z;
+    const double ntot = (double)d.nx * d.ny * d.nz;
+    QJ_DISPATCH_N(d.nz, {
+      k_adj_z<T, N_><<<nblocks<N_>(g.ncols), 256, 0, s>>>(
+          (cpx<T>*)H, (const cpx<T>*)U3, g, tw, ilog2(d.ny), d.nx, d.ny, kfx, kfy, kfz,
+          four_pi_G, ntot, dc_mode);
+    });
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipError_t e = launch_poisson_y<T>(d, tb, 0, H, nullptr, nullptr, s);
+  if (e != hipSuccess) return e;
+  return launch_poisson_x<T>(d, tb, 0, H, nullptr, nullptr, rhobar, nullptr, nullptr, nullptr, s);
+}
+
 #define QJ_INST(T)                                                                              \
   template hipError_t launch_twiddles<T>(void*, int, hipStream_t);                              \
   template hipError_t launch_drift_table<T>(void*, int, double, double, hipStream_t);           \
   template hipError_t launch_kick<T>(void*, const void*, const void*, double, long long,        \
                                      hipStream_t);                                              \
   template hipError_t launch_drift_axis<T>(const Dims&, const SpecTables<T>&, int, void*,        \
-                                           const void*, const void*, double, int, hipStream_t); \
+                                           const void*, const void*, double, int, hipStream_t,  \
+                                           int);                                                \
   template hipError_t launch_source_x<T>(const Dims&, const SpecTables<T>&, int, void*, void*,  \
                                          int, void*, double, double, void*, double*,            \
                                          unsigned int*, double*, hipStream_t);                  \
@@ -753,7 +990,17 @@ hipError_t launch_poisson_x(const Dims& d, const SpecTables<T>& tb, int grad, co
                                           const void*, const void*, void*, void*, const void*,  \
                                           double*, hipStream_t);
 
+#define QJ_INST_ADJ(T)                                                                          \
+  template hipError_t launch_kick_adj<T>(void*, const void*, const void*, double, void*,        \
+                                         long long, hipStream_t);                               \
+  template hipError_t launch_rho_adj<T>(void*, const void*, const void*, long long, hipStream_t); \
+  template hipError_t launch_adj_poisson<T>(const Dims&, const SpecTables<T>&, const void*,     \
+                                            const void*, double, int, void*, void*, void*,      \
+                                            void*, hipStream_t);
+
 QJ_INST(float)
 QJ_INST(double)
+QJ_INST_ADJ(float)
+QJ_INST_ADJ(double)
 
 }  // namespace qj

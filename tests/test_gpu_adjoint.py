@@ -1,0 +1,152 @@
+"""The hand-written adjoint of the KDK step (needs a GPU).
+
+Checks, on the same seeded inputs:
+  * central finite differences of the FP64 (complex128) forward along random
+    unit directions in psi, in the star masses and in the star velocities;
+  * PyTorch autograd through the plain complex128 restatement of the
+    reference step (quantum-jax_amd/plain.py), i.e. the exact gradient of the
+    same discrete function computed by a different route;
+  * the deposit adjoint against autograd of the plain deposit;
+  * complex64 gradients against the complex128 ones.
+jax.grad of the reference is not available (no JAX in this environment).
+"""
+
+import importlib
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+C64, C128 = torch.complex64, torch.complex128
+
+
+def dev(x, dtype):
+    return torch.as_tensor(np.ascontiguousarray(x)).to("cuda", dtype)
+
+
+def case(g, n_p, seed, dtype):
+    psi = O.filtered_gaussian_psi(g, seed, sigma_k=6 * np.pi)
+    pos, vel, m_s = O.star_ic(g, n_p, seed + 1000, sigma_v=0.3)
+    rng = np.random.default_rng(seed + 7)
+    m = m_s * (0.5 + rng.random(n_p))
+    target = np.abs(O.filtered_gaussian_psi(g, seed + 2000, sigma_k=6 * np.pi)) ** 2
+    return (dev(psi, dtype), dev(pos, torch.float64), dev(vel, torch.float64),
+            dev(m, torch.float64), target)
+
+
+def loss_fn(qj, ad, plan, phys, dt, nsteps, segment, psi, pos, vel, m, target_t):
+    out = ad.kdk_run(psi, pos, vel, m, plan, phys, dt, nsteps, segment)
+    return ad.density_loss(out[0], out[1], m, target_t, plan)
+
+
+def grads(qj, ad, plan, phys, dt, nsteps, segment, psi, pos, vel, m, target):
+    psi = psi.clone().requires_grad_(True)
+    vel = vel.clone().requires_grad_(True)
+    m = m.clone().requires_grad_(True)
+    tgt = dev(target, plan.real_dtype)
+    L = loss_fn(qj, ad, plan, phys, dt, nsteps, segment, psi, pos, vel, m, tgt)
+    L.backward()
+    return L.item(), psi.grad, vel.grad, m.grad
+
+
+def test_adjoint_matches_central_differences_fp64(qj, cuda):
+    ad = importlib.import_module("quantum-jax_amd.adjoint")
+    g = O.Grid(32, 32, 32)
+    plan = qj.Plan(32, dtype=C128)
+    phys = qj.Physics(rho_bar=1.1, four_pi_G=20.0)
+    dt, nsteps, seg = 2e-3, 7, 3
+    psi, pos, vel, m, target = case(g, 300, 5, C128)
+    L0, gpsi, gvel, gm = grads(qj, ad, plan, phys, dt, nsteps, seg, psi, pos, vel, m, target)
+    tgt = dev(target, torch.float64)
+
+    def L(p, v, mm):
+        with torch.no_grad():
+            return loss_fn(qj, ad, plan, phys, dt, nsteps, seg, p, pos, v, mm, tgt).item()
+
+    rng = np.random.default_rng(11)
+    eps = 1e-3
+    for _ in range(8):
+        d = rng.standard_normal(g.shape) + 1j * rng.standard_normal(g.shape)
+        d /= np.linalg.norm(d)
+        dd = dev(d, C128)
+        fd = (L(psi + eps * dd, vel, m) - L(psi - eps * dd, vel, m)) / (2 * eps)
+        an = (gpsi.conj() * dd).real.sum().item()
+        assert abs(fd - an) <= 1e-3 * abs(an) + 1e-9 * abs(L0), (fd, an)
+    for _ in range(2):
+        d = rng.standard_normal(m.shape)
+        d /= np.linalg.norm(d)
+        dd = dev(d, torch.float64) * m.abs().max().item()
+        fd = (L(psi, vel, m + eps * dd) - L(psi, vel, m - eps * dd)) / (2 * eps)
+        an = (gm * dd).sum().item()
+        assert abs(fd - an) <= 1e-3 * abs(an) + 1e-9 * abs(L0), (fd, an)
+    d = rng.standard_normal(vel.shape)
+    d /= np.linalg.norm(d)
+    dd = dev(d, torch.float64) * 0.01
+    fd = (L(psi, vel + eps * dd, m) - L(psi, vel - eps * dd, m)) / (2 * eps)
+    an = (gvel * dd).sum().item()
+    assert abs(fd - an) <= 1e-3 * abs(an) + 1e-9 * abs(L0), (fd, an)
+
+
+def test_adjoint_matches_torch_autograd_of_plain_path(qj, cuda):
+    ad = importlib.import_module("quantum-jax_amd.adjoint")
+    plain = importlib.import_module("quantum-jax_amd.plain")
+    g = O.Grid(16, 16, 16)
+    plan = qj.Plan(16, dtype=C128)
+    phys = qj.Physics(rho_bar=1.1, four_pi_G=20.0)
+    dt, nsteps = 2e-3, 4
+    psi, pos, vel, m, target = case(g, 80, 6, C128)
+    _, gpsi, gvel, gm = grads(qj, ad, plan, phys, dt, nsteps, 2, psi, pos, vel, m, target)
+
+    pk = plain.PlainKDK(g.shape, dtype=C128, rho_bar=1.1, four_pi_G=20.0)
+    p = psi.clone().requires_grad_(True)
+    v = vel.clone().requires_grad_(True)
+    mm = m.clone().requires_grad_(True)
+    x = pos.clone()
+    a = torch.zeros((), dtype=torch.float64, device="cuda")
+    pp, vv = p, v
+    for _ in range(nsteps):
+        pp, x, vv, a, _ = pk.step_stars(pp, x, vv, mm, dt, a)
+    rho = pp.abs() ** 2 + pk.bin_particles(x, mm)
+    L = ((rho - dev(target, torch.float64)) ** 2).sum()
+    L.backward()
+    rel = lambda u, w: ((u - w).norm() / w.norm()).item()
+    assert rel(gpsi, p.grad) < 1e-8
+    assert rel(gvel, v.grad) < 1e-8
+    assert rel(gm, mm.grad) < 1e-8
+
+
+def test_deposit_adjoint_matches_autograd(qj, cuda):
+    ad = importlib.import_module("quantum-jax_amd.adjoint")
+    plain = importlib.import_module("quantum-jax_amd.plain")
+    g = O.Grid(16, 32, 8, 1.0, 2.0, 0.5)
+    plan = qj.Plan(16, 32, 8, L=(1.0, 2.0, 0.5), dtype=C128)
+    rng = np.random.default_rng(3)
+    pos = dev(rng.random((200, 3)) * np.array([1.0, 2.0, 0.5]), torch.float64)
+    m = dev(rng.random(200), torch.float64)
+    w = dev(rng.standard_normal(g.shape), torch.float64)
+    x1, m1 = pos.clone().requires_grad_(True), m.clone().requires_grad_(True)
+    (ad.bin_particles_ad(x1, m1, plan) * w).sum().backward()
+    pk = plain.PlainKDK(g.shape, L=(1.0, 2.0, 0.5), dtype=C128)
+    x2, m2 = pos.clone().requires_grad_(True), m.clone().requires_grad_(True)
+    (pk.bin_particles(x2, m2) * w).sum().backward()
+    assert torch.allclose(x1.grad, x2.grad, rtol=1e-10, atol=1e-12)
+    assert torch.allclose(m1.grad, m2.grad, rtol=1e-10, atol=1e-12)
+
+
+def test_c64_adjoint_tracks_c128(qj, cuda):
+    ad = importlib.import_module("quantum-jax_amd.adjoint")
+    g = O.Grid(32, 32, 32)
+    phys = qj.Physics(rho_bar=1.1, four_pi_G=20.0)
+    dt, nsteps = 2e-3, 5
+    res = {}
+    for cd in (C64, C128):
+        plan = qj.Plan(32, dtype=cd)
+        psi, pos, vel, m, target = case(g, 300, 8, cd)
+        res[cd] = grads(qj, ad, plan, phys, dt, nsteps, 2, psi, pos, vel, m, target)
+    rel = lambda u, w: ((u.to(w.dtype) - w).norm() / w.norm()).item()
+    assert rel(res[C64][1], res[C128][1]) < 1e-4
+    assert rel(res[C64][3], res[C128][3]) < 1e-4
