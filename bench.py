@@ -204,6 +204,13 @@ def main():
     log(f"dominant kernel {dom}: {dstat['avg_ms']} ms, {achieved} GB/s")
     step_bytes = sum(v["bytes_per_launch"] * v["launches_per_step"] for v in stages.values()
                      if v["bytes_per_launch"])
+    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE
+    # (scaled by a 1 GiB copy) for this workload, committed under profiles/
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "kdk_stars_256_hbm_traffic.json")
+    if n == 256 and n_p == (1 << 18) and os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh)["bytes_per_launch"].get(dom)
     out = {
         "metric": "KDK cell-updates/s at 256^3 (c64 vs plain c128, same GPU) + adjoint step cost",
         "value": value,
@@ -221,7 +228,8 @@ def main():
                    "grid": [n, n, n], "stars": n_p, "dt": dt, "parallelism": "none (1 GPU)"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes": dstat["bytes_per_launch"]},
         "step_bytes_algorithmic": int(step_bytes),
         "step_gbs_effective": round(step_bytes / (ms_step * 1e-3) / 1e9, 1),
         "schedule_s_fraction": round(227.4 * value / 8.0e12, 4) if n == 256 else None,
