@@ -110,6 +110,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-gas", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -321,6 +322,59 @@ def main():
         out["speedup_vs_plain_c128"] = value / plain128
         log(f"plain c128 {plain128:.3e}, hip c128 {hip128:.3e}, speedup {value / plain128:.1f}x")
         del p, x, v, pk
+        torch.cuda.empty_cache()
+
+    # ---- quantum + isothermal gas workload: 256^3, 100 steps --------------
+    if not args.no_gas:
+        ng, kg, dtg, csg = n, 100, dt, 0.1
+        log(f"gas: {ng}^3 c64 psi + FP32 gas, {kg} steps")
+        rho_g0, vx0, vy0, vz0 = qj.ic.gas_ic((ng, ng, ng), 17)
+        rb = float(np.mean(np.abs(psi0) ** 2 + rho_g0))
+        gphys = qj.Physics(rho_bar=rb, cs=csg)
+        res = {}
+        for tag, cd in (("c64", torch.complex64), ("c128", torch.complex128)):
+            rd = torch.float32 if cd == torch.complex64 else torch.float64
+            pg = qj.Plan(ng, dtype=cd, device=dev)
+            gas = [torch.as_tensor(a).to(dev, rd) for a in (rho_g0, vx0, vy0, vz0)]
+            sg = qj.KDKSimulation(pg, gphys, dtg, torch.as_tensor(psi0).to(dev, cd), gas=gas)
+            M0 = diag_mod.mass(pg, sg.psi, rho_gas=sg.gas_fields()[0])
+            sg.step(2)
+            torch.cuda.synchronize()
+            ev0.record()
+            sg.step(kg)
+            sg.finalize()
+            ev1.record()
+            torch.cuda.synchronize()
+            ms_g = ev0.elapsed_time(ev1) / kg
+            if tag == "c64":
+                L = lib_mod.lib()
+                L.qj_timing_enable(pg.handle, 1)
+                sg.step(10)
+                lib_mod.check(L.qj_timing_read(pg.handle, ms_arr, cnt_arr, nst), "timing_read")
+                L.qj_timing_enable(pg.handle, 0)
+                gst = {names[k]: round(ms_arr[k] / cnt_arr[k], 5) for k in range(nst) if cnt_arr[k]}
+            gf = sg.gas_fields()
+            vmax = max(float(v.abs().max().item()) for v in gf[1:])
+            res[tag] = dict(ms=ms_g, psi=sg.psi.to(torch.complex128), rho=gf[0].double().clone(),
+                            M0=M0, M1=diag_mod.mass(pg, sg.psi, rho_gas=gf[0]), vmax=vmax)
+            del sg, pg, gas
+            torch.cuda.empty_cache()
+        a, b = res["c64"], res["c128"]
+        dxg = 1.0 / ng
+        out["gas"] = {
+            "workload": f"{ng}^3 c64 psi + FP32 isothermal gas rho=0.2(1+0.1 g), v=0, cs={csg}, "
+                        f"dt={dtg}, {kg} steps (after 2 warm-up steps), rho_bar = IC mean",
+            "value": ng ** 3 / (a["ms"] * 1e-3), "unit": "cell-updates/s", "ms_per_step": a["ms"],
+            "c128_ms_per_step": b["ms"],
+            "stage_ms": gst,
+            "psi_rel_l2_vs_fp64": ((a["psi"] - b["psi"]).norm() / b["psi"].norm()).item(),
+            "gas_rho_rel_l2_vs_fp64": ((a["rho"] - b["rho"]).norm() / b["rho"].norm()).item(),
+            "mass_drift_c64": abs(a["M1"] - a["M0"]) / a["M0"],
+            "mass_drift_c128": abs(b["M1"] - b["M0"]) / b["M0"],
+            "max_cfl": dtg * (csg + a["vmax"]) / dxg,
+        }
+        log(f"gas: {out['gas']}")
+        del res, a, b
         torch.cuda.empty_cache()
 
     # ---- adjoint step cost: 128^3 c64 + 2^16 stars, 256 steps, checkpoint 16 --

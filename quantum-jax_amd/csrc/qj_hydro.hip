@@ -6,10 +6,6 @@
 // primitive half-step predictor (:295-311), face extrapolation (:208-221),
 // the Rusanov flux of the averaged state (:238-264) on the six faces and
 // the conservative update (:224-235, :338-347).
-//
-// The face states of a neighbour need that neighbour's predictor, which
-// needs its limited slopes: each thread evaluates the cell state of itself
-// and its six face neighbours (a 2-cell stencil reach) from the cache.
 #include "qj_common.h"
 #include "qj_kernels.h"
 
@@ -23,17 +19,6 @@ struct HGeom {
   T dx, dt, cs, vol;
 };
 
-template <class T>
-struct Cell {
-  T p[4];      // predicted primitives rho', vx', vy', vz'
-  T s[3][4];   // limited slopes [axis][var]
-};
-
-template <class T>
-__device__ __forceinline__ int wrap(int i, int n) {
-  return (i < 0) ? i + n : (i >= n ? i - n : i);
-}
-
 // quantum-fluid.py:352-407 for one variable along one axis
 template <class T>
 __device__ __forceinline__ T limit(T f, T fm, T fp, T g, T dx) {
@@ -45,46 +30,6 @@ __device__ __forceinline__ T limit(T f, T fm, T fp, T g, T dx) {
   num = -(f - fp) / dx;
   lim = fmax((T)0, fmin((T)1, num / denom));
   return lim * g;
-}
-
-template <class T>
-__device__ __forceinline__ Cell<T> cell_state(const HGeom<T>& g, const T* const* f, int x, int y,
-                                              int z) {
-  const int n[3] = {g.nx, g.ny, g.nz};
-  const int c[3] = {x, y, z};
-  auto at = [&](int xx, int yy, int zz) { return (xx * g.ny + yy) * g.nz + zz; };
-  const int ic = at(x, y, z);
-  T v0[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) v0[k] = f[k][ic];
-  Cell<T> cs;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    int cm[3] = {c[0], c[1], c[2]}, cp[3] = {c[0], c[1], c[2]};
-    cm[a] = wrap<T>(c[a] - 1, n[a]);
-    cp[a] = wrap<T>(c[a] + 1, n[a]);
-    const int im = at(cm[0], cm[1], cm[2]), ip = at(cp[0], cp[1], cp[2]);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const T fm = f[k][im], fp = f[k][ip];
-      const T grad = (fp - fm) / ((T)2 * g.dx);  // quantum-fluid.py:201-203
-      cs.s[a][k] = limit<T>(v0[k], fm, fp, grad, g.dx);
-    }
-  }
-  const T rho = v0[0], vx = v0[1], vy = v0[2], vz = v0[3];
-  const T hdt = (T)0.5 * g.dt;
-  const T c2 = g.cs * g.cs;
-  // quantum-fluid.py:295-311
-  cs.p[0] = rho - hdt * (vx * cs.s[0][0] + rho * cs.s[0][1] + vy * cs.s[1][0] + rho * cs.s[1][2] +
-                         vz * cs.s[2][0] + rho * cs.s[2][3]);
-  cs.p[1] = vx - hdt * (vx * cs.s[0][1] + vy * cs.s[1][1] + vz * cs.s[2][1] +
-                        ((T)1 / rho) * cs.s[0][0] * g.cs * g.cs);
-  cs.p[2] = vy - hdt * (vx * cs.s[0][2] + vy * cs.s[1][2] + vz * cs.s[2][2] +
-                        ((T)1 / rho) * cs.s[1][0] * g.cs * g.cs);
-  cs.p[3] = vz - hdt * (vx * cs.s[0][3] + vy * cs.s[1][3] + vz * cs.s[2][3] +
-                        ((T)1 / rho) * cs.s[2][0] * g.cs * g.cs);
-  (void)c2;
-  return cs;
 }
 
 // Rusanov flux (quantum-fluid.py:238-264); u is the face-normal velocity.
@@ -106,70 +51,215 @@ __device__ __forceinline__ void flux(T rL, T uL, T vL, T wL, T rR, T uR, T vR, T
   F[3] -= C * (T)0.5 * (rR * wR - rL * wL);
 }
 
-// Flux through the +a face of the cell with state L, neighbour state R.
-// Returns it in conserved-variable order (Mass, Momx, Momy, Momz).
-template <class T>
-__device__ __forceinline__ void face_flux(const Cell<T>& L, const Cell<T>& R, int a, T dx, T cs,
-                                          T out[4]) {
-  T fl[4], fr[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    fl[k] = L.p[k] + (T)0.5 * L.s[a][k] * dx;  // quantum-fluid.py:209
-    fr[k] = R.p[k] - (T)0.5 * R.s[a][k] * dx;  // quantum-fluid.py:210-211
-  }
-  T F[4];
-  if (a == 0) {
-    flux<T>(fl[0], fl[1], fl[2], fl[3], fr[0], fr[1], fr[2], fr[3], cs, F);
-    out[0] = F[0]; out[1] = F[1]; out[2] = F[2]; out[3] = F[3];
-  } else if (a == 1) {  // quantum-fluid.py:331-333
-    flux<T>(fl[0], fl[2], fl[3], fl[1], fr[0], fr[2], fr[3], fr[1], cs, F);
-    out[0] = F[0]; out[2] = F[1]; out[3] = F[2]; out[1] = F[3];
-  } else {              // quantum-fluid.py:334-336
-    flux<T>(fl[0], fl[3], fl[1], fl[2], fr[0], fr[3], fr[1], fr[2], cs, F);
-    out[0] = F[0]; out[3] = F[1]; out[1] = F[2]; out[2] = F[3];
-  }
-}
+// ---------------------------------------------------------------------------
+// Streaming version: a workgroup owns a TY x TZ tile of (y, z) columns and
+// marches along x over XC planes.  Raw planes (rho, v, with the owed gravity
+// kick already applied) live in a 4-slot LDS ring with a 2-cell (y, z) halo;
+// the predicted state and y/z slopes of the current plane's tile plus a
+// 1-cell ring live in LDS (double-buffered), and each thread keeps its own
+// column's predicted state and x slopes for planes x-1, x, x+1 in registers.
+// Each cell state is evaluated once (plus the ring), instead of seven times.
+// ---------------------------------------------------------------------------
+constexpr int kHT = 16;                 // max tile edge (y and z)
+constexpr int kHR = kHT + 4;            // raw region edge (2-cell halo)
+constexpr int kHS = kHT + 2;            // state region edge (1-cell ring)
 
 template <class T>
-__global__ __launch_bounds__(256) void k_hydro(HGeom<T> g, const T* __restrict__ rho,
-                                               const T* __restrict__ vx, const T* __restrict__ vy,
-                                               const T* __restrict__ vz, T* __restrict__ rho_o,
-                                               T* __restrict__ vx_o, T* __restrict__ vy_o,
-                                               T* __restrict__ vz_o) {
-  const int z = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y, x = blockIdx.z;
-  if (z >= g.nz) return;
-  const T* f[4] = {rho, vx, vy, vz};
-  const int n[3] = {g.nx, g.ny, g.nz};
-  const int c[3] = {x, y, z};
-  const Cell<T> me = cell_state(g, f, x, y, z);
-  const int ic = (x * g.ny + y) * g.nz + z;
-  // quantum-fluid.py:338 get_conserved with the un-predicted state
-  const T r0 = rho[ic];
-  T U[4] = {r0 * g.vol, r0 * vx[ic] * g.vol, r0 * vy[ic] * g.vol, r0 * vz[ic] * g.vol};
-  const T fac = g.dt * g.dx * g.dx;
+struct XState {
+  T p[4];
+  T sx[4];
+};
+
+template <class T>
+__global__ __launch_bounds__(256) void k_hydro_stream(HGeom<T> g, int TY, int TZ, int XC,
+                                                      const T* __restrict__ rho,
+                                                      const T* __restrict__ vx,
+                                                      const T* __restrict__ vy,
+                                                      const T* __restrict__ vz,
+                                                      const T* __restrict__ A4, T kick,
+                                                      T* __restrict__ rho_o, T* __restrict__ vx_o,
+                                                      T* __restrict__ vy_o, T* __restrict__ vz_o) {
+  __shared__ T raw[4][4][kHR * kHR];     // [slot][var][cell]
+  __shared__ T st[2][12][kHS * kHS];     // [buf][p0..3, sy0..3, sz0..3][cell]
+  const int tid = threadIdx.x;
+  const int nthr = TY * TZ;
+  const int RZ = TZ + 4, RY = TY + 4, SZ = TZ + 2, SY = TY + 2;
+  const int ntz = g.nz / TZ, nty = g.ny / TY;
+  const int tile = blockIdx.x % (nty * ntz);
+  const int xc = blockIdx.x / (nty * ntz);
+  const int y0 = (tile / ntz) * TY, z0 = (tile % ntz) * TZ, x0 = xc * XC;
+  const int ty = tid / TZ, tz = tid % TZ;
+  const bool own = tid < nthr;
+  const T dx = g.dx, hdt = (T)0.5 * g.dt;
+
+  auto gidx = [&](int x, int y, int z) {
+    return (((x & (g.nx - 1)) * g.ny + (y & (g.ny - 1))) * g.nz) + (z & (g.nz - 1));
+  };
+  auto load_plane = [&](int x) {
+    const int slot = x & 3;
+    for (int c = tid; c < RY * RZ; c += blockDim.x) {
+      const int yy = c / RZ - 2, zz = c % RZ - 2;
+      const int i = gidx(x, y0 + yy, z0 + zz);
+      T r = rho[i], u = vx[i], v = vy[i], w = vz[i];
+      if (kick != (T)0) {  // owed gravity kick of the gas velocity, quantum-fluid.py:274-276
+        u += A4[4 * i] * kick;
+        v += A4[4 * i + 1] * kick;
+        w += A4[4 * i + 2] * kick;
+      }
+      const int o = (yy + 2) * RZ + (zz + 2);
+      raw[slot][0][o] = r;
+      raw[slot][1][o] = u;
+      raw[slot][2][o] = v;
+      raw[slot][3][o] = w;
+    }
+  };
+  // predicted state and limited slopes of region cell (yy, zz) at plane x
+  auto cell = [&](int x, int yy, int zz, T p[4], T s[3][4]) {
+    const int o = (yy + 2) * RZ + (zz + 2);
+    const int sm = (x - 1) & 3, sc = x & 3, sp = (x + 1) & 3;
+    T v0[4];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    int cm[3] = {c[0], c[1], c[2]}, cp[3] = {c[0], c[1], c[2]};
-    cm[a] = wrap<T>(c[a] - 1, n[a]);
-    cp[a] = wrap<T>(c[a] + 1, n[a]);
-    const Cell<T> lo = cell_state(g, f, cm[0], cm[1], cm[2]);
-    const Cell<T> hi = cell_state(g, f, cp[0], cp[1], cp[2]);
-    T Fp[4], Fm[4];
-    face_flux(me, hi, a, g.dx, g.cs, Fp);
-    face_flux(lo, me, a, g.dx, g.cs, Fm);
-    // quantum-fluid.py:226-233: F += -fac*flux; F += fac*roll(flux, 1)
+    for (int k = 0; k < 4; ++k) v0[k] = raw[sc][k][o];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      U[k] += -fac * Fp[k];
-      U[k] += fac * Fm[k];
+      T fm = raw[sm][k][o], fp = raw[sp][k][o];
+      s[0][k] = limit<T>(v0[k], fm, fp, (fp - fm) / ((T)2 * dx), dx);
+      fm = raw[sc][k][o - RZ];
+      fp = raw[sc][k][o + RZ];
+      s[1][k] = limit<T>(v0[k], fm, fp, (fp - fm) / ((T)2 * dx), dx);
+      fm = raw[sc][k][o - 1];
+      fp = raw[sc][k][o + 1];
+      s[2][k] = limit<T>(v0[k], fm, fp, (fp - fm) / ((T)2 * dx), dx);
+    }
+    const T r = v0[0], u = v0[1], v = v0[2], w = v0[3];
+    // quantum-fluid.py:295-311
+    p[0] = r - hdt * (u * s[0][0] + r * s[0][1] + v * s[1][0] + r * s[1][2] + w * s[2][0] +
+                      r * s[2][3]);
+    p[1] = u - hdt * (u * s[0][1] + v * s[1][1] + w * s[2][1] + ((T)1 / r) * s[0][0] * g.cs * g.cs);
+    p[2] = v - hdt * (u * s[0][2] + v * s[1][2] + w * s[2][2] + ((T)1 / r) * s[1][0] * g.cs * g.cs);
+    p[3] = w - hdt * (u * s[0][3] + v * s[1][3] + w * s[2][3] + ((T)1 / r) * s[2][0] * g.cs * g.cs);
+  };
+  // states of plane x for the tile and its ring into st[x & 1]; own column
+  // (p, s_x) returned in xs
+  auto states = [&](int x, XState<T>& xs) {
+    const int b = x & 1;
+    const int ring = SY * SZ - nthr;
+    for (int c = tid; c < nthr + ring; c += blockDim.x) {
+      int yy, zz;
+      if (c < nthr) {
+        yy = c / TZ;
+        zz = c % TZ;
+      } else {  // ring cell number r = c - nthr, walked around the tile
+        const int r = c - nthr;
+        if (r < SZ) { yy = -1; zz = r - 1; }
+        else if (r < 2 * SZ) { yy = TY; zz = r - SZ - 1; }
+        else { const int q = r - 2 * SZ; yy = q / 2; zz = (q & 1) ? TZ : -1; }
+      }
+      T p[4], s3[3][4];
+      cell(x, yy, zz, p, s3);
+      const int o = (yy + 1) * SZ + (zz + 1);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        st[b][k][o] = p[k];
+        st[b][4 + k][o] = s3[1][k];
+        st[b][8 + k][o] = s3[2][k];
+      }
+      if (c == tid && own) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          xs.p[k] = p[k];
+          xs.sx[k] = s3[0][k];
+        }
+      }
+    }
+  };
+
+  XState<T> xm, xc0, xp;
+  // warm-up: raw planes x0-2 .. x0+1, states of x0-1 (own column) and x0
+  for (int x = x0 - 2; x <= x0 + 1; ++x) load_plane(x);
+  __syncthreads();
+  if (own) {
+    T p[4], s3[3][4];
+    cell(x0 - 1, ty, tz, p, s3);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      xm.p[k] = p[k];
+      xm.sx[k] = s3[0][k];
     }
   }
-  // quantum-fluid.py:191-197
-  rho_o[ic] = U[0] / g.vol;
-  vx_o[ic] = U[1] / U[0];
-  vy_o[ic] = U[2] / U[0];
-  vz_o[ic] = U[3] / U[0];
+  states(x0, xc0);
+  const T fac = g.dt * g.dx * g.dx;
+  for (int x = x0; x < x0 + XC; ++x) {
+    load_plane(x + 2);
+    __syncthreads();
+    states(x + 1, xp);
+    __syncthreads();
+    if (own) {
+      const int b = x & 1;
+      const int o = (ty + 1) * SZ + (tz + 1);
+      const int ro = (ty + 2) * RZ + (tz + 2);
+      const T r0 = raw[x & 3][0][ro];
+      // quantum-fluid.py:338 get_conserved on the (kicked) state of plane x
+      T U[4] = {r0 * g.vol, r0 * raw[x & 3][1][ro] * g.vol, r0 * raw[x & 3][2][ro] * g.vol,
+                r0 * raw[x & 3][3][ro] * g.vol};
+      T Fp[4], Fm[4], fl[4], fr[4];
+      // x faces: L/R from the own column's registers
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        fl[k] = xc0.p[k] + (T)0.5 * xc0.sx[k] * dx;
+        fr[k] = xp.p[k] - (T)0.5 * xp.sx[k] * dx;
+      }
+      flux<T>(fl[0], fl[1], fl[2], fl[3], fr[0], fr[1], fr[2], fr[3], g.cs, Fp);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        fl[k] = xm.p[k] + (T)0.5 * xm.sx[k] * dx;
+        fr[k] = xc0.p[k] - (T)0.5 * xc0.sx[k] * dx;
+      }
+      flux<T>(fl[0], fl[1], fl[2], fl[3], fr[0], fr[1], fr[2], fr[3], g.cs, Fm);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        U[k] += -fac * Fp[k];
+        U[k] += fac * Fm[k];
+      }
+      // y and z faces from the state plane (rotated flux arguments,
+      // quantum-fluid.py:331-336)
+#pragma unroll
+      for (int a = 1; a < 3; ++a) {
+        const int step = (a == 1) ? SZ : 1;
+        const int sl = 4 * a;  // s_y at 4..7, s_z at 8..11
+        T F[2][4];
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+          const int oL = side ? o : o - step, oR = side ? o + step : o;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            fl[k] = st[b][k][oL] + (T)0.5 * st[b][sl + k][oL] * dx;
+            fr[k] = st[b][k][oR] - (T)0.5 * st[b][sl + k][oR] * dx;
+          }
+          T f[4];
+          if (a == 1) {
+            flux<T>(fl[0], fl[2], fl[3], fl[1], fr[0], fr[2], fr[3], fr[1], g.cs, f);
+            F[side][0] = f[0]; F[side][2] = f[1]; F[side][3] = f[2]; F[side][1] = f[3];
+          } else {
+            flux<T>(fl[0], fl[3], fl[1], fl[2], fr[0], fr[3], fr[1], fr[2], g.cs, f);
+            F[side][0] = f[0]; F[side][3] = f[1]; F[side][1] = f[2]; F[side][2] = f[3];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          U[k] += -fac * F[1][k];
+          U[k] += fac * F[0][k];
+        }
+      }
+      const int i = gidx(x, y0 + ty, z0 + tz);
+      rho_o[i] = U[0] / g.vol;  // quantum-fluid.py:191-197
+      vx_o[i] = U[1] / U[0];
+      vy_o[i] = U[2] / U[0];
+      vz_o[i] = U[3] / U[0];
+    }
+    xm = xc0;
+    xc0 = xp;
+  }
 }
 
 template <class T>
@@ -200,12 +290,7 @@ template <class T>
 hipError_t launch_hydro(const Dims& d, const void* rho, const void* vx, const void* vy,
                         const void* vz, const void* A4, double kick, double dt, double cs,
                         void* rho_o, void* vx_o, void* vy_o, void* vz_o, hipStream_t s) {
-  if (kick != 0.0) {
-    // pending gravitational kick of the gas velocity (applied in place)
-    hipError_t e = launch_gas_kick<T>((long long)d.nx * d.ny * d.nz, (void*)vx, (void*)vy,
-                                      (void*)vz, A4, kick, s);
-    if (e != hipSuccess) return e;
-  }
+  if (kick != 0.0 && A4 == nullptr) return hipErrorInvalidValue;
   HGeom<T> g;
   g.nx = d.nx;
   g.ny = d.ny;
@@ -215,10 +300,12 @@ hipError_t launch_hydro(const Dims& d, const void* rho, const void* vx, const vo
   g.dt = (T)dt;
   g.cs = (T)cs;
   g.vol = (T)(dx * dx * dx);
-  dim3 block(d.nz < 256 ? d.nz : 256);
-  dim3 grid((d.nz + block.x - 1) / block.x, d.ny, d.nx);
-  k_hydro<T><<<grid, block, 0, s>>>(g, (const T*)rho, (const T*)vx, (const T*)vy, (const T*)vz,
-                                    (T*)rho_o, (T*)vx_o, (T*)vy_o, (T*)vz_o);
+  const int TY = d.ny < kHT ? d.ny : kHT, TZ = d.nz < kHT ? d.nz : kHT;
+  const int XC = d.nx < 32 ? d.nx : 32;
+  const int nblk = (d.ny / TY) * (d.nz / TZ) * (d.nx / XC);
+  k_hydro_stream<T><<<nblk, TY * TZ, 0, s>>>(g, TY, TZ, XC, (const T*)rho, (const T*)vx,
+                                              (const T*)vy, (const T*)vz, (const T*)A4, (T)kick,
+                                              (T*)rho_o, (T*)vx_o, (T*)vy_o, (T*)vz_o);
   return hipGetLastError();
 }
 

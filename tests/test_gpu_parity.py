@@ -185,8 +185,10 @@ def test_interp_and_get_acceleration_match_oracle(qj, cuda, dtype):
 
 
 @pytest.mark.parametrize("dtype", [C64, C128])
-def test_hydro_matches_oracle(qj, cuda, dtype):
-    g = O.Grid(32, 32, 32)
+@pytest.mark.parametrize("n", [8, 32, 64])
+def test_hydro_matches_oracle(qj, cuda, dtype, n):
+    # 8: one tile, one x chunk; 64: 4x4 tiles of 16x16 columns, two x chunks
+    g = O.Grid(n, n, n)
     X, Y, Z = O.cell_centres(g)
     rng = np.random.default_rng(11)
     rho = 0.2 * (1 + 0.3 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y)) \
