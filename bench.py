@@ -346,17 +346,18 @@ def main():
             ev1.record()
             torch.cuda.synchronize()
             ms_g = ev0.elapsed_time(ev1) / kg
-            if tag == "c64":
+            gf = sg.gas_fields()
+            vmax = max(float(v.abs().max().item()) for v in gf[1:])
+            res[tag] = dict(ms=ms_g, psi=sg.psi.to(torch.complex128).clone(),
+                            rho=gf[0].double().clone(),
+                            M0=M0, M1=diag_mod.mass(pg, sg.psi, rho_gas=gf[0]), vmax=vmax)
+            if tag == "c64":   # stage timing after the state was taken
                 L = lib_mod.lib()
                 L.qj_timing_enable(pg.handle, 1)
                 sg.step(10)
                 lib_mod.check(L.qj_timing_read(pg.handle, ms_arr, cnt_arr, nst), "timing_read")
                 L.qj_timing_enable(pg.handle, 0)
                 gst = {names[k]: round(ms_arr[k] / cnt_arr[k], 5) for k in range(nst) if cnt_arr[k]}
-            gf = sg.gas_fields()
-            vmax = max(float(v.abs().max().item()) for v in gf[1:])
-            res[tag] = dict(ms=ms_g, psi=sg.psi.to(torch.complex128), rho=gf[0].double().clone(),
-                            M0=M0, M1=diag_mod.mass(pg, sg.psi, rho_gas=gf[0]), vmax=vmax)
             del sg, pg, gas
             torch.cuda.empty_cache()
         a, b = res["c64"], res["c128"]

@@ -19,16 +19,26 @@ struct HGeom {
   T dx, dt, cs, vol;
 };
 
-// quantum-fluid.py:352-407 for one variable along one axis
+// Reciprocal: the hardware approximation for FP32 (within 1 ulp), an IEEE
+// division for FP64.  Used where the reference divides several values by
+// one denominator.
 template <class T>
-__device__ __forceinline__ T limit(T f, T fm, T fp, T g, T dx) {
+__device__ __forceinline__ T rcp(T x) {
+  if constexpr (sizeof(T) == 4) return __builtin_amdgcn_rcpf(x);
+  else return (T)1 / x;
+}
+
+// Centred gradient (quantum-fluid.py:201-203) and the ratio-form minmod
+// limiter (:352-407) of one variable along one axis.
+template <class T>
+__device__ __forceinline__ T limit(T f, T fm, T fp, T inv_dx) {
   const T eps = (T)1.0e-12;
+  T g = (fp - fm) * ((T)0.5 * inv_dx);
   const T denom = (g > (T)0) ? g + eps : ((g < (T)0) ? g - eps : eps);
-  T num = (f - fm) / dx;
-  T lim = fmax((T)0, fmin((T)1, num / denom));
+  const T rd = rcp(denom);
+  T lim = fmax((T)0, fmin((T)1, ((f - fm) * inv_dx) * rd));
   g = lim * g;
-  num = -(f - fp) / dx;
-  lim = fmax((T)0, fmin((T)1, num / denom));
+  lim = fmax((T)0, fmin((T)1, (-(f - fp) * inv_dx) * rd));
   return lim * g;
 }
 
@@ -40,10 +50,11 @@ __device__ __forceinline__ void flux(T rL, T uL, T vL, T wL, T rR, T uR, T vR, T
   const T mv = (T)0.5 * (rL * vL + rR * vR);
   const T mw = (T)0.5 * (rL * wL + rR * wR);
   const T P = rho_star * cs * cs;
+  const T ir = rcp(rho_star);
   F[0] = mu;
-  F[1] = mu * mu / rho_star + P;
-  F[2] = mu * mv / rho_star;
-  F[3] = mu * mw / rho_star;
+  F[1] = mu * mu * ir + P;
+  F[2] = mu * mv * ir;
+  F[3] = mu * mw * ir;
   const T C = fmax(cs + fabs(uL), cs + fabs(uR));
   F[0] -= C * (T)0.5 * (rR - rL);
   F[1] -= C * (T)0.5 * (rR * uR - rL * uL);
@@ -71,7 +82,7 @@ struct XState {
 };
 
 template <class T>
-__global__ __launch_bounds__(256) void k_hydro_stream(HGeom<T> g, int TY, int TZ, int XC,
+__global__ __launch_bounds__(384) void k_hydro_stream(HGeom<T> g, int TY, int TZ, int XC,
                                                       const T* __restrict__ rho,
                                                       const T* __restrict__ vx,
                                                       const T* __restrict__ vy,
@@ -90,7 +101,7 @@ __global__ __launch_bounds__(256) void k_hydro_stream(HGeom<T> g, int TY, int TZ
   const int y0 = (tile / ntz) * TY, z0 = (tile % ntz) * TZ, x0 = xc * XC;
   const int ty = tid / TZ, tz = tid % TZ;
   const bool own = tid < nthr;
-  const T dx = g.dx, hdt = (T)0.5 * g.dt;
+  const T dx = g.dx, hdt = (T)0.5 * g.dt, inv_dx = (T)1 / g.dx;
 
   auto gidx = [&](int x, int y, int z) {
     return (((x & (g.nx - 1)) * g.ny + (y & (g.ny - 1))) * g.nz) + (z & (g.nz - 1));
@@ -122,54 +133,52 @@ __global__ __launch_bounds__(256) void k_hydro_stream(HGeom<T> g, int TY, int TZ
     for (int k = 0; k < 4; ++k) v0[k] = raw[sc][k][o];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      T fm = raw[sm][k][o], fp = raw[sp][k][o];
-      s[0][k] = limit<T>(v0[k], fm, fp, (fp - fm) / ((T)2 * dx), dx);
-      fm = raw[sc][k][o - RZ];
-      fp = raw[sc][k][o + RZ];
-      s[1][k] = limit<T>(v0[k], fm, fp, (fp - fm) / ((T)2 * dx), dx);
-      fm = raw[sc][k][o - 1];
-      fp = raw[sc][k][o + 1];
-      s[2][k] = limit<T>(v0[k], fm, fp, (fp - fm) / ((T)2 * dx), dx);
+      s[0][k] = limit<T>(v0[k], raw[sm][k][o], raw[sp][k][o], inv_dx);
+      s[1][k] = limit<T>(v0[k], raw[sc][k][o - RZ], raw[sc][k][o + RZ], inv_dx);
+      s[2][k] = limit<T>(v0[k], raw[sc][k][o - 1], raw[sc][k][o + 1], inv_dx);
     }
     const T r = v0[0], u = v0[1], v = v0[2], w = v0[3];
+    const T ir = rcp(r);
     // quantum-fluid.py:295-311
     p[0] = r - hdt * (u * s[0][0] + r * s[0][1] + v * s[1][0] + r * s[1][2] + w * s[2][0] +
                       r * s[2][3]);
-    p[1] = u - hdt * (u * s[0][1] + v * s[1][1] + w * s[2][1] + ((T)1 / r) * s[0][0] * g.cs * g.cs);
-    p[2] = v - hdt * (u * s[0][2] + v * s[1][2] + w * s[2][2] + ((T)1 / r) * s[1][0] * g.cs * g.cs);
-    p[3] = w - hdt * (u * s[0][3] + v * s[1][3] + w * s[2][3] + ((T)1 / r) * s[2][0] * g.cs * g.cs);
+    p[1] = u - hdt * (u * s[0][1] + v * s[1][1] + w * s[2][1] + ir * s[0][0] * g.cs * g.cs);
+    p[2] = v - hdt * (u * s[0][2] + v * s[1][2] + w * s[2][2] + ir * s[1][0] * g.cs * g.cs);
+    p[3] = w - hdt * (u * s[0][3] + v * s[1][3] + w * s[2][3] + ir * s[2][0] * g.cs * g.cs);
   };
   // states of plane x for the tile and its ring into st[x & 1]; own column
   // (p, s_x) returned in xs
+  // one cell per thread: the tile's cells on threads 0..nthr-1, the ring on
+  // the next (2*TY + 2*TZ + 4) threads
   auto states = [&](int x, XState<T>& xs) {
     const int b = x & 1;
     const int ring = SY * SZ - nthr;
-    for (int c = tid; c < nthr + ring; c += blockDim.x) {
-      int yy, zz;
-      if (c < nthr) {
-        yy = c / TZ;
-        zz = c % TZ;
-      } else {  // ring cell number r = c - nthr, walked around the tile
-        const int r = c - nthr;
-        if (r < SZ) { yy = -1; zz = r - 1; }
-        else if (r < 2 * SZ) { yy = TY; zz = r - SZ - 1; }
-        else { const int q = r - 2 * SZ; yy = q / 2; zz = (q & 1) ? TZ : -1; }
-      }
-      T p[4], s3[3][4];
-      cell(x, yy, zz, p, s3);
-      const int o = (yy + 1) * SZ + (zz + 1);
+    const int c = tid;
+    if (c >= nthr + ring) return;
+    int yy, zz;
+    if (c < nthr) {
+      yy = c / TZ;
+      zz = c % TZ;
+    } else {  // ring cell number r, walked around the tile
+      const int r = c - nthr;
+      if (r < SZ) { yy = -1; zz = r - 1; }
+      else if (r < 2 * SZ) { yy = TY; zz = r - SZ - 1; }
+      else { const int q = r - 2 * SZ; yy = q / 2; zz = (q & 1) ? TZ : -1; }
+    }
+    T p[4], s3[3][4];
+    cell(x, yy, zz, p, s3);
+    const int o = (yy + 1) * SZ + (zz + 1);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      st[b][k][o] = p[k];
+      st[b][4 + k][o] = s3[1][k];
+      st[b][8 + k][o] = s3[2][k];
+    }
+    if (c < nthr) {
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        st[b][k][o] = p[k];
-        st[b][4 + k][o] = s3[1][k];
-        st[b][8 + k][o] = s3[2][k];
-      }
-      if (c == tid && own) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          xs.p[k] = p[k];
-          xs.sx[k] = s3[0][k];
-        }
+        xs.p[k] = p[k];
+        xs.sx[k] = s3[0][k];
       }
     }
   };
@@ -252,10 +261,11 @@ __global__ __launch_bounds__(256) void k_hydro_stream(HGeom<T> g, int TY, int TZ
         }
       }
       const int i = gidx(x, y0 + ty, z0 + tz);
+      const T iM = rcp(U[0]);
       rho_o[i] = U[0] / g.vol;  // quantum-fluid.py:191-197
-      vx_o[i] = U[1] / U[0];
-      vy_o[i] = U[2] / U[0];
-      vz_o[i] = U[3] / U[0];
+      vx_o[i] = U[1] * iM;
+      vy_o[i] = U[2] * iM;
+      vz_o[i] = U[3] * iM;
     }
     xm = xc0;
     xc0 = xp;
@@ -303,7 +313,8 @@ hipError_t launch_hydro(const Dims& d, const void* rho, const void* vx, const vo
   const int TY = d.ny < kHT ? d.ny : kHT, TZ = d.nz < kHT ? d.nz : kHT;
   const int XC = d.nx < 32 ? d.nx : 32;
   const int nblk = (d.ny / TY) * (d.nz / TZ) * (d.nx / XC);
-  k_hydro_stream<T><<<nblk, TY * TZ, 0, s>>>(g, TY, TZ, XC, (const T*)rho, (const T*)vx,
+  const int nthr = ((TY + 2) * (TZ + 2) + 63) / 64 * 64;  // tile cells + ring, whole waves
+  k_hydro_stream<T><<<nblk, nthr, 0, s>>>(g, TY, TZ, XC, (const T*)rho, (const T*)vx,
                                               (const T*)vy, (const T*)vz, (const T*)A4, (T)kick,
                                               (T*)rho_o, (T*)vx_o, (T*)vy_o, (T*)vz_o);
   return hipGetLastError();
