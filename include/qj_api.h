@@ -209,6 +209,39 @@ int qj_deposit_adjoint(qj_plan* plan, const double* pos, const double* mass, dou
 int qj_plan_set_option(qj_plan* plan, int option, int value);
 int qj_plan_get_option(const qj_plan* plan, int option, int* value);
 
+/* ---- ensembles of independent boxes (no counterpart in the reference,
+ * which runs one box per process: quantum-stars.py:454) -----------------
+ * An ensemble steps many boxes (any mix of sizes, precisions, step kinds
+ * and dt) together.  For each step it records the launches that
+ * qj_kdk_steps would issue for every member, puts members whose launch
+ * sequences have the same shape into one group and issues one batched
+ * launch per stage and group (the group's members are blockIdx.z of the
+ * launch).  The recorded step is cached and, in the graph modes, captured
+ * once into a hipGraph that is replayed every step.  Results are bitwise
+ * identical to qj_kdk_steps on each member.
+ *
+ * members: nmembers plans, states, params and workspaces (arrays).  Plans,
+ * states and workspaces must be distinct per member, stay alive, and not be
+ * used outside the ensemble while it lives.  A cached step is keyed by the
+ * members' states, params and plan options, so changing any of them between
+ * calls is allowed (it records anew). */
+#define QJ_ENS_EAGER 0      /* member after member, ordinary launches       */
+#define QJ_ENS_BATCHED 1    /* one batched launch per stage and group       */
+#define QJ_ENS_GRAPH 2      /* batched launches replayed from a hipGraph    */
+#define QJ_ENS_GRAPH_FORK 3 /* as 2, one concurrent graph branch per group  */
+typedef struct qj_ensemble qj_ensemble;
+int qj_ensemble_create(qj_ensemble** ens, int nmembers, qj_plan* const* plans,
+                       qj_state* const* states, const qj_params* params, void* const* works);
+int qj_ensemble_destroy(qj_ensemble* ens);
+int qj_ensemble_set_mode(qj_ensemble* ens, int mode);
+int qj_ensemble_steps(qj_ensemble* ens, int nsteps, void* stream);
+int qj_ensemble_finalize(qj_ensemble* ens, void* stream);
+/* Groups of the last recorded step: count, then per group its members
+ * (written to members[0..cap)), member count and launches per step. */
+int qj_ensemble_group_count(const qj_ensemble* ens);
+int qj_ensemble_group_info(const qj_ensemble* ens, int group, int* members, int cap,
+                           int* nmembers, int* launches);
+
 /* ---- stage timing (tracing hook; no counterpart in the reference, which
  * only wall-clocks its chunk loop at quantum-stars.py:451,467) ----------
  * When enabled, every kernel launch of the plan is bracketed by a pair of

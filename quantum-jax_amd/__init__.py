@@ -16,6 +16,7 @@ from .operators import (  # noqa: F401
     solve_hydro,
 )
 from .simulation import KDKSimulation, compute_step, compute_step_gas  # noqa: F401
+from .ensemble import Ensemble  # noqa: F401
 from . import ic  # noqa: F401
 from ._lib import LIB_PATH, QjError, lib  # noqa: F401
 

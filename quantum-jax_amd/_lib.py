@@ -17,6 +17,7 @@ QJ_C64 = 0
 QJ_C128 = 1
 DIAG_AMAX, DIAG_DTS, DIAG_MASS, DIAG_NAN, DIAG_COUNT = 0, 1, 2, 3, 8
 OPT_FUSE_X_SOURCE = 0
+ENS_EAGER, ENS_BATCHED, ENS_GRAPH, ENS_GRAPH_FORK = 0, 1, 2, 3
 
 vp = C.c_void_p
 dp = C.POINTER(C.c_double)
@@ -80,6 +81,15 @@ SIGNATURES = {
     "qj_deposit_adjoint": (i32, [vp, vp, vp, f64, i64, vp, vp, vp, vp]),
     "qj_plan_set_option": (i32, [vp, i32, i32]),
     "qj_plan_get_option": (i32, [vp, i32, C.POINTER(C.c_int)]),
+    "qj_ensemble_create": (i32, [C.POINTER(vp), i32, C.POINTER(vp), C.POINTER(C.POINTER(QjState)),
+                                 C.POINTER(QjParams), C.POINTER(vp)]),
+    "qj_ensemble_destroy": (i32, [vp]),
+    "qj_ensemble_set_mode": (i32, [vp, i32]),
+    "qj_ensemble_steps": (i32, [vp, i32, vp]),
+    "qj_ensemble_finalize": (i32, [vp, vp]),
+    "qj_ensemble_group_count": (i32, [vp]),
+    "qj_ensemble_group_info": (i32, [vp, i32, C.POINTER(C.c_int), i32, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int)]),
     "qj_stage_count": (i32, []),
     "qj_stage_name": (C.c_char_p, [i32]),
     "qj_timing_enable": (i32, [vp, i32]),

@@ -31,6 +31,11 @@ COMMON = [
 EXTRA = {
     "qj_particles.hip": ["-ffp-contract=off"],
 }
+# translation units compiled several times with different defines (object
+# suffix, flags): the spectral kernels build as four parallel parts
+PARTS = {
+    "qj_spectral.hip": [(f"_{k}", [f"-DQJ_PART={k}"]) for k in range(4)],
+}
 
 
 def _hipcc() -> str:
@@ -49,9 +54,12 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     os.makedirs(OBJ, exist_ok=True)
     srcs = sources()
 
-    def compile_one(f):
-        out = os.path.join(OBJ, f[:-4] + ".o")
-        cmd = [hipcc, *COMMON, *EXTRA.get(f, []), "-I", CSRC,
+    units = [(f, suf, defs) for f in srcs for suf, defs in PARTS.get(f, [("", [])])]
+
+    def compile_one(unit):
+        f, suf, defs = unit
+        out = os.path.join(OBJ, f[:-4] + suf + ".o")
+        cmd = [hipcc, *COMMON, *EXTRA.get(f, []), *defs, "-I", CSRC,
                "-I", os.path.join(HERE, "..", "include"),
                "-c", os.path.join(CSRC, f), "-o", out]
         r = subprocess.run(cmd, capture_output=True, text=True)
@@ -61,9 +69,9 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
             print(r.stderr, file=sys.stderr)
         return out
 
-    jobs = jobs or min(len(srcs), max(1, min(8, os.cpu_count() or 1)))
+    jobs = jobs or min(len(units), max(1, min(8, os.cpu_count() or 1)))
     with ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(compile_one, srcs))
+        objs = list(ex.map(compile_one, units))
     tmp = LIB + ".tmp"
     cmd = [hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -21,6 +21,15 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
+}  // namespace
+
+int qj::set_error(int code, const char* msg) {
+  g_err = msg;
+  return code;
+}
+
+namespace {
+
 int hipfail(hipError_t e, const char* where) {
   g_err = std::string(where) + ": " + hipGetErrorString(e);
   return QJ_EHIP;
@@ -35,18 +44,21 @@ int hipfail(hipError_t e, const char* where) {
 bool pow2_ok(int n) { return n >= 8 && n <= 512 && (n & (n - 1)) == 0; }
 
 template <class T>
-__global__ void k_fill(T* a, long long n, T v) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long stride = (long long)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) a[i] = v;
-}
+struct k_fill {
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(T* a, long long n, T v) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) a[i] = v;
+  }
+};
 
 template <class T>
 hipError_t fill(void* a, long long n, T v, hipStream_t s) {
   long long blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   if (blocks < 1) blocks = 1;
-  k_fill<T><<<(int)blocks, 256, 0, s>>>((T*)a, n, v);
+  launch<k_fill<T>>((int)blocks, 256, s, (T*)a, n, v);
   return hipGetLastError();
 }
 
@@ -114,7 +126,7 @@ struct StageTimer {
   hipStream_t s;
   hipEvent_t a = nullptr;
   StageTimer(qj_plan* p_, int id_, hipStream_t s_) : p(p_), id(id_), s(s_) {
-    if (p->timing) {
+    if (p->timing && !g_recorder) {
       a = p->take();
       if (a) (void)hipEventRecord(a, s);
     }

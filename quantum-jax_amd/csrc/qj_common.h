@@ -7,6 +7,8 @@
 #include <stdint.h>
 #include <type_traits>
 
+#include "qj_launch.h"
+
 namespace qj {
 
 // ---------------------------------------------------------------------------
@@ -21,13 +23,23 @@ template <class T>
 __device__ __forceinline__ cpx<T> operator+(cpx<T> a, cpx<T> b) { return {a.r + b.r, a.i + b.i}; }
 template <class T>
 __device__ __forceinline__ cpx<T> operator-(cpx<T> a, cpx<T> b) { return {a.r - b.r, a.i - b.i}; }
+// Fused multiply-add written out, so that every kernel rounds a complex
+// product the same way whatever the compiler's contraction choices in its
+// context (the fused and split schedules of a sweep then agree bitwise).
+template <class T>
+__device__ __forceinline__ T fmad(T a, T b, T c) {
+  if constexpr (std::is_same<T, float>::value)
+    return __builtin_fmaf(a, b, c);
+  else
+    return __builtin_fma(a, b, c);
+}
 template <class T>
 __device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
-  return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r};
+  return {fmad(a.r, b.r, -(a.i * b.i)), fmad(a.r, b.i, a.i * b.r)};
 }
 template <class T>
 __device__ __forceinline__ cpx<T> cmulc(cpx<T> a, cpx<T> b) {  // a * conj(b)
-  return {a.r * b.r + a.i * b.i, a.i * b.r - a.r * b.i};
+  return {fmad(a.r, b.r, a.i * b.i), fmad(a.i, b.r, -(a.r * b.i))};
 }
 template <class T>
 __device__ __forceinline__ cpx<T> conjg(cpx<T> a) { return {a.r, -a.i}; }

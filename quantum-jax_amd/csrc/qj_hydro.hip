@@ -82,208 +82,211 @@ struct XState {
 };
 
 template <class T>
-__global__ __launch_bounds__(384) void k_hydro_stream(HGeom<T> g, int TY, int TZ, int XC,
-                                                      const T* __restrict__ rho,
-                                                      const T* __restrict__ vx,
-                                                      const T* __restrict__ vy,
-                                                      const T* __restrict__ vz,
-                                                      const T* __restrict__ A4, T kick,
-                                                      T* __restrict__ rho_o, T* __restrict__ vx_o,
-                                                      T* __restrict__ vy_o, T* __restrict__ vz_o) {
-  __shared__ T raw[4][4][kHR * kHR];     // [slot][var][cell]
-  __shared__ T st[2][12][kHS * kHS];     // [buf][p0..3, sy0..3, sz0..3][cell]
-  const int tid = threadIdx.x;
-  const int nthr = TY * TZ;
-  const int RZ = TZ + 4, RY = TY + 4, SZ = TZ + 2, SY = TY + 2;
-  const int ntz = g.nz / TZ, nty = g.ny / TY;
-  const int tile = blockIdx.x % (nty * ntz);
-  const int xc = blockIdx.x / (nty * ntz);
-  const int y0 = (tile / ntz) * TY, z0 = (tile % ntz) * TZ, x0 = xc * XC;
-  const int ty = tid / TZ, tz = tid % TZ;
-  const bool own = tid < nthr;
-  const T dx = g.dx, hdt = (T)0.5 * g.dt, inv_dx = (T)1 / g.dx;
+struct k_hydro_stream {
+  static constexpr int kThreads = 384;
+  static __device__ __forceinline__ void run(HGeom<T> g, int TY, int TZ, int XC,
+                                             const T* __restrict__ rho, const T* __restrict__ vx,
+                                             const T* __restrict__ vy, const T* __restrict__ vz,
+                                             const T* __restrict__ A4, T kick,
+                                             T* __restrict__ rho_o, T* __restrict__ vx_o,
+                                             T* __restrict__ vy_o, T* __restrict__ vz_o) {
+    __shared__ T raw[4][4][kHR * kHR];     // [slot][var][cell]
+    __shared__ T st[2][12][kHS * kHS];     // [buf][p0..3, sy0..3, sz0..3][cell]
+    const int tid = threadIdx.x;
+    const int nthr = TY * TZ;
+    const int RZ = TZ + 4, RY = TY + 4, SZ = TZ + 2, SY = TY + 2;
+    const int ntz = g.nz / TZ, nty = g.ny / TY;
+    const int tile = blockIdx.x % (nty * ntz);
+    const int xc = blockIdx.x / (nty * ntz);
+    const int y0 = (tile / ntz) * TY, z0 = (tile % ntz) * TZ, x0 = xc * XC;
+    const int ty = tid / TZ, tz = tid % TZ;
+    const bool own = tid < nthr;
+    const T dx = g.dx, hdt = (T)0.5 * g.dt, inv_dx = (T)1 / g.dx;
 
-  auto gidx = [&](int x, int y, int z) {
-    return (((x & (g.nx - 1)) * g.ny + (y & (g.ny - 1))) * g.nz) + (z & (g.nz - 1));
-  };
-  auto load_plane = [&](int x) {
-    const int slot = x & 3;
-    for (int c = tid; c < RY * RZ; c += blockDim.x) {
-      const int yy = c / RZ - 2, zz = c % RZ - 2;
-      const int i = gidx(x, y0 + yy, z0 + zz);
-      T r = rho[i], u = vx[i], v = vy[i], w = vz[i];
-      if (kick != (T)0) {  // owed gravity kick of the gas velocity, quantum-fluid.py:274-276
-        u += A4[4 * i] * kick;
-        v += A4[4 * i + 1] * kick;
-        w += A4[4 * i + 2] * kick;
+    auto gidx = [&](int x, int y, int z) {
+      return (((x & (g.nx - 1)) * g.ny + (y & (g.ny - 1))) * g.nz) + (z & (g.nz - 1));
+    };
+    auto load_plane = [&](int x) {
+      const int slot = x & 3;
+      for (int c = tid; c < RY * RZ; c += blockDim.x) {
+        const int yy = c / RZ - 2, zz = c % RZ - 2;
+        const int i = gidx(x, y0 + yy, z0 + zz);
+        T r = rho[i], u = vx[i], v = vy[i], w = vz[i];
+        if (kick != (T)0) {  // owed gravity kick of the gas velocity, quantum-fluid.py:274-276
+          u += A4[4 * i] * kick;
+          v += A4[4 * i + 1] * kick;
+          w += A4[4 * i + 2] * kick;
+        }
+        const int o = (yy + 2) * RZ + (zz + 2);
+        raw[slot][0][o] = r;
+        raw[slot][1][o] = u;
+        raw[slot][2][o] = v;
+        raw[slot][3][o] = w;
       }
+    };
+    // predicted state and limited slopes of region cell (yy, zz) at plane x
+    auto cell = [&](int x, int yy, int zz, T p[4], T s[3][4]) {
       const int o = (yy + 2) * RZ + (zz + 2);
-      raw[slot][0][o] = r;
-      raw[slot][1][o] = u;
-      raw[slot][2][o] = v;
-      raw[slot][3][o] = w;
-    }
-  };
-  // predicted state and limited slopes of region cell (yy, zz) at plane x
-  auto cell = [&](int x, int yy, int zz, T p[4], T s[3][4]) {
-    const int o = (yy + 2) * RZ + (zz + 2);
-    const int sm = (x - 1) & 3, sc = x & 3, sp = (x + 1) & 3;
-    T v0[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) v0[k] = raw[sc][k][o];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      s[0][k] = limit<T>(v0[k], raw[sm][k][o], raw[sp][k][o], inv_dx);
-      s[1][k] = limit<T>(v0[k], raw[sc][k][o - RZ], raw[sc][k][o + RZ], inv_dx);
-      s[2][k] = limit<T>(v0[k], raw[sc][k][o - 1], raw[sc][k][o + 1], inv_dx);
-    }
-    const T r = v0[0], u = v0[1], v = v0[2], w = v0[3];
-    const T ir = rcp(r);
-    // quantum-fluid.py:295-311
-    p[0] = r - hdt * (u * s[0][0] + r * s[0][1] + v * s[1][0] + r * s[1][2] + w * s[2][0] +
-                      r * s[2][3]);
-    p[1] = u - hdt * (u * s[0][1] + v * s[1][1] + w * s[2][1] + ir * s[0][0] * g.cs * g.cs);
-    p[2] = v - hdt * (u * s[0][2] + v * s[1][2] + w * s[2][2] + ir * s[1][0] * g.cs * g.cs);
-    p[3] = w - hdt * (u * s[0][3] + v * s[1][3] + w * s[2][3] + ir * s[2][0] * g.cs * g.cs);
-  };
-  // states of plane x for the tile and its ring into st[x & 1]; own column
-  // (p, s_x) returned in xs
-  // one cell per thread: the tile's cells on threads 0..nthr-1, the ring on
-  // the next (2*TY + 2*TZ + 4) threads
-  auto states = [&](int x, XState<T>& xs) {
-    const int b = x & 1;
-    const int ring = SY * SZ - nthr;
-    const int c = tid;
-    if (c >= nthr + ring) return;
-    int yy, zz;
-    if (c < nthr) {
-      yy = c / TZ;
-      zz = c % TZ;
-    } else {  // ring cell number r, walked around the tile
-      const int r = c - nthr;
-      if (r < SZ) { yy = -1; zz = r - 1; }
-      else if (r < 2 * SZ) { yy = TY; zz = r - SZ - 1; }
-      else { const int q = r - 2 * SZ; yy = q / 2; zz = (q & 1) ? TZ : -1; }
-    }
-    T p[4], s3[3][4];
-    cell(x, yy, zz, p, s3);
-    const int o = (yy + 1) * SZ + (zz + 1);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      st[b][k][o] = p[k];
-      st[b][4 + k][o] = s3[1][k];
-      st[b][8 + k][o] = s3[2][k];
-    }
-    if (c < nthr) {
-#pragma unroll
+      const int sm = (x - 1) & 3, sc = x & 3, sp = (x + 1) & 3;
+      T v0[4];
+  #pragma unroll
+      for (int k = 0; k < 4; ++k) v0[k] = raw[sc][k][o];
+  #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        xs.p[k] = p[k];
-        xs.sx[k] = s3[0][k];
+        s[0][k] = limit<T>(v0[k], raw[sm][k][o], raw[sp][k][o], inv_dx);
+        s[1][k] = limit<T>(v0[k], raw[sc][k][o - RZ], raw[sc][k][o + RZ], inv_dx);
+        s[2][k] = limit<T>(v0[k], raw[sc][k][o - 1], raw[sc][k][o + 1], inv_dx);
       }
-    }
-  };
+      const T r = v0[0], u = v0[1], v = v0[2], w = v0[3];
+      const T ir = rcp(r);
+      // quantum-fluid.py:295-311
+      p[0] = r - hdt * (u * s[0][0] + r * s[0][1] + v * s[1][0] + r * s[1][2] + w * s[2][0] +
+                        r * s[2][3]);
+      p[1] = u - hdt * (u * s[0][1] + v * s[1][1] + w * s[2][1] + ir * s[0][0] * g.cs * g.cs);
+      p[2] = v - hdt * (u * s[0][2] + v * s[1][2] + w * s[2][2] + ir * s[1][0] * g.cs * g.cs);
+      p[3] = w - hdt * (u * s[0][3] + v * s[1][3] + w * s[2][3] + ir * s[2][0] * g.cs * g.cs);
+    };
+    // states of plane x for the tile and its ring into st[x & 1]; own column
+    // (p, s_x) returned in xs
+    // one cell per thread: the tile's cells on threads 0..nthr-1, the ring on
+    // the next (2*TY + 2*TZ + 4) threads
+    auto states = [&](int x, XState<T>& xs) {
+      const int b = x & 1;
+      const int ring = SY * SZ - nthr;
+      const int c = tid;
+      if (c >= nthr + ring) return;
+      int yy, zz;
+      if (c < nthr) {
+        yy = c / TZ;
+        zz = c % TZ;
+      } else {  // ring cell number r, walked around the tile
+        const int r = c - nthr;
+        if (r < SZ) { yy = -1; zz = r - 1; }
+        else if (r < 2 * SZ) { yy = TY; zz = r - SZ - 1; }
+        else { const int q = r - 2 * SZ; yy = q / 2; zz = (q & 1) ? TZ : -1; }
+      }
+      T p[4], s3[3][4];
+      cell(x, yy, zz, p, s3);
+      const int o = (yy + 1) * SZ + (zz + 1);
+  #pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        st[b][k][o] = p[k];
+        st[b][4 + k][o] = s3[1][k];
+        st[b][8 + k][o] = s3[2][k];
+      }
+      if (c < nthr) {
+  #pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          xs.p[k] = p[k];
+          xs.sx[k] = s3[0][k];
+        }
+      }
+    };
 
-  XState<T> xm, xc0, xp;
-  // warm-up: raw planes x0-2 .. x0+1, states of x0-1 (own column) and x0
-  for (int x = x0 - 2; x <= x0 + 1; ++x) load_plane(x);
-  __syncthreads();
-  if (own) {
-    T p[4], s3[3][4];
-    cell(x0 - 1, ty, tz, p, s3);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      xm.p[k] = p[k];
-      xm.sx[k] = s3[0][k];
-    }
-  }
-  states(x0, xc0);
-  const T fac = g.dt * g.dx * g.dx;
-  for (int x = x0; x < x0 + XC; ++x) {
-    load_plane(x + 2);
-    __syncthreads();
-    states(x + 1, xp);
+    XState<T> xm, xc0, xp;
+    // warm-up: raw planes x0-2 .. x0+1, states of x0-1 (own column) and x0
+    for (int x = x0 - 2; x <= x0 + 1; ++x) load_plane(x);
     __syncthreads();
     if (own) {
-      const int b = x & 1;
-      const int o = (ty + 1) * SZ + (tz + 1);
-      const int ro = (ty + 2) * RZ + (tz + 2);
-      const T r0 = raw[x & 3][0][ro];
-      // quantum-fluid.py:338 get_conserved on the (kicked) state of plane x
-      T U[4] = {r0 * g.vol, r0 * raw[x & 3][1][ro] * g.vol, r0 * raw[x & 3][2][ro] * g.vol,
-                r0 * raw[x & 3][3][ro] * g.vol};
-      T Fp[4], Fm[4], fl[4], fr[4];
-      // x faces: L/R from the own column's registers
-#pragma unroll
+      T p[4], s3[3][4];
+      cell(x0 - 1, ty, tz, p, s3);
+  #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        fl[k] = xc0.p[k] + (T)0.5 * xc0.sx[k] * dx;
-        fr[k] = xp.p[k] - (T)0.5 * xp.sx[k] * dx;
+        xm.p[k] = p[k];
+        xm.sx[k] = s3[0][k];
       }
-      flux<T>(fl[0], fl[1], fl[2], fl[3], fr[0], fr[1], fr[2], fr[3], g.cs, Fp);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        fl[k] = xm.p[k] + (T)0.5 * xm.sx[k] * dx;
-        fr[k] = xc0.p[k] - (T)0.5 * xc0.sx[k] * dx;
-      }
-      flux<T>(fl[0], fl[1], fl[2], fl[3], fr[0], fr[1], fr[2], fr[3], g.cs, Fm);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        U[k] += -fac * Fp[k];
-        U[k] += fac * Fm[k];
-      }
-      // y and z faces from the state plane (rotated flux arguments,
-      // quantum-fluid.py:331-336)
-#pragma unroll
-      for (int a = 1; a < 3; ++a) {
-        const int step = (a == 1) ? SZ : 1;
-        const int sl = 4 * a;  // s_y at 4..7, s_z at 8..11
-        T F[2][4];
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-          const int oL = side ? o : o - step, oR = side ? o + step : o;
-#pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            fl[k] = st[b][k][oL] + (T)0.5 * st[b][sl + k][oL] * dx;
-            fr[k] = st[b][k][oR] - (T)0.5 * st[b][sl + k][oR] * dx;
-          }
-          T f[4];
-          if (a == 1) {
-            flux<T>(fl[0], fl[2], fl[3], fl[1], fr[0], fr[2], fr[3], fr[1], g.cs, f);
-            F[side][0] = f[0]; F[side][2] = f[1]; F[side][3] = f[2]; F[side][1] = f[3];
-          } else {
-            flux<T>(fl[0], fl[3], fl[1], fl[2], fr[0], fr[3], fr[1], fr[2], g.cs, f);
-            F[side][0] = f[0]; F[side][3] = f[1]; F[side][1] = f[2]; F[side][2] = f[3];
-          }
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          U[k] += -fac * F[1][k];
-          U[k] += fac * F[0][k];
-        }
-      }
-      const int i = gidx(x, y0 + ty, z0 + tz);
-      const T iM = rcp(U[0]);
-      rho_o[i] = U[0] / g.vol;  // quantum-fluid.py:191-197
-      vx_o[i] = U[1] * iM;
-      vy_o[i] = U[2] * iM;
-      vz_o[i] = U[3] * iM;
     }
-    xm = xc0;
-    xc0 = xp;
+    states(x0, xc0);
+    const T fac = g.dt * g.dx * g.dx;
+    for (int x = x0; x < x0 + XC; ++x) {
+      load_plane(x + 2);
+      __syncthreads();
+      states(x + 1, xp);
+      __syncthreads();
+      if (own) {
+        const int b = x & 1;
+        const int o = (ty + 1) * SZ + (tz + 1);
+        const int ro = (ty + 2) * RZ + (tz + 2);
+        const T r0 = raw[x & 3][0][ro];
+        // quantum-fluid.py:338 get_conserved on the (kicked) state of plane x
+        T U[4] = {r0 * g.vol, r0 * raw[x & 3][1][ro] * g.vol, r0 * raw[x & 3][2][ro] * g.vol,
+                  r0 * raw[x & 3][3][ro] * g.vol};
+        T Fp[4], Fm[4], fl[4], fr[4];
+        // x faces: L/R from the own column's registers
+  #pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          fl[k] = xc0.p[k] + (T)0.5 * xc0.sx[k] * dx;
+          fr[k] = xp.p[k] - (T)0.5 * xp.sx[k] * dx;
+        }
+        flux<T>(fl[0], fl[1], fl[2], fl[3], fr[0], fr[1], fr[2], fr[3], g.cs, Fp);
+  #pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          fl[k] = xm.p[k] + (T)0.5 * xm.sx[k] * dx;
+          fr[k] = xc0.p[k] - (T)0.5 * xc0.sx[k] * dx;
+        }
+        flux<T>(fl[0], fl[1], fl[2], fl[3], fr[0], fr[1], fr[2], fr[3], g.cs, Fm);
+  #pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          U[k] += -fac * Fp[k];
+          U[k] += fac * Fm[k];
+        }
+        // y and z faces from the state plane (rotated flux arguments,
+        // quantum-fluid.py:331-336)
+  #pragma unroll
+        for (int a = 1; a < 3; ++a) {
+          const int step = (a == 1) ? SZ : 1;
+          const int sl = 4 * a;  // s_y at 4..7, s_z at 8..11
+          T F[2][4];
+  #pragma unroll
+          for (int side = 0; side < 2; ++side) {
+            const int oL = side ? o : o - step, oR = side ? o + step : o;
+  #pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              fl[k] = st[b][k][oL] + (T)0.5 * st[b][sl + k][oL] * dx;
+              fr[k] = st[b][k][oR] - (T)0.5 * st[b][sl + k][oR] * dx;
+            }
+            T f[4];
+            if (a == 1) {
+              flux<T>(fl[0], fl[2], fl[3], fl[1], fr[0], fr[2], fr[3], fr[1], g.cs, f);
+              F[side][0] = f[0]; F[side][2] = f[1]; F[side][3] = f[2]; F[side][1] = f[3];
+            } else {
+              flux<T>(fl[0], fl[3], fl[1], fl[2], fr[0], fr[3], fr[1], fr[2], g.cs, f);
+              F[side][0] = f[0]; F[side][3] = f[1]; F[side][1] = f[2]; F[side][2] = f[3];
+            }
+          }
+  #pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            U[k] += -fac * F[1][k];
+            U[k] += fac * F[0][k];
+          }
+        }
+        const int i = gidx(x, y0 + ty, z0 + tz);
+        const T iM = rcp(U[0]);
+        rho_o[i] = U[0] / g.vol;  // quantum-fluid.py:191-197
+        vx_o[i] = U[1] * iM;
+        vy_o[i] = U[2] * iM;
+        vz_o[i] = U[3] * iM;
+      }
+      xm = xc0;
+      xc0 = xp;
+    }
   }
-}
+};
 
 template <class T>
-__global__ __launch_bounds__(256) void k_gas_kick(long long n, T* __restrict__ vx,
-                                                  T* __restrict__ vy, T* __restrict__ vz,
-                                                  const T* __restrict__ A4, T h) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long stride = (long long)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) {  // quantum-fluid.py:274-276
-    vx[i] += A4[4 * i] * h;
-    vy[i] += A4[4 * i + 1] * h;
-    vz[i] += A4[4 * i + 2] * h;
+struct k_gas_kick {
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(long long n, T* __restrict__ vx, T* __restrict__ vy,
+                                             T* __restrict__ vz, const T* __restrict__ A4, T h) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {  // quantum-fluid.py:274-276
+      vx[i] += A4[4 * i] * h;
+      vy[i] += A4[4 * i + 1] * h;
+      vz[i] += A4[4 * i + 2] * h;
+    }
   }
-}
+};
 
 }  // namespace
 
@@ -292,7 +295,7 @@ hipError_t launch_gas_kick(long long n, void* vx, void* vy, void* vz, const void
                            hipStream_t s) {
   long long blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  k_gas_kick<T><<<(int)blocks, 256, 0, s>>>(n, (T*)vx, (T*)vy, (T*)vz, (const T*)A4, (T)h);
+  launch<k_gas_kick<T>>((int)blocks, 256, s, n, (T*)vx, (T*)vy, (T*)vz, (const T*)A4, (T)h);
   return hipGetLastError();
 }
 
@@ -314,7 +317,7 @@ hipError_t launch_hydro(const Dims& d, const void* rho, const void* vx, const vo
   const int XC = d.nx < 32 ? d.nx : 32;
   const int nblk = (d.ny / TY) * (d.nz / TZ) * (d.nx / XC);
   const int nthr = ((TY + 2) * (TZ + 2) + 63) / 64 * 64;  // tile cells + ring, whole waves
-  k_hydro_stream<T><<<nblk, nthr, 0, s>>>(g, TY, TZ, XC, (const T*)rho, (const T*)vx,
+  launch<k_hydro_stream<T>>(nblk, nthr, s, g, TY, TZ, XC, (const T*)rho, (const T*)vx,
                                               (const T*)vy, (const T*)vz, (const T*)A4, (T)kick,
                                               (T*)rho_o, (T*)vx_o, (T*)vy_o, (T*)vz_o);
   return hipGetLastError();

@@ -130,4 +130,7 @@ template <class T>
 hipError_t launch_gas_kick(long long n, void* vx, void* vy, void* vz, const void* A4, double h,
                            hipStream_t s);
 
+// Sets the message returned by qj_last_error() and returns code (qj_api.hip).
+int set_error(int code, const char* msg);
+
 }  // namespace qj

@@ -56,61 +56,65 @@ __global__ void k_drift_table(cpx<T>* tab, int n, double L, double c) {
 // pointwise kick: psi <- exp(-i coef (V + Vb)) psi  (quantum-stars.py:301)
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ __launch_bounds__(256) void k_kick(cpx<T>* __restrict__ psi, const T* __restrict__ V,
-                                              const T* __restrict__ Vb, double coef,
-                                              long long n) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long stride = (long long)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) {
-    double v = (double)V[i];
-    if (Vb) v += (double)Vb[i];
-    psi[i] = cmul(psi[i], expmi<T>(coef * v));
+struct k_kick {
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(cpx<T>* __restrict__ psi, const T* __restrict__ V,
+                                             const T* __restrict__ Vb, double coef, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+      double v = (double)V[i];
+      if (Vb) v += (double)Vb[i];
+      psi[i] = cmul(psi[i], expmi<T>(coef * v));
+    }
   }
-}
+};
 
 // ---------------------------------------------------------------------------
 // drift sweep along the contiguous axis, optionally preceded by the kick
 // ---------------------------------------------------------------------------
 template <class T, int N, bool KICK>
-__global__ __launch_bounds__(256) void k_drift_c(cpx<T>* __restrict__ psi, PassGeom g,
-                                                 const cpx<T>* __restrict__ D,
-                                                 const cpx<T>* __restrict__ tw,
-                                                 const T* __restrict__ V,
-                                                 const T* __restrict__ Vb, double kick_coef,
-                                                 int conj_d) {
-  using S = PencilShape<N>;
-  __shared__ cpx<T> lds[S::kLds];
-  Lane<N, true> ln(threadIdx.x);
-  const int q = blockIdx.x * S::C + ln.col;
-  const bool act = q < g.ncols;
-  const int base = act ? (int)pencil_base(g, q) : 0;
-  const int se = (int)g.s_elem;
-  cpx<T> v[S::E];
-#pragma unroll
-  for (int m = 0; m < S::E; ++m) {
-    const int idx = base + (ln.j + S::TT * m) * se;
-    v[m] = act ? psi[idx] : cpx<T>{0, 0};
-    if constexpr (KICK) {
-      if (act) {
-        double th = (double)V[idx];
-        if (Vb) th += (double)Vb[idx];
-        v[m] = cmul(v[m], expmi<T>(kick_coef * th));
+struct k_drift_c {
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(cpx<T>* __restrict__ psi, PassGeom g,
+                                             const cpx<T>* __restrict__ D,
+                                             const cpx<T>* __restrict__ tw, const T* __restrict__ V,
+                                             const T* __restrict__ Vb, double kick_coef,
+                                             int conj_d) {
+    using S = PencilShape<N>;
+    __shared__ cpx<T> lds[S::kLds];
+    Lane<N, true> ln(threadIdx.x);
+    const int q = blockIdx.x * S::C + ln.col;
+    const bool act = q < g.ncols;
+    const int base = act ? (int)pencil_base(g, q) : 0;
+    const int se = (int)g.s_elem;
+    cpx<T> v[S::E];
+  #pragma unroll
+    for (int m = 0; m < S::E; ++m) {
+      const int idx = base + (ln.j + S::TT * m) * se;
+      v[m] = act ? psi[idx] : cpx<T>{0, 0};
+      if constexpr (KICK) {
+        if (act) {
+          double th = (double)V[idx];
+          if (Vb) th += (double)Vb[idx];
+          v[m] = cmul(v[m], expmi<T>(kick_coef * th));
+        }
       }
     }
+    cpx<T>* my = lds + ln.col * S::E * S::P;
+    pencil_fft<N, -1, T>(v, ln.j, my, tw);
+  #pragma unroll
+    for (int m = 0; m < S::E; ++m) {
+      const cpx<T> d0 = D[ln.j + S::TT * m];
+      v[m] = cmul(v[m], conj_d ? conjg(d0) : d0);
+    }
+    pencil_fft<N, +1, T>(v, ln.j, my, tw);
+    if (act) {
+  #pragma unroll
+      for (int m = 0; m < S::E; ++m) psi[base + (ln.j + S::TT * m) * se] = v[m];
+    }
   }
-  cpx<T>* my = lds + ln.col * S::E * S::P;
-  pencil_fft<N, -1, T>(v, ln.j, my, tw);
-#pragma unroll
-  for (int m = 0; m < S::E; ++m) {
-    const cpx<T> d = D[ln.j + S::TT * m];
-    v[m] = conj_d ? cmulc(v[m], d) : cmul(v[m], d);
-  }
-  pencil_fft<N, +1, T>(v, ln.j, my, tw);
-  if (act) {
-#pragma unroll
-    for (int m = 0; m < S::E; ++m) psi[base + (ln.j + S::TT * m) * se] = v[m];
-  }
-}
+};
 
 // Common set-up of a strided sweep: VW neighbouring pencils per lane,
 // pencil slot fastest across lanes.
@@ -161,22 +165,26 @@ __device__ __forceinline__ void store_rows(const cpx<T> (&v)[VW][E], cpx<T>* __r
 // drift sweep along a strided axis (x or y)
 // ---------------------------------------------------------------------------
 template <class T, int N>
-__global__ __launch_bounds__(256) void k_drift_s(cpx<T>* __restrict__ psi, PassGeom g,
-                                                 const cpx<T>* __restrict__ D,
-                                                 const cpx<T>* __restrict__ tw, int conj_d) {
-  QJ_STRIDED_SETUP(T, N)
-  cpx<T> v[VW][E];
-  load_rows<VW, E>(v, psi, base, se, j, TT, act);
-  pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
-#pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const cpx<T> d = D[j + TT * m];
-#pragma unroll
-    for (int w = 0; w < VW; ++w) v[w][m] = conj_d ? cmulc(v[w][m], d) : cmul(v[w][m], d);
+struct k_drift_s {
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(cpx<T>* __restrict__ psi, PassGeom g,
+                                             const cpx<T>* __restrict__ D,
+                                             const cpx<T>* __restrict__ tw, int conj_d) {
+    QJ_STRIDED_SETUP(T, N)
+    cpx<T> v[VW][E];
+    load_rows<VW, E>(v, psi, base, se, j, TT, act);
+    pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
+  #pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const cpx<T> d0 = D[j + TT * m];
+      const cpx<T> d = conj_d ? conjg(d0) : d0;
+  #pragma unroll
+      for (int w = 0; w < VW; ++w) v[w][m] = cmul(v[w][m], d);
+    }
+    pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
+    store_rows<VW, E>(v, psi, base, se, j, TT, act);
   }
-  pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
-  store_rows<VW, E>(v, psi, base, se, j, TT, act);
-}
+};
 
 // ---------------------------------------------------------------------------
 // block-level FP64 sum with a deterministic last-block finish
@@ -240,97 +248,107 @@ __device__ void block_sum_finish(double local, bool bad, double* partials, unsig
 //   store kx = 0..nx/2 into H.
 // ---------------------------------------------------------------------------
 template <class T, int N, bool DRIFT>
-__global__ __launch_bounds__(256) void k_source_x(
-    cpx<T>* __restrict__ psi, T* __restrict__ src, int clear_src, T* __restrict__ brho,
-    double b_coeff, double rho_bar, PassGeom g, const cpx<T>* __restrict__ D,
-    const cpx<T>* __restrict__ tw, cpx<T>* __restrict__ H, double* __restrict__ partials,
-    unsigned int* __restrict__ counter, double* __restrict__ diag) {
-  QJ_STRIDED_SETUP(T, N)
-  cpx<T> v[VW][E];
-  T rho[VW][E];
-#pragma unroll
-  for (int m = 0; m < E; ++m)
-#pragma unroll
-    for (int w = 0; w < VW; ++w) rho[w][m] = (T)0;
-  if (psi) {
-    load_rows<VW, E>(v, psi, base, se, j, TT, act);
-    if constexpr (DRIFT) {
-      pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
-#pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const cpx<T> d = D[j + TT * m];
-#pragma unroll
-        for (int w = 0; w < VW; ++w) v[w][m] = cmul(v[w][m], d);
-      }
-      pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
-      store_rows<VW, E>(v, psi, base, se, j, TT, act);
-    }
-#pragma unroll
+struct k_source_x {
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(cpx<T>* __restrict__ psi, T* __restrict__ src,
+                                             int clear_src, T* __restrict__ brho, double b_coeff,
+                                             double rho_bar, PassGeom g,
+                                             const cpx<T>* __restrict__ D,
+                                             const cpx<T>* __restrict__ tw, cpx<T>* __restrict__ H,
+                                             double* __restrict__ partials,
+                                             unsigned int* __restrict__ counter,
+                                             double* __restrict__ diag) {
+    QJ_STRIDED_SETUP(T, N)
+    cpx<T> v[VW][E];
+    T rho[VW][E];
+  #pragma unroll
     for (int m = 0; m < E; ++m)
-#pragma unroll
-      for (int w = 0; w < VW; ++w) rho[w][m] = v[w][m].r * v[w][m].r + v[w][m].i * v[w][m].i;
-  }
-  double local = 0.0;
-  bool bad = false;
-#pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const int idx = base + (j + TT * m) * se;
-    if (act) {
-      if (src) {
-        T t[VW];
-        ldr<VW>(src, idx, t);
-#pragma unroll
-        for (int w = 0; w < VW; ++w) rho[w][m] += t[w];
-        if (clear_src) {
-          T z[VW];
-#pragma unroll
-          for (int w = 0; w < VW; ++w) z[w] = (T)0;
-          str<VW>(src, idx, z);
+  #pragma unroll
+      for (int w = 0; w < VW; ++w) rho[w][m] = (T)0;
+    if (psi) {
+      load_rows<VW, E>(v, psi, base, se, j, TT, act);
+      if constexpr (DRIFT) {
+        pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
+  #pragma unroll
+        for (int m = 0; m < E; ++m) {
+          const cpx<T> d = D[j + TT * m];
+  #pragma unroll
+          for (int w = 0; w < VW; ++w) v[w][m] = cmul(v[w][m], d);
+        }
+        pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
+        store_rows<VW, E>(v, psi, base, se, j, TT, act);
+      }
+  #pragma unroll
+      for (int m = 0; m < E; ++m)
+  #pragma unroll
+        for (int w = 0; w < VW; ++w)
+          rho[w][m] = fmad(v[w][m].r, v[w][m].r, v[w][m].i * v[w][m].i);
+    }
+    double local = 0.0;
+    bool bad = false;
+  #pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int idx = base + (j + TT * m) * se;
+      if (act) {
+        if (src) {
+          T t[VW];
+          ldr<VW>(src, idx, t);
+  #pragma unroll
+          for (int w = 0; w < VW; ++w) rho[w][m] += t[w];
+          if (clear_src) {
+            T z[VW];
+  #pragma unroll
+            for (int w = 0; w < VW; ++w) z[w] = (T)0;
+            str<VW>(src, idx, z);
+          }
+        }
+        if (brho) {
+          T t[VW];
+  #pragma unroll
+          for (int w = 0; w < VW; ++w) t[w] = (T)(b_coeff * (double)rho[w][m]);
+          str<VW>(brho, idx, t);
+        }
+  #pragma unroll
+        for (int w = 0; w < VW; ++w) {
+          local += (double)rho[w][m];
+          bad |= !isfinite(rho[w][m]);
         }
       }
-      if (brho) {
-        T t[VW];
-#pragma unroll
-        for (int w = 0; w < VW; ++w) t[w] = (T)(b_coeff * (double)rho[w][m]);
-        str<VW>(brho, idx, t);
-      }
-#pragma unroll
-      for (int w = 0; w < VW; ++w) {
-        local += (double)rho[w][m];
-        bad |= !isfinite(rho[w][m]);
+  #pragma unroll
+      for (int w = 0; w < VW; ++w) v[w][m] = {act ? (T)((double)rho[w][m] - rho_bar) : (T)0, (T)0};
+    }
+    pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
+    if (act) {
+  #pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int k = j + TT * m;
+        if (k <= N / 2) {
+          cpx<T> t[VW];
+  #pragma unroll
+          for (int w = 0; w < VW; ++w) t[w] = v[w][m];
+          stc<VW>(H, base + k * se, t);
+        }
       }
     }
-#pragma unroll
-    for (int w = 0; w < VW; ++w) v[w][m] = {act ? (T)((double)rho[w][m] - rho_bar) : (T)0, (T)0};
+    block_sum_finish(local, bad, partials, counter, diag);
   }
-  pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
-  if (act) {
-#pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const int k = j + TT * m;
-      if (k <= N / 2) {
-        cpx<T> t[VW];
-#pragma unroll
-        for (int w = 0; w < VW; ++w) t[w] = v[w][m];
-        stc<VW>(H, base + k * se, t);
-      }
-    }
-  }
-  block_sum_finish(local, bad, partials, counter, diag);
-}
+};
 
 // ---------------------------------------------------------------------------
 // generic in-place complex FFT sweep along a strided axis
 // ---------------------------------------------------------------------------
 template <class T, int N, int DIR>
-__global__ __launch_bounds__(256) void k_fft_s(cpx<T>* __restrict__ a, PassGeom g,
-                                               const cpx<T>* __restrict__ tw) {
-  QJ_STRIDED_SETUP(T, N)
-  cpx<T> v[VW][E];
-  load_rows<VW, E>(v, a, base, se, j, TT, act);
-  pencil_fft_strided<N, DIR, T, VW>(v, j, col, lds, tw);
-  store_rows<VW, E>(v, a, base, se, j, TT, act);
-}
+struct k_fft_s {
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(cpx<T>* __restrict__ a, PassGeom g,
+                                             const cpx<T>* __restrict__ tw) {
+    QJ_STRIDED_SETUP(T, N)
+    cpx<T> v[VW][E];
+    load_rows<VW, E>(v, a, base, se, j, TT, act);
+    pencil_fft_strided<N, DIR, T, VW>(v, j, col, lds, tw);
+    store_rows<VW, E>(v, a, base, se, j, TT, act);
+  }
+};
 
 // ---------------------------------------------------------------------------
 // Poisson z pass on the half spectrum: FFT_z, V_hat = -4piG rho_hat/k^2
@@ -338,91 +356,96 @@ __global__ __launch_bounds__(256) void k_fft_s(cpx<T>* __restrict__ a, PassGeom 
 // IFFT_z of V_hat and, with GRAD, of i kz V_hat (Nyquist derivative 0).
 // ---------------------------------------------------------------------------
 template <class T, int N, bool GRAD>
-__global__ __launch_bounds__(256) void k_poisson_z(cpx<T>* __restrict__ H, cpx<T>* __restrict__ Uz,
-                                                   PassGeom g, const cpx<T>* __restrict__ tw,
-                                                   int ny_log2, int nx, int ny, double kfx,
-                                                   double kfy, double kfz, double four_pi_G,
-                                                   double rho_bar, double ntot, int dc_mode,
-                                                   const double* __restrict__ diag) {
-  using S = PencilShape<N>;
-  __shared__ cpx<T> lds[S::kLds];
-  Lane<N, true> ln(threadIdx.x);
-  const int q = blockIdx.x * S::C + ln.col;
-  const bool act = q < g.ncols;
-  const int base = act ? (int)pencil_base(g, q) : 0;
-  cpx<T>* my = lds + ln.col * S::E * S::P;
-  cpx<T> v[S::E];
-#pragma unroll
-  for (int m = 0; m < S::E; ++m) v[m] = act ? H[base + ln.j + S::TT * m] : cpx<T>{0, 0};
-  pencil_fft<N, -1, T>(v, ln.j, my, tw);
-  const int x = q >> ny_log2, y = q & (ny - 1);
-  const double kx = kfx * (double)freq_index(x, nx);
-  const double ky = kfy * (double)freq_index(y, ny);
-  const double kxy2 = kx * kx + ky * ky;
-  const double inv_ntot = 1.0 / ntot;
-  cpx<T> w[GRAD ? S::E : 1];
-#pragma unroll
-  for (int m = 0; m < S::E; ++m) {
-    const int kzi = ln.j + S::TT * m;
-    const double kz = kfz * (double)freq_index(kzi, N);
-    const double k2 = kxy2 + kz * kz;
-    const T Gk = (T)(-four_pi_G * inv_ntot / (k2 == 0.0 ? 1.0 : k2));
-    v[m] = scale(v[m], Gk);
-    if (x == 0 && y == 0 && kzi == 0) {
-      double dc = (dc_mode == 0) ? -four_pi_G * (diag[kDiagMass] - ntot * rho_bar) * inv_ntot : 0.0;
-      v[m] = {(T)dc, (T)0};
+struct k_poisson_z {
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(cpx<T>* __restrict__ H, cpx<T>* __restrict__ Uz,
+                                             PassGeom g, const cpx<T>* __restrict__ tw, int ny_log2,
+                                             int nx, int ny, double kfx, double kfy, double kfz,
+                                             double four_pi_G, double rho_bar, double ntot,
+                                             int dc_mode, const double* __restrict__ diag) {
+    using S = PencilShape<N>;
+    __shared__ cpx<T> lds[S::kLds];
+    Lane<N, true> ln(threadIdx.x);
+    const int q = blockIdx.x * S::C + ln.col;
+    const bool act = q < g.ncols;
+    const int base = act ? (int)pencil_base(g, q) : 0;
+    cpx<T>* my = lds + ln.col * S::E * S::P;
+    cpx<T> v[S::E];
+  #pragma unroll
+    for (int m = 0; m < S::E; ++m) v[m] = act ? H[base + ln.j + S::TT * m] : cpx<T>{0, 0};
+    pencil_fft<N, -1, T>(v, ln.j, my, tw);
+    const int x = q >> ny_log2, y = q & (ny - 1);
+    const double kx = kfx * (double)freq_index(x, nx);
+    const double ky = kfy * (double)freq_index(y, ny);
+    const double kxy2 = kx * kx + ky * ky;
+    const double inv_ntot = 1.0 / ntot;
+    cpx<T> w[GRAD ? S::E : 1];
+  #pragma unroll
+    for (int m = 0; m < S::E; ++m) {
+      const int kzi = ln.j + S::TT * m;
+      const double kz = kfz * (double)freq_index(kzi, N);
+      const double k2 = kxy2 + kz * kz;
+      const T Gk = (T)(-four_pi_G * inv_ntot / (k2 == 0.0 ? 1.0 : k2));
+      v[m] = scale(v[m], Gk);
+      if (x == 0 && y == 0 && kzi == 0) {
+        double dc = (dc_mode == 0) ? -four_pi_G * (diag[kDiagMass] - ntot * rho_bar) * inv_ntot : 0.0;
+        v[m] = {(T)dc, (T)0};
+      }
+      if constexpr (GRAD) {
+        const T kzt = (kzi == N / 2) ? (T)0 : (T)kz;
+        w[m] = {-kzt * v[m].i, kzt * v[m].r};
+      }
+    }
+    pencil_fft<N, +1, T>(v, ln.j, my, tw);
+    if (act) {
+  #pragma unroll
+      for (int m = 0; m < S::E; ++m) H[base + ln.j + S::TT * m] = v[m];
     }
     if constexpr (GRAD) {
-      const T kzt = (kzi == N / 2) ? (T)0 : (T)kz;
-      w[m] = {-kzt * v[m].i, kzt * v[m].r};
+      pencil_fft<N, +1, T>(w, ln.j, my, tw);
+      if (act) {
+  #pragma unroll
+        for (int m = 0; m < S::E; ++m) Uz[base + ln.j + S::TT * m] = w[m];
+      }
     }
   }
-  pencil_fft<N, +1, T>(v, ln.j, my, tw);
-  if (act) {
-#pragma unroll
-    for (int m = 0; m < S::E; ++m) H[base + ln.j + S::TT * m] = v[m];
-  }
-  if constexpr (GRAD) {
-    pencil_fft<N, +1, T>(w, ln.j, my, tw);
-    if (act) {
-#pragma unroll
-      for (int m = 0; m < S::E; ++m) Uz[base + ln.j + S::TT * m] = w[m];
-    }
-  }
-}
+};
 
 // ---------------------------------------------------------------------------
 // Poisson y pass (inverse): U -> IFFT_y U (in place); with GRAD also
 // IFFT_y(i ky U) -> Uy and IFFT_y Uz (in place).
 // ---------------------------------------------------------------------------
 template <class T, int N, bool GRAD>
-__global__ __launch_bounds__(256) void k_poisson_y(cpx<T>* __restrict__ H, cpx<T>* __restrict__ Uz,
-                                                   cpx<T>* __restrict__ Uy, PassGeom g,
-                                                   const cpx<T>* __restrict__ tw, double kfy) {
-  QJ_STRIDED_SETUP(T, N)
-  cpx<T> v[VW][E];
-  load_rows<VW, E>(v, H, base, se, j, TT, act);
-  if constexpr (GRAD) {
-    cpx<T> w[VW][E];
-#pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const int k = j + TT * m;
-      const T ky = (k == N / 2) ? (T)0 : (T)(kfy * (double)freq_index(k, N));
-#pragma unroll
-      for (int x = 0; x < VW; ++x) w[x][m] = {-ky * v[x][m].i, ky * v[x][m].r};
+struct k_poisson_y {
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(cpx<T>* __restrict__ H, cpx<T>* __restrict__ Uz,
+                                             cpx<T>* __restrict__ Uy, PassGeom g,
+                                             const cpx<T>* __restrict__ tw, double kfy) {
+    QJ_STRIDED_SETUP(T, N)
+    cpx<T> v[VW][E];
+    load_rows<VW, E>(v, H, base, se, j, TT, act);
+    if constexpr (GRAD) {
+      cpx<T> w[VW][E];
+  #pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int k = j + TT * m;
+        const T ky = (k == N / 2) ? (T)0 : (T)(kfy * (double)freq_index(k, N));
+  #pragma unroll
+        for (int x = 0; x < VW; ++x) w[x][m] = {-ky * v[x][m].i, ky * v[x][m].r};
+      }
+      pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
+      store_rows<VW, E>(v, H, base, se, j, TT, act);
+      pencil_fft_strided<N, +1, T, VW>(w, j, col, lds, tw);
+      store_rows<VW, E>(w, Uy, base, se, j, TT, act);
+      load_rows<VW, E>(v, Uz, base, se, j, TT, act);
+      pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
+      store_rows<VW, E>(v, Uz, base, se, j, TT, act);
+    } else {
+      pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
+      store_rows<VW, E>(v, H, base, se, j, TT, act);
     }
-    pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
-    store_rows<VW, E>(v, H, base, se, j, TT, act);
-    pencil_fft_strided<N, +1, T, VW>(w, j, col, lds, tw);
-    store_rows<VW, E>(w, Uy, base, se, j, TT, act);
-    load_rows<VW, E>(v, Uz, base, se, j, TT, act);
-    pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
-    store_rows<VW, E>(v, Uz, base, se, j, TT, act);
-  } else {
-    pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
-    store_rows<VW, E>(v, H, base, se, j, TT, act);
   }
-}
+};
 
 // Rows of the full x spectrum from the half spectrum (Hermitian along x).
 template <int N, int VW, int E, class T>
@@ -454,88 +477,91 @@ __device__ __forceinline__ void load_herm_rows(cpx<T> (&v)[VW][E], const cpx<T>*
 // quantum-stars.py:192 and :256-260.
 // ---------------------------------------------------------------------------
 template <class T, int N, bool GRAD>
-__global__ __launch_bounds__(256) void k_poisson_x(const cpx<T>* __restrict__ H,
-                                                   const cpx<T>* __restrict__ Uy,
-                                                   const cpx<T>* __restrict__ Uzz, PassGeom g,
-                                                   const cpx<T>* __restrict__ tw, double kfx,
-                                                   T* __restrict__ V, T* __restrict__ A4,
-                                                   const T* __restrict__ addend,
-                                                   double* __restrict__ diag) {
-  QJ_STRIDED_SETUP(T, N)
-  cpx<T> v[VW][E];
-  load_herm_rows<N, VW, E>(v, H, base, se, j, TT, act);
-  if constexpr (!GRAD) {
-    pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
-    if (act) {
-#pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const int idx = base + (j + TT * m) * se;
-        T t[VW];
-        if (addend) ldr<VW>(addend, idx, t);
-#pragma unroll
-        for (int w = 0; w < VW; ++w) t[w] = addend ? v[w][m].r + t[w] : v[w][m].r;
-        str<VW>(V, idx, t);
-      }
-    }
-  } else {
-    cpx<T> u[VW][E];
-#pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const int k = j + TT * m;
-      const T kx = (k == N / 2) ? (T)0 : (T)(kfx * (double)freq_index(k, N));
-#pragma unroll
-      for (int w = 0; w < VW; ++w) u[w][m] = {-kx * v[w][m].i, kx * v[w][m].r};
-    }
-    pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
-    T vr[VW][E];
-#pragma unroll
-    for (int m = 0; m < E; ++m) {
-      T t[VW];
-#pragma unroll
-      for (int w = 0; w < VW; ++w) t[w] = vr[w][m] = v[w][m].r;
-      if (act) str<VW>(V, base + (j + TT * m) * se, t);
-    }
-    load_herm_rows<N, VW, E>(v, Uy, base, se, j, TT, act);
-#pragma unroll
-    for (int m = 0; m < E; ++m)
-#pragma unroll
-      for (int w = 0; w < VW; ++w) u[w][m] = {u[w][m].r - v[w][m].i, u[w][m].i + v[w][m].r};
-    pencil_fft_strided<N, +1, T, VW>(u, j, col, lds, tw);
-    load_herm_rows<N, VW, E>(v, Uzz, base, se, j, TT, act);
-    pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
-    T amax = (T)0;
-    if (act) {
-#pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const int idx = base + (j + TT * m) * se;
-#pragma unroll
-        for (int w = 0; w < VW; ++w) {
-          const T a0 = -u[w][m].r, a1 = -u[w][m].i, a2 = -v[w][m].r;
-          if constexpr (sizeof(T) == 4) {
-            *reinterpret_cast<float4*>(A4 + 4 * (idx + w)) = make_float4(a0, a1, a2, vr[w][m]);
-          } else {
-            A4[4 * (idx + w) + 0] = a0;
-            A4[4 * (idx + w) + 1] = a1;
-            A4[4 * (idx + w) + 2] = a2;
-            A4[4 * (idx + w) + 3] = vr[w][m];
-          }
-          const T m0 = fabs(a0), m1 = fabs(a1), m2 = fabs(a2);
-          amax = (m0 > amax || m0 != m0) ? m0 : amax;
-          amax = (m1 > amax || m1 != m1) ? m1 : amax;
-          amax = (m2 > amax || m2 != m2) ? m2 : amax;
+struct k_poisson_x {
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const cpx<T>* __restrict__ H,
+                                             const cpx<T>* __restrict__ Uy,
+                                             const cpx<T>* __restrict__ Uzz, PassGeom g,
+                                             const cpx<T>* __restrict__ tw, double kfx,
+                                             T* __restrict__ V, T* __restrict__ A4,
+                                             const T* __restrict__ addend,
+                                             double* __restrict__ diag) {
+    QJ_STRIDED_SETUP(T, N)
+    cpx<T> v[VW][E];
+    load_herm_rows<N, VW, E>(v, H, base, se, j, TT, act);
+    if constexpr (!GRAD) {
+      pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
+      if (act) {
+  #pragma unroll
+        for (int m = 0; m < E; ++m) {
+          const int idx = base + (j + TT * m) * se;
+          T t[VW];
+          if (addend) ldr<VW>(addend, idx, t);
+  #pragma unroll
+          for (int w = 0; w < VW; ++w) t[w] = addend ? v[w][m].r + t[w] : v[w][m].r;
+          str<VW>(V, idx, t);
         }
       }
+    } else {
+      cpx<T> u[VW][E];
+  #pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int k = j + TT * m;
+        const T kx = (k == N / 2) ? (T)0 : (T)(kfx * (double)freq_index(k, N));
+  #pragma unroll
+        for (int w = 0; w < VW; ++w) u[w][m] = {-kx * v[w][m].i, kx * v[w][m].r};
+      }
+      pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
+      T vr[VW][E];
+  #pragma unroll
+      for (int m = 0; m < E; ++m) {
+        T t[VW];
+  #pragma unroll
+        for (int w = 0; w < VW; ++w) t[w] = vr[w][m] = v[w][m].r;
+        if (act) str<VW>(V, base + (j + TT * m) * se, t);
+      }
+      load_herm_rows<N, VW, E>(v, Uy, base, se, j, TT, act);
+  #pragma unroll
+      for (int m = 0; m < E; ++m)
+  #pragma unroll
+        for (int w = 0; w < VW; ++w) u[w][m] = {u[w][m].r - v[w][m].i, u[w][m].i + v[w][m].r};
+      pencil_fft_strided<N, +1, T, VW>(u, j, col, lds, tw);
+      load_herm_rows<N, VW, E>(v, Uzz, base, se, j, TT, act);
+      pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
+      T amax = (T)0;
+      if (act) {
+  #pragma unroll
+        for (int m = 0; m < E; ++m) {
+          const int idx = base + (j + TT * m) * se;
+  #pragma unroll
+          for (int w = 0; w < VW; ++w) {
+            const T a0 = -u[w][m].r, a1 = -u[w][m].i, a2 = -v[w][m].r;
+            if constexpr (sizeof(T) == 4) {
+              *reinterpret_cast<float4*>(A4 + 4 * (idx + w)) = make_float4(a0, a1, a2, vr[w][m]);
+            } else {
+              A4[4 * (idx + w) + 0] = a0;
+              A4[4 * (idx + w) + 1] = a1;
+              A4[4 * (idx + w) + 2] = a2;
+              A4[4 * (idx + w) + 3] = vr[w][m];
+            }
+            const T m0 = fabs(a0), m1 = fabs(a1), m2 = fabs(a2);
+            amax = (m0 > amax || m0 != m0) ? m0 : amax;
+            amax = (m1 > amax || m1 != m1) ? m1 : amax;
+            amax = (m2 > amax || m2 != m2) ? m2 : amax;
+          }
+        }
+      }
+      // wave max (NaN-propagating), one atomic per wave
+      double am = (double)amax;
+  #pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        double other = __shfl_xor(am, o, 64);
+        am = (other > am || other != other) ? other : am;
+      }
+      if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&diag[kDiagAMax], am);
     }
-    // wave max (NaN-propagating), one atomic per wave
-    double am = (double)amax;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      double other = __shfl_xor(am, o, 64);
-      am = (other > am || other != other) ? other : am;
-    }
-    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&diag[kDiagAMax], am);
   }
-}
+};
 
 // ===========================================================================
 // adjoint passes
@@ -545,164 +571,173 @@ __global__ __launch_bounds__(256) void k_poisson_x(const cpx<T>* __restrict__ H,
 // (Adjoint of psi_out = exp(-i coef V) psi_in: the phase factor is unitary
 // and dL/dV = coef Im(conj(psi_bar_out) psi_out) = coef Im(conj(psi_bar_in) psi_in).)
 template <class T>
-__global__ __launch_bounds__(256) void k_kick_adj(cpx<T>* __restrict__ pb,
-                                                  const cpx<T>* __restrict__ pr,
-                                                  const T* __restrict__ V, double coef,
-                                                  T* __restrict__ Vbar, long long n) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long stride = (long long)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) {
-    const cpx<T> e = expmi<T>(coef * (double)V[i]);  // exp(-i coef V)
-    const cpx<T> b = cmulc(pb[i], e);                 // * exp(+i coef V)
-    pb[i] = b;
-    const cpx<T> r = pr[i];
-    Vbar[i] = (T)(coef * ((double)b.r * r.i - (double)b.i * r.r));
+struct k_kick_adj {
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(cpx<T>* __restrict__ pb, const cpx<T>* __restrict__ pr,
+                                             const T* __restrict__ V, double coef,
+                                             T* __restrict__ Vbar, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+      const cpx<T> e = expmi<T>(coef * (double)V[i]);  // exp(-i coef V)
+      const cpx<T> b = cmulc(pb[i], e);                 // * exp(+i coef V)
+      pb[i] = b;
+      const cpx<T> r = pr[i];
+      Vbar[i] = (T)(coef * ((double)b.r * r.i - (double)b.i * r.r));
+    }
   }
-}
+};
 
 // psi_bar += 2 rhobar psi_ref  (adjoint of rho = |psi|^2)
 template <class T>
-__global__ __launch_bounds__(256) void k_rho_adj(cpx<T>* __restrict__ pb,
-                                                 const cpx<T>* __restrict__ pr,
-                                                 const T* __restrict__ rb, long long n) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long stride = (long long)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) {
-    const T f = (T)2 * rb[i];
-    const cpx<T> r = pr[i];
-    cpx<T> b = pb[i];
-    b.r += f * r.r;
-    b.i += f * r.i;
-    pb[i] = b;
+struct k_rho_adj {
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(cpx<T>* __restrict__ pb, const cpx<T>* __restrict__ pr,
+                                             const T* __restrict__ rb, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+      const T f = (T)2 * rb[i];
+      const cpx<T> r = pr[i];
+      cpx<T> b = pb[i];
+      b.r += f * r.r;
+      b.i += f * r.i;
+      pb[i] = b;
+    }
   }
-}
+};
 
 // Adjoint Poisson source along x: half spectra of
 //   T1 = FFT_x(Vbar) + i kx FFT_x(abar_x),  T2 = FFT_x(abar_y),  T3 = FFT_x(abar_z).
 template <class T, int N>
-__global__ __launch_bounds__(256) void k_adj_src_x(const T* __restrict__ Vbar,
-                                                   const T* __restrict__ abar4, PassGeom g,
-                                                   const cpx<T>* __restrict__ tw, double kfx,
-                                                   cpx<T>* __restrict__ T1,
-                                                   cpx<T>* __restrict__ T2,
-                                                   cpx<T>* __restrict__ T3) {
-  QJ_STRIDED_SETUP(T, N)
-  cpx<T> v[VW][E], u[VW][E];
-  auto load_real = [&](cpx<T>(&o)[VW][E], int comp) {
-#pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const int idx = base + (j + TT * m) * se;
-#pragma unroll
-      for (int w = 0; w < VW; ++w) {
-        T x = (T)0;
-        if (act) x = comp < 0 ? Vbar[idx + w] : abar4[4 * (idx + w) + comp];
-        o[w][m] = {x, (T)0};
+struct k_adj_src_x {
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(const T* __restrict__ Vbar,
+                                             const T* __restrict__ abar4, PassGeom g,
+                                             const cpx<T>* __restrict__ tw, double kfx,
+                                             cpx<T>* __restrict__ T1, cpx<T>* __restrict__ T2,
+                                             cpx<T>* __restrict__ T3) {
+    QJ_STRIDED_SETUP(T, N)
+    cpx<T> v[VW][E], u[VW][E];
+    auto load_real = [&](cpx<T>(&o)[VW][E], int comp) {
+  #pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int idx = base + (j + TT * m) * se;
+  #pragma unroll
+        for (int w = 0; w < VW; ++w) {
+          T x = (T)0;
+          if (act) x = comp < 0 ? Vbar[idx + w] : abar4[4 * (idx + w) + comp];
+          o[w][m] = {x, (T)0};
+        }
       }
-    }
-  };
-  auto store_half = [&](const cpx<T>(&o)[VW][E], cpx<T>* dst) {
-    if (!act) return;
-#pragma unroll
+    };
+    auto store_half = [&](const cpx<T>(&o)[VW][E], cpx<T>* dst) {
+      if (!act) return;
+  #pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int k = j + TT * m;
+        if (k <= N / 2) {
+          cpx<T> t[VW];
+  #pragma unroll
+          for (int w = 0; w < VW; ++w) t[w] = o[w][m];
+          stc<VW>(dst, base + k * se, t);
+        }
+      }
+    };
+    load_real(v, -1);
+    pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
+    load_real(u, 0);
+    pencil_fft_strided<N, -1, T, VW>(u, j, col, lds, tw);
+  #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int k = j + TT * m;
-      if (k <= N / 2) {
-        cpx<T> t[VW];
-#pragma unroll
-        for (int w = 0; w < VW; ++w) t[w] = o[w][m];
-        stc<VW>(dst, base + k * se, t);
-      }
+      const T kx = (k == N / 2) ? (T)0 : (T)(kfx * (double)freq_index(k, N));
+  #pragma unroll
+      for (int w = 0; w < VW; ++w) v[w][m] = {v[w][m].r - kx * u[w][m].i, v[w][m].i + kx * u[w][m].r};
     }
-  };
-  load_real(v, -1);
-  pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
-  load_real(u, 0);
-  pencil_fft_strided<N, -1, T, VW>(u, j, col, lds, tw);
-#pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const int k = j + TT * m;
-    const T kx = (k == N / 2) ? (T)0 : (T)(kfx * (double)freq_index(k, N));
-#pragma unroll
-    for (int w = 0; w < VW; ++w) v[w][m] = {v[w][m].r - kx * u[w][m].i, v[w][m].i + kx * u[w][m].r};
+    store_half(v, T1);
+    load_real(v, 1);
+    pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
+    store_half(v, T2);
+    load_real(v, 2);
+    pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
+    store_half(v, T3);
   }
-  store_half(v, T1);
-  load_real(v, 1);
-  pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
-  store_half(v, T2);
-  load_real(v, 2);
-  pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
-  store_half(v, T3);
-}
+};
 
 // Adjoint Poisson y pass (forward FFT): T1 <- FFT_y(T1) + i ky FFT_y(T2),
 // T3 <- FFT_y(T3).
 template <class T, int N>
-__global__ __launch_bounds__(256) void k_adj_y(cpx<T>* __restrict__ T1,
-                                               const cpx<T>* __restrict__ T2,
-                                               cpx<T>* __restrict__ T3, PassGeom g,
-                                               const cpx<T>* __restrict__ tw, double kfy) {
-  QJ_STRIDED_SETUP(T, N)
-  cpx<T> v[VW][E], u[VW][E];
-  load_rows<VW, E>(v, T1, base, se, j, TT, act);
-  pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
-  load_rows<VW, E>(u, T2, base, se, j, TT, act);
-  pencil_fft_strided<N, -1, T, VW>(u, j, col, lds, tw);
-#pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const int k = j + TT * m;
-    const T ky = (k == N / 2) ? (T)0 : (T)(kfy * (double)freq_index(k, N));
-#pragma unroll
-    for (int w = 0; w < VW; ++w) v[w][m] = {v[w][m].r - ky * u[w][m].i, v[w][m].i + ky * u[w][m].r};
+struct k_adj_y {
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(cpx<T>* __restrict__ T1, const cpx<T>* __restrict__ T2,
+                                             cpx<T>* __restrict__ T3, PassGeom g,
+                                             const cpx<T>* __restrict__ tw, double kfy) {
+    QJ_STRIDED_SETUP(T, N)
+    cpx<T> v[VW][E], u[VW][E];
+    load_rows<VW, E>(v, T1, base, se, j, TT, act);
+    pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
+    load_rows<VW, E>(u, T2, base, se, j, TT, act);
+    pencil_fft_strided<N, -1, T, VW>(u, j, col, lds, tw);
+  #pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int k = j + TT * m;
+      const T ky = (k == N / 2) ? (T)0 : (T)(kfy * (double)freq_index(k, N));
+  #pragma unroll
+      for (int w = 0; w < VW; ++w) v[w][m] = {v[w][m].r - ky * u[w][m].i, v[w][m].i + ky * u[w][m].r};
+    }
+    store_rows<VW, E>(v, T1, base, se, j, TT, act);
+    load_rows<VW, E>(v, T3, base, se, j, TT, act);
+    pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
+    store_rows<VW, E>(v, T3, base, se, j, TT, act);
   }
-  store_rows<VW, E>(v, T1, base, se, j, TT, act);
-  load_rows<VW, E>(v, T3, base, se, j, TT, act);
-  pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
-  store_rows<VW, E>(v, T3, base, se, j, TT, act);
-}
+};
 
 // Adjoint Poisson z pass: S = FFT_z(T1) + i kz FFT_z(T3); S *= G(k)/ntot;
 // T1 <- IFFT_z(S).
 template <class T, int N>
-__global__ __launch_bounds__(256) void k_adj_z(cpx<T>* __restrict__ T1,
-                                               const cpx<T>* __restrict__ T3, PassGeom g,
-                                               const cpx<T>* __restrict__ tw, int ny_log2,
-                                               int nx, int ny, double kfx, double kfy,
-                                               double kfz, double four_pi_G, double ntot,
-                                               int dc_mode) {
-  using S = PencilShape<N>;
-  __shared__ cpx<T> lds[S::kLds];
-  Lane<N, true> ln(threadIdx.x);
-  const int q = blockIdx.x * S::C + ln.col;
-  const bool act = q < g.ncols;
-  const int base = act ? (int)pencil_base(g, q) : 0;
-  cpx<T>* my = lds + ln.col * S::E * S::P;
-  cpx<T> v[S::E], u[S::E];
-#pragma unroll
-  for (int m = 0; m < S::E; ++m) v[m] = act ? T1[base + ln.j + S::TT * m] : cpx<T>{0, 0};
-  pencil_fft<N, -1, T>(v, ln.j, my, tw);
-#pragma unroll
-  for (int m = 0; m < S::E; ++m) u[m] = act ? T3[base + ln.j + S::TT * m] : cpx<T>{0, 0};
-  pencil_fft<N, -1, T>(u, ln.j, my, tw);
-  const int x = q >> ny_log2, y = q & (ny - 1);
-  const double kx = kfx * (double)freq_index(x, nx);
-  const double ky = kfy * (double)freq_index(y, ny);
-  const double kxy2 = kx * kx + ky * ky;
-#pragma unroll
-  for (int m = 0; m < S::E; ++m) {
-    const int kzi = ln.j + S::TT * m;
-    const double kz = kfz * (double)freq_index(kzi, N);
-    const T kzt = (kzi == N / 2) ? (T)0 : (T)kz;
-    cpx<T> sv = {v[m].r - kzt * u[m].i, v[m].i + kzt * u[m].r};
-    const double k2 = kxy2 + kz * kz;
-    double Gk = -four_pi_G / (k2 == 0.0 ? 1.0 : k2);
-    if (k2 == 0.0 && dc_mode != 0) Gk = 0.0;
-    v[m] = scale(sv, (T)(Gk / ntot));
+struct k_adj_z {
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(cpx<T>* __restrict__ T1, const cpx<T>* __restrict__ T3,
+                                             PassGeom g, const cpx<T>* __restrict__ tw, int ny_log2,
+                                             int nx, int ny, double kfx, double kfy, double kfz,
+                                             double four_pi_G, double ntot, int dc_mode) {
+    using S = PencilShape<N>;
+    __shared__ cpx<T> lds[S::kLds];
+    Lane<N, true> ln(threadIdx.x);
+    const int q = blockIdx.x * S::C + ln.col;
+    const bool act = q < g.ncols;
+    const int base = act ? (int)pencil_base(g, q) : 0;
+    cpx<T>* my = lds + ln.col * S::E * S::P;
+    cpx<T> v[S::E], u[S::E];
+  #pragma unroll
+    for (int m = 0; m < S::E; ++m) v[m] = act ? T1[base + ln.j + S::TT * m] : cpx<T>{0, 0};
+    pencil_fft<N, -1, T>(v, ln.j, my, tw);
+  #pragma unroll
+    for (int m = 0; m < S::E; ++m) u[m] = act ? T3[base + ln.j + S::TT * m] : cpx<T>{0, 0};
+    pencil_fft<N, -1, T>(u, ln.j, my, tw);
+    const int x = q >> ny_log2, y = q & (ny - 1);
+    const double kx = kfx * (double)freq_index(x, nx);
+    const double ky = kfy * (double)freq_index(y, ny);
+    const double kxy2 = kx * kx + ky * ky;
+  #pragma unroll
+    for (int m = 0; m < S::E; ++m) {
+      const int kzi = ln.j + S::TT * m;
+      const double kz = kfz * (double)freq_index(kzi, N);
+      const T kzt = (kzi == N / 2) ? (T)0 : (T)kz;
+      cpx<T> sv = {v[m].r - kzt * u[m].i, v[m].i + kzt * u[m].r};
+      const double k2 = kxy2 + kz * kz;
+      double Gk = -four_pi_G / (k2 == 0.0 ? 1.0 : k2);
+      if (k2 == 0.0 && dc_mode != 0) Gk = 0.0;
+      v[m] = scale(sv, (T)(Gk / ntot));
+    }
+    pencil_fft<N, +1, T>(v, ln.j, my, tw);
+    if (act) {
+  #pragma unroll
+      for (int m = 0; m < S::E; ++m) T1[base + ln.j + S::TT * m] = v[m];
+    }
   }
-  pencil_fft<N, +1, T>(v, ln.j, my, tw);
-  if (act) {
-#pragma unroll
-    for (int m = 0; m < S::E; ++m) T1[base + ln.j + S::TT * m] = v[m];
-  }
-}
+};
 
 // ---------------------------------------------------------------------------
 // launch helpers
@@ -759,7 +794,7 @@ hipError_t launch_kick(void* psi, const void* V, const void* Vb, double coef, lo
                        hipStream_t s) {
   long long blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  k_kick<T><<<(int)blocks, 256, 0, s>>>((cpx<T>*)psi, (const T*)V, (const T*)Vb, coef, n);
+  launch<k_kick<T>>((int)blocks, 256, s, (cpx<T>*)psi, (const T*)V, (const T*)Vb, coef, n);
   return hipGetLastError();
 }
 
@@ -776,10 +811,10 @@ hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, v
     const cpx<T>* tw = (const cpx<T>*)tb.tw_z;
     QJ_DISPATCH_N(d.nz, {
       if (kick)
-        k_drift_c<T, N_, true><<<nblocks<N_>(g.ncols), 256, 0, s>>>(p, g, D, tw, v, vb, kick_coef,
+        launch<k_drift_c<T, N_, true>>(nblocks<N_>(g.ncols), 256, s, p, g, D, tw, v, vb, kick_coef,
                                                                     conj_d);
       else
-        k_drift_c<T, N_, false><<<nblocks<N_>(g.ncols), 256, 0, s>>>(p, g, D, tw, v, vb, kick_coef,
+        launch<k_drift_c<T, N_, false>>(nblocks<N_>(g.ncols), 256, s, p, g, D, tw, v, vb, kick_coef,
                                                                      conj_d);
     });
   } else {
@@ -789,7 +824,7 @@ hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, v
     const cpx<T>* D = (const cpx<T>*)(y ? tb.drift_y : tb.drift_x);
     const cpx<T>* tw = (const cpx<T>*)(y ? tb.tw_y : tb.tw_x);
     QJ_DISPATCH_N(y ? d.ny : d.nx, {
-      k_drift_s<T, N_><<<nblocks_s<T, N_>(g.ncols), 256, 0, s>>>(p, g, D, tw, conj_d);
+      launch<k_drift_s<T, N_>>(nblocks_s<T, N_>(g.ncols), 256, s, p, g, D, tw, conj_d);
     });
   }
   return hipGetLastError();
@@ -810,11 +845,13 @@ int source_x_blocks_t(const Dims& d) {
   }
 }
 
+#if !defined(QJ_PART) || QJ_PART == 0
 int source_x_blocks(const Dims& d) {
   // partials are sized for the larger of the two precisions' grids
   const int a = source_x_blocks_t<float>(d), b = source_x_blocks_t<double>(d);
   return a > b ? a : b;
 }
+#endif
 
 template <class T>
 hipError_t launch_source_x(const Dims& d, const SpecTables<T>& tb, int drift, void* psi,
@@ -826,11 +863,11 @@ hipError_t launch_source_x(const Dims& d, const SpecTables<T>& tb, int drift, vo
   const cpx<T>* tw = (const cpx<T>*)tb.tw_x;
   QJ_DISPATCH_N(d.nx, {
     if (drift)
-      k_source_x<T, N_, true><<<nblocks_s<T, N_>(g.ncols), 256, 0, s>>>(
+      launch<k_source_x<T, N_, true>>(nblocks_s<T, N_>(g.ncols), 256, s,
           (cpx<T>*)psi, (T*)src, clear_src, (T*)brho, b_coeff, rho_bar, g, D, tw, (cpx<T>*)H,
           partials, counter, diag);
     else
-      k_source_x<T, N_, false><<<nblocks_s<T, N_>(g.ncols), 256, 0, s>>>(
+      launch<k_source_x<T, N_, false>>(nblocks_s<T, N_>(g.ncols), 256, s,
           (cpx<T>*)psi, (T*)src, clear_src, (T*)brho, b_coeff, rho_bar, g, D, tw, (cpx<T>*)H,
           partials, counter, diag);
   });
@@ -842,7 +879,7 @@ hipError_t launch_fft_y_half(const Dims& d, const SpecTables<T>& tb, void* H, hi
   PassGeom g = geom_y(d, d.nx / 2 + 1);
   const cpx<T>* tw = (const cpx<T>*)tb.tw_y;
   QJ_DISPATCH_N(d.ny, {
-    k_fft_s<T, N_, -1><<<nblocks_s<T, N_>(g.ncols), 256, 0, s>>>((cpx<T>*)H, g, tw);
+    launch<k_fft_s<T, N_, -1>>(nblocks_s<T, N_>(g.ncols), 256, s, (cpx<T>*)H, g, tw);
   });
   return hipGetLastError();
 }
@@ -858,11 +895,11 @@ hipError_t launch_poisson_z(const Dims& d, const SpecTables<T>& tb, int grad, vo
   const int nyl = ilog2(d.ny);
   QJ_DISPATCH_N(d.nz, {
     if (grad)
-      k_poisson_z<T, N_, true><<<nblocks<N_>(g.ncols), 256, 0, s>>>(
+      launch<k_poisson_z<T, N_, true>>(nblocks<N_>(g.ncols), 256, s,
           (cpx<T>*)H, (cpx<T>*)Uz, g, tw, nyl, d.nx, d.ny, kfx, kfy, kfz, four_pi_G, rho_bar,
           ntot, dc_mode, diag);
     else
-      k_poisson_z<T, N_, false><<<nblocks<N_>(g.ncols), 256, 0, s>>>(
+      launch<k_poisson_z<T, N_, false>>(nblocks<N_>(g.ncols), 256, s,
           (cpx<T>*)H, (cpx<T>*)Uz, g, tw, nyl, d.nx, d.ny, kfx, kfy, kfz, four_pi_G, rho_bar,
           ntot, dc_mode, diag);
   });
@@ -877,10 +914,10 @@ hipError_t launch_poisson_y(const Dims& d, const SpecTables<T>& tb, int grad, vo
   const double kfy = 2.0 * M_PI / d.Ly;
   QJ_DISPATCH_N(d.ny, {
     if (grad)
-      k_poisson_y<T, N_, true><<<nblocks_s<T, N_>(g.ncols), 256, 0, s>>>(
+      launch<k_poisson_y<T, N_, true>>(nblocks_s<T, N_>(g.ncols), 256, s,
           (cpx<T>*)H, (cpx<T>*)Uz, (cpx<T>*)Uy, g, tw, kfy);
     else
-      k_poisson_y<T, N_, false><<<nblocks_s<T, N_>(g.ncols), 256, 0, s>>>(
+      launch<k_poisson_y<T, N_, false>>(nblocks_s<T, N_>(g.ncols), 256, s,
           (cpx<T>*)H, (cpx<T>*)Uz, (cpx<T>*)Uy, g, tw, kfy);
   });
   return hipGetLastError();
@@ -895,11 +932,11 @@ hipError_t launch_poisson_x(const Dims& d, const SpecTables<T>& tb, int grad, co
   const double kfx = 2.0 * M_PI / d.Lx;
   QJ_DISPATCH_N(d.nx, {
     if (grad)
-      k_poisson_x<T, N_, true><<<nblocks_s<T, N_>(g.ncols), 256, 0, s>>>(
+      launch<k_poisson_x<T, N_, true>>(nblocks_s<T, N_>(g.ncols), 256, s,
           (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A4,
           (const T*)addend, diag);
     else
-      k_poisson_x<T, N_, false><<<nblocks_s<T, N_>(g.ncols), 256, 0, s>>>(
+      launch<k_poisson_x<T, N_, false>>(nblocks_s<T, N_>(g.ncols), 256, s,
           (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A4,
           (const T*)addend, diag);
   });
@@ -911,7 +948,7 @@ hipError_t launch_kick_adj(void* psi_bar, const void* psi_ref, const void* V, do
                            void* Vbar, long long n, hipStream_t s) {
   long long blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  k_kick_adj<T><<<(int)blocks, 256, 0, s>>>((cpx<T>*)psi_bar, (const cpx<T>*)psi_ref,
+  launch<k_kick_adj<T>>((int)blocks, 256, s, (cpx<T>*)psi_bar, (const cpx<T>*)psi_ref,
                                             (const T*)V, coef, (T*)Vbar, n);
   return hipGetLastError();
 }
@@ -921,7 +958,7 @@ hipError_t launch_rho_adj(void* psi_bar, const void* psi_ref, const void* rhobar
                           hipStream_t s) {
   long long blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  k_rho_adj<T><<<(int)blocks, 256, 0, s>>>((cpx<T>*)psi_bar, (const cpx<T>*)psi_ref,
+  launch<k_rho_adj<T>>((int)blocks, 256, s, (cpx<T>*)psi_bar, (const cpx<T>*)psi_ref,
                                            (const T*)rhobar, n);
   return hipGetLastError();
 }
@@ -936,7 +973,7 @@ hipError_t launch_adj_poisson(const Dims& d, const SpecTables<T>& tb, const void
     PassGeom g = geom_x(d);
     const cpx<T>* tw = (const cpx<T>*)tb.tw_x;
     QJ_DISPATCH_N(d.nx, {
-      k_adj_src_x<T, N_><<<nblocks_s<T, N_>(g.ncols), 256, 0, s>>>(
+      launch<k_adj_src_x<T, N_>>(nblocks_s<T, N_>(g.ncols), 256, s,
           (const T*)Vbar, (const T*)abar4, g, tw, kfx, (cpx<T>*)H, (cpx<T>*)U2, (cpx<T>*)U3);
     });
     hipError_t e = hipGetLastError();
@@ -946,7 +983,7 @@ hipError_t launch_adj_poisson(const Dims& d, const SpecTables<T>& tb, const void
     PassGeom g = geom_y(d, nxh);
     const cpx<T>* tw = (const cpx<T>*)tb.tw_y;
     QJ_DISPATCH_N(d.ny, {
-      k_adj_y<T, N_><<<nblocks_s<T, N_>(g.ncols), 256, 0, s>>>(
+      launch<k_adj_y<T, N_>>(nblocks_s<T, N_>(g.ncols), 256, s,
           (cpx<T>*)H, (const cpx<T>*)U2, (cpx<T>*)U3, g, tw, kfy);
     });
     hipError_t e = hipGetLastError();
@@ -957,7 +994,7 @@ hipError_t launch_adj_poisson(const Dims& d, const SpecTables<T>& tb, const void
     const cpx<T>* tw = (const cpx<T>*)tb.tw_z;
     const double ntot = (double)d.nx * d.ny * d.nz;
     QJ_DISPATCH_N(d.nz, {
-      k_adj_z<T, N_><<<nblocks<N_>(g.ncols), 256, 0, s>>>(
+      launch<k_adj_z<T, N_>>(nblocks<N_>(g.ncols), 256, s,
           (cpx<T>*)H, (const cpx<T>*)U3, g, tw, ilog2(d.ny), d.nx, d.ny, kfx, kfy, kfz,
           four_pi_G, ntot, dc_mode);
     });
@@ -998,9 +1035,19 @@ hipError_t launch_adj_poisson(const Dims& d, const SpecTables<T>& tb, const void
                                             const void*, double, int, void*, void*, void*,      \
                                             void*, hipStream_t);
 
+// The build compiles this file once per QJ_PART (0..3) so that the four
+// instantiation sets build in parallel; without QJ_PART all are emitted.
+#if !defined(QJ_PART) || QJ_PART == 0
 QJ_INST(float)
+#endif
+#if !defined(QJ_PART) || QJ_PART == 1
 QJ_INST(double)
+#endif
+#if !defined(QJ_PART) || QJ_PART == 2
 QJ_INST_ADJ(float)
+#endif
+#if !defined(QJ_PART) || QJ_PART == 3
 QJ_INST_ADJ(double)
+#endif
 
 }  // namespace qj

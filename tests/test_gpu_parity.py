@@ -312,20 +312,34 @@ def test_plain_torch_comparator_matches_oracle(cuda):
     assert rel(out[2], ref[2]) < 1e-12
 
 
-def test_x_schedules_agree_bitwise(qj, cuda):
-    # the fused and the split x sweep compute the same operations
+@pytest.mark.parametrize("kind", ["plain", "gas", "stars"])
+def test_x_schedules_agree(qj, cuda, kind):
+    # the fused and the split x sweep compute the same operations: bitwise for
+    # plain and gas steps; with stars the deposit's float atomics add in a
+    # run-dependent order, so two runs agree to rounding only
     _lib = __import__("importlib").import_module("quantum-jax_amd._lib")
-    g = O.Grid(64, 32, 32)
+    g = O.Grid(32, 32, 32) if kind == "gas" else O.Grid(64, 32, 32)
     psi, pos, vel, m_s = _stars_case(g, 800, 16)
     out = []
     for fuse in (1, 0):
         plan = plan_for(qj, g, C64)
         plan.set_option(_lib.OPT_FUSE_X_SOURCE, fuse)
         assert plan.get_option(_lib.OPT_FUSE_X_SOURCE) == fuse
-        sim = qj.KDKSimulation(plan, qj.Physics(rho_bar=1.1), 1e-3, dev(psi, C64),
-                               dev(pos, torch.float64), dev(vel, torch.float64), mass=m_s)
+        if kind == "stars":
+            sim = qj.KDKSimulation(plan, qj.Physics(rho_bar=1.1), 1e-3, dev(psi, C64),
+                                   dev(pos, torch.float64), dev(vel, torch.float64), mass=m_s)
+        elif kind == "gas":
+            gas = [dev(a, torch.float32) for a in O.gas_ic(g, 3, sigma_k=8 * np.pi)]
+            rb = float(np.mean(np.abs(psi) ** 2) + gas[0].double().mean().item())
+            sim = qj.KDKSimulation(plan, qj.Physics(rho_bar=rb, four_pi_G=20.0), 1e-3,
+                                   dev(psi, C64), gas=gas)
+        else:
+            sim = qj.KDKSimulation(plan, qj.Physics(four_pi_G=20.0), 1e-3, dev(psi, C64))
         sim.step(3)
         st = sim.state()
-        out.append((st["psi"].clone(), st["pos"].clone(), st["vel"].clone()))
+        out.append([st[k].clone() for k in ("psi", "pos", "vel", "rho", "vx") if k in st])
     for a, b in zip(*out):
-        assert torch.equal(a, b)
+        if kind == "stars":
+            assert rel(a, b) < 1e-6
+        else:
+            assert torch.equal(a, b)
