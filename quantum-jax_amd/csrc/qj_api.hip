@@ -88,7 +88,6 @@ struct qj_plan {
   void* drift[3] = {nullptr, nullptr, nullptr};
   double drift_key_dt = NAN, drift_key_mph = NAN;
   double* partials = nullptr;
-  unsigned int* counter = nullptr;
   int nblk_src = 0;
   int opt[QJ_OPT_COUNT] = {1};  // QJ_OPT_FUSE_X_SOURCE on  // schedule options (qj_plan_set_option)
   // optional per-stage event timing (tracing hook of the bench harness)
@@ -205,7 +204,8 @@ int poisson_tail(qj_plan* p, const Work& w, int grad, double four_pi_G, double r
                  double* diag, hipStream_t s) {
   SpecTables<T> tb = tables<T>(p);
   QJ_STAGE(kStageFftY, launch_fft_y_half<T>(p->d, tb, w.H, s), "fft_y");
-  QJ_STAGE(kStagePoissonZ, launch_poisson_z<T>(p->d, tb, grad, w.H, w.Uz, four_pi_G, rho_bar, dc_mode, diag, s),
+  QJ_STAGE(kStagePoissonZ, launch_poisson_z<T>(p->d, tb, grad, w.H, w.Uz, four_pi_G, rho_bar, dc_mode,
+                                                  p->partials, diag, s),
            "poisson_z");
   QJ_STAGE(kStagePoissonY, launch_poisson_y<T>(p->d, tb, grad, w.H, w.Uz, w.Uy, s), "poisson_y");
   QJ_STAGE(kStagePoissonX, launch_poisson_x<T>(p->d, tb, grad, w.H, w.Uy, w.Uz, V, A4, addend, diag, s),
@@ -229,7 +229,7 @@ int get_potential_t(qj_plan* p, const void* rho, double rho_bar, double four_pi_
   // diag scratch: the source mass lands in the plan-owned partials' tail
   double* diag = p->partials + p->nblk_src;
   QJ_STAGE(kStageSourceX, launch_source_x<T>(p->d, tb, 0, nullptr, (void*)rho, 0, brho, b, rho_bar, w.H,
-                              p->partials, p->counter, diag, s),
+                              p->partials, diag, s),
            "source_x");
   return poisson_tail<T>(p, w, 0, four_pi_G, rho_bar, dc_mode, V, nullptr, brho, diag, s);
 }
@@ -242,7 +242,7 @@ int accel_grid_t(qj_plan* p, const void* rho, double rho_bar, double four_pi_G, 
   // the source mass and max|a| land in the caller's diag, else in plan scratch
   double* dg = diag ? diag : p->partials + p->nblk_src;
   QJ_STAGE(kStageSourceX, launch_source_x<T>(p->d, tb, 0, nullptr, (void*)rho, 0, nullptr, 0.0, rho_bar, w.H,
-                              p->partials, p->counter, dg, s),
+                              p->partials, dg, s),
            "source_x");
   void* Vout = V ? V : w.R;  // the pass always writes V
   return poisson_tail<T>(p, w, 1, four_pi_G, rho_bar, dc_mode, Vout, a4, nullptr, dg, s);
@@ -280,8 +280,6 @@ int qj_plan_create(qj_plan** out, int nx, int ny, int nz, double Lx, double Ly, 
   p->nblk_src = source_x_blocks(p->d);
   if (e == hipSuccess)
     e = hipMalloc((void**)&p->partials, (size_t)(p->nblk_src + kDiagCount) * sizeof(double));
-  if (e == hipSuccess) e = hipMalloc((void**)&p->counter, 256);
-  if (e == hipSuccess) e = hipMemset(p->counter, 0, 256);
   if (e == hipSuccess)
     e = hipMemset(p->partials, 0, (size_t)(p->nblk_src + kDiagCount) * sizeof(double));
   for (int a = 0; a < 3 && e == hipSuccess; ++a) {
@@ -309,7 +307,6 @@ int qj_plan_destroy(qj_plan* p) {
     if (p->drift[a]) (void)hipFree(p->drift[a]);
   }
   if (p->partials) (void)hipFree(p->partials);
-  if (p->counter) (void)hipFree(p->counter);
   delete p;
   return QJ_OK;
 }
@@ -521,7 +518,7 @@ int initial_grid_t(qj_plan* p, qj_state* st, const qj_params* pr, const Work& w,
                     : (gas ? (st->gas_swapped ? st->gas_tmp[0] : st->gas[0]) : nullptr);
   QJ_STAGE(kStageSourceX,
            launch_source_x<T>(p->d, tb, 0, st->psi, src, stars ? 1 : 0, brho, pr->b_coeff,
-                              pr->rho_bar, w.H, p->partials, p->counter, st->diag, s),
+                              pr->rho_bar, w.H, p->partials, st->diag, s),
            "source_x");
   if (int rc = poisson_tail<T>(p, w, (stars || gas) ? 1 : 0, pr->four_pi_G, pr->rho_bar,
                                pr->dc_mode, st->grid, A4, nullptr, st->diag, s))
@@ -578,14 +575,14 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
     if (p->opt[QJ_OPT_FUSE_X_SOURCE]) {
       QJ_STAGE(kStageDriftXSource,
                launch_source_x<T>(p->d, tb, 1, st->psi, src, stars ? 1 : 0, brho, pr->b_coeff,
-                                  pr->rho_bar, w.H, p->partials, p->counter, st->diag, s),
+                                  pr->rho_bar, w.H, p->partials, st->diag, s),
                "drift x + source");
     } else {
       QJ_STAGE(kStageDriftX,
                launch_drift_axis<T>(p->d, tb, 0, st->psi, nullptr, nullptr, 0.0, 0, s), "drift x");
       QJ_STAGE(kStageSourceX,
                launch_source_x<T>(p->d, tb, 0, st->psi, src, stars ? 1 : 0, brho, pr->b_coeff,
-                                  pr->rho_bar, w.H, p->partials, p->counter, st->diag, s),
+                                  pr->rho_bar, w.H, p->partials, st->diag, s),
                "source x");
     }
     if (int rc = poisson_tail<T>(p, w, grad, pr->four_pi_G, pr->rho_bar, pr->dc_mode, V, A4,

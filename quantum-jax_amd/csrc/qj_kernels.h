@@ -43,16 +43,17 @@ hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, v
 template <class T>
 hipError_t launch_source_x(const Dims& d, const SpecTables<T>& tb, int drift, void* psi,
                            void* src, int clear_src, void* brho, double b_coeff, double rho_bar,
-                           void* H, double* partials, unsigned int* counter, double* diag,
-                           hipStream_t s);
+                           void* H, double* partials, double* diag, hipStream_t s);
 
 template <class T>
 hipError_t launch_fft_y_half(const Dims& d, const SpecTables<T>& tb, void* H, hipStream_t s);
 
 template <class T>
+// partials: the per-block FP64 source sums of the preceding launch_source_x;
+// the pass writes their compensated total to diag[kDiagMass].
 hipError_t launch_poisson_z(const Dims& d, const SpecTables<T>& tb, int grad, void* H, void* Uz,
-                            double four_pi_G, double rho_bar, int dc_mode, const double* diag,
-                            hipStream_t s);
+                            double four_pi_G, double rho_bar, int dc_mode,
+                            const double* partials, double* diag, hipStream_t s);
 
 template <class T>
 hipError_t launch_poisson_y(const Dims& d, const SpecTables<T>& tb, int grad, void* H, void* Uz,

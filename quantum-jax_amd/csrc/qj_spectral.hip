@@ -187,13 +187,15 @@ struct k_drift_s {
 };
 
 // ---------------------------------------------------------------------------
-// block-level FP64 sum with a deterministic last-block finish
+// FP64 source mass: each block of the source pass stores its partial sum; the
+// Poisson z pass (the next kernel in stream order, which needs the k = 0
+// bin) adds the partials with a compensated sum in its first block.  The
+// kernel boundary makes the partials visible, so no grid-wide fence is
+// needed (an agent-scope fence per block writes back the XCD's L2).
 // ---------------------------------------------------------------------------
-__device__ void block_sum_finish(double local, bool bad, double* partials, unsigned int* counter,
-                                 double* diag) {
+__device__ void block_partial(double local, bool bad, double* partials, double* diag) {
   __shared__ double red[4];
   __shared__ int flag[4];
-  __shared__ bool last;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   double s = wave_sum(local);
   int anybad = __any(bad);
@@ -203,43 +205,47 @@ __device__ void block_sum_finish(double local, bool bad, double* partials, unsig
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double bs = (red[0] + red[1]) + (red[2] + red[3]);
-    partials[blockIdx.x] = bs;
+    partials[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
     if (flag[0] | flag[1] | flag[2] | flag[3])
       atomicExch(reinterpret_cast<unsigned long long*>(&diag[kDiagNaN]),
                  (unsigned long long)__double_as_longlong(1.0));
-    __threadfence();
-    unsigned int t = atomicAdd(counter, 1u);
-    last = (t == gridDim.x - 1);
+  }
+}
+
+__device__ __forceinline__ void two_sum_acc(double& s, double& c, double x) {
+  const double t = s + x;
+  c += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
+  s = t;
+}
+
+// Compensated (Neumaier per thread, error-free merges across lanes and
+// waves) sum of x[0..n) by one 256-thread block, in a fixed order; the
+// result is returned to every thread.
+__device__ double block_compensated_sum(const double* __restrict__ x, int n) {
+  __shared__ double ws[4], wc[4];
+  __shared__ double total;
+  double s = 0.0, c = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) two_sum_acc(s, c, x[i]);
+  for (int off = 32; off > 0; off >>= 1) {
+    const double so = __shfl_down(s, off), co = __shfl_down(c, off);
+    two_sum_acc(s, c, so);
+    c += co;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    ws[threadIdx.x >> 6] = s;
+    wc[threadIdx.x >> 6] = c;
   }
   __syncthreads();
-  if (last) {
-    __threadfence();
-    // Neumaier-compensated sum of the block partials
-    double sum = 0.0, comp = 0.0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) {
-      double x = __builtin_nontemporal_load(&partials[i]);
-      double t = sum + x;
-      comp += (fabs(sum) >= fabs(x)) ? (sum - t) + x : (x - t) + sum;
-      sum = t;
+  if (threadIdx.x == 0) {
+    double S = ws[0], Cc = wc[0];
+    for (int w = 1; w < 4; ++w) {
+      two_sum_acc(S, Cc, ws[w]);
+      Cc += wc[w];
     }
-    __shared__ double hs[256], hc[256];
-    hs[threadIdx.x] = sum;
-    hc[threadIdx.x] = comp;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double S = 0.0, Cc = 0.0;
-      for (int i = 0; i < (int)blockDim.x; ++i) {
-        double x = hs[i];
-        double t = S + x;
-        Cc += (fabs(S) >= fabs(x)) ? (S - t) + x : (x - t) + S;
-        S = t;
-        Cc += hc[i];
-      }
-      diag[kDiagMass] = S + Cc;
-      *counter = 0u;
-    }
+    total = S + Cc;
   }
+  __syncthreads();
+  return total;
 }
 
 // ---------------------------------------------------------------------------
@@ -256,7 +262,6 @@ struct k_source_x {
                                              const cpx<T>* __restrict__ D,
                                              const cpx<T>* __restrict__ tw, cpx<T>* __restrict__ H,
                                              double* __restrict__ partials,
-                                             unsigned int* __restrict__ counter,
                                              double* __restrict__ diag) {
     QJ_STRIDED_SETUP(T, N)
     cpx<T> v[VW][E];
@@ -330,7 +335,7 @@ struct k_source_x {
         }
       }
     }
-    block_sum_finish(local, bad, partials, counter, diag);
+    block_partial(local, bad, partials, diag);
   }
 };
 
@@ -362,9 +367,16 @@ struct k_poisson_z {
                                              PassGeom g, const cpx<T>* __restrict__ tw, int ny_log2,
                                              int nx, int ny, double kfx, double kfy, double kfz,
                                              double four_pi_G, double rho_bar, double ntot,
-                                             int dc_mode, const double* __restrict__ diag) {
+                                             int dc_mode, const double* __restrict__ partials,
+                                             int nparts, double* __restrict__ diag) {
     using S = PencilShape<N>;
     __shared__ cpx<T> lds[S::kLds];
+    // block 0 holds the kx = ky = 0 pencil: FP64 source mass for the k = 0 bin
+    double mass = 0.0;
+    if (blockIdx.x == 0) {
+      mass = block_compensated_sum(partials, nparts);
+      if (threadIdx.x == 0) diag[kDiagMass] = mass;
+    }
     Lane<N, true> ln(threadIdx.x);
     const int q = blockIdx.x * S::C + ln.col;
     const bool act = q < g.ncols;
@@ -388,7 +400,7 @@ struct k_poisson_z {
       const T Gk = (T)(-four_pi_G * inv_ntot / (k2 == 0.0 ? 1.0 : k2));
       v[m] = scale(v[m], Gk);
       if (x == 0 && y == 0 && kzi == 0) {
-        double dc = (dc_mode == 0) ? -four_pi_G * (diag[kDiagMass] - ntot * rho_bar) * inv_ntot : 0.0;
+        double dc = (dc_mode == 0) ? -four_pi_G * (mass - ntot * rho_bar) * inv_ntot : 0.0;
         v[m] = {(T)dc, (T)0};
       }
       if constexpr (GRAD) {
@@ -856,8 +868,7 @@ int source_x_blocks(const Dims& d) {
 template <class T>
 hipError_t launch_source_x(const Dims& d, const SpecTables<T>& tb, int drift, void* psi,
                            void* src, int clear_src, void* brho, double b_coeff, double rho_bar,
-                           void* H, double* partials, unsigned int* counter, double* diag,
-                           hipStream_t s) {
+                           void* H, double* partials, double* diag, hipStream_t s) {
   PassGeom g = geom_x(d);
   const cpx<T>* D = (const cpx<T>*)tb.drift_x;
   const cpx<T>* tw = (const cpx<T>*)tb.tw_x;
@@ -865,11 +876,11 @@ hipError_t launch_source_x(const Dims& d, const SpecTables<T>& tb, int drift, vo
     if (drift)
       launch<k_source_x<T, N_, true>>(nblocks_s<T, N_>(g.ncols), 256, s,
           (cpx<T>*)psi, (T*)src, clear_src, (T*)brho, b_coeff, rho_bar, g, D, tw, (cpx<T>*)H,
-          partials, counter, diag);
+          partials, diag);
     else
       launch<k_source_x<T, N_, false>>(nblocks_s<T, N_>(g.ncols), 256, s,
           (cpx<T>*)psi, (T*)src, clear_src, (T*)brho, b_coeff, rho_bar, g, D, tw, (cpx<T>*)H,
-          partials, counter, diag);
+          partials, diag);
   });
   return hipGetLastError();
 }
@@ -886,8 +897,9 @@ hipError_t launch_fft_y_half(const Dims& d, const SpecTables<T>& tb, void* H, hi
 
 template <class T>
 hipError_t launch_poisson_z(const Dims& d, const SpecTables<T>& tb, int grad, void* H, void* Uz,
-                            double four_pi_G, double rho_bar, int dc_mode, const double* diag,
-                            hipStream_t s) {
+                            double four_pi_G, double rho_bar, int dc_mode,
+                            const double* partials, double* diag, hipStream_t s) {
+  const int nparts = source_x_blocks_t<T>(d);
   PassGeom g = geom_z(d, d.nx / 2 + 1);
   const cpx<T>* tw = (const cpx<T>*)tb.tw_z;
   const double kfx = 2.0 * M_PI / d.Lx, kfy = 2.0 * M_PI / d.Ly, kfz = 2.0 * M_PI / d.Lz;
@@ -897,11 +909,11 @@ hipError_t launch_poisson_z(const Dims& d, const SpecTables<T>& tb, int grad, vo
     if (grad)
       launch<k_poisson_z<T, N_, true>>(nblocks<N_>(g.ncols), 256, s,
           (cpx<T>*)H, (cpx<T>*)Uz, g, tw, nyl, d.nx, d.ny, kfx, kfy, kfz, four_pi_G, rho_bar,
-          ntot, dc_mode, diag);
+          ntot, dc_mode, partials, nparts, diag);
     else
       launch<k_poisson_z<T, N_, false>>(nblocks<N_>(g.ncols), 256, s,
           (cpx<T>*)H, (cpx<T>*)Uz, g, tw, nyl, d.nx, d.ny, kfx, kfy, kfz, four_pi_G, rho_bar,
-          ntot, dc_mode, diag);
+          ntot, dc_mode, partials, nparts, diag);
   });
   return hipGetLastError();
 }
@@ -1015,12 +1027,13 @@ hipError_t launch_adj_poisson(const Dims& d, const SpecTables<T>& tb, const void
                                            const void*, const void*, double, int, hipStream_t,  \
                                            int);                                                \
   template hipError_t launch_source_x<T>(const Dims&, const SpecTables<T>&, int, void*, void*,  \
-                                         int, void*, double, double, void*, double*,            \
-                                         unsigned int*, double*, hipStream_t);                  \
+                                         int, void*, double, double, void*, double*, double*,   \
+                                         hipStream_t);                                          \
   template hipError_t launch_fft_y_half<T>(const Dims&, const SpecTables<T>&, void*,            \
                                            hipStream_t);                                        \
   template hipError_t launch_poisson_z<T>(const Dims&, const SpecTables<T>&, int, void*, void*, \
-                                          double, double, int, const double*, hipStream_t);     \
+                                          double, double, int, const double*, double*,          \
+                                          hipStream_t);                                         \
   template hipError_t launch_poisson_y<T>(const Dims&, const SpecTables<T>&, int, void*, void*, \
                                           void*, hipStream_t);                                  \
   template hipError_t launch_poisson_x<T>(const Dims&, const SpecTables<T>&, int, const void*,  \
