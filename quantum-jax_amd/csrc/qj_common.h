@@ -113,10 +113,36 @@ __device__ __forceinline__ double reduce_2pi(double th) {
   return fma(-n, kTwoPiLo, r);
 }
 
-// exp(-i*theta) with an FP64 argument
+// sin r and cos r for |r| <= 1/2 from their Taylor series in nested Horner
+// form, sin r = r (1 - r^2/(2*3) (1 - r^2/(4*5) (1 - ...))) and
+// cos r = 1 - r^2/(1*2) (1 - r^2/(3*4) (1 - ...)); K levels leave a
+// truncation error below 0.5^(2K+2)/(2K+2)!, under half an ulp of T for
+// K = 5 (float) and K = 8 (double).
+template <class T, int K>
+__device__ __forceinline__ void sincos_small(T r, T& s, T& c) {
+  const T r2 = r * r;
+  T ps = (T)1, pc = (T)1;
+  sfor<0, K>([&](auto kk) {
+    constexpr int k = K - (int)decltype(kk)::value;  // K .. 1
+    constexpr T as = (T)(1.0 / ((2.0 * k) * (2.0 * k + 1.0)));
+    constexpr T ac = (T)(1.0 / ((2.0 * k - 1.0) * (2.0 * k)));
+    ps = fmad(-(r2 * as), ps, (T)1);
+    pc = fmad(-(r2 * ac), pc, (T)1);
+  });
+  s = r * ps;
+  c = pc;
+}
+
+// exp(-i*theta) with an FP64 argument: range reduction in FP64, then the
+// short series for the common small reduced phase, else the library sincos.
 template <class T>
 __device__ __forceinline__ cpx<T> expmi(double th) {
   double r = reduce_2pi(th);
+  if (fabs(r) <= 0.5) {
+    T s, c;
+    sincos_small<T, std::is_same<T, float>::value ? 5 : 8>((T)r, s, c);
+    return {c, -s};
+  }
   if constexpr (std::is_same<T, float>::value) {
     float s, c;
     sincosf((float)r, &s, &c);
