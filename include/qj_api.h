@@ -128,7 +128,8 @@ int qj_solve_hydro(qj_plan* plan, const void* rho, const void* vx, const void* v
  */
 typedef struct {
   void* psi;        /* complex N                                              */
-  void* rho_s;      /* real N, star density scratch (must start zeroed)       */
+  void* rho_s;      /* real N, star density scratch (must start zeroed unless
+                       star_scratch is given)                                 */
   void* gas[4];     /* rho, vx, vy, vz: real N each (gas step) or NULL        */
   void* gas_tmp[4]; /* second buffers for the out-of-place gas update         */
   double* pos;      /* (np,3)                                                 */
@@ -144,6 +145,11 @@ typedef struct {
   int gas_swapped;  /* the live gas fields are gas_tmp (after an odd count)   */
   double pending;   /* time of the owed kick of psi and gas velocity          */
   double pending_star; /* time of the owed closing kick of the star velocities */
+  /* optional: qj_star_scratch_size() bytes, zeroed once before first use.
+   * When given, the star deposit is tile-binned (per-tile LDS accumulation
+   * and plain stores, no global float atomics) and rho_s needs no clearing;
+   * when NULL, the deposit adds with float atomics into rho_s. */
+  void* star_scratch;
 } qj_state;
 
 typedef struct {
@@ -158,6 +164,10 @@ typedef struct {
 
 int qj_kdk_steps(qj_plan* plan, qj_state* state, const qj_params* prm, int nsteps, void* work,
                  void* stream);
+/* Bytes of qj_state.star_scratch for np stars: per-tile counters and
+ * offsets and up to 8 registrations per star (bin_stars of
+ * quantum-stars.py:212-247, restructured as owner-computes by tile). */
+size_t qj_star_scratch_size(const qj_plan* plan, long long np);
 int qj_kdk_finalize(qj_plan* plan, qj_state* state, const qj_params* prm, void* stream);
 
 /* ---- adjoint of the KDK step (no counterpart in the reference, which

@@ -510,14 +510,29 @@ int initial_grid_t(qj_plan* p, qj_state* st, const qj_params* pr, const Work& w,
   const bool gas = st->gas[0] != nullptr;
   void* A4 = (stars || gas) ? plane<T>(st->grid, p, 1) : nullptr;
   void* brho = (pr->b_coeff != 0.0) ? st->brho : nullptr;
-  if (stars)
+  if (stars && st->star_scratch) {
+    // tile-binned deposit of the current positions (overwrites rho_s)
+    StarBins sb = carve_star_scratch(p->d, st->np, st->star_scratch);
+    QJ_CHECK(fill<int>(sb.count, sb.ntiles, 0, s), "bin reset");
+    QJ_STAGE(kStageDeposit,
+             launch_star_step_binned<T>(p->d, st->pos, st->vel, st->np, nullptr, 0.0, 0.0, 0.0, 0,
+                                        sb, s),
+             "bin");
+    QJ_STAGE(kStageDeposit,
+             launch_tile_deposit<T>(p->d, st->pos, st->mass, st->m_scalar, st->np, sb, st->rho_s,
+                                    s),
+             "deposit");
+  } else if (stars) {
     QJ_STAGE(kStageDeposit,
              launch_deposit<T>(p->d, st->pos, st->mass, st->m_scalar, st->np, st->rho_s, s),
              "deposit");
+  }
+  // the atomic deposit accumulates, so its plane is cleared after reading
+  const int clear = (stars && !st->star_scratch) ? 1 : 0;
   void* src = stars ? st->rho_s
                     : (gas ? (st->gas_swapped ? st->gas_tmp[0] : st->gas[0]) : nullptr);
   QJ_STAGE(kStageSourceX,
-           launch_source_x<T>(p->d, tb, 0, st->psi, src, stars ? 1 : 0, brho, pr->b_coeff,
+           launch_source_x<T>(p->d, tb, 0, st->psi, src, clear, brho, pr->b_coeff,
                               pr->rho_bar, w.H, p->partials, st->diag, s),
            "source_x");
   if (int rc = poisson_tail<T>(p, w, (stars || gas) ? 1 : 0, pr->four_pi_G, pr->rho_bar,
@@ -543,6 +558,8 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
   auto live_gas = [&](int k) { return st->gas_swapped ? st->gas_tmp[k] : st->gas[k]; };
   auto spare_gas = [&](int k) { return st->gas_swapped ? st->gas[k] : st->gas_tmp[k]; };
   void* src = nullptr;
+  // the atomic deposit accumulates, so its plane is cleared after reading
+  const int clear = (stars && !st->star_scratch) ? 1 : 0;
 
   for (int step = 0; step < nsteps; ++step) {
     if (!st->grid_valid) {
@@ -551,11 +568,22 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
     const double kick_t = st->pending + 0.5 * dt;  // owed half kick + this step's first
     // stars: owed closing kick of the previous step and this step's opening
     // half kick (same grid, same positions), drift, deposit at the new positions
-    if (stars)
+    if (stars && st->star_scratch) {
+      StarBins sb = carve_star_scratch(p->d, st->np, st->star_scratch);
+      QJ_STAGE(kStageStarStep,
+               launch_star_step_binned<T>(p->d, st->pos, st->vel, st->np, A4, st->pending_star,
+                                          0.5 * dt, dt, 1, sb, s),
+               "star step");
+      QJ_STAGE(kStageDeposit,
+               launch_tile_deposit<T>(p->d, st->pos, st->mass, st->m_scalar, st->np, sb,
+                                      st->rho_s, s),
+               "deposit");
+    } else if (stars) {
       QJ_STAGE(kStageStarStep,
                launch_star_step<T>(p->d, st->pos, st->vel, st->np, A4, st->pending_star, 0.5 * dt,
                                    dt, st->mass, st->m_scalar, st->rho_s, s),
                "star step");
+    }
     // gas: owed + first half kick of the velocity, then the MUSCL update
     if (gas) {
       QJ_STAGE(kStageHydro,
@@ -574,14 +602,14 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
     src = stars ? st->rho_s : (gas ? live_gas(0) : nullptr);
     if (p->opt[QJ_OPT_FUSE_X_SOURCE]) {
       QJ_STAGE(kStageDriftXSource,
-               launch_source_x<T>(p->d, tb, 1, st->psi, src, stars ? 1 : 0, brho, pr->b_coeff,
+               launch_source_x<T>(p->d, tb, 1, st->psi, src, clear, brho, pr->b_coeff,
                                   pr->rho_bar, w.H, p->partials, st->diag, s),
                "drift x + source");
     } else {
       QJ_STAGE(kStageDriftX,
                launch_drift_axis<T>(p->d, tb, 0, st->psi, nullptr, nullptr, 0.0, 0, s), "drift x");
       QJ_STAGE(kStageSourceX,
-               launch_source_x<T>(p->d, tb, 0, st->psi, src, stars ? 1 : 0, brho, pr->b_coeff,
+               launch_source_x<T>(p->d, tb, 0, st->psi, src, clear, brho, pr->b_coeff,
                                   pr->rho_bar, w.H, p->partials, st->diag, s),
                "source x");
     }
@@ -733,6 +761,11 @@ int qj_kdk_steps(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void
   hipStream_t s = (hipStream_t)stream;
   return p->dtype == QJ_C64 ? kdk_steps_t<float>(p, st, pr, nsteps, work, s)
                             : kdk_steps_t<double>(p, st, pr, nsteps, work, s);
+}
+
+size_t qj_star_scratch_size(const qj_plan* p, long long np) {
+  if (!p || np < 0) return 0;
+  return star_scratch_bytes(p->d, np);
 }
 
 size_t qj_adjoint_scratch_size(const qj_plan* p) {

@@ -131,6 +131,36 @@ template <class T>
 hipError_t launch_gas_kick(long long n, void* vx, void* vy, void* vz, const void* A4, double h,
                            hipStream_t s);
 
+// Tile-binned star deposit (owner computes; no global float atomics).
+constexpr int kTileX = 8, kTileY = 8, kTileZ = 64;
+constexpr int kTileMaxCells = kTileX * kTileY * kTileZ;
+struct TileGeom {
+  int lx, ly, lz;     // log2 of the tile extents
+  int ntx, nty, ntz;  // tiles per axis
+  int ntiles;
+};
+struct StarBins {
+  int* count;   // ntiles, zero between steps
+  int* offset;  // ntiles + 1
+  int* sorted;  // 8 np registrations at most
+  int* rank;    // 8 np: slot of each registration within its tile
+  int ntiles;
+};
+TileGeom make_tiles(const Dims& d);
+size_t star_scratch_bytes(const Dims& d, long long np);
+StarBins carve_star_scratch(const Dims& d, long long np, void* scratch);
+// Kick/drift/wrap as launch_star_step (move = 1) or nothing (move = 0), and
+// registration counting of the (new) positions.
+template <class T>
+hipError_t launch_star_step_binned(const Dims& d, double* pos, double* vel, long long np,
+                                   const void* A4, double h_owed, double h1, double dt, int move,
+                                   const StarBins& sb, hipStream_t s);
+// Scan, scatter and per-tile deposit: overwrites every cell of rho.
+template <class T>
+hipError_t launch_tile_deposit(const Dims& d, const double* pos, const double* mass,
+                               double m_scalar, long long np, const StarBins& sb, void* rho,
+                               hipStream_t s);
+
 // Sets the message returned by qj_last_error() and returns code (qj_api.hip).
 int set_error(int code, const char* msg);
 

@@ -225,6 +225,171 @@ struct k_deposit_adj {
   }
 };
 
+// ---------------------------------------------------------------------------
+// Tile-binned deposit (owner computes).  The grid is cut into tiles of
+// 2^lx x 2^ly x 2^lz cells.  Each particle is registered in every tile its
+// CIC stencil touches (1 to 8, distinct); one block per tile then adds the
+// corners that fall inside its tile in LDS and writes the whole tile with
+// plain coalesced stores, zeros included.  No global float atomics, and
+// every cell of rho is overwritten (no clearing needed beforehand).
+//   count  : per-tile registration counter (zero between steps)
+//   rank   : slot of each registration within its tile (8 per particle)
+//   offset : exclusive scan of count, ntiles + 1 entries
+//   sorted : registrations grouped by tile (particle indices)
+// ---------------------------------------------------------------------------
+
+// The distinct tiles touched by the stencil of c, in a fixed order.
+__device__ __forceinline__ int stencil_tiles(const TileGeom& t, const Cic& c, int ids[8]) {
+  const int ax[2] = {c.ix[0] >> t.lx, c.ix[1] >> t.lx};
+  const int ay[2] = {c.iy[0] >> t.ly, c.iy[1] >> t.ly};
+  const int az[2] = {c.iz[0] >> t.lz, c.iz[1] >> t.lz};
+  const int nax = ax[1] != ax[0] ? 2 : 1, nay = ay[1] != ay[0] ? 2 : 1;
+  const int naz = az[1] != az[0] ? 2 : 1;
+  int n = 0;
+  for (int a = 0; a < nax; ++a)
+    for (int b = 0; b < nay; ++b)
+      for (int e = 0; e < naz; ++e) ids[n++] = (ax[a] * t.nty + ay[b]) * t.ntz + az[e];
+  return n;
+}
+
+// Star step with registration counting instead of the deposit.  MOVE = false
+// only counts the current positions (the deposit of the initial state).
+template <class T, bool MOVE>
+struct k_star_step_bin {
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(CicGeom g, TileGeom tg, double* __restrict__ pos,
+                                             double* __restrict__ vel, long long np,
+                                             const T* __restrict__ A4, double h_owed, double h1,
+                                             double dt, int* __restrict__ count,
+                                             int* __restrict__ rank) {
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    double x[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
+    if constexpr (MOVE) {
+      double v[3] = {vel[3 * p], vel[3 * p + 1], vel[3 * p + 2]};
+      if (h_owed != 0.0 || h1 != 0.0) {
+        Cic c = cic(g, x[0], x[1], x[2]);
+        double a[3];
+        interp_one(g, c, A4, a);
+        if (h_owed != 0.0) {
+  #pragma unroll
+          for (int d = 0; d < 3; ++d) v[d] = v[d] + a[d] * h_owed;  // quantum-stars.py:321
+        }
+  #pragma unroll
+        for (int d = 0; d < 3; ++d) v[d] = v[d] + a[d] * h1;  // quantum-stars.py:304
+      }
+      const double L[3] = {g.Lx, g.Ly, g.Lz};
+  #pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        x[d] = x[d] + v[d] * dt;        // quantum-stars.py:311
+        x[d] = fmod_floor(x[d], L[d]);  // quantum-stars.py:312
+      }
+  #pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        pos[3 * p + d] = x[d];
+        vel[3 * p + d] = v[d];
+      }
+    }
+    Cic c = cic(g, x[0], x[1], x[2]);
+    if (!c.ok) return;  // non-finite positions deposit nothing
+    int ids[8];
+    const int n = stencil_tiles(tg, c, ids);
+    for (int k = 0; k < n; ++k) rank[8 * p + k] = atomicAdd(&count[ids[k]], 1);
+  }
+};
+
+// Exclusive scan of the tile counts (one block of 1024 threads); resets count.
+struct k_bin_scan {
+  static constexpr int kThreads = 1024;
+  static __device__ __forceinline__ void run(int* __restrict__ count, int* __restrict__ offset,
+                                             int ntiles) {
+    __shared__ int wsum[16];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int per = (ntiles + 1023) / 1024;
+    const int b = tid * per, e = min(b + per, ntiles);
+    int local = 0;
+    for (int i = b; i < e; ++i) local += count[i];
+    // inclusive scan of the per-thread sums within the wave, then across waves
+    int incl = local;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += y;
+    }
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    int base = 0;
+    for (int w = 0; w < wv; ++w) base += wsum[w];
+    int run = base + incl - local;
+    for (int i = b; i < e; ++i) {
+      const int c = count[i];
+      offset[i] = run;
+      run += c;
+      count[i] = 0;
+    }
+    if (tid == 1023) offset[ntiles] = base + incl;
+  }
+};
+
+// Registrations into their tile's slots (the ranks of the counting pass).
+struct k_bin_scatter {
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(CicGeom g, TileGeom tg,
+                                             const double* __restrict__ pos, long long np,
+                                             const int* __restrict__ rank,
+                                             const int* __restrict__ offset,
+                                             int* __restrict__ sorted) {
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    Cic c = cic(g, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]);
+    if (!c.ok) return;
+    int ids[8];
+    const int n = stencil_tiles(tg, c, ids);
+    for (int k = 0; k < n; ++k) sorted[offset[ids[k]] + rank[8 * p + k]] = (int)p;
+  }
+};
+
+// One block per tile: LDS accumulation of the corners inside the tile (the
+// corner values of deposit_one), then the whole tile is stored.
+template <class T>
+struct k_tile_deposit {
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(CicGeom g, TileGeom tg,
+                                             const double* __restrict__ pos,
+                                             const double* __restrict__ mass, double m_scalar,
+                                             const int* __restrict__ offset,
+                                             const int* __restrict__ sorted, T* __restrict__ rho) {
+    __shared__ T acc[kTileMaxCells];
+    const int t = blockIdx.x;
+    const int tz = t % tg.ntz, ty = (t / tg.ntz) % tg.nty, tx = t / (tg.ntz * tg.nty);
+    const int vol = 1 << (tg.lx + tg.ly + tg.lz);
+    for (int l = threadIdx.x; l < vol; l += blockDim.x) acc[l] = (T)0;
+    __syncthreads();
+    const int b = offset[t], e = offset[t + 1];
+    for (int i = b + (int)threadIdx.x; i < e; i += blockDim.x) {
+      const int p = sorted[i];
+      Cic c = cic(g, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]);
+      const double fac = (mass ? mass[p] : m_scalar) / g.vol;
+  #pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int sx = kCorner[k][0], sy = kCorner[k][1], sz = kCorner[k][2];
+        const int ix = c.ix[sx], iy = c.iy[sy], iz = c.iz[sz];
+        if ((ix >> tg.lx) != tx || (iy >> tg.ly) != ty || (iz >> tg.lz) != tz) continue;
+        const double w = c.wx[sx] * c.wy[sy] * c.wz[sz] * fac;
+        const int l = (((ix & ((1 << tg.lx) - 1)) << tg.ly | (iy & ((1 << tg.ly) - 1))) << tg.lz) |
+                      (iz & ((1 << tg.lz) - 1));
+        atomicAdd(&acc[l], (T)w);
+      }
+    }
+    __syncthreads();
+    const int x0 = tx << tg.lx, y0 = ty << tg.ly, z0 = tz << tg.lz;
+    const int mz = (1 << tg.lz) - 1, my = (1 << tg.ly) - 1;
+    for (int l = threadIdx.x; l < vol; l += blockDim.x) {
+      const int lz = l & mz, ly = (l >> tg.lz) & my, lx = l >> (tg.lz + tg.ly);
+      rho[((long long)(x0 + lx) * g.ny + (y0 + ly)) * g.nz + (z0 + lz)] = acc[l];
+    }
+  }
+};
+
 inline int pblocks(long long np) { return (int)((np + 255) / 256); }
 
 }  // namespace
@@ -291,6 +456,71 @@ hipError_t launch_deposit_adj(const Dims& d, const double* pos, const double* ma
   return hipGetLastError();
 }
 
+TileGeom make_tiles(const Dims& d) {
+  auto lg = [](int v) {
+    int r = 0;
+    while ((1 << r) < v) ++r;
+    return r;
+  };
+  TileGeom t;
+  t.lx = lg(d.nx < kTileX ? d.nx : kTileX);
+  t.ly = lg(d.ny < kTileY ? d.ny : kTileY);
+  t.lz = lg(d.nz < kTileZ ? d.nz : kTileZ);
+  t.ntx = d.nx >> t.lx;
+  t.nty = d.ny >> t.ly;
+  t.ntz = d.nz >> t.lz;
+  t.ntiles = t.ntx * t.nty * t.ntz;
+  return t;
+}
+
+size_t star_scratch_bytes(const Dims& d, long long np) {
+  const TileGeom t = make_tiles(d);
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  return al((size_t)t.ntiles * 4) + al((size_t)(t.ntiles + 1) * 4) + 2 * al((size_t)np * 8 * 4);
+}
+
+StarBins carve_star_scratch(const Dims& d, long long np, void* scratch) {
+  const TileGeom t = make_tiles(d);
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  StarBins sb;
+  char* c = (char*)scratch;
+  sb.count = (int*)c;
+  sb.offset = (int*)(c + al((size_t)t.ntiles * 4));
+  sb.sorted = (int*)(c + al((size_t)t.ntiles * 4) + al((size_t)(t.ntiles + 1) * 4));
+  sb.rank = (int*)((char*)sb.sorted + al((size_t)np * 8 * 4));
+  sb.ntiles = t.ntiles;
+  return sb;
+}
+
+template <class T>
+hipError_t launch_star_step_binned(const Dims& d, double* pos, double* vel, long long np,
+                                   const void* A4, double h_owed, double h1, double dt, int move,
+                                   const StarBins& sb, hipStream_t s) {
+  if (np <= 0) return hipSuccess;
+  const TileGeom tg = make_tiles(d);
+  if (move)
+    launch<k_star_step_bin<T, true>>(pblocks(np), 256, s, make_geom(d), tg, pos, vel, np,
+                                     (const T*)A4, h_owed, h1, dt, sb.count, sb.rank);
+  else
+    launch<k_star_step_bin<T, false>>(pblocks(np), 256, s, make_geom(d), tg, pos, vel, np,
+                                      (const T*)A4, h_owed, h1, dt, sb.count, sb.rank);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_tile_deposit(const Dims& d, const double* pos, const double* mass,
+                               double m_scalar, long long np, const StarBins& sb, void* rho,
+                               hipStream_t s) {
+  const TileGeom tg = make_tiles(d);
+  launch<k_bin_scan>(1, 1024, s, sb.count, sb.offset, tg.ntiles);
+  if (np > 0)
+    launch<k_bin_scatter>(pblocks(np), 256, s, make_geom(d), tg, pos, np, (const int*)sb.rank,
+                          (const int*)sb.offset, sb.sorted);
+  launch<k_tile_deposit<T>>(tg.ntiles, 256, s, make_geom(d), tg, pos, mass, m_scalar,
+                            (const int*)sb.offset, (const int*)sb.sorted, (T*)rho);
+  return hipGetLastError();
+}
+
 #define QJ_PINST(T)                                                                              \
   template hipError_t launch_cic_indices<T>(const Dims&, const double*, long long, int*, int*,     \
                                             double*, double*, hipStream_t);                        \
@@ -306,7 +536,12 @@ hipError_t launch_deposit_adj(const Dims& d, const double* pos, const double* ma
   template hipError_t launch_interp_adj<T>(const Dims&, const double*, long long, const void*,     \
                                            double, double*, double, double*, void*, hipStream_t); \
   template hipError_t launch_deposit_adj<T>(const Dims&, const double*, const double*, double,     \
-                                            long long, const void*, double*, double*, hipStream_t);
+                                            long long, const void*, double*, double*, hipStream_t); \
+  template hipError_t launch_star_step_binned<T>(const Dims&, double*, double*, long long,          \
+                                                 const void*, double, double, double, int,         \
+                                                 const StarBins&, hipStream_t);                    \
+  template hipError_t launch_tile_deposit<T>(const Dims&, const double*, const double*, double,    \
+                                             long long, const StarBins&, void*, hipStream_t);
 
 QJ_PINST(float)
 QJ_PINST(double)

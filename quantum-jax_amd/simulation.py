@@ -42,6 +42,7 @@ class KDKSimulation:
         self.mass_t = None
         self.m_scalar = 0.0
         self.rho_s = None
+        self.star_scratch = None
         if self.np:
             self.pos = plan.particles(pos, "pos").clone()
             self.vel = plan.particles(vel, "vel").clone()
@@ -50,6 +51,9 @@ class KDKSimulation:
             else:
                 self.m_scalar = float(mass)
             self.rho_s = torch.zeros(plan.shape, dtype=rdt, device=dev)
+            # tile-binned deposit scratch (counters must start at zero)
+            nb = int(_lib.lib().qj_star_scratch_size(plan.handle, self.np))
+            self.star_scratch = torch.zeros(nb, dtype=torch.uint8, device=dev)
         self.gas = None
         self._gas_tmp = None
         if gas is not None:
@@ -78,6 +82,7 @@ class KDKSimulation:
         st.gas_swapped = 0
         st.pending = 0.0
         st.pending_star = 0.0
+        st.star_scratch = ptr(self.star_scratch).value
         self._st = st
         self._params = _lib.QjParams(self.dt, phys.m_per_hbar, phys.four_pi_G, phys.rho_bar,
                                      phys.b_coeff, phys.cs, phys.dc_mode)
