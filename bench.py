@@ -119,13 +119,14 @@ def ensemble_leg(qj, dev, phys, nsteps=50):
     dts = rng.uniform(2e-4, 1e-3, len(sizes))
     n_p = 1 << 14
     log(f"ensemble: {len(sizes)} boxes (16x64^3, 24x128^3, 8x256^3), {n_p} stars each")
-    sims = []
+    sims, ics = [], []
     for b, (n, dtb) in enumerate(zip(sizes, dts)):
         pl = qj.Plan(n, dtype=torch.complex64, device=dev)
         x, v, m = qj.ic.star_ic(n_p, 1017 + b)
-        sims.append(qj.KDKSimulation(pl, phys, float(dtb), gpu_psi((n, n, n), 17 + b, dev),
-                                     torch.as_tensor(x).to(dev), torch.as_tensor(v).to(dev),
-                                     mass=m))
+        ic = (gpu_psi((n, n, n), 17 + b, dev), torch.as_tensor(x).to(dev),
+              torch.as_tensor(v).to(dev), torch.full((n_p,), m, dtype=torch.float64, device=dev))
+        ics.append(ic)
+        sims.append(qj.KDKSimulation(pl, phys, float(dtb), ic[0], ic[1], ic[2], mass=m))
     cells = sum(n ** 3 for n in sizes)
     ens = qj.Ensemble(sims, mode="eager")
     ens.step(1)
@@ -157,6 +158,52 @@ def ensemble_leg(qj, dev, phys, nsteps=50):
     groups = [(len(m), nl) for m, nl in ens.groups()]
     ens.close()
     del ens, sims
+    torch.cuda.empty_cache()
+
+    # forward + adjoint of the same 48 boxes (checkpoint every 10 steps)
+    ad = importlib.import_module("quantum-jax_amd.adjoint")
+    seg = 10
+    plans = [qj.Plan(n, dtype=torch.complex64, device=dev) for n in sizes]
+    tgts = [gpu_psi((n, n, n), 5017 + b, dev).abs().pow(2).to(torch.float32)
+            for b, n in enumerate(sizes)]
+
+    def run(grad):
+        ps = [ic[0].clone().requires_grad_(grad) for ic in ics]
+        ms = [ic[3].clone().requires_grad_(grad) for ic in ics]
+        with torch.set_grad_enabled(grad):
+            po, xo, _ = ad.ensemble_kdk_run(ps, [ic[1] for ic in ics], [ic[2] for ic in ics], ms,
+                                            plans, [phys] * len(sizes), list(dts), nsteps, seg,
+                                            mode=tuned["mode"])
+            L = sum(ad.density_loss(po[b], xo[b], ms[b], tgts[b], plans[b])
+                    for b in range(len(sizes)))
+            if grad:
+                L.backward()
+                ok = all(bool(torch.isfinite(p.grad).all()) for p in ps)
+                return ok
+        return True
+
+    log(f"ensemble adjoint: {nsteps} steps, checkpoint every {seg}")
+    run(False)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    run(False)
+    torch.cuda.synchronize()
+    t_f = time.perf_counter() - t
+    run(True)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    ok = run(True)
+    torch.cuda.synchronize()
+    t_fb = time.perf_counter() - t
+    log(f"ensemble adjoint: forward {1e3 * t_f / nsteps:.3f} ms/step, forward + adjoint "
+        f"{1e3 * t_fb / nsteps:.3f} ms/step")
+    adj = {"workload": f"same 48 boxes, {nsteps} steps, checkpoint every {seg}, loss sum over boxes "
+                       "of sum((|psi|^2 + rho_s - rho_target)^2), grad w.r.t. psi0 and masses",
+           "forward_ms_per_step": 1e3 * t_f / nsteps,
+           "forward_plus_adjoint_ms_per_step": 1e3 * t_fb / nsteps,
+           "cost_ratio": t_fb / t_f, "grad_finite": ok}
+    del plans, tgts, ics
+    torch.cuda.empty_cache()
     return {
         "workload": "48 complex64 boxes: 16x64^3 + 24x128^3 + 8x256^3 (188,743,680 cells), "
                     f"2^14 FP64 stars each, per-box seed 17+b, dt ~ U[2e-4, 1e-3], {nsteps} steps",
@@ -166,6 +213,7 @@ def ensemble_leg(qj, dev, phys, nsteps=50):
         "autotuned": {"mode": tuned["mode"], "fuse_x_source": tuned["fuse_x_source"],
                       "tuning_s": t_tune},
         "launch_groups": groups,
+        "adjoint": adj,
     }
 
 

@@ -246,6 +246,13 @@ int qj_ensemble_destroy(qj_ensemble* ens);
 int qj_ensemble_set_mode(qj_ensemble* ens, int mode);
 int qj_ensemble_steps(qj_ensemble* ens, int nsteps, void* stream);
 int qj_ensemble_finalize(qj_ensemble* ens, void* stream);
+/* qj_kdk_record and qj_kdk_adjoint of every member (tapes[m] holds nsteps
+ * steps of member m; grid0s and its entries nullable; adjs[m] is member m's
+ * adjoint), recorded, batched and replayed like qj_ensemble_steps. */
+int qj_ensemble_record(qj_ensemble* ens, int nsteps, qj_tape_step* const* tapes,
+                       void* const* grid0s, void* stream);
+int qj_ensemble_adjoint(qj_ensemble* ens, int nsteps, qj_tape_step* const* tapes,
+                        void* const* grid0s, qj_adjoint* adjs, void* stream);
 /* Groups of the last recorded step: count, then per group its members
  * (written to members[0..cap)), member count and launches per step. */
 int qj_ensemble_group_count(const qj_ensemble* ens);

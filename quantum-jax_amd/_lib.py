@@ -89,6 +89,9 @@ SIGNATURES = {
     "qj_ensemble_set_mode": (i32, [vp, i32]),
     "qj_ensemble_steps": (i32, [vp, i32, vp]),
     "qj_ensemble_finalize": (i32, [vp, vp]),
+    "qj_ensemble_record": (i32, [vp, i32, C.POINTER(C.POINTER(QjTapeStep)), C.POINTER(vp), vp]),
+    "qj_ensemble_adjoint": (i32, [vp, i32, C.POINTER(C.POINTER(QjTapeStep)), C.POINTER(vp),
+                                  C.POINTER(QjAdjoint), vp]),
     "qj_ensemble_group_count": (i32, [vp]),
     "qj_ensemble_group_info": (i32, [vp, i32, C.POINTER(C.c_int), i32, C.POINTER(C.c_int),
                                      C.POINTER(C.c_int)]),

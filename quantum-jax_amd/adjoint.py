@@ -151,3 +151,100 @@ def density_loss(psi, pos, mass, rho_target, plan: Plan):
     the total Poisson source (differentiable)."""
     rho = psi.real ** 2 + psi.imag ** 2 + bin_particles_ad(pos, mass, plan)
     return ((rho - rho_target) ** 2).sum()
+
+
+class _EnsembleKDKRun(torch.autograd.Function):
+    """nsteps KDK steps of many independent members through one Ensemble.
+
+    Inputs are the members' (psi, pos, vel, mass) tensors, flattened as
+    psis + poss + vels + masses; outputs are the final psis + poss + vels.
+    The backward restores each segment checkpoint into the same member
+    buffers (so the recorded ensemble programs repeat across segments) and
+    runs the batched tape recording and adjoint (qj_ensemble_record /
+    qj_ensemble_adjoint).
+    """
+
+    @staticmethod
+    def forward(ctx, meta, *flat):
+        from .ensemble import Ensemble
+        M = meta["n"]
+        psis, poss, vels, masses = (flat[k * M:(k + 1) * M] for k in range(4))
+        sims = [KDKSimulation(meta["plans"][m], meta["phys"][m], meta["dts"][m], psis[m].detach(),
+                              poss[m].detach(), vels[m].detach(), mass=masses[m].detach())
+                for m in range(M)]
+        ens = Ensemble(sims, mode=meta["mode"])
+        nsteps, seg = meta["nsteps"], meta["segment"]
+        ckpts = []
+        done = 0
+        while done < nsteps:
+            ens.finalize()  # checkpoints hold end-of-step states (owed kicks applied)
+            ckpts.append([(s._psi.clone(), s.pos.clone(), s.vel.clone()) for s in sims])
+            n = min(seg, nsteps - done)
+            ens.step(n)
+            done += n
+        ens.finalize()
+        ctx.meta, ctx.ens, ctx.sims, ctx.ckpts = meta, ens, sims, ckpts
+        ctx.save_for_backward(*[m.detach() for m in masses])
+        outs = [s._psi.clone() for s in sims] + [s.pos.clone() for s in sims] + \
+               [s.vel.clone() for s in sims]
+        return tuple(outs)
+
+    @staticmethod
+    def backward(ctx, *grads):
+        meta, ens, sims = ctx.meta, ctx.ens, ctx.sims
+        M, nsteps, seg = meta["n"], meta["nsteps"], meta["segment"]
+        masses = ctx.saved_tensors
+        L = _lib.lib()
+        adj_t, adj_s = [], (_lib.QjAdjoint * M)()
+        for m, s in enumerate(sims):
+            plan = s.plan
+            gp, gx, gv = grads[m], grads[M + m], grads[2 * M + m]
+            pb = (gp if gp is not None else torch.zeros_like(s._psi)).contiguous().clone()
+            xb = (gx if gx is not None else torch.zeros_like(s.pos)).contiguous().clone()
+            vb = (gv if gv is not None else torch.zeros_like(s.vel)).contiguous().clone()
+            mb = torch.zeros_like(masses[m])
+            sc = torch.empty(int(L.qj_adjoint_scratch_size(plan.handle)), dtype=torch.uint8,
+                             device=plan.device)
+            adj_t.append((pb, xb, vb, mb, sc))
+            adj_s[m] = _lib.QjAdjoint(pb.data_ptr(), xb.data_ptr(), vb.data_ptr(), mb.data_ptr(),
+                                      sc.data_ptr())
+        tapes = [_Tape(s.plan, min(seg, nsteps), s.np) for s in sims]
+        tape_ptrs = (C.POINTER(_lib.QjTapeStep) * M)(
+            *[C.cast(t.steps, C.POINTER(_lib.QjTapeStep)) for t in tapes])
+        grid0s = (_lib.vp * M)(*[t.grid0.data_ptr() for t in tapes])
+        starts = list(range(0, nsteps, seg))
+        with torch.cuda.device(ens.device):
+            stream = C.c_void_p(torch.cuda.current_stream(ens.device).cuda_stream)
+            for s0, ck in reversed(list(zip(starts, ctx.ckpts))):
+                nst = min(seg, nsteps - s0)
+                for s, (p, x, v) in zip(sims, ck):
+                    s._psi.copy_(p)
+                    s.pos.copy_(x)
+                    s.vel.copy_(v)
+                    st = s._st
+                    st.grid_valid, st.gas_swapped, st.pending, st.pending_star = 0, 0, 0.0, 0.0
+                _lib.check(L.qj_ensemble_record(ens._h, nst, tape_ptrs, grid0s, stream),
+                           "qj_ensemble_record")
+                _lib.check(L.qj_ensemble_adjoint(ens._h, nst, tape_ptrs, grid0s, adj_s, stream),
+                           "qj_ensemble_adjoint")
+        ctx.ckpts = None
+        ens.close()
+        res = [None] + [a[0] for a in adj_t] + [a[1] for a in adj_t] + [a[2] for a in adj_t] + \
+              [a[3] for a in adj_t]
+        return tuple(res)
+
+
+def ensemble_kdk_run(psis, poss, vels, masses, plans, physs, dts, nsteps: int,
+                     segment: int = 16, mode: str = "graph"):
+    """Differentiable nsteps KDK steps of M independent boxes, stepped as one
+    Ensemble (batched per shape, graph-replayed) forward and backward.
+
+    psis, poss, vels, masses : per-member tensors (masses float64 (np,));
+    plans : one Plan per member; physs, dts : per-member Physics and dt.
+    Returns (psis_out, poss_out, vels_out), lists of tensors.
+    """
+    M = len(psis)
+    meta = {"n": M, "plans": list(plans), "phys": list(physs), "dts": [float(d) for d in dts],
+            "nsteps": int(nsteps), "segment": int(segment), "mode": mode}
+    out = _EnsembleKDKRun.apply(meta, *psis, *poss, *vels, *masses)
+    return list(out[:M]), list(out[M:2 * M]), list(out[2 * M:])

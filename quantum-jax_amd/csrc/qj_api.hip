@@ -62,6 +62,34 @@ hipError_t fill(void* a, long long n, T v, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Device-to-device copy as a kernel (recordable by the launch recorder, so
+// tape copies can be batched and graph-captured with the rest of a step).
+template <class V>
+struct k_copy {
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(V* __restrict__ dst, const V* __restrict__ src,
+                                             long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+  }
+};
+
+hipError_t copy_async(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return hipSuccess;
+  const uintptr_t a = (uintptr_t)dst | (uintptr_t)src;
+  if ((a | bytes) % 8 != 0) return hipErrorInvalidValue;  // every tape buffer is 8-byte data
+  const bool wide = (a | bytes) % 16 == 0;
+  const long long n = (long long)(bytes / (wide ? 16 : 8));
+  long long blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  if (wide)
+    launch<k_copy<uint4>>((int)blocks, 256, s, (uint4*)dst, (const uint4*)src, n);
+  else
+    launch<k_copy<uint2>>((int)blocks, 256, s, (uint2*)dst, (const uint2*)src, n);
+  return hipGetLastError();
+}
+
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // Kernel stages that the optional event timer tells apart.
@@ -664,19 +692,15 @@ int kdk_record_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, qj_t
   const size_t xb = (size_t)st->np * 3 * sizeof(double);
   if (!st->grid_valid)
     if (int rc = initial_grid_t<T>(p, st, pr, w, s)) return rc;
-  if (grid0) QJ_CHECK(hipMemcpyAsync(grid0, st->grid, grid_bytes(p), hipMemcpyDeviceToDevice, s),
-                      "tape grid0");
+  if (grid0) QJ_CHECK(copy_async(grid0, st->grid, grid_bytes(p), s), "tape grid0");
   for (int k = 0; k < nsteps; ++k) {
     qj_tape_step& t = tape[k];
-    QJ_CHECK(hipMemcpyAsync(t.psi0, st->psi, cb, hipMemcpyDeviceToDevice, s), "tape psi0");
-    if (st->np > 0)
-      QJ_CHECK(hipMemcpyAsync(t.x0, st->pos, xb, hipMemcpyDeviceToDevice, s), "tape x0");
+    QJ_CHECK(copy_async(t.psi0, st->psi, cb, s), "tape psi0");
+    if (st->np > 0) QJ_CHECK(copy_async(t.x0, st->pos, xb, s), "tape x0");
     if (int rc = kdk_steps_t<T>(p, st, pr, 1, work, s)) return rc;
-    QJ_CHECK(hipMemcpyAsync(t.psi2, st->psi, cb, hipMemcpyDeviceToDevice, s), "tape psi2");
-    QJ_CHECK(hipMemcpyAsync(t.grid1, st->grid, grid_bytes(p), hipMemcpyDeviceToDevice, s),
-             "tape grid1");
-    if (st->np > 0)
-      QJ_CHECK(hipMemcpyAsync(t.x1, st->pos, xb, hipMemcpyDeviceToDevice, s), "tape x1");
+    QJ_CHECK(copy_async(t.psi2, st->psi, cb, s), "tape psi2");
+    QJ_CHECK(copy_async(t.grid1, st->grid, grid_bytes(p), s), "tape grid1");
+    if (st->np > 0) QJ_CHECK(copy_async(t.x1, st->pos, xb, s), "tape x1");
     if (int rc = kdk_finalize_t<T>(p, st, pr, s)) return rc;
   }
   return QJ_OK;

@@ -1,7 +1,9 @@
 // Ensemble runner: many independent boxes stepped together.
 //
-// One ensemble step is recorded per member by running qj_kdk_steps under a
-// launch Recorder (qj_launch.h): nothing is issued, every launch is kept as
+// An ensemble operation (one step; a segment recorded with its adjoint tape;
+// the adjoint over a tape) is recorded per member by running the member's
+// entry point (qj_kdk_steps, qj_kdk_record, qj_kdk_adjoint) under a launch
+// Recorder (qj_launch.h): nothing is issued, every launch is kept as
 // (batched wrapper, grid, block, argument bytes).  Members whose recorded
 // sequences match launch for launch (same kernels, grids and blocks: same
 // grid size, precision, step kind and schedule options) form a group; each
@@ -23,7 +25,7 @@ using namespace qj;
 
 namespace {
 
-constexpr int kMaxPrograms = 6;
+constexpr int kMaxPrograms = 12;
 
 size_t align_up(size_t b, size_t a) { return (b + a - 1) / a * a; }
 
@@ -38,6 +40,8 @@ struct Group {
   std::vector<int> members;
   std::vector<BatchLaunch> launches;
 };
+
+enum Op { kOpStep = 1, kOpRecord = 2, kOpAdjoint = 3 };
 
 struct Program {
   std::vector<unsigned char> key;
@@ -111,10 +115,12 @@ struct qj_ensemble {
     if (cap) (void)hipStreamDestroy(cap);
   }
 
-  // Entry key: every member's state, params and schedule options.
-  std::vector<unsigned char> key() const {
-    std::vector<unsigned char> k;
-    k.reserve((size_t)n * (sizeof(qj_state) + sizeof(qj_params) + 4 * QJ_OPT_COUNT));
+  // Entry key: the operation and its arguments (extra), then every member's
+  // state, params and schedule options.
+  std::vector<unsigned char> key(const std::vector<unsigned char>& extra = {}) const {
+    std::vector<unsigned char> k(extra);
+    k.reserve(extra.size() +
+              (size_t)n * (sizeof(qj_state) + sizeof(qj_params) + 4 * QJ_OPT_COUNT));
     auto put = [&k](const void* p, size_t b) {
       const unsigned char* c = (const unsigned char*)p;
       k.insert(k.end(), c, c + b);
@@ -131,7 +137,9 @@ struct qj_ensemble {
     return k;
   }
 
-  int build(Program* pr, hipStream_t s) {
+  // member(m, state copy, stream) issues member m's part of the operation
+  template <class F>
+  int build(Program* pr, F&& member, hipStream_t s) {
     std::vector<std::vector<LaunchRec>> recs(n);
     pr->post.resize(n);
     for (int m = 0; m < n; ++m) {
@@ -140,7 +148,7 @@ struct qj_ensemble {
       int rc;
       {
         RecordScope scope(&R);
-        rc = qj_kdk_steps(plans[m], &st, &params[m], 1, works[m], s);
+        rc = member(m, &st, s);
       }
       if (rc) return rc;
       recs[m] = std::move(R.recs);
@@ -184,7 +192,9 @@ struct qj_ensemble {
     return QJ_OK;
   }
 
-  Program* find_or_build(const std::vector<unsigned char>& k, hipStream_t s, int* rc) {
+  template <class F>
+  Program* find_or_build(const std::vector<unsigned char>& k, F&& member, hipStream_t s,
+                         int* rc) {
     *rc = QJ_OK;
     for (Program* p : progs)
       if (p->key == k) return p;
@@ -203,7 +213,7 @@ struct qj_ensemble {
     }
     Program* p = new Program();
     p->key = k;
-    if ((*rc = build(p, s))) {
+    if ((*rc = build(p, member, s))) {
       delete p;
       return nullptr;
     }
@@ -269,9 +279,11 @@ struct qj_ensemble {
     return QJ_OK;
   }
 
-  int step(hipStream_t s) {
+  // Runs one recorded operation on all members (batched, or from its graph).
+  template <class F>
+  int run(const std::vector<unsigned char>& extra, F&& member, hipStream_t s) {
     int rc;
-    Program* pr = find_or_build(key(), s, &rc);
+    Program* pr = find_or_build(key(extra), member, s, &rc);
     if (!pr) return rc;
     pr->last_use = ++clock;
     if (mode == QJ_ENS_BATCHED) {
@@ -285,7 +297,27 @@ struct qj_ensemble {
     last = pr;
     return QJ_OK;
   }
+
+  int step(hipStream_t s) {
+    const int op = kOpStep;
+    std::vector<unsigned char> extra((const unsigned char*)&op, (const unsigned char*)(&op + 1));
+    return run(
+        extra,
+        [this](int m, qj_state* st, hipStream_t ss) {
+          return qj_kdk_steps(plans[m], st, &params[m], 1, works[m], ss);
+        },
+        s);
+  }
 };
+
+namespace {
+
+void put_bytes(std::vector<unsigned char>& k, const void* p, size_t b) {
+  const unsigned char* c = (const unsigned char*)p;
+  k.insert(k.end(), c, c + b);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -340,6 +372,63 @@ int qj_ensemble_steps(qj_ensemble* e, int nsteps, void* stream) {
   for (int k = 0; k < nsteps; ++k)
     if (int rc = e->step(s)) return rc;
   return QJ_OK;
+}
+
+int qj_ensemble_record(qj_ensemble* e, int nsteps, qj_tape_step* const* tapes,
+                       void* const* grid0s, void* stream) {
+  if (!e) return set_error(QJ_EARG, "null ensemble");
+  if (nsteps < 0 || (nsteps > 0 && !tapes))
+    return set_error(QJ_EARG, "qj_ensemble_record: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  auto member = [e, nsteps, tapes, grid0s](int m, qj_state* st, hipStream_t ss) {
+    return qj_kdk_record(e->plans[m], st, &e->params[m], nsteps, tapes[m],
+                         grid0s ? grid0s[m] : nullptr, e->works[m], ss);
+  };
+  if (e->mode == QJ_ENS_EAGER) {
+    for (int m = 0; m < e->n; ++m)
+      if (int rc = member(m, e->states[m], s)) return rc;
+    return QJ_OK;
+  }
+  std::vector<unsigned char> extra;
+  const int op = kOpRecord;
+  put_bytes(extra, &op, sizeof(op));
+  put_bytes(extra, &nsteps, sizeof(nsteps));
+  for (int m = 0; m < e->n; ++m) {
+    if (!tapes[m]) return set_error(QJ_EARG, "qj_ensemble_record: null tape");
+    put_bytes(extra, tapes[m], sizeof(qj_tape_step) * (size_t)nsteps);
+    const void* g0 = grid0s ? grid0s[m] : nullptr;
+    put_bytes(extra, &g0, sizeof(g0));
+  }
+  return e->run(extra, member, s);
+}
+
+int qj_ensemble_adjoint(qj_ensemble* e, int nsteps, qj_tape_step* const* tapes,
+                        void* const* grid0s, qj_adjoint* adjs, void* stream) {
+  if (!e) return set_error(QJ_EARG, "null ensemble");
+  if (nsteps < 0 || !adjs || (nsteps > 0 && !tapes))
+    return set_error(QJ_EARG, "qj_ensemble_adjoint: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  auto member = [e, nsteps, tapes, grid0s, adjs](int m, qj_state* st, hipStream_t ss) {
+    return qj_kdk_adjoint(e->plans[m], st, &e->params[m], nsteps, tapes[m],
+                          grid0s ? grid0s[m] : nullptr, &adjs[m], e->works[m], ss);
+  };
+  if (e->mode == QJ_ENS_EAGER) {
+    for (int m = 0; m < e->n; ++m)
+      if (int rc = member(m, e->states[m], s)) return rc;
+    return QJ_OK;
+  }
+  std::vector<unsigned char> extra;
+  const int op = kOpAdjoint;
+  put_bytes(extra, &op, sizeof(op));
+  put_bytes(extra, &nsteps, sizeof(nsteps));
+  for (int m = 0; m < e->n; ++m) {
+    if (!tapes[m]) return set_error(QJ_EARG, "qj_ensemble_adjoint: null tape");
+    put_bytes(extra, tapes[m], sizeof(qj_tape_step) * (size_t)nsteps);
+    const void* g0 = grid0s ? grid0s[m] : nullptr;
+    put_bytes(extra, &g0, sizeof(g0));
+    put_bytes(extra, &adjs[m], sizeof(qj_adjoint));
+  }
+  return e->run(extra, member, s);
 }
 
 int qj_ensemble_finalize(qj_ensemble* e, void* stream) {
