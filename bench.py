@@ -57,10 +57,14 @@ def cpu_model():
 
 
 def stage_bytes(name, nx, ny, nz, c, r, n_p, stars):
-    """Algorithmic HBM bytes of one launch of a stage (read + write once)."""
+    """Algorithmic HBM bytes of one launch of a stage (read + write once).
+
+    Stars use the tile-binned deposit: it overwrites every cell of rho_s
+    (N r), so the source pass only reads rho_s; its other bytes are the
+    star positions and registrations (about 64 B per star)."""
     N = nx * ny * nz
     Nh = (nx // 2 + 1) * ny * nz
-    src = 2 * r if stars else r          # stars: read rho_s and clear it
+    src = r
     table = {
         "drift_z_kick": N * (2 * c + r),
         "drift_z": 2 * N * c, "drift_y": 2 * N * c, "drift_x": 2 * N * c,
@@ -70,8 +74,8 @@ def stage_bytes(name, nx, ny, nz, c, r, n_p, stars):
         "poisson_z": 3 * Nh * c,
         "poisson_y": 5 * Nh * c,
         "poisson_x": 3 * Nh * c + 5 * N * r,      # V plane + (a_x, a_y, a_z, V) cells
-        "deposit": n_p * (24 + 8 * r),
-        "star_step": n_p * (96 + 8 * 4 * r + 8 * r),
+        "deposit": N * r + n_p * 64,
+        "star_step": n_p * (96 + 8 * 4 * r + 8),
         "star_kick": n_p * (72 + 8 * 4 * r),
         "kick": N * (2 * c + r),
     }
