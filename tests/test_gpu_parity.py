@@ -378,3 +378,43 @@ def test_tile_binned_deposit_matches_atomic_deposit(qj, cuda, dtype, g):
         s = O.compute_step_stars(*s, 1e-3, m, g, ph)
     assert rel(out[0][0], s[0]) < TOLS[dtype]
     assert rel(out[0][1], s[1]) < TOLS[dtype]
+
+
+_C1 = {}
+
+
+def _c1_oracle():
+    if not _C1:
+        g = O.Grid(64, 64, 64)
+        ph = O.Physics(m_per_hbar=1.0, four_pi_G=1.0, rho_bar=1.0)
+        psi0 = O.filtered_gaussian_psi(g, 17)
+        p = psi0
+        for _ in range(100):
+            p = O.compute_step_plain(p, 1e-3, g, ph)
+        _C1.update(g=g, psi0=psi0, p=p, V=O.get_potential(np.abs(p) ** 2, g, ph))
+    return _C1["g"], _C1["psi0"], _C1["p"], _C1["V"]
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+def test_plain_config_64_cubed_100_steps(qj, cuda, dtype):
+    # the plain configuration at its full size: 64^3, hbar/m = 1, 4 pi G = 1,
+    # dt = 1e-3, 100 steps, filtered-Gaussian psi (sigma_k = 16 pi, seed 17);
+    # outputs psi, |psi|^2 and V = get_potential(|psi|^2)
+    g, psi0, p, V = _c1_oracle()
+    plan = plan_for(qj, g, dtype)
+    phys = qj.Physics(m_per_hbar=1.0, four_pi_G=1.0, rho_bar=1.0)
+    sim = qj.KDKSimulation(plan, phys, 1e-3, dev(psi0, dtype))
+    sim.step(100)
+    psi = sim.psi
+    tol = {C128: 1e-10, C64: 1e-4}[dtype]
+    assert rel(psi, p) < tol
+    assert rel(psi.abs() ** 2, np.abs(p) ** 2) < tol
+    Vg = qj.get_potential(psi.abs() ** 2, plan, phys).double()
+    if dtype == C128:
+        assert rel(Vg, V) < 10 * tol
+    else:
+        # the k=0 rule turns the c64 mass drift (~1e-5) into a uniform offset of
+        # V: compare the zero-mean part, and the mean with the device's own mass
+        assert rel(Vg - Vg.mean(), V - V.mean()) < 10 * tol
+        mean_rho = (psi.abs() ** 2).double().mean().item()
+        assert Vg.mean().item() == pytest.approx(-1.0 * (mean_rho - 1.0), abs=1e-7)
