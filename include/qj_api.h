@@ -128,8 +128,7 @@ int qj_solve_hydro(qj_plan* plan, const void* rho, const void* vx, const void* v
  */
 typedef struct {
   void* psi;        /* complex N                                              */
-  void* rho_s;      /* real N, star density scratch (must start zeroed unless
-                       star_scratch is given)                                 */
+  void* rho_s;      /* real N, star density scratch                          */
   void* gas[4];     /* rho, vx, vy, vz: real N each (gas step) or NULL        */
   void* gas_tmp[4]; /* second buffers for the out-of-place gas update         */
   double* pos;      /* (np,3)                                                 */
@@ -145,10 +144,10 @@ typedef struct {
   int gas_swapped;  /* the live gas fields are gas_tmp (after an odd count)   */
   double pending;   /* time of the owed kick of psi and gas velocity          */
   double pending_star; /* time of the owed closing kick of the star velocities */
-  /* optional: qj_star_scratch_size() bytes, zeroed once before first use.
-   * When given, the star deposit is tile-binned (per-tile LDS accumulation
-   * and plain stores, no global float atomics) and rho_s needs no clearing;
-   * when NULL, the deposit adds with float atomics into rho_s. */
+  /* optional: qj_star_scratch_size() bytes.  When given (and
+   * QJ_OPT_BINNED_DEPOSIT is on), the star deposit is tile-binned: per-tile
+   * LDS accumulation and plain stores, no global float atomics; otherwise
+   * the deposit adds with float atomics into rho_s after clearing it. */
   void* star_scratch;
 } qj_state;
 
@@ -215,7 +214,11 @@ int qj_deposit_adjoint(qj_plan* plan, const double* pos, const double* mass, dou
  * run as one kernel (fewer HBM bytes, more work per workgroup); 0 = two
  * kernels.  Results are identical; the autotuner picks per grid. */
 #define QJ_OPT_FUSE_X_SOURCE 0
-#define QJ_OPT_COUNT 1
+/* QJ_OPT_BINNED_DEPOSIT: with qj_state.star_scratch given, 1 = tile-binned
+ * star deposit (default), 0 = float-atomic deposit (cheaper for sparse
+ * stars on small grids).  Results agree to rounding. */
+#define QJ_OPT_BINNED_DEPOSIT 1
+#define QJ_OPT_COUNT 2
 int qj_plan_set_option(qj_plan* plan, int option, int value);
 int qj_plan_get_option(const qj_plan* plan, int option, int* value);
 

@@ -17,6 +17,7 @@ QJ_C64 = 0
 QJ_C128 = 1
 DIAG_AMAX, DIAG_DTS, DIAG_MASS, DIAG_NAN, DIAG_COUNT = 0, 1, 2, 3, 8
 OPT_FUSE_X_SOURCE = 0
+OPT_BINNED_DEPOSIT = 1
 ENS_EAGER, ENS_BATCHED, ENS_GRAPH, ENS_GRAPH_FORK = 0, 1, 2, 3
 
 vp = C.c_void_p

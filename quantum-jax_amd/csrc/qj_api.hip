@@ -117,7 +117,7 @@ struct qj_plan {
   double drift_key_dt = NAN, drift_key_mph = NAN;
   double* partials = nullptr;
   int nblk_src = 0;
-  int opt[QJ_OPT_COUNT] = {1};  // QJ_OPT_FUSE_X_SOURCE on  // schedule options (qj_plan_set_option)
+  int opt[QJ_OPT_COUNT] = {1, 1};  // schedule options (qj_plan_set_option): fused x, binned
   // optional per-stage event timing (tracing hook of the bench harness)
   int timing = 0;
   struct Rec {
@@ -538,7 +538,8 @@ int initial_grid_t(qj_plan* p, qj_state* st, const qj_params* pr, const Work& w,
   const bool gas = st->gas[0] != nullptr;
   void* A4 = (stars || gas) ? plane<T>(st->grid, p, 1) : nullptr;
   void* brho = (pr->b_coeff != 0.0) ? st->brho : nullptr;
-  if (stars && st->star_scratch) {
+  const bool binned = st->star_scratch && p->opt[QJ_OPT_BINNED_DEPOSIT];
+  if (stars && binned) {
     // tile-binned deposit of the current positions (overwrites rho_s)
     StarBins sb = carve_star_scratch(p->d, st->np, st->star_scratch);
     QJ_CHECK(fill<int>(sb.count, sb.ntiles, 0, s), "bin reset");
@@ -551,12 +552,13 @@ int initial_grid_t(qj_plan* p, qj_state* st, const qj_params* pr, const Work& w,
                                     s),
              "deposit");
   } else if (stars) {
+    // the float-atomic deposit accumulates into a cleared plane
+    QJ_CHECK(fill<T>(st->rho_s, p->ncell(), (T)0, s), "rho_s clear");
     QJ_STAGE(kStageDeposit,
              launch_deposit<T>(p->d, st->pos, st->mass, st->m_scalar, st->np, st->rho_s, s),
              "deposit");
   }
-  // the atomic deposit accumulates, so its plane is cleared after reading
-  const int clear = (stars && !st->star_scratch) ? 1 : 0;
+  const int clear = 0;  // both deposits leave rho_s as the source of this state
   void* src = stars ? st->rho_s
                     : (gas ? (st->gas_swapped ? st->gas_tmp[0] : st->gas[0]) : nullptr);
   QJ_STAGE(kStageSourceX,
@@ -586,8 +588,8 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
   auto live_gas = [&](int k) { return st->gas_swapped ? st->gas_tmp[k] : st->gas[k]; };
   auto spare_gas = [&](int k) { return st->gas_swapped ? st->gas[k] : st->gas_tmp[k]; };
   void* src = nullptr;
-  // the atomic deposit accumulates, so its plane is cleared after reading
-  const int clear = (stars && !st->star_scratch) ? 1 : 0;
+  const bool binned = st->star_scratch && p->opt[QJ_OPT_BINNED_DEPOSIT];
+  const int clear = 0;  // both deposits leave rho_s as the source of this state
 
   for (int step = 0; step < nsteps; ++step) {
     if (!st->grid_valid) {
@@ -596,7 +598,7 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
     const double kick_t = st->pending + 0.5 * dt;  // owed half kick + this step's first
     // stars: owed closing kick of the previous step and this step's opening
     // half kick (same grid, same positions), drift, deposit at the new positions
-    if (stars && st->star_scratch) {
+    if (stars && binned) {
       StarBins sb = carve_star_scratch(p->d, st->np, st->star_scratch);
       QJ_STAGE(kStageStarStep,
                launch_star_step_binned<T>(p->d, st->pos, st->vel, st->np, A4, st->pending_star,
@@ -607,6 +609,8 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
                                       st->rho_s, s),
                "deposit");
     } else if (stars) {
+      // the float-atomic deposit accumulates into a cleared plane
+      QJ_CHECK(fill<T>(st->rho_s, p->ncell(), (T)0, s), "rho_s clear");
       QJ_STAGE(kStageStarStep,
                launch_star_step<T>(p->d, st->pos, st->vel, st->np, A4, st->pending_star, 0.5 * dt,
                                    dt, st->mass, st->m_scalar, st->rho_s, s),

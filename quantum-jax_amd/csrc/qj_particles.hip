@@ -466,6 +466,9 @@ TileGeom make_tiles(const Dims& d) {
   t.lx = lg(d.nx < kTileX ? d.nx : kTileX);
   t.ly = lg(d.ny < kTileY ? d.ny : kTileY);
   t.lz = lg(d.nz < kTileZ ? d.nz : kTileZ);
+  // small grids: shorter z extents (down to 8) so that the per-tile deposit
+  // still has about 2048 blocks (8 per CU)
+  while (t.lz > 3 && ((d.nx >> t.lx) * (d.ny >> t.ly) * (d.nz >> t.lz)) < 2048) --t.lz;
   t.ntx = d.nx >> t.lx;
   t.nty = d.ny >> t.ly;
   t.ntz = d.nz >> t.lz;

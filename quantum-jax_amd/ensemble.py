@@ -9,7 +9,8 @@ launch per stage (blockIdx.z = member), and the step is replayed from a
 hipGraph.  Results are bitwise identical to stepping each member alone.
 
 ``autotune`` times the schedule candidates on the live members (execution
-mode, and per shape group the x-sweep schedule) and keeps the fastest;
+mode, and per shape group the x-sweep schedule and, with stars, the
+tile-binned or float-atomic deposit) and keeps the fastest;
 member states are saved before and restored after, so tuning does not
 advance the simulation.
 """
@@ -173,16 +174,22 @@ class Ensemble:
         torch.cuda.synchronize(self.device)
         return a.elapsed_time(b) / reps
 
+    # run-time schedule options the autotuner searches: (result key, option,
+    # applies to the group of this shape key)
+    KNOBS = (("fuse_x_source", _lib.OPT_FUSE_X_SOURCE, lambda key: True),
+             ("binned_deposit", _lib.OPT_BINNED_DEPOSIT, lambda key: key[2] == "stars"))
+
     def autotune(self, reps: int = 3, modes=("batched", "graph", "graph_fork"),
                  cache_path: str | None = None, verbose: bool = False) -> dict:
-        """Pick the execution mode and, per shape group, the x-sweep schedule.
+        """Pick the execution mode and, per shape group, the schedule options.
 
         Coordinate search: the mode is chosen with every group on its current
-        schedule, then each group's QJ_OPT_FUSE_X_SOURCE is flipped in turn
-        and kept when the ensemble step gets faster.  Results are cached per
-        ensemble signature in this process (and in cache_path, a JSON file,
+        options, then each group's options (the x-sweep schedule; for star
+        groups also the tile-binned vs float-atomic deposit) are flipped in
+        turn and kept when the ensemble step gets faster.  Results are cached
+        per ensemble signature in this process (and in cache_path, a JSON file,
         when given).  Returns {"mode", "fuse_x_source": {group: 0|1},
-        "ms_per_step", "trials"}.
+        "binned_deposit": {group: 0|1}, "ms_per_step", "trials"}.
         """
         sig = self.signature()
         cache = dict(_TUNED)
@@ -206,26 +213,29 @@ class Ensemble:
                 if ms < best:
                     best_mode, best = m, ms
             self.set_mode(best_mode)
-            fuse = {}
-            for key, members in groups.items():
-                plans = [self.sims[i].plan for i in members]
-                cur = plans[0].get_option(_lib.OPT_FUSE_X_SOURCE)
-                for p in plans:
-                    p.set_option(_lib.OPT_FUSE_X_SOURCE, 1 - cur)
-                ms = self._time(reps)
-                trials.append({"group": str(key), "fuse_x_source": 1 - cur, "ms": ms})
-                if verbose:
-                    print(f"[autotune] {key} fuse_x_source={1 - cur}: {ms:.3f} ms/step")
-                if ms < best:
-                    best, fuse[str(key)] = ms, 1 - cur
-                else:
+            chosen = {name: {} for name, _, _ in self.KNOBS}
+            for name, opt, applies in self.KNOBS:
+                for key, members in groups.items():
+                    if not applies(key):
+                        continue
+                    plans = [self.sims[i].plan for i in members]
+                    cur = plans[0].get_option(opt)
                     for p in plans:
-                        p.set_option(_lib.OPT_FUSE_X_SOURCE, cur)
-                    fuse[str(key)] = cur
+                        p.set_option(opt, 1 - cur)
+                    ms = self._time(reps)
+                    trials.append({"group": str(key), name: 1 - cur, "ms": ms})
+                    if verbose:
+                        print(f"[autotune] {key} {name}={1 - cur}: {ms:.3f} ms/step")
+                    if ms < best:
+                        best, chosen[name][str(key)] = ms, 1 - cur
+                    else:
+                        for p in plans:
+                            p.set_option(opt, cur)
+                        chosen[name][str(key)] = cur
         finally:
             self._restore(snap)
             torch.cuda.synchronize(self.device)
-        res = {"mode": best_mode, "fuse_x_source": fuse, "ms_per_step": best, "trials": trials}
+        res = {"mode": best_mode, **chosen, "ms_per_step": best, "trials": trials}
         _TUNED[sig] = res
         if cache_path:
             cache[sig] = res
@@ -235,8 +245,9 @@ class Ensemble:
 
     def _apply(self, res: dict, groups: dict):
         self.set_mode(res["mode"])
-        for key, members in groups.items():
-            v = res["fuse_x_source"].get(str(key))
-            if v is not None:
-                for i in members:
-                    self.sims[i].plan.set_option(_lib.OPT_FUSE_X_SOURCE, int(v))
+        for name, opt, _ in self.KNOBS:
+            for key, members in groups.items():
+                v = res.get(name, {}).get(str(key))
+                if v is not None:
+                    for i in members:
+                        self.sims[i].plan.set_option(opt, int(v))

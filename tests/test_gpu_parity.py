@@ -418,3 +418,27 @@ def test_plain_config_64_cubed_100_steps(qj, cuda, dtype):
         assert rel(Vg - Vg.mean(), V - V.mean()) < 10 * tol
         mean_rho = (psi.abs() ** 2).double().mean().item()
         assert Vg.mean().item() == pytest.approx(-1.0 * (mean_rho - 1.0), abs=1e-7)
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+def test_deposit_option_can_change_between_steps(qj, cuda, dtype):
+    # binned -> atomic -> binned between steps (as the autotuner does on live
+    # states) gives the same trajectory as the binned deposit throughout
+    _lib = __import__("importlib").import_module("quantum-jax_amd._lib")
+    g = O.Grid(32, 32, 32)
+    psi, pos, vel, m_s = _stars_case(g, 2000, 31)
+    out = []
+    for flip in (False, True):
+        plan = plan_for(qj, g, dtype)
+        sim = qj.KDKSimulation(plan, qj.Physics(rho_bar=1.1, four_pi_G=20.0), 1e-3,
+                               dev(psi, dtype), dev(pos, torch.float64), dev(vel, torch.float64),
+                               mass=m_s)
+        for k in range(6):
+            if flip:
+                plan.set_option(_lib.OPT_BINNED_DEPOSIT, 0 if k in (2, 3) else 1)
+            sim.step(1)
+        st = sim.state()
+        out.append((st["psi"].clone(), st["pos"].clone(), st["vel"].clone()))
+    tol = {C64: 1e-6, C128: 1e-13}[dtype]
+    for a, b in zip(*out):
+        assert rel(a, b) < tol
