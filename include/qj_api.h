@@ -149,6 +149,10 @@ typedef struct {
    * LDS accumulation and plain stores, no global float atomics; otherwise
    * the deposit adds with float atomics into rho_s after clearing it. */
   void* star_scratch;
+  /* optional: real N, a static external potential added to V in the psi kick
+   * (V = get_potential(rho) + external_potential(), quantum-tidal.py:175-202);
+   * psi-only states */
+  void* v_ext;
 } qj_state;
 
 typedef struct {
@@ -163,6 +167,12 @@ typedef struct {
 
 int qj_kdk_steps(qj_plan* plan, qj_state* state, const qj_params* prm, int nsteps, void* work,
                  void* stream);
+/* nsteps steps with per-step kick and drift times (host arrays of nsteps):
+ * the kicks of step k use kick_dt[k]/2 and the drift drift_dt[k].  The
+ * comoving step of quantum-cosmo.py:166-197 is kick_dt = dt pot_fac and
+ * drift_dt = dt kin_fac.  Psi-only states. */
+int qj_kdk_steps_sched(qj_plan* plan, qj_state* state, const qj_params* prm, int nsteps,
+                       const double* kick_dt, const double* drift_dt, void* work, void* stream);
 /* Bytes of qj_state.star_scratch for np stars: per-tile counters and
  * offsets and up to 8 registrations per star (bin_stars of
  * quantum-stars.py:212-247, restructured as owner-computes by tile). */

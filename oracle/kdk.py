@@ -48,6 +48,7 @@ __all__ = [
     "apply_grav_accel", "compute_step_gas",
     "mass_total", "energy_total",
     "filtered_gaussian_psi", "filtered_gaussian_field", "star_ic", "gas_ic",
+    "compute_dt_cosmo", "compute_step_cosmo", "external_point_mass", "compute_step_tidal",
 ]
 
 _WORKERS = 1
@@ -548,3 +549,50 @@ def default_threads():
         return len(os.sched_getaffinity(0))
     except AttributeError:  # pragma: no cover
         return os.cpu_count() or 1
+
+
+# ---------------------------------------------------------------------------
+# kick/drift variants: comoving cosmology and a static external potential
+# ---------------------------------------------------------------------------
+def compute_dt_cosmo(a, da, H0, omega_m, omega_l, n_quad=1000):
+    """quantum-cosmo.py:149-163: dt between a and a+da from a midpoint
+    quadrature of 1/adot over [0, a] and [0, a+da]."""
+    h = 1.0 / n_quad
+    lin = np.linspace(0.5 * h, 1.0 - 0.5 * h, n_quad)
+
+    def t_of(aa):
+        at = aa * lin
+        adot = H0 * np.sqrt(omega_m / at + omega_l * at ** 2)
+        return np.mean(1.0 / adot) * aa
+
+    return t_of(a + da) - t_of(a)
+
+
+def compute_step_cosmo(psi, a, da, g: Grid, ph: Physics, H0, omega_m, omega_l):
+    """quantum-cosmo.py:166-197 (unmerged kicks): returns (psi, a + da)."""
+    kin_fac = 0.5 * (a ** -2 + (a + da) ** -2)
+    pot_fac = 0.5 * (a ** -1 + (a + da) ** -1)
+    dt = compute_dt_cosmo(a, da, H0, omega_m, omega_l)
+    V = get_potential(np.abs(psi) ** 2, g, ph)
+    psi = kick(psi, V, ph.m_per_hbar * dt * pot_fac / 2.0)
+    psi = drift(psi, dt * kin_fac, g, ph)
+    V = get_potential(np.abs(psi) ** 2, g, ph)
+    psi = kick(psi, V, ph.m_per_hbar * dt * pot_fac / 2.0)
+    return psi, a + da
+
+
+def external_point_mass(g: Grid, M, G):
+    """quantum-tidal.py:175-179: -G M / (r + dx/2), r from the box centre."""
+    X, Y, Z = cell_centres(g)
+    dx = g.Lx / g.nx
+    r = np.sqrt((X - 0.5 * g.Lx) ** 2 + (Y - 0.5 * g.Lx) ** 2 + (Z - 0.5 * g.Lx) ** 2)
+    return -G * M / (r + 0.5 * dx)
+
+
+def compute_step_tidal(psi, dt, V_ext, g: Grid, ph: Physics):
+    """quantum-tidal.py:185-207 (unmerged kicks with V + V_ext)."""
+    V = get_potential(np.abs(psi) ** 2, g, ph) + V_ext
+    psi = kick(psi, V, ph.m_per_hbar * dt / 2.0)
+    psi = drift(psi, dt, g, ph)
+    V = get_potential(np.abs(psi) ** 2, g, ph) + V_ext
+    return kick(psi, V, ph.m_per_hbar * dt / 2.0)

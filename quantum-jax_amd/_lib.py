@@ -34,7 +34,7 @@ class QjState(C.Structure):
         ("pos", vp), ("vel", vp), ("mass", vp), ("m_scalar", f64), ("np", i64),
         ("grid", vp), ("brho", vp), ("diag", vp),
         ("grid_valid", i32), ("gas_swapped", i32), ("pending", f64), ("pending_star", f64),
-        ("star_scratch", vp),
+        ("star_scratch", vp), ("v_ext", vp),
     ]
 
 
@@ -75,6 +75,7 @@ SIGNATURES = {
     "qj_solve_hydro": (i32, [vp, vp, vp, vp, vp, f64, f64, vp, vp, vp, vp, vp]),
     "qj_kdk_steps": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), i32, vp, vp]),
     "qj_kdk_finalize": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), vp]),
+    "qj_kdk_steps_sched": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), i32, dp, dp, vp, vp]),
     "qj_adjoint_scratch_size": (C.c_size_t, [vp]),
     "qj_star_scratch_size": (C.c_size_t, [vp, i64]),
     "qj_kdk_record": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), i32,

@@ -417,8 +417,8 @@ int qj_kick(qj_plan* p, void* psi, const void* V, double coef, void* stream) {
   if (int rc = check_plan(p)) return rc;
   if (!psi || !V) return fail(QJ_EARG, "qj_kick: null buffer");
   hipStream_t s = (hipStream_t)stream;
-  hipError_t e = QJ_PRECISION(p, launch_kick<float>(psi, V, nullptr, coef, p->ncell(), s),
-                              launch_kick<double>(psi, V, nullptr, coef, p->ncell(), s));
+  hipError_t e = QJ_PRECISION(p, launch_kick<float>(psi, V, nullptr, nullptr, coef, p->ncell(), s),
+                              launch_kick<double>(psi, V, nullptr, nullptr, coef, p->ncell(), s));
   return e == hipSuccess ? QJ_OK : hipfail(e, "qj_kick");
 }
 
@@ -431,12 +431,12 @@ int qj_drift(qj_plan* p, void* psi, double dt, double m_per_hbar, void* stream) 
     if (int rc = ensure_drift<float>(p, dt, m_per_hbar, s)) return rc;
     SpecTables<float> tb = tables<float>(p);
     for (int a = 2; a >= 0; --a)
-      QJ_STAGE(kStageDriftX - a, launch_drift_axis<float>(p->d, tb, a, psi, nullptr, nullptr, 0.0, 0, s), "drift");
+      QJ_STAGE(kStageDriftX - a, launch_drift_axis<float>(p->d, tb, a, psi, nullptr, nullptr, nullptr, 0.0, 0, s), "drift");
   } else {
     if (int rc = ensure_drift<double>(p, dt, m_per_hbar, s)) return rc;
     SpecTables<double> tb = tables<double>(p);
     for (int a = 2; a >= 0; --a)
-      QJ_STAGE(kStageDriftX - a, launch_drift_axis<double>(p->d, tb, a, psi, nullptr, nullptr, 0.0, 0, s), "drift");
+      QJ_STAGE(kStageDriftX - a, launch_drift_axis<double>(p->d, tb, a, psi, nullptr, nullptr, nullptr, 0.0, 0, s), "drift");
   }
   return QJ_OK;
 }
@@ -574,10 +574,11 @@ int initial_grid_t(qj_plan* p, qj_state* st, const qj_params* pr, const Work& w,
 
 template <class T>
 int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void* work,
-                hipStream_t s) {
+                hipStream_t s, const double* kick_dt = nullptr, const double* drift_dt = nullptr) {
   Work w = carve(p, work);
   SpecTables<T> tb = tables<T>(p);
-  if (int rc = ensure_drift<T>(p, pr->dt, pr->m_per_hbar, s)) return rc;
+  if (!drift_dt)
+    if (int rc = ensure_drift<T>(p, pr->dt, pr->m_per_hbar, s)) return rc;
   const bool stars = st->np > 0;
   const bool gas = st->gas[0] != nullptr;
   const bool grad = stars || gas;
@@ -595,7 +596,12 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
     if (!st->grid_valid) {
       if (int rc = initial_grid_t<T>(p, st, pr, w, s)) return rc;
     }
-    const double kick_t = st->pending + 0.5 * dt;  // owed half kick + this step's first
+    // per-step kick and drift times (a schedule, e.g. the comoving factors of
+    // quantum-cosmo.py:170-175), else dt for both
+    const double dk = kick_dt ? kick_dt[step] : dt;
+    if (drift_dt)
+      if (int rc = ensure_drift<T>(p, drift_dt[step], pr->m_per_hbar, s)) return rc;
+    const double kick_t = st->pending + 0.5 * dk;  // owed half kick + this step's first
     // stars: owed closing kick of the previous step and this step's opening
     // half kick (same grid, same positions), drift, deposit at the new positions
     if (stars && binned) {
@@ -627,9 +633,10 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
     }
     // psi: kick folded into the z drift sweep, then y and x (+ Poisson source)
     QJ_STAGE(kStageDriftZKick,
-             launch_drift_axis<T>(p->d, tb, 2, st->psi, V, brho, pr->m_per_hbar * kick_t, 1, s),
+             launch_drift_axis<T>(p->d, tb, 2, st->psi, V, brho, st->v_ext, pr->m_per_hbar * kick_t, 1,
+                                  s),
              "drift z");
-    QJ_STAGE(kStageDriftY, launch_drift_axis<T>(p->d, tb, 1, st->psi, nullptr, nullptr, 0.0, 0, s),
+    QJ_STAGE(kStageDriftY, launch_drift_axis<T>(p->d, tb, 1, st->psi, nullptr, nullptr, nullptr, 0.0, 0, s),
              "drift y");
     src = stars ? st->rho_s : (gas ? live_gas(0) : nullptr);
     if (p->opt[QJ_OPT_FUSE_X_SOURCE]) {
@@ -639,7 +646,7 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
                "drift x + source");
     } else {
       QJ_STAGE(kStageDriftX,
-               launch_drift_axis<T>(p->d, tb, 0, st->psi, nullptr, nullptr, 0.0, 0, s), "drift x");
+               launch_drift_axis<T>(p->d, tb, 0, st->psi, nullptr, nullptr, nullptr, 0.0, 0, s), "drift x");
       QJ_STAGE(kStageSourceX,
                launch_source_x<T>(p->d, tb, 0, st->psi, src, clear, brho, pr->b_coeff,
                                   pr->rho_bar, w.H, p->partials, st->diag, s),
@@ -649,7 +656,7 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
                                  nullptr, st->diag, s))
       return rc;
     // the closing half kicks (psi, gas and stars) are owed to the next step
-    st->pending = 0.5 * dt;
+    st->pending = 0.5 * dk;
     if (stars) st->pending_star = 0.5 * dt;
   }
   return QJ_OK;
@@ -662,7 +669,7 @@ int kdk_finalize_t(qj_plan* p, qj_state* st, const qj_params* pr, hipStream_t s)
   if (st->pending != 0.0) {
     void* brho = (pr->b_coeff != 0.0) ? st->brho : nullptr;
     QJ_STAGE(kStageKick,
-             launch_kick<T>(st->psi, V, brho, pr->m_per_hbar * st->pending, p->ncell(), s), "kick");
+             launch_kick<T>(st->psi, V, brho, st->v_ext, pr->m_per_hbar * st->pending, p->ncell(), s), "kick");
     if (st->gas[0]) {
       void* g1 = st->gas_swapped ? st->gas_tmp[1] : st->gas[1];
       void* g2 = st->gas_swapped ? st->gas_tmp[2] : st->gas[2];
@@ -734,7 +741,7 @@ int kdk_adjoint_t(qj_plan* p, const qj_state* st, const qj_params* pr, int nstep
                launch_interp_adj<T>(p->d, pos, st->np, A4, h, a->v_bar, dt_xv, a->x_bar, abar4, s),
                "interp adjoint");
     QJ_STAGE(kStageAdjKick,
-             launch_kick_adj<T>(a->psi_bar, psi_ref, V, c, Vbar, n, s), "kick adjoint");
+             launch_kick_adj<T>(a->psi_bar, psi_ref, V, st->v_ext, c, Vbar, n, s), "kick adjoint");
     QJ_STAGE(kStageAdjPoisson,
              launch_adj_poisson<T>(p->d, tb, Vbar, abar4, pr->four_pi_G, pr->dc_mode, w.H, w.Uz,
                                    w.Uy, rhobar, s),
@@ -756,7 +763,7 @@ int kdk_adjoint_t(qj_plan* p, const qj_state* st, const qj_params* pr, int nstep
     // pos1 = pos0 + dt vel1 is folded into the next interp adjoint (dt_xv)
     for (int ax = 0; ax < 3; ++ax)
       QJ_STAGE(kStageAdjDrift,
-               launch_drift_axis<T>(p->d, tb, ax, a->psi_bar, nullptr, nullptr, 0.0, 0, s, 1),
+               launch_drift_axis<T>(p->d, tb, ax, a->psi_bar, nullptr, nullptr, nullptr, 0.0, 0, s, 1),
                "drift adjoint");
     // opening half step (grid0 at x0, psi0)
     if (int rc = half(g0, t.psi0, t.x0, dt)) return rc;
@@ -786,9 +793,25 @@ int qj_kdk_steps(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void
   if (pr->b_coeff != 0.0 && !st->brho)
     return fail(QJ_EARG, "qj_kdk_steps: self-interaction needs the brho buffer");
   if (!(pr->m_per_hbar != 0.0)) return fail(QJ_EARG, "qj_kdk_steps: m_per_hbar must be non-zero");
+  if (st->v_ext && (st->np > 0 || st->gas[0]))
+    return fail(QJ_EUNSUPPORTED,
+                "qj_kdk_steps: the external potential acts on psi only (no stars or gas)");
   hipStream_t s = (hipStream_t)stream;
   return p->dtype == QJ_C64 ? kdk_steps_t<float>(p, st, pr, nsteps, work, s)
                             : kdk_steps_t<double>(p, st, pr, nsteps, work, s);
+}
+
+int qj_kdk_steps_sched(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps,
+                       const double* kick_dt, const double* drift_dt, void* work, void* stream) {
+  if (nsteps > 0 && (!kick_dt || !drift_dt))
+    return fail(QJ_EARG, "qj_kdk_steps_sched: null schedule");
+  if (st && (st->np > 0 || st->gas[0]))
+    return fail(QJ_EUNSUPPORTED, "qj_kdk_steps_sched: psi-only states (no stars or gas)");
+  if (int rc = qj_kdk_steps(p, st, pr, 0, work, stream)) return rc;  // argument checks
+  hipStream_t s = (hipStream_t)stream;
+  return p->dtype == QJ_C64
+             ? kdk_steps_t<float>(p, st, pr, nsteps, work, s, kick_dt, drift_dt)
+             : kdk_steps_t<double>(p, st, pr, nsteps, work, s, kick_dt, drift_dt);
 }
 
 size_t qj_star_scratch_size(const qj_plan* p, long long np) {

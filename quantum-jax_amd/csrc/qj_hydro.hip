@@ -75,6 +75,22 @@ constexpr int kHT = 16;                 // max tile edge (y and z)
 constexpr int kHR = kHT + 4;            // raw region edge (2-cell halo)
 constexpr int kHS = kHT + 2;            // state region edge (1-cell ring)
 
+// (a_x, a_y, a_z) of the interleaved grid cell i in one 16-byte load (float)
+template <class T>
+__device__ __forceinline__ void ld3(const T* __restrict__ A4, int i, T a[3]) {
+  if constexpr (sizeof(T) == 4) {
+    const float4 t = *reinterpret_cast<const float4*>(A4 + 4 * i);
+    a[0] = t.x;
+    a[1] = t.y;
+    a[2] = t.z;
+  } else {
+    const double2 t = *reinterpret_cast<const double2*>(A4 + 4 * i);
+    a[0] = t.x;
+    a[1] = t.y;
+    a[2] = A4[4 * i + 2];
+  }
+}
+
 template <class T>
 struct XState {
   T p[4];
@@ -106,23 +122,48 @@ struct k_hydro_stream {
     auto gidx = [&](int x, int y, int z) {
       return (((x & (g.nx - 1)) * g.ny + (y & (g.ny - 1))) * g.nz) + (z & (g.nz - 1));
     };
-    auto load_plane = [&](int x) {
-      const int slot = x & 3;
-      for (int c = tid; c < RY * RZ; c += blockDim.x) {
+    // A plane of the raw region (tile + 2-cell halo, at most 2 cells per
+    // thread) is fetched into registers one iteration before it is written to
+    // its LDS ring slot, so the global loads overlap the states and fluxes.
+    struct Fetch {
+      T q[2][4];
+    };
+    auto fetch = [&](int x, Fetch& f) {
+  #pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const int c = tid + k * (int)blockDim.x;
+        if (c >= RY * RZ) break;
         const int yy = c / RZ - 2, zz = c % RZ - 2;
         const int i = gidx(x, y0 + yy, z0 + zz);
         T r = rho[i], u = vx[i], v = vy[i], w = vz[i];
         if (kick != (T)0) {  // owed gravity kick of the gas velocity, quantum-fluid.py:274-276
-          u += A4[4 * i] * kick;
-          v += A4[4 * i + 1] * kick;
-          w += A4[4 * i + 2] * kick;
+          T a[3];
+          ld3(A4, i, a);
+          u += a[0] * kick;
+          v += a[1] * kick;
+          w += a[2] * kick;
         }
-        const int o = (yy + 2) * RZ + (zz + 2);
-        raw[slot][0][o] = r;
-        raw[slot][1][o] = u;
-        raw[slot][2][o] = v;
-        raw[slot][3][o] = w;
+        f.q[k][0] = r;
+        f.q[k][1] = u;
+        f.q[k][2] = v;
+        f.q[k][3] = w;
       }
+    };
+    auto put = [&](int x, const Fetch& f) {
+      const int slot = x & 3;
+  #pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const int c = tid + k * (int)blockDim.x;
+        if (c >= RY * RZ) break;
+        const int o = (c / RZ) * RZ + (c % RZ);
+  #pragma unroll
+        for (int v = 0; v < 4; ++v) raw[slot][v][o] = f.q[k][v];
+      }
+    };
+    auto load_plane = [&](int x) {
+      Fetch f;
+      fetch(x, f);
+      put(x, f);
     };
     // predicted state and limited slopes of region cell (yy, zz) at plane x
     auto cell = [&](int x, int yy, int zz, T p[4], T s[3][4]) {
@@ -198,8 +239,15 @@ struct k_hydro_stream {
     }
     states(x0, xc0);
     const T fac = g.dt * g.dx * g.dx;
+    Fetch pre;
+    fetch(x0 + 2, pre);
+    // the first put overwrites the slot of plane x0-2, read by the warm-up
+    __syncthreads();
     for (int x = x0; x < x0 + XC; ++x) {
-      load_plane(x + 2);
+      // plane x+2 (fetched during the previous iteration) into its slot,
+      // which planes x-1 .. x+1 in use leave free; then fetch plane x+3
+      put(x + 2, pre);
+      if (x + 3 <= x0 + XC + 1) fetch(x + 3, pre);
       __syncthreads();
       states(x + 1, xp);
       __syncthreads();
@@ -317,6 +365,7 @@ hipError_t launch_hydro(const Dims& d, const void* rho, const void* vx, const vo
   const int XC = d.nx < 32 ? d.nx : 32;
   const int nblk = (d.ny / TY) * (d.nz / TZ) * (d.nx / XC);
   const int nthr = ((TY + 2) * (TZ + 2) + 63) / 64 * 64;  // tile cells + ring, whole waves
+  if ((TY + 4) * (TZ + 4) > 2 * nthr) return hipErrorInvalidConfiguration;  // 2 cells per thread
   launch<k_hydro_stream<T>>(nblk, nthr, s, g, TY, TZ, XC, (const T*)rho, (const T*)vx,
                                               (const T*)vy, (const T*)vz, (const T*)A4, (T)kick,
                                               (T*)rho_o, (T*)vx_o, (T*)vy_o, (T*)vz_o);

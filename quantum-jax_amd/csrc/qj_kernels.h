@@ -31,14 +31,14 @@ template <class T>
 hipError_t launch_drift_table(void* tab, int n, double L, double c, hipStream_t s);
 
 template <class T>
-hipError_t launch_kick(void* psi, const void* V, const void* Vb, double coef, long long n,
-                       hipStream_t s);
+hipError_t launch_kick(void* psi, const void* V, const void* Vb, const void* Vx, double coef,
+                       long long n, hipStream_t s);
 
 // conj_d = 1 applies the conjugate phase table (the drift of -dt).
 template <class T>
 hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, void* psi,
-                             const void* V, const void* Vb, double kick_coef, int kick,
-                             hipStream_t s, int conj_d = 0);
+                             const void* V, const void* Vb, const void* Vx, double kick_coef,
+                             int kick, hipStream_t s, int conj_d = 0);
 
 template <class T>
 hipError_t launch_source_x(const Dims& d, const SpecTables<T>& tb, int drift, void* psi,
@@ -107,8 +107,8 @@ hipError_t launch_deposit_adj(const Dims& d, const double* pos, const double* ma
 // ---- adjoint spectral passes -------------------------------------------------
 // psi_bar <- exp(+i coef V) psi_bar; Vbar = coef * Im(conj(psi_bar) psi_ref)
 template <class T>
-hipError_t launch_kick_adj(void* psi_bar, const void* psi_ref, const void* V, double coef,
-                           void* Vbar, long long n, hipStream_t s);
+hipError_t launch_kick_adj(void* psi_bar, const void* psi_ref, const void* V, const void* Vx,
+                           double coef, void* Vbar, long long n, hipStream_t s);
 // psi_bar += 2 rhobar psi_ref
 template <class T>
 hipError_t launch_rho_adj(void* psi_bar, const void* psi_ref, const void* rhobar, long long n,

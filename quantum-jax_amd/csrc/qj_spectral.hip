@@ -59,12 +59,14 @@ template <class T>
 struct k_kick {
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(cpx<T>* __restrict__ psi, const T* __restrict__ V,
-                                             const T* __restrict__ Vb, double coef, long long n) {
+                                             const T* __restrict__ Vb, const T* __restrict__ Vx,
+                                             double coef, long long n) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long stride = (long long)gridDim.x * blockDim.x;
     for (; i < n; i += stride) {
       double v = (double)V[i];
       if (Vb) v += (double)Vb[i];
+      if (Vx) v += (double)Vx[i];
       psi[i] = cmul(psi[i], expmi<T>(coef * v));
     }
   }
@@ -79,7 +81,8 @@ struct k_drift_c {
   static __device__ __forceinline__ void run(cpx<T>* __restrict__ psi, PassGeom g,
                                              const cpx<T>* __restrict__ D,
                                              const cpx<T>* __restrict__ tw, const T* __restrict__ V,
-                                             const T* __restrict__ Vb, double kick_coef,
+                                             const T* __restrict__ Vb,
+                                             const T* __restrict__ Vx, double kick_coef,
                                              int conj_d) {
     using S = PencilShape<N>;
     __shared__ cpx<T> lds[S::kLds];
@@ -97,6 +100,7 @@ struct k_drift_c {
         if (act) {
           double th = (double)V[idx];
           if (Vb) th += (double)Vb[idx];
+          if (Vx) th += (double)Vx[idx];
           v[m] = cmul(v[m], expmi<T>(kick_coef * th));
         }
       }
@@ -586,12 +590,14 @@ template <class T>
 struct k_kick_adj {
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(cpx<T>* __restrict__ pb, const cpx<T>* __restrict__ pr,
-                                             const T* __restrict__ V, double coef,
-                                             T* __restrict__ Vbar, long long n) {
+                                             const T* __restrict__ V, const T* __restrict__ Vx,
+                                             double coef, T* __restrict__ Vbar, long long n) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long stride = (long long)gridDim.x * blockDim.x;
     for (; i < n; i += stride) {
-      const cpx<T> e = expmi<T>(coef * (double)V[i]);  // exp(-i coef V)
+      // the phase includes the static external potential, which has no adjoint
+      const double vt = (double)V[i] + (Vx ? (double)Vx[i] : 0.0);
+      const cpx<T> e = expmi<T>(coef * vt);             // exp(-i coef V)
       const cpx<T> b = cmulc(pb[i], e);                 // * exp(+i coef V)
       pb[i] = b;
       const cpx<T> r = pr[i];
@@ -802,32 +808,34 @@ hipError_t launch_drift_table(void* tab, int n, double L, double c, hipStream_t 
 }
 
 template <class T>
-hipError_t launch_kick(void* psi, const void* V, const void* Vb, double coef, long long n,
-                       hipStream_t s) {
+hipError_t launch_kick(void* psi, const void* V, const void* Vb, const void* Vx, double coef,
+                       long long n, hipStream_t s) {
   long long blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  launch<k_kick<T>>((int)blocks, 256, s, (cpx<T>*)psi, (const T*)V, (const T*)Vb, coef, n);
+  launch<k_kick<T>>((int)blocks, 256, s, (cpx<T>*)psi, (const T*)V, (const T*)Vb, (const T*)Vx,
+                    coef, n);
   return hipGetLastError();
 }
 
 template <class T>
 hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, void* psi,
-                             const void* V, const void* Vb, double kick_coef, int kick,
-                             hipStream_t s, int conj_d) {
+                             const void* V, const void* Vb, const void* Vx, double kick_coef,
+                             int kick, hipStream_t s, int conj_d) {
   cpx<T>* p = (cpx<T>*)psi;
   const T* v = (const T*)V;
   const T* vb = (const T*)Vb;
+  const T* vx = (const T*)Vx;
   if (axis == 2) {
     PassGeom g = geom_z(d, d.nx);
     const cpx<T>* D = (const cpx<T>*)tb.drift_z;
     const cpx<T>* tw = (const cpx<T>*)tb.tw_z;
     QJ_DISPATCH_N(d.nz, {
       if (kick)
-        launch<k_drift_c<T, N_, true>>(nblocks<N_>(g.ncols), 256, s, p, g, D, tw, v, vb, kick_coef,
-                                                                    conj_d);
+        launch<k_drift_c<T, N_, true>>(nblocks<N_>(g.ncols), 256, s, p, g, D, tw, v, vb, vx,
+                                       kick_coef, conj_d);
       else
-        launch<k_drift_c<T, N_, false>>(nblocks<N_>(g.ncols), 256, s, p, g, D, tw, v, vb, kick_coef,
-                                                                     conj_d);
+        launch<k_drift_c<T, N_, false>>(nblocks<N_>(g.ncols), 256, s, p, g, D, tw, v, vb, vx,
+                                        kick_coef, conj_d);
     });
   } else {
     if (kick) return hipErrorInvalidValue;  // the kick is fused into the z sweep only
@@ -956,12 +964,12 @@ hipError_t launch_poisson_x(const Dims& d, const SpecTables<T>& tb, int grad, co
 }
 
 template <class T>
-hipError_t launch_kick_adj(void* psi_bar, const void* psi_ref, const void* V, double coef,
-                           void* Vbar, long long n, hipStream_t s) {
+hipError_t launch_kick_adj(void* psi_bar, const void* psi_ref, const void* V, const void* Vx,
+                           double coef, void* Vbar, long long n, hipStream_t s) {
   long long blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   launch<k_kick_adj<T>>((int)blocks, 256, s, (cpx<T>*)psi_bar, (const cpx<T>*)psi_ref,
-                                            (const T*)V, coef, (T*)Vbar, n);
+                        (const T*)V, (const T*)Vx, coef, (T*)Vbar, n);
   return hipGetLastError();
 }
 
@@ -1021,10 +1029,11 @@ hipError_t launch_adj_poisson(const Dims& d, const SpecTables<T>& tb, const void
 #define QJ_INST(T)                                                                              \
   template hipError_t launch_twiddles<T>(void*, int, hipStream_t);                              \
   template hipError_t launch_drift_table<T>(void*, int, double, double, hipStream_t);           \
-  template hipError_t launch_kick<T>(void*, const void*, const void*, double, long long,        \
-                                     hipStream_t);                                              \
+  template hipError_t launch_kick<T>(void*, const void*, const void*, const void*, double,       \
+                                     long long, hipStream_t);                                   \
   template hipError_t launch_drift_axis<T>(const Dims&, const SpecTables<T>&, int, void*,        \
-                                           const void*, const void*, double, int, hipStream_t,  \
+                                           const void*, const void*, const void*, double, int,  \
+                                           hipStream_t,                                         \
                                            int);                                                \
   template hipError_t launch_source_x<T>(const Dims&, const SpecTables<T>&, int, void*, void*,  \
                                          int, void*, double, double, void*, double*, double*,   \
@@ -1041,7 +1050,8 @@ hipError_t launch_adj_poisson(const Dims& d, const SpecTables<T>& tb, const void
                                           double*, hipStream_t);
 
 #define QJ_INST_ADJ(T)                                                                          \
-  template hipError_t launch_kick_adj<T>(void*, const void*, const void*, double, void*,        \
+  template hipError_t launch_kick_adj<T>(void*, const void*, const void*, const void*, double,   \
+                                         void*,                                                 \
                                          long long, hipStream_t);                               \
   template hipError_t launch_rho_adj<T>(void*, const void*, const void*, long long, hipStream_t); \
   template hipError_t launch_adj_poisson<T>(const Dims&, const SpecTables<T>&, const void*,     \

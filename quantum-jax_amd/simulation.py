@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import ctypes as C
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -24,11 +25,13 @@ class KDKSimulation:
     pos, vel : float64 (np,3) star positions/velocities (stars step)
     mass  : float (common star mass) or float64 (np,) per-particle masses
     gas   : (rho, vx, vy, vz) real fields (gas step)
+    v_ext : static external potential added to V in the psi kicks
+            (quantum-tidal.py:175-202; psi-only states)
     """
 
     def __init__(self, plan: Plan, phys: Physics, dt: float, psi: torch.Tensor,
                  pos: torch.Tensor | None = None, vel: torch.Tensor | None = None,
-                 mass=0.0, gas=None, t: float = 0.0):
+                 mass=0.0, gas=None, t: float = 0.0, v_ext: torch.Tensor | None = None):
         self.plan, self.phys, self.dt = plan, phys, float(dt)
         self._psi = plan.field(psi, "psi", complex_=True).clone()
         dev, rdt = plan.device, plan.real_dtype
@@ -83,6 +86,8 @@ class KDKSimulation:
         st.pending = 0.0
         st.pending_star = 0.0
         st.star_scratch = ptr(self.star_scratch).value
+        self.v_ext = plan.field(v_ext, "v_ext").clone() if v_ext is not None else None
+        st.v_ext = ptr(self.v_ext).value
         self._st = st
         self._params = _lib.QjParams(self.dt, phys.m_per_hbar, phys.four_pi_G, phys.rho_bar,
                                      phys.b_coeff, phys.cs, phys.dc_mode)
@@ -102,6 +107,24 @@ class KDKSimulation:
                                                ptr(self.plan.workspace()), self.plan.stream()),
                        "qj_kdk_steps")
         self.t += n * self.dt
+        return self
+
+    def step_schedule(self, kick_dts, drift_dts, elapsed: float | None = None):
+        """Steps with per-step kick and drift times (e.g. the comoving
+        schedule of cosmo.comoving_schedule); psi-only states.  elapsed, when
+        given, is added to t."""
+        kick = np.ascontiguousarray(kick_dts, dtype=np.float64)
+        drift = np.ascontiguousarray(drift_dts, dtype=np.float64)
+        if kick.shape != drift.shape or kick.ndim != 1:
+            raise ValueError("kick_dts and drift_dts must be 1-D arrays of one length")
+        dp = C.POINTER(C.c_double)
+        with torch.cuda.device(self.plan.device):
+            _lib.check(_lib.lib().qj_kdk_steps_sched(
+                self.plan.handle, C.byref(self._st), C.byref(self._params), int(kick.size),
+                kick.ctypes.data_as(dp), drift.ctypes.data_as(dp), ptr(self.plan.workspace()),
+                self.plan.stream()), "qj_kdk_steps_sched")
+        if elapsed is not None:
+            self.t += float(elapsed)
         return self
 
     def finalize(self):

@@ -50,7 +50,7 @@ def test_state_struct_layout_matches_header():
     fields = re.findall(r"\b([a-z_]+)(?:\[\d\])?;", re.sub(r"/\*.*?\*/", "", body, flags=re.S))
     assert fields == [f[0] for f in _lib.QjState._fields_]
     # 2 + 4 + 4 + 6 pointer-sized members, two ints, two doubles
-    assert ctypes.sizeof(_lib.QjState) == 176
+    assert ctypes.sizeof(_lib.QjState) == 184
 
 
 def test_no_cpu_fallback_when_library_missing(monkeypatch):

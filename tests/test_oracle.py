@@ -378,3 +378,71 @@ def test_ic_definitions():
     assert abs(vel.std() - 0.05) < 0.005
     rho, *_ = O.gas_ic(g, 17)
     assert rho.min() >= 0.01 and abs(rho.mean() - 0.2) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# kick/drift variants (quantum-cosmo.py, quantum-tidal.py)
+# ---------------------------------------------------------------------------
+def test_compute_dt_matches_closed_form_matter_only():
+    # omega_l = 0: t(a) = 2 a^(3/2) / (3 H0 sqrt(omega_m)) (midpoint rule on a
+    # sqrt-singular derivative: a few 1e-4 relative)
+    H0, om = 0.07, 0.3
+    for a, da in ((0.01, 1e-4), (0.5, 1e-3), (1.0, -1e-3)):
+        exact = 2.0 / (3.0 * H0 * np.sqrt(om)) * ((a + da) ** 1.5 - a ** 1.5)
+        assert O.compute_dt_cosmo(a, da, H0, om, 0.0) == pytest.approx(exact, rel=1e-3)
+
+
+def test_compute_dt_is_antisymmetric():
+    # dt(a+da, -da) = -dt(a, da): the same quadrature of t(a) at both ends
+    for a, da in ((0.02, 3e-4), (0.7, 1e-2)):
+        f = O.compute_dt_cosmo(a, da, 0.07, 0.3, 0.7)
+        b = O.compute_dt_cosmo(a + da, -da, 0.07, 0.3, 0.7)
+        assert b == pytest.approx(-f, rel=1e-12)
+
+
+def test_cosmo_step_composes_kick_pot_and_drift_kin():
+    # kin_fac scales the drift, pot_fac the kicks: at a = 2 the two differ by 2x,
+    # so a swapped factor fails; composed here from the pinned kick and drift
+    g = O.Grid(16, 16, 16)
+    ph = O.Physics(four_pi_G=30.0)
+    psi = O.filtered_gaussian_psi(g, 3, sigma_k=6 * np.pi)
+    a, da, H0, om, ol = 2.0, 4e-3, 3.0, 0.3, 0.7
+    out, a1 = O.compute_step_cosmo(psi, a, da, g, ph, H0, om, ol)
+    dt = O.compute_dt_cosmo(a, da, H0, om, ol)
+    kin = 0.5 * (a ** -2 + (a + da) ** -2)
+    pot = 0.5 * (a ** -1 + (a + da) ** -1)
+    assert kin == pytest.approx(0.25 * 0.5 * (1 + (1 + da / a) ** -2), rel=1e-14)
+    p = O.kick(psi, O.get_potential(np.abs(psi) ** 2, g, ph), dt * pot / 2)
+    p = O.drift(p, dt * kin, g, ph)
+    p = O.kick(p, O.get_potential(np.abs(p) ** 2, g, ph), dt * pot / 2)
+    assert np.allclose(out, p, rtol=0, atol=1e-13) and a1 == a + da
+
+
+def test_cosmo_step_is_time_reversible():
+    g = O.Grid(16, 16, 16)
+    ph = O.Physics(four_pi_G=30.0)
+    psi = O.filtered_gaussian_psi(g, 4, sigma_k=6 * np.pi)
+    a, da = 0.5, 2e-3
+    p, a1 = O.compute_step_cosmo(psi, a, da, g, ph, 1.0, 0.3, 0.7)
+    back, a2 = O.compute_step_cosmo(p, a1, -da, g, ph, 1.0, 0.3, 0.7)
+    assert np.abs(back - psi).max() < 1e-12 and a2 == pytest.approx(a)
+
+
+def test_tidal_step_constant_external_potential_is_a_global_phase():
+    g = O.Grid(16, 16, 16)
+    ph = O.Physics(four_pi_G=30.0, m_per_hbar=1.3)
+    psi = O.filtered_gaussian_psi(g, 5, sigma_k=6 * np.pi)
+    dt, c = 2e-3, 0.37
+    p_ext = O.compute_step_tidal(psi, dt, np.full(g.shape, c), g, ph)
+    p = O.compute_step_plain(psi, dt, g, ph)
+    assert np.allclose(p_ext, p * np.exp(-1j * ph.m_per_hbar * dt * c), rtol=0, atol=1e-13)
+
+
+def test_external_point_mass_profile():
+    g = O.Grid(8, 8, 8)
+    V = O.external_point_mass(g, 2.0, 0.5)
+    X, Y, Z = O.cell_centres(g)
+    r = np.sqrt((X - 0.5) ** 2 + (Y - 0.5) ** 2 + (Z - 0.5) ** 2)
+    # cells nearest the centre sit at r = sqrt(3)/16; softening dx/2 = 1/16
+    assert V.min() == pytest.approx(-1.0 / (np.sqrt(3) / 16 + 1 / 16))
+    assert np.all(np.diff(V.ravel()[np.argsort(r.ravel())]) >= -1e-15)  # rises with r
