@@ -7,9 +7,12 @@ velocities, equal masses totalling 0.1.  Gas: rho = 0.2 (1 + 0.1 g) with g
 the real part of the filtered field scaled to zero mean and unit variance,
 clipped below at 0.01; v = 0.
 
-The reference builds its own ICs differently (random phases sorted by |k|,
-quantum-fluid.py:509-522); these seeded constructions define the workloads
-that the tests and the benchmark share.
+These seeded constructions define the workloads that the tests and the
+benchmark share.  The reference's own ICs are below them: random phases
+sorted by |k| (quantum-fluid.py:509-522) and the Gaussian columns with
+uniform stars (quantum-stars.py:376-428).  Its HDF5 IC reader
+(quantum-cosmo.py:242-251) is not provided: it needs h5py and a data file
+that are not available here.
 """
 
 from __future__ import annotations
@@ -68,3 +71,52 @@ def gas_ic(shape, seed: int, L=(1.0, 1.0, 1.0), base: float = 0.2, amp: float = 
     rho = np.maximum(base * (1.0 + amp * f), floor)
     z = np.zeros(shape)
     return rho, z.copy(), z.copy(), z.copy()
+
+
+# ---------------------------------------------------------------------------
+# The reference's own initial conditions
+# ---------------------------------------------------------------------------
+def random_phase_psi(shape, L, sigma: float, m_per_hbar: float, mean_rho: float,
+                     seed: int = 17):
+    """quantum-fluid.py:509-522: unit-modulus random phases assigned in
+    increasing |k| order (stable argsort of k^2, so the lowest modes take the
+    first numbers of NumPy's legacy seeded generator), amplitude
+    sqrt(exp(-k^2 / (2 sigma^2 m_per_hbar^2))), inverse FFT, and mean
+    |psi|^2 = mean_rho (the reference's frac_dm * rho_bar)."""
+    k2 = k_squared(shape, L)
+    rs = np.random.RandomState(seed)  # np.random.seed(seed); np.random.rand
+    order = np.argsort(k2.ravel(), kind="stable")
+    psi = np.zeros(k2.size, dtype=complex)
+    psi[order] = np.exp(1.0j * 2.0 * np.pi * rs.rand(k2.size))
+    psi = psi.reshape(k2.shape) * np.sqrt(np.exp(-k2 / (2.0 * sigma ** 2 * m_per_hbar ** 2)))
+    psi = np.fft.ifftn(psi)
+    return psi * np.sqrt(mean_rho / np.mean(np.abs(psi) ** 2))
+
+
+# (weight, x / Lx, y / Ly) of the Gaussian columns of quantum-stars.py:379-420
+STAR_BLOBS = ((2.0, 0.5, 0.4), (1.5, 0.6, 0.5), (1.0, 0.4, 0.6), (1.0, 0.6, 0.4),
+              (1.0, 0.6, 0.6), (1.0, 0.6, 0.4), (1.0, 0.5, 0.4), (1.0, 0.5, 0.4))
+
+
+def blob_psi(shape, L, mean_rho: float, amp: float = 100.0, sigma: float = 4.0,
+             floor: float = 10.0, blobs=STAR_BLOBS):
+    """quantum-stars.py:376-423: rho = floor + sum of Gaussian columns (in x
+    and y, uniform in z) of weight w amp / (sigma^3 2 pi), rescaled to mean
+    mean_rho (frac_dm * rho_bar); psi = sqrt(rho)."""
+    axes = [np.linspace(0.5 * Li / n, Li - 0.5 * Li / n, n) for n, Li in zip(shape, L)]
+    X, Y, _ = np.meshgrid(*axes, indexing="ij")
+    rho = np.full(shape, float(floor))
+    for w, cx, cy in blobs:
+        rho = rho + (w * amp * np.exp(-((X - cx * L[0]) ** 2 + (Y - cy * L[1]) ** 2) / 2.0
+                                      / sigma ** 2) / (sigma ** 3 * np.sqrt(2.0 * np.pi) ** 2))
+    rho *= mean_rho / np.mean(rho)
+    return np.sqrt(rho) + 0.0j
+
+
+def uniform_star_ic(n_s: int, L, seed: int = 17):
+    """quantum-stars.py:425-428: positions uniform in the box and velocities
+    uniform in [-1, 1] from NumPy's legacy generator seeded with seed."""
+    rs = np.random.RandomState(seed)
+    pos = rs.uniform(0.0, 1.0, (n_s, 3)) * np.asarray(L, dtype=np.float64)
+    vel = rs.uniform(-1.0, 1.0, (n_s, 3))
+    return pos, vel
