@@ -173,6 +173,15 @@ int qj_kdk_steps(qj_plan* plan, qj_state* state, const qj_params* prm, int nstep
  * drift_dt = dt kin_fac.  Psi-only states. */
 int qj_kdk_steps_sched(qj_plan* plan, qj_state* state, const qj_params* prm, int nsteps,
                        const double* kick_dt, const double* drift_dt, void* work, void* stream);
+/* do_accretion(psi, pos, vel, m_bh) of quantum-bh.py:306-339, in place:
+ * black hole 0 takes dM/dt dt from the field at its cell (the reference
+ * lowers the amplitude there by sqrt(dM/dt dt / vol) at fixed phase) and
+ * dM/dt dt is added to each of the n_bh masses (the reference adds the
+ * scalar to its whole mass array).  pos, vel, m_bh: the black holes' FP64
+ * arrays. */
+int qj_bh_accretion(qj_plan* plan, void* psi, const double* pos, const double* vel, double* m_bh,
+                    long long n_bh, double G, double c, double sigma, double m_per_hbar,
+                    double dt, void* stream);
 /* Bytes of qj_state.star_scratch for np stars: per-tile counters and
  * offsets and up to 8 registrations per star (bin_stars of
  * quantum-stars.py:212-247, restructured as owner-computes by tile). */

@@ -49,6 +49,7 @@ __all__ = [
     "mass_total", "energy_total",
     "filtered_gaussian_psi", "filtered_gaussian_field", "star_ic", "gas_ic",
     "compute_dt_cosmo", "compute_step_cosmo", "external_point_mass", "compute_step_tidal",
+    "do_accretion", "compute_step_bh",
 ]
 
 _WORKERS = 1
@@ -596,3 +597,33 @@ def compute_step_tidal(psi, dt, V_ext, g: Grid, ph: Physics):
     psi = drift(psi, dt, g, ph)
     V = get_potential(np.abs(psi) ** 2, g, ph) + V_ext
     return kick(psi, V, ph.m_per_hbar * dt / 2.0)
+
+
+# ---------------------------------------------------------------------------
+# black-hole accretion (quantum-bh.py)
+# ---------------------------------------------------------------------------
+def do_accretion(psi, pos, vel, m_bh, g: Grid, ph: Physics, G, c, sigma, dt):
+    """quantum-bh.py:306-339 (returns new psi, m_bh)."""
+    dx = g.Lx / g.nx
+    dxs = np.array([dx, dx, dx])
+    i = np.floor((pos - 0.5 * dxs) / dxs)
+    i = np.mod(i, np.array([g.nx, g.ny, g.nz])).astype(int)
+    s = 0
+    psi_at_bh = psi[i[s, 0], i[s, 1], i[s, 2]]
+    psi_amp = np.abs(psi_at_bh)
+    psi_theta = np.angle(psi_at_bh)
+    v = np.sqrt(sigma ** 2 + vel[0, 0] ** 2 + vel[0, 1] ** 2 + vel[0, 2] ** 2)
+    xi = 2.0 * np.pi * G * m_bh[0] * ph.m_per_hbar / v
+    dM_dt = (32.0 * np.pi ** 2 * (G * m_bh[0]) ** 3 * ph.m_per_hbar * psi_amp ** 2
+             / (c ** 3 * v * (1.0 - np.exp(-xi))))
+    new_amp = np.abs(psi_amp) - np.sqrt(dM_dt * dt / dx ** 3)
+    psi = psi.copy()
+    psi[i[s, 0], i[s, 1], i[s, 2]] = new_amp * np.exp(1.0j * psi_theta)
+    return psi, m_bh + dM_dt * dt
+
+
+def compute_step_bh(psi, m_bh, pos, vel, t, dt, g: Grid, ph: Physics, G, c, sigma):
+    """quantum-bh.py:345-381: the KDK step with per-hole masses, then accretion."""
+    psi, pos, vel, t, _, _ = compute_step_stars(psi, pos, vel, t, 0.0, np.inf, dt, m_bh, g, ph)
+    psi, m_bh = do_accretion(psi, pos, vel, m_bh, g, ph, G, c, sigma, dt)
+    return psi, m_bh, pos, vel, t

@@ -11,11 +11,11 @@ name carries a hyphen).
 
 from .core import Physics, Plan  # noqa: F401
 from .operators import (  # noqa: F401
-    acceleration_grid, apply_grav_accel, bin_particles, bin_stars, drift,
+    acceleration_grid, apply_grav_accel, bin_particles, bin_stars, do_accretion, drift,
     get_acceleration, get_cic_indices_and_weights, get_potential, interp, kick,
     solve_hydro,
 )
-from .simulation import KDKSimulation, compute_step, compute_step_gas  # noqa: F401
+from .simulation import BHSimulation, KDKSimulation, compute_step, compute_step_gas  # noqa: F401
 from .ensemble import Ensemble  # noqa: F401
 from . import checkpoint, cosmo, ic  # noqa: F401
 from ._lib import LIB_PATH, QjError, lib  # noqa: F401

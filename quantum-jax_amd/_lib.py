@@ -78,6 +78,7 @@ SIGNATURES = {
     "qj_kdk_steps_sched": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), i32, dp, dp, vp, vp]),
     "qj_adjoint_scratch_size": (C.c_size_t, [vp]),
     "qj_star_scratch_size": (C.c_size_t, [vp, i64]),
+    "qj_bh_accretion": (i32, [vp, vp, vp, vp, vp, i64, f64, f64, f64, f64, f64, vp]),
     "qj_kdk_record": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), i32,
                             C.POINTER(QjTapeStep), vp, vp, vp]),
     "qj_kdk_adjoint": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), i32,

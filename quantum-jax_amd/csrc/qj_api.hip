@@ -814,6 +814,21 @@ int qj_kdk_steps_sched(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps
              : kdk_steps_t<double>(p, st, pr, nsteps, work, s, kick_dt, drift_dt);
 }
 
+int qj_bh_accretion(qj_plan* p, void* psi, const double* pos, const double* vel, double* m_bh,
+                    long long n_bh, double G, double c, double sigma, double m_per_hbar,
+                    double dt, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!psi || !pos || !vel || !m_bh || n_bh < 1)
+    return fail(QJ_EARG, "qj_bh_accretion: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = QJ_PRECISION(
+      p,
+      launch_bh_accretion<float>(p->d, psi, pos, vel, m_bh, n_bh, G, c, sigma, m_per_hbar, dt, s),
+      launch_bh_accretion<double>(p->d, psi, pos, vel, m_bh, n_bh, G, c, sigma, m_per_hbar, dt,
+                                  s));
+  return e == hipSuccess ? QJ_OK : hipfail(e, "qj_bh_accretion");
+}
+
 size_t qj_star_scratch_size(const qj_plan* p, long long np) {
   if (!p || np < 0) return 0;
   return star_scratch_bytes(p->d, np);

@@ -161,6 +161,12 @@ hipError_t launch_tile_deposit(const Dims& d, const double* pos, const double* m
                                double m_scalar, long long np, const StarBins& sb, void* rho,
                                hipStream_t s);
 
+// Accretion onto black hole 0 (quantum-bh.py:306-339), in place on psi and m_bh.
+template <class T>
+hipError_t launch_bh_accretion(const Dims& d, void* psi, const double* pos, const double* vel,
+                               double* m_bh, long long n_bh, double G, double c, double sigma,
+                               double mph, double dt, hipStream_t s);
+
 // Sets the message returned by qj_last_error() and returns code (qj_api.hip).
 int set_error(int code, const char* msg);
 

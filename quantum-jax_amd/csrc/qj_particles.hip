@@ -390,6 +390,46 @@ struct k_tile_deposit {
   }
 };
 
+// Accretion of the field onto black hole 0 (quantum-bh.py:306-339): the cell
+// that holds it (floored, per-axis cell size dx as in the reference), the
+// rate dM/dt = 32 pi^2 (G m)^3 (m/hbar) |psi|^2 / (c^3 v (1 - exp(-xi))) with
+// v = sqrt(sigma^2 + |vel|^2) and xi = 2 pi G m (m/hbar) / v, the amplitude
+// at that cell lowered by sqrt(dM/dt dt / vol) at fixed phase, and dM/dt dt
+// added to the mass array (the reference adds the scalar to every entry).
+// One thread, FP64.
+template <class T>
+struct k_bh_accretion {
+  static constexpr int kThreads = 64;
+  static __device__ __forceinline__ void run(CicGeom g, cpx<T>* __restrict__ psi,
+                                             const double* __restrict__ pos,
+                                             const double* __restrict__ vel,
+                                             double* __restrict__ m_bh, long long n_bh, double G,
+                                             double c, double sigma, double mph, double dt) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int ii[3];
+    const int n[3] = {g.nx, g.ny, g.nz};
+  #pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double fi = floor((pos[d] - 0.5 * g.dx) / g.dx);
+      if (!(isfinite(fi) && fabs(fi) < 4.0e18)) return;
+      ii[d] = imod(fi, n[d]);
+    }
+    const long long idx = ((long long)ii[0] * g.ny + ii[1]) * g.nz + ii[2];
+    const cpx<T> p = psi[idx];
+    const double amp = sqrt((double)p.r * p.r + (double)p.i * p.i);
+    const double th = atan2((double)p.i, (double)p.r);
+    const double v = sqrt(sigma * sigma + vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]);
+    const double Gm = G * m_bh[0];
+    const double xi = 2.0 * M_PI * Gm * mph / v;
+    const double dMdt = 32.0 * M_PI * M_PI * (Gm * Gm * Gm) * mph * (amp * amp) /
+                        (c * c * c * v * (1.0 - exp(-xi)));
+    const double na = amp - sqrt(dMdt * dt / g.vol);
+    psi[idx] = {(T)(na * cos(th)), (T)(na * sin(th))};
+    const double dm = dMdt * dt;
+    for (long long b = 0; b < n_bh; ++b) m_bh[b] = m_bh[b] + dm;
+  }
+};
+
 inline int pblocks(long long np) { return (int)((np + 255) / 256); }
 
 }  // namespace
@@ -524,6 +564,15 @@ hipError_t launch_tile_deposit(const Dims& d, const double* pos, const double* m
   return hipGetLastError();
 }
 
+template <class T>
+hipError_t launch_bh_accretion(const Dims& d, void* psi, const double* pos, const double* vel,
+                               double* m_bh, long long n_bh, double G, double c, double sigma,
+                               double mph, double dt, hipStream_t s) {
+  launch<k_bh_accretion<T>>(1, 64, s, make_geom(d), (cpx<T>*)psi, pos, vel, m_bh, n_bh, G, c,
+                            sigma, mph, dt);
+  return hipGetLastError();
+}
+
 #define QJ_PINST(T)                                                                              \
   template hipError_t launch_cic_indices<T>(const Dims&, const double*, long long, int*, int*,     \
                                             double*, double*, hipStream_t);                        \
@@ -544,7 +593,10 @@ hipError_t launch_tile_deposit(const Dims& d, const double* pos, const double* m
                                                  const void*, double, double, double, int,         \
                                                  const StarBins&, hipStream_t);                    \
   template hipError_t launch_tile_deposit<T>(const Dims&, const double*, const double*, double,    \
-                                             long long, const StarBins&, void*, hipStream_t);
+                                             long long, const StarBins&, void*, hipStream_t);     \
+  template hipError_t launch_bh_accretion<T>(const Dims&, void*, const double*, const double*,    \
+                                             double*, long long, double, double, double, double,  \
+                                             double, hipStream_t);
 
 QJ_PINST(float)
 QJ_PINST(double)

@@ -182,3 +182,18 @@ __all__ = [
     "bin_particles", "bin_stars", "interp", "get_acceleration", "apply_grav_accel",
     "solve_hydro",
 ]
+
+
+def do_accretion(psi, pos, vel, m_bh, dt: float, plan: Plan, phys: Physics, G: float, c: float,
+                 sigma: float):
+    """(psi, m_bh) after accretion onto black hole 0; quantum-bh.py:306-339."""
+    psi = plan.field(psi, "psi", complex_=True).clone()
+    plan.particles(pos, "pos")
+    plan.particles(vel, "vel")
+    m = plan.particles(m_bh, "m_bh", cols=0).clone()
+    with torch.cuda.device(plan.device):
+        _lib.check(_lib.lib().qj_bh_accretion(plan.handle, ptr(psi), ptr(pos), ptr(vel), ptr(m),
+                                              m.shape[0], float(G), float(c), float(sigma),
+                                              phys.m_per_hbar, float(dt), plan.stream()),
+                   "qj_bh_accretion")
+    return psi, m

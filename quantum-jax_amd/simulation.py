@@ -197,3 +197,43 @@ def compute_step_gas(psi, rho, vx, vy, vz, t, *, plan: Plan, phys: Physics, dt: 
     sim.step(1)
     st = sim.state()
     return st["psi"], st["rho"], st["vx"], st["vy"], st["vz"], st["t"]
+
+
+class BHSimulation(KDKSimulation):
+    """Black holes on the field (quantum-bh.py:345-381).
+
+    The holes are particles with per-hole masses m_bh (state), and after every
+    step hole 0 accretes from the field at its cell (quantum-bh.py:306-339).
+    Accretion changes psi and the masses between a step's closing kick and
+    the next step's opening kick, so here each step applies its closing kick,
+    accretes, and the next step solves the potential again (two Poisson
+    solves per step, as in the reference), instead of merging the kicks.
+    """
+
+    def __init__(self, plan: Plan, phys: Physics, dt: float, psi: torch.Tensor,
+                 pos: torch.Tensor, vel: torch.Tensor, m_bh: torch.Tensor, G: float, c: float,
+                 sigma: float, t: float = 0.0):
+        super().__init__(plan, phys, dt, psi, pos, vel, mass=m_bh, t=t)
+        self.G, self.c, self.sigma = float(G), float(c), float(sigma)
+
+    def step(self, n: int = 1):
+        L = _lib.lib()
+        for _ in range(max(0, int(n))):
+            super().step(1)
+            self.finalize()
+            with torch.cuda.device(self.plan.device):
+                _lib.check(L.qj_bh_accretion(self.plan.handle, ptr(self._psi), ptr(self.pos),
+                                             ptr(self.vel), ptr(self.mass_t), self.np, self.G,
+                                             self.c, self.sigma, self.phys.m_per_hbar, self.dt,
+                                             self.plan.stream()), "qj_bh_accretion")
+            self._st.grid_valid = 0  # the next step solves for the accreted state
+        return self
+
+    @property
+    def m_bh(self) -> torch.Tensor:
+        return self.mass_t
+
+    def state(self) -> dict:
+        d = super().state()
+        d["m_bh"] = self.mass_t
+        return d

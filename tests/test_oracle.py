@@ -446,3 +446,41 @@ def test_external_point_mass_profile():
     # cells nearest the centre sit at r = sqrt(3)/16; softening dx/2 = 1/16
     assert V.min() == pytest.approx(-1.0 / (np.sqrt(3) / 16 + 1 / 16))
     assert np.all(np.diff(V.ravel()[np.argsort(r.ravel())]) >= -1e-15)  # rises with r
+
+
+# ---------------------------------------------------------------------------
+# black-hole accretion (quantum-bh.py:306-339)
+# ---------------------------------------------------------------------------
+def _bh_case(amp_scale=1.0):
+    g = O.Grid(16, 16, 16)
+    ph = O.Physics(m_per_hbar=2.0)
+    psi = amp_scale * O.filtered_gaussian_psi(g, 8, sigma_k=6 * np.pi)
+    pos = np.array([[0.31, 0.52, 0.77], [0.1, 0.2, 0.3]])
+    vel = np.array([[0.3, -0.2, 0.1], [0.0, 0.0, 0.0]])
+    m_bh = np.array([0.05, 0.02])
+    return g, ph, psi, pos, vel, m_bh
+
+
+def test_accretion_invariants():
+    g, ph, psi, pos, vel, m_bh = _bh_case()
+    p2, m2 = O.do_accretion(psi, pos, vel, m_bh, g, ph, G=1.0, c=50.0, sigma=0.4, dt=1e-3)
+    changed = np.flatnonzero(p2 != psi)
+    # one cell (the floored cell of hole 0), its phase kept
+    cell = tuple(int(np.floor((pos[0, d] - 0.5 / 16) * 16)) % 16 for d in range(3))
+    assert changed.tolist() == [np.ravel_multi_index(cell, g.shape)]
+    assert np.angle(p2[cell]) == pytest.approx(np.angle(psi[cell]), abs=1e-14)
+    # the mass taken is (|psi| - |psi'|)^2 vol, added to every entry of m_bh
+    # (m2 - m_bh: an increment ~1e-9 on masses ~0.05 keeps ~8 digits)
+    dm = (np.abs(psi[cell]) - np.abs(p2[cell])) ** 2 / 16 ** 3
+    assert np.allclose(m2 - m_bh, dm, rtol=1e-7, atol=0)
+    assert dm > 0
+
+
+def test_accretion_rate_scales_with_density():
+    # dM/dt is proportional to |psi|^2 at the cell: sqrt(dM/dt) to |psi|
+    out = []
+    for s in (1.0, 2.0):
+        g, ph, psi, pos, vel, m_bh = _bh_case(s)
+        p2, m2 = O.do_accretion(psi, pos, vel, m_bh, g, ph, G=1.0, c=50.0, sigma=0.4, dt=1e-3)
+        out.append(m2[0] - m_bh[0])
+    assert out[1] == pytest.approx(4.0 * out[0], rel=1e-12)
