@@ -190,11 +190,14 @@ int qj_kdk_finalize(qj_plan* plan, qj_state* state, const qj_params* prm, void* 
 
 /* ---- adjoint of the KDK step (no counterpart in the reference, which
  * never differentiates despite README.md:7) ----------------------------
- * qj_kdk_record runs nsteps unmerged steps (quantum-stars.py:295-330; the
- * state must be finalized) and stores, per step, psi at the start (psi0),
- * psi after the drift (psi2), the grid after the step's Poisson solve
- * (grid1, 5N reals as in qj_state) and the star positions before and after
- * the drift (x0, x1); grid0 (nullable) receives the grid at the start.
+ * qj_kdk_record runs nsteps steps (quantum-stars.py:295-330; the state
+ * must be finalized) and stores, per step, psi after the drift (psi2), the
+ * grid after the step's Poisson solve (grid1, 5N reals as in qj_state), the
+ * star positions before and after the drift (x0, x1), and psi at the start
+ * (psi0: needed for step 0 only, NULL allowed for later steps); grid0
+ * (nullable) receives the grid at the start.  The adjoint merges the
+ * closing half kick of each step with the opening one of the next (one
+ * adjoint Poisson solve per step).
  * qj_kdk_adjoint then maps the adjoint of the end state (psi_bar, x_bar,
  * v_bar; psi_bar = dL/dRe psi + i dL/dIm psi) back through those steps in
  * place and accumulates m_bar (dL/dm per star).  scratch holds

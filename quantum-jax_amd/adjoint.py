@@ -65,8 +65,10 @@ class _Tape:
     def __init__(self, plan: Plan, nsteps: int, n_p: int):
         dev = plan.device
         N = plan.ncell
-        self.psi0 = torch.empty((nsteps,) + plan.shape, dtype=plan.complex_dtype, device=dev)
-        self.psi2 = torch.empty_like(self.psi0)
+        # psi at the start is needed for the first step only (the adjoint merges
+        # each closing half kick with the next opening one)
+        self.psi0 = torch.empty((1,) + plan.shape, dtype=plan.complex_dtype, device=dev)
+        self.psi2 = torch.empty((nsteps,) + plan.shape, dtype=plan.complex_dtype, device=dev)
         self.grid1 = torch.empty((nsteps, 5 * N), dtype=plan.real_dtype, device=dev)
         self.grid0 = torch.empty(5 * N, dtype=plan.real_dtype, device=dev)
         self.x0 = torch.empty((nsteps, max(n_p, 1), 3), dtype=torch.float64, device=dev)
@@ -74,7 +76,7 @@ class _Tape:
         self.steps = (_lib.QjTapeStep * nsteps)()
         for k in range(nsteps):
             t = self.steps[k]
-            t.psi0 = self.psi0[k].data_ptr()
+            t.psi0 = self.psi0[0].data_ptr() if k == 0 else None
             t.psi2 = self.psi2[k].data_ptr()
             t.grid1 = self.grid1[k].data_ptr()
             t.x0 = self.x0[k].data_ptr() if n_p else None

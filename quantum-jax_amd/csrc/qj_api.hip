@@ -706,7 +706,7 @@ int kdk_record_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, qj_t
   if (grid0) QJ_CHECK(copy_async(grid0, st->grid, grid_bytes(p), s), "tape grid0");
   for (int k = 0; k < nsteps; ++k) {
     qj_tape_step& t = tape[k];
-    QJ_CHECK(copy_async(t.psi0, st->psi, cb, s), "tape psi0");
+    if (t.psi0) QJ_CHECK(copy_async(t.psi0, st->psi, cb, s), "tape psi0");
     if (st->np > 0) QJ_CHECK(copy_async(t.x0, st->pos, xb, s), "tape x0");
     if (int rc = kdk_steps_t<T>(p, st, pr, 1, work, s)) return rc;
     QJ_CHECK(copy_async(t.psi2, st->psi, cb, s), "tape psi2");
@@ -730,18 +730,21 @@ int kdk_adjoint_t(qj_plan* p, const qj_state* st, const qj_params* pr, int nstep
   T* Vbar = (T*)a->scratch;
   T* abar4 = Vbar + n;
   T* rhobar = abar4 + 4 * n;
-  // one adjoint half step: the closing kick of psi (V, psi_ref) and of the
-  // stars (A4, pos), the density and Poisson adjoints, the deposit adjoint
-  auto half = [&](const void* grid, const void* psi_ref, const double* pos, double dt_xv) -> int {
+  // one adjoint half step: the kick of psi (V, psi_ref) and of the stars
+  // (A4, pos) with kick time hh (phase coefficient cc), the density and
+  // Poisson adjoints, the deposit adjoint
+  auto half = [&](const void* grid, const void* psi_ref, const double* pos, double dt_xv,
+                  double hh, double cc) -> int {
     const void* V = grid;
     const void* A4 = (const char*)grid + (size_t)n * sizeof(T);
     QJ_CHECK(fill<T>(abar4, 4 * n, (T)0, s), "abar clear");
     if (stars)
       QJ_STAGE(kStageAdjParticles,
-               launch_interp_adj<T>(p->d, pos, st->np, A4, h, a->v_bar, dt_xv, a->x_bar, abar4, s),
+               launch_interp_adj<T>(p->d, pos, st->np, A4, hh, a->v_bar, dt_xv, a->x_bar, abar4,
+                                    s),
                "interp adjoint");
     QJ_STAGE(kStageAdjKick,
-             launch_kick_adj<T>(a->psi_bar, psi_ref, V, st->v_ext, c, Vbar, n, s), "kick adjoint");
+             launch_kick_adj<T>(a->psi_bar, psi_ref, V, st->v_ext, cc, Vbar, n, s), "kick adjoint");
     QJ_STAGE(kStageAdjPoisson,
              launch_adj_poisson<T>(p->d, tb, Vbar, abar4, pr->four_pi_G, pr->dc_mode, w.H, w.Uz,
                                    w.Uy, rhobar, s),
@@ -754,19 +757,30 @@ int kdk_adjoint_t(qj_plan* p, const qj_state* st, const qj_params* pr, int nstep
                "deposit adjoint");
     return QJ_OK;
   };
+  // Reverse sweep.  The closing kick of step k-1 and the opening kick of step
+  // k act with one potential (grid1 of step k-1) at one set of positions, so
+  // their adjoints are one half step with doubled kick time, taken at the
+  // input of the pair (psi2 and x1 of step k-1): Im(conj(psi_bar) psi) is
+  // the same before and after the common phase, and both star kicks see the
+  // same v_bar.  One adjoint Poisson solve per step, as the forward merges
+  // the kicks.
   for (int k = nsteps - 1; k >= 0; --k) {
     const qj_tape_step& t = tape[k];
-    const void* g0 = (k == 0) ? grid0 : tape[k - 1].grid1;
-    // closing half step (grid1 at x1, psi2)
-    if (int rc = half(t.grid1, t.psi2, t.x1, 0.0)) return rc;
+    // closing half step of the last step
+    if (k == nsteps - 1)
+      if (int rc = half(t.grid1, t.psi2, t.x1, 0.0, h, c)) return rc;
     // drift: psi_bar <- F^H psi_bar (conjugate phase tables = drift of -dt);
     // pos1 = pos0 + dt vel1 is folded into the next interp adjoint (dt_xv)
     for (int ax = 0; ax < 3; ++ax)
       QJ_STAGE(kStageAdjDrift,
                launch_drift_axis<T>(p->d, tb, ax, a->psi_bar, nullptr, nullptr, nullptr, 0.0, 0, s, 1),
                "drift adjoint");
-    // opening half step (grid0 at x0, psi0)
-    if (int rc = half(g0, t.psi0, t.x0, dt)) return rc;
+    if (k > 0) {  // opening half of step k with the closing half of step k-1
+      const qj_tape_step& u = tape[k - 1];
+      if (int rc = half(u.grid1, u.psi2, u.x1, dt, 2.0 * h, 2.0 * c)) return rc;
+    } else {  // opening half step of the first step (grid0 at x0, psi0)
+      if (int rc = half(grid0, t.psi0, t.x0, dt, h, c)) return rc;
+    }
   }
   return QJ_OK;
 }
@@ -849,7 +863,7 @@ int qj_kdk_record(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, qj_
   if (st->pending != 0.0 || st->pending_star != 0.0)
     return fail(QJ_EARG, "qj_kdk_record: finalize the state first (owed half kick)");
   for (int k = 0; k < nsteps; ++k)
-    if (!tape[k].psi0 || !tape[k].psi2 || !tape[k].grid1 ||
+    if ((k == 0 && !tape[k].psi0) || !tape[k].psi2 || !tape[k].grid1 ||
         (st->np > 0 && (!tape[k].x0 || !tape[k].x1)))
       return fail(QJ_EARG, "qj_kdk_record: null tape buffer");
   hipStream_t s = (hipStream_t)stream;
@@ -868,6 +882,7 @@ int qj_kdk_adjoint(qj_plan* p, const qj_state* st, const qj_params* pr, int nste
     return fail(QJ_EARG, "qj_kdk_adjoint: null adjoint buffer");
   if (st->gas[0] || pr->b_coeff != 0.0)
     return fail(QJ_EUNSUPPORTED, "qj_kdk_adjoint: gas and self-interaction have no adjoint");
+  if (nsteps > 0 && !tape[0].psi0) return fail(QJ_EARG, "qj_kdk_adjoint: null psi0 of step 0");
   hipStream_t s = (hipStream_t)stream;
   return p->dtype == QJ_C64
              ? kdk_adjoint_t<float>(p, st, pr, nsteps, tape, grid0, adj, work, s)
