@@ -730,6 +730,7 @@ int kdk_adjoint_t(qj_plan* p, const qj_state* st, const qj_params* pr, int nstep
   T* Vbar = (T*)a->scratch;
   T* abar4 = Vbar + n;
   T* rhobar = abar4 + 4 * n;
+  const bool binned = st->star_scratch && p->opt[QJ_OPT_BINNED_DEPOSIT];
   // one adjoint half step: the kick of psi (V, psi_ref) and of the stars
   // (A4, pos) with kick time hh (phase coefficient cc), the density and
   // Poisson adjoints, the deposit adjoint
@@ -737,12 +738,21 @@ int kdk_adjoint_t(qj_plan* p, const qj_state* st, const qj_params* pr, int nstep
                   double hh, double cc) -> int {
     const void* V = grid;
     const void* A4 = (const char*)grid + (size_t)n * sizeof(T);
-    QJ_CHECK(fill<T>(abar4, 4 * n, (T)0, s), "abar clear");
-    if (stars)
+    if (stars && binned) {
+      // the star kicks' adjoint deposited tile-binned (writes every cell of abar4)
+      StarBins sb = carve_star_scratch(p->d, st->np, st->star_scratch);
       QJ_STAGE(kStageAdjParticles,
-               launch_interp_adj<T>(p->d, pos, st->np, A4, hh, a->v_bar, dt_xv, a->x_bar, abar4,
-                                    s),
+               launch_interp_adj_binned<T>(p->d, pos, st->np, A4, hh, a->v_bar, dt_xv, a->x_bar,
+                                           abar4, sb, s),
                "interp adjoint");
+    } else {
+      QJ_CHECK(fill<T>(abar4, 4 * n, (T)0, s), "abar clear");
+      if (stars)
+        QJ_STAGE(kStageAdjParticles,
+                 launch_interp_adj<T>(p->d, pos, st->np, A4, hh, a->v_bar, dt_xv, a->x_bar, abar4,
+                                      s),
+                 "interp adjoint");
+    }
     QJ_STAGE(kStageAdjKick,
              launch_kick_adj<T>(a->psi_bar, psi_ref, V, st->v_ext, cc, Vbar, n, s), "kick adjoint");
     QJ_STAGE(kStageAdjPoisson,

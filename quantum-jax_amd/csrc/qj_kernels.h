@@ -161,6 +161,14 @@ hipError_t launch_tile_deposit(const Dims& d, const double* pos, const double* m
                                double m_scalar, long long np, const StarBins& sb, void* rho,
                                hipStream_t s);
 
+// Interpolation adjoint whose h vbar corner values go to abar4 through the
+// tile-binned deposit (whole tiles written, no clearing needed).
+template <class T>
+hipError_t launch_interp_adj_binned(const Dims& d, const double* pos, long long np,
+                                    const void* A4, double h, double* vbar, double dt_xv,
+                                    double* xbar, void* abar4, const StarBins& sb,
+                                    hipStream_t s);
+
 // Accretion onto black hole 0 (quantum-bh.py:306-339), in place on psi and m_bh.
 template <class T>
 hipError_t launch_bh_accretion(const Dims& d, void* psi, const double* pos, const double* vel,

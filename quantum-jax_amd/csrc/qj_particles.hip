@@ -150,17 +150,48 @@ __device__ __forceinline__ void corner_grad(const CicGeom& g, const Cic& c, int 
   gw[2] = c.wx[sx] * c.wy[sy] * dz;
 }
 
+// ---------------------------------------------------------------------------
+// Tile-binned deposit (owner computes).  The grid is cut into tiles of
+// 2^lx x 2^ly x 2^lz cells.  Each particle is registered in every tile its
+// CIC stencil touches (1 to 8, distinct); one block per tile then adds the
+// corners that fall inside its tile in LDS and writes the whole tile with
+// plain coalesced stores, zeros included.  No global float atomics, and
+// every cell of rho is overwritten (no clearing needed beforehand).
+//   count  : per-tile registration counter (zero between steps)
+//   rank   : slot of each registration within its tile (8 per particle)
+//   offset : exclusive scan of count, ntiles + 1 entries
+//   sorted : registrations grouped by tile (particle indices)
+// ---------------------------------------------------------------------------
+
+// The distinct tiles touched by the stencil of c, in a fixed order.
+__device__ __forceinline__ int stencil_tiles(const TileGeom& t, const Cic& c, int ids[8]) {
+  const int ax[2] = {c.ix[0] >> t.lx, c.ix[1] >> t.lx};
+  const int ay[2] = {c.iy[0] >> t.ly, c.iy[1] >> t.ly};
+  const int az[2] = {c.iz[0] >> t.lz, c.iz[1] >> t.lz};
+  const int nax = ax[1] != ax[0] ? 2 : 1, nay = ay[1] != ay[0] ? 2 : 1;
+  const int naz = az[1] != az[0] ? 2 : 1;
+  int n = 0;
+  for (int a = 0; a < nax; ++a)
+    for (int b = 0; b < nay; ++b)
+      for (int e = 0; e < naz; ++e) ids[n++] = (ax[a] * t.nty + ay[b]) * t.ntz + az[e];
+  return n;
+}
+
 // Adjoint of vel += h * interp(A4, pos) (and, first, of the drift
 // pos += dt*vel when dt_xv != 0: vbar += dt_xv * xbar).  With
 // acc_bar = h * vbar: abar[c] += w_c acc_bar (the transpose of the gather,
 // into (a_x, a_y, a_z, -) cells) and xbar += sum_c grad w_c (a[c] . acc_bar).
-template <class T>
+// BIN: instead of adding h vbar into abar4 with atomics, register the star
+// in the tiles of its stencil (the tile-binned deposit adds it afterwards).
+template <class T, bool BIN = false>
 struct k_interp_adj {
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(CicGeom g, const double* __restrict__ pos,
                                              long long np, const T* __restrict__ A4, double h,
                                              double* __restrict__ vbar, double dt_xv,
-                                             double* __restrict__ xbar, T* __restrict__ abar4) {
+                                             double* __restrict__ xbar, T* __restrict__ abar4,
+                                             TileGeom tg, int* __restrict__ count,
+                                             int* __restrict__ rank) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     double vb[3], xb[3];
@@ -174,13 +205,20 @@ struct k_interp_adj {
     Cic c = cic(g, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]);
     if (!c.ok) return;
     const double ab[3] = {h * vb[0], h * vb[1], h * vb[2]};
+    if constexpr (BIN) {
+      int ids[8];
+      const int n = stencil_tiles(tg, c, ids);
+      for (int k = 0; k < n; ++k) rank[8 * p + k] = atomicAdd(&count[ids[k]], 1);
+    }
   #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const int sx = kCorner[k][0], sy = kCorner[k][1], sz = kCorner[k][2];
       const int idx = (c.ix[sx] * g.ny + c.iy[sy]) * g.nz + c.iz[sz];
       const double w = c.wx[sx] * c.wy[sy] * c.wz[sz];
+      if constexpr (!BIN) {
   #pragma unroll
-      for (int d = 0; d < 3; ++d) unsafeAtomicAdd(&abar4[4 * idx + d], (T)(w * ab[d]));
+        for (int d = 0; d < 3; ++d) unsafeAtomicAdd(&abar4[4 * idx + d], (T)(w * ab[d]));
+      }
       double a[4];
       ld4(A4, idx, a);
       const double s = a[0] * ab[0] + a[1] * ab[1] + a[2] * ab[2];
@@ -225,32 +263,6 @@ struct k_deposit_adj {
   }
 };
 
-// ---------------------------------------------------------------------------
-// Tile-binned deposit (owner computes).  The grid is cut into tiles of
-// 2^lx x 2^ly x 2^lz cells.  Each particle is registered in every tile its
-// CIC stencil touches (1 to 8, distinct); one block per tile then adds the
-// corners that fall inside its tile in LDS and writes the whole tile with
-// plain coalesced stores, zeros included.  No global float atomics, and
-// every cell of rho is overwritten (no clearing needed beforehand).
-//   count  : per-tile registration counter (zero between steps)
-//   rank   : slot of each registration within its tile (8 per particle)
-//   offset : exclusive scan of count, ntiles + 1 entries
-//   sorted : registrations grouped by tile (particle indices)
-// ---------------------------------------------------------------------------
-
-// The distinct tiles touched by the stencil of c, in a fixed order.
-__device__ __forceinline__ int stencil_tiles(const TileGeom& t, const Cic& c, int ids[8]) {
-  const int ax[2] = {c.ix[0] >> t.lx, c.ix[1] >> t.lx};
-  const int ay[2] = {c.iy[0] >> t.ly, c.iy[1] >> t.ly};
-  const int az[2] = {c.iz[0] >> t.lz, c.iz[1] >> t.lz};
-  const int nax = ax[1] != ax[0] ? 2 : 1, nay = ay[1] != ay[0] ? 2 : 1;
-  const int naz = az[1] != az[0] ? 2 : 1;
-  int n = 0;
-  for (int a = 0; a < nax; ++a)
-    for (int b = 0; b < nay; ++b)
-      for (int e = 0; e < naz; ++e) ids[n++] = (ax[a] * t.nty + ay[b]) * t.ntz + az[e];
-  return n;
-}
 
 // Star step with registration counting instead of the deposit.  MOVE = false
 // only counts the current positions (the deposit of the initial state).
@@ -390,6 +402,58 @@ struct k_tile_deposit {
   }
 };
 
+// Vector form of the per-tile deposit for the adjoint of the star kicks:
+// abar[c] = sum over stars of w_c h vbar (the corner values of the atomic
+// interpolation adjoint), written as whole (a_x, a_y, a_z, 0) tiles.
+template <class T>
+struct k_tile_deposit_vec {
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(CicGeom g, TileGeom tg,
+                                             const double* __restrict__ pos,
+                                             const double* __restrict__ vbar, double h,
+                                             const int* __restrict__ offset,
+                                             const int* __restrict__ sorted,
+                                             T* __restrict__ abar4) {
+    __shared__ T acc[3][kTileMaxCells];
+    const int t = blockIdx.x;
+    const int tz = t % tg.ntz, ty = (t / tg.ntz) % tg.nty, tx = t / (tg.ntz * tg.nty);
+    const int vol = 1 << (tg.lx + tg.ly + tg.lz);
+    for (int l = threadIdx.x; l < vol; l += blockDim.x) acc[0][l] = acc[1][l] = acc[2][l] = (T)0;
+    __syncthreads();
+    const int b = offset[t], e = offset[t + 1];
+    for (int i = b + (int)threadIdx.x; i < e; i += blockDim.x) {
+      const int p = sorted[i];
+      Cic c = cic(g, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]);
+      const double ab[3] = {h * vbar[3 * p], h * vbar[3 * p + 1], h * vbar[3 * p + 2]};
+  #pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int sx = kCorner[k][0], sy = kCorner[k][1], sz = kCorner[k][2];
+        const int ix = c.ix[sx], iy = c.iy[sy], iz = c.iz[sz];
+        if ((ix >> tg.lx) != tx || (iy >> tg.ly) != ty || (iz >> tg.lz) != tz) continue;
+        const double w = c.wx[sx] * c.wy[sy] * c.wz[sz];
+        const int l = (((ix & ((1 << tg.lx) - 1)) << tg.ly | (iy & ((1 << tg.ly) - 1))) << tg.lz) |
+                      (iz & ((1 << tg.lz) - 1));
+  #pragma unroll
+        for (int d = 0; d < 3; ++d) atomicAdd(&acc[d][l], (T)(w * ab[d]));
+      }
+    }
+    __syncthreads();
+    const int x0 = tx << tg.lx, y0 = ty << tg.ly, z0 = tz << tg.lz;
+    const int mz = (1 << tg.lz) - 1, my = (1 << tg.ly) - 1;
+    for (int l = threadIdx.x; l < vol; l += blockDim.x) {
+      const int lz = l & mz, ly = (l >> tg.lz) & my, lx = l >> (tg.lz + tg.ly);
+      const long long idx = ((long long)(x0 + lx) * g.ny + (y0 + ly)) * g.nz + (z0 + lz);
+      T* o = abar4 + 4 * idx;
+      if constexpr (sizeof(T) == 4) {
+        *reinterpret_cast<float4*>(o) = make_float4(acc[0][l], acc[1][l], acc[2][l], 0.0f);
+      } else {
+        *reinterpret_cast<double2*>(o) = make_double2(acc[0][l], acc[1][l]);
+        *reinterpret_cast<double2*>(o + 2) = make_double2(acc[2][l], 0.0);
+      }
+    }
+  }
+};
+
 // Accretion of the field onto black hole 0 (quantum-bh.py:306-339): the cell
 // that holds it (floored, per-axis cell size dx as in the reference), the
 // rate dM/dt = 32 pi^2 (G m)^3 (m/hbar) |psi|^2 / (c^3 v (1 - exp(-xi))) with
@@ -481,8 +545,9 @@ hipError_t launch_interp_adj(const Dims& d, const double* pos, long long np, con
                              double h, double* vbar, double dt_xv, double* xbar, void* abar4,
                              hipStream_t s) {
   if (np <= 0) return hipSuccess;
-  launch<k_interp_adj<T>>(pblocks(np), 256, s, make_geom(d), pos, np, (const T*)A4, h, vbar, dt_xv,
-                                              xbar, (T*)abar4);
+  launch<k_interp_adj<T, false>>(pblocks(np), 256, s, make_geom(d), pos, np, (const T*)A4, h,
+                                 vbar, dt_xv, xbar, (T*)abar4, TileGeom{}, (int*)nullptr,
+                                 (int*)nullptr);
   return hipGetLastError();
 }
 
@@ -573,6 +638,24 @@ hipError_t launch_bh_accretion(const Dims& d, void* psi, const double* pos, cons
   return hipGetLastError();
 }
 
+template <class T>
+hipError_t launch_interp_adj_binned(const Dims& d, const double* pos, long long np,
+                                    const void* A4, double h, double* vbar, double dt_xv,
+                                    double* xbar, void* abar4, const StarBins& sb,
+                                    hipStream_t s) {
+  const TileGeom tg = make_tiles(d);
+  if (np > 0)
+    launch<k_interp_adj<T, true>>(pblocks(np), 256, s, make_geom(d), pos, np, (const T*)A4, h,
+                                  vbar, dt_xv, xbar, (T*)nullptr, tg, sb.count, sb.rank);
+  launch<k_bin_scan>(1, 1024, s, sb.count, sb.offset, tg.ntiles);
+  if (np > 0)
+    launch<k_bin_scatter>(pblocks(np), 256, s, make_geom(d), tg, pos, np, (const int*)sb.rank,
+                          (const int*)sb.offset, sb.sorted);
+  launch<k_tile_deposit_vec<T>>(tg.ntiles, 256, s, make_geom(d), tg, pos, (const double*)vbar, h,
+                                (const int*)sb.offset, (const int*)sb.sorted, (T*)abar4);
+  return hipGetLastError();
+}
+
 #define QJ_PINST(T)                                                                              \
   template hipError_t launch_cic_indices<T>(const Dims&, const double*, long long, int*, int*,     \
                                             double*, double*, hipStream_t);                        \
@@ -594,6 +677,9 @@ hipError_t launch_bh_accretion(const Dims& d, void* psi, const double* pos, cons
                                                  const StarBins&, hipStream_t);                    \
   template hipError_t launch_tile_deposit<T>(const Dims&, const double*, const double*, double,    \
                                              long long, const StarBins&, void*, hipStream_t);     \
+  template hipError_t launch_interp_adj_binned<T>(const Dims&, const double*, long long,           \
+                                                  const void*, double, double*, double, double*,   \
+                                                  void*, const StarBins&, hipStream_t);            \
   template hipError_t launch_bh_accretion<T>(const Dims&, void*, const double*, const double*,    \
                                              double*, long long, double, double, double, double,  \
                                              double, hipStream_t);

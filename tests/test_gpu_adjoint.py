@@ -150,3 +150,22 @@ def test_c64_adjoint_tracks_c128(qj, cuda):
     rel = lambda u, w: ((u.to(w.dtype) - w).norm() / w.norm()).item()
     assert rel(res[C64][1], res[C128][1]) < 1e-4
     assert rel(res[C64][3], res[C128][3]) < 1e-4
+
+
+@pytest.mark.parametrize("cd,tol", [(C128, 1e-11), (C64, 1e-4)])
+def test_binned_and_atomic_adjoint_deposits_agree(qj, cuda, cd, tol):
+    """The tile-binned transpose of the star interpolation (whole abar tiles
+    written from LDS sums) and the float-atomic one give the same gradients
+    up to the order of the additions."""
+    ad = importlib.import_module("quantum-jax_amd.adjoint")
+    g = O.Grid(32, 32, 16)
+    phys = qj.Physics(rho_bar=1.1, four_pi_G=20.0)
+    res = []
+    for binned in (1, 0):
+        plan = qj.Plan(32, 32, 16, dtype=cd)
+        plan.set_option(qj._lib.OPT_BINNED_DEPOSIT, binned)
+        psi, pos, vel, m, target = case(g, 500, 9, cd)
+        res.append(grads(qj, ad, plan, phys, 2e-3, 6, 3, psi, pos, vel, m, target))
+    rel = lambda u, w: ((u - w).norm() / w.norm()).item()
+    for a, b in zip(res[0][1:], res[1][1:]):
+        assert rel(a, b) < tol
