@@ -79,3 +79,30 @@ def test_package_ic_equals_oracle_ic():
     ga = ic.gas_ic(g.shape, 5, (g.Lx, g.Ly, g.Lz))
     gb = O.gas_ic(g, 5)
     assert np.allclose(ga[0], gb[0], rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("shape", [(48, 16, 16), (16, 4, 16), (16, 16, 1024), (0, 8, 8)])
+def test_unsupported_plan_shapes_are_rejected(shape):
+    """Argument checks of qj_plan_create return before any device work."""
+    import importlib
+    _lib = importlib.import_module("quantum-jax_amd._lib")
+    core = importlib.import_module("quantum-jax_amd.core")
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    assert L.qj_plan_create(ctypes.byref(h), *shape, 1.0, 1.0, 1.0, _lib.QJ_C64) == -2
+    assert "powers of two" in L.qj_last_error().decode()
+    assert not h.value
+    with pytest.raises(ValueError, match="powers of two"):
+        core.Plan(*shape)
+
+
+def test_bad_plan_arguments_are_rejected():
+    import importlib
+    _lib = importlib.import_module("quantum-jax_amd._lib")
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    assert L.qj_plan_create(ctypes.byref(h), 16, 16, 16, 1.0, 0.0, 1.0, _lib.QJ_C64) == -1
+    assert "box lengths" in L.qj_last_error().decode()
+    assert L.qj_plan_create(ctypes.byref(h), 16, 16, 16, 1.0, 1.0, 1.0, 7) == -1
+    assert "dtype" in L.qj_last_error().decode()
+    assert L.qj_plan_create(None, 16, 16, 16, 1.0, 1.0, 1.0, _lib.QJ_C64) == -1
