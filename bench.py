@@ -455,7 +455,10 @@ def main():
         log(f"gas: {ng}^3 c64 psi + FP32 gas, {kg} steps")
         rho_g0, vx0, vy0, vz0 = qj.ic.gas_ic((ng, ng, ng), 17)
         rb = float(np.mean(np.abs(psi0) ** 2 + rho_g0))
-        gphys = qj.Physics(rho_bar=rb, cs=csg)
+        # the benchmark's gas scheme: MUSCL-Hancock, minmod, HLL flux, CFL 0.4
+        # (dt = min(dt, 0.4 dx / max(cs + |v|)) from the device each step)
+        cflg = 0.4
+        gphys = qj.Physics(rho_bar=rb, cs=csg, flux="hll")
         res = {}
         for tag, cd in (("c64", torch.complex64), ("c128", torch.complex128)):
             rd = torch.float32 if cd == torch.complex64 else torch.float64
@@ -463,18 +466,19 @@ def main():
             gas = [torch.as_tensor(a).to(dev, rd) for a in (rho_g0, vx0, vy0, vz0)]
             sg = qj.KDKSimulation(pg, gphys, dtg, torch.as_tensor(psi0).to(dev, cd), gas=gas)
             M0 = diag_mod.mass(pg, sg.psi, rho_gas=sg.gas_fields()[0])
-            sg.step(2)
+            sg.step_cfl(2, cfl=cflg)
             torch.cuda.synchronize()
             ev0.record()
-            sg.step(kg)
+            sg.step_cfl(kg, cfl=cflg)
             sg.finalize()
             ev1.record()
             torch.cuda.synchronize()
             ms_g = ev0.elapsed_time(ev1) / kg
+            dts_g = sg.dts[2:]
             gf = sg.gas_fields()
             vmax = max(float(v.abs().max().item()) for v in gf[1:])
             res[tag] = dict(ms=ms_g, psi=sg.psi.to(torch.complex128).clone(),
-                            rho=gf[0].double().clone(),
+                            rho=gf[0].double().clone(), dt_min=min(dts_g), dt_max=max(dts_g),
                             M0=M0, M1=diag_mod.mass(pg, sg.psi, rho_gas=gf[0]), vmax=vmax)
             if tag == "c64":   # stage timing after the state was taken
                 L = lib_mod.lib()
@@ -483,21 +487,40 @@ def main():
                 lib_mod.check(L.qj_timing_read(pg.handle, ms_arr, cnt_arr, nst), "timing_read")
                 L.qj_timing_enable(pg.handle, 0)
                 gst = {names[k]: round(ms_arr[k] / cnt_arr[k], 5) for k in range(nst) if cnt_arr[k]}
+                # the reference's own scheme: Rusanov flux of the averaged
+                # state at the fixed dt, no per-step CFL read-back
+                sr = qj.KDKSimulation(pg, qj.Physics(rho_bar=rb, cs=csg), dtg,
+                                      torch.as_tensor(psi0).to(dev, cd),
+                                      gas=[torch.as_tensor(a).to(dev, rd)
+                                           for a in (rho_g0, vx0, vy0, vz0)])
+                sr.step(2)
+                torch.cuda.synchronize()
+                ev0.record()
+                sr.step(kg)
+                sr.finalize()
+                ev1.record()
+                torch.cuda.synchronize()
+                ms_rus = ev0.elapsed_time(ev1) / kg
+                del sr
             del sg, pg, gas
             torch.cuda.empty_cache()
         a, b = res["c64"], res["c128"]
         dxg = 1.0 / ng
         out["gas"] = {
             "workload": f"{ng}^3 c64 psi + FP32 isothermal gas rho=0.2(1+0.1 g), v=0, cs={csg}, "
-                        f"dt={dtg}, {kg} steps (after 2 warm-up steps), rho_bar = IC mean",
+                        f"MUSCL-Hancock + minmod + HLL flux, CFL {cflg} with dt = min({dtg}, "
+                        f"{cflg} dx / max(cs + |v|)) from the device each step, {kg} steps "
+                        f"(after 2 warm-up steps), rho_bar = IC mean",
             "value": ng ** 3 / (a["ms"] * 1e-3), "unit": "cell-updates/s", "ms_per_step": a["ms"],
             "c128_ms_per_step": b["ms"],
+            "dt_range": [a["dt_min"], a["dt_max"]],
+            "rusanov_fixed_dt_ms_per_step": ms_rus,
             "stage_ms": gst,
             "psi_rel_l2_vs_fp64": ((a["psi"] - b["psi"]).norm() / b["psi"].norm()).item(),
             "gas_rho_rel_l2_vs_fp64": ((a["rho"] - b["rho"]).norm() / b["rho"].norm()).item(),
             "mass_drift_c64": abs(a["M1"] - a["M0"]) / a["M0"],
             "mass_drift_c128": abs(b["M1"] - b["M0"]) / b["M0"],
-            "max_cfl": dtg * (csg + a["vmax"]) / dxg,
+            "max_cfl": a["dt_max"] * (csg + a["vmax"]) / dxg,
         }
         log(f"gas: {out['gas']}")
         del res, a, b
@@ -575,14 +598,22 @@ def main():
             st = O.compute_step_stars(*st, dt, m_s, g, ph)
             times.append(time.perf_counter() - t)
         tmed = float(np.median(times))
+        # the same step with one FFT worker (the scalar figure)
+        O.set_workers(1)
+        t = time.perf_counter()
+        st = O.compute_step_stars(*st, dt, m_s, g, ph)
+        t1 = time.perf_counter() - t
+        O.set_workers(thr)
         out["cpu_baseline"] = {
             "value": N / tmed, "unit": "cell-updates/s", "cores": thr, "kind": "port",
             "sample": f"1 warm-up + {len(times)} timed steps of the same {n}^3 + {n_p}-star "
                       f"workload, FP64 NumPy/SciPy restatement of the reference operator order "
                       f"(unmerged kicks, 14 FFTs/step, scipy.fft workers={thr}); "
-                      f"median {tmed:.2f} s/step; CPU: {cpu_model()}",
+                      f"median {tmed:.2f} s/step; then 1 step with workers=1: {t1:.2f} s; "
+                      f"CPU: {cpu_model()}",
+            "value_workers_1": N / t1,
         }
-        log(f"cpu: {tmed:.2f} s/step")
+        log(f"cpu: {tmed:.2f} s/step ({thr} workers), {t1:.2f} s/step (1 worker)")
 
     print(json.dumps(out), flush=True)
 

@@ -109,11 +109,19 @@ int qj_apply_grav_accel(qj_plan* plan, const void* rho, void* vx, void* vy, void
                         double rho_bar, double four_pi_G, int dc_mode, void* a4, double* diag,
                         void* work, void* stream);
 
+/* Gas face fluxes.  QJ_FLUX_RUSANOV is the reference's (the flux of the
+ * averaged state plus Rusanov dissipation, quantum-fluid.py:238-264);
+ * QJ_FLUX_HLL is the HLL flux with Davis signal speeds, which the reference
+ * does not have. */
+#define QJ_FLUX_RUSANOV 0
+#define QJ_FLUX_HLL 1
+
 /* solve_hydro(rho, vx, vy, vz, dt); quantum-fluid.py:281-349 (cubic cells;
- * out-of-place: the *_o buffers must not alias the inputs). */
+ * out-of-place: the *_o buffers must not alias the inputs); flux is one of
+ * QJ_FLUX_*. */
 int qj_solve_hydro(qj_plan* plan, const void* rho, const void* vx, const void* vy, const void* vz,
-                   double dt, double cs, void* rho_o, void* vx_o, void* vy_o, void* vz_o,
-                   void* stream);
+                   double dt, double cs, int flux, void* rho_o, void* vx_o, void* vy_o,
+                   void* vz_o, void* stream);
 
 /* ---- fused stepping: compute_step + update + fori_loop ----------------
  * quantum-stars.py:295-351,454 (stars), quantum-fluid.py:414-459 (gas) and
@@ -163,16 +171,37 @@ typedef struct {
   double b_coeff;
   double cs;
   int dc_mode;
+  /* gas face flux: QJ_FLUX_RUSANOV (quantum-fluid.py:238-264) or QJ_FLUX_HLL */
+  int flux;
 } qj_params;
 
 int qj_kdk_steps(qj_plan* plan, qj_state* state, const qj_params* prm, int nsteps, void* work,
                  void* stream);
 /* nsteps steps with per-step kick and drift times (host arrays of nsteps):
- * the kicks of step k use kick_dt[k]/2 and the drift drift_dt[k].  The
- * comoving step of quantum-cosmo.py:166-197 is kick_dt = dt pot_fac and
- * drift_dt = dt kin_fac.  Psi-only states. */
+ * the kicks of step k use kick_dt[k]/2 and the drift (and the gas update)
+ * drift_dt[k].  The comoving step of quantum-cosmo.py:166-197 is
+ * kick_dt = dt pot_fac and drift_dt = dt kin_fac; a CFL-limited gas run
+ * (qj_gas_cfl_dt) passes kick_dt = drift_dt = its dt.  Psi and gas states
+ * (no stars). */
 int qj_kdk_steps_sched(qj_plan* plan, qj_state* state, const qj_params* prm, int nsteps,
                        const double* kick_dt, const double* drift_dt, void* work, void* stream);
+
+/* CFL time step of the gas state (quantum-fluid-accretion.py:696-701,
+ * isothermal: c_eff = cs): out[0] = max over cells of cs + |v|, with v the
+ * live velocity plus the owed half kick; out[1] = min(dt_max,
+ * cfl dx / (out[0] + 1e-30)).  out: 2 doubles of device memory (a NaN
+ * velocity gives NaN).  Stream-ordered; the caller reads out[1] to choose
+ * the next step's dt. */
+int qj_gas_cfl_dt(qj_plan* plan, const qj_state* state, const qj_params* prm, double cfl,
+                  double dt_max, double* out, void* stream);
+
+/* nsteps gas steps, each with dt = the qj_gas_cfl_dt value of the state it
+ * starts from (quantum-fluid-accretion.py:696-701: the CFL cap is evaluated
+ * inside every step).  Synchronises the stream once per step to read dt;
+ * dts (host, nsteps, may be NULL) receives the steps taken; cfl_dev is 2
+ * doubles of device scratch.  Psi and gas states (no stars). */
+int qj_kdk_steps_cfl(qj_plan* plan, qj_state* state, const qj_params* prm, int nsteps, double cfl,
+                     double dt_max, double* dts, double* cfl_dev, void* work, void* stream);
 /* do_accretion(psi, pos, vel, m_bh) of quantum-bh.py:306-339, in place:
  * black hole 0 takes dM/dt dt from the field at its cell (the reference
  * lowers the amplitude there by sqrt(dM/dt dt / vol) at fixed phase) and

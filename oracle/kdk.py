@@ -22,7 +22,13 @@ Parity status per function (pins live in ``tests/test_oracle.py``):
   compute_step_stars/_plain   pinned (time reversibility, 2nd order in dt,
                               mass conservation)
   hydro (solve_hydro & co.)   pinned (uniform state, conservation, minmod
-                              identity, sound speed, 1-D Riemann symmetry)
+                              identity, sound speed, 1-D Riemann symmetry,
+                              exact isothermal Riemann solution)
+  cfl_dt                      pinned (Courant number of a uniform flow equals
+                              cfl; the dt_max cap)
+  get_flux_hll                pinned (consistency F(U,U) = F(U), supersonic
+                              upwinding, the HLL integral-average form,
+                              exact isothermal Riemann solution)
   compute_step_gas            pinned (mass conservation, reversibility of the
                               spectral part, uniform-gas reduction to plain)
 Unpinned: the reference's FP32 rounding trajectory (needs a JAX import).
@@ -44,7 +50,7 @@ __all__ = [
     "interp_grid", "acceleration_grid", "get_acceleration",
     "compute_step_stars", "compute_step_plain",
     "get_conserved", "get_primitive", "get_gradient", "extrap_to_face",
-    "apply_fluxes", "get_flux", "slope_limiter", "solve_hydro",
+    "apply_fluxes", "get_flux", "get_flux_hll", "slope_limiter", "solve_hydro", "cfl_dt",
     "apply_grav_accel", "compute_step_gas",
     "mass_total", "energy_total",
     "filtered_gaussian_psi", "filtered_gaussian_field", "star_ic", "gas_ic",
@@ -116,6 +122,7 @@ class Physics:
     rho_bar: float = 1.0         # quantum-stars.py:83
     b_coeff: float = 0.0         # quantum-stars.py:117-125 (self-interaction)
     cs: float = 0.1              # quantum-fluid.py:71 (sound speed)
+    flux: str = "rusanov"        # gas face flux: "rusanov" (quantum-fluid.py:238-264) or "hll"
 
 
 def kvec(n: int, L: float) -> np.ndarray:
@@ -380,6 +387,32 @@ def get_flux(rho_L, vx_L, vy_L, vz_L, rho_R, vx_R, vy_R, vz_R, cs):
     return flux_Mass, flux_Momx, flux_Momy, flux_Momz
 
 
+def get_flux_hll(rho_L, vx_L, vy_L, vz_L, rho_R, vx_R, vy_R, vz_R, cs):
+    """HLL flux of the isothermal system (not in the reference, whose flux is
+    get_flux above; the benchmark description asks for HLL).
+
+    Harten, Lax and van Leer (SIAM Rev. 25, 1983) with Davis's signal-speed
+    estimates S_L = min(u_L, u_R) - cs, S_R = max(u_L, u_R) + cs:
+    F = F_L if S_L >= 0, F_R if S_R <= 0, else
+    (S_R F_L - S_L F_R + S_L S_R (U_R - U_L)) / (S_R - S_L),
+    with U = (rho, rho u, rho v, rho w) and
+    F(U) = (rho u, rho u^2 + rho cs^2, rho u v, rho u w); u is face-normal.
+    """
+    c2 = cs * cs
+    mL, mR = rho_L * vx_L, rho_R * vx_R
+    FL = (mL, mL * vx_L + rho_L * c2, mL * vy_L, mL * vz_L)
+    FR = (mR, mR * vx_R + rho_R * c2, mR * vy_R, mR * vz_R)
+    UL = (rho_L, mL, rho_L * vy_L, rho_L * vz_L)
+    UR = (rho_R, mR, rho_R * vy_R, rho_R * vz_R)
+    SL = np.minimum(vx_L, vx_R) - cs
+    SR = np.maximum(vx_L, vx_R) + cs
+    out = []
+    for fl, fr, ul, ur in zip(FL, FR, UL, UR):
+        mid = (SR * fl - SL * fr + SL * SR * (ur - ul)) / (SR - SL)
+        out.append(np.where(SL >= 0, fl, np.where(SR <= 0, fr, mid)))
+    return tuple(out)
+
+
 def slope_limiter(f, dx, f_dx, f_dy, f_dz):
     """Ratio-form minmod of quantum-fluid.py:352-407 (eps = 1e-12)."""
     eps = 1.0e-12
@@ -396,8 +429,10 @@ def slope_limiter(f, dx, f_dx, f_dy, f_dz):
     return tuple(out)
 
 
-def solve_hydro(rho, vx, vy, vz, dt, dx, vol, cs):
-    """MUSCL-Hancock isothermal update, quantum-fluid.py:281-349."""
+def solve_hydro(rho, vx, vy, vz, dt, dx, vol, cs, flux="rusanov"):
+    """MUSCL-Hancock isothermal update, quantum-fluid.py:281-349; flux "hll"
+    replaces the face flux by get_flux_hll (same rotated arguments)."""
+    get_flux_ = {"rusanov": get_flux, "hll": get_flux_hll}[flux]
     rho_dx, rho_dy, rho_dz = get_gradient(rho, dx)
     vx_dx, vx_dy, vx_dz = get_gradient(vx, dx)
     vy_dx, vy_dy, vy_dz = get_gradient(vy, dx)
@@ -426,13 +461,13 @@ def solve_hydro(rho, vx, vy, vz, dt, dx, vol, cs):
     vz_XL, vz_XR, vz_YL, vz_YR, vz_ZL, vz_ZR = extrap_to_face(
         vz_prime, vz_dx, vz_dy, vz_dz, dx)
 
-    fMX, fPxX, fPyX, fPzX = get_flux(rho_XL, vx_XL, vy_XL, vz_XL,
-                                     rho_XR, vx_XR, vy_XR, vz_XR, cs)
+    fMX, fPxX, fPyX, fPzX = get_flux_(rho_XL, vx_XL, vy_XL, vz_XL,
+                                      rho_XR, vx_XR, vy_XR, vz_XR, cs)
     # quantum-fluid.py:331-336: rotated argument order for Y and Z
-    fMY, fPyY, fPzY, fPxY = get_flux(rho_YL, vy_YL, vz_YL, vx_YL,
-                                     rho_YR, vy_YR, vz_YR, vx_YR, cs)
-    fMZ, fPzZ, fPxZ, fPyZ = get_flux(rho_ZL, vz_ZL, vx_ZL, vy_ZL,
-                                     rho_ZR, vz_ZR, vx_ZR, vy_ZR, cs)
+    fMY, fPyY, fPzY, fPxY = get_flux_(rho_YL, vy_YL, vz_YL, vx_YL,
+                                      rho_YR, vy_YR, vz_YR, vx_YR, cs)
+    fMZ, fPzZ, fPxZ, fPyZ = get_flux_(rho_ZL, vz_ZL, vx_ZL, vy_ZL,
+                                      rho_ZR, vz_ZR, vx_ZR, vy_ZR, cs)
 
     Mass, Momx, Momy, Momz = get_conserved(rho, vx, vy, vz, vol)
     Mass = apply_fluxes(Mass, fMX, fMY, fMZ, dx, dt)
@@ -440,6 +475,13 @@ def solve_hydro(rho, vx, vy, vz, dt, dx, vol, cs):
     Momy = apply_fluxes(Momy, fPyX, fPyY, fPyZ, dx, dt)
     Momz = apply_fluxes(Momz, fPzX, fPzY, fPzZ, dx, dt)
     return get_primitive(Mass, Momx, Momy, Momz, vol)
+
+
+def cfl_dt(vx, vy, vz, cs, dx, cfl, dt_max):
+    """Hydro CFL cap of quantum-fluid-accretion.py:696-701 for an isothermal
+    gas (c_eff = cs): min(dt_max, cfl dx / (max(cs + |v|) + 1e-30))."""
+    cmax = np.max(cs + np.sqrt(vx * vx + vy * vy + vz * vz))
+    return min(dt_max, cfl * dx / (cmax + 1e-30))
 
 
 def apply_grav_accel(rho, vx, vy, vz, dt, g: Grid, ph: Physics):
@@ -456,7 +498,7 @@ def compute_step_gas(psi, rho, vx, vy, vz, t, dt, g: Grid, ph: Physics):
     psi = kick(psi, V, ph.m_per_hbar * dt / 2.0)
     vx, vy, vz = apply_grav_accel(rho_tot, vx, vy, vz, dt / 2.0, g, ph)
     psi = drift(psi, dt, g, ph)
-    rho, vx, vy, vz = solve_hydro(rho, vx, vy, vz, dt, dx, vol, ph.cs)
+    rho, vx, vy, vz = solve_hydro(rho, vx, vy, vz, dt, dx, vol, ph.cs, ph.flux)
     rho_tot = np.abs(psi) ** 2 + rho
     V = get_potential(rho_tot, g, ph)
     psi = kick(psi, V, ph.m_per_hbar * dt / 2.0)

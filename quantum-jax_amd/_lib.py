@@ -16,6 +16,8 @@ LIB_PATH = os.path.join(HERE, "libqjamd.so")
 QJ_C64 = 0
 QJ_C128 = 1
 DIAG_AMAX, DIAG_DTS, DIAG_MASS, DIAG_NAN, DIAG_COUNT = 0, 1, 2, 3, 8
+FLUX_RUSANOV, FLUX_HLL = 0, 1
+FLUXES = {"rusanov": FLUX_RUSANOV, "hll": FLUX_HLL}
 OPT_FUSE_X_SOURCE = 0
 OPT_BINNED_DEPOSIT = 1
 ENS_EAGER, ENS_BATCHED, ENS_GRAPH, ENS_GRAPH_FORK = 0, 1, 2, 3
@@ -42,7 +44,7 @@ class QjParams(C.Structure):
     """Mirror of ``qj_params``."""
     _fields_ = [
         ("dt", f64), ("m_per_hbar", f64), ("four_pi_G", f64), ("rho_bar", f64),
-        ("b_coeff", f64), ("cs", f64), ("dc_mode", i32),
+        ("b_coeff", f64), ("cs", f64), ("dc_mode", i32), ("flux", i32),
     ]
 
 
@@ -72,7 +74,9 @@ SIGNATURES = {
     "qj_interp": (i32, [vp, vp, i64, vp, vp, vp]),
     "qj_get_acceleration": (i32, [vp, vp, i64, vp, f64, f64, i32, vp, vp, vp, vp, vp]),
     "qj_apply_grav_accel": (i32, [vp, vp, vp, vp, vp, f64, f64, f64, i32, vp, vp, vp, vp]),
-    "qj_solve_hydro": (i32, [vp, vp, vp, vp, vp, f64, f64, vp, vp, vp, vp, vp]),
+    "qj_solve_hydro": (i32, [vp, vp, vp, vp, vp, f64, f64, i32, vp, vp, vp, vp, vp]),
+    "qj_gas_cfl_dt": (i32, [vp, vp, vp, f64, f64, vp, vp]),
+    "qj_kdk_steps_cfl": (i32, [vp, vp, vp, i32, f64, f64, dp, vp, vp, vp]),
     "qj_kdk_steps": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), i32, vp, vp]),
     "qj_kdk_finalize": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), vp]),
     "qj_kdk_steps_sched": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), i32, dp, dp, vp, vp]),

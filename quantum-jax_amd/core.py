@@ -25,6 +25,8 @@ class Physics:
     rho_bar     quantum-stars.py:83         b_coeff    self-interaction, :117-125
     cs          quantum-fluid.py:71         dc_mode    0: reference k=0 rule
                                                         1: zero-mean potential
+    flux        gas face flux: "rusanov" (quantum-fluid.py:238-264, the
+                reference's) or "hll" (HLL with Davis signal speeds)
     """
     m_per_hbar: float = 1.0
     four_pi_G: float = 1.0
@@ -32,6 +34,13 @@ class Physics:
     b_coeff: float = 0.0
     cs: float = 0.1
     dc_mode: int = 0
+    flux: str = "rusanov"
+
+    @property
+    def flux_kind(self) -> int:
+        if self.flux not in _lib.FLUXES:
+            raise ValueError(f"flux must be one of {sorted(_lib.FLUXES)}")
+        return _lib.FLUXES[self.flux]
 
 
 def _is_pow2(n: int) -> bool:

@@ -4,6 +4,7 @@
 #include <string.h>
 
 #include <string>
+#include <cmath>
 #include <vector>
 
 #include "../../include/qj_api.h"
@@ -506,19 +507,21 @@ int qj_apply_grav_accel(qj_plan* p, const void* rho, void* vx, void* vy, void* v
 }
 
 int qj_solve_hydro(qj_plan* p, const void* rho, const void* vx, const void* vy, const void* vz,
-                   double dt, double cs, void* rho_o, void* vx_o, void* vy_o, void* vz_o,
-                   void* stream) {
+                   double dt, double cs, int flux, void* rho_o, void* vx_o, void* vy_o,
+                   void* vz_o, void* stream) {
   if (int rc = check_plan(p)) return rc;
   if (!rho || !vx || !vy || !vz || !rho_o || !vx_o || !vy_o || !vz_o)
     return fail(QJ_EARG, "qj_solve_hydro: null buffer");
+  if (flux != QJ_FLUX_RUSANOV && flux != QJ_FLUX_HLL)
+    return fail(QJ_EARG, "qj_solve_hydro: flux must be QJ_FLUX_RUSANOV or QJ_FLUX_HLL");
   if (p->d.nx != p->d.ny || p->d.nx != p->d.nz || p->d.Lx != p->d.Ly || p->d.Lx != p->d.Lz)
     return fail(QJ_EUNSUPPORTED, "qj_solve_hydro: the gas update needs a cubic grid and box");
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = QJ_PRECISION(
       p,
-      launch_hydro<float>(p->d, rho, vx, vy, vz, nullptr, 0.0, dt, cs, rho_o,
+      launch_hydro<float>(p->d, rho, vx, vy, vz, nullptr, 0.0, dt, cs, flux, rho_o,
                           vx_o, vy_o, vz_o, s),
-      launch_hydro<double>(p->d, rho, vx, vy, vz, nullptr, 0.0, dt, cs, rho_o,
+      launch_hydro<double>(p->d, rho, vx, vy, vz, nullptr, 0.0, dt, cs, flux, rho_o,
                            vx_o, vy_o, vz_o, s));
   return e == hipSuccess ? QJ_OK : hipfail(e, "qj_solve_hydro");
 }
@@ -623,11 +626,13 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
                "star step");
     }
     // gas: owed + first half kick of the velocity, then the MUSCL update
+    // over this step's drift time
     if (gas) {
+      const double dd = drift_dt ? drift_dt[step] : dt;
       QJ_STAGE(kStageHydro,
                launch_hydro<T>(p->d, live_gas(0), live_gas(1), live_gas(2), live_gas(3), A4, kick_t,
-                               dt, pr->cs, spare_gas(0), spare_gas(1), spare_gas(2), spare_gas(3),
-                               s),
+                               dd, pr->cs, pr->flux, spare_gas(0), spare_gas(1), spare_gas(2),
+                               spare_gas(3), s),
                "hydro");
       st->gas_swapped ^= 1;
     }
@@ -803,6 +808,8 @@ int qj_kdk_steps(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void
                  void* stream) {
   if (int rc = check_plan(p)) return rc;
   if (!st || !pr || !work || nsteps < 0) return fail(QJ_EARG, "qj_kdk_steps: bad arguments");
+  if (pr->flux != QJ_FLUX_RUSANOV && pr->flux != QJ_FLUX_HLL)
+    return fail(QJ_EARG, "qj_kdk_steps: params.flux must be QJ_FLUX_RUSANOV or QJ_FLUX_HLL");
   if (!st->psi || !st->grid || !st->diag) return fail(QJ_EARG, "qj_kdk_steps: null state buffer");
   if (st->np > 0 && (!st->pos || !st->vel || !st->rho_s))
     return fail(QJ_EARG, "qj_kdk_steps: stars need pos, vel and rho_s");
@@ -829,13 +836,55 @@ int qj_kdk_steps_sched(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps
                        const double* kick_dt, const double* drift_dt, void* work, void* stream) {
   if (nsteps > 0 && (!kick_dt || !drift_dt))
     return fail(QJ_EARG, "qj_kdk_steps_sched: null schedule");
-  if (st && (st->np > 0 || st->gas[0]))
-    return fail(QJ_EUNSUPPORTED, "qj_kdk_steps_sched: psi-only states (no stars or gas)");
+  if (st && st->np > 0)
+    return fail(QJ_EUNSUPPORTED, "qj_kdk_steps_sched: psi and gas states (no stars)");
   if (int rc = qj_kdk_steps(p, st, pr, 0, work, stream)) return rc;  // argument checks
   hipStream_t s = (hipStream_t)stream;
   return p->dtype == QJ_C64
              ? kdk_steps_t<float>(p, st, pr, nsteps, work, s, kick_dt, drift_dt)
              : kdk_steps_t<double>(p, st, pr, nsteps, work, s, kick_dt, drift_dt);
+}
+
+int qj_gas_cfl_dt(qj_plan* p, const qj_state* st, const qj_params* pr, double cfl, double dt_max,
+                  double* out, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!st || !pr || !out || !(cfl > 0.0) || !(dt_max > 0.0))
+    return fail(QJ_EARG, "qj_gas_cfl_dt: bad arguments");
+  if (!st->gas[0] || !st->gas_tmp[0]) return fail(QJ_EARG, "qj_gas_cfl_dt: the state has no gas");
+  const int k0 = st->gas_swapped;
+  void* const* g = k0 ? st->gas_tmp : st->gas;
+  // the owed half kick of the last step is part of the reference's state
+  const double h = (st->grid_valid && st->pending != 0.0) ? st->pending : 0.0;
+  const void* A4 = h != 0.0 ? (p->dtype == QJ_C64 ? plane<float>(st->grid, p, 1)
+                                                  : plane<double>(st->grid, p, 1))
+                            : nullptr;
+  const double dx = p->d.Lx / p->d.nx;
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = QJ_PRECISION(
+      p, launch_gas_cfl<float>(p->ncell(), g[1], g[2], g[3], A4, h, pr->cs, dx, cfl, dt_max, out, s),
+      launch_gas_cfl<double>(p->ncell(), g[1], g[2], g[3], A4, h, pr->cs, dx, cfl, dt_max, out, s));
+  return e == hipSuccess ? QJ_OK : hipfail(e, "qj_gas_cfl_dt");
+}
+
+int qj_kdk_steps_cfl(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, double cfl,
+                     double dt_max, double* dts, double* cfl_dev, void* work, void* stream) {
+  if (!cfl_dev || nsteps < 0) return fail(QJ_EARG, "qj_kdk_steps_cfl: bad arguments");
+  if (int rc = qj_kdk_steps_sched(p, st, pr, 0, nullptr, nullptr, work, stream)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  for (int k = 0; k < nsteps; ++k) {
+    if (int rc = qj_gas_cfl_dt(p, st, pr, cfl, dt_max, cfl_dev, stream)) return rc;
+    double dt = 0.0;
+    hipError_t e = hipMemcpyAsync(&dt, cfl_dev + 1, sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hipfail(e, "qj_kdk_steps_cfl: dt read-back");
+    if (!(dt > 0.0) || !std::isfinite(dt))
+      return fail(QJ_EARG, "qj_kdk_steps_cfl: non-finite CFL time step (non-finite gas velocity?)");
+    const int rc = p->dtype == QJ_C64 ? kdk_steps_t<float>(p, st, pr, 1, work, s, &dt, &dt)
+                                      : kdk_steps_t<double>(p, st, pr, 1, work, s, &dt, &dt);
+    if (rc) return rc;
+    if (dts) dts[k] = dt;
+  }
+  return QJ_OK;
 }
 
 int qj_bh_accretion(qj_plan* p, void* psi, const double* pos, const double* vel, double* m_bh,

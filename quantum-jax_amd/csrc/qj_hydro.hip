@@ -4,8 +4,9 @@
 // Per cell it evaluates, in the reference's operation order: centred
 // gradients (:200-205), the ratio-form minmod limiter (:352-407), the
 // primitive half-step predictor (:295-311), face extrapolation (:208-221),
-// the Rusanov flux of the averaged state (:238-264) on the six faces and
-// the conservative update (:224-235, :338-347).
+// the Rusanov flux of the averaged state (:238-264) or, by option, the HLL
+// flux on the six faces and the conservative update (:224-235, :338-347).
+#include "../../include/qj_api.h"
 #include "qj_common.h"
 #include "qj_kernels.h"
 
@@ -62,6 +63,40 @@ __device__ __forceinline__ void flux(T rL, T uL, T vL, T wL, T rR, T uR, T vR, T
   F[3] -= C * (T)0.5 * (rR * wR - rL * wL);
 }
 
+// HLL flux (Harten, Lax and van Leer 1983) of the isothermal system with
+// Davis's signal speeds S_L = min(uL, uR) - cs, S_R = max(uL, uR) + cs;
+// U = (rho, rho u, rho v, rho w), F(U) = (rho u, rho u^2 + rho cs^2, rho u v,
+// rho u w).  Not in the reference (its flux is the Rusanov form above).
+template <class T>
+__device__ __forceinline__ void flux_hll(T rL, T uL, T vL, T wL, T rR, T uR, T vR, T wR, T cs,
+                                         T F[4]) {
+  const T c2 = cs * cs;
+  const T mL = rL * uL, mR = rR * uR;
+  const T FL[4] = {mL, mL * uL + rL * c2, mL * vL, mL * wL};
+  const T FR[4] = {mR, mR * uR + rR * c2, mR * vR, mR * wR};
+  const T SL = fmin(uL, uR) - cs, SR = fmax(uL, uR) + cs;
+  if (SL >= (T)0) {
+  #pragma unroll
+    for (int k = 0; k < 4; ++k) F[k] = FL[k];
+  } else if (SR <= (T)0) {
+  #pragma unroll
+    for (int k = 0; k < 4; ++k) F[k] = FR[k];
+  } else {
+    const T UL[4] = {rL, mL, rL * vL, rL * wL};
+    const T UR[4] = {rR, mR, rR * vR, rR * wR};
+    const T inv = rcp(SR - SL), SLR = SL * SR;
+  #pragma unroll
+    for (int k = 0; k < 4; ++k) F[k] = (SR * FL[k] - SL * FR[k] + SLR * (UR[k] - UL[k])) * inv;
+  }
+}
+
+template <class T, int FLUX>
+__device__ __forceinline__ void face_flux(T rL, T uL, T vL, T wL, T rR, T uR, T vR, T wR, T cs,
+                                          T F[4]) {
+  if constexpr (FLUX == QJ_FLUX_HLL) flux_hll<T>(rL, uL, vL, wL, rR, uR, vR, wR, cs, F);
+  else flux<T>(rL, uL, vL, wL, rR, uR, vR, wR, cs, F);
+}
+
 // ---------------------------------------------------------------------------
 // Streaming version: a workgroup owns a TY x TZ tile of (y, z) columns and
 // marches along x over XC planes.  Raw planes (rho, v, with the owed gravity
@@ -97,7 +132,7 @@ struct XState {
   T sx[4];
 };
 
-template <class T>
+template <class T, int FLUX>
 struct k_hydro_stream {
   static constexpr int kThreads = 384;
   static __device__ __forceinline__ void run(HGeom<T> g, int TY, int TZ, int XC,
@@ -266,13 +301,13 @@ struct k_hydro_stream {
           fl[k] = xc0.p[k] + (T)0.5 * xc0.sx[k] * dx;
           fr[k] = xp.p[k] - (T)0.5 * xp.sx[k] * dx;
         }
-        flux<T>(fl[0], fl[1], fl[2], fl[3], fr[0], fr[1], fr[2], fr[3], g.cs, Fp);
+        face_flux<T, FLUX>(fl[0], fl[1], fl[2], fl[3], fr[0], fr[1], fr[2], fr[3], g.cs, Fp);
   #pragma unroll
         for (int k = 0; k < 4; ++k) {
           fl[k] = xm.p[k] + (T)0.5 * xm.sx[k] * dx;
           fr[k] = xc0.p[k] - (T)0.5 * xc0.sx[k] * dx;
         }
-        flux<T>(fl[0], fl[1], fl[2], fl[3], fr[0], fr[1], fr[2], fr[3], g.cs, Fm);
+        face_flux<T, FLUX>(fl[0], fl[1], fl[2], fl[3], fr[0], fr[1], fr[2], fr[3], g.cs, Fm);
   #pragma unroll
         for (int k = 0; k < 4; ++k) {
           U[k] += -fac * Fp[k];
@@ -295,10 +330,10 @@ struct k_hydro_stream {
             }
             T f[4];
             if (a == 1) {
-              flux<T>(fl[0], fl[2], fl[3], fl[1], fr[0], fr[2], fr[3], fr[1], g.cs, f);
+              face_flux<T, FLUX>(fl[0], fl[2], fl[3], fl[1], fr[0], fr[2], fr[3], fr[1], g.cs, f);
               F[side][0] = f[0]; F[side][2] = f[1]; F[side][3] = f[2]; F[side][1] = f[3];
             } else {
-              flux<T>(fl[0], fl[3], fl[1], fl[2], fr[0], fr[3], fr[1], fr[2], g.cs, f);
+              face_flux<T, FLUX>(fl[0], fl[3], fl[1], fl[2], fr[0], fr[3], fr[1], fr[2], g.cs, f);
               F[side][0] = f[0]; F[side][3] = f[1]; F[side][1] = f[2]; F[side][2] = f[3];
             }
           }
@@ -336,7 +371,73 @@ struct k_gas_kick {
   }
 };
 
+// Largest signal speed max(cs + |v|) over the cells (quantum-fluid-accretion.py:697-699,
+// isothermal: c_eff = cs), with v the velocity after the owed kick
+// v + h a (h = 0: the stored velocity).  NaN-propagating wave maxima, one
+// atomic per wave into out[0] (cleared by k_cfl_clear).
+template <class T>
+struct k_gas_signal {
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(long long n, const T* __restrict__ vx,
+                                             const T* __restrict__ vy, const T* __restrict__ vz,
+                                             const T* __restrict__ A4, T h, double cs,
+                                             double* __restrict__ out) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    double m = 0.0;
+    for (; i < n; i += stride) {
+      T u = vx[i], v = vy[i], w = vz[i];
+      if (A4) {
+        u += A4[4 * i] * h;
+        v += A4[4 * i + 1] * h;
+        w += A4[4 * i + 2] * h;
+      }
+      const double c = cs + sqrt((double)u * u + (double)v * v + (double)w * w);
+      m = (c > m || c != c) ? c : m;
+    }
+  #pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double other = __shfl_xor(m, o, 64);
+      m = (other > m || other != other) ? other : m;
+    }
+    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&out[0], m);
+  }
+};
+
+struct k_cfl_clear {
+  static constexpr int kThreads = 64;
+  static __device__ __forceinline__ void run(double* out) {
+    if (threadIdx.x == 0) out[0] = 0.0;
+  }
+};
+
+// dt = min(dt_max, cfl dx / (Cmax + 1e-30)) (quantum-fluid-accretion.py:700-701);
+// a NaN signal speed gives a NaN dt.
+struct k_cfl_finish {
+  static constexpr int kThreads = 64;
+  static __device__ __forceinline__ void run(double* out, double cfl, double dx, double dt_max) {
+    if (threadIdx.x == 0) {
+      const double c = out[0];
+      const double dth = cfl * dx / (c + 1e-30);
+      out[1] = (c != c) ? c : (dth < dt_max ? dth : dt_max);
+    }
+  }
+};
+
 }  // namespace
+
+template <class T>
+hipError_t launch_gas_cfl(long long n, const void* vx, const void* vy, const void* vz,
+                          const void* A4, double h, double cs, double dx, double cfl,
+                          double dt_max, double* out, hipStream_t s) {
+  long long blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  launch<k_cfl_clear>(1, 64, s, out);
+  launch<k_gas_signal<T>>((int)blocks, 256, s, n, (const T*)vx, (const T*)vy, (const T*)vz,
+                          (const T*)(h != 0.0 ? A4 : nullptr), (T)h, cs, out);
+  launch<k_cfl_finish>(1, 64, s, out, cfl, dx, dt_max);
+  return hipGetLastError();
+}
 
 template <class T>
 hipError_t launch_gas_kick(long long n, void* vx, void* vy, void* vz, const void* A4, double h,
@@ -350,8 +451,10 @@ hipError_t launch_gas_kick(long long n, void* vx, void* vy, void* vz, const void
 template <class T>
 hipError_t launch_hydro(const Dims& d, const void* rho, const void* vx, const void* vy,
                         const void* vz, const void* A4, double kick, double dt, double cs,
-                        void* rho_o, void* vx_o, void* vy_o, void* vz_o, hipStream_t s) {
+                        int flux_kind, void* rho_o, void* vx_o, void* vy_o, void* vz_o,
+                        hipStream_t s) {
   if (kick != 0.0 && A4 == nullptr) return hipErrorInvalidValue;
+  if (flux_kind != QJ_FLUX_RUSANOV && flux_kind != QJ_FLUX_HLL) return hipErrorInvalidValue;
   HGeom<T> g;
   g.nx = d.nx;
   g.ny = d.ny;
@@ -366,20 +469,33 @@ hipError_t launch_hydro(const Dims& d, const void* rho, const void* vx, const vo
   const int nblk = (d.ny / TY) * (d.nz / TZ) * (d.nx / XC);
   const int nthr = ((TY + 2) * (TZ + 2) + 63) / 64 * 64;  // tile cells + ring, whole waves
   if ((TY + 4) * (TZ + 4) > 2 * nthr) return hipErrorInvalidConfiguration;  // 2 cells per thread
-  launch<k_hydro_stream<T>>(nblk, nthr, s, g, TY, TZ, XC, (const T*)rho, (const T*)vx,
-                                              (const T*)vy, (const T*)vz, (const T*)A4, (T)kick,
-                                              (T*)rho_o, (T*)vx_o, (T*)vy_o, (T*)vz_o);
+  if (flux_kind == QJ_FLUX_HLL)
+    launch<k_hydro_stream<T, QJ_FLUX_HLL>>(nblk, nthr, s, g, TY, TZ, XC, (const T*)rho,
+                                           (const T*)vx, (const T*)vy, (const T*)vz,
+                                           (const T*)A4, (T)kick, (T*)rho_o, (T*)vx_o,
+                                           (T*)vy_o, (T*)vz_o);
+  else
+    launch<k_hydro_stream<T, QJ_FLUX_RUSANOV>>(nblk, nthr, s, g, TY, TZ, XC, (const T*)rho,
+                                               (const T*)vx, (const T*)vy, (const T*)vz,
+                                               (const T*)A4, (T)kick, (T*)rho_o, (T*)vx_o,
+                                               (T*)vy_o, (T*)vz_o);
   return hipGetLastError();
 }
 
 template hipError_t launch_hydro<float>(const Dims&, const void*, const void*, const void*,
-                                        const void*, const void*, double, double, double, void*,
-                                        void*, void*, void*, hipStream_t);
+                                        const void*, const void*, double, double, double, int,
+                                        void*, void*, void*, void*, hipStream_t);
 template hipError_t launch_hydro<double>(const Dims&, const void*, const void*, const void*,
-                                         const void*, const void*, double, double, double, void*,
-                                         void*, void*, void*, hipStream_t);
+                                         const void*, const void*, double, double, double, int,
+                                         void*, void*, void*, void*, hipStream_t);
 template hipError_t launch_gas_kick<float>(long long, void*, void*, void*, const void*, double,
                                            hipStream_t);
+template hipError_t launch_gas_cfl<float>(long long, const void*, const void*, const void*,
+                                          const void*, double, double, double, double, double,
+                                          double*, hipStream_t);
+template hipError_t launch_gas_cfl<double>(long long, const void*, const void*, const void*,
+                                           const void*, double, double, double, double, double,
+                                           double*, hipStream_t);
 template hipError_t launch_gas_kick<double>(long long, void*, void*, void*, const void*, double,
                                             hipStream_t);
 

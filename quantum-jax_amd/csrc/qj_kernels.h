@@ -123,13 +123,21 @@ hipError_t launch_adj_poisson(const Dims& d, const SpecTables<T>& tb, const void
 
 // ---- gas ----------------------------------------------------------------
 // Applies v += a*kick first when kick != 0 (A4 may be null when kick == 0).
+// flux_kind: QJ_FLUX_RUSANOV (the reference's) or QJ_FLUX_HLL.
 template <class T>
 hipError_t launch_hydro(const Dims& d, const void* rho, const void* vx, const void* vy,
                         const void* vz, const void* A4, double kick, double dt, double cs,
-                        void* rho_o, void* vx_o, void* vy_o, void* vz_o, hipStream_t s);
+                        int flux_kind, void* rho_o, void* vx_o, void* vy_o, void* vz_o,
+                        hipStream_t s);
 template <class T>
 hipError_t launch_gas_kick(long long n, void* vx, void* vy, void* vz, const void* A4, double h,
                            hipStream_t s);
+// out[0] = max over cells of cs + |v + h a| (a from A4, skipped when h == 0),
+// out[1] = min(dt_max, cfl dx / (out[0] + 1e-30)).
+template <class T>
+hipError_t launch_gas_cfl(long long n, const void* vx, const void* vy, const void* vz,
+                          const void* A4, double h, double cs, double dx, double cfl,
+                          double dt_max, double* out, hipStream_t s);
 
 // Tile-binned star deposit (owner computes; no global float atomics).
 constexpr int kTileX = 8, kTileY = 8, kTileZ = 64;

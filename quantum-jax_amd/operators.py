@@ -167,12 +167,14 @@ def apply_grav_accel(rho, vx, vy, vz, dt: float, plan: Plan, phys: Physics):
 
 
 def solve_hydro(rho, vx, vy, vz, dt: float, plan: Plan, phys: Physics):
-    """MUSCL-Hancock isothermal update; quantum-fluid.py:281-349."""
+    """MUSCL-Hancock isothermal update; quantum-fluid.py:281-349 (face flux
+    phys.flux: the reference's Rusanov form, or HLL)."""
     ins = [plan.field(v, n) for v, n in ((rho, "rho"), (vx, "vx"), (vy, "vy"), (vz, "vz"))]
     outs = [torch.empty_like(v) for v in ins]
     with torch.cuda.device(plan.device):
         _lib.check(_L().qj_solve_hydro(plan.handle, *[ptr(v) for v in ins], float(dt),
-                                       phys.cs, *[ptr(v) for v in outs], plan.stream()),
+                                       phys.cs, phys.flux_kind, *[ptr(v) for v in outs],
+                                       plan.stream()),
                    "solve_hydro")
     return tuple(outs)
 

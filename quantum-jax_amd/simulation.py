@@ -10,6 +10,7 @@ reference's end-of-step state.
 from __future__ import annotations
 
 import ctypes as C
+import math
 
 import numpy as np
 import torch
@@ -89,8 +90,10 @@ class KDKSimulation:
         self.v_ext = plan.field(v_ext, "v_ext").clone() if v_ext is not None else None
         st.v_ext = ptr(self.v_ext).value
         self._st = st
+        self._cfl_out = None
+        self.dts = []
         self._params = _lib.QjParams(self.dt, phys.m_per_hbar, phys.four_pi_G, phys.rho_bar,
-                                     phys.b_coeff, phys.cs, phys.dc_mode)
+                                     phys.b_coeff, phys.cs, phys.dc_mode, phys.flux_kind)
 
     # -- stepping ------------------------------------------------------------
     def set_dt(self, dt: float):
@@ -111,8 +114,8 @@ class KDKSimulation:
 
     def step_schedule(self, kick_dts, drift_dts, elapsed: float | None = None):
         """Steps with per-step kick and drift times (e.g. the comoving
-        schedule of cosmo.comoving_schedule); psi-only states.  elapsed, when
-        given, is added to t."""
+        schedule of cosmo.comoving_schedule); psi and gas states.  elapsed,
+        when given, is added to t."""
         kick = np.ascontiguousarray(kick_dts, dtype=np.float64)
         drift = np.ascontiguousarray(drift_dts, dtype=np.float64)
         if kick.shape != drift.shape or kick.ndim != 1:
@@ -125,6 +128,49 @@ class KDKSimulation:
                 self.plan.stream()), "qj_kdk_steps_sched")
         if elapsed is not None:
             self.t += float(elapsed)
+        return self
+
+    def cfl_dt(self, cfl: float = 0.4, dt_max: float | None = None) -> float:
+        """CFL time step of the gas state, min(dt_max, cfl dx / max(cs + |v|))
+        (quantum-fluid-accretion.py:696-701; isothermal, so c_eff = cs).  The
+        maximum is taken on the device over the state the next step starts
+        from (owed half kick included); reading it synchronises the host.
+        dt_max defaults to self.dt."""
+        if self.gas is None:
+            raise ValueError("cfl_dt needs a gas state")
+        dt_max = self.dt if dt_max is None else float(dt_max)
+        if self._cfl_out is None:
+            self._cfl_out = torch.empty(2, dtype=torch.float64, device=self.plan.device)
+        with torch.cuda.device(self.plan.device):
+            _lib.check(_lib.lib().qj_gas_cfl_dt(self.plan.handle, C.byref(self._st),
+                                                C.byref(self._params), float(cfl), dt_max,
+                                                ptr(self._cfl_out), self.plan.stream()),
+                       "qj_gas_cfl_dt")
+        return float(self._cfl_out[1].item())
+
+    def step_cfl(self, n: int = 1, cfl: float = 0.4, dt_max: float | None = None):
+        """n gas steps, each with the CFL time step of the state it starts
+        from (cfl_dt; quantum-fluid-accretion.py:696-701).  The steps taken
+        are appended to self.dts."""
+        n = max(0, int(n))
+        if self.gas is None:
+            raise ValueError("step_cfl needs a gas state")
+        if n == 0:
+            return self
+        dt_max = self.dt if dt_max is None else float(dt_max)
+        if self._cfl_out is None:
+            self._cfl_out = torch.empty(2, dtype=torch.float64, device=self.plan.device)
+        dts = np.zeros(n, dtype=np.float64)
+        with torch.cuda.device(self.plan.device):
+            rc = _lib.lib().qj_kdk_steps_cfl(self.plan.handle, C.byref(self._st),
+                                             C.byref(self._params), n, float(cfl), dt_max,
+                                             dts.ctypes.data_as(C.POINTER(C.c_double)),
+                                             ptr(self._cfl_out), ptr(self.plan.workspace()),
+                                             self.plan.stream())
+        taken = [float(d) for d in dts if d > 0.0]
+        self.dts.extend(taken)
+        self.t += math.fsum(taken)
+        _lib.check(rc, "qj_kdk_steps_cfl")
         return self
 
     def finalize(self):

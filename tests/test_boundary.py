@@ -53,6 +53,18 @@ def test_state_struct_layout_matches_header():
     assert ctypes.sizeof(_lib.QjState) == 184
 
 
+def test_params_struct_layout_matches_header():
+    import importlib
+    _lib = importlib.import_module("quantum-jax_amd._lib")
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    body = text[text.index("typedef struct {\n  double dt;"):text.index("} qj_params;")]
+    fields = re.findall(r"\b([A-Za-z_]+);", body)
+    assert fields == [f[0] for f in _lib.QjParams._fields_]
+    assert ctypes.sizeof(_lib.QjParams) == 56
+    consts = dict(re.findall(r"#define (QJ_FLUX_[A-Z]+) (\d+)", text))
+    assert {k[8:].lower(): int(v) for k, v in consts.items()} == _lib.FLUXES
+
+
 def test_no_cpu_fallback_when_library_missing(monkeypatch):
     import importlib
     _lib = importlib.import_module("quantum-jax_amd._lib")

@@ -186,7 +186,9 @@ def test_interp_and_get_acceleration_match_oracle(qj, cuda, dtype):
 
 @pytest.mark.parametrize("dtype", [C64, C128])
 @pytest.mark.parametrize("n", [8, 32, 64])
-def test_hydro_matches_oracle(qj, cuda, dtype, n):
+# cs = 0.05 makes many faces supersonic (both upwind branches of HLL)
+@pytest.mark.parametrize("flux,cs", [("rusanov", 0.3), ("hll", 0.3), ("hll", 0.05)])
+def test_hydro_matches_oracle(qj, cuda, dtype, n, flux, cs):
     # 8: one tile, one x chunk; 64: 4x4 tiles of 16x16 columns, two x chunks
     g = O.Grid(n, n, n)
     X, Y, Z = O.cell_centres(g)
@@ -200,8 +202,8 @@ def test_hydro_matches_oracle(qj, cuda, dtype, n):
     r = rdt(dtype)
     ins = [dev(a, r) for a in (rho, vx, vy, vz)]
     insr = [a.double().cpu().numpy() for a in ins]
-    out = qj.solve_hydro(*ins, 1e-3, plan, qj.Physics(cs=0.3))
-    ref = O.solve_hydro(*insr, 1e-3, g.dx, g.vol, 0.3)
+    out = qj.solve_hydro(*ins, 1e-3, plan, qj.Physics(cs=cs, flux=flux))
+    ref = O.solve_hydro(*insr, 1e-3, g.dx, g.vol, cs, flux)
     for o, rf in zip(out, ref):
         assert rel(o, rf) < (1e-6 if dtype == C64 else 1e-13)
 
@@ -275,19 +277,21 @@ def test_plain_steps_match_oracle(qj, cuda, dtype):
 
 
 @pytest.mark.parametrize("dtype", [C64, C128])
-def test_gas_steps_match_oracle(qj, cuda, dtype):
+@pytest.mark.parametrize("flux", ["rusanov", "hll"])
+def test_gas_steps_match_oracle(qj, cuda, dtype, flux):
     g = O.Grid(32, 32, 32)
     psi = _psi(g, 14)
     rho, vx, vy, vz = O.gas_ic(g, 14, sigma_k=8 * np.pi)
     rng = np.random.default_rng(2)
     vx = 0.05 * rng.standard_normal(g.shape)
     rb = float(np.mean(np.abs(psi) ** 2 + rho))
-    ph = O.Physics(rho_bar=rb, cs=0.1)
+    ph = O.Physics(rho_bar=rb, cs=0.1, flux=flux)
     plan = plan_for(qj, g, dtype)
     r = rdt(dtype)
     gas = [dev(a, r) for a in (rho, vx, vy, vz)]
     gasr = [a.double().cpu().numpy() for a in gas]
-    sim = qj.KDKSimulation(plan, qj.Physics(rho_bar=rb, cs=0.1), 1e-3, dev(psi, dtype), gas=gas)
+    sim = qj.KDKSimulation(plan, qj.Physics(rho_bar=rb, cs=0.1, flux=flux), 1e-3, dev(psi, dtype),
+                           gas=gas)
     sim.step(3)
     st = sim.state()
     s = (psi, *gasr, 0.0)

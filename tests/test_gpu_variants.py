@@ -3,7 +3,8 @@
 quantum-tidal.py: a static external potential (softened point mass) added
 to V in the psi kicks; quantum-cosmo.py: the comoving step, whose kicks use
 dt pot_fac and drift dt kin_fac with dt from the quadrature of 1/adot, run
-through the per-step schedule entry point (merged kicks).
+through the per-step schedule entry point (merged kicks);
+quantum-fluid-accretion.py: gas steps with the hydro CFL cap as dt.
 """
 
 import numpy as np
@@ -79,3 +80,48 @@ def test_variants_reject_stars(qj, cuda):
                             dev(vel, torch.float64), mass=m)
     with pytest.raises(qj.QjError):
         sim2.step_schedule([1e-3], [1e-3])
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+@pytest.mark.parametrize("flux", ["rusanov", "hll"])
+def test_cfl_limited_gas_steps_match_oracle(qj, cuda, dtype, flux):
+    """Gas steps whose dt is the CFL cap of the state they start from
+    (quantum-fluid-accretion.py:696-701), through the schedule entry point;
+    the cap binds (cs + |v| ~ 1.5, dx = 1/32) and changes from step to step."""
+    g = O.Grid(32, 32, 32)
+    psi = O.filtered_gaussian_psi(g, 27, sigma_k=8 * np.pi)
+    rho, _, _, _ = O.gas_ic(g, 27, sigma_k=8 * np.pi)
+    rng = np.random.default_rng(28)
+    vx, vy, vz = (0.3 * rng.standard_normal(g.shape) for _ in range(3))
+    cs, cfl, dt_max = 1.0, 0.4, 1e-2
+    rb = float(np.mean(np.abs(psi) ** 2 + rho))
+    r = torch.float32 if dtype == C64 else torch.float64
+    gas = [dev(a, r) for a in (rho, vx, vy, vz)]
+    sim = qj.KDKSimulation(qj.Plan(32, dtype=dtype), qj.Physics(rho_bar=rb, cs=cs, flux=flux),
+                           dt_max, dev(psi, dtype), gas=gas)
+    s = (psi, *[a.double().cpu().numpy() for a in gas], 0.0)
+    ph = O.Physics(rho_bar=rb, cs=cs, flux=flux)
+    dts = []
+    for _ in range(4):
+        dt = O.cfl_dt(s[2], s[3], s[4], cs, g.dx, cfl, dt_max)
+        dts.append(dt)
+        s = O.compute_step_gas(*s, dt, g, ph)
+    sim.step_cfl(4, cfl=cfl, dt_max=dt_max)
+    assert dts[0] < dt_max and len(set(dts)) == 4
+    assert np.allclose(sim.dts, dts, rtol=(1e-6 if dtype == C64 else 1e-13), atol=0)
+    assert abs(sim.t - sum(sim.dts)) < 1e-15
+    st = sim.state()
+    assert rel(st["psi"], s[0]) < TOL[dtype]
+    for k, name in enumerate(("rho", "vx", "vy", "vz")):
+        assert rel(st[name], s[1 + k]) < (5e-5 if dtype == C64 else 1e-11), name
+    # after finalize (no owed kick) the cap is that of the finalized state
+    want = O.cfl_dt(*[st[k].double().cpu().numpy() for k in ("vx", "vy", "vz")], cs, g.dx, cfl,
+                    dt_max)
+    assert abs(sim.cfl_dt(cfl, dt_max) / want - 1) < (1e-6 if dtype == C64 else 1e-13)
+
+
+def test_cfl_rejects_states_without_gas(qj, cuda):
+    sim = qj.KDKSimulation(qj.Plan(16), qj.Physics(), 1e-3,
+                           dev(O.filtered_gaussian_psi(O.Grid(16, 16, 16), 29), C64))
+    with pytest.raises(ValueError):
+        sim.cfl_dt()
