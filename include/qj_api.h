@@ -197,11 +197,14 @@ int qj_gas_cfl_dt(qj_plan* plan, const qj_state* state, const qj_params* prm, do
 
 /* nsteps gas steps, each with dt = the qj_gas_cfl_dt value of the state it
  * starts from (quantum-fluid-accretion.py:696-701: the CFL cap is evaluated
- * inside every step).  Synchronises the stream once per step to read dt;
- * dts (host, nsteps, may be NULL) receives the steps taken; cfl_dev is 2
- * doubles of device scratch.  Psi and gas states (no stars). */
+ * inside every step).  dt is computed and consumed on the device (the drift
+ * tables, the kicks and the gas update read it there); the host reads the
+ * steps taken back once per 64 steps, so the call synchronises the stream.
+ * dts (host, nsteps doubles, may be NULL) receives them.  A non-finite dt
+ * (a non-finite velocity) returns an error after its chunk.  Psi and gas
+ * states (no stars). */
 int qj_kdk_steps_cfl(qj_plan* plan, qj_state* state, const qj_params* prm, int nsteps, double cfl,
-                     double dt_max, double* dts, double* cfl_dev, void* work, void* stream);
+                     double dt_max, double* dts, void* work, void* stream);
 /* do_accretion(psi, pos, vel, m_bh) of quantum-bh.py:306-339, in place:
  * black hole 0 takes dM/dt dt from the field at its cell (the reference
  * lowers the amplitude there by sqrt(dM/dt dt / vol) at fixed phase) and

@@ -76,7 +76,7 @@ SIGNATURES = {
     "qj_apply_grav_accel": (i32, [vp, vp, vp, vp, vp, f64, f64, f64, i32, vp, vp, vp, vp]),
     "qj_solve_hydro": (i32, [vp, vp, vp, vp, vp, f64, f64, i32, vp, vp, vp, vp, vp]),
     "qj_gas_cfl_dt": (i32, [vp, vp, vp, f64, f64, vp, vp]),
-    "qj_kdk_steps_cfl": (i32, [vp, vp, vp, i32, f64, f64, dp, vp, vp, vp]),
+    "qj_kdk_steps_cfl": (i32, [vp, vp, vp, i32, f64, f64, dp, vp, vp]),
     "qj_kdk_steps": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), i32, vp, vp]),
     "qj_kdk_finalize": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), vp]),
     "qj_kdk_steps_sched": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), i32, dp, dp, vp, vp]),

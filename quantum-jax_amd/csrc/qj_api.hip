@@ -307,10 +307,9 @@ int qj_plan_create(qj_plan** out, int nx, int ny, int nz, double Lx, double Ly, 
     if (e == hipSuccess) e = hipMalloc(&p->drift[a], (size_t)n[a] * 2 * p->rsize);
   }
   p->nblk_src = source_x_blocks(p->d);
-  if (e == hipSuccess)
-    e = hipMalloc((void**)&p->partials, (size_t)(p->nblk_src + kDiagCount) * sizeof(double));
-  if (e == hipSuccess)
-    e = hipMemset(p->partials, 0, (size_t)(p->nblk_src + kDiagCount) * sizeof(double));
+  const size_t nscratch = (size_t)p->nblk_src + kDiagCount + kSlotCount + kCflChunk;
+  if (e == hipSuccess) e = hipMalloc((void**)&p->partials, nscratch * sizeof(double));
+  if (e == hipSuccess) e = hipMemset(p->partials, 0, nscratch * sizeof(double));
   for (int a = 0; a < 3 && e == hipSuccess; ++a) {
     e = dtype == QJ_C64 ? launch_twiddles<float>(p->tw[a], n[a], nullptr)
                         : launch_twiddles<double>(p->tw[a], n[a], nullptr);
@@ -575,12 +574,16 @@ int initial_grid_t(qj_plan* p, qj_state* st, const qj_params* pr, const Work& w,
   return QJ_OK;
 }
 
+// dslot (the CFL stepper): this step's dt and kick time live in the device
+// step slot; the drift tables are rebuilt from it each step and the owed
+// kick stays on the device (the caller reads it back into st->pending).
 template <class T>
 int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void* work,
-                hipStream_t s, const double* kick_dt = nullptr, const double* drift_dt = nullptr) {
+                hipStream_t s, const double* kick_dt = nullptr, const double* drift_dt = nullptr,
+                const double* dslot = nullptr) {
   Work w = carve(p, work);
   SpecTables<T> tb = tables<T>(p);
-  if (!drift_dt)
+  if (!drift_dt && !dslot)
     if (int rc = ensure_drift<T>(p, pr->dt, pr->m_per_hbar, s)) return rc;
   const bool stars = st->np > 0;
   const bool gas = st->gas[0] != nullptr;
@@ -602,8 +605,17 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
     // per-step kick and drift times (a schedule, e.g. the comoving factors of
     // quantum-cosmo.py:170-175), else dt for both
     const double dk = kick_dt ? kick_dt[step] : dt;
-    if (drift_dt)
+    if (dslot) {
+      const int n[3] = {p->d.nx, p->d.ny, p->d.nz};
+      const double L[3] = {p->d.Lx, p->d.Ly, p->d.Lz};
+      for (int a = 0; a < 3; ++a)
+        QJ_CHECK(launch_drift_table<T>(p->drift[a], n[a], L[a], 1.0 / (2.0 * pr->m_per_hbar), s,
+                                       dslot),
+                 "drift table");
+      p->drift_key_dt = NAN;  // the tables now follow the device dt
+    } else if (drift_dt) {
       if (int rc = ensure_drift<T>(p, drift_dt[step], pr->m_per_hbar, s)) return rc;
+    }
     const double kick_t = st->pending + 0.5 * dk;  // owed half kick + this step's first
     // stars: owed closing kick of the previous step and this step's opening
     // half kick (same grid, same positions), drift, deposit at the new positions
@@ -632,14 +644,14 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
       QJ_STAGE(kStageHydro,
                launch_hydro<T>(p->d, live_gas(0), live_gas(1), live_gas(2), live_gas(3), A4, kick_t,
                                dd, pr->cs, pr->flux, spare_gas(0), spare_gas(1), spare_gas(2),
-                               spare_gas(3), s),
+                               spare_gas(3), s, dslot),
                "hydro");
       st->gas_swapped ^= 1;
     }
     // psi: kick folded into the z drift sweep, then y and x (+ Poisson source)
     QJ_STAGE(kStageDriftZKick,
-             launch_drift_axis<T>(p->d, tb, 2, st->psi, V, brho, st->v_ext, pr->m_per_hbar * kick_t, 1,
-                                  s),
+             launch_drift_axis<T>(p->d, tb, 2, st->psi, V, brho, st->v_ext,
+                                  pr->m_per_hbar * (dslot ? 1.0 : kick_t), 1, s, 0, dslot),
              "drift z");
     QJ_STAGE(kStageDriftY, launch_drift_axis<T>(p->d, tb, 1, st->psi, nullptr, nullptr, nullptr, 0.0, 0, s),
              "drift y");
@@ -661,8 +673,48 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
                                  nullptr, st->diag, s))
       return rc;
     // the closing half kicks (psi, gas and stars) are owed to the next step
-    st->pending = 0.5 * dk;
+    if (!dslot) st->pending = 0.5 * dk;
     if (stars) st->pending_star = 0.5 * dt;
+  }
+  return QJ_OK;
+}
+
+// The CFL-limited gas stepper: before each step the device computes
+// dt = min(dt_max, cfl dx / max(cs + |v|)) of the state the step starts
+// from (quantum-fluid-accretion.py:696-701), and the step reads dt and its
+// kick time from the device step slot, so steps follow each other without
+// host round trips.  The host reads the chunk's dts and the owed kick once
+// per kCflChunk steps.
+template <class T>
+int kdk_steps_cfl_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, double cfl,
+                    double dt_max, double* dts, void* work, hipStream_t s) {
+  double* slot = p->partials + p->nblk_src + kDiagCount;
+  double* dts_dev = slot + kSlotCount;
+  const double dx = p->d.Lx / p->d.nx;
+  QJ_CHECK(fill<double>(slot + kSlotOwed, 1, st->pending, s), "step slot");
+  for (int k0 = 0; k0 < nsteps; k0 += kCflChunk) {
+    const int m = nsteps - k0 < kCflChunk ? nsteps - k0 : kCflChunk;
+    for (int k = 0; k < m; ++k) {
+      void* const* g = st->gas_swapped ? st->gas_tmp : st->gas;
+      // the owed kick is zero on a state whose grid is not computed yet
+      const void* A4 = st->grid_valid ? plane<T>(st->grid, p, 1) : nullptr;
+      QJ_CHECK(launch_gas_cfl<T>(p->ncell(), g[1], g[2], g[3], A4, 0.0, pr->cs, dx, cfl, dt_max,
+                                 slot, dts_dev, k, p->partials, p->nblk_src, s),
+               "cfl");
+      if (int rc = kdk_steps_t<T>(p, st, pr, 1, work, s, nullptr, nullptr, slot)) return rc;
+    }
+    double h[kCflChunk + 1];
+    hipError_t e = hipMemcpyAsync(h, dts_dev, (size_t)m * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(h + m, slot + kSlotOwed, sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hipfail(e, "qj_kdk_steps_cfl: read-back");
+    st->pending = h[m];
+    for (int k = 0; k < m; ++k) {
+      if (dts) dts[k0 + k] = h[k];
+      if (!(h[k] > 0.0) || !std::isfinite(h[k]))
+        return fail(QJ_EARG, "qj_kdk_steps_cfl: non-finite CFL time step (non-finite gas velocity?)");
+    }
   }
   return QJ_OK;
 }
@@ -861,30 +913,23 @@ int qj_gas_cfl_dt(qj_plan* p, const qj_state* st, const qj_params* pr, double cf
   const double dx = p->d.Lx / p->d.nx;
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = QJ_PRECISION(
-      p, launch_gas_cfl<float>(p->ncell(), g[1], g[2], g[3], A4, h, pr->cs, dx, cfl, dt_max, out, s),
-      launch_gas_cfl<double>(p->ncell(), g[1], g[2], g[3], A4, h, pr->cs, dx, cfl, dt_max, out, s));
+      p,
+      launch_gas_cfl<float>(p->ncell(), g[1], g[2], g[3], A4, h, pr->cs, dx, cfl, dt_max, out,
+                            nullptr, 0, p->partials, p->nblk_src, s),
+      launch_gas_cfl<double>(p->ncell(), g[1], g[2], g[3], A4, h, pr->cs, dx, cfl, dt_max, out,
+                             nullptr, 0, p->partials, p->nblk_src, s));
   return e == hipSuccess ? QJ_OK : hipfail(e, "qj_gas_cfl_dt");
 }
 
 int qj_kdk_steps_cfl(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, double cfl,
-                     double dt_max, double* dts, double* cfl_dev, void* work, void* stream) {
-  if (!cfl_dev || nsteps < 0) return fail(QJ_EARG, "qj_kdk_steps_cfl: bad arguments");
+                     double dt_max, double* dts, void* work, void* stream) {
+  if (nsteps < 0 || !(cfl > 0.0) || !(dt_max > 0.0))
+    return fail(QJ_EARG, "qj_kdk_steps_cfl: bad arguments");
   if (int rc = qj_kdk_steps_sched(p, st, pr, 0, nullptr, nullptr, work, stream)) return rc;
+  if (!st->gas[0] || !st->gas_tmp[0]) return fail(QJ_EARG, "qj_kdk_steps_cfl: the state has no gas");
   hipStream_t s = (hipStream_t)stream;
-  for (int k = 0; k < nsteps; ++k) {
-    if (int rc = qj_gas_cfl_dt(p, st, pr, cfl, dt_max, cfl_dev, stream)) return rc;
-    double dt = 0.0;
-    hipError_t e = hipMemcpyAsync(&dt, cfl_dev + 1, sizeof(double), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return hipfail(e, "qj_kdk_steps_cfl: dt read-back");
-    if (!(dt > 0.0) || !std::isfinite(dt))
-      return fail(QJ_EARG, "qj_kdk_steps_cfl: non-finite CFL time step (non-finite gas velocity?)");
-    const int rc = p->dtype == QJ_C64 ? kdk_steps_t<float>(p, st, pr, 1, work, s, &dt, &dt)
-                                      : kdk_steps_t<double>(p, st, pr, 1, work, s, &dt, &dt);
-    if (rc) return rc;
-    if (dts) dts[k] = dt;
-  }
-  return QJ_OK;
+  return p->dtype == QJ_C64 ? kdk_steps_cfl_t<float>(p, st, pr, nsteps, cfl, dt_max, dts, work, s)
+                            : kdk_steps_cfl_t<double>(p, st, pr, nsteps, cfl, dt_max, dts, work, s);
 }
 
 int qj_bh_accretion(qj_plan* p, void* psi, const double* pos, const double* vel, double* m_bh,

@@ -140,7 +140,12 @@ struct k_hydro_stream {
                                              const T* __restrict__ vy, const T* __restrict__ vz,
                                              const T* __restrict__ A4, T kick,
                                              T* __restrict__ rho_o, T* __restrict__ vx_o,
-                                             T* __restrict__ vy_o, T* __restrict__ vz_o) {
+                                             T* __restrict__ vy_o, T* __restrict__ vz_o,
+                                             const double* __restrict__ dslot) {
+    if (dslot) {  // step time and owed + opening kick from the device step slot
+      g.dt = (T)dslot[kSlotDt];
+      kick = (T)dslot[kSlotKick];
+    }
     __shared__ T raw[4][4][kHR * kHR];     // [slot][var][cell]
     __shared__ T st[2][12][kHS * kHS];     // [buf][p0..3, sy0..3, sz0..3][cell]
     const int tid = threadIdx.x;
@@ -373,15 +378,19 @@ struct k_gas_kick {
 
 // Largest signal speed max(cs + |v|) over the cells (quantum-fluid-accretion.py:697-699,
 // isothermal: c_eff = cs), with v the velocity after the owed kick
-// v + h a (h = 0: the stored velocity).  NaN-propagating wave maxima, one
-// atomic per wave into out[0] (cleared by k_cfl_clear).
+// v + h a (h = 0: the stored velocity).  NaN-propagating; each block stores
+// its maximum in part[blockIdx.x] and k_cfl_finish reduces them (no
+// contended atomics on one address).
 template <class T>
 struct k_gas_signal {
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(long long n, const T* __restrict__ vx,
                                              const T* __restrict__ vy, const T* __restrict__ vz,
-                                             const T* __restrict__ A4, T h, double cs,
-                                             double* __restrict__ out) {
+                                             const T* __restrict__ A4, T h,
+                                             const double* __restrict__ hdev, double cs,
+                                             double* __restrict__ part) {
+    __shared__ double wmax[4];
+    if (hdev) h = (T)*hdev;  // the kick owed on the device (step slot)
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long stride = (long long)gridDim.x * blockDim.x;
     double m = 0.0;
@@ -400,26 +409,42 @@ struct k_gas_signal {
       const double other = __shfl_xor(m, o, 64);
       m = (other > m || other != other) ? other : m;
     }
-    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&out[0], m);
-  }
-};
-
-struct k_cfl_clear {
-  static constexpr int kThreads = 64;
-  static __device__ __forceinline__ void run(double* out) {
-    if (threadIdx.x == 0) out[0] = 0.0;
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double b = wmax[0];
+      for (int w = 1; w < 4; ++w) b = (wmax[w] > b || wmax[w] != wmax[w]) ? wmax[w] : b;
+      part[blockIdx.x] = b;
+    }
   }
 };
 
 // dt = min(dt_max, cfl dx / (Cmax + 1e-30)) (quantum-fluid-accretion.py:700-701);
-// a NaN signal speed gives a NaN dt.
+// a NaN signal speed gives a NaN dt.  With advance, the step slot moves on:
+// kick = owed + dt/2, owed = dt/2, dts[k] = dt.
 struct k_cfl_finish {
   static constexpr int kThreads = 64;
-  static __device__ __forceinline__ void run(double* out, double cfl, double dx, double dt_max) {
+  static __device__ __forceinline__ void run(const double* __restrict__ part, int nblk,
+                                             double* out, double cfl, double dx, double dt_max,
+                                             int advance, double* dts, int k) {
+    double m = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += 64) m = (part[i] > m || part[i] != part[i]) ? part[i] : m;
+  #pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double other = __shfl_xor(m, o, 64);
+      m = (other > m || other != other) ? other : m;
+    }
     if (threadIdx.x == 0) {
-      const double c = out[0];
+      out[kSlotCmax] = m;
+      const double c = m;
       const double dth = cfl * dx / (c + 1e-30);
-      out[1] = (c != c) ? c : (dth < dt_max ? dth : dt_max);
+      const double dt = (c != c) ? c : (dth < dt_max ? dth : dt_max);
+      out[kSlotDt] = dt;
+      if (advance) {
+        out[kSlotKick] = out[kSlotOwed] + 0.5 * dt;
+        out[kSlotOwed] = 0.5 * dt;
+        dts[k] = dt;
+      }
     }
   }
 };
@@ -429,13 +454,16 @@ struct k_cfl_finish {
 template <class T>
 hipError_t launch_gas_cfl(long long n, const void* vx, const void* vy, const void* vz,
                           const void* A4, double h, double cs, double dx, double cfl,
-                          double dt_max, double* out, hipStream_t s) {
+                          double dt_max, double* out, double* dts, int k, double* part,
+                          int max_blocks, hipStream_t s) {
   long long blocks = (n + 255) / 256;
-  if (blocks > 8192) blocks = 8192;
-  launch<k_cfl_clear>(1, 64, s, out);
+  if (blocks > max_blocks) blocks = max_blocks;
+  // with dts, out is a whole step slot and the owed kick lives in it
+  const double* hdev = (dts && A4) ? out + kSlotOwed : nullptr;
   launch<k_gas_signal<T>>((int)blocks, 256, s, n, (const T*)vx, (const T*)vy, (const T*)vz,
-                          (const T*)(h != 0.0 ? A4 : nullptr), (T)h, cs, out);
-  launch<k_cfl_finish>(1, 64, s, out, cfl, dx, dt_max);
+                          (const T*)A4, (T)h, hdev, cs, part);
+  launch<k_cfl_finish>(1, 64, s, (const double*)part, (int)blocks, out, cfl, dx, dt_max,
+                       dts ? 1 : 0, dts, k);
   return hipGetLastError();
 }
 
@@ -452,8 +480,8 @@ template <class T>
 hipError_t launch_hydro(const Dims& d, const void* rho, const void* vx, const void* vy,
                         const void* vz, const void* A4, double kick, double dt, double cs,
                         int flux_kind, void* rho_o, void* vx_o, void* vy_o, void* vz_o,
-                        hipStream_t s) {
-  if (kick != 0.0 && A4 == nullptr) return hipErrorInvalidValue;
+                        hipStream_t s, const double* dslot) {
+  if ((kick != 0.0 || dslot) && A4 == nullptr) return hipErrorInvalidValue;
   if (flux_kind != QJ_FLUX_RUSANOV && flux_kind != QJ_FLUX_HLL) return hipErrorInvalidValue;
   HGeom<T> g;
   g.nx = d.nx;
@@ -473,29 +501,29 @@ hipError_t launch_hydro(const Dims& d, const void* rho, const void* vx, const vo
     launch<k_hydro_stream<T, QJ_FLUX_HLL>>(nblk, nthr, s, g, TY, TZ, XC, (const T*)rho,
                                            (const T*)vx, (const T*)vy, (const T*)vz,
                                            (const T*)A4, (T)kick, (T*)rho_o, (T*)vx_o,
-                                           (T*)vy_o, (T*)vz_o);
+                                           (T*)vy_o, (T*)vz_o, dslot);
   else
     launch<k_hydro_stream<T, QJ_FLUX_RUSANOV>>(nblk, nthr, s, g, TY, TZ, XC, (const T*)rho,
                                                (const T*)vx, (const T*)vy, (const T*)vz,
                                                (const T*)A4, (T)kick, (T*)rho_o, (T*)vx_o,
-                                               (T*)vy_o, (T*)vz_o);
+                                               (T*)vy_o, (T*)vz_o, dslot);
   return hipGetLastError();
 }
 
 template hipError_t launch_hydro<float>(const Dims&, const void*, const void*, const void*,
                                         const void*, const void*, double, double, double, int,
-                                        void*, void*, void*, void*, hipStream_t);
+                                        void*, void*, void*, void*, hipStream_t, const double*);
 template hipError_t launch_hydro<double>(const Dims&, const void*, const void*, const void*,
                                          const void*, const void*, double, double, double, int,
-                                         void*, void*, void*, void*, hipStream_t);
+                                         void*, void*, void*, void*, hipStream_t, const double*);
 template hipError_t launch_gas_kick<float>(long long, void*, void*, void*, const void*, double,
                                            hipStream_t);
 template hipError_t launch_gas_cfl<float>(long long, const void*, const void*, const void*,
                                           const void*, double, double, double, double, double,
-                                          double*, hipStream_t);
+                                          double*, double*, int, double*, int, hipStream_t);
 template hipError_t launch_gas_cfl<double>(long long, const void*, const void*, const void*,
                                            const void*, double, double, double, double, double,
-                                           double*, hipStream_t);
+                                           double*, double*, int, double*, int, hipStream_t);
 template hipError_t launch_gas_kick<double>(long long, void*, void*, void*, const void*, double,
                                             hipStream_t);
 

@@ -28,7 +28,8 @@ struct SpecTables {
 template <class T>
 hipError_t launch_twiddles(void* tw, int n, hipStream_t s);
 template <class T>
-hipError_t launch_drift_table(void* tab, int n, double L, double c, hipStream_t s);
+hipError_t launch_drift_table(void* tab, int n, double L, double c, hipStream_t s,
+                              const double* dslot = nullptr);
 
 template <class T>
 hipError_t launch_kick(void* psi, const void* V, const void* Vb, const void* Vx, double coef,
@@ -38,7 +39,8 @@ hipError_t launch_kick(void* psi, const void* V, const void* Vb, const void* Vx,
 template <class T>
 hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, void* psi,
                              const void* V, const void* Vb, const void* Vx, double kick_coef,
-                             int kick, hipStream_t s, int conj_d = 0);
+                             int kick, hipStream_t s, int conj_d = 0,
+                             const double* dslot = nullptr);
 
 template <class T>
 hipError_t launch_source_x(const Dims& d, const SpecTables<T>& tb, int drift, void* psi,
@@ -59,7 +61,8 @@ template <class T>
 hipError_t launch_poisson_y(const Dims& d, const SpecTables<T>& tb, int grad, void* H, void* Uz,
                             void* Uy, hipStream_t s);
 
-// Writes V and, with grad, the cell-interleaved grid A4 = (a_x, a_y, a_z, V).
+// Writes V and, with grad, the cell-interleaved grid A4 = (a_x, a_y, a_z, V)
+// and folds max |a| into diag[kDiagAMax].
 template <class T>
 hipError_t launch_poisson_x(const Dims& d, const SpecTables<T>& tb, int grad, const void* H,
                             const void* Uy, const void* Uzz, void* V, void* A4,
@@ -122,22 +125,33 @@ hipError_t launch_adj_poisson(const Dims& d, const SpecTables<T>& tb, const void
                               void* U3, void* rhobar, hipStream_t s);
 
 // ---- gas ----------------------------------------------------------------
+// Device step slot of the CFL-limited stepper (doubles): the reduced signal
+// speed, this step's dt, its owed + opening kick time, and the kick owed
+// after it.  Kernels given the slot read dt and the kick time from it.
+constexpr int kSlotCmax = 0, kSlotDt = 1, kSlotKick = 2, kSlotOwed = 3, kSlotCount = 4;
+constexpr int kCflChunk = 64;  // CFL steps per host read-back
+
 // Applies v += a*kick first when kick != 0 (A4 may be null when kick == 0).
-// flux_kind: QJ_FLUX_RUSANOV (the reference's) or QJ_FLUX_HLL.
+// flux_kind: QJ_FLUX_RUSANOV (the reference's) or QJ_FLUX_HLL.  dslot
+// (optional): dt and kick come from the device step slot instead.
 template <class T>
 hipError_t launch_hydro(const Dims& d, const void* rho, const void* vx, const void* vy,
                         const void* vz, const void* A4, double kick, double dt, double cs,
                         int flux_kind, void* rho_o, void* vx_o, void* vy_o, void* vz_o,
-                        hipStream_t s);
+                        hipStream_t s, const double* dslot = nullptr);
 template <class T>
 hipError_t launch_gas_kick(long long n, void* vx, void* vy, void* vz, const void* A4, double h,
                            hipStream_t s);
-// out[0] = max over cells of cs + |v + h a| (a from A4, skipped when h == 0),
-// out[1] = min(dt_max, cfl dx / (out[0] + 1e-30)).
+// out[kSlotCmax] = max over cells of cs + |v + h a| (a from A4, skipped
+// when A4 is null), out[kSlotDt] = min(dt_max, cfl dx / (Cmax + 1e-30)).
+// With dts, out is a whole step slot: h is read from out[kSlotOwed], the
+// slot advances (kick = owed + dt/2, owed = dt/2) and dts[k] = dt.  part:
+// scratch for max_blocks per-block maxima.
 template <class T>
 hipError_t launch_gas_cfl(long long n, const void* vx, const void* vy, const void* vz,
                           const void* A4, double h, double cs, double dx, double cfl,
-                          double dt_max, double* out, hipStream_t s);
+                          double dt_max, double* out, double* dts, int k, double* part,
+                          int max_blocks, hipStream_t s);
 
 // Tile-binned star deposit (owner computes; no global float atomics).
 constexpr int kTileX = 8, kTileY = 8, kTileZ = 64;

@@ -108,7 +108,9 @@ struct k_star_step {
   }
 };
 
-// Closing half kick and dt_s = min |dx / vel| (quantum-stars.py:321,328).
+// Closing half kick and dt_s = min |dx / vel| (quantum-stars.py:321,328)
+// over the finite values (a non-finite velocity is left out of the minimum;
+// jnp.min would return NaN), one atomic per block.
 template <class T>
 struct k_star_kick {
   static constexpr int kThreads = 256;
@@ -127,15 +129,22 @@ struct k_star_kick {
         double v = vel[3 * p + d] + a[d] * h;
         vel[3 * p + d] = v;
         double r = fabs(g.dx / v);
-        dts = (r < dts || r != r) ? r : dts;
+        dts = (r < dts) ? r : dts;  // NaN compares false: skipped
       }
     }
   #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
       double other = __shfl_xor(dts, o, 64);
-      dts = (other < dts || other != other) ? other : dts;
+      dts = (other < dts) ? other : dts;
     }
-    if ((threadIdx.x & 63) == 0 && dts == dts) atomic_min_nonneg(&diag[kDiagDtS], dts);
+    __shared__ double wmin[kThreads / 64];
+    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = dts;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+  #pragma unroll
+      for (int w = 1; w < kThreads / 64; ++w) dts = (wmin[w] < dts) ? wmin[w] : dts;
+      if (dts < __builtin_inf()) atomic_min_nonneg(&diag[kDiagDtS], dts);
+    }
   }
 };
 

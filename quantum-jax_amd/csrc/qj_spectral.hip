@@ -39,11 +39,13 @@ __global__ void k_twiddles(cpx<T>* tw, int n) {
 }
 
 // D[k] = exp(-i * c * kphys^2) / n, kphys = 2 pi/L * freq(k); c = dt/(2 mph).
-// quantum-stars.py:308 (one axis of the separable kinetic phase).
+// quantum-stars.py:308 (one axis of the separable kinetic phase).  With
+// dslot, c is a factor and dt comes from the device step slot.
 template <class T>
-__global__ void k_drift_table(cpx<T>* tab, int n, double L, double c) {
+__global__ void k_drift_table(cpx<T>* tab, int n, double L, double c, const double* dslot) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
+  if (dslot) c *= dslot[kSlotDt];
   double kp = 2.0 * M_PI / L * (double)freq_index(k, n);
   double th = reduce_2pi(c * kp * kp);
   double s, co;
@@ -83,9 +85,10 @@ struct k_drift_c {
                                              const cpx<T>* __restrict__ tw, const T* __restrict__ V,
                                              const T* __restrict__ Vb,
                                              const T* __restrict__ Vx, double kick_coef,
-                                             int conj_d) {
+                                             int conj_d, const double* __restrict__ dslot) {
     using S = PencilShape<N>;
     __shared__ cpx<T> lds[S::kLds];
+    if (KICK && dslot) kick_coef *= dslot[kSlotKick];  // kick time from the device step slot
     Lane<N, true> ln(threadIdx.x);
     const int q = blockIdx.x * S::C + ln.col;
     const bool act = q < g.ncols;
@@ -489,7 +492,8 @@ __device__ __forceinline__ void load_herm_rows(cpx<T> (&v)[VW][E], const cpx<T>*
 // ---------------------------------------------------------------------------
 // Poisson x pass (inverse, Hermitian-completed): V = IFFT_x(UU); with GRAD
 // a_x + i a_y = -IFFT_x(i kx UU + i Uy), a_z = -IFFT_x(Uzz), written as the
-// cell-interleaved grid A4 = (a_x, a_y, a_z, V), and max |a|.
+// cell-interleaved grid A4 = (a_x, a_y, a_z, V), and max |a| folded into
+// the running maximum diag[kDiagAMax] with one atomic per block.
 // quantum-stars.py:192 and :256-260.
 // ---------------------------------------------------------------------------
 template <class T, int N, bool GRAD>
@@ -567,14 +571,23 @@ struct k_poisson_x {
           }
         }
       }
-      // wave max (NaN-propagating), one atomic per wave
+      // block max (NaN-propagating), one atomic per block: device-scope
+      // atomics on one address serialise, so one per wave would stretch the
+      // kernel's tail
       double am = (double)amax;
   #pragma unroll
       for (int o = 32; o > 0; o >>= 1) {
         double other = __shfl_xor(am, o, 64);
         am = (other > am || other != other) ? other : am;
       }
-      if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&diag[kDiagAMax], am);
+      __shared__ double wmax[kThreads / 64];
+      if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = am;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+  #pragma unroll
+        for (int w = 1; w < kThreads / 64; ++w) am = (wmax[w] > am || wmax[w] != wmax[w]) ? wmax[w] : am;
+        atomic_max_nonneg(&diag[kDiagAMax], am);
+      }
     }
   }
 };
@@ -802,8 +815,9 @@ hipError_t launch_twiddles(void* tw, int n, hipStream_t s) {
 }
 
 template <class T>
-hipError_t launch_drift_table(void* tab, int n, double L, double c, hipStream_t s) {
-  k_drift_table<T><<<(n + 255) / 256, 256, 0, s>>>((cpx<T>*)tab, n, L, c);
+hipError_t launch_drift_table(void* tab, int n, double L, double c, hipStream_t s,
+                              const double* dslot) {
+  k_drift_table<T><<<(n + 255) / 256, 256, 0, s>>>((cpx<T>*)tab, n, L, c, dslot);
   return hipGetLastError();
 }
 
@@ -820,7 +834,7 @@ hipError_t launch_kick(void* psi, const void* V, const void* Vb, const void* Vx,
 template <class T>
 hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, void* psi,
                              const void* V, const void* Vb, const void* Vx, double kick_coef,
-                             int kick, hipStream_t s, int conj_d) {
+                             int kick, hipStream_t s, int conj_d, const double* dslot) {
   cpx<T>* p = (cpx<T>*)psi;
   const T* v = (const T*)V;
   const T* vb = (const T*)Vb;
@@ -832,10 +846,10 @@ hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, v
     QJ_DISPATCH_N(d.nz, {
       if (kick)
         launch<k_drift_c<T, N_, true>>(nblocks<N_>(g.ncols), 256, s, p, g, D, tw, v, vb, vx,
-                                       kick_coef, conj_d);
+                                       kick_coef, conj_d, dslot);
       else
         launch<k_drift_c<T, N_, false>>(nblocks<N_>(g.ncols), 256, s, p, g, D, tw, v, vb, vx,
-                                        kick_coef, conj_d);
+                                        kick_coef, conj_d, dslot);
     });
   } else {
     if (kick) return hipErrorInvalidValue;  // the kick is fused into the z sweep only
@@ -951,12 +965,13 @@ hipError_t launch_poisson_x(const Dims& d, const SpecTables<T>& tb, int grad, co
   const cpx<T>* tw = (const cpx<T>*)tb.tw_x;
   const double kfx = 2.0 * M_PI / d.Lx;
   QJ_DISPATCH_N(d.nx, {
+    const int nb = nblocks_s<T, N_>(g.ncols);
     if (grad)
-      launch<k_poisson_x<T, N_, true>>(nblocks_s<T, N_>(g.ncols), 256, s,
+      launch<k_poisson_x<T, N_, true>>(nb, 256, s,
           (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A4,
           (const T*)addend, diag);
     else
-      launch<k_poisson_x<T, N_, false>>(nblocks_s<T, N_>(g.ncols), 256, s,
+      launch<k_poisson_x<T, N_, false>>(nb, 256, s,
           (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A4,
           (const T*)addend, diag);
   });
@@ -1028,13 +1043,13 @@ hipError_t launch_adj_poisson(const Dims& d, const SpecTables<T>& tb, const void
 
 #define QJ_INST(T)                                                                              \
   template hipError_t launch_twiddles<T>(void*, int, hipStream_t);                              \
-  template hipError_t launch_drift_table<T>(void*, int, double, double, hipStream_t);           \
+  template hipError_t launch_drift_table<T>(void*, int, double, double, hipStream_t,             \
+                                            const double*);           \
   template hipError_t launch_kick<T>(void*, const void*, const void*, const void*, double,       \
                                      long long, hipStream_t);                                   \
   template hipError_t launch_drift_axis<T>(const Dims&, const SpecTables<T>&, int, void*,        \
                                            const void*, const void*, const void*, double, int,  \
-                                           hipStream_t,                                         \
-                                           int);                                                \
+                                           hipStream_t, int, const double*);                    \
   template hipError_t launch_source_x<T>(const Dims&, const SpecTables<T>&, int, void*, void*,  \
                                          int, void*, double, double, void*, double*, double*,   \
                                          hipStream_t);                                          \

@@ -158,15 +158,12 @@ class KDKSimulation:
         if n == 0:
             return self
         dt_max = self.dt if dt_max is None else float(dt_max)
-        if self._cfl_out is None:
-            self._cfl_out = torch.empty(2, dtype=torch.float64, device=self.plan.device)
         dts = np.zeros(n, dtype=np.float64)
         with torch.cuda.device(self.plan.device):
             rc = _lib.lib().qj_kdk_steps_cfl(self.plan.handle, C.byref(self._st),
                                              C.byref(self._params), n, float(cfl), dt_max,
                                              dts.ctypes.data_as(C.POINTER(C.c_double)),
-                                             ptr(self._cfl_out), ptr(self.plan.workspace()),
-                                             self.plan.stream())
+                                             ptr(self.plan.workspace()), self.plan.stream())
         taken = [float(d) for d in dts if d > 0.0]
         self.dts.extend(taken)
         self.t += math.fsum(taken)

@@ -6,7 +6,7 @@
 * a stars step on a flat, anisotropic box (tiles shrunk along z);
 * empty star sets (deposit, interpolation, acceleration, stepper);
 * non-finite star positions (skipped by the deposit, NaN accelerations, the
-  rest of the run unaffected).
+  rest of the run unaffected, dt_s over the finite velocities).
 Rejected plan shapes are checked without a GPU in test_boundary.py.
 """
 
@@ -148,3 +148,5 @@ def test_nonfinite_star_positions(qj, cuda, dtype):
     assert rel(runs[0]["pos"][keep], runs[1]["pos"]) < (1e-10 if dtype == C64 else 1e-13)
     assert rel(runs[0]["vel"][keep], runs[1]["vel"]) < tol
     assert torch.isnan(runs[0]["vel"][bad]).all()
+    # dt_s is the minimum over the finite velocities
+    assert abs(runs[0]["dt_s"] / runs[1]["dt_s"] - 1) < tol
