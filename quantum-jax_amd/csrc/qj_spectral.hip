@@ -80,6 +80,9 @@ struct k_kick {
 template <class T, int N, bool KICK>
 struct k_drift_c {
   static constexpr int kThreads = 256;
+  // A block walks pencil groups blockIdx.x, blockIdx.x + gridDim.x, ...; the
+  // next group's psi (and V) rows are loaded while the current group is
+  // transformed, so each wave keeps memory traffic in flight through its FFTs.
   static __device__ __forceinline__ void run(cpx<T>* __restrict__ psi, PassGeom g,
                                              const cpx<T>* __restrict__ D,
                                              const cpx<T>* __restrict__ tw, const T* __restrict__ V,
@@ -90,35 +93,54 @@ struct k_drift_c {
     __shared__ cpx<T> lds[S::kLds];
     if (KICK && dslot) kick_coef *= dslot[kSlotKick];  // kick time from the device step slot
     Lane<N, true> ln(threadIdx.x);
-    const int q = blockIdx.x * S::C + ln.col;
-    const bool act = q < g.ncols;
-    const int base = act ? (int)pencil_base(g, q) : 0;
     const int se = (int)g.s_elem;
-    cpx<T> v[S::E];
+    const int ngroups = (g.ncols + S::C - 1) / S::C;
+    cpx<T> nv[S::E];
+    T nth[KICK ? S::E : 1];
+    auto fetch = [&](int grp) {
+      const int q = grp * S::C + ln.col;
+      const bool act = q < g.ncols;
+      const int base = act ? (int)pencil_base(g, q) : 0;
   #pragma unroll
-    for (int m = 0; m < S::E; ++m) {
-      const int idx = base + (ln.j + S::TT * m) * se;
-      v[m] = act ? psi[idx] : cpx<T>{0, 0};
-      if constexpr (KICK) {
-        if (act) {
-          double th = (double)V[idx];
-          if (Vb) th += (double)Vb[idx];
-          if (Vx) th += (double)Vx[idx];
-          v[m] = cmul(v[m], expmi<T>(kick_coef * th));
+      for (int m = 0; m < S::E; ++m) {
+        const int idx = base + (ln.j + S::TT * m) * se;
+        nv[m] = act ? psi[idx] : cpx<T>{0, 0};
+        if constexpr (KICK) nth[m] = act ? V[idx] : (T)0;
+      }
+    };
+    int grp = blockIdx.x;
+    if (grp < ngroups) fetch(grp);
+    cpx<T>* my = lds + ln.col * S::E * S::P;
+    for (; grp < ngroups; grp += gridDim.x) {
+      const int q = grp * S::C + ln.col;
+      const bool act = q < g.ncols;
+      const int base = act ? (int)pencil_base(g, q) : 0;
+      cpx<T> v[S::E];
+  #pragma unroll
+      for (int m = 0; m < S::E; ++m) {
+        v[m] = nv[m];
+        if constexpr (KICK) {
+          if (act) {
+            const int idx = base + (ln.j + S::TT * m) * se;
+            double th = (double)nth[m];
+            if (Vb) th += (double)Vb[idx];
+            if (Vx) th += (double)Vx[idx];
+            v[m] = cmul(v[m], expmi<T>(kick_coef * th));
+          }
         }
       }
-    }
-    cpx<T>* my = lds + ln.col * S::E * S::P;
-    pencil_fft<N, -1, T>(v, ln.j, my, tw);
+      if (grp + (int)gridDim.x < ngroups) fetch(grp + gridDim.x);
+      pencil_fft<N, -1, T>(v, ln.j, my, tw);
   #pragma unroll
-    for (int m = 0; m < S::E; ++m) {
-      const cpx<T> d0 = D[ln.j + S::TT * m];
-      v[m] = cmul(v[m], conj_d ? conjg(d0) : d0);
-    }
-    pencil_fft<N, +1, T>(v, ln.j, my, tw);
-    if (act) {
+      for (int m = 0; m < S::E; ++m) {
+        const cpx<T> d0 = D[ln.j + S::TT * m];
+        v[m] = cmul(v[m], conj_d ? conjg(d0) : d0);
+      }
+      pencil_fft<N, +1, T>(v, ln.j, my, tw);
+      if (act) {
   #pragma unroll
-      for (int m = 0; m < S::E; ++m) psi[base + (ln.j + S::TT * m) * se] = v[m];
+        for (int m = 0; m < S::E; ++m) psi[base + (ln.j + S::TT * m) * se] = v[m];
+      }
     }
   }
 };
@@ -777,6 +799,13 @@ template <int N>
 inline int nblocks(int ncols) {
   return (ncols + PencilShape<N>::C - 1) / PencilShape<N>::C;
 }
+// Blocks of a group-walking contiguous sweep: at most four resident blocks
+// per CU (256 CUs), each taking several pencil groups.
+template <int N>
+inline int persist_blocks(int ncols) {
+  const int n = nblocks<N>(ncols);
+  return n < 1024 ? n : 1024;
+}
 template <class T, int N>
 inline int nblocks_s(int ncols) {
   const int per = PencilShape<N>::C * StridedVW<T, N>::value;
@@ -845,11 +874,11 @@ hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, v
     const cpx<T>* tw = (const cpx<T>*)tb.tw_z;
     QJ_DISPATCH_N(d.nz, {
       if (kick)
-        launch<k_drift_c<T, N_, true>>(nblocks<N_>(g.ncols), 256, s, p, g, D, tw, v, vb, vx,
-                                       kick_coef, conj_d, dslot);
+        launch<k_drift_c<T, N_, true>>(persist_blocks<N_>(g.ncols), 256, s, p, g, D, tw, v, vb,
+                                       vx, kick_coef, conj_d, dslot);
       else
-        launch<k_drift_c<T, N_, false>>(nblocks<N_>(g.ncols), 256, s, p, g, D, tw, v, vb, vx,
-                                        kick_coef, conj_d, dslot);
+        launch<k_drift_c<T, N_, false>>(persist_blocks<N_>(g.ncols), 256, s, p, g, D, tw, v, vb,
+                                        vx, kick_coef, conj_d, dslot);
     });
   } else {
     if (kick) return hipErrorInvalidValue;  // the kick is fused into the z sweep only
