@@ -223,7 +223,9 @@ int qj_kdk_finalize(qj_plan* plan, qj_state* state, const qj_params* prm, void* 
 /* ---- adjoint of the KDK step (no counterpart in the reference, which
  * never differentiates despite README.md:7) ----------------------------
  * qj_kdk_record runs nsteps steps (quantum-stars.py:295-330; the state
- * must be finalized) and stores, per step, psi after the drift (psi2), the
+ * must be finalized; the steps merge consecutive half kicks as qj_kdk_steps
+ * does, and the state is finalized at the end) and stores, per step, psi
+ * after the drift (psi2), the
  * grid after the step's Poisson solve (grid1, 5N reals as in qj_state), the
  * star positions before and after the drift (x0, x1), and psi at the start
  * (psi0: needed for step 0 only, NULL allowed for later steps); grid0

@@ -765,13 +765,14 @@ int kdk_record_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, qj_t
     qj_tape_step& t = tape[k];
     if (t.psi0) QJ_CHECK(copy_async(t.psi0, st->psi, cb, s), "tape psi0");
     if (st->np > 0) QJ_CHECK(copy_async(t.x0, st->pos, xb, s), "tape x0");
+    // merged steps, as the forward runs them: the closing half kick of step k
+    // is owed to (and merged into) the opening kick of step k + 1
     if (int rc = kdk_steps_t<T>(p, st, pr, 1, work, s)) return rc;
     QJ_CHECK(copy_async(t.psi2, st->psi, cb, s), "tape psi2");
     QJ_CHECK(copy_async(t.grid1, st->grid, grid_bytes(p), s), "tape grid1");
     if (st->np > 0) QJ_CHECK(copy_async(t.x1, st->pos, xb, s), "tape x1");
-    if (int rc = kdk_finalize_t<T>(p, st, pr, s)) return rc;
   }
-  return QJ_OK;
+  return kdk_finalize_t<T>(p, st, pr, s);
 }
 
 template <class T>
