@@ -371,7 +371,9 @@ struct k_bin_scatter {
 
 // One block per tile: LDS accumulation of the corners inside the tile (the
 // corner values of deposit_one), then the whole tile is stored.
-template <class T>
+// VOL: LDS cells per tile (the tile volume rounded up to a supported size),
+// so small tiles take small LDS and more blocks fit per CU.
+template <class T, int VOL = kTileMaxCells>
 struct k_tile_deposit {
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(CicGeom g, TileGeom tg,
@@ -379,7 +381,7 @@ struct k_tile_deposit {
                                              const double* __restrict__ mass, double m_scalar,
                                              const int* __restrict__ offset,
                                              const int* __restrict__ sorted, T* __restrict__ rho) {
-    __shared__ T acc[kTileMaxCells];
+    __shared__ T acc[VOL];
     const int t = blockIdx.x;
     const int tz = t % tg.ntz, ty = (t / tg.ntz) % tg.nty, tx = t / (tg.ntz * tg.nty);
     const int vol = 1 << (tg.lx + tg.ly + tg.lz);
@@ -414,7 +416,7 @@ struct k_tile_deposit {
 // Vector form of the per-tile deposit for the adjoint of the star kicks:
 // abar[c] = sum over stars of w_c h vbar (the corner values of the atomic
 // interpolation adjoint), written as whole (a_x, a_y, a_z, 0) tiles.
-template <class T>
+template <class T, int VOL = kTileMaxCells>
 struct k_tile_deposit_vec {
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(CicGeom g, TileGeom tg,
@@ -423,7 +425,7 @@ struct k_tile_deposit_vec {
                                              const int* __restrict__ offset,
                                              const int* __restrict__ sorted,
                                              T* __restrict__ abar4) {
-    __shared__ T acc[3][kTileMaxCells];
+    __shared__ T acc[3][VOL];
     const int t = blockIdx.x;
     const int tz = t % tg.ntz, ty = (t / tg.ntz) % tg.nty, tx = t / (tg.ntz * tg.nty);
     const int vol = 1 << (tg.lx + tg.ly + tg.lz);
@@ -504,6 +506,16 @@ struct k_bh_accretion {
 };
 
 inline int pblocks(long long np) { return (int)((np + 255) / 256); }
+
+// Runs stmt with VOL_ = the smallest supported LDS volume holding a tile.
+#define QJ_TILE_VOL(tg, ...)                                                 \
+  do {                                                                       \
+    const int vol_ = 1 << ((tg).lx + (tg).ly + (tg).lz);                     \
+    if (vol_ <= 512) { constexpr int VOL_ = 512; __VA_ARGS__; }              \
+    else if (vol_ <= 1024) { constexpr int VOL_ = 1024; __VA_ARGS__; }       \
+    else if (vol_ <= 2048) { constexpr int VOL_ = 2048; __VA_ARGS__; }       \
+    else { constexpr int VOL_ = kTileMaxCells; __VA_ARGS__; }                \
+  } while (0)
 
 }  // namespace
 
@@ -633,8 +645,9 @@ hipError_t launch_tile_deposit(const Dims& d, const double* pos, const double* m
   if (np > 0)
     launch<k_bin_scatter>(pblocks(np), 256, s, make_geom(d), tg, pos, np, (const int*)sb.rank,
                           (const int*)sb.offset, sb.sorted);
-  launch<k_tile_deposit<T>>(tg.ntiles, 256, s, make_geom(d), tg, pos, mass, m_scalar,
-                            (const int*)sb.offset, (const int*)sb.sorted, (T*)rho);
+  QJ_TILE_VOL(tg, launch<k_tile_deposit<T, VOL_>>(tg.ntiles, 256, s, make_geom(d), tg, pos, mass,
+                                                  m_scalar, (const int*)sb.offset,
+                                                  (const int*)sb.sorted, (T*)rho));
   return hipGetLastError();
 }
 
@@ -660,8 +673,10 @@ hipError_t launch_interp_adj_binned(const Dims& d, const double* pos, long long 
   if (np > 0)
     launch<k_bin_scatter>(pblocks(np), 256, s, make_geom(d), tg, pos, np, (const int*)sb.rank,
                           (const int*)sb.offset, sb.sorted);
-  launch<k_tile_deposit_vec<T>>(tg.ntiles, 256, s, make_geom(d), tg, pos, (const double*)vbar, h,
-                                (const int*)sb.offset, (const int*)sb.sorted, (T*)abar4);
+  QJ_TILE_VOL(tg, launch<k_tile_deposit_vec<T, VOL_>>(tg.ntiles, 256, s, make_geom(d), tg, pos,
+                                                      (const double*)vbar, h,
+                                                      (const int*)sb.offset,
+                                                      (const int*)sb.sorted, (T*)abar4));
   return hipGetLastError();
 }
 
