@@ -366,9 +366,10 @@ def test_hydro_sound_wave_speed(flux):
     assert abs(shift - cs * dt * nsteps) < 0.02 * cs * dt * nsteps
 
 
-def test_gas_step_conserves_mass():
+@pytest.mark.parametrize("flux", FLUXES)
+def test_gas_step_conserves_mass(flux):
     g = O.Grid(16, 16, 16)
-    ph = O.Physics(rho_bar=1.2, cs=0.5)
+    ph = O.Physics(rho_bar=1.2, cs=0.5, flux=flux)
     psi = O.filtered_gaussian_psi(g, 34, sigma_k=8 * np.pi)
     rho, vx, vy, vz = _gas(g, 35)
     rho = 0.2 * rho
