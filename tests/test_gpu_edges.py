@@ -186,3 +186,27 @@ def test_largest_grid_512_cubed_is_time_reversible(qj, cuda):
     assert dpos.abs().max().item() < 1e-9
     assert rel(st["vel"], vel) < 1e-6
     assert abs(st["psi"].abs().pow(2).double().sum().item() / m0 - 1) < 1e-5
+
+
+def test_gas_256_cubed_hll_cfl_conserves_mass(qj, cuda):
+    """The gas configuration at its full size (256^3 complex64 psi + FP32 gas,
+    HLL flux, CFL 0.4): sum(|psi|^2 + rho_gas) vol is conserved by the KDK
+    step (the flux form conserves gas mass exactly, the drift and kicks
+    conserve |psi|^2), and the CFL steps stay within dt_max."""
+    n = 256
+    g = torch.Generator(device="cuda").manual_seed(32)
+    psi = torch.complex(torch.randn((n, n, n), generator=g, device="cuda"),
+                        torch.randn((n, n, n), generator=g, device="cuda"))
+    psi = (psi / psi.abs().pow(2).mean().sqrt()).to(C64)
+    rho = 0.2 * (1 + 0.1 * torch.randn((n, n, n), generator=g, device="cuda")).clamp_min(0.05)
+    vel = [0.05 * torch.randn((n, n, n), generator=g, device="cuda") for _ in range(3)]
+    rb = float(psi.abs().pow(2).mean().item() + rho.mean().item())
+    sim = qj.KDKSimulation(qj.Plan(n, dtype=C64), qj.Physics(rho_bar=rb, cs=0.1, flux="hll"),
+                           5e-4, psi, gas=[rho.float(), *[v.float() for v in vel]])
+    m0 = psi.abs().pow(2).double().sum().item() + rho.double().sum().item()
+    sim.step_cfl(4, cfl=0.4)
+    st = sim.state()
+    m1 = st["psi"].abs().pow(2).double().sum().item() + st["rho"].double().sum().item()
+    assert abs(m1 / m0 - 1) < 1e-5          # FP32 rounding, about 2e-7 per step
+    assert all(0 < d <= 5e-4 for d in sim.dts) and len(sim.dts) == 4
+    assert torch.isfinite(st["vx"]).all()
