@@ -577,10 +577,14 @@ int initial_grid_t(qj_plan* p, qj_state* st, const qj_params* pr, const Work& w,
 // dslot (the CFL stepper): this step's dt and kick time live in the device
 // step slot; the drift tables are rebuilt from it each step and the owed
 // kick stays on the device (the caller reads it back into st->pending).
+// grid_out (one step only): the step's Poisson solve writes the new grid
+// there and st->grid then points to it (the tape recorder uses this to
+// write each step's grid straight into its tape).
 template <class T>
 int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void* work,
                 hipStream_t s, const double* kick_dt = nullptr, const double* drift_dt = nullptr,
-                const double* dslot = nullptr) {
+                const double* dslot = nullptr, void* grid_out = nullptr) {
+  if (grid_out && nsteps != 1) return fail(QJ_EARG, "kdk_steps_t: grid_out needs one step");
   Work w = carve(p, work);
   SpecTables<T> tb = tables<T>(p);
   if (!drift_dt && !dslot)
@@ -669,9 +673,12 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
                                   pr->rho_bar, w.H, p->partials, st->diag, s),
                "source x");
     }
-    if (int rc = poisson_tail<T>(p, w, grad, pr->four_pi_G, pr->rho_bar, pr->dc_mode, V, A4,
+    void* Vn = grid_out ? grid_out : V;
+    void* A4n = grid_out ? (grad ? plane<T>(grid_out, p, 1) : nullptr) : A4;
+    if (int rc = poisson_tail<T>(p, w, grad, pr->four_pi_G, pr->rho_bar, pr->dc_mode, Vn, A4n,
                                  nullptr, st->diag, s))
       return rc;
+    if (grid_out) st->grid = grid_out;
     // the closing half kicks (psi, gas and stars) are owed to the next step
     if (!dslot) st->pending = 0.5 * dk;
     if (stars) st->pending_star = 0.5 * dt;
@@ -761,17 +768,31 @@ int kdk_record_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, qj_t
   if (!st->grid_valid)
     if (int rc = initial_grid_t<T>(p, st, pr, w, s)) return rc;
   if (grid0) QJ_CHECK(copy_async(grid0, st->grid, grid_bytes(p), s), "tape grid0");
-  for (int k = 0; k < nsteps; ++k) {
+  // each step's Poisson solve writes its grid straight into the tape (the
+  // next step reads it there); the state's own grid gets the last one back
+  void* const own_grid = st->grid;
+  int rc = QJ_OK;
+  // (every exit below restores st->grid: the tape buffers are the caller's)
+  auto copy = [&](void* dst, const void* src, size_t bytes, const char* what) {
+    const hipError_t e = copy_async(dst, src, bytes, s);
+    if (e != hipSuccess) rc = hipfail(e, what);
+  };
+  for (int k = 0; k < nsteps && rc == QJ_OK; ++k) {
     qj_tape_step& t = tape[k];
-    if (t.psi0) QJ_CHECK(copy_async(t.psi0, st->psi, cb, s), "tape psi0");
-    if (st->np > 0) QJ_CHECK(copy_async(t.x0, st->pos, xb, s), "tape x0");
+    if (t.psi0) copy(t.psi0, st->psi, cb, "tape psi0");
+    if (rc == QJ_OK && st->np > 0) copy(t.x0, st->pos, xb, "tape x0");
     // merged steps, as the forward runs them: the closing half kick of step k
     // is owed to (and merged into) the opening kick of step k + 1
-    if (int rc = kdk_steps_t<T>(p, st, pr, 1, work, s)) return rc;
-    QJ_CHECK(copy_async(t.psi2, st->psi, cb, s), "tape psi2");
-    QJ_CHECK(copy_async(t.grid1, st->grid, grid_bytes(p), s), "tape grid1");
-    if (st->np > 0) QJ_CHECK(copy_async(t.x1, st->pos, xb, s), "tape x1");
+    if (rc == QJ_OK) rc = kdk_steps_t<T>(p, st, pr, 1, work, s, nullptr, nullptr, nullptr, t.grid1);
+    if (rc == QJ_OK) copy(t.psi2, st->psi, cb, "tape psi2");
+    if (rc == QJ_OK && st->np > 0) copy(t.x1, st->pos, xb, "tape x1");
   }
+  if (st->grid != own_grid) {
+    const hipError_t e = copy_async(own_grid, st->grid, grid_bytes(p), s);
+    st->grid = own_grid;
+    if (e != hipSuccess && rc == QJ_OK) return hipfail(e, "tape: grid back into the state");
+  }
+  if (rc) return rc;
   return kdk_finalize_t<T>(p, st, pr, s);
 }
 
