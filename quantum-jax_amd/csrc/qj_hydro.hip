@@ -29,17 +29,27 @@ __device__ __forceinline__ T rcp(T x) {
   else return (T)1 / x;
 }
 
+// clip(x, 0, 1) (one med3 in FP32)
+template <class T>
+__device__ __forceinline__ T clip01(T x) {
+  if constexpr (sizeof(T) == 4) return __builtin_amdgcn_fmed3f(x, 0.0f, 1.0f);
+  else return fmax((T)0, fmin((T)1, x));
+}
+
 // Centred gradient (quantum-fluid.py:201-203) and the ratio-form minmod
-// limiter (:352-407) of one variable along one axis.
+// limiter (:352-407) of one variable along one axis.  The reference's
+// denominator is g + eps (g > 0), g - eps (g < 0) or eps (g = 0); g +
+// copysign(eps, g) differs only for g = -0, where the slope lim * g is -0
+// either way.
 template <class T>
 __device__ __forceinline__ T limit(T f, T fm, T fp, T inv_dx) {
   const T eps = (T)1.0e-12;
   T g = (fp - fm) * ((T)0.5 * inv_dx);
-  const T denom = (g > (T)0) ? g + eps : ((g < (T)0) ? g - eps : eps);
+  const T denom = g + copysign(eps, g);
   const T rd = rcp(denom);
-  T lim = fmax((T)0, fmin((T)1, ((f - fm) * inv_dx) * rd));
+  T lim = clip01(((f - fm) * inv_dx) * rd);
   g = lim * g;
-  lim = fmax((T)0, fmin((T)1, (-(f - fp) * inv_dx) * rd));
+  lim = clip01((-(f - fp) * inv_dx) * rd);
   return lim * g;
 }
 
@@ -56,7 +66,8 @@ __device__ __forceinline__ void flux(T rL, T uL, T vL, T wL, T rR, T uR, T vR, T
   F[1] = mu * mu * ir + P;
   F[2] = mu * mv * ir;
   F[3] = mu * mw * ir;
-  const T C = fmax(cs + fabs(uL), cs + fabs(uR));
+  // max(cs + |uL|, cs + |uR|) = cs + max(|uL|, |uR|) exactly (rounding is monotonic)
+  const T C = cs + fmax(fabs(uL), fabs(uR));
   F[0] -= C * (T)0.5 * (rR - rL);
   F[1] -= C * (T)0.5 * (rR * uR - rL * uL);
   F[2] -= C * (T)0.5 * (rR * vR - rL * vL);
