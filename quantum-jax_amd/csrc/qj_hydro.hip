@@ -290,6 +290,18 @@ struct k_hydro_stream {
     }
     states(x0, xc0);
     const T fac = g.dt * g.dx * g.dx;
+    // the -x face flux of plane x0; each later plane takes the +x flux of the
+    // plane before (same thread, same column), so each x face is evaluated once
+    T Fm[4];
+    if (own) {
+      T fl[4], fr[4];
+  #pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        fl[k] = xm.p[k] + (T)0.5 * xm.sx[k] * dx;
+        fr[k] = xc0.p[k] - (T)0.5 * xc0.sx[k] * dx;
+      }
+      face_flux<T, FLUX>(fl[0], fl[1], fl[2], fl[3], fr[0], fr[1], fr[2], fr[3], g.cs, Fm);
+    }
     Fetch pre;
     fetch(x0 + 2, pre);
     // the first put overwrites the slot of plane x0-2, read by the warm-up
@@ -310,8 +322,9 @@ struct k_hydro_stream {
         // quantum-fluid.py:338 get_conserved on the (kicked) state of plane x
         T U[4] = {r0 * g.vol, r0 * raw[x & 3][1][ro] * g.vol, r0 * raw[x & 3][2][ro] * g.vol,
                   r0 * raw[x & 3][3][ro] * g.vol};
-        T Fp[4], Fm[4], fl[4], fr[4];
-        // x faces: L/R from the own column's registers
+        T Fp[4], fl[4], fr[4];
+        // +x face from the own column's registers; the -x face is the +x
+        // face of the plane before
   #pragma unroll
         for (int k = 0; k < 4; ++k) {
           fl[k] = xc0.p[k] + (T)0.5 * xc0.sx[k] * dx;
@@ -320,14 +333,9 @@ struct k_hydro_stream {
         face_flux<T, FLUX>(fl[0], fl[1], fl[2], fl[3], fr[0], fr[1], fr[2], fr[3], g.cs, Fp);
   #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          fl[k] = xm.p[k] + (T)0.5 * xm.sx[k] * dx;
-          fr[k] = xc0.p[k] - (T)0.5 * xc0.sx[k] * dx;
-        }
-        face_flux<T, FLUX>(fl[0], fl[1], fl[2], fl[3], fr[0], fr[1], fr[2], fr[3], g.cs, Fm);
-  #pragma unroll
-        for (int k = 0; k < 4; ++k) {
           U[k] += -fac * Fp[k];
           U[k] += fac * Fm[k];
+          Fm[k] = Fp[k];
         }
         // y and z faces from the state plane (rotated flux arguments,
         // quantum-fluid.py:331-336)
