@@ -242,6 +242,12 @@ typedef struct {
   void* grid1;
   double* x0;
   double* x1;
+  /* optional (stars with the tile-binned deposit): qj_star_scratch_size()
+   * bytes that qj_kdk_record fills with the tile registrations of the
+   * positions x1 and qj_kdk_adjoint reuses instead of recounting them;
+   * NULL = recount.  Keep QJ_OPT_BINNED_DEPOSIT unchanged between the two
+   * calls: with it off the record leaves bins unwritten. */
+  void* bins;
 } qj_tape_step;
 
 typedef struct {

@@ -50,7 +50,8 @@ class QjParams(C.Structure):
 
 class QjTapeStep(C.Structure):
     """Mirror of ``qj_tape_step``."""
-    _fields_ = [("psi0", vp), ("psi2", vp), ("grid1", vp), ("x0", vp), ("x1", vp)]
+    _fields_ = [("psi0", vp), ("psi2", vp), ("grid1", vp), ("x0", vp), ("x1", vp),
+                ("bins", vp)]
 
 
 class QjAdjoint(C.Structure):

@@ -73,6 +73,9 @@ class _Tape:
         self.grid0 = torch.empty(5 * N, dtype=plan.real_dtype, device=dev)
         self.x0 = torch.empty((nsteps, max(n_p, 1), 3), dtype=torch.float64, device=dev)
         self.x1 = torch.empty_like(self.x0)
+        # tile registrations of x1 per step (reused by the adjoint's deposit)
+        nb = int(_lib.lib().qj_star_scratch_size(plan.handle, n_p)) if n_p else 0
+        self.bins = torch.empty((nsteps, max(nb, 1)), dtype=torch.uint8, device=dev)
         self.steps = (_lib.QjTapeStep * nsteps)()
         for k in range(nsteps):
             t = self.steps[k]
@@ -81,6 +84,7 @@ class _Tape:
             t.grid1 = self.grid1[k].data_ptr()
             t.x0 = self.x0[k].data_ptr() if n_p else None
             t.x1 = self.x1[k].data_ptr() if n_p else None
+            t.bins = self.bins[k].data_ptr() if n_p else None
 
 
 def _adjoint_segment(plan: Plan, phys: Physics, dt: float, ck, mass, nst: int, tape: _Tape,

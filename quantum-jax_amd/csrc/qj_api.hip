@@ -786,6 +786,9 @@ int kdk_record_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, qj_t
     if (rc == QJ_OK) rc = kdk_steps_t<T>(p, st, pr, 1, work, s, nullptr, nullptr, nullptr, t.grid1);
     if (rc == QJ_OK) copy(t.psi2, st->psi, cb, "tape psi2");
     if (rc == QJ_OK && st->np > 0) copy(t.x1, st->pos, xb, "tape x1");
+    // the tile registrations the step's binned deposit made of x1
+    if (rc == QJ_OK && t.bins && st->np > 0 && st->star_scratch && p->opt[QJ_OPT_BINNED_DEPOSIT])
+      copy(t.bins, st->star_scratch, star_scratch_bytes(p->d, st->np), "tape bins");
   }
   if (st->grid != own_grid) {
     const hipError_t e = copy_async(own_grid, st->grid, grid_bytes(p), s);
@@ -814,15 +817,17 @@ int kdk_adjoint_t(qj_plan* p, const qj_state* st, const qj_params* pr, int nstep
   // (A4, pos) with kick time hh (phase coefficient cc), the density and
   // Poisson adjoints, the deposit adjoint
   auto half = [&](const void* grid, const void* psi_ref, const double* pos, double dt_xv,
-                  double hh, double cc) -> int {
+                  double hh, double cc, const void* bins) -> int {
     const void* V = grid;
     const void* A4 = (const char*)grid + (size_t)n * sizeof(T);
     if (stars && binned) {
-      // the star kicks' adjoint deposited tile-binned (writes every cell of abar4)
-      StarBins sb = carve_star_scratch(p->d, st->np, st->star_scratch);
+      // the star kicks' adjoint deposited tile-binned (writes every cell of
+      // abar4), with the forward's taped registrations of pos when given
+      StarBins sb = carve_star_scratch(p->d, st->np, bins ? const_cast<void*>(bins)
+                                                          : st->star_scratch);
       QJ_STAGE(kStageAdjParticles,
                launch_interp_adj_binned<T>(p->d, pos, st->np, A4, hh, a->v_bar, dt_xv, a->x_bar,
-                                           abar4, sb, s),
+                                           abar4, sb, bins != nullptr, s),
                "interp adjoint");
     } else {
       QJ_CHECK(fill<T>(abar4, 4 * n, (T)0, s), "abar clear");
@@ -857,7 +862,7 @@ int kdk_adjoint_t(qj_plan* p, const qj_state* st, const qj_params* pr, int nstep
     const qj_tape_step& t = tape[k];
     // closing half step of the last step
     if (k == nsteps - 1)
-      if (int rc = half(t.grid1, t.psi2, t.x1, 0.0, h, c)) return rc;
+      if (int rc = half(t.grid1, t.psi2, t.x1, 0.0, h, c, t.bins)) return rc;
     // drift: psi_bar <- F^H psi_bar (conjugate phase tables = drift of -dt);
     // pos1 = pos0 + dt vel1 is folded into the next interp adjoint (dt_xv)
     for (int ax = 0; ax < 3; ++ax)
@@ -866,9 +871,9 @@ int kdk_adjoint_t(qj_plan* p, const qj_state* st, const qj_params* pr, int nstep
                "drift adjoint");
     if (k > 0) {  // opening half of step k with the closing half of step k-1
       const qj_tape_step& u = tape[k - 1];
-      if (int rc = half(u.grid1, u.psi2, u.x1, dt, 2.0 * h, 2.0 * c)) return rc;
+      if (int rc = half(u.grid1, u.psi2, u.x1, dt, 2.0 * h, 2.0 * c, u.bins)) return rc;
     } else {  // opening half step of the first step (grid0 at x0, psi0)
-      if (int rc = half(grid0, t.psi0, t.x0, dt, h, c)) return rc;
+      if (int rc = half(grid0, t.psi0, t.x0, dt, h, c, nullptr)) return rc;
     }
   }
   return QJ_OK;

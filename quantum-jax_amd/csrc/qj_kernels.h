@@ -184,11 +184,13 @@ hipError_t launch_tile_deposit(const Dims& d, const double* pos, const double* m
                                hipStream_t s);
 
 // Interpolation adjoint whose h vbar corner values go to abar4 through the
-// tile-binned deposit (whole tiles written, no clearing needed).
+// tile-binned deposit (whole tiles written, no clearing needed).  taped: sb
+// already holds the registrations of pos (offsets and sorted), so they are
+// not counted again.
 template <class T>
 hipError_t launch_interp_adj_binned(const Dims& d, const double* pos, long long np,
                                     const void* A4, double h, double* vbar, double dt_xv,
-                                    double* xbar, void* abar4, const StarBins& sb,
+                                    double* xbar, void* abar4, const StarBins& sb, bool taped,
                                     hipStream_t s);
 
 // Accretion onto black hole 0 (quantum-bh.py:306-339), in place on psi and m_bh.

@@ -214,10 +214,12 @@ struct k_interp_adj {
     Cic c = cic(g, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]);
     if (!c.ok) return;
     const double ab[3] = {h * vb[0], h * vb[1], h * vb[2]};
-    if constexpr (BIN) {
-      int ids[8];
-      const int n = stencil_tiles(tg, c, ids);
-      for (int k = 0; k < n; ++k) rank[8 * p + k] = atomicAdd(&count[ids[k]], 1);
+    if constexpr (BIN) {  // count == nullptr: registrations taped by the forward
+      if (count) {
+        int ids[8];
+        const int n = stencil_tiles(tg, c, ids);
+        for (int k = 0; k < n; ++k) rank[8 * p + k] = atomicAdd(&count[ids[k]], 1);
+      }
     }
   #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -663,16 +665,19 @@ hipError_t launch_bh_accretion(const Dims& d, void* psi, const double* pos, cons
 template <class T>
 hipError_t launch_interp_adj_binned(const Dims& d, const double* pos, long long np,
                                     const void* A4, double h, double* vbar, double dt_xv,
-                                    double* xbar, void* abar4, const StarBins& sb,
+                                    double* xbar, void* abar4, const StarBins& sb, bool taped,
                                     hipStream_t s) {
   const TileGeom tg = make_tiles(d);
   if (np > 0)
     launch<k_interp_adj<T, true>>(pblocks(np), 256, s, make_geom(d), pos, np, (const T*)A4, h,
-                                  vbar, dt_xv, xbar, (T*)nullptr, tg, sb.count, sb.rank);
-  launch<k_bin_scan>(1, 1024, s, sb.count, sb.offset, tg.ntiles);
-  if (np > 0)
-    launch<k_bin_scatter>(pblocks(np), 256, s, make_geom(d), tg, pos, np, (const int*)sb.rank,
-                          (const int*)sb.offset, sb.sorted);
+                                  vbar, dt_xv, xbar, (T*)nullptr, tg,
+                                  taped ? (int*)nullptr : sb.count, sb.rank);
+  if (!taped) {  // else offset and sorted hold the forward's registrations of pos
+    launch<k_bin_scan>(1, 1024, s, sb.count, sb.offset, tg.ntiles);
+    if (np > 0)
+      launch<k_bin_scatter>(pblocks(np), 256, s, make_geom(d), tg, pos, np,
+                            (const int*)sb.rank, (const int*)sb.offset, sb.sorted);
+  }
   QJ_TILE_VOL(tg, launch<k_tile_deposit_vec<T, VOL_>>(tg.ntiles, 256, s, make_geom(d), tg, pos,
                                                       (const double*)vbar, h,
                                                       (const int*)sb.offset,
@@ -703,7 +708,7 @@ hipError_t launch_interp_adj_binned(const Dims& d, const double* pos, long long 
                                              long long, const StarBins&, void*, hipStream_t);     \
   template hipError_t launch_interp_adj_binned<T>(const Dims&, const double*, long long,           \
                                                   const void*, double, double*, double, double*,   \
-                                                  void*, const StarBins&, hipStream_t);            \
+                                                  void*, const StarBins&, bool, hipStream_t);      \
   template hipError_t launch_bh_accretion<T>(const Dims&, void*, const double*, const double*,    \
                                              double*, long long, double, double, double, double,  \
                                              double, hipStream_t);
