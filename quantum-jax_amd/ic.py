@@ -10,15 +10,17 @@ clipped below at 0.01; v = 0.
 These seeded constructions define the workloads that the tests and the
 benchmark share.  The reference's own ICs are below them: random phases
 sorted by |k| (quantum-fluid.py:509-522) and the Gaussian columns with
-uniform stars (quantum-stars.py:376-428).  Its HDF5 IC reader
-(quantum-cosmo.py:242-251) is not provided: it needs h5py and a data file
-that are not available here.
+uniform stars (quantum-stars.py:376-428).  The cosmological variant's HDF5
+initial condition (quantum-cosmo.py:242-244) is ``hdf5_psi``, read by the
+package's own HDF5 reader (``hdf5.py``).
 """
 
 from __future__ import annotations
 
 import numpy as np
 import scipy.fft as sfft
+
+from .hdf5 import load_psi as hdf5_psi  # noqa: F401
 
 
 def kvec(n: int, L: float) -> np.ndarray:
