@@ -73,10 +73,10 @@ def stage_bytes(name, nx, ny, nz, c, r, n_p, stars):
         "fft_y_half": 2 * Nh * c,
         "poisson_z": 3 * Nh * c,
         "poisson_y": 5 * Nh * c,
-        "poisson_x": 3 * Nh * c + 5 * N * r,      # V plane + (a_x, a_y, a_z, V) cells
+        "poisson_x": 3 * Nh * c + 4 * N * r,      # V plane + (a_x, a_y, a_z) cells
         "deposit": N * r + n_p * 64,
-        "star_step": n_p * (96 + 8 * 4 * r + 8),
-        "star_kick": n_p * (72 + 8 * 4 * r),
+        "star_step": n_p * (96 + 8 * 3 * r + 8),
+        "star_kick": n_p * (72 + 8 * 3 * r),
         "kick": N * (2 * c + r),
     }
     return table.get(name)

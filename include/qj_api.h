@@ -67,12 +67,13 @@ int qj_get_potential(qj_plan* plan, const void* rho, double rho_bar, double four
                      double b_coeff, int dc_mode, void* V, void* work, void* stream);
 
 /* a_d = -Re IFFT(1j k_d V_hat) on the grid and V; quantum-stars.py:254-260,
- * quantum-fluid.py:268-272.  a4 is (nx, ny, nz, 4) reals holding
- * (a_x, a_y, a_z, V) per cell (the reference's a_grid, stacked on the last
- * axis, plus V); V (nullable) is the plain (nx, ny, nz) potential.  max |a|
- * is folded into diag[QJ_DIAG_AMAX] (diag nullable). */
+ * quantum-fluid.py:268-272.  a_grid is (nx, ny, nz, 3) reals holding
+ * (a_x, a_y, a_z) per cell (the reference's a_grid, stacked on the last
+ * axis); V (nullable) is the (nx, ny, nz) potential.  max |a| is folded
+ * into diag[QJ_DIAG_AMAX] (diag nullable). */
 int qj_acceleration_grid(qj_plan* plan, const void* rho, double rho_bar, double four_pi_G,
-                         int dc_mode, void* V, void* a4, double* diag, void* work, void* stream);
+                         int dc_mode, void* V, void* a_grid, double* diag, void* work,
+                         void* stream);
 
 /* psi <- exp(-1j coef V) psi; coef = m_per_hbar*dt/2. quantum-stars.py:301,318 */
 int qj_kick(qj_plan* plan, void* psi, const void* V, double coef, void* stream);
@@ -91,22 +92,22 @@ int qj_cic_indices(qj_plan* plan, const double* pos, long long np, int* i, int* 
 int qj_bin_particles(qj_plan* plan, const double* pos, const double* mass, double m_scalar,
                      long long np, void* rho, int accumulate, void* stream);
 
-/* 8-corner CIC interpolation of the (a_x, a_y, a_z) components of a4 to the
+/* 8-corner CIC interpolation of a_grid ((nx, ny, nz, 3) reals) to the
  * particles; quantum-stars.py:262-287.  acc is double (np, 3). */
-int qj_interp(qj_plan* plan, const double* pos, long long np, const void* a4, double* acc,
+int qj_interp(qj_plan* plan, const double* pos, long long np, const void* a_grid, double* acc,
               void* stream);
 
 /* get_acceleration(pos, rho) -> (acc, a_max); quantum-stars.py:250-288.
- * a4 ((nx,ny,nz,4) reals) receives the grid acceleration and V. */
+ * a_grid ((nx,ny,nz,3) reals) receives the grid acceleration. */
 int qj_get_acceleration(qj_plan* plan, const double* pos, long long np, const void* rho,
-                        double rho_bar, double four_pi_G, int dc_mode, double* acc, void* a4,
+                        double rho_bar, double four_pi_G, int dc_mode, double* acc, void* a_grid,
                         double* diag, void* work, void* stream);
 
 /* apply_grav_accel(rho, vx, vy, vz, dt); quantum-fluid.py:267-278 (in place
- * on vx, vy, vz; a4 ((nx,ny,nz,4) reals of scratch) receives the grid
+ * on vx, vy, vz; a_grid ((nx,ny,nz,3) reals of scratch) receives the grid
  * acceleration). */
 int qj_apply_grav_accel(qj_plan* plan, const void* rho, void* vx, void* vy, void* vz, double dt,
-                        double rho_bar, double four_pi_G, int dc_mode, void* a4, double* diag,
+                        double rho_bar, double four_pi_G, int dc_mode, void* a_grid, double* diag,
                         void* work, void* stream);
 
 /* Gas face fluxes.  QJ_FLUX_RUSANOV is the reference's (the flux of the
@@ -144,7 +145,7 @@ typedef struct {
   const double* mass; /* (np) or NULL                                         */
   double m_scalar;  /* star mass when mass == NULL                            */
   long long np;
-  void* grid;       /* V (N reals) then A4 (N x 4 reals: a_x, a_y, a_z, V)     */
+  void* grid;       /* V (N reals) then A3 (N x 3 reals: a_x, a_y, a_z)        */
   void* brho;       /* real N, b*rho for self-interaction; NULL when b == 0   */
   double* diag;     /* QJ_DIAG_COUNT doubles                                  */
   /* host-side bookkeeping, initialise to 0 */
@@ -226,7 +227,7 @@ int qj_kdk_finalize(qj_plan* plan, qj_state* state, const qj_params* prm, void* 
  * must be finalized; the steps merge consecutive half kicks as qj_kdk_steps
  * does, and the state is finalized at the end) and stores, per step, psi
  * after the drift (psi2), the
- * grid after the step's Poisson solve (grid1, 5N reals as in qj_state), the
+ * grid after the step's Poisson solve (grid1, 4N reals as in qj_state), the
  * star positions before and after the drift (x0, x1), and psi at the start
  * (psi0: needed for step 0 only, NULL allowed for later steps); grid0
  * (nullable) receives the grid at the start.  The adjoint merges the

@@ -16,6 +16,9 @@ LIB_PATH = os.path.join(HERE, "libqjamd.so")
 QJ_C64 = 0
 QJ_C128 = 1
 DIAG_AMAX, DIAG_DTS, DIAG_MASS, DIAG_NAN, DIAG_COUNT = 0, 1, 2, 3, 8
+# reals per cell of the acceleration grid (a_x, a_y, a_z); the state's grid
+# holds V (one real per cell) and then this grid
+ACC_COMPONENTS = 3
 FLUX_RUSANOV, FLUX_HLL = 0, 1
 FLUXES = {"rusanov": FLUX_RUSANOV, "hll": FLUX_HLL}
 OPT_FUSE_X_SOURCE = 0

@@ -69,8 +69,9 @@ class _Tape:
         # each closing half kick with the next opening one)
         self.psi0 = torch.empty((1,) + plan.shape, dtype=plan.complex_dtype, device=dev)
         self.psi2 = torch.empty((nsteps,) + plan.shape, dtype=plan.complex_dtype, device=dev)
-        self.grid1 = torch.empty((nsteps, 5 * N), dtype=plan.real_dtype, device=dev)
-        self.grid0 = torch.empty(5 * N, dtype=plan.real_dtype, device=dev)
+        G = (1 + _lib.ACC_COMPONENTS) * N  # V, then the acceleration grid
+        self.grid1 = torch.empty((nsteps, G), dtype=plan.real_dtype, device=dev)
+        self.grid0 = torch.empty(G, dtype=plan.real_dtype, device=dev)
         self.x0 = torch.empty((nsteps, max(n_p, 1), 3), dtype=torch.float64, device=dev)
         self.x1 = torch.empty_like(self.x0)
         # tile registrations of x1 per step (reused by the adjoint's deposit)

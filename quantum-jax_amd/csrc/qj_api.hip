@@ -229,7 +229,7 @@ int ensure_drift(qj_plan* p, double dt, double mph, hipStream_t s) {
 // Poisson chain after the source pass: H -> grid (V [, a]).
 template <class T>
 int poisson_tail(qj_plan* p, const Work& w, int grad, double four_pi_G, double rho_bar,
-                 int dc_mode, void* V, void* A4, const void* addend,
+                 int dc_mode, void* V, void* A3, const void* addend,
                  double* diag, hipStream_t s) {
   SpecTables<T> tb = tables<T>(p);
   QJ_STAGE(kStageFftY, launch_fft_y_half<T>(p->d, tb, w.H, s), "fft_y");
@@ -237,12 +237,12 @@ int poisson_tail(qj_plan* p, const Work& w, int grad, double four_pi_G, double r
                                                   p->partials, diag, s),
            "poisson_z");
   QJ_STAGE(kStagePoissonY, launch_poisson_y<T>(p->d, tb, grad, w.H, w.Uz, w.Uy, s), "poisson_y");
-  QJ_STAGE(kStagePoissonX, launch_poisson_x<T>(p->d, tb, grad, w.H, w.Uy, w.Uz, V, A4, addend, diag, s),
+  QJ_STAGE(kStagePoissonX, launch_poisson_x<T>(p->d, tb, grad, w.H, w.Uy, w.Uz, V, A3, addend, diag, s),
            "poisson_x");
   return QJ_OK;
 }
 
-// Real plane k of a buffer of N-real planes (the grid is V then A4).
+// Real plane k of a buffer of N-real planes (the grid is V then A3).
 template <class T>
 void* plane(void* base, const qj_plan* p, int k) {
   return (char*)base + (size_t)k * p->ncell() * sizeof(T);
@@ -265,7 +265,7 @@ int get_potential_t(qj_plan* p, const void* rho, double rho_bar, double four_pi_
 
 template <class T>
 int accel_grid_t(qj_plan* p, const void* rho, double rho_bar, double four_pi_G, int dc_mode,
-                 void* V, void* a4, double* diag, void* work, hipStream_t s) {
+                 void* V, void* a_grid, double* diag, void* work, hipStream_t s) {
   Work w = carve(p, work);
   SpecTables<T> tb = tables<T>(p);
   // the source mass and max|a| land in the caller's diag, else in plan scratch
@@ -274,7 +274,7 @@ int accel_grid_t(qj_plan* p, const void* rho, double rho_bar, double four_pi_G, 
                               p->partials, dg, s),
            "source_x");
   void* Vout = V ? V : w.R;  // the pass always writes V
-  return poisson_tail<T>(p, w, 1, four_pi_G, rho_bar, dc_mode, Vout, a4, nullptr, dg, s);
+  return poisson_tail<T>(p, w, 1, four_pi_G, rho_bar, dc_mode, Vout, a_grid, nullptr, dg, s);
 }
 
 }  // namespace
@@ -405,12 +405,12 @@ int qj_get_potential(qj_plan* p, const void* rho, double rho_bar, double four_pi
 }
 
 int qj_acceleration_grid(qj_plan* p, const void* rho, double rho_bar, double four_pi_G,
-                         int dc_mode, void* V, void* a4, double* diag, void* work, void* stream) {
+                         int dc_mode, void* V, void* a_grid, double* diag, void* work, void* stream) {
   if (int rc = check_plan(p)) return rc;
-  if (!rho || !a4 || !work) return fail(QJ_EARG, "qj_acceleration_grid: null buffer");
+  if (!rho || !a_grid || !work) return fail(QJ_EARG, "qj_acceleration_grid: null buffer");
   hipStream_t s = (hipStream_t)stream;
-  return QJ_PRECISION(p, accel_grid_t<float>(p, rho, rho_bar, four_pi_G, dc_mode, V, a4, diag, work, s),
-                      accel_grid_t<double>(p, rho, rho_bar, four_pi_G, dc_mode, V, a4, diag, work, s));
+  return QJ_PRECISION(p, accel_grid_t<float>(p, rho, rho_bar, four_pi_G, dc_mode, V, a_grid, diag, work, s),
+                      accel_grid_t<double>(p, rho, rho_bar, four_pi_G, dc_mode, V, a_grid, diag, work, s));
 }
 
 int qj_kick(qj_plan* p, void* psi, const void* V, double coef, void* stream) {
@@ -464,44 +464,44 @@ int qj_bin_particles(qj_plan* p, const double* pos, const double* mass, double m
   return e == hipSuccess ? QJ_OK : hipfail(e, "qj_bin_particles");
 }
 
-int qj_interp(qj_plan* p, const double* pos, long long np, const void* a4, double* acc,
+int qj_interp(qj_plan* p, const double* pos, long long np, const void* a_grid, double* acc,
               void* stream) {
   if (int rc = check_plan(p)) return rc;
-  if (np < 0 || (np > 0 && (!pos || !a4 || !acc))) return fail(QJ_EARG, "qj_interp: bad arguments");
+  if (np < 0 || (np > 0 && (!pos || !a_grid || !acc))) return fail(QJ_EARG, "qj_interp: bad arguments");
   hipStream_t s = (hipStream_t)stream;
   hipError_t e;
   if (p->dtype == QJ_C64)
-    e = launch_interp<float>(p->d, pos, np, a4, acc, s);
+    e = launch_interp<float>(p->d, pos, np, a_grid, acc, s);
   else
-    e = launch_interp<double>(p->d, pos, np, a4, acc, s);
+    e = launch_interp<double>(p->d, pos, np, a_grid, acc, s);
   return e == hipSuccess ? QJ_OK : hipfail(e, "qj_interp");
 }
 
 int qj_get_acceleration(qj_plan* p, const double* pos, long long np, const void* rho,
-                        double rho_bar, double four_pi_G, int dc_mode, double* acc, void* a4,
+                        double rho_bar, double four_pi_G, int dc_mode, double* acc, void* a_grid,
                         double* diag, void* work, void* stream) {
   if (int rc = check_plan(p)) return rc;
-  if (!rho || !a4 || !work || np < 0 || (np > 0 && (!pos || !acc)))
+  if (!rho || !a_grid || !work || np < 0 || (np > 0 && (!pos || !acc)))
     return fail(QJ_EARG, "qj_get_acceleration: bad arguments");
-  int rc = qj_acceleration_grid(p, rho, rho_bar, four_pi_G, dc_mode, nullptr, a4, diag, work, stream);
+  int rc = qj_acceleration_grid(p, rho, rho_bar, four_pi_G, dc_mode, nullptr, a_grid, diag, work, stream);
   if (rc) return rc;
-  return qj_interp(p, pos, np, a4, acc, stream);
+  return qj_interp(p, pos, np, a_grid, acc, stream);
 }
 
 int qj_apply_grav_accel(qj_plan* p, const void* rho, void* vx, void* vy, void* vz, double dt,
-                        double rho_bar, double four_pi_G, int dc_mode, void* a4, double* diag,
+                        double rho_bar, double four_pi_G, int dc_mode, void* a_grid, double* diag,
                         void* work, void* stream) {
   if (int rc = check_plan(p)) return rc;
-  if (!rho || !vx || !vy || !vz || !a4 || !work)
+  if (!rho || !vx || !vy || !vz || !a_grid || !work)
     return fail(QJ_EARG, "qj_apply_grav_accel: null buffer");
-  int rc = qj_acceleration_grid(p, rho, rho_bar, four_pi_G, dc_mode, nullptr, a4, diag, work, stream);
+  int rc = qj_acceleration_grid(p, rho, rho_bar, four_pi_G, dc_mode, nullptr, a_grid, diag, work, stream);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   hipError_t e;
   if (p->dtype == QJ_C64)
-    e = launch_gas_kick<float>(p->ncell(), vx, vy, vz, a4, dt, s);
+    e = launch_gas_kick<float>(p->ncell(), vx, vy, vz, a_grid, dt, s);
   else
-    e = launch_gas_kick<double>(p->ncell(), vx, vy, vz, a4, dt, s);
+    e = launch_gas_kick<double>(p->ncell(), vx, vy, vz, a_grid, dt, s);
   return e == hipSuccess ? QJ_OK : hipfail(e, "qj_apply_grav_accel");
 }
 
@@ -532,13 +532,13 @@ int qj_solve_hydro(qj_plan* p, const void* rho, const void* vx, const void* vy, 
 // ===========================================================================
 namespace {
 
-// V and A4 of the current state (star deposit, source pass, Poisson chain).
+// V and A3 of the current state (star deposit, source pass, Poisson chain).
 template <class T>
 int initial_grid_t(qj_plan* p, qj_state* st, const qj_params* pr, const Work& w, hipStream_t s) {
   SpecTables<T> tb = tables<T>(p);
   const bool stars = st->np > 0;
   const bool gas = st->gas[0] != nullptr;
-  void* A4 = (stars || gas) ? plane<T>(st->grid, p, 1) : nullptr;
+  void* A3 = (stars || gas) ? plane<T>(st->grid, p, 1) : nullptr;
   void* brho = (pr->b_coeff != 0.0) ? st->brho : nullptr;
   const bool binned = st->star_scratch && p->opt[QJ_OPT_BINNED_DEPOSIT];
   if (stars && binned) {
@@ -568,7 +568,7 @@ int initial_grid_t(qj_plan* p, qj_state* st, const qj_params* pr, const Work& w,
                               pr->rho_bar, w.H, p->partials, st->diag, s),
            "source_x");
   if (int rc = poisson_tail<T>(p, w, (stars || gas) ? 1 : 0, pr->four_pi_G, pr->rho_bar,
-                               pr->dc_mode, st->grid, A4, nullptr, st->diag, s))
+                               pr->dc_mode, st->grid, A3, nullptr, st->diag, s))
     return rc;
   st->grid_valid = 1;
   return QJ_OK;
@@ -593,7 +593,7 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
   const bool gas = st->gas[0] != nullptr;
   const bool grad = stars || gas;
   void* V = st->grid;
-  void* A4 = grad ? plane<T>(st->grid, p, 1) : nullptr;
+  void* A3 = grad ? plane<T>(st->grid, p, 1) : nullptr;
   void* brho = (pr->b_coeff != 0.0) ? st->brho : nullptr;
   const double dt = pr->dt;
   auto live_gas = [&](int k) { return st->gas_swapped ? st->gas_tmp[k] : st->gas[k]; };
@@ -626,7 +626,7 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
     if (stars && binned) {
       StarBins sb = carve_star_scratch(p->d, st->np, st->star_scratch);
       QJ_STAGE(kStageStarStep,
-               launch_star_step_binned<T>(p->d, st->pos, st->vel, st->np, A4, st->pending_star,
+               launch_star_step_binned<T>(p->d, st->pos, st->vel, st->np, A3, st->pending_star,
                                           0.5 * dt, dt, 1, sb, s),
                "star step");
       QJ_STAGE(kStageDeposit,
@@ -637,7 +637,7 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
       // the float-atomic deposit accumulates into a cleared plane
       QJ_CHECK(fill<T>(st->rho_s, p->ncell(), (T)0, s), "rho_s clear");
       QJ_STAGE(kStageStarStep,
-               launch_star_step<T>(p->d, st->pos, st->vel, st->np, A4, st->pending_star, 0.5 * dt,
+               launch_star_step<T>(p->d, st->pos, st->vel, st->np, A3, st->pending_star, 0.5 * dt,
                                    dt, st->mass, st->m_scalar, st->rho_s, s),
                "star step");
     }
@@ -646,7 +646,7 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
     if (gas) {
       const double dd = drift_dt ? drift_dt[step] : dt;
       QJ_STAGE(kStageHydro,
-               launch_hydro<T>(p->d, live_gas(0), live_gas(1), live_gas(2), live_gas(3), A4, kick_t,
+               launch_hydro<T>(p->d, live_gas(0), live_gas(1), live_gas(2), live_gas(3), A3, kick_t,
                                dd, pr->cs, pr->flux, spare_gas(0), spare_gas(1), spare_gas(2),
                                spare_gas(3), s, dslot),
                "hydro");
@@ -674,8 +674,8 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
                "source x");
     }
     void* Vn = grid_out ? grid_out : V;
-    void* A4n = grid_out ? (grad ? plane<T>(grid_out, p, 1) : nullptr) : A4;
-    if (int rc = poisson_tail<T>(p, w, grad, pr->four_pi_G, pr->rho_bar, pr->dc_mode, Vn, A4n,
+    void* A3n = grid_out ? (grad ? plane<T>(grid_out, p, 1) : nullptr) : A3;
+    if (int rc = poisson_tail<T>(p, w, grad, pr->four_pi_G, pr->rho_bar, pr->dc_mode, Vn, A3n,
                                  nullptr, st->diag, s))
       return rc;
     if (grid_out) st->grid = grid_out;
@@ -704,8 +704,8 @@ int kdk_steps_cfl_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, d
     for (int k = 0; k < m; ++k) {
       void* const* g = st->gas_swapped ? st->gas_tmp : st->gas;
       // the owed kick is zero on a state whose grid is not computed yet
-      const void* A4 = st->grid_valid ? plane<T>(st->grid, p, 1) : nullptr;
-      QJ_CHECK(launch_gas_cfl<T>(p->ncell(), g[1], g[2], g[3], A4, 0.0, pr->cs, dx, cfl, dt_max,
+      const void* A3 = st->grid_valid ? plane<T>(st->grid, p, 1) : nullptr;
+      QJ_CHECK(launch_gas_cfl<T>(p->ncell(), g[1], g[2], g[3], A3, 0.0, pr->cs, dx, cfl, dt_max,
                                  slot, dts_dev, k, p->partials, p->nblk_src, s),
                "cfl");
       if (int rc = kdk_steps_t<T>(p, st, pr, 1, work, s, nullptr, nullptr, slot)) return rc;
@@ -729,7 +729,7 @@ int kdk_steps_cfl_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, d
 template <class T>
 int kdk_finalize_t(qj_plan* p, qj_state* st, const qj_params* pr, hipStream_t s) {
   void* V = st->grid;
-  void* A4 = plane<T>(st->grid, p, 1);
+  void* A3 = plane<T>(st->grid, p, 1);
   if (st->pending != 0.0) {
     void* brho = (pr->b_coeff != 0.0) ? st->brho : nullptr;
     QJ_STAGE(kStageKick,
@@ -738,7 +738,7 @@ int kdk_finalize_t(qj_plan* p, qj_state* st, const qj_params* pr, hipStream_t s)
       void* g1 = st->gas_swapped ? st->gas_tmp[1] : st->gas[1];
       void* g2 = st->gas_swapped ? st->gas_tmp[2] : st->gas[2];
       void* g3 = st->gas_swapped ? st->gas_tmp[3] : st->gas[3];
-      QJ_STAGE(kStageGasKick, launch_gas_kick<T>(p->ncell(), g1, g2, g3, A4, st->pending, s),
+      QJ_STAGE(kStageGasKick, launch_gas_kick<T>(p->ncell(), g1, g2, g3, A3, st->pending, s),
                "gas kick");
     }
     st->pending = 0.0;
@@ -746,7 +746,7 @@ int kdk_finalize_t(qj_plan* p, qj_state* st, const qj_params* pr, hipStream_t s)
   if (st->pending_star != 0.0 && st->np > 0) {
     QJ_CHECK(fill<double>(st->diag + kDiagDtS, 1, __builtin_inf(), s), "dt_s reset");
     QJ_STAGE(kStageStarKick,
-             launch_star_kick<T>(p->d, st->pos, st->vel, st->np, A4, st->pending_star, st->diag, s),
+             launch_star_kick<T>(p->d, st->pos, st->vel, st->np, A3, st->pending_star, st->diag, s),
              "star kick");
     st->pending_star = 0.0;
   }
@@ -756,7 +756,7 @@ int kdk_finalize_t(qj_plan* p, qj_state* st, const qj_params* pr, hipStream_t s)
 // ---------------------------------------------------------------------------
 // recording and adjoint of the (unmerged) KDK step, quantum-stars.py:295-330
 // ---------------------------------------------------------------------------
-size_t grid_bytes(const qj_plan* p) { return 5 * (size_t)p->ncell() * p->rsize; }
+size_t grid_bytes(const qj_plan* p) { return (1 + kAccComp) * (size_t)p->ncell() * p->rsize; }
 
 template <class T>
 int kdk_record_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, qj_tape_step* tape,
@@ -810,37 +810,37 @@ int kdk_adjoint_t(qj_plan* p, const qj_state* st, const qj_params* pr, int nstep
   const double dt = pr->dt, h = 0.5 * dt, c = pr->m_per_hbar * h;
   const bool stars = st->np > 0;
   T* Vbar = (T*)a->scratch;
-  T* abar4 = Vbar + n;
-  T* rhobar = abar4 + 4 * n;
+  T* abar3 = Vbar + n;
+  T* rhobar = abar3 + kAccComp * n;
   const bool binned = st->star_scratch && p->opt[QJ_OPT_BINNED_DEPOSIT];
   // one adjoint half step: the kick of psi (V, psi_ref) and of the stars
-  // (A4, pos) with kick time hh (phase coefficient cc), the density and
+  // (A3, pos) with kick time hh (phase coefficient cc), the density and
   // Poisson adjoints, the deposit adjoint
   auto half = [&](const void* grid, const void* psi_ref, const double* pos, double dt_xv,
                   double hh, double cc, const void* bins) -> int {
     const void* V = grid;
-    const void* A4 = (const char*)grid + (size_t)n * sizeof(T);
+    const void* A3 = (const char*)grid + (size_t)n * sizeof(T);
     if (stars && binned) {
       // the star kicks' adjoint deposited tile-binned (writes every cell of
-      // abar4), with the forward's taped registrations of pos when given
+      // abar3), with the forward's taped registrations of pos when given
       StarBins sb = carve_star_scratch(p->d, st->np, bins ? const_cast<void*>(bins)
                                                           : st->star_scratch);
       QJ_STAGE(kStageAdjParticles,
-               launch_interp_adj_binned<T>(p->d, pos, st->np, A4, hh, a->v_bar, dt_xv, a->x_bar,
-                                           abar4, sb, bins != nullptr, s),
+               launch_interp_adj_binned<T>(p->d, pos, st->np, A3, hh, a->v_bar, dt_xv, a->x_bar,
+                                           abar3, sb, bins != nullptr, s),
                "interp adjoint");
     } else {
-      QJ_CHECK(fill<T>(abar4, 4 * n, (T)0, s), "abar clear");
+      QJ_CHECK(fill<T>(abar3, kAccComp * n, (T)0, s), "abar clear");
       if (stars)
         QJ_STAGE(kStageAdjParticles,
-                 launch_interp_adj<T>(p->d, pos, st->np, A4, hh, a->v_bar, dt_xv, a->x_bar, abar4,
+                 launch_interp_adj<T>(p->d, pos, st->np, A3, hh, a->v_bar, dt_xv, a->x_bar, abar3,
                                       s),
                  "interp adjoint");
     }
     QJ_STAGE(kStageAdjKick,
              launch_kick_adj<T>(a->psi_bar, psi_ref, V, st->v_ext, cc, Vbar, n, s), "kick adjoint");
     QJ_STAGE(kStageAdjPoisson,
-             launch_adj_poisson<T>(p->d, tb, Vbar, abar4, pr->four_pi_G, pr->dc_mode, w.H, w.Uz,
+             launch_adj_poisson<T>(p->d, tb, Vbar, abar3, pr->four_pi_G, pr->dc_mode, w.H, w.Uz,
                                    w.Uy, rhobar, s),
              "adjoint Poisson");
     QJ_STAGE(kStageAdjRho, launch_rho_adj<T>(a->psi_bar, psi_ref, rhobar, n, s), "rho adjoint");
@@ -934,16 +934,16 @@ int qj_gas_cfl_dt(qj_plan* p, const qj_state* st, const qj_params* pr, double cf
   void* const* g = k0 ? st->gas_tmp : st->gas;
   // the owed half kick of the last step is part of the reference's state
   const double h = (st->grid_valid && st->pending != 0.0) ? st->pending : 0.0;
-  const void* A4 = h != 0.0 ? (p->dtype == QJ_C64 ? plane<float>(st->grid, p, 1)
+  const void* A3 = h != 0.0 ? (p->dtype == QJ_C64 ? plane<float>(st->grid, p, 1)
                                                   : plane<double>(st->grid, p, 1))
                             : nullptr;
   const double dx = p->d.Lx / p->d.nx;
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = QJ_PRECISION(
       p,
-      launch_gas_cfl<float>(p->ncell(), g[1], g[2], g[3], A4, h, pr->cs, dx, cfl, dt_max, out,
+      launch_gas_cfl<float>(p->ncell(), g[1], g[2], g[3], A3, h, pr->cs, dx, cfl, dt_max, out,
                             nullptr, 0, p->partials, p->nblk_src, s),
-      launch_gas_cfl<double>(p->ncell(), g[1], g[2], g[3], A4, h, pr->cs, dx, cfl, dt_max, out,
+      launch_gas_cfl<double>(p->ncell(), g[1], g[2], g[3], A3, h, pr->cs, dx, cfl, dt_max, out,
                              nullptr, 0, p->partials, p->nblk_src, s));
   return e == hipSuccess ? QJ_OK : hipfail(e, "qj_gas_cfl_dt");
 }
@@ -980,7 +980,7 @@ size_t qj_star_scratch_size(const qj_plan* p, long long np) {
 }
 
 size_t qj_adjoint_scratch_size(const qj_plan* p) {
-  return p ? 6 * (size_t)p->ncell() * p->rsize : 0;
+  return p ? (2 + kAccComp) * (size_t)p->ncell() * p->rsize : 0;
 }
 
 int qj_kdk_record(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, qj_tape_step* tape,

@@ -81,27 +81,18 @@ __device__ __forceinline__ void deposit_one(const CicGeom& g, const Cic& c, doub
   }
 }
 
-// 4 reals of the interleaved grid cell idx (a_x, a_y, a_z, V).
+// (a_x, a_y, a_z) of cell idx of the acceleration grid (12 or 24 bytes,
+// 4- or 8-byte aligned: one dwordx3 load for float).
 template <class T>
-__device__ __forceinline__ void ld4(const T* __restrict__ A4, int idx, double o[4]) {
-  if constexpr (sizeof(T) == 4) {
-    const float4 t = *reinterpret_cast<const float4*>(A4 + 4 * idx);
-    o[0] = t.x;
-    o[1] = t.y;
-    o[2] = t.z;
-    o[3] = t.w;
-  } else {
-    const double2 a = *reinterpret_cast<const double2*>(A4 + 4 * idx);
-    const double2 b = *reinterpret_cast<const double2*>(A4 + 4 * idx + 2);
-    o[0] = a.x;
-    o[1] = a.y;
-    o[2] = b.x;
-    o[3] = b.y;
-  }
+__device__ __forceinline__ void ld3(const T* __restrict__ A3, int idx, double o[3]) {
+  const T* q = A3 + kAccComp * idx;
+  o[0] = q[0];
+  o[1] = q[1];
+  o[2] = q[2];
 }
 
 template <class T>
-__device__ __forceinline__ void interp_one(const CicGeom& g, const Cic& c, const T* A4,
+__device__ __forceinline__ void interp_one(const CicGeom& g, const Cic& c, const T* A3,
                                            double acc[3]) {
   acc[0] = acc[1] = acc[2] = 0.0;
   if (!c.ok) {
@@ -109,11 +100,11 @@ __device__ __forceinline__ void interp_one(const CicGeom& g, const Cic& c, const
     return;
   }
   // all 8 corner loads are issued before the accumulation
-  double a[8][4];
+  double a[8][3];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int sx = kCorner[k][0], sy = kCorner[k][1], sz = kCorner[k][2];
-    ld4(A4, (c.ix[sx] * g.ny + c.iy[sy]) * g.nz + c.iz[sz], a[k]);
+    ld3(A3, (c.ix[sx] * g.ny + c.iy[sy]) * g.nz + c.iz[sz], a[k]);
   }
 #pragma unroll
   for (int k = 0; k < 8; ++k) {

@@ -121,20 +121,13 @@ constexpr int kHT = 16;                 // max tile edge (y and z)
 constexpr int kHR = kHT + 4;            // raw region edge (2-cell halo)
 constexpr int kHS = kHT + 2;            // state region edge (1-cell ring)
 
-// (a_x, a_y, a_z) of the interleaved grid cell i in one 16-byte load (float)
+// (a_x, a_y, a_z) of cell i of the acceleration grid
 template <class T>
-__device__ __forceinline__ void ld3(const T* __restrict__ A4, int i, T a[3]) {
-  if constexpr (sizeof(T) == 4) {
-    const float4 t = *reinterpret_cast<const float4*>(A4 + 4 * i);
-    a[0] = t.x;
-    a[1] = t.y;
-    a[2] = t.z;
-  } else {
-    const double2 t = *reinterpret_cast<const double2*>(A4 + 4 * i);
-    a[0] = t.x;
-    a[1] = t.y;
-    a[2] = A4[4 * i + 2];
-  }
+__device__ __forceinline__ void ld_acc(const T* __restrict__ A3, int i, T a[3]) {
+  const T* q = A3 + kAccComp * i;
+  a[0] = q[0];
+  a[1] = q[1];
+  a[2] = q[2];
 }
 
 template <class T>
@@ -149,7 +142,7 @@ struct k_hydro_stream {
   static __device__ __forceinline__ void run(HGeom<T> g, int TY, int TZ, int XC,
                                              const T* __restrict__ rho, const T* __restrict__ vx,
                                              const T* __restrict__ vy, const T* __restrict__ vz,
-                                             const T* __restrict__ A4, T kick,
+                                             const T* __restrict__ A3, T kick,
                                              T* __restrict__ rho_o, T* __restrict__ vx_o,
                                              T* __restrict__ vy_o, T* __restrict__ vz_o,
                                              const double* __restrict__ dslot) {
@@ -189,7 +182,7 @@ struct k_hydro_stream {
         T r = rho[i], u = vx[i], v = vy[i], w = vz[i];
         if (kick != (T)0) {  // owed gravity kick of the gas velocity, quantum-fluid.py:274-276
           T a[3];
-          ld3(A4, i, a);
+          ld_acc(A3, i, a);
           u += a[0] * kick;
           v += a[1] * kick;
           w += a[2] * kick;
@@ -384,13 +377,13 @@ template <class T>
 struct k_gas_kick {
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(long long n, T* __restrict__ vx, T* __restrict__ vy,
-                                             T* __restrict__ vz, const T* __restrict__ A4, T h) {
+                                             T* __restrict__ vz, const T* __restrict__ A3, T h) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long stride = (long long)gridDim.x * blockDim.x;
     for (; i < n; i += stride) {  // quantum-fluid.py:274-276
-      vx[i] += A4[4 * i] * h;
-      vy[i] += A4[4 * i + 1] * h;
-      vz[i] += A4[4 * i + 2] * h;
+      vx[i] += A3[kAccComp * i] * h;
+      vy[i] += A3[kAccComp * i + 1] * h;
+      vz[i] += A3[kAccComp * i + 2] * h;
     }
   }
 };
@@ -405,7 +398,7 @@ struct k_gas_signal {
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(long long n, const T* __restrict__ vx,
                                              const T* __restrict__ vy, const T* __restrict__ vz,
-                                             const T* __restrict__ A4, T h,
+                                             const T* __restrict__ A3, T h,
                                              const double* __restrict__ hdev, double cs,
                                              double* __restrict__ part) {
     __shared__ double wmax[4];
@@ -415,10 +408,10 @@ struct k_gas_signal {
     double m = 0.0;
     for (; i < n; i += stride) {
       T u = vx[i], v = vy[i], w = vz[i];
-      if (A4) {
-        u += A4[4 * i] * h;
-        v += A4[4 * i + 1] * h;
-        w += A4[4 * i + 2] * h;
+      if (A3) {
+        u += A3[kAccComp * i] * h;
+        v += A3[kAccComp * i + 1] * h;
+        w += A3[kAccComp * i + 2] * h;
       }
       const double c = cs + sqrt((double)u * u + (double)v * v + (double)w * w);
       m = (c > m || c != c) ? c : m;
@@ -472,35 +465,35 @@ struct k_cfl_finish {
 
 template <class T>
 hipError_t launch_gas_cfl(long long n, const void* vx, const void* vy, const void* vz,
-                          const void* A4, double h, double cs, double dx, double cfl,
+                          const void* A3, double h, double cs, double dx, double cfl,
                           double dt_max, double* out, double* dts, int k, double* part,
                           int max_blocks, hipStream_t s) {
   long long blocks = (n + 255) / 256;
   if (blocks > max_blocks) blocks = max_blocks;
   // with dts, out is a whole step slot and the owed kick lives in it
-  const double* hdev = (dts && A4) ? out + kSlotOwed : nullptr;
+  const double* hdev = (dts && A3) ? out + kSlotOwed : nullptr;
   launch<k_gas_signal<T>>((int)blocks, 256, s, n, (const T*)vx, (const T*)vy, (const T*)vz,
-                          (const T*)A4, (T)h, hdev, cs, part);
+                          (const T*)A3, (T)h, hdev, cs, part);
   launch<k_cfl_finish>(1, 64, s, (const double*)part, (int)blocks, out, cfl, dx, dt_max,
                        dts ? 1 : 0, dts, k);
   return hipGetLastError();
 }
 
 template <class T>
-hipError_t launch_gas_kick(long long n, void* vx, void* vy, void* vz, const void* A4, double h,
+hipError_t launch_gas_kick(long long n, void* vx, void* vy, void* vz, const void* A3, double h,
                            hipStream_t s) {
   long long blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  launch<k_gas_kick<T>>((int)blocks, 256, s, n, (T*)vx, (T*)vy, (T*)vz, (const T*)A4, (T)h);
+  launch<k_gas_kick<T>>((int)blocks, 256, s, n, (T*)vx, (T*)vy, (T*)vz, (const T*)A3, (T)h);
   return hipGetLastError();
 }
 
 template <class T>
 hipError_t launch_hydro(const Dims& d, const void* rho, const void* vx, const void* vy,
-                        const void* vz, const void* A4, double kick, double dt, double cs,
+                        const void* vz, const void* A3, double kick, double dt, double cs,
                         int flux_kind, void* rho_o, void* vx_o, void* vy_o, void* vz_o,
                         hipStream_t s, const double* dslot) {
-  if ((kick != 0.0 || dslot) && A4 == nullptr) return hipErrorInvalidValue;
+  if ((kick != 0.0 || dslot) && A3 == nullptr) return hipErrorInvalidValue;
   if (flux_kind != QJ_FLUX_RUSANOV && flux_kind != QJ_FLUX_HLL) return hipErrorInvalidValue;
   HGeom<T> g;
   g.nx = d.nx;
@@ -519,12 +512,12 @@ hipError_t launch_hydro(const Dims& d, const void* rho, const void* vx, const vo
   if (flux_kind == QJ_FLUX_HLL)
     launch<k_hydro_stream<T, QJ_FLUX_HLL>>(nblk, nthr, s, g, TY, TZ, XC, (const T*)rho,
                                            (const T*)vx, (const T*)vy, (const T*)vz,
-                                           (const T*)A4, (T)kick, (T*)rho_o, (T*)vx_o,
+                                           (const T*)A3, (T)kick, (T*)rho_o, (T*)vx_o,
                                            (T*)vy_o, (T*)vz_o, dslot);
   else
     launch<k_hydro_stream<T, QJ_FLUX_RUSANOV>>(nblk, nthr, s, g, TY, TZ, XC, (const T*)rho,
                                                (const T*)vx, (const T*)vy, (const T*)vz,
-                                               (const T*)A4, (T)kick, (T*)rho_o, (T*)vx_o,
+                                               (const T*)A3, (T)kick, (T*)rho_o, (T*)vx_o,
                                                (T*)vy_o, (T*)vz_o, dslot);
   return hipGetLastError();
 }

@@ -7,6 +7,10 @@
 
 namespace qj {
 
+// Reals per cell of the acceleration grid A3 = (a_x, a_y, a_z): the
+// reference's a_grid stacked on the last axis (quantum-stars.py:256-259).
+constexpr int kAccComp = 3;
+
 struct Dims {
   int nx, ny, nz;
   double Lx, Ly, Lz;
@@ -61,17 +65,17 @@ template <class T>
 hipError_t launch_poisson_y(const Dims& d, const SpecTables<T>& tb, int grad, void* H, void* Uz,
                             void* Uy, hipStream_t s);
 
-// Writes V and, with grad, the cell-interleaved grid A4 = (a_x, a_y, a_z, V)
+// Writes V and, with grad, the cell-interleaved grid A3 = (a_x, a_y, a_z)
 // and folds max |a| into diag[kDiagAMax].
 template <class T>
 hipError_t launch_poisson_x(const Dims& d, const SpecTables<T>& tb, int grad, const void* H,
-                            const void* Uy, const void* Uzz, void* V, void* A4,
+                            const void* Uy, const void* Uzz, void* V, void* A3,
                             const void* addend, double* diag, hipStream_t s);
 
 // Number of workgroups source_x launches (size of the FP64 partials array).
 int source_x_blocks(const Dims& d);
 
-// ---- particles (the grid acceleration is A4 = (a_x, a_y, a_z, V) per cell)
+// ---- particles (the grid acceleration is A3 = (a_x, a_y, a_z) per cell)
 template <class T>
 hipError_t launch_cic_indices(const Dims& d, const double* pos, long long np, int* i, int* ip1,
                               double* w_i, double* w_ip1, hipStream_t s);
@@ -79,27 +83,27 @@ template <class T>
 hipError_t launch_deposit(const Dims& d, const double* pos, const double* mass, double m_scalar,
                           long long np, void* rho, hipStream_t s);
 template <class T>
-hipError_t launch_interp(const Dims& d, const double* pos, long long np, const void* A4,
+hipError_t launch_interp(const Dims& d, const double* pos, long long np, const void* A3,
                          double* acc, hipStream_t s);
-// One star step up to the deposit: acc = interp(A4, pos);
+// One star step up to the deposit: acc = interp(A3, pos);
 // vel += acc*h_owed (the previous step's closing half kick); vel += acc*h1;
 // pos = mod(pos + vel*dt, L); deposit(pos) into rho.
 template <class T>
-hipError_t launch_star_step(const Dims& d, double* pos, double* vel, long long np, const void* A4,
+hipError_t launch_star_step(const Dims& d, double* pos, double* vel, long long np, const void* A3,
                             double h_owed, double h1, double dt, const double* mass,
                             double m_scalar, void* rho, hipStream_t s);
-// vel += interp(A4)*h; diag[dt_s] = min |dx/vel| (diag[dt_s] must hold +inf)
+// vel += interp(A3)*h; diag[dt_s] = min |dx/vel| (diag[dt_s] must hold +inf)
 template <class T>
 hipError_t launch_star_kick(const Dims& d, const double* pos, double* vel, long long np,
-                            const void* A4, double h, double* diag, hipStream_t s);
+                            const void* A3, double h, double* diag, hipStream_t s);
 
 // ---- adjoints of the particle operations ----------------------------------
-// Adjoint of vel += h*interp(A4, pos), preceded (dt_xv != 0) by that of
-// pos += dt_xv*vel: vbar += dt_xv*xbar; then abar4[c].xyz += w_c*h*vbar
-// (atomics; abar4 is (nx,ny,nz,4), slot 3 unused) and xbar += grad terms.
+// Adjoint of vel += h*interp(A3, pos), preceded (dt_xv != 0) by that of
+// pos += dt_xv*vel: vbar += dt_xv*xbar; then abar3[c].xyz += w_c*h*vbar
+// (atomics; abar3 is (nx,ny,nz,3)) and xbar += grad terms.
 template <class T>
-hipError_t launch_interp_adj(const Dims& d, const double* pos, long long np, const void* A4,
-                             double h, double* vbar, double dt_xv, double* xbar, void* abar4,
+hipError_t launch_interp_adj(const Dims& d, const double* pos, long long np, const void* A3,
+                             double h, double* vbar, double dt_xv, double* xbar, void* abar3,
                              hipStream_t s);
 // Adjoint of the deposit: mbar += interp(rhobar)/vol, xbar += grad terms.
 template <class T>
@@ -118,10 +122,10 @@ hipError_t launch_rho_adj(void* psi_bar, const void* psi_ref, const void* rhobar
                           hipStream_t s);
 // rhobar = P(Vbar + div abar), P the Poisson operator (kernel -4 pi G/k^2 with
 // the k = 0 bin -4 pi G for dc_mode 0, 0 for dc_mode 1), div the spectral
-// divergence with the Nyquist derivative zeroed; abar4 is (nx,ny,nz,4).
+// divergence with the Nyquist derivative zeroed; abar3 is (nx,ny,nz,3).
 template <class T>
 hipError_t launch_adj_poisson(const Dims& d, const SpecTables<T>& tb, const void* Vbar,
-                              const void* abar4, double four_pi_G, int dc_mode, void* H, void* U2,
+                              const void* abar3, double four_pi_G, int dc_mode, void* H, void* U2,
                               void* U3, void* rhobar, hipStream_t s);
 
 // ---- gas ----------------------------------------------------------------
@@ -131,25 +135,25 @@ hipError_t launch_adj_poisson(const Dims& d, const SpecTables<T>& tb, const void
 constexpr int kSlotCmax = 0, kSlotDt = 1, kSlotKick = 2, kSlotOwed = 3, kSlotCount = 4;
 constexpr int kCflChunk = 64;  // CFL steps per host read-back
 
-// Applies v += a*kick first when kick != 0 (A4 may be null when kick == 0).
+// Applies v += a*kick first when kick != 0 (A3 may be null when kick == 0).
 // flux_kind: QJ_FLUX_RUSANOV (the reference's) or QJ_FLUX_HLL.  dslot
 // (optional): dt and kick come from the device step slot instead.
 template <class T>
 hipError_t launch_hydro(const Dims& d, const void* rho, const void* vx, const void* vy,
-                        const void* vz, const void* A4, double kick, double dt, double cs,
+                        const void* vz, const void* A3, double kick, double dt, double cs,
                         int flux_kind, void* rho_o, void* vx_o, void* vy_o, void* vz_o,
                         hipStream_t s, const double* dslot = nullptr);
 template <class T>
-hipError_t launch_gas_kick(long long n, void* vx, void* vy, void* vz, const void* A4, double h,
+hipError_t launch_gas_kick(long long n, void* vx, void* vy, void* vz, const void* A3, double h,
                            hipStream_t s);
-// out[kSlotCmax] = max over cells of cs + |v + h a| (a from A4, skipped
-// when A4 is null), out[kSlotDt] = min(dt_max, cfl dx / (Cmax + 1e-30)).
+// out[kSlotCmax] = max over cells of cs + |v + h a| (a from A3, skipped
+// when A3 is null), out[kSlotDt] = min(dt_max, cfl dx / (Cmax + 1e-30)).
 // With dts, out is a whole step slot: h is read from out[kSlotOwed], the
 // slot advances (kick = owed + dt/2, owed = dt/2) and dts[k] = dt.  part:
 // scratch for max_blocks per-block maxima.
 template <class T>
 hipError_t launch_gas_cfl(long long n, const void* vx, const void* vy, const void* vz,
-                          const void* A4, double h, double cs, double dx, double cfl,
+                          const void* A3, double h, double cs, double dx, double cfl,
                           double dt_max, double* out, double* dts, int k, double* part,
                           int max_blocks, hipStream_t s);
 
@@ -175,7 +179,7 @@ StarBins carve_star_scratch(const Dims& d, long long np, void* scratch);
 // registration counting of the (new) positions.
 template <class T>
 hipError_t launch_star_step_binned(const Dims& d, double* pos, double* vel, long long np,
-                                   const void* A4, double h_owed, double h1, double dt, int move,
+                                   const void* A3, double h_owed, double h1, double dt, int move,
                                    const StarBins& sb, hipStream_t s);
 // Scan, scatter and per-tile deposit: overwrites every cell of rho.
 template <class T>
@@ -183,14 +187,14 @@ hipError_t launch_tile_deposit(const Dims& d, const double* pos, const double* m
                                double m_scalar, long long np, const StarBins& sb, void* rho,
                                hipStream_t s);
 
-// Interpolation adjoint whose h vbar corner values go to abar4 through the
+// Interpolation adjoint whose h vbar corner values go to abar3 through the
 // tile-binned deposit (whole tiles written, no clearing needed).  taped: sb
 // already holds the registrations of pos (offsets and sorted), so they are
 // not counted again.
 template <class T>
 hipError_t launch_interp_adj_binned(const Dims& d, const double* pos, long long np,
-                                    const void* A4, double h, double* vbar, double dt_xv,
-                                    double* xbar, void* abar4, const StarBins& sb, bool taped,
+                                    const void* A3, double h, double* vbar, double dt_xv,
+                                    double* xbar, void* abar3, const StarBins& sb, bool taped,
                                     hipStream_t s);
 
 // Accretion onto black hole 0 (quantum-bh.py:306-339), in place on psi and m_bh.

@@ -53,13 +53,13 @@ template <class T>
 struct k_interp {
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(CicGeom g, const double* __restrict__ pos,
-                                             long long np, const T* __restrict__ A4,
+                                             long long np, const T* __restrict__ A3,
                                              double* __restrict__ acc) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     Cic c = cic(g, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]);
     double a[3];
-    interp_one(g, c, A4, a);
+    interp_one(g, c, A3, a);
     acc[3 * p] = a[0];
     acc[3 * p + 1] = a[1];
     acc[3 * p + 2] = a[2];
@@ -74,7 +74,7 @@ struct k_star_step {
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(CicGeom g, double* __restrict__ pos,
                                              double* __restrict__ vel, long long np,
-                                             const T* __restrict__ A4, double h_owed, double h1,
+                                             const T* __restrict__ A3, double h_owed, double h1,
                                              double dt, const double* __restrict__ mass,
                                              double m_scalar, T* __restrict__ rho) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -84,7 +84,7 @@ struct k_star_step {
     if (h_owed != 0.0 || h1 != 0.0) {
       Cic c = cic(g, x[0], x[1], x[2]);
       double a[3];
-      interp_one(g, c, A4, a);
+      interp_one(g, c, A3, a);
       if (h_owed != 0.0) {
   #pragma unroll
         for (int d = 0; d < 3; ++d) v[d] = v[d] + a[d] * h_owed;  // quantum-stars.py:321
@@ -116,14 +116,14 @@ struct k_star_kick {
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(CicGeom g, const double* __restrict__ pos,
                                              double* __restrict__ vel, long long np,
-                                             const T* __restrict__ A4, double h,
+                                             const T* __restrict__ A3, double h,
                                              double* __restrict__ diag) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double dts = __builtin_inf();
     if (p < np) {
       Cic c = cic(g, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]);
       double a[3];
-      interp_one(g, c, A4, a);
+      interp_one(g, c, A3, a);
   #pragma unroll
       for (int d = 0; d < 3; ++d) {
         double v = vel[3 * p + d] + a[d] * h;
@@ -186,19 +186,19 @@ __device__ __forceinline__ int stencil_tiles(const TileGeom& t, const Cic& c, in
   return n;
 }
 
-// Adjoint of vel += h * interp(A4, pos) (and, first, of the drift
+// Adjoint of vel += h * interp(A3, pos) (and, first, of the drift
 // pos += dt*vel when dt_xv != 0: vbar += dt_xv * xbar).  With
 // acc_bar = h * vbar: abar[c] += w_c acc_bar (the transpose of the gather,
 // into (a_x, a_y, a_z, -) cells) and xbar += sum_c grad w_c (a[c] . acc_bar).
-// BIN: instead of adding h vbar into abar4 with atomics, register the star
+// BIN: instead of adding h vbar into abar3 with atomics, register the star
 // in the tiles of its stencil (the tile-binned deposit adds it afterwards).
 template <class T, bool BIN = false>
 struct k_interp_adj {
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(CicGeom g, const double* __restrict__ pos,
-                                             long long np, const T* __restrict__ A4, double h,
+                                             long long np, const T* __restrict__ A3, double h,
                                              double* __restrict__ vbar, double dt_xv,
-                                             double* __restrict__ xbar, T* __restrict__ abar4,
+                                             double* __restrict__ xbar, T* __restrict__ abar3,
                                              TileGeom tg, int* __restrict__ count,
                                              int* __restrict__ rank) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -228,10 +228,10 @@ struct k_interp_adj {
       const double w = c.wx[sx] * c.wy[sy] * c.wz[sz];
       if constexpr (!BIN) {
   #pragma unroll
-        for (int d = 0; d < 3; ++d) unsafeAtomicAdd(&abar4[4 * idx + d], (T)(w * ab[d]));
+        for (int d = 0; d < 3; ++d) unsafeAtomicAdd(&abar3[kAccComp * idx + d], (T)(w * ab[d]));
       }
-      double a[4];
-      ld4(A4, idx, a);
+      double a[3];
+      ld3(A3, idx, a);
       const double s = a[0] * ab[0] + a[1] * ab[1] + a[2] * ab[2];
       double gw[3];
       corner_grad(g, c, k, gw);
@@ -282,7 +282,7 @@ struct k_star_step_bin {
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(CicGeom g, TileGeom tg, double* __restrict__ pos,
                                              double* __restrict__ vel, long long np,
-                                             const T* __restrict__ A4, double h_owed, double h1,
+                                             const T* __restrict__ A3, double h_owed, double h1,
                                              double dt, int* __restrict__ count,
                                              int* __restrict__ rank) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -293,7 +293,7 @@ struct k_star_step_bin {
       if (h_owed != 0.0 || h1 != 0.0) {
         Cic c = cic(g, x[0], x[1], x[2]);
         double a[3];
-        interp_one(g, c, A4, a);
+        interp_one(g, c, A3, a);
         if (h_owed != 0.0) {
   #pragma unroll
           for (int d = 0; d < 3; ++d) v[d] = v[d] + a[d] * h_owed;  // quantum-stars.py:321
@@ -417,7 +417,7 @@ struct k_tile_deposit {
 
 // Vector form of the per-tile deposit for the adjoint of the star kicks:
 // abar[c] = sum over stars of w_c h vbar (the corner values of the atomic
-// interpolation adjoint), written as whole (a_x, a_y, a_z, 0) tiles.
+// interpolation adjoint), written as whole (a_x, a_y, a_z) tiles.
 template <class T, int VOL = kTileMaxCells>
 struct k_tile_deposit_vec {
   static constexpr int kThreads = 256;
@@ -426,7 +426,7 @@ struct k_tile_deposit_vec {
                                              const double* __restrict__ vbar, double h,
                                              const int* __restrict__ offset,
                                              const int* __restrict__ sorted,
-                                             T* __restrict__ abar4) {
+                                             T* __restrict__ abar3) {
     __shared__ T acc[3][VOL];
     const int t = blockIdx.x;
     const int tz = t % tg.ntz, ty = (t / tg.ntz) % tg.nty, tx = t / (tg.ntz * tg.nty);
@@ -456,13 +456,10 @@ struct k_tile_deposit_vec {
     for (int l = threadIdx.x; l < vol; l += blockDim.x) {
       const int lz = l & mz, ly = (l >> tg.lz) & my, lx = l >> (tg.lz + tg.ly);
       const long long idx = ((long long)(x0 + lx) * g.ny + (y0 + ly)) * g.nz + (z0 + lz);
-      T* o = abar4 + 4 * idx;
-      if constexpr (sizeof(T) == 4) {
-        *reinterpret_cast<float4*>(o) = make_float4(acc[0][l], acc[1][l], acc[2][l], 0.0f);
-      } else {
-        *reinterpret_cast<double2*>(o) = make_double2(acc[0][l], acc[1][l]);
-        *reinterpret_cast<double2*>(o + 2) = make_double2(acc[2][l], 0.0);
-      }
+      T* o = abar3 + kAccComp * idx;
+      o[0] = acc[0][l];
+      o[1] = acc[1][l];
+      o[2] = acc[2][l];
     }
   }
 };
@@ -538,38 +535,38 @@ hipError_t launch_deposit(const Dims& d, const double* pos, const double* mass, 
 }
 
 template <class T>
-hipError_t launch_interp(const Dims& d, const double* pos, long long np, const void* A4,
+hipError_t launch_interp(const Dims& d, const double* pos, long long np, const void* A3,
                          double* acc, hipStream_t s) {
   if (np <= 0) return hipSuccess;
-  launch<k_interp<T>>(pblocks(np), 256, s, make_geom(d), pos, np, (const T*)A4, acc);
+  launch<k_interp<T>>(pblocks(np), 256, s, make_geom(d), pos, np, (const T*)A3, acc);
   return hipGetLastError();
 }
 
 template <class T>
-hipError_t launch_star_step(const Dims& d, double* pos, double* vel, long long np, const void* A4,
+hipError_t launch_star_step(const Dims& d, double* pos, double* vel, long long np, const void* A3,
                             double h_owed, double h1, double dt, const double* mass,
                             double m_scalar, void* rho, hipStream_t s) {
   if (np <= 0) return hipSuccess;
-  launch<k_star_step<T>>(pblocks(np), 256, s, make_geom(d), pos, vel, np, (const T*)A4, h_owed,
+  launch<k_star_step<T>>(pblocks(np), 256, s, make_geom(d), pos, vel, np, (const T*)A3, h_owed,
                                              h1, dt, mass, m_scalar, (T*)rho);
   return hipGetLastError();
 }
 
 template <class T>
 hipError_t launch_star_kick(const Dims& d, const double* pos, double* vel, long long np,
-                            const void* A4, double h, double* diag, hipStream_t s) {
+                            const void* A3, double h, double* diag, hipStream_t s) {
   if (np <= 0) return hipSuccess;
-  launch<k_star_kick<T>>(pblocks(np), 256, s, make_geom(d), pos, vel, np, (const T*)A4, h, diag);
+  launch<k_star_kick<T>>(pblocks(np), 256, s, make_geom(d), pos, vel, np, (const T*)A3, h, diag);
   return hipGetLastError();
 }
 
 template <class T>
-hipError_t launch_interp_adj(const Dims& d, const double* pos, long long np, const void* A4,
-                             double h, double* vbar, double dt_xv, double* xbar, void* abar4,
+hipError_t launch_interp_adj(const Dims& d, const double* pos, long long np, const void* A3,
+                             double h, double* vbar, double dt_xv, double* xbar, void* abar3,
                              hipStream_t s) {
   if (np <= 0) return hipSuccess;
-  launch<k_interp_adj<T, false>>(pblocks(np), 256, s, make_geom(d), pos, np, (const T*)A4, h,
-                                 vbar, dt_xv, xbar, (T*)abar4, TileGeom{}, (int*)nullptr,
+  launch<k_interp_adj<T, false>>(pblocks(np), 256, s, make_geom(d), pos, np, (const T*)A3, h,
+                                 vbar, dt_xv, xbar, (T*)abar3, TileGeom{}, (int*)nullptr,
                                  (int*)nullptr);
   return hipGetLastError();
 }
@@ -625,16 +622,16 @@ StarBins carve_star_scratch(const Dims& d, long long np, void* scratch) {
 
 template <class T>
 hipError_t launch_star_step_binned(const Dims& d, double* pos, double* vel, long long np,
-                                   const void* A4, double h_owed, double h1, double dt, int move,
+                                   const void* A3, double h_owed, double h1, double dt, int move,
                                    const StarBins& sb, hipStream_t s) {
   if (np <= 0) return hipSuccess;
   const TileGeom tg = make_tiles(d);
   if (move)
     launch<k_star_step_bin<T, true>>(pblocks(np), 256, s, make_geom(d), tg, pos, vel, np,
-                                     (const T*)A4, h_owed, h1, dt, sb.count, sb.rank);
+                                     (const T*)A3, h_owed, h1, dt, sb.count, sb.rank);
   else
     launch<k_star_step_bin<T, false>>(pblocks(np), 256, s, make_geom(d), tg, pos, vel, np,
-                                      (const T*)A4, h_owed, h1, dt, sb.count, sb.rank);
+                                      (const T*)A3, h_owed, h1, dt, sb.count, sb.rank);
   return hipGetLastError();
 }
 
@@ -664,12 +661,12 @@ hipError_t launch_bh_accretion(const Dims& d, void* psi, const double* pos, cons
 
 template <class T>
 hipError_t launch_interp_adj_binned(const Dims& d, const double* pos, long long np,
-                                    const void* A4, double h, double* vbar, double dt_xv,
-                                    double* xbar, void* abar4, const StarBins& sb, bool taped,
+                                    const void* A3, double h, double* vbar, double dt_xv,
+                                    double* xbar, void* abar3, const StarBins& sb, bool taped,
                                     hipStream_t s) {
   const TileGeom tg = make_tiles(d);
   if (np > 0)
-    launch<k_interp_adj<T, true>>(pblocks(np), 256, s, make_geom(d), pos, np, (const T*)A4, h,
+    launch<k_interp_adj<T, true>>(pblocks(np), 256, s, make_geom(d), pos, np, (const T*)A3, h,
                                   vbar, dt_xv, xbar, (T*)nullptr, tg,
                                   taped ? (int*)nullptr : sb.count, sb.rank);
   if (!taped) {  // else offset and sorted hold the forward's registrations of pos
@@ -681,7 +678,7 @@ hipError_t launch_interp_adj_binned(const Dims& d, const double* pos, long long 
   QJ_TILE_VOL(tg, launch<k_tile_deposit_vec<T, VOL_>>(tg.ntiles, 256, s, make_geom(d), tg, pos,
                                                       (const double*)vbar, h,
                                                       (const int*)sb.offset,
-                                                      (const int*)sb.sorted, (T*)abar4));
+                                                      (const int*)sb.sorted, (T*)abar3));
   return hipGetLastError();
 }
 

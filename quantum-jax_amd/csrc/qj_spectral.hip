@@ -514,7 +514,7 @@ __device__ __forceinline__ void load_herm_rows(cpx<T> (&v)[VW][E], const cpx<T>*
 // ---------------------------------------------------------------------------
 // Poisson x pass (inverse, Hermitian-completed): V = IFFT_x(UU); with GRAD
 // a_x + i a_y = -IFFT_x(i kx UU + i Uy), a_z = -IFFT_x(Uzz), written as the
-// cell-interleaved grid A4 = (a_x, a_y, a_z, V), and max |a| folded into
+// cell-interleaved grid A3 = (a_x, a_y, a_z), and max |a| folded into
 // the running maximum diag[kDiagAMax] with one atomic per block.
 // quantum-stars.py:192 and :256-260.
 // ---------------------------------------------------------------------------
@@ -525,7 +525,7 @@ struct k_poisson_x {
                                              const cpx<T>* __restrict__ Uy,
                                              const cpx<T>* __restrict__ Uzz, PassGeom g,
                                              const cpx<T>* __restrict__ tw, double kfx,
-                                             T* __restrict__ V, T* __restrict__ A4,
+                                             T* __restrict__ V, T* __restrict__ A3,
                                              const T* __restrict__ addend,
                                              double* __restrict__ diag) {
     QJ_STRIDED_SETUP(T, N)
@@ -554,12 +554,11 @@ struct k_poisson_x {
         for (int w = 0; w < VW; ++w) u[w][m] = {-kx * v[w][m].i, kx * v[w][m].r};
       }
       pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
-      T vr[VW][E];
   #pragma unroll
       for (int m = 0; m < E; ++m) {
         T t[VW];
   #pragma unroll
-        for (int w = 0; w < VW; ++w) t[w] = vr[w][m] = v[w][m].r;
+        for (int w = 0; w < VW; ++w) t[w] = v[w][m].r;
         if (act) str<VW>(V, base + (j + TT * m) * se, t);
       }
       load_herm_rows<N, VW, E>(v, Uy, base, se, j, TT, act);
@@ -578,14 +577,10 @@ struct k_poisson_x {
   #pragma unroll
           for (int w = 0; w < VW; ++w) {
             const T a0 = -u[w][m].r, a1 = -u[w][m].i, a2 = -v[w][m].r;
-            if constexpr (sizeof(T) == 4) {
-              *reinterpret_cast<float4*>(A4 + 4 * (idx + w)) = make_float4(a0, a1, a2, vr[w][m]);
-            } else {
-              A4[4 * (idx + w) + 0] = a0;
-              A4[4 * (idx + w) + 1] = a1;
-              A4[4 * (idx + w) + 2] = a2;
-              A4[4 * (idx + w) + 3] = vr[w][m];
-            }
+            T* o = A3 + kAccComp * (idx + w);
+            o[0] = a0;
+            o[1] = a1;
+            o[2] = a2;
             const T m0 = fabs(a0), m1 = fabs(a1), m2 = fabs(a2);
             amax = (m0 > amax || m0 != m0) ? m0 : amax;
             amax = (m1 > amax || m1 != m1) ? m1 : amax;
@@ -666,7 +661,7 @@ template <class T, int N>
 struct k_adj_src_x {
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(const T* __restrict__ Vbar,
-                                             const T* __restrict__ abar4, PassGeom g,
+                                             const T* __restrict__ abar3, PassGeom g,
                                              const cpx<T>* __restrict__ tw, double kfx,
                                              cpx<T>* __restrict__ T1, cpx<T>* __restrict__ T2,
                                              cpx<T>* __restrict__ T3) {
@@ -679,7 +674,7 @@ struct k_adj_src_x {
   #pragma unroll
         for (int w = 0; w < VW; ++w) {
           T x = (T)0;
-          if (act) x = comp < 0 ? Vbar[idx + w] : abar4[4 * (idx + w) + comp];
+          if (act) x = comp < 0 ? Vbar[idx + w] : abar3[kAccComp * (idx + w) + comp];
           o[w][m] = {x, (T)0};
         }
       }
@@ -988,7 +983,7 @@ hipError_t launch_poisson_y(const Dims& d, const SpecTables<T>& tb, int grad, vo
 
 template <class T>
 hipError_t launch_poisson_x(const Dims& d, const SpecTables<T>& tb, int grad, const void* H,
-                            const void* Uy, const void* Uzz, void* V, void* A4,
+                            const void* Uy, const void* Uzz, void* V, void* A3,
                             const void* addend, double* diag, hipStream_t s) {
   PassGeom g = geom_x(d);
   const cpx<T>* tw = (const cpx<T>*)tb.tw_x;
@@ -997,11 +992,11 @@ hipError_t launch_poisson_x(const Dims& d, const SpecTables<T>& tb, int grad, co
     const int nb = nblocks_s<T, N_>(g.ncols);
     if (grad)
       launch<k_poisson_x<T, N_, true>>(nb, 256, s,
-          (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A4,
+          (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A3,
           (const T*)addend, diag);
     else
       launch<k_poisson_x<T, N_, false>>(nb, 256, s,
-          (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A4,
+          (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A3,
           (const T*)addend, diag);
   });
   return hipGetLastError();
@@ -1029,7 +1024,7 @@ hipError_t launch_rho_adj(void* psi_bar, const void* psi_ref, const void* rhobar
 
 template <class T>
 hipError_t launch_adj_poisson(const Dims& d, const SpecTables<T>& tb, const void* Vbar,
-                              const void* abar4, double four_pi_G, int dc_mode, void* H, void* U2,
+                              const void* abar3, double four_pi_G, int dc_mode, void* H, void* U2,
                               void* U3, void* rhobar, hipStream_t s) {
   const double kfx = 2.0 * M_PI / d.Lx, kfy = 2.0 * M_PI / d.Ly, kfz = 2.0 * M_PI / d.Lz;
   const int nxh = d.nx / 2 + 1;
@@ -1038,7 +1033,7 @@ hipError_t launch_adj_poisson(const Dims& d, const SpecTables<T>& tb, const void
     const cpx<T>* tw = (const cpx<T>*)tb.tw_x;
     QJ_DISPATCH_N(d.nx, {
       launch<k_adj_src_x<T, N_>>(nblocks_s<T, N_>(g.ncols), 256, s,
-          (const T*)Vbar, (const T*)abar4, g, tw, kfx, (cpx<T>*)H, (cpx<T>*)U2, (cpx<T>*)U3);
+          (const T*)Vbar, (const T*)abar3, g, tw, kfx, (cpx<T>*)H, (cpx<T>*)U2, (cpx<T>*)U3);
     });
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -34,25 +34,26 @@ def get_potential(rho: torch.Tensor, plan: Plan, phys: Physics) -> torch.Tensor:
     return V
 
 
-def _new_a4(plan: Plan) -> torch.Tensor:
-    return torch.empty(plan.shape + (4,), dtype=plan.real_dtype, device=plan.device)
+def _new_a_grid(plan: Plan) -> torch.Tensor:
+    return torch.empty(plan.shape + (_lib.ACC_COMPONENTS,), dtype=plan.real_dtype,
+                       device=plan.device)
 
 
 def acceleration_grid(rho: torch.Tensor, plan: Plan, phys: Physics, diag=None):
-    """(V, a4): a4[..., d] = -Re IFFT(1j k_d V_hat) for d < 3 and a4[..., 3] = V.
+    """(V, a_grid): a_grid[..., d] = -Re IFFT(1j k_d V_hat), d = x, y, z.
 
-    quantum-stars.py:254-259 (a4[..., :3] is the reference's a_grid, stacked
-    on the last axis).  max |a| is folded into diag[0] when diag is given.
+    quantum-stars.py:254-259 (the reference's a_grid, stacked on the last
+    axis).  max |a| is folded into diag[0] when diag is given.
     """
     plan.field(rho, "rho")
     V = torch.empty_like(rho)
-    a4 = _new_a4(plan)
+    a = _new_a_grid(plan)
     with torch.cuda.device(plan.device):
         _lib.check(_L().qj_acceleration_grid(plan.handle, ptr(rho), phys.rho_bar,
-                                             phys.four_pi_G, phys.dc_mode, ptr(V), ptr(a4),
+                                             phys.four_pi_G, phys.dc_mode, ptr(V), ptr(a),
                                              ptr(diag), ptr(plan.workspace()), plan.stream()),
                    "acceleration_grid")
-    return V, a4
+    return V, a
 
 
 def kick(psi: torch.Tensor, V: torch.Tensor, coef: float, plan: Plan) -> torch.Tensor:
@@ -122,16 +123,17 @@ def bin_stars(pos: torch.Tensor, m_s: float, plan: Plan) -> torch.Tensor:
     return bin_particles(pos, float(m_s), plan)
 
 
-def interp(pos: torch.Tensor, a4: torch.Tensor, plan: Plan) -> torch.Tensor:
-    """8-corner CIC gather of a4[..., :3] (a4 is (nx,ny,nz,4)); quantum-stars.py:262-287."""
+def interp(pos: torch.Tensor, a_grid: torch.Tensor, plan: Plan) -> torch.Tensor:
+    """8-corner CIC gather of a_grid ((nx,ny,nz,3)); quantum-stars.py:262-287."""
     plan.particles(pos, "pos")
-    if (a4.shape != plan.shape + (4,) or a4.dtype != plan.real_dtype
-            or a4.device != plan.device or not a4.is_contiguous()):
-        raise ValueError(f"a4 must be a contiguous {plan.real_dtype} tensor of shape "
-                         f"{plan.shape + (4,)} on {plan.device}")
+    shape = plan.shape + (_lib.ACC_COMPONENTS,)
+    if (a_grid.shape != shape or a_grid.dtype != plan.real_dtype
+            or a_grid.device != plan.device or not a_grid.is_contiguous()):
+        raise ValueError(f"a_grid must be a contiguous {plan.real_dtype} tensor of shape "
+                         f"{shape} on {plan.device}")
     acc = torch.empty((pos.shape[0], 3), dtype=torch.float64, device=plan.device)
     with torch.cuda.device(plan.device):
-        _lib.check(_L().qj_interp(plan.handle, ptr(pos), pos.shape[0], ptr(a4), ptr(acc),
+        _lib.check(_L().qj_interp(plan.handle, ptr(pos), pos.shape[0], ptr(a_grid), ptr(acc),
                                   plan.stream()), "interp")
     return acc
 
@@ -141,7 +143,7 @@ def get_acceleration(pos: torch.Tensor, rho: torch.Tensor, plan: Plan, phys: Phy
     plan.particles(pos, "pos")
     plan.field(rho, "rho")
     diag = plan.new_diag()
-    a = _new_a4(plan)
+    a = _new_a_grid(plan)
     acc = torch.empty((pos.shape[0], 3), dtype=torch.float64, device=plan.device)
     with torch.cuda.device(plan.device):
         _lib.check(_L().qj_get_acceleration(plan.handle, ptr(pos), pos.shape[0], ptr(rho),
@@ -156,7 +158,7 @@ def apply_grav_accel(rho, vx, vy, vz, dt: float, plan: Plan, phys: Physics):
     """quantum-fluid.py:267-278 -> (vx, vy, vz) (new tensors)."""
     plan.field(rho, "rho")
     out = [plan.field(v, n).clone() for v, n in ((vx, "vx"), (vy, "vy"), (vz, "vz"))]
-    a = _new_a4(plan)
+    a = _new_a_grid(plan)
     with torch.cuda.device(plan.device):
         _lib.check(_L().qj_apply_grav_accel(plan.handle, ptr(rho), ptr(out[0]), ptr(out[1]),
                                             ptr(out[2]), float(dt), phys.rho_bar,

@@ -63,8 +63,8 @@ class KDKSimulation:
         if gas is not None:
             self.gas = [plan.field(g, n).clone() for g, n in zip(gas, ("rho", "vx", "vy", "vz"))]
             self._gas_tmp = [torch.empty_like(g) for g in self.gas]
-        # V (nx,ny,nz) followed by A4 (nx,ny,nz,4) = (a_x, a_y, a_z, V) per cell
-        self.grid = torch.empty(5 * plan.ncell, dtype=rdt, device=dev)
+        # V (nx,ny,nz) followed by the acceleration grid (nx,ny,nz,3)
+        self.grid = torch.empty((1 + _lib.ACC_COMPONENTS) * plan.ncell, dtype=rdt, device=dev)
         self.diag = plan.new_diag()
         self.brho = torch.empty(plan.shape, dtype=rdt, device=dev) if phys.b_coeff else None
         st = _lib.QjState()
@@ -196,14 +196,10 @@ class KDKSimulation:
         return self.grid[: self.plan.ncell].view(self.plan.shape)
 
     @property
-    def a4(self) -> torch.Tensor:
-        """(nx,ny,nz,4) = (a_x, a_y, a_z, V) of the current state (stars/gas)."""
-        return self.grid[self.plan.ncell:].view(self.plan.shape + (4,))
-
-    @property
     def a_grid(self) -> torch.Tensor:
-        """The reference's a_grid, (nx,ny,nz,3) stacked on the last axis."""
-        return self.a4[..., :3]
+        """The reference's a_grid of the current state (stars/gas), (nx,ny,nz,3)
+        stacked on the last axis."""
+        return self.grid[self.plan.ncell:].view(self.plan.shape + (_lib.ACC_COMPONENTS,))
 
     def a_max(self) -> float:
         return float(self.diag[_lib.DIAG_AMAX].item())
