@@ -299,6 +299,19 @@ struct k_source_x {
     for (int m = 0; m < E; ++m)
   #pragma unroll
       for (int w = 0; w < VW; ++w) rho[w][m] = (T)0;
+    // the star density rows are loaded with psi's, so their latency is hidden
+    // behind the drift transforms (LDS, not registers, limits the blocks per
+    // CU): 120 -> 88 us at 256^3 c64
+    T sv[VW][E];
+    if (src && act) {
+  #pragma unroll
+      for (int m = 0; m < E; ++m) {
+        T t[VW];
+        ldr<VW>(src, base + (j + TT * m) * se, t);
+  #pragma unroll
+        for (int w = 0; w < VW; ++w) sv[w][m] = t[w];
+      }
+    }
     if (psi) {
       load_rows<VW, E>(v, psi, base, se, j, TT, act);
       if constexpr (DRIFT) {
@@ -325,10 +338,8 @@ struct k_source_x {
       const int idx = base + (j + TT * m) * se;
       if (act) {
         if (src) {
-          T t[VW];
-          ldr<VW>(src, idx, t);
   #pragma unroll
-          for (int w = 0; w < VW; ++w) rho[w][m] += t[w];
+          for (int w = 0; w < VW; ++w) rho[w][m] += sv[w][m];
           if (clear_src) {
             T z[VW];
   #pragma unroll
@@ -351,16 +362,44 @@ struct k_source_x {
   #pragma unroll
       for (int w = 0; w < VW; ++w) v[w][m] = {act ? (T)((double)rho[w][m] - rho_bar) : (T)0, (T)0};
     }
-    pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
-    if (act) {
+    if constexpr (VW == 2) {
+      // the lane's two real pencils as one complex transform Z of
+      // z = r0 + i r1, split with the mirrored bin (one LDS exchange):
+      // R0[k] = (Z[k] + conj Z[N-k]) / 2, R1[k] = (Z[k] - conj Z[N-k]) / 2i
+      cpx<T> z[1][E];
   #pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const int k = j + TT * m;
-        if (k <= N / 2) {
-          cpx<T> t[VW];
+      for (int m = 0; m < E; ++m) z[0][m] = {v[0][m].r, v[1][m].r};
+      pencil_fft_strided<N, -1, T, 1>(z, j, col, lds, tw);
+      __syncthreads();  // the transform's last exchange is read out
   #pragma unroll
-          for (int w = 0; w < VW; ++w) t[w] = v[w][m];
-          stc<VW>(H, base + k * se, t);
+      for (int m = 0; m < E; ++m) lds[(j + TT * m) * S::C + col] = z[0][m];
+      __syncthreads();
+      if (act) {
+  #pragma unroll
+        for (int m = 0; m < E; ++m) {
+          const int k = j + TT * m;
+          if (k <= N / 2) {
+            const cpx<T> a = z[0][m], b = lds[((N - k) & (N - 1)) * S::C + col];
+            // b is read unconjugated: conj(b) enters through the signs
+            cpx<T> t[2];
+            t[0] = {(T)0.5 * (a.r + b.r), (T)0.5 * (a.i - b.i)};
+            t[1] = {(T)0.5 * (a.i + b.i), (T)0.5 * (b.r - a.r)};
+            stc<2>(H, base + k * se, t);
+          }
+        }
+      }
+    } else {
+      pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
+      if (act) {
+  #pragma unroll
+        for (int m = 0; m < E; ++m) {
+          const int k = j + TT * m;
+          if (k <= N / 2) {
+            cpx<T> t[VW];
+  #pragma unroll
+            for (int w = 0; w < VW; ++w) t[w] = v[w][m];
+            stc<VW>(H, base + k * se, t);
+          }
         }
       }
     }
