@@ -552,7 +552,8 @@ __device__ __forceinline__ void load_herm_rows(cpx<T> (&v)[VW][E], const cpx<T>*
 
 // ---------------------------------------------------------------------------
 // Poisson x pass (inverse, Hermitian-completed): V = IFFT_x(UU); with GRAD
-// a_x + i a_y = -IFFT_x(i kx UU + i Uy), a_z = -IFFT_x(Uzz), written as the
+// a_x + i a_y = -IFFT_x(i kx UU + i Uy), a_z = -IFFT_x(Uzz) (V and a_z from
+// one transform, V + i a_z = IFFT_x(UU - i Uzz)), written as the
 // cell-interleaved grid A3 = (a_x, a_y, a_z), and max |a| folded into
 // the running maximum diag[kDiagAMax] with one atomic per block.
 // quantum-stars.py:192 and :256-260.
@@ -584,38 +585,40 @@ struct k_poisson_x {
         }
       }
     } else {
+      // Two x transforms for four real outputs, all three spectra loaded up
+      // front: u = i kx UU + i Uy transforms to -(a_x + i a_y), and
+      // UU - i Uzz to V + i a_z (V and a_z are real: their spectra are
+      // Hermitian along x).
       cpx<T> u[VW][E];
+      load_herm_rows<N, VW, E>(u, Uy, base, se, j, TT, act);
+      {
+        cpx<T> z[VW][E];
+        load_herm_rows<N, VW, E>(z, Uzz, base, se, j, TT, act);
   #pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const int k = j + TT * m;
-        const T kx = (k == N / 2) ? (T)0 : (T)(kfx * (double)freq_index(k, N));
+        for (int m = 0; m < E; ++m) {
+          const int k = j + TT * m;
+          const T kx = (k == N / 2) ? (T)0 : (T)(kfx * (double)freq_index(k, N));
   #pragma unroll
-        for (int w = 0; w < VW; ++w) u[w][m] = {-kx * v[w][m].i, kx * v[w][m].r};
+          for (int w = 0; w < VW; ++w) {
+            u[w][m] = {-kx * v[w][m].i - u[w][m].i, kx * v[w][m].r + u[w][m].r};
+            v[w][m] = {v[w][m].r + z[w][m].i, v[w][m].i - z[w][m].r};
+          }
+        }
       }
-      pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
-  #pragma unroll
-      for (int m = 0; m < E; ++m) {
-        T t[VW];
-  #pragma unroll
-        for (int w = 0; w < VW; ++w) t[w] = v[w][m].r;
-        if (act) str<VW>(V, base + (j + TT * m) * se, t);
-      }
-      load_herm_rows<N, VW, E>(v, Uy, base, se, j, TT, act);
-  #pragma unroll
-      for (int m = 0; m < E; ++m)
-  #pragma unroll
-        for (int w = 0; w < VW; ++w) u[w][m] = {u[w][m].r - v[w][m].i, u[w][m].i + v[w][m].r};
       pencil_fft_strided<N, +1, T, VW>(u, j, col, lds, tw);
-      load_herm_rows<N, VW, E>(v, Uzz, base, se, j, TT, act);
       pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
       T amax = (T)0;
       if (act) {
   #pragma unroll
         for (int m = 0; m < E; ++m) {
           const int idx = base + (j + TT * m) * se;
+          T t[VW];
+  #pragma unroll
+          for (int w = 0; w < VW; ++w) t[w] = v[w][m].r;
+          str<VW>(V, idx, t);
   #pragma unroll
           for (int w = 0; w < VW; ++w) {
-            const T a0 = -u[w][m].r, a1 = -u[w][m].i, a2 = -v[w][m].r;
+            const T a0 = -u[w][m].r, a1 = -u[w][m].i, a2 = v[w][m].i;
             T* o = A3 + kAccComp * (idx + w);
             o[0] = a0;
             o[1] = a1;
