@@ -193,25 +193,74 @@ __device__ __forceinline__ void store_rows(const cpx<T> (&v)[VW][E], cpx<T>* __r
 // ---------------------------------------------------------------------------
 // drift sweep along a strided axis (x or y)
 // ---------------------------------------------------------------------------
+// With two pencils per lane a block walks pencil groups blockIdx.x,
+// blockIdx.x + gridDim.x, ... and loads the next group's rows while the
+// current group is transformed (the z sweep's scheme; LDS holds that shape at
+// two blocks per CU, so the second row set costs no occupancy).  With one
+// pencil per lane the second row set would cost occupancy, and each block
+// takes one group (the launch gives a block per group).
 template <class T, int N>
 struct k_drift_s {
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(cpx<T>* __restrict__ psi, PassGeom g,
                                              const cpx<T>* __restrict__ D,
                                              const cpx<T>* __restrict__ tw, int conj_d) {
-    QJ_STRIDED_SETUP(T, N)
-    cpx<T> v[VW][E];
-    load_rows<VW, E>(v, psi, base, se, j, TT, act);
-    pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
+    using S = PencilShape<N>;
+    constexpr int VW = StridedVW<T, N>::value;
+    constexpr int E = S::E;
+    constexpr int TT = S::TT;
+    __shared__ cpx<T> lds[S::kLdsStrided * VW];
+    const int col = threadIdx.x % S::C, j = threadIdx.x / S::C;
+    const int se = (int)g.s_elem;
+    auto at = [&](int grp, bool& act) {
+      const int q0 = (grp * S::C + col) * VW;
+      act = q0 < g.ncols;
+      return act ? (int)pencil_base(g, q0) : 0;
+    };
+    auto transform = [&](cpx<T>(&v)[VW][E]) {
+      pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
   #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const cpx<T> d0 = D[j + TT * m];
-      const cpx<T> d = conj_d ? conjg(d0) : d0;
+      for (int m = 0; m < E; ++m) {
+        const cpx<T> d0 = D[j + TT * m];
+        const cpx<T> d = conj_d ? conjg(d0) : d0;
   #pragma unroll
-      for (int w = 0; w < VW; ++w) v[w][m] = cmul(v[w][m], d);
+        for (int w = 0; w < VW; ++w) v[w][m] = cmul(v[w][m], d);
+      }
+      pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
+    };
+    if constexpr (VW == 2) {
+      const int ngroups = (g.ncols + S::C * VW - 1) / (S::C * VW);
+      cpx<T> nv[VW][E];
+      int grp = blockIdx.x;
+      if (grp < ngroups) {
+        bool a;
+        const int b = at(grp, a);
+        load_rows<VW, E>(nv, psi, b, se, j, TT, a);
+      }
+      for (; grp < ngroups; grp += gridDim.x) {
+        bool act;
+        const int base = at(grp, act);
+        cpx<T> v[VW][E];
+  #pragma unroll
+        for (int m = 0; m < E; ++m)
+  #pragma unroll
+          for (int w = 0; w < VW; ++w) v[w][m] = nv[w][m];
+        if (grp + (int)gridDim.x < ngroups) {
+          bool a;
+          const int b = at(grp + (int)gridDim.x, a);
+          load_rows<VW, E>(nv, psi, b, se, j, TT, a);
+        }
+        transform(v);
+        store_rows<VW, E>(v, psi, base, se, j, TT, act);
+      }
+    } else {
+      bool act;
+      const int base = at(blockIdx.x, act);
+      cpx<T> v[VW][E];
+      load_rows<VW, E>(v, psi, base, se, j, TT, act);
+      transform(v);
+      store_rows<VW, E>(v, psi, base, se, j, TT, act);
     }
-    pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
-    store_rows<VW, E>(v, psi, base, se, j, TT, act);
   }
 };
 
@@ -850,6 +899,13 @@ inline int nblocks_s(int ncols) {
   const int per = PencilShape<N>::C * StridedVW<T, N>::value;
   return (ncols + per - 1) / per;
 }
+// Blocks of a group-walking strided sweep (two pencils per lane): two per CU
+// (their LDS allows two), all resident at once; otherwise a block per group.
+template <class T, int N>
+inline int persist_blocks_s(int ncols) {
+  const int n = nblocks_s<T, N>(ncols);
+  return (StridedVW<T, N>::value == 2 && n > 512) ? 512 : n;
+}
 
 #define QJ_DISPATCH_N(n, ...)                      \
   switch (n) {                                     \
@@ -926,7 +982,7 @@ hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, v
     const cpx<T>* D = (const cpx<T>*)(y ? tb.drift_y : tb.drift_x);
     const cpx<T>* tw = (const cpx<T>*)(y ? tb.tw_y : tb.tw_x);
     QJ_DISPATCH_N(y ? d.ny : d.nx, {
-      launch<k_drift_s<T, N_>>(nblocks_s<T, N_>(g.ncols), 256, s, p, g, D, tw, conj_d);
+      launch<k_drift_s<T, N_>>(persist_blocks_s<T, N_>(g.ncols), 256, s, p, g, D, tw, conj_d);
     });
   }
   return hipGetLastError();
