@@ -191,14 +191,64 @@ __device__ __forceinline__ void store_rows(const cpx<T> (&v)[VW][E], cpx<T>* __r
 }
 
 // ---------------------------------------------------------------------------
-// drift sweep along a strided axis (x or y)
+// In-place strided sweep over all pencil groups: op(v, j, col) transforms a
+// group held in registers.  With two pencils per lane a block walks groups
+// blockIdx.x, blockIdx.x + gridDim.x, ... and loads the next group's rows
+// while the current group is transformed (the z sweep's scheme; LDS holds
+// that shape at two blocks per CU, so the second row set costs no
+// occupancy; the launch gives persist_blocks_s blocks).  With one pencil per
+// lane the second row set would cost occupancy, and each block takes one
+// group.
 // ---------------------------------------------------------------------------
-// With two pencils per lane a block walks pencil groups blockIdx.x,
-// blockIdx.x + gridDim.x, ... and loads the next group's rows while the
-// current group is transformed (the z sweep's scheme; LDS holds that shape at
-// two blocks per CU, so the second row set costs no occupancy).  With one
-// pencil per lane the second row set would cost occupancy, and each block
-// takes one group (the launch gives a block per group).
+template <class T, int N, class Op>
+__device__ __forceinline__ void strided_sweep(cpx<T>* __restrict__ a, const PassGeom& g, Op&& op) {
+  using S = PencilShape<N>;
+  constexpr int VW = StridedVW<T, N>::value;
+  constexpr int E = S::E;
+  constexpr int TT = S::TT;
+  const int col = threadIdx.x % S::C, j = threadIdx.x / S::C;
+  const int se = (int)g.s_elem;
+  auto at = [&](int grp, bool& act) {
+    const int q0 = (grp * S::C + col) * VW;
+    act = q0 < g.ncols;
+    return act ? (int)pencil_base(g, q0) : 0;
+  };
+  if constexpr (VW == 2) {
+    const int ngroups = (g.ncols + S::C * VW - 1) / (S::C * VW);
+    cpx<T> nv[VW][E];
+    int grp = blockIdx.x;
+    if (grp < ngroups) {
+      bool act;
+      const int base = at(grp, act);
+      load_rows<VW, E>(nv, a, base, se, j, TT, act);
+    }
+    for (; grp < ngroups; grp += gridDim.x) {
+      bool act;
+      const int base = at(grp, act);
+      cpx<T> v[VW][E];
+  #pragma unroll
+      for (int m = 0; m < E; ++m)
+  #pragma unroll
+        for (int w = 0; w < VW; ++w) v[w][m] = nv[w][m];
+      if (grp + (int)gridDim.x < ngroups) {
+        bool an;
+        const int bn = at(grp + (int)gridDim.x, an);
+        load_rows<VW, E>(nv, a, bn, se, j, TT, an);
+      }
+      op(v, j, col);
+      store_rows<VW, E>(v, a, base, se, j, TT, act);
+    }
+  } else {
+    bool act;
+    const int base = at(blockIdx.x, act);
+    cpx<T> v[VW][E];
+    load_rows<VW, E>(v, a, base, se, j, TT, act);
+    op(v, j, col);
+    store_rows<VW, E>(v, a, base, se, j, TT, act);
+  }
+}
+
+// drift sweep along a strided axis (x or y)
 template <class T, int N>
 struct k_drift_s {
   static constexpr int kThreads = 256;
@@ -207,60 +257,18 @@ struct k_drift_s {
                                              const cpx<T>* __restrict__ tw, int conj_d) {
     using S = PencilShape<N>;
     constexpr int VW = StridedVW<T, N>::value;
-    constexpr int E = S::E;
-    constexpr int TT = S::TT;
     __shared__ cpx<T> lds[S::kLdsStrided * VW];
-    const int col = threadIdx.x % S::C, j = threadIdx.x / S::C;
-    const int se = (int)g.s_elem;
-    auto at = [&](int grp, bool& act) {
-      const int q0 = (grp * S::C + col) * VW;
-      act = q0 < g.ncols;
-      return act ? (int)pencil_base(g, q0) : 0;
-    };
-    auto transform = [&](cpx<T>(&v)[VW][E]) {
+    strided_sweep<T, N>(psi, g, [&](cpx<T>(&v)[VW][S::E], int j, int col) {
       pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
   #pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const cpx<T> d0 = D[j + TT * m];
+      for (int m = 0; m < S::E; ++m) {
+        const cpx<T> d0 = D[j + S::TT * m];
         const cpx<T> d = conj_d ? conjg(d0) : d0;
   #pragma unroll
         for (int w = 0; w < VW; ++w) v[w][m] = cmul(v[w][m], d);
       }
       pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
-    };
-    if constexpr (VW == 2) {
-      const int ngroups = (g.ncols + S::C * VW - 1) / (S::C * VW);
-      cpx<T> nv[VW][E];
-      int grp = blockIdx.x;
-      if (grp < ngroups) {
-        bool a;
-        const int b = at(grp, a);
-        load_rows<VW, E>(nv, psi, b, se, j, TT, a);
-      }
-      for (; grp < ngroups; grp += gridDim.x) {
-        bool act;
-        const int base = at(grp, act);
-        cpx<T> v[VW][E];
-  #pragma unroll
-        for (int m = 0; m < E; ++m)
-  #pragma unroll
-          for (int w = 0; w < VW; ++w) v[w][m] = nv[w][m];
-        if (grp + (int)gridDim.x < ngroups) {
-          bool a;
-          const int b = at(grp + (int)gridDim.x, a);
-          load_rows<VW, E>(nv, psi, b, se, j, TT, a);
-        }
-        transform(v);
-        store_rows<VW, E>(v, psi, base, se, j, TT, act);
-      }
-    } else {
-      bool act;
-      const int base = at(blockIdx.x, act);
-      cpx<T> v[VW][E];
-      load_rows<VW, E>(v, psi, base, se, j, TT, act);
-      transform(v);
-      store_rows<VW, E>(v, psi, base, se, j, TT, act);
-    }
+    });
   }
 };
 
@@ -464,11 +472,12 @@ struct k_fft_s {
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(cpx<T>* __restrict__ a, PassGeom g,
                                              const cpx<T>* __restrict__ tw) {
-    QJ_STRIDED_SETUP(T, N)
-    cpx<T> v[VW][E];
-    load_rows<VW, E>(v, a, base, se, j, TT, act);
-    pencil_fft_strided<N, DIR, T, VW>(v, j, col, lds, tw);
-    store_rows<VW, E>(v, a, base, se, j, TT, act);
+    using S = PencilShape<N>;
+    constexpr int VW = StridedVW<T, N>::value;
+    __shared__ cpx<T> lds[S::kLdsStrided * VW];
+    strided_sweep<T, N>(a, g, [&](cpx<T>(&v)[VW][S::E], int j, int col) {
+      pencil_fft_strided<N, DIR, T, VW>(v, j, col, lds, tw);
+    });
   }
 };
 
@@ -1036,7 +1045,7 @@ hipError_t launch_fft_y_half(const Dims& d, const SpecTables<T>& tb, void* H, hi
   PassGeom g = geom_y(d, d.nx / 2 + 1);
   const cpx<T>* tw = (const cpx<T>*)tb.tw_y;
   QJ_DISPATCH_N(d.ny, {
-    launch<k_fft_s<T, N_, -1>>(nblocks_s<T, N_>(g.ncols), 256, s, (cpx<T>*)H, g, tw);
+    launch<k_fft_s<T, N_, -1>>(persist_blocks_s<T, N_>(g.ncols), 256, s, (cpx<T>*)H, g, tw);
   });
   return hipGetLastError();
 }
