@@ -815,9 +815,14 @@ int kdk_adjoint_t(qj_plan* p, const qj_state* st, const qj_params* pr, int nstep
   const bool binned = st->star_scratch && p->opt[QJ_OPT_BINNED_DEPOSIT];
   // one adjoint half step: the kick of psi (V, psi_ref) and of the stars
   // (A3, pos) with kick time hh (phase coefficient cc), the density and
-  // Poisson adjoints, the deposit adjoint
+  // Poisson adjoints, the deposit adjoint.  kick_done: the psi kick's adjoint
+  // (psi_bar phase and Vbar) was applied by the z sweep of the drift adjoint
+  // before it; defer_rho: the density adjoint is left to the x sweep of the
+  // drift adjoint after it (both fusions keep the formulas of the separate
+  // kernels)
   auto half = [&](const void* grid, const void* psi_ref, const double* pos, double dt_xv,
-                  double hh, double cc, const void* bins) -> int {
+                  double hh, double cc, const void* bins, bool kick_done,
+                  bool defer_rho) -> int {
     const void* V = grid;
     const void* A3 = (const char*)grid + (size_t)n * sizeof(T);
     if (stars && binned) {
@@ -837,13 +842,16 @@ int kdk_adjoint_t(qj_plan* p, const qj_state* st, const qj_params* pr, int nstep
                                       s),
                  "interp adjoint");
     }
-    QJ_STAGE(kStageAdjKick,
-             launch_kick_adj<T>(a->psi_bar, psi_ref, V, st->v_ext, cc, Vbar, n, s), "kick adjoint");
+    if (!kick_done)
+      QJ_STAGE(kStageAdjKick,
+               launch_kick_adj<T>(a->psi_bar, psi_ref, V, st->v_ext, cc, Vbar, n, s),
+               "kick adjoint");
     QJ_STAGE(kStageAdjPoisson,
              launch_adj_poisson<T>(p->d, tb, Vbar, abar3, pr->four_pi_G, pr->dc_mode, w.H, w.Uz,
                                    w.Uy, rhobar, s),
              "adjoint Poisson");
-    QJ_STAGE(kStageAdjRho, launch_rho_adj<T>(a->psi_bar, psi_ref, rhobar, n, s), "rho adjoint");
+    if (!defer_rho)
+      QJ_STAGE(kStageAdjRho, launch_rho_adj<T>(a->psi_bar, psi_ref, rhobar, n, s), "rho adjoint");
     if (stars)
       QJ_STAGE(kStageAdjParticles,
                launch_deposit_adj<T>(p->d, pos, st->mass, st->m_scalar, st->np, rhobar, a->m_bar,
@@ -858,22 +866,41 @@ int kdk_adjoint_t(qj_plan* p, const qj_state* st, const qj_params* pr, int nstep
   // the same before and after the common phase, and both star kicks see the
   // same v_bar.  One adjoint Poisson solve per step, as the forward merges
   // the kicks.
+  const void* rho_ref = nullptr;  // psi_ref of the deferred density adjoint
   for (int k = nsteps - 1; k >= 0; --k) {
     const qj_tape_step& t = tape[k];
     // closing half step of the last step
-    if (k == nsteps - 1)
-      if (int rc = half(t.grid1, t.psi2, t.x1, 0.0, h, c, t.bins)) return rc;
-    // drift: psi_bar <- F^H psi_bar (conjugate phase tables = drift of -dt);
-    // pos1 = pos0 + dt vel1 is folded into the next interp adjoint (dt_xv)
-    for (int ax = 0; ax < 3; ++ax)
-      QJ_STAGE(kStageAdjDrift,
-               launch_drift_axis<T>(p->d, tb, ax, a->psi_bar, nullptr, nullptr, nullptr, 0.0, 0, s, 1),
-               "drift adjoint");
-    if (k > 0) {  // opening half of step k with the closing half of step k-1
+    if (k == nsteps - 1) {
+      if (int rc = half(t.grid1, t.psi2, t.x1, 0.0, h, c, t.bins, false, true)) return rc;
+      rho_ref = t.psi2;
+    }
+    // the half step after the drift: opening half of step k with the closing
+    // half of step k-1, or the opening half step of the first step (grid0 at
+    // x0, psi0)
+    const void* ngrid = k > 0 ? tape[k - 1].grid1 : grid0;
+    const void* nref = k > 0 ? tape[k - 1].psi2 : t.psi0;
+    const double ncc = k > 0 ? 2.0 * c : c;
+    // drift: psi_bar <- F^H psi_bar (conjugate phase tables = drift of -dt),
+    // x sweep first (with the deferred density adjoint), z sweep last (with
+    // the next half step's kick adjoint); pos1 = pos0 + dt vel1 is folded
+    // into the next interp adjoint (dt_xv)
+    QJ_STAGE(kStageAdjDrift,
+             launch_drift_adj_axis<T>(p->d, tb, 0, a->psi_bar, rhobar, rho_ref, nullptr, nullptr,
+                                      0.0, nullptr, nullptr, s),
+             "drift adjoint");
+    QJ_STAGE(kStageAdjDrift,
+             launch_drift_axis<T>(p->d, tb, 1, a->psi_bar, nullptr, nullptr, nullptr, 0.0, 0, s, 1),
+             "drift adjoint");
+    QJ_STAGE(kStageAdjDrift,
+             launch_drift_adj_axis<T>(p->d, tb, 2, a->psi_bar, nullptr, nullptr, ngrid, st->v_ext,
+                                      ncc, nref, Vbar, s),
+             "drift adjoint");
+    if (k > 0) {
       const qj_tape_step& u = tape[k - 1];
-      if (int rc = half(u.grid1, u.psi2, u.x1, dt, 2.0 * h, 2.0 * c, u.bins)) return rc;
-    } else {  // opening half step of the first step (grid0 at x0, psi0)
-      if (int rc = half(grid0, t.psi0, t.x0, dt, h, c, nullptr)) return rc;
+      if (int rc = half(u.grid1, u.psi2, u.x1, dt, 2.0 * h, 2.0 * c, u.bins, true, true)) return rc;
+      rho_ref = u.psi2;
+    } else {
+      if (int rc = half(grid0, t.psi0, t.x0, dt, h, c, nullptr, true, false)) return rc;
     }
   }
   return QJ_OK;

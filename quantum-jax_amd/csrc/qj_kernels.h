@@ -120,6 +120,17 @@ hipError_t launch_kick_adj(void* psi_bar, const void* psi_ref, const void* V, co
 template <class T>
 hipError_t launch_rho_adj(void* psi_bar, const void* psi_ref, const void* rhobar, long long n,
                           hipStream_t s);
+// One axis of the drift adjoint (conjugate phase) on psi_bar, with
+//   x or y axis and rhobar given: the density adjoint psi_bar += 2 rhobar
+//     rho_ref applied to the rows before the transform (as launch_rho_adj);
+//   z axis and Vbar given: the kick adjoint of the next half step applied
+//     after it (as launch_kick_adj with V, Vx, coef and psi_ref = kick_ref).
+// Without them it is the plain conjugate drift of launch_drift_axis.
+template <class T>
+hipError_t launch_drift_adj_axis(const Dims& d, const SpecTables<T>& tb, int axis, void* psi_bar,
+                                 const void* rhobar, const void* rho_ref, const void* V,
+                                 const void* Vx, double coef, const void* kick_ref, void* Vbar,
+                                 hipStream_t s);
 // rhobar = P(Vbar + div abar), P the Poisson operator (kernel -4 pi G/k^2 with
 // the k = 0 bin -4 pi G for dc_mode 0, 0 for dc_mode 1), div the spectral
 // divergence with the Nyquist derivative zeroed; abar3 is (nx,ny,nz,3).

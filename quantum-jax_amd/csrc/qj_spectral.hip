@@ -77,7 +77,7 @@ struct k_kick {
 // ---------------------------------------------------------------------------
 // drift sweep along the contiguous axis, optionally preceded by the kick
 // ---------------------------------------------------------------------------
-template <class T, int N, bool KICK>
+template <class T, int N, bool KICK, bool KADJ = false>
 struct k_drift_c {
   static constexpr int kThreads = 256;
   // A block walks pencil groups blockIdx.x, blockIdx.x + gridDim.x, ...; the
@@ -88,7 +88,9 @@ struct k_drift_c {
                                              const cpx<T>* __restrict__ tw, const T* __restrict__ V,
                                              const T* __restrict__ Vb,
                                              const T* __restrict__ Vx, double kick_coef,
-                                             int conj_d, const double* __restrict__ dslot) {
+                                             int conj_d, const double* __restrict__ dslot,
+                                             const cpx<T>* __restrict__ pref,
+                                             T* __restrict__ Vbar) {
     using S = PencilShape<N>;
     __shared__ cpx<T> lds[S::kLds];
     if (KICK && dslot) kick_coef *= dslot[kSlotKick];  // kick time from the device step slot
@@ -137,6 +139,21 @@ struct k_drift_c {
         v[m] = cmul(v[m], conj_d ? conjg(d0) : d0);
       }
       pencil_fft<N, +1, T>(v, ln.j, my, tw);
+      if constexpr (KADJ) {
+        // adjoint of the next kick, psi_bar <- exp(+i c V) psi_bar, with
+        // Vbar = c Im(conj(psi_bar) psi_ref) (the formula of k_kick_adj)
+        if (act) {
+  #pragma unroll
+          for (int m = 0; m < S::E; ++m) {
+            const int idx = base + (ln.j + S::TT * m) * se;
+            const double vt = (double)V[idx] + (Vx ? (double)Vx[idx] : 0.0);
+            const cpx<T> b = cmulc(v[m], expmi<T>(kick_coef * vt));
+            v[m] = b;
+            const cpx<T> r = pref[idx];
+            Vbar[idx] = (T)(kick_coef * ((double)b.r * r.i - (double)b.i * r.r));
+          }
+        }
+      }
       if (act) {
   #pragma unroll
         for (int m = 0; m < S::E; ++m) psi[base + (ln.j + S::TT * m) * se] = v[m];
@@ -191,8 +208,8 @@ __device__ __forceinline__ void store_rows(const cpx<T> (&v)[VW][E], cpx<T>* __r
 }
 
 // ---------------------------------------------------------------------------
-// In-place strided sweep over all pencil groups: op(v, j, col) transforms a
-// group held in registers.  With two pencils per lane a block walks groups
+// In-place strided sweep over all pencil groups: op(v, j, col, base, act)
+// transforms a group held in registers.  With two pencils per lane a block walks groups
 // blockIdx.x, blockIdx.x + gridDim.x, ... and loads the next group's rows
 // while the current group is transformed (the z sweep's scheme; LDS holds
 // that shape at two blocks per CU, so the second row set costs no
@@ -235,7 +252,7 @@ __device__ __forceinline__ void strided_sweep(cpx<T>* __restrict__ a, const Pass
         const int bn = at(grp + (int)gridDim.x, an);
         load_rows<VW, E>(nv, a, bn, se, j, TT, an);
       }
-      op(v, j, col);
+      op(v, j, col, base, act);
       store_rows<VW, E>(v, a, base, se, j, TT, act);
     }
   } else {
@@ -243,22 +260,42 @@ __device__ __forceinline__ void strided_sweep(cpx<T>* __restrict__ a, const Pass
     const int base = at(blockIdx.x, act);
     cpx<T> v[VW][E];
     load_rows<VW, E>(v, a, base, se, j, TT, act);
-    op(v, j, col);
+    op(v, j, col, base, act);
     store_rows<VW, E>(v, a, base, se, j, TT, act);
   }
 }
 
-// drift sweep along a strided axis (x or y)
-template <class T, int N>
+// drift sweep along a strided axis (x or y); with RHO the density adjoint
+// psi_bar += 2 rhobar psi_ref (the formula of k_rho_adj) is applied to the
+// rows as they arrive
+template <class T, int N, bool RHO = false>
 struct k_drift_s {
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(cpx<T>* __restrict__ psi, PassGeom g,
                                              const cpx<T>* __restrict__ D,
-                                             const cpx<T>* __restrict__ tw, int conj_d) {
+                                             const cpx<T>* __restrict__ tw, int conj_d,
+                                             const T* __restrict__ rb,
+                                             const cpx<T>* __restrict__ pref) {
     using S = PencilShape<N>;
     constexpr int VW = StridedVW<T, N>::value;
     __shared__ cpx<T> lds[S::kLdsStrided * VW];
-    strided_sweep<T, N>(psi, g, [&](cpx<T>(&v)[VW][S::E], int j, int col) {
+    const int se = (int)g.s_elem;
+    strided_sweep<T, N>(psi, g, [&](cpx<T>(&v)[VW][S::E], int j, int col, int base, bool act) {
+      if constexpr (RHO) {
+        if (act) {
+  #pragma unroll
+          for (int m = 0; m < S::E; ++m) {
+            const int idx = base + (j + S::TT * m) * se;
+  #pragma unroll
+            for (int w = 0; w < VW; ++w) {
+              const T f = (T)2 * rb[idx + w];
+              const cpx<T> r = pref[idx + w];
+              v[w][m].r += f * r.r;
+              v[w][m].i += f * r.i;
+            }
+          }
+        }
+      }
       pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
   #pragma unroll
       for (int m = 0; m < S::E; ++m) {
@@ -475,7 +512,7 @@ struct k_fft_s {
     using S = PencilShape<N>;
     constexpr int VW = StridedVW<T, N>::value;
     __shared__ cpx<T> lds[S::kLdsStrided * VW];
-    strided_sweep<T, N>(a, g, [&](cpx<T>(&v)[VW][S::E], int j, int col) {
+    strided_sweep<T, N>(a, g, [&](cpx<T>(&v)[VW][S::E], int j, int col, int, bool) {
       pencil_fft_strided<N, DIR, T, VW>(v, j, col, lds, tw);
     });
   }
@@ -979,10 +1016,12 @@ hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, v
     QJ_DISPATCH_N(d.nz, {
       if (kick)
         launch<k_drift_c<T, N_, true>>(persist_blocks<N_>(g.ncols), 256, s, p, g, D, tw, v, vb,
-                                       vx, kick_coef, conj_d, dslot);
+                                       vx, kick_coef, conj_d, dslot, (const cpx<T>*)nullptr,
+                                       (T*)nullptr);
       else
         launch<k_drift_c<T, N_, false>>(persist_blocks<N_>(g.ncols), 256, s, p, g, D, tw, v, vb,
-                                        vx, kick_coef, conj_d, dslot);
+                                        vx, kick_coef, conj_d, dslot, (const cpx<T>*)nullptr,
+                                        (T*)nullptr);
     });
   } else {
     if (kick) return hipErrorInvalidValue;  // the kick is fused into the z sweep only
@@ -991,7 +1030,8 @@ hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, v
     const cpx<T>* D = (const cpx<T>*)(y ? tb.drift_y : tb.drift_x);
     const cpx<T>* tw = (const cpx<T>*)(y ? tb.tw_y : tb.tw_x);
     QJ_DISPATCH_N(y ? d.ny : d.nx, {
-      launch<k_drift_s<T, N_>>(persist_blocks_s<T, N_>(g.ncols), 256, s, p, g, D, tw, conj_d);
+      launch<k_drift_s<T, N_>>(persist_blocks_s<T, N_>(g.ncols), 256, s, p, g, D, tw, conj_d,
+                               (const T*)nullptr, (const cpx<T>*)nullptr);
     });
   }
   return hipGetLastError();
@@ -1112,6 +1152,37 @@ hipError_t launch_poisson_x(const Dims& d, const SpecTables<T>& tb, int grad, co
 }
 
 template <class T>
+hipError_t launch_drift_adj_axis(const Dims& d, const SpecTables<T>& tb, int axis, void* psi_bar,
+                                 const void* rhobar, const void* rho_ref, const void* V,
+                                 const void* Vx, double coef, const void* kick_ref, void* Vbar,
+                                 hipStream_t s) {
+  cpx<T>* p = (cpx<T>*)psi_bar;
+  if (axis == 2) {
+    if (!Vbar) return launch_drift_axis<T>(d, tb, 2, psi_bar, nullptr, nullptr, nullptr, 0.0, 0, s, 1);
+    PassGeom g = geom_z(d, d.nx);
+    const cpx<T>* D = (const cpx<T>*)tb.drift_z;
+    const cpx<T>* tw = (const cpx<T>*)tb.tw_z;
+    QJ_DISPATCH_N(d.nz, {
+      launch<k_drift_c<T, N_, false, true>>(persist_blocks<N_>(g.ncols), 256, s, p, g, D, tw,
+                                            (const T*)V, (const T*)nullptr, (const T*)Vx, coef, 1,
+                                            (const double*)nullptr, (const cpx<T>*)kick_ref,
+                                            (T*)Vbar);
+    });
+    return hipGetLastError();
+  }
+  if (!rhobar) return launch_drift_axis<T>(d, tb, axis, psi_bar, nullptr, nullptr, nullptr, 0.0, 0, s, 1);
+  const bool y = axis == 1;
+  PassGeom g = y ? geom_y(d, d.nx) : geom_x(d);
+  const cpx<T>* D = (const cpx<T>*)(y ? tb.drift_y : tb.drift_x);
+  const cpx<T>* tw = (const cpx<T>*)(y ? tb.tw_y : tb.tw_x);
+  QJ_DISPATCH_N(y ? d.ny : d.nx, {
+    launch<k_drift_s<T, N_, true>>(persist_blocks_s<T, N_>(g.ncols), 256, s, p, g, D, tw, 1,
+                                   (const T*)rhobar, (const cpx<T>*)rho_ref);
+  });
+  return hipGetLastError();
+}
+
+template <class T>
 hipError_t launch_kick_adj(void* psi_bar, const void* psi_ref, const void* V, const void* Vx,
                            double coef, void* Vbar, long long n, hipStream_t s) {
   long long blocks = (n + 255) / 256;
@@ -1202,6 +1273,10 @@ hipError_t launch_adj_poisson(const Dims& d, const SpecTables<T>& tb, const void
                                          void*,                                                 \
                                          long long, hipStream_t);                               \
   template hipError_t launch_rho_adj<T>(void*, const void*, const void*, long long, hipStream_t); \
+  template hipError_t launch_drift_adj_axis<T>(const Dims&, const SpecTables<T>&, int, void*,   \
+                                               const void*, const void*, const void*,           \
+                                               const void*, double, const void*, void*,         \
+                                               hipStream_t);                                    \
   template hipError_t launch_adj_poisson<T>(const Dims&, const SpecTables<T>&, const void*,     \
                                             const void*, double, int, void*, void*, void*,      \
                                             void*, hipStream_t);
