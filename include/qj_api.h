@@ -228,8 +228,9 @@ int qj_kdk_finalize(qj_plan* plan, qj_state* state, const qj_params* prm, void* 
  * does, and the state is finalized at the end) and stores, per step, psi
  * after the drift (psi2), the
  * grid after the step's Poisson solve (grid1, 4N reals as in qj_state), the
- * star positions before and after the drift (x0, x1), and psi at the start
- * (psi0: needed for step 0 only, NULL allowed for later steps); grid0
+ * star positions after the drift (x1), and the star positions and psi at
+ * the start (x0, psi0: needed for step 0 only, NULL allowed for later steps:
+ * step k starts from x1 and psi2 of step k - 1); grid0
  * (nullable) receives the grid at the start.  The adjoint merges the
  * closing half kick of each step with the opening one of the next (one
  * adjoint Poisson solve per step).

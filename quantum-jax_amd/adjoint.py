@@ -72,8 +72,8 @@ class _Tape:
         G = (1 + _lib.ACC_COMPONENTS) * N  # V, then the acceleration grid
         self.grid1 = torch.empty((nsteps, G), dtype=plan.real_dtype, device=dev)
         self.grid0 = torch.empty(G, dtype=plan.real_dtype, device=dev)
-        self.x0 = torch.empty((nsteps, max(n_p, 1), 3), dtype=torch.float64, device=dev)
-        self.x1 = torch.empty_like(self.x0)
+        self.x0 = torch.empty((1, max(n_p, 1), 3), dtype=torch.float64, device=dev)
+        self.x1 = torch.empty((nsteps, max(n_p, 1), 3), dtype=torch.float64, device=dev)
         # tile registrations of x1 per step (reused by the adjoint's deposit)
         nb = int(_lib.lib().qj_star_scratch_size(plan.handle, n_p)) if n_p else 0
         self.bins = torch.empty((nsteps, max(nb, 1)), dtype=torch.uint8, device=dev)
@@ -83,7 +83,7 @@ class _Tape:
             t.psi0 = self.psi0[0].data_ptr() if k == 0 else None
             t.psi2 = self.psi2[k].data_ptr()
             t.grid1 = self.grid1[k].data_ptr()
-            t.x0 = self.x0[k].data_ptr() if n_p else None
+            t.x0 = self.x0[0].data_ptr() if n_p and k == 0 else None
             t.x1 = self.x1[k].data_ptr() if n_p else None
             t.bins = self.bins[k].data_ptr() if n_p else None
 

@@ -581,10 +581,18 @@ int initial_grid_t(qj_plan* p, qj_state* st, const qj_params* pr, const Work& w,
 // there and st->grid then points to it (the tape recorder uses this to
 // write each step's grid straight into its tape).
 template <class T>
+// grid_out / psi_out (one step, the tape recorder): the Poisson solve writes
+// its grid to grid_out, and psi after the drift goes to psi_out (the z sweep
+// reads st->psi and writes psi_mid, the x sweep reads psi_mid and writes
+// psi_out; fused x-source schedule only); st->grid / st->psi then point there.
 int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void* work,
                 hipStream_t s, const double* kick_dt = nullptr, const double* drift_dt = nullptr,
-                const double* dslot = nullptr, void* grid_out = nullptr) {
-  if (grid_out && nsteps != 1) return fail(QJ_EARG, "kdk_steps_t: grid_out needs one step");
+                const double* dslot = nullptr, void* grid_out = nullptr, void* psi_mid = nullptr,
+                void* psi_out = nullptr) {
+  if ((grid_out || psi_out) && nsteps != 1)
+    return fail(QJ_EARG, "kdk_steps_t: grid_out / psi_out need one step");
+  if (psi_out && (!psi_mid || !p->opt[QJ_OPT_FUSE_X_SOURCE]))
+    return fail(QJ_EARG, "kdk_steps_t: psi_out needs psi_mid and the fused x-source schedule");
   Work w = carve(p, work);
   SpecTables<T> tb = tables<T>(p);
   if (!drift_dt && !dslot)
@@ -653,18 +661,21 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
       st->gas_swapped ^= 1;
     }
     // psi: kick folded into the z drift sweep, then y and x (+ Poisson source)
+    void* mid = (psi_out && psi_mid != st->psi) ? psi_mid : nullptr;  // z sweep out of place
     QJ_STAGE(kStageDriftZKick,
              launch_drift_axis<T>(p->d, tb, 2, st->psi, V, brho, st->v_ext,
-                                  pr->m_per_hbar * (dslot ? 1.0 : kick_t), 1, s, 0, dslot),
+                                  pr->m_per_hbar * (dslot ? 1.0 : kick_t), 1, s, 0, dslot, mid),
              "drift z");
+    if (mid) st->psi = mid;
     QJ_STAGE(kStageDriftY, launch_drift_axis<T>(p->d, tb, 1, st->psi, nullptr, nullptr, nullptr, 0.0, 0, s),
              "drift y");
     src = stars ? st->rho_s : (gas ? live_gas(0) : nullptr);
     if (p->opt[QJ_OPT_FUSE_X_SOURCE]) {
       QJ_STAGE(kStageDriftXSource,
                launch_source_x<T>(p->d, tb, 1, st->psi, src, clear, brho, pr->b_coeff,
-                                  pr->rho_bar, w.H, p->partials, st->diag, s),
+                                  pr->rho_bar, w.H, p->partials, st->diag, s, psi_out),
                "drift x + source");
+      if (psi_out) st->psi = psi_out;
     } else {
       QJ_STAGE(kStageDriftX,
                launch_drift_axis<T>(p->d, tb, 0, st->psi, nullptr, nullptr, nullptr, 0.0, 0, s), "drift x");
@@ -769,10 +780,16 @@ int kdk_record_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, qj_t
     if (int rc = initial_grid_t<T>(p, st, pr, w, s)) return rc;
   if (grid0) QJ_CHECK(copy_async(grid0, st->grid, grid_bytes(p), s), "tape grid0");
   // each step's Poisson solve writes its grid straight into the tape (the
-  // next step reads it there); the state's own grid gets the last one back
+  // next step reads it there); the state's own grid gets the last one back.
+  // With the fused x-source schedule psi after the drift goes straight into
+  // the tape as well: the x sweep writes psi2 of the step, the next step's z
+  // sweep reads it there and writes the state's own psi.
   void* const own_grid = st->grid;
+  void* const own_psi = st->psi;
+  const bool psi_direct = p->opt[QJ_OPT_FUSE_X_SOURCE] != 0;
   int rc = QJ_OK;
-  // (every exit below restores st->grid: the tape buffers are the caller's)
+  // (every exit below restores st->grid and st->psi: the tape buffers are
+  // the caller's)
   auto copy = [&](void* dst, const void* src, size_t bytes, const char* what) {
     const hipError_t e = copy_async(dst, src, bytes, s);
     if (e != hipSuccess) rc = hipfail(e, what);
@@ -780,11 +797,13 @@ int kdk_record_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, qj_t
   for (int k = 0; k < nsteps && rc == QJ_OK; ++k) {
     qj_tape_step& t = tape[k];
     if (t.psi0) copy(t.psi0, st->psi, cb, "tape psi0");
-    if (rc == QJ_OK && st->np > 0) copy(t.x0, st->pos, xb, "tape x0");
+    if (rc == QJ_OK && st->np > 0 && t.x0) copy(t.x0, st->pos, xb, "tape x0");
     // merged steps, as the forward runs them: the closing half kick of step k
     // is owed to (and merged into) the opening kick of step k + 1
-    if (rc == QJ_OK) rc = kdk_steps_t<T>(p, st, pr, 1, work, s, nullptr, nullptr, nullptr, t.grid1);
-    if (rc == QJ_OK) copy(t.psi2, st->psi, cb, "tape psi2");
+    if (rc == QJ_OK)
+      rc = kdk_steps_t<T>(p, st, pr, 1, work, s, nullptr, nullptr, nullptr, t.grid1,
+                          psi_direct ? own_psi : nullptr, psi_direct ? t.psi2 : nullptr);
+    if (rc == QJ_OK && !psi_direct) copy(t.psi2, st->psi, cb, "tape psi2");
     if (rc == QJ_OK && st->np > 0) copy(t.x1, st->pos, xb, "tape x1");
     // the tile registrations the step's binned deposit made of x1
     if (rc == QJ_OK && t.bins && st->np > 0 && st->star_scratch && p->opt[QJ_OPT_BINNED_DEPOSIT])
@@ -793,7 +812,12 @@ int kdk_record_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, qj_t
   if (st->grid != own_grid) {
     const hipError_t e = copy_async(own_grid, st->grid, grid_bytes(p), s);
     st->grid = own_grid;
-    if (e != hipSuccess && rc == QJ_OK) return hipfail(e, "tape: grid back into the state");
+    if (e != hipSuccess && rc == QJ_OK) rc = hipfail(e, "tape: grid back into the state");
+  }
+  if (st->psi != own_psi) {
+    const hipError_t e = copy_async(own_psi, st->psi, cb, s);
+    st->psi = own_psi;
+    if (e != hipSuccess && rc == QJ_OK) rc = hipfail(e, "tape: psi back into the state");
   }
   if (rc) return rc;
   return kdk_finalize_t<T>(p, st, pr, s);
@@ -1022,7 +1046,7 @@ int qj_kdk_record(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, qj_
     return fail(QJ_EARG, "qj_kdk_record: finalize the state first (owed half kick)");
   for (int k = 0; k < nsteps; ++k)
     if ((k == 0 && !tape[k].psi0) || !tape[k].psi2 || !tape[k].grid1 ||
-        (st->np > 0 && (!tape[k].x0 || !tape[k].x1)))
+        (st->np > 0 && ((k == 0 && !tape[k].x0) || !tape[k].x1)))
       return fail(QJ_EARG, "qj_kdk_record: null tape buffer");
   hipStream_t s = (hipStream_t)stream;
   return p->dtype == QJ_C64 ? kdk_record_t<float>(p, st, pr, nsteps, tape, grid0, work, s)
@@ -1041,6 +1065,8 @@ int qj_kdk_adjoint(qj_plan* p, const qj_state* st, const qj_params* pr, int nste
   if (st->gas[0] || pr->b_coeff != 0.0)
     return fail(QJ_EUNSUPPORTED, "qj_kdk_adjoint: gas and self-interaction have no adjoint");
   if (nsteps > 0 && !tape[0].psi0) return fail(QJ_EARG, "qj_kdk_adjoint: null psi0 of step 0");
+  if (nsteps > 0 && st->np > 0 && !tape[0].x0)
+    return fail(QJ_EARG, "qj_kdk_adjoint: null x0 of step 0");
   hipStream_t s = (hipStream_t)stream;
   return p->dtype == QJ_C64
              ? kdk_adjoint_t<float>(p, st, pr, nsteps, tape, grid0, adj, work, s)

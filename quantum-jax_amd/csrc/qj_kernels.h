@@ -41,15 +41,18 @@ hipError_t launch_kick(void* psi, const void* V, const void* Vb, const void* Vx,
 
 // conj_d = 1 applies the conjugate phase table (the drift of -dt).
 template <class T>
+// psi_out (z axis only): the sweep reads psi and writes psi_out.
 hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, void* psi,
                              const void* V, const void* Vb, const void* Vx, double kick_coef,
                              int kick, hipStream_t s, int conj_d = 0,
-                             const double* dslot = nullptr);
+                             const double* dslot = nullptr, void* psi_out = nullptr);
 
 template <class T>
+// psi_out (with drift): the drifted psi goes to psi_out instead of psi.
 hipError_t launch_source_x(const Dims& d, const SpecTables<T>& tb, int drift, void* psi,
                            void* src, int clear_src, void* brho, double b_coeff, double rho_bar,
-                           void* H, double* partials, double* diag, hipStream_t s);
+                           void* H, double* partials, double* diag, hipStream_t s,
+                           void* psi_out = nullptr);
 
 template <class T>
 hipError_t launch_fft_y_half(const Dims& d, const SpecTables<T>& tb, void* H, hipStream_t s);

@@ -90,7 +90,7 @@ struct k_drift_c {
                                              const T* __restrict__ Vx, double kick_coef,
                                              int conj_d, const double* __restrict__ dslot,
                                              const cpx<T>* __restrict__ pref,
-                                             T* __restrict__ Vbar) {
+                                             T* __restrict__ Vbar, cpx<T>* __restrict__ pout) {
     using S = PencilShape<N>;
     __shared__ cpx<T> lds[S::kLds];
     if (KICK && dslot) kick_coef *= dslot[kSlotKick];  // kick time from the device step slot
@@ -155,8 +155,10 @@ struct k_drift_c {
         }
       }
       if (act) {
+        // out of place when pout is given (the rows were read from psi)
+        cpx<T>* dst = pout ? pout : psi;
   #pragma unroll
-        for (int m = 0; m < S::E; ++m) psi[base + (ln.j + S::TT * m) * se] = v[m];
+        for (int m = 0; m < S::E; ++m) dst[base + (ln.j + S::TT * m) * se] = v[m];
       }
     }
   }
@@ -385,7 +387,8 @@ struct k_source_x {
                                              const cpx<T>* __restrict__ D,
                                              const cpx<T>* __restrict__ tw, cpx<T>* __restrict__ H,
                                              double* __restrict__ partials,
-                                             double* __restrict__ diag) {
+                                             double* __restrict__ diag,
+                                             cpx<T>* __restrict__ pout) {
     QJ_STRIDED_SETUP(T, N)
     cpx<T> v[VW][E];
     T rho[VW][E];
@@ -417,7 +420,8 @@ struct k_source_x {
           for (int w = 0; w < VW; ++w) v[w][m] = cmul(v[w][m], d);
         }
         pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
-        store_rows<VW, E>(v, psi, base, se, j, TT, act);
+        // out of place when pout is given (the rows were read from psi)
+        store_rows<VW, E>(v, pout ? pout : psi, base, se, j, TT, act);
       }
   #pragma unroll
       for (int m = 0; m < E; ++m)
@@ -1004,7 +1008,8 @@ hipError_t launch_kick(void* psi, const void* V, const void* Vb, const void* Vx,
 template <class T>
 hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, void* psi,
                              const void* V, const void* Vb, const void* Vx, double kick_coef,
-                             int kick, hipStream_t s, int conj_d, const double* dslot) {
+                             int kick, hipStream_t s, int conj_d, const double* dslot,
+                             void* psi_out) {
   cpx<T>* p = (cpx<T>*)psi;
   const T* v = (const T*)V;
   const T* vb = (const T*)Vb;
@@ -1017,14 +1022,15 @@ hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, v
       if (kick)
         launch<k_drift_c<T, N_, true>>(persist_blocks<N_>(g.ncols), 256, s, p, g, D, tw, v, vb,
                                        vx, kick_coef, conj_d, dslot, (const cpx<T>*)nullptr,
-                                       (T*)nullptr);
+                                       (T*)nullptr, (cpx<T>*)psi_out);
       else
         launch<k_drift_c<T, N_, false>>(persist_blocks<N_>(g.ncols), 256, s, p, g, D, tw, v, vb,
                                         vx, kick_coef, conj_d, dslot, (const cpx<T>*)nullptr,
-                                        (T*)nullptr);
+                                        (T*)nullptr, (cpx<T>*)psi_out);
     });
   } else {
-    if (kick) return hipErrorInvalidValue;  // the kick is fused into the z sweep only
+    // the kick is fused into the z sweep only, and only the z sweep runs out of place
+    if (kick || psi_out) return hipErrorInvalidValue;
     const bool y = axis == 1;
     PassGeom g = y ? geom_y(d, d.nx) : geom_x(d);
     const cpx<T>* D = (const cpx<T>*)(y ? tb.drift_y : tb.drift_x);
@@ -1063,7 +1069,9 @@ int source_x_blocks(const Dims& d) {
 template <class T>
 hipError_t launch_source_x(const Dims& d, const SpecTables<T>& tb, int drift, void* psi,
                            void* src, int clear_src, void* brho, double b_coeff, double rho_bar,
-                           void* H, double* partials, double* diag, hipStream_t s) {
+                           void* H, double* partials, double* diag, hipStream_t s,
+                           void* psi_out) {
+  if (psi_out && !drift) return hipErrorInvalidValue;  // only the drift writes psi
   PassGeom g = geom_x(d);
   const cpx<T>* D = (const cpx<T>*)tb.drift_x;
   const cpx<T>* tw = (const cpx<T>*)tb.tw_x;
@@ -1071,11 +1079,11 @@ hipError_t launch_source_x(const Dims& d, const SpecTables<T>& tb, int drift, vo
     if (drift)
       launch<k_source_x<T, N_, true>>(nblocks_s<T, N_>(g.ncols), 256, s,
           (cpx<T>*)psi, (T*)src, clear_src, (T*)brho, b_coeff, rho_bar, g, D, tw, (cpx<T>*)H,
-          partials, diag);
+          partials, diag, (cpx<T>*)psi_out);
     else
       launch<k_source_x<T, N_, false>>(nblocks_s<T, N_>(g.ncols), 256, s,
           (cpx<T>*)psi, (T*)src, clear_src, (T*)brho, b_coeff, rho_bar, g, D, tw, (cpx<T>*)H,
-          partials, diag);
+          partials, diag, (cpx<T>*)psi_out);
   });
   return hipGetLastError();
 }
@@ -1166,7 +1174,7 @@ hipError_t launch_drift_adj_axis(const Dims& d, const SpecTables<T>& tb, int axi
       launch<k_drift_c<T, N_, false, true>>(persist_blocks<N_>(g.ncols), 256, s, p, g, D, tw,
                                             (const T*)V, (const T*)nullptr, (const T*)Vx, coef, 1,
                                             (const double*)nullptr, (const cpx<T>*)kick_ref,
-                                            (T*)Vbar);
+                                            (T*)Vbar, (cpx<T>*)nullptr);
     });
     return hipGetLastError();
   }
@@ -1253,10 +1261,10 @@ hipError_t launch_adj_poisson(const Dims& d, const SpecTables<T>& tb, const void
                                      long long, hipStream_t);                                   \
   template hipError_t launch_drift_axis<T>(const Dims&, const SpecTables<T>&, int, void*,        \
                                            const void*, const void*, const void*, double, int,  \
-                                           hipStream_t, int, const double*);                    \
+                                           hipStream_t, int, const double*, void*);             \
   template hipError_t launch_source_x<T>(const Dims&, const SpecTables<T>&, int, void*, void*,  \
                                          int, void*, double, double, void*, double*, double*,   \
-                                         hipStream_t);                                          \
+                                         hipStream_t, void*);                                   \
   template hipError_t launch_fft_y_half<T>(const Dims&, const SpecTables<T>&, void*,            \
                                            hipStream_t);                                        \
   template hipError_t launch_poisson_z<T>(const Dims&, const SpecTables<T>&, int, void*, void*, \
