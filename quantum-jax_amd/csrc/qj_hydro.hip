@@ -117,9 +117,17 @@ __device__ __forceinline__ void face_flux(T rL, T uL, T vL, T wL, T rR, T uR, T 
 // column's predicted state and x slopes for planes x-1, x, x+1 in registers.
 // Each cell state is evaluated once (plus the ring), instead of seven times.
 // ---------------------------------------------------------------------------
-constexpr int kHT = 16;                 // max tile edge (y and z)
-constexpr int kHR = kHT + 4;            // raw region edge (2-cell halo)
-constexpr int kHS = kHT + 2;            // state region edge (1-cell ring)
+// Max tile edge (y and z) per precision: FP32 runs 8 x 8 tiles (19 KB of LDS
+// per block, 0.66 ms at 256^3, where 16 x 16 tiles with 57 KB, two blocks
+// per CU, ran 0.84 ms); FP64 keeps 16 x 16 (its whole c128 gas step 2.31 ms
+// against 2.43 with 8 x 8 tiles).
+template <class T>
+struct HTile {
+  static constexpr int T_ = sizeof(T) == 4 ? 8 : 16;   // tile edge
+  static constexpr int R = T_ + 4;                     // raw region edge (2-cell halo)
+  static constexpr int S = T_ + 2;                     // state region edge (1-cell ring)
+  static constexpr int threads = (S * S + 63) / 64 * 64;  // tile cells + ring, whole waves
+};
 
 // (a_x, a_y, a_z) of cell i of the acceleration grid
 template <class T>
@@ -138,7 +146,7 @@ struct XState {
 
 template <class T, int FLUX>
 struct k_hydro_stream {
-  static constexpr int kThreads = 384;
+  static constexpr int kThreads = HTile<T>::threads;
   static __device__ __forceinline__ void run(HGeom<T> g, int TY, int TZ, int XC,
                                              const T* __restrict__ rho, const T* __restrict__ vx,
                                              const T* __restrict__ vy, const T* __restrict__ vz,
@@ -150,8 +158,8 @@ struct k_hydro_stream {
       g.dt = (T)dslot[kSlotDt];
       kick = (T)dslot[kSlotKick];
     }
-    __shared__ T raw[4][4][kHR * kHR];     // [slot][var][cell]
-    __shared__ T st[2][12][kHS * kHS];     // [buf][p0..3, sy0..3, sz0..3][cell]
+    __shared__ T raw[4][4][HTile<T>::R * HTile<T>::R];  // [slot][var][cell]
+    __shared__ T st[2][12][HTile<T>::S * HTile<T>::S];  // [buf][p0..3, sy0..3, sz0..3][cell]
     const int tid = threadIdx.x;
     const int nthr = TY * TZ;
     const int RZ = TZ + 4, RY = TY + 4, SZ = TZ + 2, SY = TY + 2;
@@ -504,11 +512,13 @@ hipError_t launch_hydro(const Dims& d, const void* rho, const void* vx, const vo
   g.dt = (T)dt;
   g.cs = (T)cs;
   g.vol = (T)(dx * dx * dx);
-  const int TY = d.ny < kHT ? d.ny : kHT, TZ = d.nz < kHT ? d.nz : kHT;
+  constexpr int HT = HTile<T>::T_;
+  const int TY = d.ny < HT ? d.ny : HT, TZ = d.nz < HT ? d.nz : HT;
   const int XC = d.nx < 32 ? d.nx : 32;
   const int nblk = (d.ny / TY) * (d.nz / TZ) * (d.nx / XC);
   const int nthr = ((TY + 2) * (TZ + 2) + 63) / 64 * 64;  // tile cells + ring, whole waves
   if ((TY + 4) * (TZ + 4) > 2 * nthr) return hipErrorInvalidConfiguration;  // 2 cells per thread
+  if (nthr > k_hydro_stream<T, QJ_FLUX_HLL>::kThreads) return hipErrorInvalidConfiguration;
   if (flux_kind == QJ_FLUX_HLL)
     launch<k_hydro_stream<T, QJ_FLUX_HLL>>(nblk, nthr, s, g, TY, TZ, XC, (const T*)rho,
                                            (const T*)vx, (const T*)vy, (const T*)vz,
