@@ -117,16 +117,17 @@ __device__ __forceinline__ void face_flux(T rL, T uL, T vL, T wL, T rR, T uR, T 
 // column's predicted state and x slopes for planes x-1, x, x+1 in registers.
 // Each cell state is evaluated once (plus the ring), instead of seven times.
 // ---------------------------------------------------------------------------
-// Max tile edge (y and z) per precision: FP32 runs 8 x 8 tiles (19 KB of LDS
-// per block, 0.66 ms at 256^3, where 16 x 16 tiles with 57 KB, two blocks
-// per CU, ran 0.84 ms); FP64 keeps 16 x 16 (its whole c128 gas step 2.31 ms
-// against 2.43 with 8 x 8 tiles).
+// Max tile edges (y, z) per precision: FP32 runs 8 x 16 tiles (33 KB of LDS
+// per 192-thread block, 0.58 ms at 256^3 with HLL; 8 x 8 ran 0.64 ms and
+// 16 x 16, 57 KB and two blocks per CU, 0.84 ms); FP64 keeps 16 x 16 (its
+// whole c128 gas step ran 2.31 ms with them and 2.43 ms with 8 x 8).
 template <class T>
 struct HTile {
-  static constexpr int T_ = sizeof(T) == 4 ? 8 : 16;   // tile edge
-  static constexpr int R = T_ + 4;                     // raw region edge (2-cell halo)
-  static constexpr int S = T_ + 2;                     // state region edge (1-cell ring)
-  static constexpr int threads = (S * S + 63) / 64 * 64;  // tile cells + ring, whole waves
+  static constexpr int Y = sizeof(T) == 4 ? 8 : 16;    // tile edges
+  static constexpr int Z = 16;
+  static constexpr int RY = Y + 4, RZ = Z + 4;         // raw region (2-cell halo)
+  static constexpr int SY = Y + 2, SZ = Z + 2;         // state region (1-cell ring)
+  static constexpr int threads = (SY * SZ + 63) / 64 * 64;  // tile cells + ring, whole waves
 };
 
 // (a_x, a_y, a_z) of cell i of the acceleration grid
@@ -158,8 +159,8 @@ struct k_hydro_stream {
       g.dt = (T)dslot[kSlotDt];
       kick = (T)dslot[kSlotKick];
     }
-    __shared__ T raw[4][4][HTile<T>::R * HTile<T>::R];  // [slot][var][cell]
-    __shared__ T st[2][12][HTile<T>::S * HTile<T>::S];  // [buf][p0..3, sy0..3, sz0..3][cell]
+    __shared__ T raw[4][4][HTile<T>::RY * HTile<T>::RZ];  // [slot][var][cell]
+    __shared__ T st[2][12][HTile<T>::SY * HTile<T>::SZ];  // [buf][p0..3, sy0..3, sz0..3][cell]
     const int tid = threadIdx.x;
     const int nthr = TY * TZ;
     const int RZ = TZ + 4, RY = TY + 4, SZ = TZ + 2, SY = TY + 2;
@@ -512,8 +513,8 @@ hipError_t launch_hydro(const Dims& d, const void* rho, const void* vx, const vo
   g.dt = (T)dt;
   g.cs = (T)cs;
   g.vol = (T)(dx * dx * dx);
-  constexpr int HT = HTile<T>::T_;
-  const int TY = d.ny < HT ? d.ny : HT, TZ = d.nz < HT ? d.nz : HT;
+  const int TY = d.ny < HTile<T>::Y ? d.ny : HTile<T>::Y;
+  const int TZ = d.nz < HTile<T>::Z ? d.nz : HTile<T>::Z;
   const int XC = d.nx < 32 ? d.nx : 32;
   const int nblk = (d.ny / TY) * (d.nz / TZ) * (d.nx / XC);
   const int nthr = ((TY + 2) * (TZ + 2) + 63) / 64 * 64;  // tile cells + ring, whole waves
