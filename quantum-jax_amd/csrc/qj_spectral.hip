@@ -389,7 +389,19 @@ struct k_source_x {
                                              double* __restrict__ partials,
                                              double* __restrict__ diag,
                                              cpx<T>* __restrict__ pout) {
-    QJ_STRIDED_SETUP(T, N)
+    // one exchange buffer that the lane's pencils take turns with (half the
+    // LDS of QJ_STRIDED_SETUP): the pass holds 162 registers, so the LDS
+    // decides between two and three waves per SIMD
+    using S = PencilShape<N>;
+    constexpr int VW = StridedVW<T, N>::value;
+    constexpr int E = S::E;
+    constexpr int TT = S::TT;
+    __shared__ cpx<T> lds[S::kLdsStrided];
+    const int col = threadIdx.x % S::C, j = threadIdx.x / S::C;
+    const int q0 = (blockIdx.x * S::C + col) * VW;
+    const bool act = q0 < g.ncols;
+    const int base = act ? (int)pencil_base(g, q0) : 0;
+    const int se = (int)g.s_elem;
     cpx<T> v[VW][E];
     T rho[VW][E];
   #pragma unroll
@@ -412,14 +424,14 @@ struct k_source_x {
     if (psi) {
       load_rows<VW, E>(v, psi, base, se, j, TT, act);
       if constexpr (DRIFT) {
-        pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
+        pencil_fft_strided<N, -1, T, VW, true>(v, j, col, lds, tw);
   #pragma unroll
         for (int m = 0; m < E; ++m) {
           const cpx<T> d = D[j + TT * m];
   #pragma unroll
           for (int w = 0; w < VW; ++w) v[w][m] = cmul(v[w][m], d);
         }
-        pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
+        pencil_fft_strided<N, +1, T, VW, true>(v, j, col, lds, tw);
         // out of place when pout is given (the rows were read from psi)
         store_rows<VW, E>(v, pout ? pout : psi, base, se, j, TT, act);
       }
@@ -487,7 +499,7 @@ struct k_source_x {
         }
       }
     } else {
-      pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
+      pencil_fft_strided<N, -1, T, VW, true>(v, j, col, lds, tw);
       if (act) {
   #pragma unroll
         for (int m = 0; m < E; ++m) {
