@@ -210,14 +210,13 @@ __device__ __forceinline__ void store_rows(const cpx<T> (&v)[VW][E], cpx<T>* __r
 }
 
 // ---------------------------------------------------------------------------
-// In-place strided sweep over all pencil groups: op(v, j, col, base, act)
-// transforms a group held in registers.  With two pencils per lane a block walks groups
-// blockIdx.x, blockIdx.x + gridDim.x, ... and loads the next group's rows
-// while the current group is transformed (the z sweep's scheme; LDS holds
-// that shape at two blocks per CU, so the second row set costs no
-// occupancy; the launch gives persist_blocks_s blocks).  With one pencil per
-// lane the second row set would cost occupancy, and each block takes one
-// group.
+// In-place strided sweep of one pencil group per block: op(v, j, col, base,
+// act) transforms the group held in registers.  The kernels that use it
+// exchange through one LDS buffer that a lane's pencils take turns with (the
+// transform's phased form), so their LDS (34.8 KB at 256 points) and
+// registers allow four waves per SIMD; walking several groups per block
+// with the next group's rows prefetched, at two waves per SIMD, was slower
+// (drift_y 56 us against 50 at 256^3 c64).
 // ---------------------------------------------------------------------------
 template <class T, int N, class Op>
 __device__ __forceinline__ void strided_sweep(cpx<T>* __restrict__ a, const PassGeom& g, Op&& op) {
@@ -227,44 +226,13 @@ __device__ __forceinline__ void strided_sweep(cpx<T>* __restrict__ a, const Pass
   constexpr int TT = S::TT;
   const int col = threadIdx.x % S::C, j = threadIdx.x / S::C;
   const int se = (int)g.s_elem;
-  auto at = [&](int grp, bool& act) {
-    const int q0 = (grp * S::C + col) * VW;
-    act = q0 < g.ncols;
-    return act ? (int)pencil_base(g, q0) : 0;
-  };
-  if constexpr (VW == 2) {
-    const int ngroups = (g.ncols + S::C * VW - 1) / (S::C * VW);
-    cpx<T> nv[VW][E];
-    int grp = blockIdx.x;
-    if (grp < ngroups) {
-      bool act;
-      const int base = at(grp, act);
-      load_rows<VW, E>(nv, a, base, se, j, TT, act);
-    }
-    for (; grp < ngroups; grp += gridDim.x) {
-      bool act;
-      const int base = at(grp, act);
-      cpx<T> v[VW][E];
-  #pragma unroll
-      for (int m = 0; m < E; ++m)
-  #pragma unroll
-        for (int w = 0; w < VW; ++w) v[w][m] = nv[w][m];
-      if (grp + (int)gridDim.x < ngroups) {
-        bool an;
-        const int bn = at(grp + (int)gridDim.x, an);
-        load_rows<VW, E>(nv, a, bn, se, j, TT, an);
-      }
-      op(v, j, col, base, act);
-      store_rows<VW, E>(v, a, base, se, j, TT, act);
-    }
-  } else {
-    bool act;
-    const int base = at(blockIdx.x, act);
-    cpx<T> v[VW][E];
-    load_rows<VW, E>(v, a, base, se, j, TT, act);
-    op(v, j, col, base, act);
-    store_rows<VW, E>(v, a, base, se, j, TT, act);
-  }
+  const int q0 = (blockIdx.x * S::C + col) * VW;
+  const bool act = q0 < g.ncols;
+  const int base = act ? (int)pencil_base(g, q0) : 0;
+  cpx<T> v[VW][E];
+  load_rows<VW, E>(v, a, base, se, j, TT, act);
+  op(v, j, col, base, act);
+  store_rows<VW, E>(v, a, base, se, j, TT, act);
 }
 
 // drift sweep along a strided axis (x or y); with RHO the density adjoint
@@ -280,7 +248,7 @@ struct k_drift_s {
                                              const cpx<T>* __restrict__ pref) {
     using S = PencilShape<N>;
     constexpr int VW = StridedVW<T, N>::value;
-    __shared__ cpx<T> lds[S::kLdsStrided * VW];
+    __shared__ cpx<T> lds[S::kLdsStrided];
     const int se = (int)g.s_elem;
     strided_sweep<T, N>(psi, g, [&](cpx<T>(&v)[VW][S::E], int j, int col, int base, bool act) {
       if constexpr (RHO) {
@@ -298,7 +266,7 @@ struct k_drift_s {
           }
         }
       }
-      pencil_fft_strided<N, -1, T, VW>(v, j, col, lds, tw);
+      pencil_fft_strided<N, -1, T, VW, true>(v, j, col, lds, tw);
   #pragma unroll
       for (int m = 0; m < S::E; ++m) {
         const cpx<T> d0 = D[j + S::TT * m];
@@ -306,7 +274,7 @@ struct k_drift_s {
   #pragma unroll
         for (int w = 0; w < VW; ++w) v[w][m] = cmul(v[w][m], d);
       }
-      pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
+      pencil_fft_strided<N, +1, T, VW, true>(v, j, col, lds, tw);
     });
   }
 };
@@ -527,9 +495,9 @@ struct k_fft_s {
                                              const cpx<T>* __restrict__ tw) {
     using S = PencilShape<N>;
     constexpr int VW = StridedVW<T, N>::value;
-    __shared__ cpx<T> lds[S::kLdsStrided * VW];
+    __shared__ cpx<T> lds[S::kLdsStrided];
     strided_sweep<T, N>(a, g, [&](cpx<T>(&v)[VW][S::E], int j, int col, int, bool) {
-      pencil_fft_strided<N, DIR, T, VW>(v, j, col, lds, tw);
+      pencil_fft_strided<N, DIR, T, VW, true>(v, j, col, lds, tw);
     });
   }
 };
@@ -961,13 +929,6 @@ inline int nblocks_s(int ncols) {
   const int per = PencilShape<N>::C * StridedVW<T, N>::value;
   return (ncols + per - 1) / per;
 }
-// Blocks of a group-walking strided sweep (two pencils per lane): two per CU
-// (their LDS allows two), all resident at once; otherwise a block per group.
-template <class T, int N>
-inline int persist_blocks_s(int ncols) {
-  const int n = nblocks_s<T, N>(ncols);
-  return (StridedVW<T, N>::value == 2 && n > 512) ? 512 : n;
-}
 
 #define QJ_DISPATCH_N(n, ...)                      \
   switch (n) {                                     \
@@ -1048,7 +1009,7 @@ hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, v
     const cpx<T>* D = (const cpx<T>*)(y ? tb.drift_y : tb.drift_x);
     const cpx<T>* tw = (const cpx<T>*)(y ? tb.tw_y : tb.tw_x);
     QJ_DISPATCH_N(y ? d.ny : d.nx, {
-      launch<k_drift_s<T, N_>>(persist_blocks_s<T, N_>(g.ncols), 256, s, p, g, D, tw, conj_d,
+      launch<k_drift_s<T, N_>>(nblocks_s<T, N_>(g.ncols), 256, s, p, g, D, tw, conj_d,
                                (const T*)nullptr, (const cpx<T>*)nullptr);
     });
   }
@@ -1105,7 +1066,7 @@ hipError_t launch_fft_y_half(const Dims& d, const SpecTables<T>& tb, void* H, hi
   PassGeom g = geom_y(d, d.nx / 2 + 1);
   const cpx<T>* tw = (const cpx<T>*)tb.tw_y;
   QJ_DISPATCH_N(d.ny, {
-    launch<k_fft_s<T, N_, -1>>(persist_blocks_s<T, N_>(g.ncols), 256, s, (cpx<T>*)H, g, tw);
+    launch<k_fft_s<T, N_, -1>>(nblocks_s<T, N_>(g.ncols), 256, s, (cpx<T>*)H, g, tw);
   });
   return hipGetLastError();
 }
@@ -1196,7 +1157,7 @@ hipError_t launch_drift_adj_axis(const Dims& d, const SpecTables<T>& tb, int axi
   const cpx<T>* D = (const cpx<T>*)(y ? tb.drift_y : tb.drift_x);
   const cpx<T>* tw = (const cpx<T>*)(y ? tb.tw_y : tb.tw_x);
   QJ_DISPATCH_N(y ? d.ny : d.nx, {
-    launch<k_drift_s<T, N_, true>>(persist_blocks_s<T, N_>(g.ncols), 256, s, p, g, D, tw, 1,
+    launch<k_drift_s<T, N_, true>>(nblocks_s<T, N_>(g.ncols), 256, s, p, g, D, tw, 1,
                                    (const T*)rhobar, (const cpx<T>*)rho_ref);
   });
   return hipGetLastError();
