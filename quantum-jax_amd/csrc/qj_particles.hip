@@ -330,8 +330,20 @@ struct k_bin_scan {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int per = (ntiles + 1023) / 1024;
     const int b = tid * per, e = min(b + per, ntiles);
+    // Up to kHeld counts per thread (ntiles <= 8192) are loaded once, all in
+    // flight together, and held in registers for the second pass.
+    constexpr int kHeld = 8;
+    const bool held_all = per <= kHeld;
+    int held[kHeld];
     int local = 0;
-    for (int i = b; i < e; ++i) local += count[i];
+    if (held_all) {
+#pragma unroll
+      for (int r = 0; r < kHeld; ++r) held[r] = (b + r < e) ? count[b + r] : 0;
+#pragma unroll
+      for (int r = 0; r < kHeld; ++r) local += held[r];
+    } else {
+      for (int i = b; i < e; ++i) local += count[i];
+    }
     // inclusive scan of the per-thread sums within the wave, then across waves
     int incl = local;
     for (int o = 1; o < 64; o <<= 1) {
@@ -343,11 +355,22 @@ struct k_bin_scan {
     int base = 0;
     for (int w = 0; w < wv; ++w) base += wsum[w];
     int run = base + incl - local;
-    for (int i = b; i < e; ++i) {
-      const int c = count[i];
-      offset[i] = run;
-      run += c;
-      count[i] = 0;
+    if (held_all) {
+#pragma unroll
+      for (int r = 0; r < kHeld; ++r) {
+        if (b + r < e) {
+          offset[b + r] = run;
+          run += held[r];
+          count[b + r] = 0;
+        }
+      }
+    } else {
+      for (int i = b; i < e; ++i) {
+        const int c = count[i];
+        offset[i] = run;
+        run += c;
+        count[i] = 0;
+      }
     }
     if (tid == 1023) offset[ntiles] = base + incl;
   }
