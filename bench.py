@@ -239,8 +239,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: every box in this setup has one GPU and the "
-              "multi-GPU decomposition is a design (see DESIGN.md), so --gpus must be 1.",
+        print("bench.py measures one GPU only: the decomposition of one grid over several "
+              "GPUs (x slabs with local y/z sweeps and all-to-all transposes for the x "
+              "sweeps) is not built, so --gpus must be 1.",
               file=sys.stderr)
         sys.exit(2)
     if not torch.cuda.is_available():
