@@ -38,12 +38,22 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_threads():
+def affinity_cpus():
     try:
-        n = len(os.sched_getaffinity(0))
+        return len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        return os.cpu_count() or 1
+
+
+def cpu_threads():
+    """Threads for the CPU restatement: every core this process may run on,
+    or fewer when the host states a smaller CPU share for the job in
+    OMP_NUM_THREADS (a shared GPU host gives each GPU's job its own share)."""
+    n = affinity_cpus()
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        n = min(n, int(share))
+    return max(1, n)
 
 
 def cpu_model():
@@ -594,7 +604,7 @@ def main():
         log(f"cpu baseline: oracle step at {n}^3, {thr} threads")
         st = O.compute_step_stars(*st, dt, m_s, g, ph)   # warm-up
         times = []
-        for _ in range(2):
+        for _ in range(3):
             t = time.perf_counter()
             st = O.compute_step_stars(*st, dt, m_s, g, ph)
             times.append(time.perf_counter() - t)
@@ -605,6 +615,7 @@ def main():
         st = O.compute_step_stars(*st, dt, m_s, g, ph)
         t1 = time.perf_counter() - t
         O.set_workers(thr)
+        n_or = 2 + len(times)
         out["cpu_baseline"] = {
             "value": N / tmed, "unit": "cell-updates/s", "cores": thr, "kind": "port",
             "sample": f"1 warm-up + {len(times)} timed steps of the same {n}^3 + {n_p}-star "
@@ -613,8 +624,30 @@ def main():
                       f"median {tmed:.2f} s/step; then 1 step with workers=1: {t1:.2f} s; "
                       f"CPU: {cpu_model()}",
             "value_workers_1": N / t1,
+            "affinity_cpus": affinity_cpus(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
         }
         log(f"cpu: {tmed:.2f} s/step ({thr} workers), {t1:.2f} s/step (1 worker)")
+        # the oracle's state after those steps checks the HIP builds stepped
+        # from the same IC at this size
+        ref_psi = torch.as_tensor(st[0])
+        ref_pos = st[1]
+        vs = {"steps": n_or}
+        for tag, cd in (("c64", torch.complex64), ("c128", torch.complex128)):
+            pl = qj.Plan(n, dtype=cd, device=dev)
+            s = qj.KDKSimulation(pl, phys, dt, torch.as_tensor(psi0).to(dev, cd),
+                                 torch.as_tensor(pos0).to(dev), torch.as_tensor(vel0).to(dev),
+                                 mass=m_s)
+            s.step(n_or)
+            sst = s.state()
+            p64 = sst["psi"].to(torch.complex128).cpu()
+            vs[f"psi_rel_l2_{tag}"] = ((p64 - ref_psi).norm() / ref_psi.norm()).item()
+            dp = (sst["pos"].cpu().numpy() - ref_pos + 0.5) % 1.0 - 0.5
+            vs[f"star_pos_max_abs_{tag}"] = float(np.abs(dp).max())
+            del s, pl, sst
+            torch.cuda.empty_cache()
+        out["accuracy_vs_oracle"] = vs
+        log(f"vs the FP64 oracle at {n}^3: {vs}")
 
     print(json.dumps(out), flush=True)
 

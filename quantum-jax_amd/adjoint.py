@@ -62,7 +62,7 @@ def bin_particles_ad(pos, mass, plan: Plan):
 class _Tape:
     """Device buffers of one recorded segment (reused across segments)."""
 
-    def __init__(self, plan: Plan, nsteps: int, n_p: int):
+    def __init__(self, plan: Plan, nsteps: int, n_p: int, bins: bool = True):
         dev = plan.device
         N = plan.ncell
         # psi at the start is needed for the first step only (the adjoint merges
@@ -85,7 +85,7 @@ class _Tape:
             t.grid1 = self.grid1[k].data_ptr()
             t.x0 = self.x0[0].data_ptr() if n_p and k == 0 else None
             t.x1 = self.x1[k].data_ptr() if n_p else None
-            t.bins = self.bins[k].data_ptr() if n_p else None
+            t.bins = self.bins[k].data_ptr() if n_p and bins else None
 
 
 def _adjoint_segment(plan: Plan, phys: Physics, dt: float, ck, mass, nst: int, tape: _Tape,
@@ -105,7 +105,7 @@ def _adjoint_segment(plan: Plan, phys: Physics, dt: float, ck, mass, nst: int, t
 class KDKRun(torch.autograd.Function):
     @staticmethod
     def forward(ctx, psi, pos, vel, mass, plan: Plan, phys: Physics, dt: float, nsteps: int,
-                segment: int):
+                segment: int, tape_bins: bool):
         sim = KDKSimulation(plan, phys, dt, psi.detach(), pos.detach(), vel.detach(),
                             mass=mass.detach())
         ckpts = []
@@ -117,6 +117,7 @@ class KDKRun(torch.autograd.Function):
             done += n
         st = sim.state()
         ctx.plan, ctx.phys, ctx.dt, ctx.nsteps, ctx.segment = plan, phys, dt, nsteps, segment
+        ctx.tape_bins = tape_bins
         ctx.ckpts = ckpts
         ctx.save_for_backward(mass.detach())
         return st["psi"].clone(), st["pos"].clone(), st["vel"].clone()
@@ -138,19 +139,24 @@ class KDKRun(torch.autograd.Function):
                               dtype=torch.uint8, device=dev)
         adj = _lib.QjAdjoint(psi_bar.data_ptr(), x_bar.data_ptr(), v_bar.data_ptr(),
                              m_bar.data_ptr(), scratch.data_ptr())
-        tape = _Tape(plan, min(ctx.segment, ctx.nsteps), n_p)
+        tape = _Tape(plan, min(ctx.segment, ctx.nsteps), n_p, ctx.tape_bins)
         starts = list(range(0, ctx.nsteps, ctx.segment))
         for s0, ck in reversed(list(zip(starts, ctx.ckpts))):
             nst = min(ctx.segment, ctx.nsteps - s0)
             _adjoint_segment(plan, phys, dt, ck, mass, nst, tape, adj)
         ctx.ckpts = None
-        return psi_bar, x_bar, v_bar, m_bar, None, None, None, None, None
+        return psi_bar, x_bar, v_bar, m_bar, None, None, None, None, None, None
 
 
 def kdk_run(psi, pos, vel, mass, plan: Plan, phys: Physics, dt: float, nsteps: int,
-            segment: int = 16):
-    """nsteps KDK steps (stars), differentiable w.r.t. psi, pos, vel and mass."""
-    return KDKRun.apply(psi, pos, vel, mass, plan, phys, float(dt), int(nsteps), int(segment))
+            segment: int = 16, tape_bins: bool = True):
+    """nsteps KDK steps (stars), differentiable w.r.t. psi, pos, vel and mass.
+
+    tape_bins: the recompute tapes each step's tile registrations for the
+    adjoint's transposed interpolation (False: the adjoint counts them
+    again; same gradients up to the order of the additions)."""
+    return KDKRun.apply(psi, pos, vel, mass, plan, phys, float(dt), int(nsteps), int(segment),
+                        bool(tape_bins))
 
 
 def density_loss(psi, pos, mass, rho_target, plan: Plan):

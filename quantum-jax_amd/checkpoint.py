@@ -5,7 +5,8 @@ The reference saves its state pytree every chunk with orbax
 run adds rho, vx, vy, vz) and writes its parameters to params.json.  orbax
 is not available here; the same fields go to ``<directory>/<step>/state.npz``
 (NumPy, read back with allow_pickle=False) and the parameters to
-``<directory>/params.json``.
+``<directory>/params.json``.  A black-hole run (quantum-bh.py:345-381) also
+saves its accretion constants (G, c, sigma), and restores as a BHSimulation.
 """
 
 from __future__ import annotations
@@ -16,8 +17,9 @@ import os
 import numpy as np
 import torch
 
+from . import _lib
 from .core import Physics, Plan
-from .simulation import KDKSimulation
+from .simulation import BHSimulation, KDKSimulation
 
 
 def save_checkpoint(sim: KDKSimulation, directory: str, step: int,
@@ -33,6 +35,10 @@ def save_checkpoint(sim: KDKSimulation, directory: str, step: int,
         arrays["dt_s"] = np.float64(st["dt_s"])
         arrays["mass"] = (sim.mass_t.cpu().numpy() if sim.mass_t is not None
                           else np.float64(sim.m_scalar))
+    if isinstance(sim, BHSimulation):
+        arrays["bh_G"] = np.float64(sim.G)
+        arrays["bh_c"] = np.float64(sim.c)
+        arrays["bh_sigma"] = np.float64(sim.sigma)
     path = os.path.join(directory, str(int(step)))
     os.makedirs(path, exist_ok=True)
     np.savez(os.path.join(path, "state.npz"), **arrays)
@@ -51,7 +57,9 @@ def load_checkpoint(directory: str, step: int) -> dict:
 
 def restore_simulation(plan: Plan, phys: Physics, dt: float, directory: str,
                        step: int) -> KDKSimulation:
-    """A KDKSimulation at the saved state (a_max carried over)."""
+    """A KDKSimulation at the saved state (a_max carried over); a
+    BHSimulation with its saved accretion constants when the checkpoint is
+    one of a black-hole run."""
     d = load_checkpoint(directory, step)
     dev = plan.device
     psi = torch.as_tensor(d["psi"]).to(dev, plan.complex_dtype)
@@ -62,7 +70,12 @@ def restore_simulation(plan: Plan, phys: Physics, dt: float, directory: str,
                   mass=(torch.as_tensor(m).to(dev) if np.ndim(m) else float(m)))
     if "rho" in d:
         kw["gas"] = [torch.as_tensor(d[k]).to(dev, plan.real_dtype) for k in ("rho", "vx", "vy", "vz")]
-    sim = KDKSimulation(plan, phys, dt, psi, **kw)
+    if "bh_G" in d:
+        sim = BHSimulation(plan, phys, dt, psi, kw["pos"], kw["vel"],
+                           torch.as_tensor(np.atleast_1d(d["mass"])).to(dev, torch.float64),
+                           G=d["bh_G"], c=d["bh_c"], sigma=d["bh_sigma"], t=kw["t"])
+    else:
+        sim = KDKSimulation(plan, phys, dt, psi, **kw)
     if "a_max" in d:
-        sim.diag[0] = float(d["a_max"])
+        sim.diag[_lib.DIAG_AMAX] = float(d["a_max"])
     return sim

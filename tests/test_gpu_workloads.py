@@ -1,0 +1,344 @@
+"""The benchmark workloads at their full sizes (needs a GPU).
+
+  * stars, 256^3 + 2^18 FP64 stars, dt = 5e-4 (the headline workload): two
+    steps of the c64 and c128 paths against the FP64 oracle from the same
+    seeded IC (psi seed 17, stars seed 1017), CIC indices of the stepped
+    stars bit-exact, ten-step time reversibility and mass conservation;
+  * the differentiable run, 128^3 + 2^16 stars, 256 steps with a checkpoint
+    every 16: central differences of the c128 forward along random unit
+    directions, and c64 gradients against c128 ones;
+  * the 48-box ensemble (16 x 64^3 + 24 x 128^3 + 8 x 256^3, 2^14 stars each):
+    the graph-fork ensemble against member-by-member stepping, and the
+    ensemble adjoint against the members' own adjoints for one box of each
+    size.
+"""
+
+import importlib
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+C64, C128 = torch.complex64, torch.complex128
+
+
+def rel(a, b):
+    a = a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
+    b = b.detach().cpu().numpy() if isinstance(b, torch.Tensor) else np.asarray(b)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def dev(x, dtype):
+    return torch.as_tensor(np.ascontiguousarray(x)).to("cuda", dtype)
+
+
+def _workers():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+# ---------------------------------------------------------------------------
+# stars, 256^3 c64 + 2^18 stars, dt = 5e-4
+# ---------------------------------------------------------------------------
+_STARS = {}
+N_STARS, DT_STARS = 1 << 18, 5e-4
+
+
+def _stars_oracle():
+    """IC and two FP64 oracle steps of the headline workload (cached)."""
+    if not _STARS:
+        g = O.Grid(256, 256, 256)
+        psi0 = O.filtered_gaussian_psi(g, 17)
+        pos0, vel0, m_s = O.star_ic(g, N_STARS, 1017)
+        ph = O.Physics(rho_bar=1.1)
+        O.set_workers(_workers())
+        try:
+            s = (psi0, pos0, vel0, 0.0, 0.0, np.inf)
+            steps = []
+            for _ in range(2):
+                s = O.compute_step_stars(*s, DT_STARS, m_s, g, ph)
+                steps.append(s)
+        finally:
+            O.set_workers(1)
+        _STARS.update(g=g, psi0=psi0, pos0=pos0, vel0=vel0, m_s=m_s, steps=steps)
+    return _STARS
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+def test_stars_256_cubed_matches_oracle(qj, cuda, dtype):
+    o = _stars_oracle()
+    g, m_s = o["g"], o["m_s"]
+    plan = qj.Plan(256, dtype=dtype)
+    sim = qj.KDKSimulation(plan, qj.Physics(rho_bar=1.1), DT_STARS, dev(o["psi0"], dtype),
+                           dev(o["pos0"], torch.float64), dev(o["vel0"], torch.float64),
+                           mass=m_s)
+    for k, ref in enumerate(o["steps"]):
+        sim.step(1)
+        st = sim.state()
+        tag = f"step {k + 1}"
+        assert rel(st["psi"], ref[0]) < {C64: 2e-5, C128: 1e-11}[dtype], tag
+        dpos = (st["pos"].cpu().numpy() - ref[1] + 0.5) % 1.0 - 0.5
+        assert np.max(np.abs(dpos)) < {C64: 1e-9, C128: 1e-13}[dtype], tag
+        assert rel(st["vel"], ref[2]) < {C64: 1e-5, C128: 1e-12}[dtype], tag
+        assert abs(st["a_max"] - ref[4]) < 1e-5 * ref[4], tag
+        assert abs(st["dt_s"] - ref[5]) < 1e-5 * ref[5], tag
+    # CIC indices and FP64 weights of the stepped oracle stars, bit-exact
+    pos2 = o["steps"][-1][1]
+    i, ip1, w_i, w_ip1 = qj.get_cic_indices_and_weights(dev(pos2, torch.float64), plan)
+    ri, rip1, rw_i, rw_ip1 = O.get_cic_indices_and_weights(pos2, g)
+    assert np.array_equal(i.cpu().numpy(), ri)
+    assert np.array_equal(ip1.cpu().numpy(), rip1)
+    assert np.array_equal(w_i.cpu().numpy(), rw_i)
+    assert np.array_equal(w_ip1.cpu().numpy(), rw_ip1)
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+def test_stars_256_cubed_binned_and_atomic_deposits_agree(qj, cuda, dtype):
+    """The tile-binned deposit (4096-cell tiles, FP64 LDS tiles in c128)
+    against the float-atomic one on the headline workload: the deposited
+    star density itself, and the states after two steps."""
+    o = _stars_oracle()
+    out = []
+    for binned in (True, False):
+        plan = qj.Plan(256, dtype=dtype)
+        sim = qj.KDKSimulation(plan, qj.Physics(rho_bar=1.1), DT_STARS, dev(o["psi0"], dtype),
+                               dev(o["pos0"], torch.float64), dev(o["vel0"], torch.float64),
+                               mass=o["m_s"])
+        if not binned:
+            sim._st.star_scratch = None
+        sim.step(2)
+        rho_s = sim.rho_s.clone()     # deposited from the positions after step 2
+        st = sim.state()
+        out.append((rho_s, st["psi"].clone(), st["pos"].clone(), st["vel"].clone()))
+        del sim, plan
+    tol = {C64: 1e-6, C128: 1e-13}[dtype]
+    for a, b in zip(*out):
+        assert rel(a, b) < tol
+    # and the deposit against the oracle's bin_stars of the same positions
+    ref = O.bin_stars(out[0][2].cpu().numpy(), o["m_s"], o["g"])
+    assert rel(out[0][0], ref) < {C64: 2e-6, C128: 1e-12}[dtype]
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+def test_stars_256_cubed_reversible_and_mass_conserving(qj, cuda, dtype):
+    o = _stars_oracle()
+    psi0 = dev(o["psi0"], dtype)
+    pos0 = dev(o["pos0"], torch.float64)
+    vel0 = dev(o["vel0"], torch.float64)
+    m_s = o["m_s"]
+    plan = qj.Plan(256, dtype=dtype)
+    sim = qj.KDKSimulation(plan, qj.Physics(rho_bar=1.1), DT_STARS, psi0, pos0, vel0, mass=m_s)
+    vol = plan.vol
+    M0 = vol * sim.psi.abs().pow(2).double().sum().item() + m_s * N_STARS
+    sim.step(10)
+    psi10 = sim.psi
+    M10 = vol * psi10.abs().pow(2).double().sum().item() + m_s * N_STARS
+    moved = rel(psi10, psi0)
+    sim.set_dt(-DT_STARS)
+    sim.step(10)
+    st = sim.state()
+    assert moved > 1e-3
+    assert abs(M10 / M0 - 1) < {C64: 1e-5, C128: 1e-12}[dtype]
+    assert rel(st["psi"], psi0) < {C64: 2e-5, C128: 1e-11}[dtype]
+    dpos = (st["pos"] - pos0 + 0.5) % 1.0 - 0.5
+    assert dpos.abs().max().item() < {C64: 1e-9, C128: 1e-13}[dtype]
+    assert rel(st["vel"], vel0) < {C64: 1e-6, C128: 1e-12}[dtype]
+
+
+# ---------------------------------------------------------------------------
+# differentiable run: 128^3 + 2^16 stars, 256 steps, checkpoint every 16
+# ---------------------------------------------------------------------------
+def _adjoint_case(dtype):
+    g = O.Grid(128, 128, 128)
+    psi = O.filtered_gaussian_psi(g, 17)
+    pos, vel, m_s = O.star_ic(g, 1 << 16, 1017)
+    target = np.abs(O.filtered_gaussian_psi(g, 2017)) ** 2
+    rdt = torch.float32 if dtype == C64 else torch.float64
+    return (dev(psi, dtype), dev(pos, torch.float64), dev(vel, torch.float64),
+            torch.full((1 << 16,), m_s, dtype=torch.float64, device="cuda"), dev(target, rdt))
+
+
+def _grads(qj, ad, dtype, nsteps=256, seg=16):
+    plan = qj.Plan(128, dtype=dtype)
+    phys = qj.Physics(rho_bar=1.1)
+    psi, pos, vel, m, tgt = _adjoint_case(dtype)
+    p = psi.clone().requires_grad_(True)
+    mm = m.clone().requires_grad_(True)
+    out = ad.kdk_run(p, pos, vel, mm, plan, phys, 5e-4, nsteps, seg)
+    L = ad.density_loss(out[0], out[1], mm, tgt, plan)
+    L.backward()
+
+    def loss(pp, m_):
+        with torch.no_grad():
+            o = ad.kdk_run(pp, pos, vel, m_, plan, phys, 5e-4, nsteps, seg)
+            return ad.density_loss(o[0], o[1], m_, tgt, plan).item()
+
+    return L.item(), p.grad, mm.grad, loss, psi, m
+
+
+def test_adjoint_128_cubed_256_steps_matches_central_differences(qj, cuda):
+    ad = importlib.import_module("quantum-jax_amd.adjoint")
+    L0, gpsi, gm, loss, psi, m = _grads(qj, ad, C128)
+    assert np.isfinite(L0) and torch.isfinite(gpsi).all() and torch.isfinite(gm).all()
+    rng = np.random.default_rng(23)
+    eps = 1e-3
+    shape = tuple(psi.shape)
+    for _ in range(2):
+        d = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+        d /= np.linalg.norm(d)
+        dd = dev(d, C128)
+        fd = (loss(psi + eps * dd, m) - loss(psi - eps * dd, m)) / (2 * eps)
+        an = (gpsi.conj() * dd).real.sum().item()
+        assert abs(fd - an) <= 1e-3 * abs(an) + 1e-9 * abs(L0), (fd, an)
+    d = rng.standard_normal(m.shape)
+    d /= np.linalg.norm(d)
+    dd = dev(d, torch.float64) * m.abs().max().item()
+    fd = (loss(psi, m + eps * dd) - loss(psi, m - eps * dd)) / (2 * eps)
+    an = (gm * dd).sum().item()
+    assert abs(fd - an) <= 1e-3 * abs(an) + 1e-9 * abs(L0), (fd, an)
+
+
+def test_adjoint_128_cubed_256_steps_c64_tracks_c128(qj, cuda):
+    """c64 against c128 over the full 256 steps.  The c64 loss carries the
+    FP32 mass drift of the sweeps (about 2e-7 per step, 5e-5 of rho after
+    256 steps, measured 5.6e-5 on the loss); the gradients agree to 1.1e-4
+    (psi) and 4.3e-5 (masses) on MI355X."""
+    ad = importlib.import_module("quantum-jax_amd.adjoint")
+    r64 = _grads(qj, ad, C64)
+    r128 = _grads(qj, ad, C128)
+    relg = lambda u, w: ((u.to(w.dtype) - w).norm() / w.norm()).item()
+    dl, dp, dm = abs(r64[0] / r128[0] - 1), relg(r64[1], r128[1]), relg(r64[2], r128[2])
+    print(f"c64 vs c128 at 128^3, 256 steps: loss {dl:.2e}, grad psi {dp:.2e}, grad m {dm:.2e}")
+    assert dl < 3e-4
+    assert dp < 5e-4
+    assert dm < 5e-4
+
+
+@pytest.mark.parametrize("cd,n", [(C64, 128), (C128, 256)], ids=["c64-128", "c128-256"])
+def test_adjoint_binned_transpose_on_large_tiles(qj, cuda, cd, n):
+    """The tile-binned transpose of the star interpolation at the tile sizes
+    of the large grids (1024-cell tiles and 2048 tiles at 128^3; 4096-cell
+    FP64 tiles at 256^3), with the recompute's taped registrations and with
+    registrations recounted, against the float-atomic transpose."""
+    ad = importlib.import_module("quantum-jax_amd.adjoint")
+    g = O.Grid(n, n, n)
+    psi = dev(O.filtered_gaussian_psi(g, 19, sigma_k=8 * np.pi), cd)
+    pos, vel, m_s = O.star_ic(g, 1 << 16, 1019, sigma_v=0.2)
+    rng = np.random.default_rng(5)
+    m = dev(m_s * (0.5 + rng.random(1 << 16)), torch.float64)
+    rdt = torch.float32 if cd == C64 else torch.float64
+    tgt = dev(np.abs(O.filtered_gaussian_psi(g, 2019, sigma_k=8 * np.pi)) ** 2, rdt)
+    phys = qj.Physics(rho_bar=1.1, four_pi_G=20.0)
+    res = []
+    for binned, taped in ((1, True), (1, False), (0, False)):
+        plan = qj.Plan(n, dtype=cd)
+        plan.set_option(qj._lib.OPT_BINNED_DEPOSIT, binned)
+        p = psi.clone().requires_grad_(True)
+        v = dev(vel, torch.float64).requires_grad_(True)
+        mm = m.clone().requires_grad_(True)
+        out = ad.kdk_run(p, dev(pos, torch.float64), v, mm, plan, phys, 1e-3, 2, 2,
+                         tape_bins=taped)
+        ad.density_loss(out[0], out[1], mm, tgt, plan).backward()
+        res.append((p.grad, v.grad, mm.grad))
+        del plan
+    relg = lambda u, w: ((u - w).norm() / w.norm()).item()
+    tol = {C64: 1e-4, C128: 1e-11}[cd]
+    for other in res[1:]:
+        for a, b in zip(res[0], other):
+            assert relg(a, b) < tol
+
+
+# ---------------------------------------------------------------------------
+# the 48-box ensemble: 16 x 64^3 + 24 x 128^3 + 8 x 256^3, 2^14 stars each
+# ---------------------------------------------------------------------------
+SIZES = [64] * 16 + [128] * 24 + [256] * 8
+
+
+def _box_psi(n, seed):
+    """A filtered-Gaussian c64 psi built on the device (seeded torch RNG)."""
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    re = torch.randn((n, n, n), generator=gen, device="cuda", dtype=torch.float64)
+    im = torch.randn((n, n, n), generator=gen, device="cuda", dtype=torch.float64)
+    k = 2 * np.pi * torch.fft.fftfreq(n, 1.0 / n, device="cuda", dtype=torch.float64)
+    ksq = k[:, None, None] ** 2 + k[None, :, None] ** 2 + k[None, None, :] ** 2
+    f = torch.fft.ifftn(torch.fft.fftn(torch.complex(re, im)) * torch.exp(-ksq / (2 * (16 * np.pi) ** 2)))
+    return (f / f.abs().pow(2).mean().sqrt()).to(C64).contiguous()
+
+
+def _ensemble_ics():
+    rng = np.random.default_rng(4242)
+    dts = rng.uniform(2e-4, 1e-3, len(SIZES))
+    ics = []
+    for b, n in enumerate(SIZES):
+        r = np.random.default_rng(1017 + b)
+        pos = r.random((1 << 14, 3))
+        vel = r.normal(0.0, 0.05, (1 << 14, 3))
+        ics.append((_box_psi(n, 17 + b), dev(pos, torch.float64), dev(vel, torch.float64),
+                    0.1 / (1 << 14)))
+    return dts, ics
+
+
+def _members(qj, dts, ics):
+    phys = qj.Physics(rho_bar=1.1)
+    return [qj.KDKSimulation(qj.Plan(n, dtype=C64), phys, float(dt), ic[0], ic[1], ic[2],
+                             mass=ic[3]) for n, dt, ic in zip(SIZES, dts, ics)]
+
+
+def test_ensemble_48_boxes_matches_member_stepping(qj, cuda):
+    dts, ics = _ensemble_ics()
+    ref = _members(qj, dts, ics)
+    mem = _members(qj, dts, ics)
+    ens = qj.Ensemble(mem, mode="graph_fork")
+    for k in range(1, 4):
+        ens.step(1)
+        for s in ref:
+            s.step(1)
+        torch.cuda.synchronize()
+    # three shape groups (one per box size), one batched launch per stage
+    assert sorted(len(m) for m, _ in ens.groups()) == [8, 16, 24]
+    ens.finalize()
+    for b, (x, y) in enumerate(zip(mem, ref)):
+        y.finalize()
+        tag = f"box {b} ({SIZES[b]}^3)"
+        # star deposits add in a run-dependent order (LDS float atomics)
+        assert rel(x._psi, y._psi) < 1e-6, tag
+        assert rel(x.pos, y.pos) < 1e-6, tag
+        assert rel(x.vel, y.vel) < 1e-5, tag
+        assert x.a_max() == pytest.approx(y.a_max(), rel=1e-5), tag
+    ens.close()
+    del ens, mem, ref
+    torch.cuda.empty_cache()
+
+
+def test_ensemble_48_boxes_adjoint_matches_member_adjoints(qj, cuda):
+    ad = importlib.import_module("quantum-jax_amd.adjoint")
+    dts, ics = _ensemble_ics()
+    phys = qj.Physics(rho_bar=1.1)
+    nsteps, seg = 4, 2
+    plans = [qj.Plan(n, dtype=C64) for n in SIZES]
+    masses = [torch.full((1 << 14,), ic[3], dtype=torch.float64, device="cuda") for ic in ics]
+    tgts = [_box_psi(n, 5017 + b).abs().pow(2).float() for b, n in enumerate(SIZES)]
+    ps = [ic[0].clone().requires_grad_(True) for ic in ics]
+    ms = [m.clone().requires_grad_(True) for m in masses]
+    po, xo, _ = ad.ensemble_kdk_run(ps, [ic[1] for ic in ics], [ic[2] for ic in ics], ms, plans,
+                                    [phys] * len(SIZES), list(dts), nsteps, seg,
+                                    mode="graph_fork")
+    L = sum(ad.density_loss(po[b], xo[b], ms[b], tgts[b], plans[b]) for b in range(len(SIZES)))
+    L.backward()
+    relg = lambda u, w: ((u - w).norm() / w.norm()).item()
+    for b in (0, 16, 40):   # one box of each size
+        p = ics[b][0].clone().requires_grad_(True)
+        m = masses[b].clone().requires_grad_(True)
+        pl = qj.Plan(SIZES[b], dtype=C64)
+        o = ad.kdk_run(p, ics[b][1], ics[b][2], m, pl, phys, float(dts[b]), nsteps, seg)
+        ad.density_loss(o[0], o[1], m, tgts[b], pl).backward()
+        assert relg(ps[b].grad, p.grad) < 1e-4, b
+        assert relg(ms[b].grad, m.grad) < 1e-4, b
