@@ -178,6 +178,19 @@ typedef struct {
 
 int qj_kdk_steps(qj_plan* plan, qj_state* state, const qj_params* prm, int nsteps, void* work,
                  void* stream);
+/* qj_kdk_steps without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  The workspace is a static device array of
+ * QJ_NOALLOC_SCRATCH_BYTES per GPU, looked up when the first plan on that
+ * GPU is created and held until the process ends; the call itself allocates
+ * nothing.  Plans whose qj_workspace_size() exceeds it (e.g. 256^3 c64;
+ * 128^3 c64 fits) are rejected with QJ_EUNSUPPORTED, as is a plan with stage
+ * timing on.  Every call on one GPU shares that array: calls that overlap in
+ * time on one GPU (two streams, two threads) race and must not be issued. */
+#define QJ_NOALLOC_SCRATCH_BYTES (40u * 1024u * 1024u)
+int qj_kdk_steps_noalloc(qj_plan* plan, qj_state* state, const qj_params* prm, int nsteps,
+                         void* stream);
 /* nsteps steps with per-step kick and drift times (host arrays of nsteps):
  * the kicks of step k use kick_dt[k]/2 and the drift (and the gas update)
  * drift_dt[k].  The comoving step of quantum-cosmo.py:166-197 is

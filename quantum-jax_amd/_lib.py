@@ -24,6 +24,8 @@ FLUXES = {"rusanov": FLUX_RUSANOV, "hll": FLUX_HLL}
 OPT_FUSE_X_SOURCE = 0
 OPT_BINNED_DEPOSIT = 1
 ENS_EAGER, ENS_BATCHED, ENS_GRAPH, ENS_GRAPH_FORK = 0, 1, 2, 3
+# bytes of the static workspace of qj_kdk_steps_noalloc (one per GPU)
+NOALLOC_SCRATCH_BYTES = 40 * 1024 * 1024
 
 vp = C.c_void_p
 dp = C.POINTER(C.c_double)
@@ -83,6 +85,7 @@ SIGNATURES = {
     "qj_kdk_steps_cfl": (i32, [vp, vp, vp, i32, f64, f64, dp, vp, vp]),
     "qj_kdk_steps": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), i32, vp, vp]),
     "qj_kdk_finalize": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), vp]),
+    "qj_kdk_steps_noalloc": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), i32, vp]),
     "qj_kdk_steps_sched": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), i32, dp, dp, vp, vp]),
     "qj_adjoint_scratch_size": (C.c_size_t, [vp]),
     "qj_star_scratch_size": (C.c_size_t, [vp, i64]),

@@ -5,6 +5,7 @@
 
 #include <string>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "../../include/qj_api.h"
@@ -105,6 +106,37 @@ const char* kStageNames[kStageCount] = {
     "poisson_z", "poisson_y", "poisson_x", "deposit", "star_step", "star_kick",
     "hydro", "gas_kick", "kick", "adj_particles", "adj_kick", "adj_poisson", "adj_rho",
     "adj_drift"};
+
+}  // namespace
+
+// Workspace of qj_kdk_steps_noalloc: a file-scope device array, one copy per
+// GPU the library runs on, held until the process ends.  Host code never
+// passes its name to a kernel or a copy (in host code the name is a host-side
+// stand-in); it uses the per-device address from hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[QJ_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on device dev (which must be current), looked
+// up once per device and kept.
+int scratch_address(int dev, void** out) {
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(QJ_EUNSUPPORTED, "no-allocation workspace: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hipfail(e, "hipGetSymbolAddress(scratch)");
+    if (!ptr) return fail(QJ_EHIP, "hipGetSymbolAddress(scratch) returned a null address");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return QJ_OK;
+}
 
 }  // namespace
 
@@ -318,6 +350,12 @@ int qj_plan_create(qj_plan** out, int nx, int ny, int nz, double Lx, double Ly, 
   if (e != hipSuccess) {
     qj_plan_destroy(p);
     return hipfail(e, "qj_plan_create");
+  }
+  // set-up of the no-allocation workspace on this device (once per device)
+  void* sp = nullptr;
+  if (int rc = scratch_address(p->device, &sp)) {
+    qj_plan_destroy(p);
+    return rc;
   }
   *out = p;
   return QJ_OK;
@@ -960,6 +998,24 @@ int qj_kdk_steps(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void
   hipStream_t s = (hipStream_t)stream;
   return p->dtype == QJ_C64 ? kdk_steps_t<float>(p, st, pr, nsteps, work, s)
                             : kdk_steps_t<double>(p, st, pr, nsteps, work, s);
+}
+
+int qj_kdk_steps_noalloc(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps,
+                         void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  const size_t need = qj_workspace_size(p);
+  if (need > (size_t)QJ_NOALLOC_SCRATCH_BYTES)
+    return fail(QJ_EUNSUPPORTED,
+                "qj_kdk_steps_noalloc: the plan's workspace (" + std::to_string(need) +
+                    " bytes) exceeds the " + std::to_string((size_t)QJ_NOALLOC_SCRATCH_BYTES) +
+                    "-byte static workspace; use qj_kdk_steps with a caller workspace");
+  if (p->timing)
+    return fail(QJ_EARG, "qj_kdk_steps_noalloc: stage timing creates events; disable it first");
+  void* work = nullptr;
+  if (int rc = scratch_address(p->device, &work)) return rc;
+  // the workspace holds only spectra that each step writes before reading
+  // them, so nothing needs clearing between calls
+  return qj_kdk_steps(p, st, pr, nsteps, work, stream);
 }
 
 int qj_kdk_steps_sched(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps,
