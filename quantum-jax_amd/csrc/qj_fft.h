@@ -18,13 +18,13 @@
 
 namespace qj {
 
-// Elements per thread for an N-point pencil.
-template <int N>
+// Elements per thread for an N-point pencil, in workgroups of TH threads.
+template <int N, int TH = 256>
 struct PencilShape {
   static_assert((N & (N - 1)) == 0 && N >= 8 && N <= 512, "pencil length must be 2^k in [8,512]");
   static constexpr int E = (N <= 16) ? N : (N <= 256 ? 16 : 32);
   static constexpr int TT = N / E;        // threads per pencil
-  static constexpr int kThreads = 256;    // workgroup size
+  static constexpr int kThreads = TH;     // workgroup size
   static constexpr int C = kThreads / TT; // pencils per workgroup
   static constexpr int P = TT + 1;        // padded LDS row (k1-major)
   static constexpr int kLds = (TT > 1) ? C * E * P : 1;  // complex elements
@@ -137,10 +137,10 @@ __device__ __forceinline__ void pencil_fft(cpx<T>* v, int j, cpx<T>* lds,
 // access for c64, and the pencils share the stage twiddles and the two
 // block barriers of the exchange.  All VW pencils use the column-fastest
 // LDS layout, one kLdsStrided block each.
-template <int N, int DIR, class T, int VW, bool PHASED = false>
+template <int N, int DIR, class T, int VW, bool PHASED = false, int TH = 256>
 __device__ __forceinline__ void pencil_fft_strided(cpx<T> (*v)[PencilShape<N>::E], int j, int col,
                                                    cpx<T>* lds, const cpx<T>* __restrict__ tw) {
-  using S = PencilShape<N>;
+  using S = PencilShape<N, TH>;
   constexpr int E = S::E, TT = S::TT, C = S::C, KS = S::KS, L = S::kLdsStrided;
 #pragma unroll
   for (int w = 0; w < VW; ++w) dft<E, DIR, T>(v[w]);
@@ -202,6 +202,21 @@ __device__ __forceinline__ void pencil_fft_strided(cpx<T> (*v)[PencilShape<N>::E
 template <class T, int N>
 struct StridedVW {
   static constexpr int value = (sizeof(T) == 4 && N == 256) ? 2 : 1;
+};
+
+// Workgroup size and pencils per lane of a strided sweep.  The wide form
+// (c64 at 256 points) gives each lane one pencil in 512-thread workgroups:
+// the same 32-column row pieces as two pencils per lane in 256 threads, with
+// half the registers per lane.  A sweep that holds several spectra per
+// pencil then runs four waves per SIMD instead of two (the Poisson x pass:
+// 225 -> 123 VGPRs, 136 -> 121 us at 256^3 in isolation).
+template <class T, int N, bool WIDE>
+struct StridedCfg {
+  static constexpr bool kWide = WIDE && sizeof(T) == 4 && N == 256;
+  static constexpr int TH = kWide ? 512 : 256;
+  static constexpr int VW = kWide ? 1 : StridedVW<T, N>::value;
+  static constexpr int C = PencilShape<N, TH>::C;
+  static int blocks(int ncols) { return (ncols + C * VW - 1) / (C * VW); }
 };
 
 // VW neighbouring complex (or real) values starting at element i.

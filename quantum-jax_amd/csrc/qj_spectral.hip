@@ -639,7 +639,8 @@ __device__ __forceinline__ void load_herm_rows(cpx<T> (&v)[VW][E], const cpx<T>*
 // ---------------------------------------------------------------------------
 template <class T, int N, bool GRAD>
 struct k_poisson_x {
-  static constexpr int kThreads = 256;
+  using Cfg = StridedCfg<T, N, true>;
+  static constexpr int kThreads = Cfg::TH;
   static __device__ __forceinline__ void run(const cpx<T>* __restrict__ H,
                                              const cpx<T>* __restrict__ Uy,
                                              const cpx<T>* __restrict__ Uzz, PassGeom g,
@@ -647,11 +648,20 @@ struct k_poisson_x {
                                              T* __restrict__ V, T* __restrict__ A3,
                                              const T* __restrict__ addend,
                                              double* __restrict__ diag) {
-    QJ_STRIDED_SETUP(T, N)
+    using S = PencilShape<N, Cfg::TH>;
+    constexpr int VW = Cfg::VW;
+    constexpr int E = S::E;
+    constexpr int TT = S::TT;
+    __shared__ cpx<T> lds[S::kLdsStrided * VW];
+    const int col = threadIdx.x % S::C, j = threadIdx.x / S::C;
+    const int q0 = (blockIdx.x * S::C + col) * VW;
+    const bool act = q0 < g.ncols;
+    const int base = act ? (int)pencil_base(g, q0) : 0;
+    const int se = (int)g.s_elem;
     cpx<T> v[VW][E];
     load_herm_rows<N, VW, E>(v, H, base, se, j, TT, act);
     if constexpr (!GRAD) {
-      pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
+      pencil_fft_strided<N, +1, T, VW, false, Cfg::TH>(v, j, col, lds, tw);
       if (act) {
   #pragma unroll
         for (int m = 0; m < E; ++m) {
@@ -684,8 +694,8 @@ struct k_poisson_x {
           }
         }
       }
-      pencil_fft_strided<N, +1, T, VW>(u, j, col, lds, tw);
-      pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
+      pencil_fft_strided<N, +1, T, VW, false, Cfg::TH>(u, j, col, lds, tw);
+      pencil_fft_strided<N, +1, T, VW, false, Cfg::TH>(v, j, col, lds, tw);
       T amax = (T)0;
       if (act) {
   #pragma unroll
@@ -1119,13 +1129,14 @@ hipError_t launch_poisson_x(const Dims& d, const SpecTables<T>& tb, int grad, co
   const cpx<T>* tw = (const cpx<T>*)tb.tw_x;
   const double kfx = 2.0 * M_PI / d.Lx;
   QJ_DISPATCH_N(d.nx, {
-    const int nb = nblocks_s<T, N_>(g.ncols);
+    using Cfg = StridedCfg<T, N_, true>;
+    const int nb = Cfg::blocks(g.ncols);
     if (grad)
-      launch<k_poisson_x<T, N_, true>>(nb, 256, s,
+      launch<k_poisson_x<T, N_, true>>(nb, Cfg::TH, s,
           (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A3,
           (const T*)addend, diag);
     else
-      launch<k_poisson_x<T, N_, false>>(nb, 256, s,
+      launch<k_poisson_x<T, N_, false>>(nb, Cfg::TH, s,
           (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A3,
           (const T*)addend, diag);
   });
