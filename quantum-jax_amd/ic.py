@@ -78,16 +78,39 @@ def gas_ic(shape, seed: int, L=(1.0, 1.0, 1.0), base: float = 0.2, amp: float = 
 # ---------------------------------------------------------------------------
 # The reference's own initial conditions
 # ---------------------------------------------------------------------------
+def k_squared_f32(shape, L=(1.0, 1.0, 1.0)):
+    """k^2 as the reference computes it at its default precision (x64 off,
+    quantum-fluid.py:50-51,128-134): k = float32(2 pi / L) * float32(i - n/2),
+    ifftshifted, and k^2 = (kx*kx + ky*ky) + kz*kz, every operation rounded
+    to float32."""
+    ks = []
+    for n, Li in zip(shape, L):
+        c = np.float32(2.0 * np.pi / Li)
+        ks.append(np.fft.ifftshift(c * np.arange(-n / 2, n / 2, dtype=np.float32)))
+    kx, ky, kz = np.meshgrid(*ks, indexing="ij")
+    return (kx * kx + ky * ky) + kz * kz
+
+
 def random_phase_psi(shape, L, sigma: float, m_per_hbar: float, mean_rho: float,
-                     seed: int = 17):
+                     seed: int = 17, k2_precision: str = "float32"):
     """quantum-fluid.py:509-522: unit-modulus random phases assigned in
     increasing |k| order (stable argsort of k^2, so the lowest modes take the
     first numbers of NumPy's legacy seeded generator), amplitude
     sqrt(exp(-k^2 / (2 sigma^2 m_per_hbar^2))), inverse FFT, and mean
-    |psi|^2 = mean_rho (the reference's frac_dm * rho_bar)."""
+    |psi|^2 = mean_rho (the reference's frac_dm * rho_bar).
+
+    k2_precision sets the sort key: "float32" is k^2 rounded as the
+    reference computes it by default (x64 off, k_squared_f32), "float64" as
+    it computes it with jax_enable_x64.  Modes of one |k| shell tie in exact
+    arithmetic, and the two roundings break those ties differently (at 32^3
+    the orders part from the 98th mode on), so each mode gets a different
+    random phase.  The amplitudes and the normalisation are FP64 either way."""
+    if k2_precision not in ("float32", "float64"):
+        raise ValueError("k2_precision must be 'float32' or 'float64'")
     k2 = k_squared(shape, L)
+    key = k_squared_f32(shape, L) if k2_precision == "float32" else k2
     rs = np.random.RandomState(seed)  # np.random.seed(seed); np.random.rand
-    order = np.argsort(k2.ravel(), kind="stable")
+    order = np.argsort(key.ravel(), kind="stable")
     psi = np.zeros(k2.size, dtype=complex)
     psi[order] = np.exp(1.0j * 2.0 * np.pi * rs.rand(k2.size))
     psi = psi.reshape(k2.shape) * np.sqrt(np.exp(-k2 / (2.0 * sigma ** 2 * m_per_hbar ** 2)))

@@ -27,6 +27,37 @@ def test_random_phase_psi_amplitude_phases_and_order():
     assert np.allclose(ph, 2 * np.pi * u[1:7], atol=1e-9)
 
 
+@pytest.mark.parametrize("prec", ["float32", "float64"])
+def test_random_phase_psi_assigns_the_stream_in_its_sort_order(prec):
+    shape, L = (16, 16, 16), (1.0, 1.0, 1.0)
+    psi = ic.random_phase_psi(shape, L, 40.0, 0.5, 1.0, seed=17, k2_precision=prec)
+    F = np.fft.fftn(psi).ravel()
+    if prec == "float32":
+        # the key, computed independently with torch's float32 CPU arithmetic
+        import torch
+        c = torch.tensor(2 * np.pi, dtype=torch.float32)
+        k = torch.fft.ifftshift(c * torch.arange(-8, 8, dtype=torch.float32))
+        kx, ky, kz = torch.meshgrid(k, k, k, indexing="ij")
+        key = ((kx * kx + ky * ky) + kz * kz).numpy().ravel()
+        assert np.array_equal(key, ic.k_squared_f32(shape, L).ravel())
+    else:
+        key = ic.k_squared(shape, L).ravel()
+    order = np.argsort(key, kind="stable")
+    u = np.random.RandomState(17).rand(F.size)
+    ph = np.angle(F[order]) % (2 * np.pi)
+    d = np.abs(ph - 2 * np.pi * u)
+    assert np.all(np.minimum(d, 2 * np.pi - d) < 1e-8)
+
+
+def test_random_phase_orders_of_the_two_precisions_differ():
+    k32 = ic.k_squared_f32((32, 32, 32)).ravel()
+    k64 = ic.k_squared((32, 32, 32)).ravel()
+    a, b = np.argsort(k32, kind="stable"), np.argsort(k64, kind="stable")
+    first = np.flatnonzero(a != b)[0]
+    assert first == 97                      # inside the |k|^2 = 9 (2 pi)^2 shell
+    assert np.isclose(k64[a[first]], k64[b[first]])   # same shell, other tie order
+
+
 def test_blob_psi_density_and_peak():
     shape, L = (32, 32, 8), (128.0, 128.0, 32.0)
     psi = ic.blob_psi(shape, L, mean_rho=5.0)
