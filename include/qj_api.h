@@ -295,7 +295,14 @@ int qj_deposit_adjoint(qj_plan* plan, const double* pos, const double* mass, dou
  * star deposit (default), 0 = float-atomic deposit (cheaper for sparse
  * stars on small grids).  Results agree to rounding. */
 #define QJ_OPT_BINNED_DEPOSIT 1
-#define QJ_OPT_COUNT 2
+/* QJ_OPT_DETERMINISTIC_DEPOSIT: 1 = the tile-binned deposit adds the
+ * corner values as 64-bit fixed-point integers per tile (order-independent
+ * sums: stars runs reproduce bit for bit, and rho_s is within FP64 rounding
+ * of the exact sum); 0 (default) = float adds in LDS, whose order varies
+ * from run to run.  Needs the binned deposit (star_scratch given and
+ * QJ_OPT_BINNED_DEPOSIT on). */
+#define QJ_OPT_DETERMINISTIC_DEPOSIT 2
+#define QJ_OPT_COUNT 3
 int qj_plan_set_option(qj_plan* plan, int option, int value);
 int qj_plan_get_option(const qj_plan* plan, int option, int* value);
 

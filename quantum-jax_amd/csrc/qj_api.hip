@@ -150,7 +150,8 @@ struct qj_plan {
   double drift_key_dt = NAN, drift_key_mph = NAN;
   double* partials = nullptr;
   int nblk_src = 0;
-  int opt[QJ_OPT_COUNT] = {1, 1};  // schedule options (qj_plan_set_option): fused x, binned
+  // schedule options (qj_plan_set_option): fused x, binned, deterministic
+  int opt[QJ_OPT_COUNT] = {1, 1, 0};
   // optional per-stage event timing (tracing hook of the bench harness)
   int timing = 0;
   struct Rec {
@@ -589,7 +590,7 @@ int initial_grid_t(qj_plan* p, qj_state* st, const qj_params* pr, const Work& w,
              "bin");
     QJ_STAGE(kStageDeposit,
              launch_tile_deposit<T>(p->d, st->pos, st->mass, st->m_scalar, st->np, sb, st->rho_s,
-                                    s),
+                                    s, p->opt[QJ_OPT_DETERMINISTIC_DEPOSIT]),
              "deposit");
   } else if (stars) {
     // the float-atomic deposit accumulates into a cleared plane
@@ -677,7 +678,7 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
                "star step");
       QJ_STAGE(kStageDeposit,
                launch_tile_deposit<T>(p->d, st->pos, st->mass, st->m_scalar, st->np, sb,
-                                      st->rho_s, s),
+                                      st->rho_s, s, p->opt[QJ_OPT_DETERMINISTIC_DEPOSIT]),
                "deposit");
     } else if (stars) {
       // the float-atomic deposit accumulates into a cleared plane

@@ -195,11 +195,12 @@ template <class T>
 hipError_t launch_star_step_binned(const Dims& d, double* pos, double* vel, long long np,
                                    const void* A3, double h_owed, double h1, double dt, int move,
                                    const StarBins& sb, hipStream_t s);
-// Scan, scatter and per-tile deposit: overwrites every cell of rho.
+// Scan, scatter and per-tile deposit: overwrites every cell of rho.  det:
+// order-independent fixed-point accumulation (bitwise reproducible).
 template <class T>
 hipError_t launch_tile_deposit(const Dims& d, const double* pos, const double* mass,
                                double m_scalar, long long np, const StarBins& sb, void* rho,
-                               hipStream_t s);
+                               hipStream_t s, int det = 0);
 
 // Interpolation adjoint whose h vbar corner values go to abar3 through the
 // tile-binned deposit (whole tiles written, no clearing needed).  taped: sb

@@ -438,6 +438,80 @@ struct k_tile_deposit {
   }
 };
 
+// Deterministic form of k_tile_deposit: the corner values are added as
+// 64-bit fixed-point integers, whose sum does not depend on the order of
+// the additions, so every run gives the same bits.  The tile's scale 2^sh
+// is a power of two chosen from an upper bound on any cell's total, B =
+// (stars registered in the tile) max|m| / vol < 2^ex, so that |sum| 2^sh <
+// 2^62; each corner value is rounded to 2^-sh (about 2^-62 B, far below
+// FP64 resolution for tiles of up to thousands of stars) and the exact sum
+// is rounded once to T.  A non-finite mass in the tile makes the tile NaN.
+template <class T, int VOL = kTileMaxCells>
+struct k_tile_deposit_det {
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(CicGeom g, TileGeom tg,
+                                             const double* __restrict__ pos,
+                                             const double* __restrict__ mass, double m_scalar,
+                                             const int* __restrict__ offset,
+                                             const int* __restrict__ sorted, T* __restrict__ rho) {
+    __shared__ unsigned long long acc[VOL];
+    __shared__ double wmax[kThreads / 64];
+    const int t = blockIdx.x;
+    const int tz = t % tg.ntz, ty = (t / tg.ntz) % tg.nty, tx = t / (tg.ntz * tg.nty);
+    const int vol = 1 << (tg.lx + tg.ly + tg.lz);
+    for (int l = threadIdx.x; l < vol; l += blockDim.x) acc[l] = 0ull;
+    const int b = offset[t], e = offset[t + 1];
+    // max |m| over the tile's stars (NaN-propagating; order-independent)
+    double mx = mass ? 0.0 : fabs(m_scalar);
+    if (mass) {
+      for (int i = b + (int)threadIdx.x; i < e; i += blockDim.x) {
+        const double a = fabs(mass[sorted[i]]);
+        mx = (a > mx || a != a) ? a : mx;
+      }
+    }
+  #pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double y = __shfl_xor(mx, o, 64);
+      mx = (y > mx || y != y) ? y : mx;
+    }
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
+    __syncthreads();
+  #pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) mx = (wmax[w] > mx || wmax[w] != wmax[w]) ? wmax[w] : mx;
+    const double bound = (double)(e - b) * mx / g.vol;
+    const bool bad = !isfinite(bound);
+    int ex = 0;
+    if (bound > 0.0 && !bad) (void)frexp(bound, &ex);  // bound < 2^ex
+    const int sh = 62 - ex;
+    if (bound > 0.0 && !bad) {
+      for (int i = b + (int)threadIdx.x; i < e; i += blockDim.x) {
+        const int p = sorted[i];
+        Cic c = cic(g, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]);
+        const double fac = (mass ? mass[p] : m_scalar) / g.vol;
+  #pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const int sx = kCorner[k][0], sy = kCorner[k][1], sz = kCorner[k][2];
+          const int ix = c.ix[sx], iy = c.iy[sy], iz = c.iz[sz];
+          if ((ix >> tg.lx) != tx || (iy >> tg.ly) != ty || (iz >> tg.lz) != tz) continue;
+          const double w = c.wx[sx] * c.wy[sy] * c.wz[sz] * fac;
+          const int l = (((ix & ((1 << tg.lx) - 1)) << tg.ly | (iy & ((1 << tg.ly) - 1))) << tg.lz) |
+                        (iz & ((1 << tg.lz) - 1));
+          // two's-complement addition: signed values add exactly
+          atomicAdd(&acc[l], (unsigned long long)(long long)rint(ldexp(w, sh)));
+        }
+      }
+    }
+    __syncthreads();
+    const int x0 = tx << tg.lx, y0 = ty << tg.ly, z0 = tz << tg.lz;
+    const int mz = (1 << tg.lz) - 1, my = (1 << tg.ly) - 1;
+    for (int l = threadIdx.x; l < vol; l += blockDim.x) {
+      const int lz = l & mz, ly = (l >> tg.lz) & my, lx = l >> (tg.lz + tg.ly);
+      const T v = bad ? (T)NAN : (T)ldexp((double)(long long)acc[l], -sh);
+      rho[((long long)(x0 + lx) * g.ny + (y0 + ly)) * g.nz + (z0 + lz)] = v;
+    }
+  }
+};
+
 // Vector form of the per-tile deposit for the adjoint of the star kicks:
 // abar[c] = sum over stars of w_c h vbar (the corner values of the atomic
 // interpolation adjoint), written as whole (a_x, a_y, a_z) tiles.
@@ -661,15 +735,20 @@ hipError_t launch_star_step_binned(const Dims& d, double* pos, double* vel, long
 template <class T>
 hipError_t launch_tile_deposit(const Dims& d, const double* pos, const double* mass,
                                double m_scalar, long long np, const StarBins& sb, void* rho,
-                               hipStream_t s) {
+                               hipStream_t s, int det) {
   const TileGeom tg = make_tiles(d);
   launch<k_bin_scan>(1, 1024, s, sb.count, sb.offset, tg.ntiles);
   if (np > 0)
     launch<k_bin_scatter>(pblocks(np), 256, s, make_geom(d), tg, pos, np, (const int*)sb.rank,
                           (const int*)sb.offset, sb.sorted);
-  QJ_TILE_VOL(tg, launch<k_tile_deposit<T, VOL_>>(tg.ntiles, 256, s, make_geom(d), tg, pos, mass,
-                                                  m_scalar, (const int*)sb.offset,
-                                                  (const int*)sb.sorted, (T*)rho));
+  if (det)
+    QJ_TILE_VOL(tg, launch<k_tile_deposit_det<T, VOL_>>(tg.ntiles, 256, s, make_geom(d), tg, pos,
+                                                        mass, m_scalar, (const int*)sb.offset,
+                                                        (const int*)sb.sorted, (T*)rho));
+  else
+    QJ_TILE_VOL(tg, launch<k_tile_deposit<T, VOL_>>(tg.ntiles, 256, s, make_geom(d), tg, pos, mass,
+                                                    m_scalar, (const int*)sb.offset,
+                                                    (const int*)sb.sorted, (T*)rho));
   return hipGetLastError();
 }
 
@@ -725,7 +804,7 @@ hipError_t launch_interp_adj_binned(const Dims& d, const double* pos, long long 
                                                  const void*, double, double, double, int,         \
                                                  const StarBins&, hipStream_t);                    \
   template hipError_t launch_tile_deposit<T>(const Dims&, const double*, const double*, double,    \
-                                             long long, const StarBins&, void*, hipStream_t);     \
+                                             long long, const StarBins&, void*, hipStream_t, int);\
   template hipError_t launch_interp_adj_binned<T>(const Dims&, const double*, long long,           \
                                                   const void*, double, double*, double, double*,   \
                                                   void*, const StarBins&, bool, hipStream_t);      \
