@@ -75,6 +75,33 @@ int qj_acceleration_grid(qj_plan* plan, const void* rho, double rho_bar, double 
                          int dc_mode, void* V, void* a_grid, double* diag, void* work,
                          void* stream);
 
+/* The low-pass-filtered gravity of the black-hole force of
+ * quantum-fluid-accretion.py:140-146,182-191,719-725: V_hat of
+ * get_potential_bg(rho, rho_bg) (the k = 0 bin divided by 1) times
+ * LP = exp(-(k^2 / (k_cut^2 + 1e-30))^4), then a = -Re IFFT(1j k V_hat LP)
+ * (accel_from_Vhat) into a_grid ((nx,ny,nz,3) reals) and the filtered
+ * potential into V (nullable).  Sample a_grid at the hole with qj_interp. */
+int qj_acceleration_grid_lp(qj_plan* plan, const void* rho, double rho_bg, double four_pi_G,
+                            double k_cut, void* V, void* a_grid, void* work, void* stream);
+
+/* bh_bondi_step_ambient of quantum-fluid-accretion.py:440-510 on the gas
+ * density rho (real N, in place; cubic grid): sink radius r_acc =
+ * r_acc_mult dx, ambient rho_inf (and, with vx/vy/vz given, the gas velocity
+ * for the Bondi-Hoyle-Lyttleton rate) averaged over the shell
+ * [max(ann_lo r_acc, r_acc + dx/4), max(ann_hi r_acc, r_lo + dx/2)], target
+ * dM/dt = 4 pi lam (G M)^2 rho_inf / (cs^2 + v_rel^2)^1.5, removal
+ * dM_cap = min(dM/dt dt, fmax M_avail) (fmax < 0: the reference's None, no
+ * fraction) spread with the weights exp(-R^2/(2 rK^2)) rho inside r_acc.
+ * bh: 7 device doubles (M_eff, x, y, z, vx, vy, vz); out: 8 device doubles
+ * (dMdt_target, rho_inf, r_B, dM_cap, r_acc, M_avail, dM_uncl, shell
+ * cells).  Cells farther than the shell from the hole are not touched (the
+ * reference clips every cell at 0, which leaves non-negative densities as
+ * they are). */
+int qj_bondi_sink(qj_plan* plan, void* rho, const void* vx, const void* vy, const void* vz,
+                  const double* bh, double cs, double G, double dt, double r_acc_mult,
+                  double lam, double fmax, double ann_lo, double ann_hi, double* out,
+                  void* stream);
+
 /* psi <- exp(-1j coef V) psi; coef = m_per_hbar*dt/2. quantum-stars.py:301,318 */
 int qj_kick(qj_plan* plan, void* psi, const void* V, double coef, void* stream);
 

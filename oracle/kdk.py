@@ -31,6 +31,16 @@ Parity status per function (pins live in ``tests/test_oracle.py``):
                               exact isothermal Riemann solution)
   compute_step_gas            pinned (mass conservation, reversibility of the
                               spectral part, uniform-gas reduction to plain)
+  lp_factor /                 pinned (single-mode closed form: the filtered
+  acceleration_grid_lp        gradient is the unfiltered one times the factor;
+                              k = 0 passes; large k_cut = unfiltered)
+  sample_cic                  pinned (constant and periodic-linear fields exact;
+                              equal to interp_grid's CIC)
+  bh_bondi_step_ambient       pinned (uniform gas: rho_inf exact, Bondi rate
+                              closed form, removed mass = dM_cap, removal
+                              profile W rho inside r_acc only; BHL rate with a
+                              uniform flow; the M_avail cap)
+  seed_mass_from_time         pinned (limits of the tanh ramp, the step form)
 Unpinned: the reference's FP32 rounding trajectory (needs a JAX import).
 """
 
@@ -56,6 +66,8 @@ __all__ = [
     "filtered_gaussian_psi", "filtered_gaussian_field", "star_ic", "gas_ic",
     "compute_dt_cosmo", "compute_step_cosmo", "external_point_mass", "compute_step_tidal",
     "do_accretion", "compute_step_bh",
+    "lp_factor", "acceleration_grid_lp", "sample_cic", "bh_bondi_step_ambient",
+    "seed_mass_from_time",
 ]
 
 _WORKERS = 1
@@ -669,3 +681,105 @@ def compute_step_bh(psi, m_bh, pos, vel, t, dt, g: Grid, ph: Physics, G, c, sigm
     psi, pos, vel, t, _, _ = compute_step_stars(psi, pos, vel, t, 0.0, np.inf, dt, m_bh, g, ph)
     psi, m_bh = do_accretion(psi, pos, vel, m_bh, g, ph, G, c, sigma, dt)
     return psi, m_bh, pos, vel, t
+
+
+# ---------------------------------------------------------------------------
+# black-hole force and ambient Bondi sink (quantum-fluid-accretion.py)
+# ---------------------------------------------------------------------------
+def lp_factor(g: Grid, k_cut):
+    """quantum-fluid-accretion.py:143-145: exp(-(k^2 / (k_cut^2 + 1e-30))^4)."""
+    m = make_grid(g)
+    return np.exp(-(m.k_sq / (k_cut ** 2 + 1e-30)) ** 4)
+
+
+def acceleration_grid_lp(rho, g: Grid, ph: Physics, k_cut):
+    """accel_from_Vhat(V_hat * LP) with V_hat of get_potential_bg(rho,
+    rho_bar) (quantum-fluid-accretion.py:182-191,719-722); (3, nx, ny, nz)."""
+    m = make_grid(g)
+    V_hat = _v_hat(rho, g, ph) * lp_factor(g, k_cut)
+    ax = -np.real(ifftn(1.0j * m.kx * V_hat))
+    ay = -np.real(ifftn(1.0j * m.ky * V_hat))
+    az = -np.real(ifftn(1.0j * m.kz * V_hat))
+    return np.stack((ax, ay, az))
+
+
+def sample_cic(field, x, g: Grid):
+    """_sample_cic of quantum-fluid-accretion.py:419-438 (cubic cells of
+    size dx; field (nx, ny, nz), x one position)."""
+    dx = g.Lx / g.nx
+    n = g.nx
+    xi = np.asarray(x, dtype=np.float64) / dx - 0.5
+    i0 = np.floor(xi).astype(int)
+    f = xi - i0
+    ii = np.array([i0[0], i0[0] + 1, i0[0], i0[0], i0[0] + 1, i0[0] + 1, i0[0], i0[0] + 1])
+    jj = np.array([i0[1], i0[1], i0[1] + 1, i0[1], i0[1] + 1, i0[1], i0[1] + 1, i0[1] + 1])
+    kk = np.array([i0[2], i0[2], i0[2], i0[2] + 1, i0[2], i0[2] + 1, i0[2] + 1, i0[2] + 1])
+    w = np.array([
+        (1 - f[0]) * (1 - f[1]) * (1 - f[2]),
+        f[0] * (1 - f[1]) * (1 - f[2]),
+        (1 - f[0]) * f[1] * (1 - f[2]),
+        (1 - f[0]) * (1 - f[1]) * f[2],
+        f[0] * f[1] * (1 - f[2]),
+        f[0] * (1 - f[1]) * f[2],
+        (1 - f[0]) * f[1] * f[2],
+        f[0] * f[1] * f[2],
+    ])
+    vals = field[np.mod(ii, n), np.mod(jj, n), np.mod(kk, n)]
+    return np.sum(w * vals)
+
+
+def bh_bondi_step_ambient(rho, M_bh_eff, x_bh, g: Grid, cs_iso, G, dt, r_acc_mult=2.5,
+                          lam=float(np.exp(1.5) / 4.0), fmax=None, ann_lo=1.1, ann_hi=1.8,
+                          vx=None, vy=None, vz=None, vxbh=0.0, vybh=0.0, vzbh=0.0):
+    """quantum-fluid-accretion.py:440-510 (cubic box; defaults of :62-66).
+    Returns (rho_new, dMdt_target, rho_inf, rB, dM_cap, r_acc, M_avail_kernel,
+    dM_uncl)."""
+    nx = rho.shape[0]
+    dx = g.Lx / g.nx
+    Lx = g.Lx
+    xs = (np.arange(nx) + 0.5) * dx
+
+    def pdelta(x0):  # :384-386
+        d = np.abs(xs - x0)
+        return np.minimum(d, Lx - d)
+
+    dX, dY, dZ = np.meshgrid(pdelta(x_bh[0]), pdelta(x_bh[1]), pdelta(x_bh[2]), indexing="ij")
+    R = np.sqrt(dX * dX + dY * dY + dZ * dZ)
+    rB = G * M_bh_eff / (cs_iso * cs_iso + 1e-30)
+    r_acc = r_acc_mult * dx
+    rK = min(0.5 * r_acc, max(0.5 * dx, rB))
+    mask_sink = R <= r_acc
+    W_sink = np.exp(-0.5 * (R / (rK + 1e-30)) ** 2) * mask_sink
+    r_lo = max(ann_lo * r_acc, r_acc + 0.25 * dx)
+    r_hi = max(ann_hi * r_acc, r_lo + 0.50 * dx)
+    shell = (R >= r_lo) & (R <= r_hi)
+    rho_inf = np.sum(rho * shell) / (np.sum(shell) + 1e-30)
+    if vx is not None and vy is not None and vz is not None:
+        vx_inf = np.sum(vx * shell) / (np.sum(shell) + 1e-30)
+        vy_inf = np.sum(vy * shell) / (np.sum(shell) + 1e-30)
+        vz_inf = np.sum(vz * shell) / (np.sum(shell) + 1e-30)
+        v_rel2 = (vxbh - vx_inf) ** 2 + (vybh - vy_inf) ** 2 + (vzbh - vz_inf) ** 2
+        denom = (cs_iso ** 2 + v_rel2) ** 1.5
+    else:
+        denom = cs_iso ** 3
+    dMdt_target = 4.0 * np.pi * lam * (G * M_bh_eff) ** 2 * rho_inf / (denom + 1e-30)
+    dM_uncl = dMdt_target * dt
+    WRho = W_sink * rho
+    sumWRho = np.sum(WRho) + 1e-30
+    phi = WRho / sumWRho
+    M_avail_kernel = np.sum(rho * mask_sink) * dx ** 3
+    dM_cap = min(dM_uncl, M_avail_kernel) if fmax is None else min(dM_uncl, fmax * M_avail_kernel)
+    rho_new = np.maximum(0.0, rho - (dM_cap / dx ** 3) * phi)
+    return rho_new, dMdt_target, rho_inf, rB, dM_cap, r_acc, M_avail_kernel, dM_uncl
+
+
+def seed_mass_from_time(t, placed, init_m, inject_t, ramp_center=None, ramp_w=None):
+    """quantum-fluid-accretion.py:682-690: init_m s(t) for a placed hole, with
+    s = (1 + tanh((t - ramp_center) / (ramp_w + 1e-30))) / 2 for t >= inject_t
+    (the ramp; ramp_center None = the step form s = [t >= inject_t])."""
+    on = 1.0 if placed > 0.5 else 0.0
+    if ramp_center is not None:
+        s = 0.5 * (1.0 + np.tanh((t - ramp_center) / (ramp_w + 1e-30)))
+        s = s * (1.0 if t >= inject_t else 0.0)
+        return init_m * s * on
+    return init_m * (1.0 if t >= inject_t else 0.0) * on

@@ -74,6 +74,9 @@ SIGNATURES = {
     "qj_version": (i32, []),
     "qj_get_potential": (i32, [vp, vp, f64, f64, f64, i32, vp, vp, vp]),
     "qj_acceleration_grid": (i32, [vp, vp, f64, f64, i32, vp, vp, vp, vp, vp]),
+    "qj_acceleration_grid_lp": (i32, [vp, vp, f64, f64, f64, vp, vp, vp, vp]),
+    "qj_bondi_sink": (i32, [vp, vp, vp, vp, vp, vp, f64, f64, f64, f64, f64, f64, f64, f64, vp,
+                            vp]),
     "qj_kick": (i32, [vp, vp, vp, f64, vp]),
     "qj_drift": (i32, [vp, vp, f64, f64, vp]),
     "qj_cic_indices": (i32, [vp, vp, i64, vp, vp, vp, vp, vp]),

@@ -263,11 +263,11 @@ int ensure_drift(qj_plan* p, double dt, double mph, hipStream_t s) {
 template <class T>
 int poisson_tail(qj_plan* p, const Work& w, int grad, double four_pi_G, double rho_bar,
                  int dc_mode, void* V, void* A3, const void* addend,
-                 double* diag, hipStream_t s) {
+                 double* diag, hipStream_t s, double lp_kc2 = 0.0) {
   SpecTables<T> tb = tables<T>(p);
   QJ_STAGE(kStageFftY, launch_fft_y_half<T>(p->d, tb, w.H, s), "fft_y");
   QJ_STAGE(kStagePoissonZ, launch_poisson_z<T>(p->d, tb, grad, w.H, w.Uz, four_pi_G, rho_bar, dc_mode,
-                                                  p->partials, diag, s),
+                                                  p->partials, diag, s, lp_kc2),
            "poisson_z");
   QJ_STAGE(kStagePoissonY, launch_poisson_y<T>(p->d, tb, grad, w.H, w.Uz, w.Uy, s), "poisson_y");
   QJ_STAGE(kStagePoissonX, launch_poisson_x<T>(p->d, tb, grad, w.H, w.Uy, w.Uz, V, A3, addend, diag, s),
@@ -298,7 +298,8 @@ int get_potential_t(qj_plan* p, const void* rho, double rho_bar, double four_pi_
 
 template <class T>
 int accel_grid_t(qj_plan* p, const void* rho, double rho_bar, double four_pi_G, int dc_mode,
-                 void* V, void* a_grid, double* diag, void* work, hipStream_t s) {
+                 void* V, void* a_grid, double* diag, void* work, hipStream_t s,
+                 double lp_kc2 = 0.0) {
   Work w = carve(p, work);
   SpecTables<T> tb = tables<T>(p);
   // the source mass and max|a| land in the caller's diag, else in plan scratch
@@ -307,7 +308,8 @@ int accel_grid_t(qj_plan* p, const void* rho, double rho_bar, double four_pi_G, 
                               p->partials, dg, s),
            "source_x");
   void* Vout = V ? V : w.R;  // the pass always writes V
-  return poisson_tail<T>(p, w, 1, four_pi_G, rho_bar, dc_mode, Vout, a_grid, nullptr, dg, s);
+  return poisson_tail<T>(p, w, 1, four_pi_G, rho_bar, dc_mode, Vout, a_grid, nullptr, dg, s,
+                         lp_kc2);
 }
 
 }  // namespace
@@ -450,6 +452,41 @@ int qj_acceleration_grid(qj_plan* p, const void* rho, double rho_bar, double fou
   hipStream_t s = (hipStream_t)stream;
   return QJ_PRECISION(p, accel_grid_t<float>(p, rho, rho_bar, four_pi_G, dc_mode, V, a_grid, diag, work, s),
                       accel_grid_t<double>(p, rho, rho_bar, four_pi_G, dc_mode, V, a_grid, diag, work, s));
+}
+
+int qj_acceleration_grid_lp(qj_plan* p, const void* rho, double rho_bg, double four_pi_G,
+                            double k_cut, void* V, void* a_grid, void* work, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!rho || !a_grid || !work) return fail(QJ_EARG, "qj_acceleration_grid_lp: null buffer");
+  if (!(k_cut > 0.0)) return fail(QJ_EARG, "qj_acceleration_grid_lp: k_cut must be positive");
+  hipStream_t s = (hipStream_t)stream;
+  const double kc2 = k_cut * k_cut + 1e-30;  // quantum-fluid-accretion.py:145
+  return QJ_PRECISION(p, accel_grid_t<float>(p, rho, rho_bg, four_pi_G, 0, V, a_grid, nullptr,
+                                             work, s, kc2),
+                      accel_grid_t<double>(p, rho, rho_bg, four_pi_G, 0, V, a_grid, nullptr,
+                                           work, s, kc2));
+}
+
+int qj_bondi_sink(qj_plan* p, void* rho, const void* vx, const void* vy, const void* vz,
+                  const double* bh, double cs, double G, double dt, double r_acc_mult,
+                  double lam, double fmax, double ann_lo, double ann_hi, double* out,
+                  void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!rho || !bh || !out) return fail(QJ_EARG, "qj_bondi_sink: null buffer");
+  if ((vx || vy || vz) && !(vx && vy && vz))
+    return fail(QJ_EARG, "qj_bondi_sink: give all three gas velocities or none");
+  if (p->d.nx != p->d.ny || p->d.nx != p->d.nz || p->d.Lx != p->d.Ly || p->d.Lx != p->d.Lz)
+    return fail(QJ_EUNSUPPORTED, "qj_bondi_sink: the sink needs a cubic grid and box");
+  if (!(r_acc_mult > 0.0) || !(ann_lo > 0.0) || !(ann_hi > 0.0) || !(cs >= 0.0))
+    return fail(QJ_EARG, "qj_bondi_sink: r_acc_mult, ann_lo, ann_hi must be positive");
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = QJ_PRECISION(
+      p,
+      launch_bondi_sink<float>(p->d, rho, vx, vy, vz, bh, cs, G, dt, r_acc_mult, lam, fmax,
+                               ann_lo, ann_hi, out, s),
+      launch_bondi_sink<double>(p->d, rho, vx, vy, vz, bh, cs, G, dt, r_acc_mult, lam, fmax,
+                                ann_lo, ann_hi, out, s));
+  return e == hipSuccess ? QJ_OK : hipfail(e, "qj_bondi_sink");
 }
 
 int qj_kick(qj_plan* p, void* psi, const void* V, double coef, void* stream) {

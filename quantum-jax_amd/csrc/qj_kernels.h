@@ -62,7 +62,8 @@ template <class T>
 // the pass writes their compensated total to diag[kDiagMass].
 hipError_t launch_poisson_z(const Dims& d, const SpecTables<T>& tb, int grad, void* H, void* Uz,
                             double four_pi_G, double rho_bar, int dc_mode,
-                            const double* partials, double* diag, hipStream_t s);
+                            const double* partials, double* diag, hipStream_t s,
+                            double lp_kc2 = 0.0);
 
 template <class T>
 hipError_t launch_poisson_y(const Dims& d, const SpecTables<T>& tb, int grad, void* H, void* Uz,
@@ -213,6 +214,15 @@ hipError_t launch_interp_adj_binned(const Dims& d, const double* pos, long long 
                                     hipStream_t s);
 
 // Accretion onto black hole 0 (quantum-bh.py:306-339), in place on psi and m_bh.
+// Ambient Bondi sink (quantum-fluid-accretion.py:440-510) on the gas
+// density, in place; bh = (M_eff, x, y, z, vx, vy, vz) in device memory;
+// out: 8 doubles of device memory; fmax < 0 = no fractional cap.
+template <class T>
+hipError_t launch_bondi_sink(const Dims& d, void* rho, const void* vx, const void* vy,
+                             const void* vz, const double* bh, double cs, double G, double dt,
+                             double r_acc_mult, double lam, double fmax, double ann_lo,
+                             double ann_hi, double* out, hipStream_t s);
+
 template <class T>
 hipError_t launch_bh_accretion(const Dims& d, void* psi, const double* pos, const double* vel,
                                double* m_bh, long long n_bh, double G, double c, double sigma,

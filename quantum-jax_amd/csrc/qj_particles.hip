@@ -601,6 +601,135 @@ struct k_bh_accretion {
   }
 };
 
+// Ambient Bondi sink of quantum-fluid-accretion.py:440-510 (fixed-grid
+// sink radius r_acc = r_acc_mult dx, ambient density and gas velocity
+// averaged over a thin shell outside it, Bondi/BHL target rate, removal
+// weighted by a Gaussian kernel times the density inside r_acc, capped by
+// the gas available there).  One block over the box of cells whose centres
+// can lie within r_hi of the hole; every cell outside the box has a zero
+// mask in the reference as well.  FP64 sums in a fixed order.
+// bh = (M_eff, x, y, z, vx, vy, vz); out = (dMdt_target, rho_inf, r_B,
+// dM_cap, r_acc, M_avail, dM_uncl, shell cell count).
+struct BondiPrm {
+  double cs, G, dt, r_acc_mult, lam, fmax, ann_lo, ann_hi;
+  int box;  // cells per axis of the box (<= n)
+};
+
+template <class T>
+struct k_bondi_sink {
+  static constexpr int kThreads = 256;
+  static __device__ __forceinline__ void run(CicGeom g, BondiPrm q, T* __restrict__ rho,
+                                             const T* __restrict__ vx, const T* __restrict__ vy,
+                                             const T* __restrict__ vz,
+                                             const double* __restrict__ bh,
+                                             double* __restrict__ out) {
+    // the reference's cubic box: dx and L of the x axis for every axis
+    const double dx = g.dx, L = g.Lx;
+    const int n = g.nx;
+    const double M = bh[0], xb[3] = {bh[1], bh[2], bh[3]};
+    const double r_acc = q.r_acc_mult * dx;
+    const double rB = q.G * M / (q.cs * q.cs + 1e-30);
+    const double rK = fmin(0.5 * r_acc, fmax(0.5 * dx, rB));
+    const double r_lo = fmax(q.ann_lo * r_acc, r_acc + 0.25 * dx);
+    const double r_hi = fmax(q.ann_hi * r_acc, r_lo + 0.5 * dx);
+    const int W = q.box, cells = W * W * W;
+    int lo[3];
+  #pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double c = floor(xb[d] / dx - 0.5);
+      lo[d] = (isfinite(c) ? (int)c : 0) - (W - 1) / 2;
+    }
+    // the cell (c) of the box: wrapped indices and distance to the hole
+    auto at = [&](int c, long long& idx, double& R) {
+      int ii[3] = {c / (W * W), (c / W) % W, c % W};
+      double r2 = 0.0;
+  #pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        int i = (lo[d] + ii[d]) % n;
+        if (i < 0) i += n;
+        ii[d] = i;
+        const double dd = fabs(((double)i + 0.5) * dx - xb[d]);  // :384-386
+        const double m = fmin(dd, L - dd);
+        r2 += m * m;
+      }
+      idx = ((long long)ii[0] * g.ny + ii[1]) * g.nz + ii[2];
+      R = sqrt(r2);
+    };
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};  // shell n, rho, vx, vy, vz; sink rho; W rho
+    for (int c = threadIdx.x; c < cells; c += blockDim.x) {
+      long long idx;
+      double R;
+      at(c, idx, R);
+      const double r = (double)rho[idx];
+      if (R >= r_lo && R <= r_hi) {
+        acc[0] += 1.0;
+        acc[1] += r;
+        if (vx) {
+          acc[2] += (double)vx[idx];
+          acc[3] += (double)vy[idx];
+          acc[4] += (double)vz[idx];
+        }
+      }
+      if (R <= r_acc) {
+        const double z = R / (rK + 1e-30);
+        acc[5] += r;
+        acc[6] += exp(-0.5 * z * z) * r;
+      }
+    }
+    __shared__ double red[7][kThreads / 64];
+    __shared__ double tot[7];
+  #pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      double v = wave_sum(acc[k]);
+      if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 7) {
+      double v = 0.0;
+      for (int w = 0; w < kThreads / 64; ++w) v += red[threadIdx.x][w];
+      tot[threadIdx.x] = v;
+    }
+    __syncthreads();
+    const double ns = tot[0];
+    const double rho_inf = tot[1] / (ns + 1e-30);
+    double denom = q.cs * q.cs * q.cs;
+    if (vx) {
+      const double dvx = bh[4] - tot[2] / (ns + 1e-30), dvy = bh[5] - tot[3] / (ns + 1e-30),
+                   dvz = bh[6] - tot[4] / (ns + 1e-30);
+      const double c2v2 = q.cs * q.cs + (dvx * dvx + dvy * dvy + dvz * dvz);
+      denom = c2v2 * sqrt(c2v2);  // (cs^2 + v_rel^2)^1.5
+    }
+    const double GM = q.G * M;
+    const double dMdt = 4.0 * M_PI * q.lam * (GM * GM) * rho_inf / (denom + 1e-30);
+    const double dM_uncl = dMdt * q.dt;
+    const double M_avail = tot[5] * (dx * dx * dx);
+    const double dM_cap = fmin(dM_uncl, q.fmax < 0.0 ? M_avail : q.fmax * M_avail);
+    const double sumWRho = tot[6] + 1e-30;
+    const double take = dM_cap / (dx * dx * dx) / sumWRho;
+    __syncthreads();
+    for (int c = threadIdx.x; c < cells; c += blockDim.x) {
+      long long idx;
+      double R;
+      at(c, idx, R);
+      if (R <= r_acc) {
+        const double z = R / (rK + 1e-30);
+        const double r = (double)rho[idx];
+        rho[idx] = (T)fmax(0.0, r - take * (exp(-0.5 * z * z) * r));
+      }
+    }
+    if (threadIdx.x == 0) {
+      out[0] = dMdt;
+      out[1] = rho_inf;
+      out[2] = rB;
+      out[3] = dM_cap;
+      out[4] = r_acc;
+      out[5] = M_avail;
+      out[6] = dM_uncl;
+      out[7] = ns;
+    }
+  }
+};
+
 inline int pblocks(long long np) { return (int)((np + 255) / 256); }
 
 // Runs stmt with VOL_ = the smallest supported LDS volume holding a tile.
@@ -762,6 +891,23 @@ hipError_t launch_bh_accretion(const Dims& d, void* psi, const double* pos, cons
 }
 
 template <class T>
+hipError_t launch_bondi_sink(const Dims& d, void* rho, const void* vx, const void* vy,
+                             const void* vz, const double* bh, double cs, double G, double dt,
+                             double r_acc_mult, double lam, double fmax_, double ann_lo,
+                             double ann_hi, double* out, hipStream_t s) {
+  const double dx = d.Lx / d.nx, r_acc = r_acc_mult * dx;
+  const double r_lo = fmax(ann_lo * r_acc, r_acc + 0.25 * dx);
+  const double r_hi = fmax(ann_hi * r_acc, r_lo + 0.5 * dx);
+  // cells whose centres can lie within r_hi of the hole, per axis
+  int box = 2 * (int)ceil(r_hi / dx) + 3;
+  if (box > d.nx) box = d.nx;
+  BondiPrm q{cs, G, dt, r_acc_mult, lam, fmax_, ann_lo, ann_hi, box};
+  launch<k_bondi_sink<T>>(1, 256, s, make_geom(d), q, (T*)rho, (const T*)vx, (const T*)vy,
+                          (const T*)vz, bh, out);
+  return hipGetLastError();
+}
+
+template <class T>
 hipError_t launch_interp_adj_binned(const Dims& d, const double* pos, long long np,
                                     const void* A3, double h, double* vbar, double dt_xv,
                                     double* xbar, void* abar3, const StarBins& sb, bool taped,
@@ -808,6 +954,10 @@ hipError_t launch_interp_adj_binned(const Dims& d, const double* pos, long long 
   template hipError_t launch_interp_adj_binned<T>(const Dims&, const double*, long long,           \
                                                   const void*, double, double*, double, double*,   \
                                                   void*, const StarBins&, bool, hipStream_t);      \
+  template hipError_t launch_bondi_sink<T>(const Dims&, void*, const void*, const void*,          \
+                                           const void*, const double*, double, double, double,    \
+                                           double, double, double, double, double, double*,       \
+                                           hipStream_t);                                          \
   template hipError_t launch_bh_accretion<T>(const Dims&, void*, const double*, const double*,    \
                                              double*, long long, double, double, double, double,  \
                                              double, hipStream_t);

@@ -506,6 +506,9 @@ struct k_fft_s {
 // Poisson z pass on the half spectrum: FFT_z, V_hat = -4piG rho_hat/k^2
 // (quantum-stars.py:191, k = 0 divided by 1 with the FP64 source sum),
 // IFFT_z of V_hat and, with GRAD, of i kz V_hat (Nyquist derivative 0).
+// lp_kc2 > 0: V_hat is multiplied by the low-pass factor
+// exp(-(k^2 / lp_kc2)^4) of quantum-fluid-accretion.py:143-145 (lp_kc2 =
+// k_cut^2 + 1e-30; the k = 0 bin keeps factor 1).
 // ---------------------------------------------------------------------------
 template <class T, int N, bool GRAD>
 struct k_poisson_z {
@@ -515,7 +518,8 @@ struct k_poisson_z {
                                              int nx, int ny, double kfx, double kfy, double kfz,
                                              double four_pi_G, double rho_bar, double ntot,
                                              int dc_mode, const double* __restrict__ partials,
-                                             int nparts, double* __restrict__ diag) {
+                                             int nparts, double* __restrict__ diag,
+                                             double lp_kc2) {
     using S = PencilShape<N>;
     __shared__ cpx<T> lds[S::kLds];
     // block 0 holds the kx = ky = 0 pencil: FP64 source mass for the k = 0 bin
@@ -544,7 +548,13 @@ struct k_poisson_z {
       const int kzi = ln.j + S::TT * m;
       const double kz = kfz * (double)freq_index(kzi, N);
       const double k2 = kxy2 + kz * kz;
-      const T Gk = (T)(-four_pi_G * inv_ntot / (k2 == 0.0 ? 1.0 : k2));
+      double gk = -four_pi_G * inv_ntot / (k2 == 0.0 ? 1.0 : k2);
+      if (lp_kc2 > 0.0) {
+        double r = k2 / lp_kc2;
+        r *= r;
+        gk *= exp(-(r * r));
+      }
+      const T Gk = (T)gk;
       v[m] = scale(v[m], Gk);
       if (x == 0 && y == 0 && kzi == 0) {
         double dc = (dc_mode == 0) ? -four_pi_G * (mass - ntot * rho_bar) * inv_ntot : 0.0;
@@ -1084,7 +1094,7 @@ hipError_t launch_fft_y_half(const Dims& d, const SpecTables<T>& tb, void* H, hi
 template <class T>
 hipError_t launch_poisson_z(const Dims& d, const SpecTables<T>& tb, int grad, void* H, void* Uz,
                             double four_pi_G, double rho_bar, int dc_mode,
-                            const double* partials, double* diag, hipStream_t s) {
+                            const double* partials, double* diag, hipStream_t s, double lp_kc2) {
   const int nparts = source_x_blocks_t<T>(d);
   PassGeom g = geom_z(d, d.nx / 2 + 1);
   const cpx<T>* tw = (const cpx<T>*)tb.tw_z;
@@ -1095,11 +1105,11 @@ hipError_t launch_poisson_z(const Dims& d, const SpecTables<T>& tb, int grad, vo
     if (grad)
       launch<k_poisson_z<T, N_, true>>(nblocks<N_>(g.ncols), 256, s,
           (cpx<T>*)H, (cpx<T>*)Uz, g, tw, nyl, d.nx, d.ny, kfx, kfy, kfz, four_pi_G, rho_bar,
-          ntot, dc_mode, partials, nparts, diag);
+          ntot, dc_mode, partials, nparts, diag, lp_kc2);
     else
       launch<k_poisson_z<T, N_, false>>(nblocks<N_>(g.ncols), 256, s,
           (cpx<T>*)H, (cpx<T>*)Uz, g, tw, nyl, d.nx, d.ny, kfx, kfy, kfz, four_pi_G, rho_bar,
-          ntot, dc_mode, partials, nparts, diag);
+          ntot, dc_mode, partials, nparts, diag, lp_kc2);
   });
   return hipGetLastError();
 }
@@ -1253,7 +1263,7 @@ hipError_t launch_adj_poisson(const Dims& d, const SpecTables<T>& tb, const void
                                            hipStream_t);                                        \
   template hipError_t launch_poisson_z<T>(const Dims&, const SpecTables<T>&, int, void*, void*, \
                                           double, double, int, const double*, double*,          \
-                                          hipStream_t);                                         \
+                                          hipStream_t, double);                                 \
   template hipError_t launch_poisson_y<T>(const Dims&, const SpecTables<T>&, int, void*, void*, \
                                           void*, hipStream_t);                                  \
   template hipError_t launch_poisson_x<T>(const Dims&, const SpecTables<T>&, int, const void*,  \

@@ -61,3 +61,71 @@ def test_bh_steps_match_oracle(qj, cuda):
     assert rel(st["pos"], s[2]) < 1e-12
     assert rel(st["vel"], s[3]) < 1e-10
     assert st["t"] == pytest.approx(s[4])
+
+
+# ---- the gas-accretion variant's black-hole operators -----------------------
+# (quantum-fluid-accretion.py:140-146,182-191,419-510)
+def _gas(g, seed):
+    rng = np.random.default_rng(seed)
+    rho = 0.5 + 0.2 * np.abs(O.filtered_gaussian_psi(g, seed, sigma_k=6 * np.pi)) ** 2
+    v = [0.05 * rng.standard_normal(g.shape) for _ in range(3)]
+    return rho, v
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+def test_lp_filtered_bh_force_matches_oracle(qj, cuda, dtype):
+    acc_mod = __import__("importlib").import_module("quantum-jax_amd.accretion")
+    g = O.Grid(32, 32, 32, 2.0, 2.0, 2.0)
+    rho, _ = _gas(g, 3)
+    rt = torch.float32 if dtype == C64 else torch.float64
+    rho_d = dev(rho, rt)
+    rho_r = rho_d.double().cpu().numpy()
+    rho_bg = float(rho_r.mean())
+    k_cut = 0.6 * np.pi / g.dx
+    plan = qj.Plan(32, L=2.0, dtype=dtype)
+    phys = qj.Physics(four_pi_G=3.0)
+    _, a = acc_mod.accel_lp(rho_d, plan, phys, k_cut, rho_bg=rho_bg)
+    ref = O.acceleration_grid_lp(rho_r, g, O.Physics(four_pi_G=3.0, rho_bar=rho_bg), k_cut)
+    assert rel(a.permute(3, 0, 1, 2), ref) < (2e-6 if dtype == C64 else 1e-12)
+    # the filter removed something: differs from the unfiltered gradient
+    assert rel(a.permute(3, 0, 1, 2), O.acceleration_grid(rho_r, g, O.Physics(
+        four_pi_G=3.0, rho_bar=rho_bg))) > 1e-3
+    x = np.array([[0.31, 1.52, 0.03], [1.99, 0.5, 1.0]])
+    got = acc_mod.bh_accel_lp(rho_d, dev(x, torch.float64), plan, phys, k_cut, rho_bg=rho_bg)
+    want = np.array([[O.sample_cic(ref[d], p, g) for d in range(3)] for p in x])
+    assert np.allclose(got.cpu().numpy(), want, rtol=0,
+                       atol=(1e-5 if dtype == C64 else 1e-12) * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+@pytest.mark.parametrize("n,x", [(32, (0.31, 0.52, 0.77)), (16, (0.02, 0.97, 0.5)),
+                                 (8, (0.5, 0.5, 0.5))], ids=["32", "16-wrap", "8-whole"])
+@pytest.mark.parametrize("flow,fmax,dt", [(False, None, 1e-3), (True, None, 1e-3),
+                                          (True, 0.3, 1e3), (False, None, 1e3)],
+                         ids=["bondi", "bhl", "fmax-cap", "avail-cap"])
+def test_bondi_sink_matches_oracle(qj, cuda, dtype, n, x, flow, fmax, dt):
+    acc_mod = __import__("importlib").import_module("quantum-jax_amd.accretion")
+    g = O.Grid(n, n, n)
+    rho, v = _gas(g, 7 + n)
+    rt = torch.float32 if dtype == C64 else torch.float64
+    rho_d = dev(rho, rt)
+    v_d = [dev(a, rt) for a in v]
+    rho_r = rho_d.double().cpu().numpy()
+    v_r = [a.double().cpu().numpy() for a in v_d]
+    M, cs, G = 0.01, 0.3, 1.0
+    vbh = (0.02, -0.01, 0.03)
+    bh = dev(np.array([M, *x, *vbh]), torch.float64)
+    plan = qj.Plan(n, dtype=dtype)
+    out, info = acc_mod.bh_bondi_step_ambient(rho_d, bh, plan, cs, G, dt, fmax=fmax,
+                                              v=v_d if flow else None)
+    kw = dict(vx=v_r[0], vy=v_r[1], vz=v_r[2], vxbh=vbh[0], vybh=vbh[1], vzbh=vbh[2]) if flow else {}
+    ref = O.bh_bondi_step_ambient(rho_r, M, np.array(x), g, cs, G, dt, fmax=fmax, **kw)
+    tol = 1e-6 if dtype == C64 else 1e-13
+    assert rel(out, ref[0]) < tol
+    info = info.cpu().numpy()
+    for k in range(7):   # dMdt, rho_inf, rB, dM_cap, r_acc, M_avail, dM_uncl
+        assert info[k] == pytest.approx(ref[1 + k], rel=tol, abs=1e-300), k
+    # the removed mass is dM_cap where nothing is clipped at zero
+    if dt < 1:
+        taken = (rho_d.double() - out.double()).sum().item() * g.dx ** 3
+        assert taken == pytest.approx(info[3], rel=10 * tol)

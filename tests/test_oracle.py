@@ -715,3 +715,102 @@ def test_cfl_dt_sets_the_courant_number():
     dt = O.cfl_dt(*v, cs, g.dx, cfl, 1.0)
     assert abs(np.max((cs + np.linalg.norm(v, axis=0)) * dt / g.dx) - cfl) < 1e-15
     assert O.cfl_dt(*v, cs, g.dx, cfl, 1e-6) == 1e-6
+
+
+# ---------------------------------------------------------------------------
+# black-hole force and ambient Bondi sink (quantum-fluid-accretion.py)
+# ---------------------------------------------------------------------------
+def test_lp_filtered_acceleration_single_mode():
+    # rho = rho_bar + A cos(k.x): the filtered gradient is the unfiltered one
+    # times exp(-(k^2/k_cut^2)^4); a very large k_cut filters nothing
+    g = O.Grid(32, 32, 32)
+    X, Y, Z = O.cell_centres(g)
+    kv = 2 * np.pi * np.array([3, -2, 5])
+    rho = 1.0 + 0.3 * np.cos(kv[0] * X + kv[1] * Y + kv[2] * Z)
+    ph = O.Physics(four_pi_G=2.0, rho_bar=1.0)
+    k2 = float(kv @ kv)
+    for k_cut in (0.8 * np.sqrt(k2), 1.3 * np.sqrt(k2)):
+        f = np.exp(-(k2 / (k_cut ** 2 + 1e-30)) ** 4)
+        a = O.acceleration_grid_lp(rho, g, ph, k_cut)
+        assert np.allclose(a, f * O.acceleration_grid(rho, g, ph), rtol=0, atol=1e-14)
+    assert np.allclose(O.acceleration_grid_lp(rho, g, ph, 1e6), O.acceleration_grid(rho, g, ph),
+                       atol=1e-15)
+    assert O.lp_factor(g, 1.0)[0, 0, 0] == 1.0
+
+
+def test_sample_cic_exact_fields_and_interp_equivalence():
+    g = O.Grid(16, 16, 16)
+    rng = np.random.default_rng(5)
+    assert O.sample_cic(np.full(g.shape, 2.5), rng.random(3), g) == pytest.approx(2.5, abs=1e-15)
+    i, j, k = np.meshgrid(*(np.arange(16.0),) * 3, indexing="ij")
+    lin = 0.3 * i - 1.1 * j + 0.7 * k
+    for _ in range(20):
+        x = (1.0 + 13.0 * rng.random(3)) / 16   # away from the periodic wrap
+        xi = x * 16 - 0.5
+        assert O.sample_cic(lin, x, g) == pytest.approx(0.3 * xi[0] - 1.1 * xi[1] + 0.7 * xi[2],
+                                                        abs=1e-12)
+    f = rng.standard_normal(g.shape)
+    pos = rng.random((30, 3))
+    a = np.array([O.sample_cic(f, p, g) for p in pos])
+    assert np.allclose(a, O.interp_grid(f[None], pos, g)[:, 0], atol=1e-14)
+
+
+def _bondi_case(n=32, rho0=0.7):
+    g = O.Grid(n, n, n)
+    return g, np.full(g.shape, rho0)
+
+
+def test_bondi_sink_uniform_gas():
+    g, rho = _bondi_case()
+    x = np.array([0.31, 0.52, 0.77])
+    M, cs, G, dt, lam = 0.02, 0.3, 1.0, 1e-3, np.exp(1.5) / 4
+    out = O.bh_bondi_step_ambient(rho, M, x, g, cs, G, dt)
+    rho_new, dMdt, rho_inf, rB, dM_cap, r_acc, M_av, dM_uncl = out
+    assert rho_inf == pytest.approx(0.7, rel=1e-14)      # shell average of a uniform field
+    assert dMdt == pytest.approx(4 * np.pi * lam * (G * M) ** 2 * 0.7 / cs ** 3, rel=1e-12)
+    assert r_acc == pytest.approx(2.5 / 32)
+    assert dM_cap == dM_uncl < M_av                      # no cap at this dt
+    dx3 = g.dx ** 3
+    assert np.sum(rho - rho_new) * dx3 == pytest.approx(dM_cap, rel=1e-10)
+    # only cells within r_acc change, by amounts proportional to the kernel
+    d = [np.minimum(np.abs((np.arange(32) + 0.5) / 32 - c), 1 - np.abs((np.arange(32) + 0.5) / 32 - c))
+         for c in x]
+    R = np.sqrt(d[0][:, None, None] ** 2 + d[1][None, :, None] ** 2 + d[2][None, None, :] ** 2)
+    inside = R <= r_acc
+    assert np.all(rho_new[~inside] == rho[~inside]) and np.all(rho_new[inside] < rho[inside])
+    rK = min(0.5 * r_acc, max(0.5 * g.dx, rB))
+    take = (rho - rho_new)[inside]
+    w = np.exp(-0.5 * (R[inside] / rK) ** 2)
+    assert np.allclose(take / w, take[0] / w[0], rtol=1e-9)
+    assert abs(inside.sum() * dx3 * 0.7 - M_av) < 1e-15
+
+
+def test_bondi_sink_caps_and_bhl_rate():
+    g, rho = _bondi_case(16)
+    x = np.array([0.02, 0.97, 0.5])                      # the stencil wraps
+    M, cs, G = 0.05, 0.2, 1.0
+    # a huge dt takes everything the kernel holds (fmax None), or the fraction
+    o = O.bh_bondi_step_ambient(rho, M, x, g, cs, G, 1e6)
+    assert o[4] == o[6] and o[6] < o[7]
+    o = O.bh_bondi_step_ambient(rho, M, x, g, cs, G, 1e6, fmax=0.3)
+    assert o[4] == pytest.approx(0.3 * o[6], rel=1e-15)
+    # a uniform gas flow v past a hole at rest: (cs^2 + v^2)^1.5 replaces cs^3
+    v = np.full(g.shape, 0.15)
+    z = np.zeros(g.shape)
+    lam = np.exp(1.5) / 4
+    o = O.bh_bondi_step_ambient(rho, M, x, g, cs, G, 1e-3, vx=v, vy=z, vz=z)
+    assert o[1] == pytest.approx(4 * np.pi * lam * (G * M) ** 2 * 0.7 / (cs ** 2 + 0.15 ** 2) ** 1.5,
+                                 rel=1e-12)
+    # a hole moving with the gas sees the static rate
+    o2 = O.bh_bondi_step_ambient(rho, M, x, g, cs, G, 1e-3, vx=v, vy=z, vz=z, vxbh=0.15)
+    assert o2[1] == pytest.approx(4 * np.pi * lam * (G * M) ** 2 * 0.7 / cs ** 3, rel=1e-12)
+
+
+def test_seed_mass_ramp():
+    f = lambda t, **kw: O.seed_mass_from_time(t, 1.0, 2.0, 5.0, **kw)
+    assert f(4.9) == 0.0 and f(5.0) == 2.0 and f(50.0) == 2.0
+    assert O.seed_mass_from_time(6.0, 0.0, 2.0, 5.0) == 0.0
+    ramp = dict(ramp_center=7.0, ramp_w=0.5)
+    assert f(4.9, **ramp) == 0.0
+    assert f(7.0, **ramp) == pytest.approx(1.0, rel=1e-15)
+    assert f(30.0, **ramp) == pytest.approx(2.0, rel=1e-12)
