@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -63,6 +64,30 @@ __global__ __launch_bounds__(K::kThreads) void k_many(const Pack<A...>* __restri
   pack_apply<K>(p, std::make_integer_sequence<int, sizeof...(A)>{});
 }
 
+// A body may ask for a minimum number of waves per SIMD (K::kMinWaves): the
+// compiler then keeps its registers within 512 / kMinWaves.  Such bodies get
+// wrappers of their own, so the others keep their default schedules.
+template <class K, class = void>
+struct MinWaves {
+  static constexpr int value = 0;
+};
+template <class K>
+struct MinWaves<K, std::void_t<decltype(K::kMinWaves)>> {
+  static constexpr int value = K::kMinWaves;
+};
+
+template <class K, class... A>
+__global__ __launch_bounds__(K::kThreads, MinWaves<K>::value) void k_one_w(A... a) {
+  K::run(a...);
+}
+
+template <class K, class... A>
+__global__ __launch_bounds__(K::kThreads, MinWaves<K>::value) void k_many_w(
+    const Pack<A...>* __restrict__ packs) {
+  const Pack<A...> p = packs[blockIdx.z];
+  pack_apply<K>(p, std::make_integer_sequence<int, sizeof...(A)>{});
+}
+
 // One captured launch: the batched wrapper, its geometry and the bytes of
 // its argument pack (pack_align is the pack's alignment).
 struct LaunchRec {
@@ -86,7 +111,10 @@ void launch(dim3 grid, dim3 block, hipStream_t s, A... a) {
   if (g_recorder) {
     const Pack<A...> p = make_pack(a...);
     LaunchRec r;
-    r.fn = reinterpret_cast<const void*>(&k_many<K, A...>);
+    if constexpr (MinWaves<K>::value > 0)
+      r.fn = reinterpret_cast<const void*>(&k_many_w<K, A...>);
+    else
+      r.fn = reinterpret_cast<const void*>(&k_many<K, A...>);
     r.grid = grid;
     r.block = block;
     r.pack_align = alignof(Pack<A...>);
@@ -95,7 +123,10 @@ void launch(dim3 grid, dim3 block, hipStream_t s, A... a) {
     g_recorder->recs.push_back(std::move(r));
     return;
   }
-  k_one<K, A...><<<grid, block, 0, s>>>(a...);
+  if constexpr (MinWaves<K>::value > 0)
+    k_one_w<K, A...><<<grid, block, 0, s>>>(a...);
+  else
+    k_one<K, A...><<<grid, block, 0, s>>>(a...);
 }
 
 // Installs a recorder for the lifetime of the object.

@@ -592,11 +592,26 @@ struct k_poisson_z {
 // ---------------------------------------------------------------------------
 template <class T, int N, bool GRAD>
 struct k_poisson_y {
-  static constexpr int kThreads = 256;
+  // the wide form (one pencil per lane, 512 threads) for the gradient pass,
+  // which holds two spectra per pencil
+  using Cfg = StridedCfg<T, N, GRAD>;
+  static constexpr int kThreads = Cfg::TH;
+  // two 512-thread blocks per CU need four waves per SIMD (128 VGPRs)
+  static constexpr int kMinWaves = Cfg::kWide ? 4 : 0;
   static __device__ __forceinline__ void run(cpx<T>* __restrict__ H, cpx<T>* __restrict__ Uz,
                                              cpx<T>* __restrict__ Uy, PassGeom g,
                                              const cpx<T>* __restrict__ tw, double kfy) {
-    QJ_STRIDED_SETUP(T, N)
+    using S = PencilShape<N, Cfg::TH>;
+    constexpr int VW = Cfg::VW;
+    constexpr int E = S::E;
+    constexpr int TT = S::TT;
+    constexpr int TH = Cfg::TH;
+    __shared__ cpx<T> lds[S::kLdsStrided * VW];
+    const int col = threadIdx.x % S::C, j = threadIdx.x / S::C;
+    const int q0 = (blockIdx.x * S::C + col) * VW;
+    const bool act = q0 < g.ncols;
+    const int base = act ? (int)pencil_base(g, q0) : 0;
+    const int se = (int)g.s_elem;
     cpx<T> v[VW][E];
     load_rows<VW, E>(v, H, base, se, j, TT, act);
     if constexpr (GRAD) {
@@ -608,15 +623,15 @@ struct k_poisson_y {
   #pragma unroll
         for (int x = 0; x < VW; ++x) w[x][m] = {-ky * v[x][m].i, ky * v[x][m].r};
       }
-      pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
+      pencil_fft_strided<N, +1, T, VW, false, TH>(v, j, col, lds, tw);
       store_rows<VW, E>(v, H, base, se, j, TT, act);
-      pencil_fft_strided<N, +1, T, VW>(w, j, col, lds, tw);
+      pencil_fft_strided<N, +1, T, VW, false, TH>(w, j, col, lds, tw);
       store_rows<VW, E>(w, Uy, base, se, j, TT, act);
       load_rows<VW, E>(v, Uz, base, se, j, TT, act);
-      pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
+      pencil_fft_strided<N, +1, T, VW, false, TH>(v, j, col, lds, tw);
       store_rows<VW, E>(v, Uz, base, se, j, TT, act);
     } else {
-      pencil_fft_strided<N, +1, T, VW>(v, j, col, lds, tw);
+      pencil_fft_strided<N, +1, T, VW, false, TH>(v, j, col, lds, tw);
       store_rows<VW, E>(v, H, base, se, j, TT, act);
     }
   }
@@ -1144,8 +1159,8 @@ hipError_t launch_poisson_y(const Dims& d, const SpecTables<T>& tb, int grad, vo
   const double kfy = 2.0 * M_PI / d.Ly;
   QJ_DISPATCH_N(d.ny, {
     if (grad)
-      launch<k_poisson_y<T, N_, true>>(nblocks_s<T, N_>(g.ncols), 256, s,
-          (cpx<T>*)H, (cpx<T>*)Uz, (cpx<T>*)Uy, g, tw, kfy);
+      launch<k_poisson_y<T, N_, true>>(StridedCfg<T, N_, true>::blocks(g.ncols),
+          StridedCfg<T, N_, true>::TH, s, (cpx<T>*)H, (cpx<T>*)Uz, (cpx<T>*)Uy, g, tw, kfy);
     else
       launch<k_poisson_y<T, N_, false>>(nblocks_s<T, N_>(g.ncols), 256, s,
           (cpx<T>*)H, (cpx<T>*)Uz, (cpx<T>*)Uy, g, tw, kfy);
