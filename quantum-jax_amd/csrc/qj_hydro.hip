@@ -37,19 +37,19 @@ __device__ __forceinline__ T clip01(T x) {
 }
 
 // Centred gradient (quantum-fluid.py:201-203) and the ratio-form minmod
-// limiter (:352-407) of one variable along one axis.  The reference's
-// denominator is g + eps (g > 0), g - eps (g < 0) or eps (g = 0); g +
-// copysign(eps, g) differs only for g = -0, where the slope lim * g is -0
-// either way.
+// limiter (:352-407) of one variable along one axis, in cell units: the
+// limited slope times dx (the reference's g and eps = 1e-12 scaled by dx, so
+// the 1/dx factors of its three ratios cancel).  The reference's denominator
+// is g + eps (g > 0), g - eps (g < 0) or eps (g = 0); g + copysign(eps, g)
+// differs only for g = -0, where the slope lim * g is -0 either way.
 template <class T>
-__device__ __forceinline__ T limit(T f, T fm, T fp, T inv_dx) {
-  const T eps = (T)1.0e-12;
-  T g = (fp - fm) * ((T)0.5 * inv_dx);
-  const T denom = g + copysign(eps, g);
+__device__ __forceinline__ T limit(T f, T fm, T fp, T eps_dx) {
+  T g = (T)0.5 * (fp - fm);
+  const T denom = g + copysign(eps_dx, g);
   const T rd = rcp(denom);
-  T lim = clip01(((f - fm) * inv_dx) * rd);
+  T lim = clip01((f - fm) * rd);
   g = lim * g;
-  lim = clip01((-(f - fp) * inv_dx) * rd);
+  lim = clip01((fp - f) * rd);
   return lim * g;
 }
 
@@ -170,7 +170,9 @@ struct k_hydro_stream {
     const int y0 = (tile / ntz) * TY, z0 = (tile % ntz) * TZ, x0 = xc * XC;
     const int ty = tid / TZ, tz = tid % TZ;
     const bool own = tid < nthr;
-    const T dx = g.dx, hdt = (T)0.5 * g.dt, inv_dx = (T)1 / g.dx;
+    // slopes are kept in cell units (times dx): hdx = dt / (2 dx) scales the
+    // predictor, and the faces are p -+ slope / 2
+    const T hdx = (T)0.5 * g.dt / g.dx, eps_dx = (T)1.0e-12 * g.dx;
 
     auto gidx = [&](int x, int y, int z) {
       return (((x & (g.nx - 1)) * g.ny + (y & (g.ny - 1))) * g.nz) + (z & (g.nz - 1));
@@ -227,18 +229,18 @@ struct k_hydro_stream {
       for (int k = 0; k < 4; ++k) v0[k] = raw[sc][k][o];
   #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        s[0][k] = limit<T>(v0[k], raw[sm][k][o], raw[sp][k][o], inv_dx);
-        s[1][k] = limit<T>(v0[k], raw[sc][k][o - RZ], raw[sc][k][o + RZ], inv_dx);
-        s[2][k] = limit<T>(v0[k], raw[sc][k][o - 1], raw[sc][k][o + 1], inv_dx);
+        s[0][k] = limit<T>(v0[k], raw[sm][k][o], raw[sp][k][o], eps_dx);
+        s[1][k] = limit<T>(v0[k], raw[sc][k][o - RZ], raw[sc][k][o + RZ], eps_dx);
+        s[2][k] = limit<T>(v0[k], raw[sc][k][o - 1], raw[sc][k][o + 1], eps_dx);
       }
       const T r = v0[0], u = v0[1], v = v0[2], w = v0[3];
       const T ir = rcp(r);
       // quantum-fluid.py:295-311
-      p[0] = r - hdt * (u * s[0][0] + r * s[0][1] + v * s[1][0] + r * s[1][2] + w * s[2][0] +
+      p[0] = r - hdx * (u * s[0][0] + r * s[0][1] + v * s[1][0] + r * s[1][2] + w * s[2][0] +
                         r * s[2][3]);
-      p[1] = u - hdt * (u * s[0][1] + v * s[1][1] + w * s[2][1] + ir * s[0][0] * g.cs * g.cs);
-      p[2] = v - hdt * (u * s[0][2] + v * s[1][2] + w * s[2][2] + ir * s[1][0] * g.cs * g.cs);
-      p[3] = w - hdt * (u * s[0][3] + v * s[1][3] + w * s[2][3] + ir * s[2][0] * g.cs * g.cs);
+      p[1] = u - hdx * (u * s[0][1] + v * s[1][1] + w * s[2][1] + ir * s[0][0] * g.cs * g.cs);
+      p[2] = v - hdx * (u * s[0][2] + v * s[1][2] + w * s[2][2] + ir * s[1][0] * g.cs * g.cs);
+      p[3] = w - hdx * (u * s[0][3] + v * s[1][3] + w * s[2][3] + ir * s[2][0] * g.cs * g.cs);
     };
     // states of plane x for the tile and its ring into st[x & 1]; own column
     // (p, s_x) returned in xs
@@ -299,8 +301,8 @@ struct k_hydro_stream {
       T fl[4], fr[4];
   #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        fl[k] = xm.p[k] + (T)0.5 * xm.sx[k] * dx;
-        fr[k] = xc0.p[k] - (T)0.5 * xc0.sx[k] * dx;
+        fl[k] = xm.p[k] + (T)0.5 * xm.sx[k];
+        fr[k] = xc0.p[k] - (T)0.5 * xc0.sx[k];
       }
       face_flux<T, FLUX>(fl[0], fl[1], fl[2], fl[3], fr[0], fr[1], fr[2], fr[3], g.cs, Fm);
     }
@@ -329,8 +331,8 @@ struct k_hydro_stream {
         // face of the plane before
   #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          fl[k] = xc0.p[k] + (T)0.5 * xc0.sx[k] * dx;
-          fr[k] = xp.p[k] - (T)0.5 * xp.sx[k] * dx;
+          fl[k] = xc0.p[k] + (T)0.5 * xc0.sx[k];
+          fr[k] = xp.p[k] - (T)0.5 * xp.sx[k];
         }
         face_flux<T, FLUX>(fl[0], fl[1], fl[2], fl[3], fr[0], fr[1], fr[2], fr[3], g.cs, Fp);
   #pragma unroll
@@ -351,8 +353,8 @@ struct k_hydro_stream {
             const int oL = side ? o : o - step, oR = side ? o + step : o;
   #pragma unroll
             for (int k = 0; k < 4; ++k) {
-              fl[k] = st[b][k][oL] + (T)0.5 * st[b][sl + k][oL] * dx;
-              fr[k] = st[b][k][oR] - (T)0.5 * st[b][sl + k][oR] * dx;
+              fl[k] = st[b][k][oL] + (T)0.5 * st[b][sl + k][oL];
+              fr[k] = st[b][k][oR] - (T)0.5 * st[b][sl + k][oR];
             }
             T f[4];
             if (a == 1) {
