@@ -160,7 +160,9 @@ struct k_hydro_stream {
       kick = (T)dslot[kSlotKick];
     }
     __shared__ T raw[4][4][HTile<T>::RY * HTile<T>::RZ];  // [slot][var][cell]
-    __shared__ T st[2][12][HTile<T>::SY * HTile<T>::SZ];  // [buf][p0..3, sy0..3, sz0..3][cell]
+    __shared__ T st[12][HTile<T>::SY * HTile<T>::SZ];  // [p0..3, sy0..3, sz0..3][cell]
+    // +y and +z face fluxes of a plane, double-buffered: [buf][y, z][var][cell]
+    __shared__ T fyz[2][2][4][HTile<T>::SY * HTile<T>::SZ];
     const int tid = threadIdx.x;
     const int nthr = TY * TZ;
     const int RZ = TZ + 4, RY = TY + 4, SZ = TZ + 2, SY = TY + 2;
@@ -242,12 +244,11 @@ struct k_hydro_stream {
       p[2] = v - hdx * (u * s[0][2] + v * s[1][2] + w * s[2][2] + ir * s[1][0] * g.cs * g.cs);
       p[3] = w - hdx * (u * s[0][3] + v * s[1][3] + w * s[2][3] + ir * s[2][0] * g.cs * g.cs);
     };
-    // states of plane x for the tile and its ring into st[x & 1]; own column
+    // states of plane x for the tile and its ring into st; own column
     // (p, s_x) returned in xs
     // one cell per thread: the tile's cells on threads 0..nthr-1, the ring on
     // the next (2*TY + 2*TZ + 4) threads
     auto states = [&](int x, XState<T>& xs) {
-      const int b = x & 1;
       const int ring = SY * SZ - nthr;
       const int c = tid;
       if (c >= nthr + ring) return;
@@ -266,9 +267,9 @@ struct k_hydro_stream {
       const int o = (yy + 1) * SZ + (zz + 1);
   #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        st[b][k][o] = p[k];
-        st[b][4 + k][o] = s3[1][k];
-        st[b][8 + k][o] = s3[2][k];
+        st[k][o] = p[k];
+        st[4 + k][o] = s3[1][k];
+        st[8 + k][o] = s3[2][k];
       }
       if (c < nthr) {
   #pragma unroll
@@ -277,6 +278,64 @@ struct k_hydro_stream {
           xs.sx[k] = s3[0][k];
         }
       }
+    };
+
+    // Flux through the +y (a = 1) or +z (a = 2) face of state cell o, from
+    // the states in st (rotated flux arguments, quantum-fluid.py:331-336).
+    auto face = [&](int a, int o, T F[4]) {
+      const int step = (a == 1) ? SZ : 1;
+      const int sl = 4 * a;  // s_y at 4..7, s_z at 8..11
+      T fl[4], fr[4], f[4];
+  #pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        fl[k] = st[k][o] + (T)0.5 * st[sl + k][o];
+        fr[k] = st[k][o + step] - (T)0.5 * st[sl + k][o + step];
+      }
+      if (a == 1) {
+        face_flux<T, FLUX>(fl[0], fl[2], fl[3], fl[1], fr[0], fr[2], fr[3], fr[1], g.cs, f);
+        F[0] = f[0]; F[2] = f[1]; F[3] = f[2]; F[1] = f[3];
+      } else {
+        face_flux<T, FLUX>(fl[0], fl[3], fl[1], fl[2], fr[0], fr[3], fr[1], fr[2], g.cs, f);
+        F[0] = f[0]; F[3] = f[1]; F[1] = f[2]; F[2] = f[3];
+      }
+    };
+    // Each y and z face of a plane evaluated once: a tile cell takes its +y
+    // and +z faces (kept in Fy, Fz and stored), the ring row below the tile
+    // its +y faces and the ring column left of it its +z faces (the tile's
+    // -y and -z faces), all stored to fyz[x & 1] for the update after the
+    // plane's next barrier.  The states of plane x stay in st until the
+    // states of plane x + 1 overwrite them after that barrier.
+    auto faces = [&](int x, T Fy[4], T Fz[4]) {
+      const int b = x & 1;
+      const int c = tid;
+      if (c < nthr) {
+        const int o = (ty + 1) * SZ + (tz + 1);
+        face(1, o, Fy);
+        face(2, o, Fz);
+  #pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          fyz[b][0][k][o] = Fy[k];
+          fyz[b][1][k][o] = Fz[k];
+        }
+        return;
+      }
+      const int r = c - nthr;
+      if (r < SZ) {  // ring row yy = -1: its +y face is the tile's -y face
+        const int zz = r - 1;
+        if (zz < 0 || zz >= TZ) return;
+        T F[4];
+        face(1, zz + 1, F);
+  #pragma unroll
+        for (int k = 0; k < 4; ++k) fyz[b][0][k][zz + 1] = F[k];
+        return;
+      }
+      const int q = r - 2 * SZ;
+      if (q < 0 || (q & 1) || q / 2 >= TY) return;
+      const int o = (q / 2 + 1) * SZ;  // ring column zz = -1 of row q / 2
+      T F[4];
+      face(2, o, F);
+  #pragma unroll
+      for (int k = 0; k < 4; ++k) fyz[b][1][k][o] = F[k];
     };
 
     XState<T> xm, xc0, xp;
@@ -311,6 +370,9 @@ struct k_hydro_stream {
     // the first put overwrites the slot of plane x0-2, read by the warm-up
     __syncthreads();
     for (int x = x0; x < x0 + XC; ++x) {
+      // the y and z face fluxes of plane x from its states
+      T Fy[4], Fz[4];
+      faces(x, Fy, Fz);
       // plane x+2 (fetched during the previous iteration) into its slot,
       // which planes x-1 .. x+1 in use leave free; then fetch plane x+3
       put(x + 2, pre);
@@ -320,7 +382,6 @@ struct k_hydro_stream {
       __syncthreads();
       if (own) {
         const int b = x & 1;
-        const int o = (ty + 1) * SZ + (tz + 1);
         const int ro = (ty + 2) * RZ + (tz + 2);
         const T r0 = raw[x & 3][0][ro];
         // quantum-fluid.py:338 get_conserved on the (kicked) state of plane x
@@ -341,35 +402,17 @@ struct k_hydro_stream {
           U[k] += fac * Fm[k];
           Fm[k] = Fp[k];
         }
-        // y and z faces from the state plane (rotated flux arguments,
-        // quantum-fluid.py:331-336)
+        // y and z faces: the +faces from registers, the -faces stored by the
+        // neighbouring cell (or ring cell) of the same plane
   #pragma unroll
-        for (int a = 1; a < 3; ++a) {
-          const int step = (a == 1) ? SZ : 1;
-          const int sl = 4 * a;  // s_y at 4..7, s_z at 8..11
-          T F[2][4];
+        for (int k = 0; k < 4; ++k) {
+          U[k] += -fac * Fy[k];
+          U[k] += fac * fyz[b][0][k][ty * SZ + (tz + 1)];
+        }
   #pragma unroll
-          for (int side = 0; side < 2; ++side) {
-            const int oL = side ? o : o - step, oR = side ? o + step : o;
-  #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-              fl[k] = st[b][k][oL] + (T)0.5 * st[b][sl + k][oL];
-              fr[k] = st[b][k][oR] - (T)0.5 * st[b][sl + k][oR];
-            }
-            T f[4];
-            if (a == 1) {
-              face_flux<T, FLUX>(fl[0], fl[2], fl[3], fl[1], fr[0], fr[2], fr[3], fr[1], g.cs, f);
-              F[side][0] = f[0]; F[side][2] = f[1]; F[side][3] = f[2]; F[side][1] = f[3];
-            } else {
-              face_flux<T, FLUX>(fl[0], fl[3], fl[1], fl[2], fr[0], fr[3], fr[1], fr[2], g.cs, f);
-              F[side][0] = f[0]; F[side][3] = f[1]; F[side][1] = f[2]; F[side][2] = f[3];
-            }
-          }
-  #pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            U[k] += -fac * F[1][k];
-            U[k] += fac * F[0][k];
-          }
+        for (int k = 0; k < 4; ++k) {
+          U[k] += -fac * Fz[k];
+          U[k] += fac * fyz[b][1][k][(ty + 1) * SZ + tz];
         }
         const int i = gidx(x, y0 + ty, z0 + tz);
         const T iM = rcp(U[0]);
