@@ -161,7 +161,8 @@ def ensemble_leg(qj, dev, phys, nsteps=50):
     tuned = ens.autotune(reps=3)
     t_tune = time.perf_counter() - t0
     log(f"ensemble autotune: {tuned['mode']} fuse_x_source {tuned['fuse_x_source']} "
-        f"binned_deposit {tuned.get('binned_deposit')} ({t_tune:.1f} s)")
+        f"binned_deposit {tuned.get('binned_deposit')} wide_sweeps {tuned.get('wide_sweeps')} "
+        f"z_sweep_blocks {tuned.get('z_sweep_blocks')} ({t_tune:.1f} s)")
     ens.step(1)
     torch.cuda.synchronize()
     a.record()
@@ -226,7 +227,9 @@ def ensemble_leg(qj, dev, phys, nsteps=50):
         "mode_ms_per_step": modes,
         "speedup_vs_eager": modes["eager"] / ms,
         "autotuned": {"mode": tuned["mode"], "fuse_x_source": tuned["fuse_x_source"],
-                      "binned_deposit": tuned.get("binned_deposit"), "tuning_s": t_tune},
+                      "binned_deposit": tuned.get("binned_deposit"),
+                      "wide_sweeps": tuned.get("wide_sweeps"),
+                      "z_sweep_blocks": tuned.get("z_sweep_blocks"), "tuning_s": t_tune},
         "launch_groups": groups,
         "adjoint": adj,
     }

@@ -329,7 +329,15 @@ int qj_deposit_adjoint(qj_plan* plan, const double* pos, const double* mass, dou
  * from run to run.  Needs the binned deposit (star_scratch given and
  * QJ_OPT_BINNED_DEPOSIT on). */
 #define QJ_OPT_DETERMINISTIC_DEPOSIT 2
-#define QJ_OPT_COUNT 3
+/* QJ_OPT_WIDE_SWEEPS: 1 (default) = the Poisson x and y gradient passes of
+ * complex64 256-point pencils run one pencil per lane in 512-thread
+ * workgroups; 0 = two pencils per lane in 256 threads.  Same results. */
+#define QJ_OPT_WIDE_SWEEPS 3
+/* QJ_OPT_Z_SWEEP_BLOCKS: workgroups of the z drift sweeps, which walk the
+ * pencil groups (0 = default, at most 512; otherwise at most this many).
+ * Same results. */
+#define QJ_OPT_Z_SWEEP_BLOCKS 4
+#define QJ_OPT_COUNT 5
 int qj_plan_set_option(qj_plan* plan, int option, int value);
 int qj_plan_get_option(const qj_plan* plan, int option, int* value);
 

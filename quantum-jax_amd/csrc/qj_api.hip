@@ -150,8 +150,9 @@ struct qj_plan {
   double drift_key_dt = NAN, drift_key_mph = NAN;
   double* partials = nullptr;
   int nblk_src = 0;
-  // schedule options (qj_plan_set_option): fused x, binned, deterministic
-  int opt[QJ_OPT_COUNT] = {1, 1, 0};
+  // schedule options (qj_plan_set_option): fused x, binned, deterministic,
+  // wide Poisson sweeps, z-sweep blocks
+  int opt[QJ_OPT_COUNT] = {1, 1, 0, 1, 0};
   // optional per-stage event timing (tracing hook of the bench harness)
   int timing = 0;
   struct Rec {
@@ -235,6 +236,8 @@ SpecTables<T> tables(const qj_plan* p) {
   t.drift_x = p->drift[0];
   t.drift_y = p->drift[1];
   t.drift_z = p->drift[2];
+  t.wide = p->opt[QJ_OPT_WIDE_SWEEPS];
+  t.zblocks = p->opt[QJ_OPT_Z_SWEEP_BLOCKS];
   return t;
 }
 
@@ -383,6 +386,8 @@ int qj_plan_destroy(qj_plan* p) {
 int qj_plan_set_option(qj_plan* p, int option, int value) {
   if (!p) return fail(QJ_EARG, "null plan");
   if (option < 0 || option >= QJ_OPT_COUNT) return fail(QJ_EARG, "unknown option");
+  if (option == QJ_OPT_Z_SWEEP_BLOCKS && (value < 0 || value > 65535))
+    return fail(QJ_EARG, "QJ_OPT_Z_SWEEP_BLOCKS must be in [0, 65535]");
   p->opt[option] = value;
   return QJ_OK;
 }

@@ -25,6 +25,9 @@ struct SpecTables {
   void* drift_x;     // exp(-i dt k^2/(2 mph)) / n per axis (cpx<T>)
   void* drift_y;
   void* drift_z;
+  // schedule choices of the plan (results do not depend on them)
+  int wide = 1;      // wide form of the Poisson x and y gradient passes
+  int zblocks = 0;   // walking blocks of the z sweeps (0: the default cap)
 };
 
 // Poisson source: rho = |psi|^2 + src (either may be null); source_x also

@@ -590,11 +590,11 @@ struct k_poisson_z {
 // Poisson y pass (inverse): U -> IFFT_y U (in place); with GRAD also
 // IFFT_y(i ky U) -> Uy and IFFT_y Uz (in place).
 // ---------------------------------------------------------------------------
-template <class T, int N, bool GRAD>
+template <class T, int N, bool GRAD, bool WIDE = true>
 struct k_poisson_y {
   // the wide form (one pencil per lane, 512 threads) for the gradient pass,
   // which holds two spectra per pencil
-  using Cfg = StridedCfg<T, N, GRAD>;
+  using Cfg = StridedCfg<T, N, GRAD && WIDE>;
   static constexpr int kThreads = Cfg::TH;
   // two 512-thread blocks per CU need four waves per SIMD (128 VGPRs)
   static constexpr int kMinWaves = Cfg::kWide ? 4 : 0;
@@ -668,9 +668,9 @@ __device__ __forceinline__ void load_herm_rows(cpx<T> (&v)[VW][E], const cpx<T>*
 // the running maximum diag[kDiagAMax] with one atomic per block.
 // quantum-stars.py:192 and :256-260.
 // ---------------------------------------------------------------------------
-template <class T, int N, bool GRAD>
+template <class T, int N, bool GRAD, bool WIDE = true>
 struct k_poisson_x {
-  using Cfg = StridedCfg<T, N, true>;
+  using Cfg = StridedCfg<T, N, WIDE>;
   static constexpr int kThreads = Cfg::TH;
   static __device__ __forceinline__ void run(const cpx<T>* __restrict__ H,
                                              const cpx<T>* __restrict__ Uy,
@@ -961,9 +961,10 @@ inline int nblocks(int ncols) {
 // each taking several pencil groups.  1024 blocks ran as two rounds of 512
 // and took 88 us at 256^3 c64 instead of 83.
 template <int N>
-inline int persist_blocks(int ncols) {
+inline int persist_blocks(int ncols, int cap = 0) {
   const int n = nblocks<N>(ncols);
-  return n < 512 ? n : 512;
+  const int c = cap > 0 ? cap : 512;
+  return n < c ? n : c;
 }
 template <class T, int N>
 inline int nblocks_s(int ncols) {
@@ -1034,7 +1035,7 @@ hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, v
     const cpx<T>* tw = (const cpx<T>*)tb.tw_z;
     const bool extra = vb || vx || conj_d;
     QJ_DISPATCH_N(d.nz, {
-      const int nb = persist_blocks<N_>(g.ncols);
+      const int nb = persist_blocks<N_>(g.ncols, tb.zblocks);
       if (kick && extra)
         launch<k_drift_c<T, N_, true>>(nb, 256, s, p, g, D, tw, v, vb, vx, kick_coef, conj_d,
                                        dslot, (const cpx<T>*)nullptr, (T*)nullptr,
@@ -1158,7 +1159,10 @@ hipError_t launch_poisson_y(const Dims& d, const SpecTables<T>& tb, int grad, vo
   const cpx<T>* tw = (const cpx<T>*)tb.tw_y;
   const double kfy = 2.0 * M_PI / d.Ly;
   QJ_DISPATCH_N(d.ny, {
-    if (grad)
+    if (grad && !tb.wide)
+      launch<k_poisson_y<T, N_, true, false>>(nblocks_s<T, N_>(g.ncols), 256, s,
+          (cpx<T>*)H, (cpx<T>*)Uz, (cpx<T>*)Uy, g, tw, kfy);
+    else if (grad)
       launch<k_poisson_y<T, N_, true>>(StridedCfg<T, N_, true>::blocks(g.ncols),
           StridedCfg<T, N_, true>::TH, s, (cpx<T>*)H, (cpx<T>*)Uz, (cpx<T>*)Uy, g, tw, kfy);
     else
@@ -1177,8 +1181,13 @@ hipError_t launch_poisson_x(const Dims& d, const SpecTables<T>& tb, int grad, co
   const double kfx = 2.0 * M_PI / d.Lx;
   QJ_DISPATCH_N(d.nx, {
     using Cfg = StridedCfg<T, N_, true>;
+    using Nar = StridedCfg<T, N_, false>;
     const int nb = Cfg::blocks(g.ncols);
-    if (grad)
+    if (grad && !tb.wide)
+      launch<k_poisson_x<T, N_, true, false>>(Nar::blocks(g.ncols), Nar::TH, s,
+          (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A3,
+          (const T*)addend, diag);
+    else if (grad)
       launch<k_poisson_x<T, N_, true>>(nb, Cfg::TH, s,
           (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A3,
           (const T*)addend, diag);
@@ -1202,7 +1211,7 @@ hipError_t launch_drift_adj_axis(const Dims& d, const SpecTables<T>& tb, int axi
     const cpx<T>* D = (const cpx<T>*)tb.drift_z;
     const cpx<T>* tw = (const cpx<T>*)tb.tw_z;
     QJ_DISPATCH_N(d.nz, {
-      launch<k_drift_c<T, N_, false, true>>(persist_blocks<N_>(g.ncols), 256, s, p, g, D, tw,
+      launch<k_drift_c<T, N_, false, true>>(persist_blocks<N_>(g.ncols, tb.zblocks), 256, s, p, g, D, tw,
                                             (const T*)V, (const T*)nullptr, (const T*)Vx, coef, 1,
                                             (const double*)nullptr, (const cpx<T>*)kick_ref,
                                             (T*)Vbar, (cpx<T>*)nullptr);

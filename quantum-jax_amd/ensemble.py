@@ -175,21 +175,30 @@ class Ensemble:
         return a.elapsed_time(b) / reps
 
     # run-time schedule options the autotuner searches: (result key, option,
-    # applies to the group of this shape key)
-    KNOBS = (("fuse_x_source", _lib.OPT_FUSE_X_SOURCE, lambda key: True),
-             ("binned_deposit", _lib.OPT_BINNED_DEPOSIT, lambda key: key[2] == "stars"))
+    # applies to the group of this shape key, candidate values).  The wide
+    # sweep form exists for complex64 256-point x and y pencils; the z sweeps'
+    # walking workgroups are a tile schedule of every group.
+    KNOBS = (("fuse_x_source", _lib.OPT_FUSE_X_SOURCE, lambda key: True, (0, 1)),
+             ("binned_deposit", _lib.OPT_BINNED_DEPOSIT, lambda key: key[2] == "stars", (0, 1)),
+             ("wide_sweeps", _lib.OPT_WIDE_SWEEPS,
+              lambda key: key[1] == "complex64" and 256 in key[0][:2], (0, 1)),
+             ("z_sweep_blocks", _lib.OPT_Z_SWEEP_BLOCKS, lambda key: True, (0, 256, 1024, 2048)))
 
     def autotune(self, reps: int = 3, modes=("batched", "graph", "graph_fork"),
                  cache_path: str | None = None, verbose: bool = False) -> dict:
         """Pick the execution mode and, per shape group, the schedule options.
 
         Coordinate search: the mode is chosen with every group on its current
-        options, then each group's options (the x-sweep schedule; for star
-        groups also the tile-binned vs float-atomic deposit) are flipped in
-        turn and kept when the ensemble step gets faster.  Results are cached
-        per ensemble signature in this process (and in cache_path, a JSON file,
-        when given).  Returns {"mode", "fuse_x_source": {group: 0|1},
-        "binned_deposit": {group: 0|1}, "ms_per_step", "trials"}.
+        options, then each group's options are set in turn to each of their
+        candidate values and kept when the ensemble step gets faster: the
+        x-sweep schedule, for star groups the tile-binned vs float-atomic
+        deposit, for complex64 256-point groups the wide or narrow Poisson
+        sweep form, and the number of walking workgroups of the z sweeps.
+        Results are cached per ensemble signature in this process (and in
+        cache_path, a JSON file, when given).  Returns {"mode",
+        "fuse_x_source": {group: value}, "binned_deposit": {...},
+        "wide_sweeps": {...}, "z_sweep_blocks": {...}, "ms_per_step",
+        "trials"}.
         """
         sig = self.signature()
         cache = dict(_TUNED)
@@ -213,25 +222,27 @@ class Ensemble:
                 if ms < best:
                     best_mode, best = m, ms
             self.set_mode(best_mode)
-            chosen = {name: {} for name, _, _ in self.KNOBS}
-            for name, opt, applies in self.KNOBS:
+            chosen = {name: {} for name, _, _, _ in self.KNOBS}
+            for name, opt, applies, values in self.KNOBS:
                 for key, members in groups.items():
                     if not applies(key):
                         continue
                     plans = [self.sims[i].plan for i in members]
                     cur = plans[0].get_option(opt)
-                    for p in plans:
-                        p.set_option(opt, 1 - cur)
-                    ms = self._time(reps)
-                    trials.append({"group": str(key), name: 1 - cur, "ms": ms})
-                    if verbose:
-                        print(f"[autotune] {key} {name}={1 - cur}: {ms:.3f} ms/step")
-                    if ms < best:
-                        best, chosen[name][str(key)] = ms, 1 - cur
-                    else:
+                    for val in values:
+                        if val == cur:
+                            continue
                         for p in plans:
-                            p.set_option(opt, cur)
-                        chosen[name][str(key)] = cur
+                            p.set_option(opt, val)
+                        ms = self._time(reps)
+                        trials.append({"group": str(key), name: val, "ms": ms})
+                        if verbose:
+                            print(f"[autotune] {key} {name}={val}: {ms:.3f} ms/step")
+                        if ms < best:
+                            best, cur = ms, val
+                    for p in plans:
+                        p.set_option(opt, cur)
+                    chosen[name][str(key)] = cur
         finally:
             self._restore(snap)
             torch.cuda.synchronize(self.device)
@@ -245,7 +256,7 @@ class Ensemble:
 
     def _apply(self, res: dict, groups: dict):
         self.set_mode(res["mode"])
-        for name, opt, _ in self.KNOBS:
+        for name, opt, _, _ in self.KNOBS:
             for key, members in groups.items():
                 v = res.get(name, {}).get(str(key))
                 if v is not None:

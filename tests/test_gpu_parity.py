@@ -476,3 +476,41 @@ def test_deposit_option_can_change_between_steps(qj, cuda, dtype):
     tol = {C64: 1e-6, C128: 1e-13}[dtype]
     for a, b in zip(*out):
         assert rel(a, b) < tol
+
+
+@pytest.mark.parametrize("kind", ["plain", "stars"])
+def test_sweep_shape_options_agree(qj, cuda, kind):
+    """The wide and narrow forms of the Poisson x/y gradient passes
+    (QJ_OPT_WIDE_SWEEPS) run the same per-pencil arithmetic in a different
+    lane layout, and QJ_OPT_Z_SWEEP_BLOCKS only changes which workgroup walks
+    which pencil group: states agree to rounding (wide/narrow) and bitwise
+    (z blocks, plain steps)."""
+    _lib = __import__("importlib").import_module("quantum-jax_amd._lib")
+    g = O.Grid(256, 256, 64, 1.0, 1.0, 0.25)
+    psi, pos, vel, m_s = _stars_case(g, 4000, 33)
+    out = {}
+    for opts in ((1, 0), (0, 0), (1, 256), (1, 2048)):
+        plan = plan_for(qj, g, C64)
+        plan.set_option(_lib.OPT_WIDE_SWEEPS, opts[0])
+        plan.set_option(_lib.OPT_Z_SWEEP_BLOCKS, opts[1])
+        if kind == "stars":
+            sim = qj.KDKSimulation(plan, qj.Physics(rho_bar=1.1, four_pi_G=20.0), 1e-3,
+                                   dev(psi, C64), dev(pos, torch.float64), dev(vel, torch.float64),
+                                   mass=m_s)
+        else:
+            sim = qj.KDKSimulation(plan, qj.Physics(four_pi_G=20.0), 1e-3, dev(psi, C64))
+        sim.step(3)
+        st = sim.state()
+        # plain steps write V only; star steps also the acceleration grid
+        out[opts] = [st[k].clone() for k in ("psi", "pos", "vel") if k in st] + [
+            sim.grid.clone() if kind == "stars" else sim.V.clone()]
+        del sim, plan
+    base = out[(1, 0)]
+    for opts, res in out.items():
+        for a, b in zip(res, base):
+            if kind == "plain" and opts[0] == 1:
+                assert torch.equal(a, b), opts
+            else:
+                assert rel(a, b) < 1e-6, opts
+    with pytest.raises(Exception):
+        plan_for(qj, g, C64).set_option(_lib.OPT_Z_SWEEP_BLOCKS, -1)
