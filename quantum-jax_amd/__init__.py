@@ -17,7 +17,7 @@ from .operators import (  # noqa: F401
 )
 from .simulation import BHSimulation, KDKSimulation, compute_step, compute_step_gas  # noqa: F401
 from .ensemble import Ensemble  # noqa: F401
-from . import accretion, checkpoint, cosmo, hdf5, ic  # noqa: F401
+from . import accretion, checkpoint, cosmo, hdf5, history, ic  # noqa: F401
 from ._lib import LIB_PATH, QjError, lib  # noqa: F401
 
 __version__ = "0.1.0"
