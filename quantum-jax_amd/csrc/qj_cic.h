@@ -30,9 +30,12 @@ inline CicGeom make_geom(const Dims& d) {
   return g;
 }
 
-// floored modulo of an integer-valued double (numpy/jnp.mod semantics)
+// floored modulo of an integer-valued double (numpy/jnp.mod semantics).
+// For a power-of-two n (every plan axis) it is the low bits of the two's
+// complement integer; the general form needs a 64-bit division.
 __device__ __forceinline__ int imod(double fi, int n) {
   long long ii = (long long)fi;
+  if ((n & (n - 1)) == 0) return (int)(ii & (long long)(n - 1));
   long long r = ii % n;
   return (int)(r < 0 ? r + n : r);
 }

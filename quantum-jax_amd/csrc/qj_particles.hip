@@ -172,18 +172,39 @@ __device__ __forceinline__ void corner_grad(const CicGeom& g, const Cic& c, int 
 //   sorted : registrations grouped by tile (particle indices)
 // ---------------------------------------------------------------------------
 
-// The distinct tiles touched by the stencil of c, in a fixed order.
-__device__ __forceinline__ int stencil_tiles(const TileGeom& t, const Cic& c, int ids[8]) {
+// The distinct tiles touched by the stencil of c, in eight fixed slots
+// k = 4a + 2b + e: slot k holds the tile of the stencil's a-th x, b-th y and
+// e-th z tile and is used (bit k of the returned mask) when the stencil
+// reaches a second tile along every axis whose index there is 1.  Slot 0 is
+// always used.  Fixed slots keep ids[] in registers and let a thread issue
+// all its registrations before it waits on any of them.
+__device__ __forceinline__ unsigned stencil_tiles(const TileGeom& t, const Cic& c, int ids[8]) {
   const int ax[2] = {c.ix[0] >> t.lx, c.ix[1] >> t.lx};
   const int ay[2] = {c.iy[0] >> t.ly, c.iy[1] >> t.ly};
   const int az[2] = {c.iz[0] >> t.lz, c.iz[1] >> t.lz};
-  const int nax = ax[1] != ax[0] ? 2 : 1, nay = ay[1] != ay[0] ? 2 : 1;
-  const int naz = az[1] != az[0] ? 2 : 1;
-  int n = 0;
-  for (int a = 0; a < nax; ++a)
-    for (int b = 0; b < nay; ++b)
-      for (int e = 0; e < naz; ++e) ids[n++] = (ax[a] * t.nty + ay[b]) * t.ntz + az[e];
-  return n;
+  const bool two[3] = {ax[1] != ax[0], ay[1] != ay[0], az[1] != az[0]};
+  unsigned mask = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int a = k >> 2, b = (k >> 1) & 1, e = k & 1;
+    ids[k] = (ax[a] * t.nty + ay[b]) * t.ntz + az[e];
+    if ((!a || two[0]) && (!b || two[1]) && (!e || two[2])) mask |= 1u << k;
+  }
+  return mask;
+}
+
+// Registers the star p in the tiles of its stencil: one counting atomic per
+// used slot, all issued before the returned ranks are stored.
+__device__ __forceinline__ void register_tiles(const TileGeom& tg, const Cic& c, long long p,
+                                               int* __restrict__ count, int* __restrict__ rank) {
+  int ids[8], r[8];
+  const unsigned m = stencil_tiles(tg, c, ids);
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    if (m >> k & 1u) r[k] = atomicAdd(&count[ids[k]], 1);
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    if (m >> k & 1u) rank[8 * p + k] = r[k];
 }
 
 // Adjoint of vel += h * interp(A3, pos) (and, first, of the drift
@@ -215,11 +236,7 @@ struct k_interp_adj {
     if (!c.ok) return;
     const double ab[3] = {h * vb[0], h * vb[1], h * vb[2]};
     if constexpr (BIN) {  // count == nullptr: registrations taped by the forward
-      if (count) {
-        int ids[8];
-        const int n = stencil_tiles(tg, c, ids);
-        for (int k = 0; k < n; ++k) rank[8 * p + k] = atomicAdd(&count[ids[k]], 1);
-      }
+      if (count) register_tiles(tg, c, p, count, rank);
     }
   #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -315,9 +332,7 @@ struct k_star_step_bin {
     }
     Cic c = cic(g, x[0], x[1], x[2]);
     if (!c.ok) return;  // non-finite positions deposit nothing
-    int ids[8];
-    const int n = stencil_tiles(tg, c, ids);
-    for (int k = 0; k < n; ++k) rank[8 * p + k] = atomicAdd(&count[ids[k]], 1);
+    register_tiles(tg, c, p, count, rank);
   }
 };
 
@@ -388,9 +403,14 @@ struct k_bin_scatter {
     if (p >= np) return;
     Cic c = cic(g, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]);
     if (!c.ok) return;
-    int ids[8];
-    const int n = stencil_tiles(tg, c, ids);
-    for (int k = 0; k < n; ++k) sorted[offset[ids[k]] + rank[8 * p + k]] = (int)p;
+    int ids[8], o[8];
+    const unsigned m = stencil_tiles(tg, c, ids);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (m >> k & 1u) o[k] = offset[ids[k]] + rank[8 * p + k];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (m >> k & 1u) sorted[o[k]] = (int)p;
   }
 };
 
