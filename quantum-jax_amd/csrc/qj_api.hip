@@ -150,6 +150,9 @@ struct qj_plan {
   double drift_key_dt = NAN, drift_key_mph = NAN;
   double* partials = nullptr;
   int nblk_src = 0;
+  // per-workgroup CFL signal maxima of the folded Poisson x pass
+  double* sig_part = nullptr;
+  int nblk_sig = 0;
   // schedule options (qj_plan_set_option): fused x, binned, deterministic,
   // wide Poisson sweeps, z-sweep blocks
   int opt[QJ_OPT_COUNT] = {1, 1, 0, 1, 0};
@@ -266,14 +269,16 @@ int ensure_drift(qj_plan* p, double dt, double mph, hipStream_t s) {
 template <class T>
 int poisson_tail(qj_plan* p, const Work& w, int grad, double four_pi_G, double rho_bar,
                  int dc_mode, void* V, void* A3, const void* addend,
-                 double* diag, hipStream_t s, double lp_kc2 = 0.0) {
+                 double* diag, hipStream_t s, double lp_kc2 = 0.0,
+                 GasSignalFold* sig = nullptr) {
   SpecTables<T> tb = tables<T>(p);
   QJ_STAGE(kStageFftY, launch_fft_y_half<T>(p->d, tb, w.H, s), "fft_y");
   QJ_STAGE(kStagePoissonZ, launch_poisson_z<T>(p->d, tb, grad, w.H, w.Uz, four_pi_G, rho_bar, dc_mode,
                                                   p->partials, diag, s, lp_kc2),
            "poisson_z");
   QJ_STAGE(kStagePoissonY, launch_poisson_y<T>(p->d, tb, grad, w.H, w.Uz, w.Uy, s), "poisson_y");
-  QJ_STAGE(kStagePoissonX, launch_poisson_x<T>(p->d, tb, grad, w.H, w.Uy, w.Uz, V, A3, addend, diag, s),
+  QJ_STAGE(kStagePoissonX,
+           launch_poisson_x<T>(p->d, tb, grad, w.H, w.Uy, w.Uz, V, A3, addend, diag, s, sig),
            "poisson_x");
   return QJ_OK;
 }
@@ -345,8 +350,11 @@ int qj_plan_create(qj_plan** out, int nx, int ny, int nz, double Lx, double Ly, 
     if (e == hipSuccess) e = hipMalloc(&p->drift[a], (size_t)n[a] * 2 * p->rsize);
   }
   p->nblk_src = source_x_blocks(p->d);
-  const size_t nscratch = (size_t)p->nblk_src + kDiagCount + kSlotCount + kCflChunk;
+  p->nblk_sig = poisson_x_blocks(p->d);
+  const size_t nfixed = (size_t)p->nblk_src + kDiagCount + kSlotCount + kCflChunk;
+  const size_t nscratch = nfixed + p->nblk_sig;
   if (e == hipSuccess) e = hipMalloc((void**)&p->partials, nscratch * sizeof(double));
+  if (e == hipSuccess) p->sig_part = p->partials + nfixed;
   if (e == hipSuccess) e = hipMemset(p->partials, 0, nscratch * sizeof(double));
   for (int a = 0; a < 3 && e == hipSuccess; ++a) {
     e = dtype == QJ_C64 ? launch_twiddles<float>(p->tw[a], n[a], nullptr)
@@ -666,10 +674,14 @@ template <class T>
 // its grid to grid_out, and psi after the drift goes to psi_out (the z sweep
 // reads st->psi and writes psi_mid, the x sweep reads psi_mid and writes
 // psi_out; fused x-source schedule only); st->grid / st->psi then point there.
+// sig (gas steps of the CFL stepper, with dslot): the Poisson x pass also
+// writes the CFL signal maxima of the state after the step (GasSignalFold).
 int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void* work,
                 hipStream_t s, const double* kick_dt = nullptr, const double* drift_dt = nullptr,
                 const double* dslot = nullptr, void* grid_out = nullptr, void* psi_mid = nullptr,
-                void* psi_out = nullptr) {
+                void* psi_out = nullptr, GasSignalFold* sig = nullptr) {
+  if (sig && (!dslot || !st->gas[0]))
+    return fail(QJ_EARG, "kdk_steps_t: the signal fold needs the step slot and a gas state");
   if ((grid_out || psi_out) && nsteps != 1)
     return fail(QJ_EARG, "kdk_steps_t: grid_out / psi_out need one step");
   if (psi_out && (!psi_mid || !p->opt[QJ_OPT_FUSE_X_SOURCE]))
@@ -767,8 +779,15 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
     }
     void* Vn = grid_out ? grid_out : V;
     void* A3n = grid_out ? (grad ? plane<T>(grid_out, p, 1) : nullptr) : A3;
+    if (sig) {  // the velocities after this step's update, the kick it leaves owed
+      for (int k = 0; k < 3; ++k) sig->v[k] = live_gas(k + 1);
+      sig->owed = dslot + kSlotOwed;
+      sig->cs = pr->cs;
+      sig->part = p->sig_part;
+      sig->cap = p->nblk_sig;
+    }
     if (int rc = poisson_tail<T>(p, w, grad, pr->four_pi_G, pr->rho_bar, pr->dc_mode, Vn, A3n,
-                                 nullptr, st->diag, s))
+                                 nullptr, st->diag, s, 0.0, sig))
       return rc;
     if (grid_out) st->grid = grid_out;
     // the closing half kicks (psi, gas and stars) are owed to the next step
@@ -791,16 +810,29 @@ int kdk_steps_cfl_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, d
   double* dts_dev = slot + kSlotCount;
   const double dx = p->d.Lx / p->d.nx;
   QJ_CHECK(fill<double>(slot + kSlotOwed, 1, st->pending, s), "step slot");
+  // The first step's signal comes from a pass over v and the grid; every
+  // later step's was folded into the previous step's Poisson x pass.
+  GasSignalFold sig{};
+  bool folded = false;
   for (int k0 = 0; k0 < nsteps; k0 += kCflChunk) {
     const int m = nsteps - k0 < kCflChunk ? nsteps - k0 : kCflChunk;
     for (int k = 0; k < m; ++k) {
-      void* const* g = st->gas_swapped ? st->gas_tmp : st->gas;
-      // the owed kick is zero on a state whose grid is not computed yet
-      const void* A3 = st->grid_valid ? plane<T>(st->grid, p, 1) : nullptr;
-      QJ_CHECK(launch_gas_cfl<T>(p->ncell(), g[1], g[2], g[3], A3, 0.0, pr->cs, dx, cfl, dt_max,
-                                 slot, dts_dev, k, p->partials, p->nblk_src, s),
-               "cfl");
-      if (int rc = kdk_steps_t<T>(p, st, pr, 1, work, s, nullptr, nullptr, slot)) return rc;
+      if (folded) {
+        // the previous step's Poisson x pass left the signal maxima
+        QJ_CHECK(launch_cfl_finish(p->sig_part, sig.nblk, cfl, dx, dt_max, slot, dts_dev, k, s),
+                 "cfl");
+      } else {
+        void* const* g = st->gas_swapped ? st->gas_tmp : st->gas;
+        // the owed kick is zero on a state whose grid is not computed yet
+        const void* A3 = st->grid_valid ? plane<T>(st->grid, p, 1) : nullptr;
+        QJ_CHECK(launch_gas_cfl<T>(p->ncell(), g[1], g[2], g[3], A3, 0.0, pr->cs, dx, cfl,
+                                   dt_max, slot, dts_dev, k, p->partials, p->nblk_src, s),
+                 "cfl");
+      }
+      if (int rc = kdk_steps_t<T>(p, st, pr, 1, work, s, nullptr, nullptr, slot, nullptr,
+                                  nullptr, nullptr, &sig))
+        return rc;
+      folded = true;
     }
     double h[kCflChunk + 1];
     hipError_t e = hipMemcpyAsync(h, dts_dev, (size_t)m * sizeof(double), hipMemcpyDeviceToHost, s);

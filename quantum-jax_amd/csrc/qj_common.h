@@ -169,6 +169,24 @@ __device__ __forceinline__ void atomic_min_nonneg(double* addr, double v) {
             (unsigned long long)__double_as_longlong(v));
 }
 
+// max(a, b) that keeps a NaN of b (a NaN already in a stays as well)
+__device__ __forceinline__ double nan_max(double a, double b) { return (b > a || b != b) ? b : a; }
+
+// Signal speed cs + |v + h a| of one gas cell (quantum-fluid-accretion.py:
+// 696-699, isothermal c_eff = cs): the kicked velocity in T with explicit
+// FMAs, the norm in FP64.  Every kernel that evaluates it calls this, so
+// they all give the same bits.
+template <class T>
+__device__ __forceinline__ double gas_signal(T u, T v, T w, T ax, T ay, T az, T h, double cs) {
+  const double du = fmad(ax, h, u), dv = fmad(ay, h, v), dw = fmad(az, h, w);
+  return cs + sqrt(__builtin_fma(dw, dw, __builtin_fma(dv, dv, du * du)));
+}
+template <class T>
+__device__ __forceinline__ double gas_signal(T u, T v, T w, double cs) {
+  const double du = u, dv = v, dw = w;
+  return cs + sqrt(__builtin_fma(dw, dw, __builtin_fma(dv, dv, du * du)));
+}
+
 // wavefront (64-lane) reductions
 template <class T>
 __device__ __forceinline__ T wave_sum(T v) {

@@ -461,14 +461,10 @@ struct k_gas_signal {
     long long stride = (long long)gridDim.x * blockDim.x;
     double m = 0.0;
     for (; i < n; i += stride) {
-      T u = vx[i], v = vy[i], w = vz[i];
-      if (A3) {
-        u += A3[kAccComp * i] * h;
-        v += A3[kAccComp * i + 1] * h;
-        w += A3[kAccComp * i + 2] * h;
-      }
-      const double c = cs + sqrt((double)u * u + (double)v * v + (double)w * w);
-      m = (c > m || c != c) ? c : m;
+      const double c = A3 ? gas_signal<T>(vx[i], vy[i], vz[i], A3[kAccComp * i],
+                                          A3[kAccComp * i + 1], A3[kAccComp * i + 2], h, cs)
+                          : gas_signal<T>(vx[i], vy[i], vz[i], cs);
+      m = nan_max(m, c);
     }
   #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -530,6 +526,12 @@ hipError_t launch_gas_cfl(long long n, const void* vx, const void* vy, const voi
                           (const T*)A3, (T)h, hdev, cs, part);
   launch<k_cfl_finish>(1, 64, s, (const double*)part, (int)blocks, out, cfl, dx, dt_max,
                        dts ? 1 : 0, dts, k);
+  return hipGetLastError();
+}
+
+hipError_t launch_cfl_finish(const double* part, int nblk, double cfl, double dx, double dt_max,
+                             double* slot, double* dts, int k, hipStream_t s) {
+  launch<k_cfl_finish>(1, 64, s, part, nblk, slot, cfl, dx, dt_max, 1, dts, k);
   return hipGetLastError();
 }
 

@@ -72,12 +72,32 @@ template <class T>
 hipError_t launch_poisson_y(const Dims& d, const SpecTables<T>& tb, int grad, void* H, void* Uz,
                             void* Uy, hipStream_t s);
 
+// The CFL signal folded into the Poisson x pass of a gas step: with the
+// acceleration of each cell in registers the pass also loads the gas
+// velocity there and writes, per workgroup, max over its cells of
+// cs + |v + h a| (h = *owed, the kick owed after the step) to part[block];
+// nblk receives the number of workgroups; a launch that would need more
+// than cap entries of part is refused.
+struct GasSignalFold {
+  const void* v[3];
+  const double* owed;
+  double cs;
+  double* part;
+  int cap;
+  int nblk;
+};
+
 // Writes V and, with grad, the cell-interleaved grid A3 = (a_x, a_y, a_z)
-// and folds max |a| into diag[kDiagAMax].
+// and folds max |a| into diag[kDiagAMax]; sig: the CFL signal fold above.
 template <class T>
 hipError_t launch_poisson_x(const Dims& d, const SpecTables<T>& tb, int grad, const void* H,
                             const void* Uy, const void* Uzz, void* V, void* A3,
-                            const void* addend, double* diag, hipStream_t s);
+                            const void* addend, double* diag, hipStream_t s,
+                            GasSignalFold* sig = nullptr);
+
+// Largest number of workgroups launch_poisson_x uses on grid d (either
+// precision, either sweep form): the size of a GasSignalFold's part array.
+int poisson_x_blocks(const Dims& d);
 
 // Number of workgroups source_x launches (size of the FP64 partials array).
 int source_x_blocks(const Dims& d);
@@ -174,6 +194,10 @@ hipError_t launch_gas_cfl(long long n, const void* vx, const void* vy, const voi
                           const void* A3, double h, double cs, double dx, double cfl,
                           double dt_max, double* out, double* dts, int k, double* part,
                           int max_blocks, hipStream_t s);
+// The reduction of launch_gas_cfl over signal maxima already in part[nblk]
+// (a GasSignalFold), advancing the step slot (dts[k] = dt).
+hipError_t launch_cfl_finish(const double* part, int nblk, double cfl, double dx, double dt_max,
+                             double* slot, double* dts, int k, hipStream_t s);
 
 // Tile-binned star deposit (owner computes; no global float atomics).
 constexpr int kTileX = 8, kTileY = 8, kTileZ = 64;

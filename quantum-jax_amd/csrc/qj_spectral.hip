@@ -667,8 +667,22 @@ __device__ __forceinline__ void load_herm_rows(cpx<T> (&v)[VW][E], const cpx<T>*
 // cell-interleaved grid A3 = (a_x, a_y, a_z), and max |a| folded into
 // the running maximum diag[kDiagAMax] with one atomic per block.
 // quantum-stars.py:192 and :256-260.
+// SIG (gas steps of the CFL stepper): the pass also loads the gas velocity
+// of its cells and writes the block maximum of the CFL signal speed
+// cs + |v + h a| (quantum-fluid-accretion.py:696-699) to sg.part[block], so
+// the next step's time step needs no pass over v and the grid of its own.
 // ---------------------------------------------------------------------------
-template <class T, int N, bool GRAD, bool WIDE = true>
+template <class T>
+struct SigArgs {
+  const T* vx;
+  const T* vy;
+  const T* vz;
+  const double* owed;  // h, the kick owed after this step
+  double cs;
+  double* part;
+};
+
+template <class T, int N, bool GRAD, bool WIDE = true, bool SIG = false>
 struct k_poisson_x {
   using Cfg = StridedCfg<T, N, WIDE>;
   static constexpr int kThreads = Cfg::TH;
@@ -678,7 +692,7 @@ struct k_poisson_x {
                                              const cpx<T>* __restrict__ tw, double kfx,
                                              T* __restrict__ V, T* __restrict__ A3,
                                              const T* __restrict__ addend,
-                                             double* __restrict__ diag) {
+                                             double* __restrict__ diag, SigArgs<T> sg) {
     using S = PencilShape<N, Cfg::TH>;
     constexpr int VW = Cfg::VW;
     constexpr int E = S::E;
@@ -727,7 +741,23 @@ struct k_poisson_x {
       }
       pencil_fft_strided<N, +1, T, VW, false, Cfg::TH>(u, j, col, lds, tw);
       pencil_fft_strided<N, +1, T, VW, false, Cfg::TH>(v, j, col, lds, tw);
+      // gas velocity of the pencil's cells (loaded after the transforms:
+      // live through them it would cost the pass its second workgroup per CU)
+      T gv[SIG ? 3 : 1][SIG ? E : 1][VW];
+      if constexpr (SIG) {
+        if (act) {
+  #pragma unroll
+          for (int m = 0; m < E; ++m) {
+            const int idx = base + (j + TT * m) * se;
+            ldr<VW>(sg.vx, idx, gv[0][m]);
+            ldr<VW>(sg.vy, idx, gv[1][m]);
+            ldr<VW>(sg.vz, idx, gv[2][m]);
+          }
+        }
+      }
       T amax = (T)0;
+      double smax = 0.0;
+      const T h = SIG ? (T)*sg.owed : (T)0;
       if (act) {
   #pragma unroll
         for (int m = 0; m < E; ++m) {
@@ -747,6 +777,9 @@ struct k_poisson_x {
             amax = (m0 > amax || m0 != m0) ? m0 : amax;
             amax = (m1 > amax || m1 != m1) ? m1 : amax;
             amax = (m2 > amax || m2 != m2) ? m2 : amax;
+            if constexpr (SIG)
+              smax = nan_max(smax, gas_signal<T>(gv[0][m][w], gv[1][m][w], gv[2][m][w], a0, a1,
+                                                 a2, h, sg.cs));
           }
         }
       }
@@ -766,6 +799,18 @@ struct k_poisson_x {
   #pragma unroll
         for (int w = 1; w < kThreads / 64; ++w) am = (wmax[w] > am || wmax[w] != wmax[w]) ? wmax[w] : am;
         atomic_max_nonneg(&diag[kDiagAMax], am);
+      }
+      if constexpr (SIG) {
+  #pragma unroll
+        for (int o = 32; o > 0; o >>= 1) smax = nan_max(smax, __shfl_xor(smax, o, 64));
+        __shared__ double wsig[kThreads / 64];
+        if ((threadIdx.x & 63) == 0) wsig[threadIdx.x >> 6] = smax;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+  #pragma unroll
+          for (int w = 1; w < kThreads / 64; ++w) smax = nan_max(smax, wsig[w]);
+          sg.part[blockIdx.x] = smax;
+        }
       }
     }
   }
@@ -1085,7 +1130,30 @@ int source_x_blocks_t(const Dims& d) {
   }
 }
 
+template <class T>
+int poisson_x_blocks_t(const Dims& d) {
+  const int ncols = d.ny * d.nz;
+  int nb = 1;
+  auto take = [&](int a, int b) { nb = a > b ? a : b; };
+  switch (d.nx) {
+    case 8: take(StridedCfg<T, 8, true>::blocks(ncols), StridedCfg<T, 8, false>::blocks(ncols)); break;
+    case 16: take(StridedCfg<T, 16, true>::blocks(ncols), StridedCfg<T, 16, false>::blocks(ncols)); break;
+    case 32: take(StridedCfg<T, 32, true>::blocks(ncols), StridedCfg<T, 32, false>::blocks(ncols)); break;
+    case 64: take(StridedCfg<T, 64, true>::blocks(ncols), StridedCfg<T, 64, false>::blocks(ncols)); break;
+    case 128: take(StridedCfg<T, 128, true>::blocks(ncols), StridedCfg<T, 128, false>::blocks(ncols)); break;
+    case 256: take(StridedCfg<T, 256, true>::blocks(ncols), StridedCfg<T, 256, false>::blocks(ncols)); break;
+    case 512: take(StridedCfg<T, 512, true>::blocks(ncols), StridedCfg<T, 512, false>::blocks(ncols)); break;
+    default: break;
+  }
+  return nb;
+}
+
 #if !defined(QJ_PART) || QJ_PART == 0
+int poisson_x_blocks(const Dims& d) {
+  const int a = poisson_x_blocks_t<float>(d), b = poisson_x_blocks_t<double>(d);
+  return a > b ? a : b;
+}
+
 int source_x_blocks(const Dims& d) {
   // partials are sized for the larger of the two precisions' grids
   const int a = source_x_blocks_t<float>(d), b = source_x_blocks_t<double>(d);
@@ -1175,26 +1243,45 @@ hipError_t launch_poisson_y(const Dims& d, const SpecTables<T>& tb, int grad, vo
 template <class T>
 hipError_t launch_poisson_x(const Dims& d, const SpecTables<T>& tb, int grad, const void* H,
                             const void* Uy, const void* Uzz, void* V, void* A3,
-                            const void* addend, double* diag, hipStream_t s) {
+                            const void* addend, double* diag, hipStream_t s,
+                            GasSignalFold* sig) {
   PassGeom g = geom_x(d);
   const cpx<T>* tw = (const cpx<T>*)tb.tw_x;
   const double kfx = 2.0 * M_PI / d.Lx;
+  if (sig && (!grad || !sig->v[0] || !sig->v[1] || !sig->v[2] || !sig->owed || !sig->part))
+    return hipErrorInvalidValue;
+  SigArgs<T> sa{};
+  if (sig) sa = SigArgs<T>{(const T*)sig->v[0], (const T*)sig->v[1], (const T*)sig->v[2],
+                           sig->owed, sig->cs, sig->part};
   QJ_DISPATCH_N(d.nx, {
     using Cfg = StridedCfg<T, N_, true>;
     using Nar = StridedCfg<T, N_, false>;
     const int nb = Cfg::blocks(g.ncols);
-    if (grad && !tb.wide)
+    const bool narrow = grad && !tb.wide;
+    if (sig) {
+      sig->nblk = narrow ? Nar::blocks(g.ncols) : nb;
+      if (sig->nblk > sig->cap) return hipErrorInvalidValue;
+    }
+    if (narrow && sig)
+      launch<k_poisson_x<T, N_, true, false, true>>(Nar::blocks(g.ncols), Nar::TH, s,
+          (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A3,
+          (const T*)addend, diag, sa);
+    else if (narrow)
       launch<k_poisson_x<T, N_, true, false>>(Nar::blocks(g.ncols), Nar::TH, s,
           (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A3,
-          (const T*)addend, diag);
+          (const T*)addend, diag, sa);
+    else if (grad && sig)
+      launch<k_poisson_x<T, N_, true, true, true>>(nb, Cfg::TH, s,
+          (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A3,
+          (const T*)addend, diag, sa);
     else if (grad)
       launch<k_poisson_x<T, N_, true>>(nb, Cfg::TH, s,
           (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A3,
-          (const T*)addend, diag);
+          (const T*)addend, diag, sa);
     else
       launch<k_poisson_x<T, N_, false>>(nb, Cfg::TH, s,
           (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A3,
-          (const T*)addend, diag);
+          (const T*)addend, diag, sa);
   });
   return hipGetLastError();
 }
@@ -1314,7 +1401,7 @@ hipError_t launch_adj_poisson(const Dims& d, const SpecTables<T>& tb, const void
                                           void*, hipStream_t);                                  \
   template hipError_t launch_poisson_x<T>(const Dims&, const SpecTables<T>&, int, const void*,  \
                                           const void*, const void*, void*, void*, const void*,  \
-                                          double*, hipStream_t);
+                                          double*, hipStream_t, GasSignalFold*);
 
 #define QJ_INST_ADJ(T)                                                                          \
   template hipError_t launch_kick_adj<T>(void*, const void*, const void*, const void*, double,   \

@@ -120,6 +120,36 @@ def test_cfl_limited_gas_steps_match_oracle(qj, cuda, dtype, flux):
     assert abs(sim.cfl_dt(cfl, dt_max) / want - 1) < (1e-6 if dtype == C64 else 1e-13)
 
 
+@pytest.mark.parametrize("dtype", [C64, C128])
+@pytest.mark.parametrize("n,wide", [(32, 1), (256, 1), (256, 0)])
+def test_cfl_signal_fold_equals_signal_pass(qj, cuda, dtype, n, wide):
+    """Within one CFL call every step after the first takes its signal speed
+    from the previous step's Poisson x pass; a call of one step computes it
+    with the separate pass over v and the grid.  Both routes give the same
+    bits: one call of 5 steps equals 5 calls of one step (dts and states)."""
+    g = O.Grid(n, n, n)
+    psi = O.filtered_gaussian_psi(g, 31, sigma_k=8 * np.pi)
+    rho, _, _, _ = O.gas_ic(g, 31, sigma_k=8 * np.pi)
+    rng = np.random.default_rng(32)
+    vel = [0.3 * rng.standard_normal(g.shape) for _ in range(3)]
+    rb = float(np.mean(np.abs(psi) ** 2 + rho))
+    r = torch.float32 if dtype == C64 else torch.float64
+    runs = []
+    for calls in (1, 5):
+        plan = qj.Plan(n, dtype=dtype)
+        plan.set_option(qj._lib.OPT_WIDE_SWEEPS, wide)
+        sim = qj.KDKSimulation(plan, qj.Physics(rho_bar=rb, cs=1.0, flux="hll"), 1e-2,
+                               dev(psi, dtype), gas=[dev(a, r) for a in (rho, *vel)])
+        for _ in range(calls):
+            sim.step_cfl(5 // calls, cfl=0.4)
+        st = sim.state()
+        runs.append((list(sim.dts), [st[k].clone() for k in ("psi", "rho", "vx", "vy", "vz")]))
+    assert len(set(runs[0][0])) == 5   # the cap binds and changes each step
+    assert runs[0][0] == runs[1][0]
+    for a, b in zip(runs[0][1], runs[1][1]):
+        assert torch.equal(a, b)
+
+
 def test_cfl_rejects_states_without_gas(qj, cuda):
     sim = qj.KDKSimulation(qj.Plan(16), qj.Physics(), 1e-3,
                            dev(O.filtered_gaussian_psi(O.Grid(16, 16, 16), 29), C64))
