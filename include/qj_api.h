@@ -337,7 +337,13 @@ int qj_deposit_adjoint(qj_plan* plan, const double* pos, const double* mass, dou
  * pencil groups (0 = default, at most 512; otherwise at most this many).
  * Same results. */
 #define QJ_OPT_Z_SWEEP_BLOCKS 4
-#define QJ_OPT_COUNT 5
+/* QJ_OPT_SIDE_STREAM (default 1): the stepper runs each step's star kick,
+ * drift and deposit and its gas update on a second stream of the plan,
+ * beside the psi drift sweeps that do not depend on them, and joins the
+ * caller's stream before the Poisson source pass; 0 = everything on the
+ * caller's stream.  Same results. */
+#define QJ_OPT_SIDE_STREAM 5
+#define QJ_OPT_COUNT 6
 int qj_plan_set_option(qj_plan* plan, int option, int value);
 int qj_plan_get_option(const qj_plan* plan, int option, int* value);
 

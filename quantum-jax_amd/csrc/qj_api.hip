@@ -154,8 +154,12 @@ struct qj_plan {
   double* sig_part = nullptr;
   int nblk_sig = 0;
   // schedule options (qj_plan_set_option): fused x, binned, deterministic,
-  // wide Poisson sweeps, z-sweep blocks
-  int opt[QJ_OPT_COUNT] = {1, 1, 0, 1, 0};
+  // wide Poisson sweeps, z-sweep blocks, side stream
+  int opt[QJ_OPT_COUNT] = {1, 1, 0, 1, 0, 1};
+  // second stream (default priority) for the star and gas work of a step,
+  // with the fork and join events
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // optional per-stage event timing (tracing hook of the bench harness)
   int timing = 0;
   struct Rec {
@@ -355,6 +359,9 @@ int qj_plan_create(qj_plan** out, int nx, int ny, int nz, double Lx, double Ly, 
   const size_t nscratch = nfixed + p->nblk_sig;
   if (e == hipSuccess) e = hipMalloc((void**)&p->partials, nscratch * sizeof(double));
   if (e == hipSuccess) p->sig_part = p->partials + nfixed;
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming);
   if (e == hipSuccess) e = hipMemset(p->partials, 0, nscratch * sizeof(double));
   for (int a = 0; a < 3 && e == hipSuccess; ++a) {
     e = dtype == QJ_C64 ? launch_twiddles<float>(p->tw[a], n[a], nullptr)
@@ -387,6 +394,9 @@ int qj_plan_destroy(qj_plan* p) {
     if (p->drift[a]) (void)hipFree(p->drift[a]);
   }
   if (p->partials) (void)hipFree(p->partials);
+  if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+  if (p->ev_join) (void)hipEventDestroy(p->ev_join);
+  if (p->side) (void)hipStreamDestroy(p->side);
   delete p;
   return QJ_OK;
 }
@@ -396,6 +406,8 @@ int qj_plan_set_option(qj_plan* p, int option, int value) {
   if (option < 0 || option >= QJ_OPT_COUNT) return fail(QJ_EARG, "unknown option");
   if (option == QJ_OPT_Z_SWEEP_BLOCKS && (value < 0 || value > 65535))
     return fail(QJ_EARG, "QJ_OPT_Z_SWEEP_BLOCKS must be in [0, 65535]");
+  if (option != QJ_OPT_Z_SWEEP_BLOCKS && value != 0 && value != 1)
+    return fail(QJ_EARG, "this option takes 0 or 1");
   p->opt[option] = value;
   return QJ_OK;
 }
@@ -722,47 +734,70 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
       if (int rc = ensure_drift<T>(p, drift_dt[step], pr->m_per_hbar, s)) return rc;
     }
     const double kick_t = st->pending + 0.5 * dk;  // owed half kick + this step's first
-    // stars: owed closing kick of the previous step and this step's opening
-    // half kick (same grid, same positions), drift, deposit at the new positions
-    if (stars && binned) {
-      StarBins sb = carve_star_scratch(p->d, st->np, st->star_scratch);
-      QJ_STAGE(kStageStarStep,
-               launch_star_step_binned<T>(p->d, st->pos, st->vel, st->np, A3, st->pending_star,
-                                          0.5 * dt, dt, 1, sb, s),
-               "star step");
-      QJ_STAGE(kStageDeposit,
-               launch_tile_deposit<T>(p->d, st->pos, st->mass, st->m_scalar, st->np, sb,
-                                      st->rho_s, s, p->opt[QJ_OPT_DETERMINISTIC_DEPOSIT]),
-               "deposit");
-    } else if (stars) {
-      // the float-atomic deposit accumulates into a cleared plane
-      QJ_CHECK(fill<T>(st->rho_s, p->ncell(), (T)0, s), "rho_s clear");
-      QJ_STAGE(kStageStarStep,
-               launch_star_step<T>(p->d, st->pos, st->vel, st->np, A3, st->pending_star, 0.5 * dt,
-                                   dt, st->mass, st->m_scalar, st->rho_s, s),
-               "star step");
-    }
-    // gas: owed + first half kick of the velocity, then the MUSCL update
-    // over this step's drift time
-    if (gas) {
-      const double dd = drift_dt ? drift_dt[step] : dt;
-      QJ_STAGE(kStageHydro,
-               launch_hydro<T>(p->d, live_gas(0), live_gas(1), live_gas(2), live_gas(3), A3, kick_t,
-                               dd, pr->cs, pr->flux, spare_gas(0), spare_gas(1), spare_gas(2),
-                               spare_gas(3), s, dslot),
-               "hydro");
-      st->gas_swapped ^= 1;
-    }
+    // The star and gas work of the step reads only the previous grid and its
+    // own arrays, and the psi drift sweeps neither read nor write those, so
+    // with the side-stream option it runs on the plan's second stream beside
+    // them; the caller's stream waits for it before the Poisson source pass
+    // (which reads rho_s and the gas density).  Recording (the ensemble)
+    // keeps one stream.
+    const bool fork = (stars || gas) && p->opt[QJ_OPT_SIDE_STREAM] && p->side && !g_recorder;
+    auto side_work = [&](hipStream_t s) -> int {
+      // stars: owed closing kick of the previous step and this step's opening
+      // half kick (same grid, same positions), drift, deposit at the new positions
+      if (stars && binned) {
+        StarBins sb = carve_star_scratch(p->d, st->np, st->star_scratch);
+        QJ_STAGE(kStageStarStep,
+                 launch_star_step_binned<T>(p->d, st->pos, st->vel, st->np, A3, st->pending_star,
+                                            0.5 * dt, dt, 1, sb, s),
+                 "star step");
+        QJ_STAGE(kStageDeposit,
+                 launch_tile_deposit<T>(p->d, st->pos, st->mass, st->m_scalar, st->np, sb,
+                                        st->rho_s, s, p->opt[QJ_OPT_DETERMINISTIC_DEPOSIT]),
+                 "deposit");
+      } else if (stars) {
+        // the float-atomic deposit accumulates into a cleared plane
+        QJ_CHECK(fill<T>(st->rho_s, p->ncell(), (T)0, s), "rho_s clear");
+        QJ_STAGE(kStageStarStep,
+                 launch_star_step<T>(p->d, st->pos, st->vel, st->np, A3, st->pending_star,
+                                     0.5 * dt, dt, st->mass, st->m_scalar, st->rho_s, s),
+                 "star step");
+      }
+      // gas: owed + first half kick of the velocity, then the MUSCL update
+      // over this step's drift time
+      if (gas) {
+        const double dd = drift_dt ? drift_dt[step] : dt;
+        QJ_STAGE(kStageHydro,
+                 launch_hydro<T>(p->d, live_gas(0), live_gas(1), live_gas(2), live_gas(3), A3,
+                                 kick_t, dd, pr->cs, pr->flux, spare_gas(0), spare_gas(1),
+                                 spare_gas(2), spare_gas(3), s, dslot),
+                 "hydro");
+        st->gas_swapped ^= 1;
+      }
+      return QJ_OK;
+    };
     // psi: kick folded into the z drift sweep, then y and x (+ Poisson source)
-    void* mid = (psi_out && psi_mid != st->psi) ? psi_mid : nullptr;  // z sweep out of place
-    QJ_STAGE(kStageDriftZKick,
-             launch_drift_axis<T>(p->d, tb, 2, st->psi, V, brho, st->v_ext,
-                                  pr->m_per_hbar * (dslot ? 1.0 : kick_t), 1, s, 0, dslot, mid),
-             "drift z");
-    if (mid) st->psi = mid;
-    QJ_STAGE(kStageDriftY, launch_drift_axis<T>(p->d, tb, 1, st->psi, nullptr, nullptr, nullptr, 0.0, 0, s),
-             "drift y");
+    auto drift_zy = [&](hipStream_t s) -> int {
+      void* mid = (psi_out && psi_mid != st->psi) ? psi_mid : nullptr;  // z sweep out of place
+      QJ_STAGE(kStageDriftZKick,
+               launch_drift_axis<T>(p->d, tb, 2, st->psi, V, brho, st->v_ext,
+                                    pr->m_per_hbar * (dslot ? 1.0 : kick_t), 1, s, 0, dslot, mid),
+               "drift z");
+      if (mid) st->psi = mid;
+      QJ_STAGE(kStageDriftY,
+               launch_drift_axis<T>(p->d, tb, 1, st->psi, nullptr, nullptr, nullptr, 0.0, 0, s),
+               "drift y");
+      return QJ_OK;
+    };
+    if (fork) {
+      QJ_CHECK(hipEventRecord(p->ev_fork, s), "fork");
+      QJ_CHECK(hipStreamWaitEvent(p->side, p->ev_fork, 0), "fork");
+    }
+    // (issuing the drift sweeps first measured the same)
+    if (int rc = side_work(fork ? p->side : s)) return rc;
+    if (int rc = drift_zy(s)) return rc;
+    if (fork) QJ_CHECK(hipEventRecord(p->ev_join, p->side), "join");
     src = stars ? st->rho_s : (gas ? live_gas(0) : nullptr);
+    if (fork) QJ_CHECK(hipStreamWaitEvent(s, p->ev_join, 0), "join");
     if (p->opt[QJ_OPT_FUSE_X_SOURCE]) {
       QJ_STAGE(kStageDriftXSource,
                launch_source_x<T>(p->d, tb, 1, st->psi, src, clear, brho, pr->b_coeff,
