@@ -33,7 +33,7 @@ inline CicGeom make_geom(const Dims& d) {
 // floored modulo of an integer-valued double (numpy/jnp.mod semantics).
 // For a power-of-two n (every plan axis) it is the low bits of the two's
 // complement integer; the general form needs a 64-bit division.
-__device__ __forceinline__ int imod(double fi, int n) {
+__host__ __device__ __forceinline__ int imod(double fi, int n) {
   long long ii = (long long)fi;
   if ((n & (n - 1)) == 0) return (int)(ii & (long long)(n - 1));
   long long r = ii % n;
@@ -119,9 +119,16 @@ __device__ __forceinline__ void interp_one(const CicGeom& g, const Cic& c, const
   }
 }
 
-// floored modulo with numpy semantics (quantum-stars.py:312 jnp.mod(pos, L))
-__device__ __forceinline__ double fmod_floor(double x, double L) {
-  double r = fmod(x, L);
+// floored modulo with numpy semantics (quantum-stars.py:312 jnp.mod(pos, L)).
+// The remainder fmod(x, L) is exact; for 0 < L and -L < x < 2L it is x, or
+// x - L when L <= x (exact by Sterbenz's lemma, L <= x <= 2L), so the
+// library's iterative fmod is only called outside that range.
+__host__ __device__ __forceinline__ double fmod_floor(double x, double L) {
+  double r;
+  if (L > 0.0 && x > -L && x < 2.0 * L)
+    r = (x >= L) ? x - L : x;
+  else
+    r = fmod(x, L);
   if (r != 0.0) {
     if ((r < 0.0) != (L < 0.0)) r += L;
   } else {
