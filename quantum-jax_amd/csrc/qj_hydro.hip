@@ -83,21 +83,28 @@ __device__ __forceinline__ void flux_hll(T rL, T uL, T vL, T wL, T rR, T uR, T v
                                          T F[4]) {
   const T c2 = cs * cs;
   const T mL = rL * uL, mR = rR * uR;
-  const T FL[4] = {mL, mL * uL + rL * c2, mL * vL, mL * wL};
-  const T FR[4] = {mR, mR * uR + rR * c2, mR * vR, mR * wR};
   const T SL = fmin(uL, uR) - cs, SR = fmax(uL, uR) + cs;
   if (SL >= (T)0) {
-  #pragma unroll
-    for (int k = 0; k < 4; ++k) F[k] = FL[k];
+    F[0] = mL;
+    F[1] = mL * uL + rL * c2;
+    F[2] = mL * vL;
+    F[3] = mL * wL;
   } else if (SR <= (T)0) {
-  #pragma unroll
-    for (int k = 0; k < 4; ++k) F[k] = FR[k];
+    F[0] = mR;
+    F[1] = mR * uR + rR * c2;
+    F[2] = mR * vR;
+    F[3] = mR * wR;
   } else {
-    const T UL[4] = {rL, mL, rL * vL, rL * wL};
-    const T UR[4] = {rR, mR, rR * vR, rR * wR};
-    const T inv = rcp(SR - SL), SLR = SL * SR;
-  #pragma unroll
-    for (int k = 0; k < 4; ++k) F[k] = (SR * FL[k] - SL * FR[k] + SLR * (UR[k] - UL[k])) * inv;
+    // (SR F_L - SL F_R + SL SR (U_R - U_L)) / (SR - SL), grouped by state:
+    // every component of F(U) - S U is (u - S) times a component of U (plus
+    // rho cs^2 in the momentum flux), so F = aL U_L - aR U_R + c2 (SR rL - SL rR) / (SR - SL)
+    // e_x with aL = SR (uL - SL) / (SR - SL), aR = SL (uR - SR) / (SR - SL).
+    const T inv = rcp(SR - SL);
+    const T aL = SR * (uL - SL) * inv, aR = SL * (uR - SR) * inv;
+    F[0] = rL * aL - rR * aR;
+    F[1] = mL * aL - mR * aR + c2 * (SR * rL - SL * rR) * inv;
+    F[2] = (rL * vL) * aL - (rR * vR) * aR;
+    F[3] = (rL * wL) * aL - (rR * wR) * aR;
   }
 }
 
@@ -369,16 +376,21 @@ struct k_hydro_stream {
     fetch(x0 + 2, pre);
     // the first put overwrites the slot of plane x0-2, read by the warm-up
     __syncthreads();
-    for (int x = x0; x < x0 + XC; ++x) {
+    const T ivol = (T)1 / g.vol;
+    // One plane: cur holds the own column's state of plane x, nxt receives
+    // that of plane x + 1; Fm is the -x face flux (the +x flux of the plane
+    // before), Fp receives the +x flux.  The loop takes planes in pairs with
+    // the roles swapped, so these registers rotate without copies.
+    auto plane = [&](int x, const XState<T>& cur, XState<T>& nxt, const T* Fm, T* Fp) {
       // the y and z face fluxes of plane x from its states
       T Fy[4], Fz[4];
       faces(x, Fy, Fz);
-      // plane x+2 (fetched during the previous iteration) into its slot,
-      // which planes x-1 .. x+1 in use leave free; then fetch plane x+3
+      // plane x+2 (fetched during the previous plane) into its slot, which
+      // planes x-1 .. x+1 in use leave free; then fetch plane x+3
       put(x + 2, pre);
       if (x + 3 <= x0 + XC + 1) fetch(x + 3, pre);
       __syncthreads();
-      states(x + 1, xp);
+      states(x + 1, nxt);
       __syncthreads();
       if (own) {
         const int b = x & 1;
@@ -387,20 +399,18 @@ struct k_hydro_stream {
         // quantum-fluid.py:338 get_conserved on the (kicked) state of plane x
         T U[4] = {r0 * g.vol, r0 * raw[x & 3][1][ro] * g.vol, r0 * raw[x & 3][2][ro] * g.vol,
                   r0 * raw[x & 3][3][ro] * g.vol};
-        T Fp[4], fl[4], fr[4];
-        // +x face from the own column's registers; the -x face is the +x
-        // face of the plane before
+        T fl[4], fr[4];
+        // +x face from the own column's registers
   #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          fl[k] = xc0.p[k] + (T)0.5 * xc0.sx[k];
-          fr[k] = xp.p[k] - (T)0.5 * xp.sx[k];
+          fl[k] = cur.p[k] + (T)0.5 * cur.sx[k];
+          fr[k] = nxt.p[k] - (T)0.5 * nxt.sx[k];
         }
         face_flux<T, FLUX>(fl[0], fl[1], fl[2], fl[3], fr[0], fr[1], fr[2], fr[3], g.cs, Fp);
   #pragma unroll
         for (int k = 0; k < 4; ++k) {
           U[k] += -fac * Fp[k];
           U[k] += fac * Fm[k];
-          Fm[k] = Fp[k];
         }
         // y and z faces: the +faces from registers, the -faces stored by the
         // neighbouring cell (or ring cell) of the same plane
@@ -416,13 +426,17 @@ struct k_hydro_stream {
         }
         const int i = gidx(x, y0 + ty, z0 + tz);
         const T iM = rcp(U[0]);
-        rho_o[i] = U[0] / g.vol;  // quantum-fluid.py:191-197
+        rho_o[i] = U[0] * ivol;  // quantum-fluid.py:191-197 (M / vol)
         vx_o[i] = U[1] * iM;
         vy_o[i] = U[2] * iM;
         vz_o[i] = U[3] * iM;
       }
-      xm = xc0;
-      xc0 = xp;
+    };
+    // XC is even (a power of two of at least 8, or 32)
+    T Fp[4];
+    for (int x = x0; x < x0 + XC; x += 2) {
+      plane(x, xc0, xp, Fm, Fp);
+      plane(x + 1, xp, xc0, Fp, Fm);
     }
   }
 };
