@@ -249,6 +249,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-gas", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--side-stream", type=int, default=1, choices=(0, 1),
+                    help="star work of the timed steps on the plan's second stream (1) or on "
+                         "the caller's stream (0; every kernel then runs alone)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -278,6 +281,7 @@ def main():
     vel_d = torch.as_tensor(vel0).to(dev)
 
     plan = qj.Plan(n, dtype=torch.complex64, device=dev)
+    plan.set_option(lib_mod.OPT_SIDE_STREAM, args.side_stream)
     sim = qj.KDKSimulation(plan, phys, dt, psi_d, pos_d, vel_d, mass=m_s)
 
     # ---- warmup + timed region ------------------------------------------

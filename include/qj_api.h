@@ -338,10 +338,10 @@ int qj_deposit_adjoint(qj_plan* plan, const double* pos, const double* mass, dou
  * Same results. */
 #define QJ_OPT_Z_SWEEP_BLOCKS 4
 /* QJ_OPT_SIDE_STREAM (default 1): the stepper runs each step's star kick,
- * drift and deposit and its gas update on a second stream of the plan,
- * beside the psi drift sweeps that do not depend on them, and joins the
- * caller's stream before the Poisson source pass; 0 = everything on the
- * caller's stream.  Same results. */
+ * drift and deposit on a second stream of the plan, beside the psi drift
+ * sweeps that do not depend on them, and joins the caller's stream before
+ * the Poisson source pass; 0 = everything on the caller's stream.  Same
+ * results. */
 #define QJ_OPT_SIDE_STREAM 5
 #define QJ_OPT_COUNT 6
 int qj_plan_set_option(qj_plan* plan, int option, int value);

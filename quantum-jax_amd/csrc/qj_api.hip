@@ -734,13 +734,15 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
       if (int rc = ensure_drift<T>(p, drift_dt[step], pr->m_per_hbar, s)) return rc;
     }
     const double kick_t = st->pending + 0.5 * dk;  // owed half kick + this step's first
-    // The star and gas work of the step reads only the previous grid and its
-    // own arrays, and the psi drift sweeps neither read nor write those, so
-    // with the side-stream option it runs on the plan's second stream beside
-    // them; the caller's stream waits for it before the Poisson source pass
-    // (which reads rho_s and the gas density).  Recording (the ensemble)
-    // keeps one stream.
-    const bool fork = (stars || gas) && p->opt[QJ_OPT_SIDE_STREAM] && p->side && !g_recorder;
+    // The star work of the step reads only the previous grid and the star
+    // arrays, and the psi drift sweeps neither read nor write those, so with
+    // the side-stream option it runs on the plan's second stream beside them;
+    // the caller's stream waits for it before the Poisson source pass (which
+    // reads rho_s).  The gas update stays on the caller's stream: its
+    // workgroups fill every CU, so the sweeps beside it only waited for them
+    // (the 256^3 gas step measured the same either way).  Recording (the
+    // ensemble) keeps one stream.
+    const bool fork = stars && p->opt[QJ_OPT_SIDE_STREAM] && p->side && !g_recorder;
     auto side_work = [&](hipStream_t s) -> int {
       // stars: owed closing kick of the previous step and this step's opening
       // half kick (same grid, same positions), drift, deposit at the new positions

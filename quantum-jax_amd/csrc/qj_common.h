@@ -27,7 +27,7 @@ __device__ __forceinline__ cpx<T> operator-(cpx<T> a, cpx<T> b) { return {a.r - 
 // product the same way whatever the compiler's contraction choices in its
 // context (the fused and split schedules of a sweep then agree bitwise).
 template <class T>
-__device__ __forceinline__ T fmad(T a, T b, T c) {
+__host__ __device__ __forceinline__ T fmad(T a, T b, T c) {
   if constexpr (std::is_same<T, float>::value)
     return __builtin_fmaf(a, b, c);
   else
@@ -50,7 +50,7 @@ __device__ __forceinline__ cpx<T> scale(cpx<T> a, T s) { return {a.r * s, a.i * 
 // compile-time loop
 // ---------------------------------------------------------------------------
 template <int I, int E, class F>
-__device__ __forceinline__ void sfor(F&& f) {
+__host__ __device__ __forceinline__ void sfor(F&& f) {
   if constexpr (I < E) {
     f(std::integral_constant<int, I>{});
     sfor<I + 1, E>(f);
@@ -113,22 +113,32 @@ __device__ __forceinline__ double reduce_2pi(double th) {
   return fma(-n, kTwoPiLo, r);
 }
 
-// sin r and cos r for |r| <= 1/2 from their Taylor series in nested Horner
-// form, sin r = r (1 - r^2/(2*3) (1 - r^2/(4*5) (1 - ...))) and
-// cos r = 1 - r^2/(1*2) (1 - r^2/(3*4) (1 - ...)); K levels leave a
-// truncation error below 0.5^(2K+2)/(2K+2)!, under half an ulp of T for
-// K = 5 (float) and K = 8 (double).
+// (-1)^k / n! for the Taylor coefficients below
+constexpr double taylor_coef(int k, int n) {
+  double f = 1.0;
+  for (int i = 2; i <= n; ++i) f *= i;
+  return ((k & 1) ? -1.0 : 1.0) / f;
+}
+
+// sin r and cos r for |r| <= 1/2 from their Taylor polynomials in r^2 by
+// Horner's rule, sin r = r (1 + r^2 (s1 + r^2 (s2 + ... s_K))) with
+// s_k = (-1)^k / (2k+1)! and cos r = 1 + r^2 (c1 + ... c_K) with
+// c_k = (-1)^k / (2k)!.  K terms leave a truncation error below
+// 0.5^(2K+2) / (2K+2)!: under half an ulp of T for K = 4 (float) and K = 7
+// (double).  One FMA per term and function.
 template <class T, int K>
-__device__ __forceinline__ void sincos_small(T r, T& s, T& c) {
+__host__ __device__ __forceinline__ void sincos_small(T r, T& s, T& c) {
   const T r2 = r * r;
-  T ps = (T)1, pc = (T)1;
-  sfor<0, K>([&](auto kk) {
-    constexpr int k = K - (int)decltype(kk)::value;  // K .. 1
-    constexpr T as = (T)(1.0 / ((2.0 * k) * (2.0 * k + 1.0)));
-    constexpr T ac = (T)(1.0 / ((2.0 * k - 1.0) * (2.0 * k)));
-    ps = fmad(-(r2 * as), ps, (T)1);
-    pc = fmad(-(r2 * ac), pc, (T)1);
+  constexpr T sK = (T)taylor_coef(K, 2 * K + 1), cK = (T)taylor_coef(K, 2 * K);
+  T ps = sK, pc = cK;
+  sfor<1, K>([&](auto kk) {
+    constexpr int k = K - (int)decltype(kk)::value;  // K-1 .. 1
+    constexpr T sk = (T)taylor_coef(k, 2 * k + 1), ck = (T)taylor_coef(k, 2 * k);
+    ps = fmad(r2, ps, sk);
+    pc = fmad(r2, pc, ck);
   });
+  ps = fmad(r2, ps, (T)1);
+  pc = fmad(r2, pc, (T)1);
   s = r * ps;
   c = pc;
 }
@@ -140,7 +150,7 @@ __device__ __forceinline__ cpx<T> expmi(double th) {
   double r = reduce_2pi(th);
   if (fabs(r) <= 0.5) {
     T s, c;
-    sincos_small<T, std::is_same<T, float>::value ? 5 : 8>((T)r, s, c);
+    sincos_small<T, std::is_same<T, float>::value ? 4 : 7>((T)r, s, c);
     return {c, -s};
   }
   if constexpr (std::is_same<T, float>::value) {

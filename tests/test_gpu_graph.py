@@ -1,8 +1,8 @@
 """KDK steps captured in a graph by the caller (needs a GPU).
 
 A caller may capture qj_kdk_steps on its stream (torch.cuda.graph here):
-the step issues no allocation, copy or memset, and its side-stream fork and
-join become graph edges.  A capture records the launches with the host
+the step issues no allocation, copy or memset, and the side-stream fork
+and join of star steps become graph edges.  A capture records the launches with the host
 state of that moment, so it is taken in the steady state (after a first
 eager step; gas steps swap their buffers each step, so two steps are
 captured for gas).  Each replay, with eager work and host synchronisation

@@ -525,9 +525,10 @@ def test_sweep_shape_options_agree(qj, cuda, kind):
 @pytest.mark.parametrize("kind", ["stars", "gas", "gas_cfl"])
 @pytest.mark.parametrize("n", [32, 128])
 def test_side_stream_option_agrees(qj, cuda, kind, n):
-    """QJ_OPT_SIDE_STREAM moves a step's star and gas work to the plan's
-    second stream beside the psi drift sweeps; the arithmetic is the same,
-    so with the deterministic deposit the states are the same bits."""
+    """QJ_OPT_SIDE_STREAM moves a step's star work to the plan's second
+    stream beside the psi drift sweeps; the arithmetic is the same, so with
+    the deterministic deposit the states are the same bits (gas steps, which
+    stay on one stream, must not change either)."""
     _lib = __import__("importlib").import_module("quantum-jax_amd._lib")
     g = O.Grid(n, n, n)
     psi, pos, vel, m_s = _stars_case(g, 30 * n, 34)
