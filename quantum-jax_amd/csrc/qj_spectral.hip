@@ -660,6 +660,52 @@ __device__ __forceinline__ void load_herm_rows(cpx<T> (&v)[VW][E], const cpx<T>*
   }
 }
 
+// Wide form (512 threads, 32 columns, sixteen lanes per 256-point pencil):
+// the lanes of a wave hold two rows j of the stride-16 decomposition that
+// are Hermitian partners, j and 16 - j (wave 0: rows 0 and 8, which are
+// their own partners), each over the same 32 columns.  Element m of row j is
+// bin k = j + 16 m, whose mirror N - k is element 15 - m of row 16 - j
+// (row 0: element 16 - m of itself), so the rows load only the bins k <= N/2
+// and take the others from the partner half of the wave (lane xor 32).
+template <class S>
+__device__ __forceinline__ void paired_row(int& j, int& col) {
+  static_assert(S::kThreads == 512 && S::C == 32 && S::TT == 16, "wide 256-point layout only");
+  const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+  col = lane & 31;
+  j = (lane < 32) ? w : (w == 0 ? 8 : 16 - w);
+}
+
+template <int N, int E>
+__device__ __forceinline__ void load_herm_paired(cpx<float> (&v)[1][E],
+                                                 const cpx<float>* __restrict__ a, int base,
+                                                 int se, int j, bool act) {
+  static_assert(N == 256 && E == 16, "the paired rows need 256-point pencils of 16 lanes");
+  // own bins: m = 0..7 (k = j + 16 m <= 127), and k = 128 for row 0
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    cpx<float> t = {0.f, 0.f};
+    if (act) t = a[base + (j + 16 * m) * se];
+    v[0][m] = t;
+  }
+  {
+    cpx<float> t = {0.f, 0.f};
+    if (act && j == 0) t = a[base + (N / 2) * se];
+    v[0][8] = t;
+  }
+  // mirrored bins: the conjugate of the partner's (or the row's own) bin
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const cpx<float> own = v[0][e];
+    const cpx<float> p = {__shfl_xor(own.r, 32, 64), __shfl_xor(own.i, 32, 64)};
+    if (j == 8)
+      v[0][15 - e] = {own.r, -own.i};
+    else if (j != 0)
+      v[0][15 - e] = {p.r, -p.i};
+    else if (e >= 1)
+      v[0][16 - e] = {own.r, -own.i};
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Poisson x pass (inverse, Hermitian-completed): V = IFFT_x(UU); with GRAD
 // a_x + i a_y = -IFFT_x(i kx UU + i Uy), a_z = -IFFT_x(Uzz) (V and a_z from
@@ -686,6 +732,8 @@ template <class T, int N, bool GRAD, bool WIDE = true, bool SIG = false>
 struct k_poisson_x {
   using Cfg = StridedCfg<T, N, WIDE>;
   static constexpr int kThreads = Cfg::TH;
+  // two 512-thread workgroups per CU (LDS) need four waves per SIMD
+  static constexpr int kMinWaves = (Cfg::kWide && GRAD) ? 4 : 0;
   static __device__ __forceinline__ void run(const cpx<T>* __restrict__ H,
                                              const cpx<T>* __restrict__ Uy,
                                              const cpx<T>* __restrict__ Uzz, PassGeom g,
@@ -698,13 +746,20 @@ struct k_poisson_x {
     constexpr int E = S::E;
     constexpr int TT = S::TT;
     __shared__ cpx<T> lds[S::kLdsStrided * VW];
-    const int col = threadIdx.x % S::C, j = threadIdx.x / S::C;
+    int col = threadIdx.x % S::C, j = threadIdx.x / S::C;
+    if constexpr (Cfg::kWide) paired_row<S>(j, col);
     const int q0 = (blockIdx.x * S::C + col) * VW;
     const bool act = q0 < g.ncols;
     const int base = act ? (int)pencil_base(g, q0) : 0;
     const int se = (int)g.s_elem;
+    auto load_herm = [&](cpx<T>(&x)[VW][E], const cpx<T>* __restrict__ a) {
+      if constexpr (Cfg::kWide)
+        load_herm_paired<N, E>(x, a, base, se, j, act);
+      else
+        load_herm_rows<N, VW, E>(x, a, base, se, j, TT, act);
+    };
     cpx<T> v[VW][E];
-    load_herm_rows<N, VW, E>(v, H, base, se, j, TT, act);
+    load_herm(v, H);
     if constexpr (!GRAD) {
       pencil_fft_strided<N, +1, T, VW, false, Cfg::TH>(v, j, col, lds, tw);
       if (act) {
@@ -724,10 +779,10 @@ struct k_poisson_x {
       // UU - i Uzz to V + i a_z (V and a_z are real: their spectra are
       // Hermitian along x).
       cpx<T> u[VW][E];
-      load_herm_rows<N, VW, E>(u, Uy, base, se, j, TT, act);
+      load_herm(u, Uy);
       {
         cpx<T> z[VW][E];
-        load_herm_rows<N, VW, E>(z, Uzz, base, se, j, TT, act);
+        load_herm(z, Uzz);
   #pragma unroll
         for (int m = 0; m < E; ++m) {
           const int k = j + TT * m;
