@@ -42,8 +42,22 @@ __device__ __forceinline__ T clip01(T x) {
 // the 1/dx factors of its three ratios cancel).  The reference's denominator
 // is g + eps (g > 0), g - eps (g < 0) or eps (g = 0); g + copysign(eps, g)
 // differs only for g = -0, where the slope lim * g is -0 either way.
+//
+// The ratio form is minmod(a, b) of the one-sided differences a = f - fm,
+// b = fp - f, up to the eps term: for a, b of one sign with |a| <= |b|,
+// clip(a / g) g = a and clip(b / g) = 1 (g = (a + b) / 2); for opposite signs
+// one of the ratios is negative and the slope is 0.  The eps term changes a
+// slope by the relative amount eps / |g|, 1e-12 / |df/dx|, which is below
+// FP32 resolution for every gradient above 1e-5 (and where it is not, the
+// slope itself is below 1e-5 dx, under FP32 resolution of the face values).
+// FP32 therefore takes minmod directly, med3(a, b, 0) (one instruction
+// instead of a reciprocal and eleven others); FP64 keeps the ratio form.
 template <class T>
 __device__ __forceinline__ T limit(T f, T fm, T fp, T eps_dx) {
+  if constexpr (sizeof(T) == 4) {
+    (void)eps_dx;
+    return __builtin_amdgcn_fmed3f(f - fm, fp - f, 0.0f);
+  }
   T g = (T)0.5 * (fp - fm);
   const T denom = g + copysign(eps_dx, g);
   const T rd = rcp(denom);
@@ -189,8 +203,12 @@ struct k_hydro_stream {
     // A plane of the raw region (tile + 2-cell halo, at most 2 cells per
     // thread) is fetched into registers one iteration before it is written to
     // its LDS ring slot, so the global loads overlap the states and fluxes.
+    // The owed gravity kick is applied in put, not here: an add on the
+    // loaded values would make the fetch wait for its loads, and the fetch
+    // of the next plane is meant to stay in flight through the current one.
     struct Fetch {
       T q[2][4];
+      T a[2][3];
     };
     auto fetch = [&](int x, Fetch& f) {
   #pragma unroll
@@ -199,18 +217,11 @@ struct k_hydro_stream {
         if (c >= RY * RZ) break;
         const int yy = c / RZ - 2, zz = c % RZ - 2;
         const int i = gidx(x, y0 + yy, z0 + zz);
-        T r = rho[i], u = vx[i], v = vy[i], w = vz[i];
-        if (kick != (T)0) {  // owed gravity kick of the gas velocity, quantum-fluid.py:274-276
-          T a[3];
-          ld_acc(A3, i, a);
-          u += a[0] * kick;
-          v += a[1] * kick;
-          w += a[2] * kick;
-        }
-        f.q[k][0] = r;
-        f.q[k][1] = u;
-        f.q[k][2] = v;
-        f.q[k][3] = w;
+        f.q[k][0] = rho[i];
+        f.q[k][1] = vx[i];
+        f.q[k][2] = vy[i];
+        f.q[k][3] = vz[i];
+        if (kick != (T)0) ld_acc(A3, i, f.a[k]);
       }
     };
     auto put = [&](int x, const Fetch& f) {
@@ -220,8 +231,14 @@ struct k_hydro_stream {
         const int c = tid + k * (int)blockDim.x;
         if (c >= RY * RZ) break;
         const int o = (c / RZ) * RZ + (c % RZ);
+        T q[4] = {f.q[k][0], f.q[k][1], f.q[k][2], f.q[k][3]};
+        if (kick != (T)0) {  // owed gravity kick of the gas velocity, quantum-fluid.py:274-276
+          q[1] += f.a[k][0] * kick;
+          q[2] += f.a[k][1] * kick;
+          q[3] += f.a[k][2] * kick;
+        }
   #pragma unroll
-        for (int v = 0; v < 4; ++v) raw[slot][v][o] = f.q[k][v];
+        for (int v = 0; v < 4; ++v) raw[slot][v][o] = q[v];
       }
     };
     auto load_plane = [&](int x) {
