@@ -725,10 +725,8 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
     if (dslot) {
       const int n[3] = {p->d.nx, p->d.ny, p->d.nz};
       const double L[3] = {p->d.Lx, p->d.Ly, p->d.Lz};
-      for (int a = 0; a < 3; ++a)
-        QJ_CHECK(launch_drift_table<T>(p->drift[a], n[a], L[a], 1.0 / (2.0 * pr->m_per_hbar), s,
-                                       dslot),
-                 "drift table");
+      QJ_CHECK(launch_drift_tables3<T>(p->drift, n, L, 1.0 / (2.0 * pr->m_per_hbar), s, dslot),
+               "drift tables");
       p->drift_key_dt = NAN;  // the tables now follow the device dt
     } else if (drift_dt) {
       if (int rc = ensure_drift<T>(p, drift_dt[step], pr->m_per_hbar, s)) return rc;

@@ -515,19 +515,23 @@ struct k_gas_signal {
 // dt = min(dt_max, cfl dx / (Cmax + 1e-30)) (quantum-fluid-accretion.py:700-701);
 // a NaN signal speed gives a NaN dt.  With advance, the step slot moves on:
 // kick = owed + dt/2, owed = dt/2, dts[k] = dt.
+// One block of 1024 threads, so that the partial maxima (2048 at 256^3) are
+// read two per thread rather than in a chain of 32 loads per thread.
 struct k_cfl_finish {
-  static constexpr int kThreads = 64;
+  static constexpr int kThreads = 1024;
   static __device__ __forceinline__ void run(const double* __restrict__ part, int nblk,
                                              double* out, double cfl, double dx, double dt_max,
                                              int advance, double* dts, int k) {
+    __shared__ double wmax[kThreads / 64];
     double m = 0.0;
-    for (int i = threadIdx.x; i < nblk; i += 64) m = (part[i] > m || part[i] != part[i]) ? part[i] : m;
+    for (int i = threadIdx.x; i < nblk; i += kThreads) m = nan_max(m, part[i]);
   #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const double other = __shfl_xor(m, o, 64);
-      m = (other > m || other != other) ? other : m;
-    }
+    for (int o = 32; o > 0; o >>= 1) m = nan_max(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
     if (threadIdx.x == 0) {
+  #pragma unroll
+      for (int w = 1; w < kThreads / 64; ++w) m = nan_max(m, wmax[w]);
       out[kSlotCmax] = m;
       const double c = m;
       const double dth = cfl * dx / (c + 1e-30);
@@ -555,14 +559,14 @@ hipError_t launch_gas_cfl(long long n, const void* vx, const void* vy, const voi
   const double* hdev = (dts && A3) ? out + kSlotOwed : nullptr;
   launch<k_gas_signal<T>>((int)blocks, 256, s, n, (const T*)vx, (const T*)vy, (const T*)vz,
                           (const T*)A3, (T)h, hdev, cs, part);
-  launch<k_cfl_finish>(1, 64, s, (const double*)part, (int)blocks, out, cfl, dx, dt_max,
+  launch<k_cfl_finish>(1, k_cfl_finish::kThreads, s, (const double*)part, (int)blocks, out, cfl, dx, dt_max,
                        dts ? 1 : 0, dts, k);
   return hipGetLastError();
 }
 
 hipError_t launch_cfl_finish(const double* part, int nblk, double cfl, double dx, double dt_max,
                              double* slot, double* dts, int k, hipStream_t s) {
-  launch<k_cfl_finish>(1, 64, s, part, nblk, slot, cfl, dx, dt_max, 1, dts, k);
+  launch<k_cfl_finish>(1, k_cfl_finish::kThreads, s, part, nblk, slot, cfl, dx, dt_max, 1, dts, k);
   return hipGetLastError();
 }
 

@@ -37,6 +37,11 @@ hipError_t launch_twiddles(void* tw, int n, hipStream_t s);
 template <class T>
 hipError_t launch_drift_table(void* tab, int n, double L, double c, hipStream_t s,
                               const double* dslot = nullptr);
+// The three axes' tables (tab[a] of n[a] entries over box length L[a]) in one
+// launch; dslot as in launch_drift_table.
+template <class T>
+hipError_t launch_drift_tables3(void* const tab[3], const int n[3], const double L[3], double c,
+                                hipStream_t s, const double* dslot);
 
 template <class T>
 hipError_t launch_kick(void* psi, const void* V, const void* Vb, const void* Vx, double coef,

@@ -42,8 +42,8 @@ __global__ void k_twiddles(cpx<T>* tw, int n) {
 // quantum-stars.py:308 (one axis of the separable kinetic phase).  With
 // dslot, c is a factor and dt comes from the device step slot.
 template <class T>
-__global__ void k_drift_table(cpx<T>* tab, int n, double L, double c, const double* dslot) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_drift_table_body(cpx<T>* tab, int n, double L, double c,
+                                                   const double* dslot, int k) {
   if (k >= n) return;
   if (dslot) c *= dslot[kSlotDt];
   double kp = 2.0 * M_PI / L * (double)freq_index(k, n);
@@ -52,6 +52,27 @@ __global__ void k_drift_table(cpx<T>* tab, int n, double L, double c, const doub
   sincos(th, &s, &co);
   double inv = 1.0 / (double)n;
   tab[k] = {(T)(co * inv), (T)(-s * inv)};
+}
+
+template <class T>
+__global__ void k_drift_table(cpx<T>* tab, int n, double L, double c, const double* dslot) {
+  k_drift_table_body<T>(tab, n, L, c, dslot, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
+// The three axes' tables in one launch (blockIdx.y = axis), for steps that
+// take dt from the device step slot and so rebuild them every step.
+template <class T>
+struct DriftTables3 {
+  cpx<T>* tab[3];
+  int n[3];
+  double L[3];
+};
+
+template <class T>
+__global__ void k_drift_tables3(DriftTables3<T> t, double c, const double* dslot) {
+  const int a = blockIdx.y;
+  k_drift_table_body<T>(t.tab[a], t.n[a], t.L[a], c, dslot,
+                        (int)(blockIdx.x * blockDim.x + threadIdx.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -1111,6 +1132,21 @@ hipError_t launch_drift_table(void* tab, int n, double L, double c, hipStream_t 
 }
 
 template <class T>
+hipError_t launch_drift_tables3(void* const tab[3], const int n[3], const double L[3], double c,
+                                hipStream_t s, const double* dslot) {
+  DriftTables3<T> t;
+  int nmax = 0;
+  for (int a = 0; a < 3; ++a) {
+    t.tab[a] = (cpx<T>*)tab[a];
+    t.n[a] = n[a];
+    t.L[a] = L[a];
+    nmax = n[a] > nmax ? n[a] : nmax;
+  }
+  k_drift_tables3<T><<<dim3((nmax + 255) / 256, 3), 256, 0, s>>>(t, c, dslot);
+  return hipGetLastError();
+}
+
+template <class T>
 hipError_t launch_kick(void* psi, const void* V, const void* Vb, const void* Vx, double coef,
                        long long n, hipStream_t s) {
   long long blocks = (n + 255) / 256;
@@ -1439,6 +1475,8 @@ hipError_t launch_adj_poisson(const Dims& d, const SpecTables<T>& tb, const void
   template hipError_t launch_twiddles<T>(void*, int, hipStream_t);                              \
   template hipError_t launch_drift_table<T>(void*, int, double, double, hipStream_t,             \
                                             const double*);           \
+  template hipError_t launch_drift_tables3<T>(void* const[3], const int[3], const double[3],     \
+                                              double, hipStream_t, const double*);              \
   template hipError_t launch_kick<T>(void*, const void*, const void*, const void*, double,       \
                                      long long, hipStream_t);                                   \
   template hipError_t launch_drift_axis<T>(const Dims&, const SpecTables<T>&, int, void*,        \
