@@ -176,7 +176,9 @@ typedef struct {
   void* brho;       /* real N, b*rho for self-interaction; NULL when b == 0   */
   double* diag;     /* QJ_DIAG_COUNT doubles                                  */
   /* host-side bookkeeping, initialise to 0 */
-  int grid_valid;   /* grid holds V and a of the current state                */
+  int grid_valid;   /* 1: grid holds V and a of the current state; 2: V, and
+                       the stars' corner accelerations are in star_scratch
+                       (QJ_OPT_STAR_GATHER); 0: not computed                   */
   int gas_swapped;  /* the live gas fields are gas_tmp (after an odd count)   */
   double pending;   /* time of the owed kick of psi and gas velocity          */
   double pending_star; /* time of the owed closing kick of the star velocities */
@@ -343,7 +345,16 @@ int qj_deposit_adjoint(qj_plan* plan, const double* pos, const double* mass, dou
  * the Poisson source pass; 0 = everything on the caller's stream.  Same
  * results. */
 #define QJ_OPT_SIDE_STREAM 5
-#define QJ_OPT_COUNT 6
+/* QJ_OPT_STAR_GATHER (default 1): in stars steps with the tile-binned
+ * deposit (no gas, not recording a tape) the deposit bins the stars by the
+ * columns of the Poisson x pass, and that pass hands each star the
+ * accelerations at its eight CIC corners instead of writing the
+ * acceleration grid; the next step's star kicks read them.  Same star and
+ * psi results (the same corner values); the state's acceleration grid
+ * (grid after V) is then not written (qj_state.grid_valid 2); 0 = the grid
+ * is written and the stars gather from it. */
+#define QJ_OPT_STAR_GATHER 6
+#define QJ_OPT_COUNT 7
 int qj_plan_set_option(qj_plan* plan, int option, int value);
 int qj_plan_get_option(const qj_plan* plan, int option, int* value);
 

@@ -155,7 +155,7 @@ struct qj_plan {
   int nblk_sig = 0;
   // schedule options (qj_plan_set_option): fused x, binned, deterministic,
   // wide Poisson sweeps, z-sweep blocks, side stream
-  int opt[QJ_OPT_COUNT] = {1, 1, 0, 1, 0, 1};
+  int opt[QJ_OPT_COUNT] = {1, 1, 0, 1, 0, 1, 1};
   // second stream (default priority) for the star and gas work of a step,
   // with the fork and join events
   hipStream_t side = nullptr;
@@ -274,7 +274,7 @@ template <class T>
 int poisson_tail(qj_plan* p, const Work& w, int grad, double four_pi_G, double rho_bar,
                  int dc_mode, void* V, void* A3, const void* addend,
                  double* diag, hipStream_t s, double lp_kc2 = 0.0,
-                 GasSignalFold* sig = nullptr) {
+                 GasSignalFold* sig = nullptr, const StarGather* gather = nullptr) {
   SpecTables<T> tb = tables<T>(p);
   QJ_STAGE(kStageFftY, launch_fft_y_half<T>(p->d, tb, w.H, s), "fft_y");
   QJ_STAGE(kStagePoissonZ, launch_poisson_z<T>(p->d, tb, grad, w.H, w.Uz, four_pi_G, rho_bar, dc_mode,
@@ -282,7 +282,8 @@ int poisson_tail(qj_plan* p, const Work& w, int grad, double four_pi_G, double r
            "poisson_z");
   QJ_STAGE(kStagePoissonY, launch_poisson_y<T>(p->d, tb, grad, w.H, w.Uz, w.Uy, s), "poisson_y");
   QJ_STAGE(kStagePoissonX,
-           launch_poisson_x<T>(p->d, tb, grad, w.H, w.Uy, w.Uz, V, A3, addend, diag, s, sig),
+           launch_poisson_x<T>(p->d, tb, grad, w.H, w.Uy, w.Uz, V, A3, addend, diag, s, sig,
+                               gather),
            "poisson_x");
   return QJ_OK;
 }
@@ -633,19 +634,35 @@ int qj_solve_hydro(qj_plan* p, const void* rho, const void* vx, const void* vy, 
 // ===========================================================================
 namespace {
 
-// V and A3 of the current state (star deposit, source pass, Poisson chain).
+// Columns of the fused star gather for a step of this state (0: the step
+// writes the acceleration grid and the stars gather from it): stars with
+// the tile-binned deposit, no gas, no tape (the adjoint reads the grid),
+// and a Poisson x pass whose column tiles the deposit supports.
 template <class T>
-int initial_grid_t(qj_plan* p, qj_state* st, const qj_params* pr, const Work& w, hipStream_t s) {
+int gather_cols(const qj_plan* p, const qj_state* st, bool tape) {
+  if (st->np <= 0 || !st->star_scratch || !p->opt[QJ_OPT_BINNED_DEPOSIT] || st->gas[0] || tape ||
+      !p->opt[QJ_OPT_STAR_GATHER])
+    return 0;
+  return poisson_x_gather_columns(p->d, sizeof(T) == 4, p->opt[QJ_OPT_WIDE_SWEEPS]);
+}
+
+// V and A3 of the current state (star deposit, source pass, Poisson chain);
+// cols > 0: the fused star gather (V and the stars' corner values).
+template <class T>
+int initial_grid_t(qj_plan* p, qj_state* st, const qj_params* pr, const Work& w, hipStream_t s,
+                   int cols = 0) {
   SpecTables<T> tb = tables<T>(p);
   const bool stars = st->np > 0;
   const bool gas = st->gas[0] != nullptr;
   void* A3 = (stars || gas) ? plane<T>(st->grid, p, 1) : nullptr;
   void* brho = (pr->b_coeff != 0.0) ? st->brho : nullptr;
   const bool binned = st->star_scratch && p->opt[QJ_OPT_BINNED_DEPOSIT];
+  StarGather ga{};
   if (stars && binned) {
     // tile-binned deposit of the current positions (overwrites rho_s)
-    StarBins sb = carve_star_scratch(p->d, st->np, st->star_scratch);
-    QJ_CHECK(fill<int>(sb.count, sb.ntiles, 0, s), "bin reset");
+    StarBins sb = carve_star_scratch(p->d, st->np, st->star_scratch, sizeof(T), cols);
+    if (cols) ga = StarGather{sb.offset, sb.sorted, sb.desc, sb.cv};
+    QJ_CHECK(fill<int>(sb.count, (long long)sb.ntiles * kCountStride, 0, s), "bin reset");
     QJ_STAGE(kStageDeposit,
              launch_star_step_binned<T>(p->d, st->pos, st->vel, st->np, nullptr, 0.0, 0.0, 0.0, 0,
                                         sb, s),
@@ -669,9 +686,10 @@ int initial_grid_t(qj_plan* p, qj_state* st, const qj_params* pr, const Work& w,
                               pr->rho_bar, w.H, p->partials, st->diag, s),
            "source_x");
   if (int rc = poisson_tail<T>(p, w, (stars || gas) ? 1 : 0, pr->four_pi_G, pr->rho_bar,
-                               pr->dc_mode, st->grid, A3, nullptr, st->diag, s))
+                               pr->dc_mode, st->grid, A3, nullptr, st->diag, s, 0.0, nullptr,
+                               ga.cv ? &ga : nullptr))
     return rc;
-  st->grid_valid = 1;
+  st->grid_valid = ga.cv ? 2 : 1;
   return QJ_OK;
 }
 
@@ -714,11 +732,22 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
   void* src = nullptr;
   const bool binned = st->star_scratch && p->opt[QJ_OPT_BINNED_DEPOSIT];
   const int clear = 0;  // both deposits leave rho_s as the source of this state
+  const int cols = gather_cols<T>(p, st, grid_out != nullptr);
+  // the corner values of the fused gather are served from the binned
+  // deposit's scratch; the float-atomic star step reads the grid, so a state
+  // that kept only corner values has its grid computed again
+  if (st->grid_valid == 2 && !(stars && binned)) st->grid_valid = 0;
 
   for (int step = 0; step < nsteps; ++step) {
     if (!st->grid_valid) {
-      if (int rc = initial_grid_t<T>(p, st, pr, w, s)) return rc;
+      if (int rc = initial_grid_t<T>(p, st, pr, w, s, cols)) return rc;
     }
+    // the star registrations of this step (column tiles with the fused
+    // gather); the kicks read the corner values when the previous Poisson
+    // x pass left them instead of the grid
+    StarBins sb{};
+    if (stars && binned) sb = carve_star_scratch(p->d, st->np, st->star_scratch, sizeof(T), cols);
+    const void* cv_in = (st->grid_valid == 2) ? sb.cv : nullptr;
     // per-step kick and drift times (a schedule, e.g. the comoving factors of
     // quantum-cosmo.py:170-175), else dt for both
     const double dk = kick_dt ? kick_dt[step] : dt;
@@ -745,10 +774,9 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
       // stars: owed closing kick of the previous step and this step's opening
       // half kick (same grid, same positions), drift, deposit at the new positions
       if (stars && binned) {
-        StarBins sb = carve_star_scratch(p->d, st->np, st->star_scratch);
         QJ_STAGE(kStageStarStep,
                  launch_star_step_binned<T>(p->d, st->pos, st->vel, st->np, A3, st->pending_star,
-                                            0.5 * dt, dt, 1, sb, s),
+                                            0.5 * dt, dt, 1, sb, s, cv_in),
                  "star step");
         QJ_STAGE(kStageDeposit,
                  launch_tile_deposit<T>(p->d, st->pos, st->mass, st->m_scalar, st->np, sb,
@@ -821,10 +849,13 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
       sig->part = p->sig_part;
       sig->cap = p->nblk_sig;
     }
+    StarGather ga{};
+    if (cols) ga = StarGather{sb.offset, sb.sorted, sb.desc, sb.cv};
     if (int rc = poisson_tail<T>(p, w, grad, pr->four_pi_G, pr->rho_bar, pr->dc_mode, Vn, A3n,
-                                 nullptr, st->diag, s, 0.0, sig))
+                                 nullptr, st->diag, s, 0.0, sig, cols ? &ga : nullptr))
       return rc;
     if (grid_out) st->grid = grid_out;
+    st->grid_valid = cols ? 2 : 1;
     // the closing half kicks (psi, gas and stars) are owed to the next step
     if (!dslot) st->pending = 0.5 * dk;
     if (stars) st->pending_star = 0.5 * dt;
@@ -903,9 +934,15 @@ int kdk_finalize_t(qj_plan* p, qj_state* st, const qj_params* pr, hipStream_t s)
     st->pending = 0.0;
   }
   if (st->pending_star != 0.0 && st->np > 0) {
+    // the corner values of the fused gather, when the last Poisson x pass
+    // left those instead of the grid
+    const void* cv = (st->grid_valid == 2)
+                         ? carve_star_scratch(p->d, st->np, st->star_scratch, sizeof(T)).cv
+                         : nullptr;
     QJ_CHECK(fill<double>(st->diag + kDiagDtS, 1, __builtin_inf(), s), "dt_s reset");
     QJ_STAGE(kStageStarKick,
-             launch_star_kick<T>(p->d, st->pos, st->vel, st->np, A3, st->pending_star, st->diag, s),
+             launch_star_kick<T>(p->d, st->pos, st->vel, st->np, A3, st->pending_star, st->diag, s,
+                                 cv),
              "star kick");
     st->pending_star = 0.0;
   }
@@ -924,7 +961,9 @@ int kdk_record_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, qj_t
   if (int rc = ensure_drift<T>(p, pr->dt, pr->m_per_hbar, s)) return rc;
   const size_t cb = (size_t)p->ncell() * 2 * sizeof(T);
   const size_t xb = (size_t)st->np * 3 * sizeof(double);
-  if (!st->grid_valid)
+  // the tape holds whole grids: a state whose stars kept only their corner
+  // values (grid_valid 2) has its grid recomputed first
+  if (st->grid_valid != 1)
     if (int rc = initial_grid_t<T>(p, st, pr, w, s)) return rc;
   if (grid0) QJ_CHECK(copy_async(grid0, st->grid, grid_bytes(p), s), "tape grid0");
   // each step's Poisson solve writes its grid straight into the tape (the
@@ -1193,7 +1232,7 @@ int qj_bh_accretion(qj_plan* p, void* psi, const double* pos, const double* vel,
 
 size_t qj_star_scratch_size(const qj_plan* p, long long np) {
   if (!p || np < 0) return 0;
-  return star_scratch_bytes(p->d, np);
+  return star_scratch_total_bytes(p->d, np, (int)p->rsize);
 }
 
 size_t qj_adjoint_scratch_size(const qj_plan* p) {

@@ -41,8 +41,12 @@ __host__ __device__ __forceinline__ int imod(double fi, int n) {
 }
 
 // One axis of get_cic_indices_and_weights (quantum-stars.py:202-208).
+// Contraction is switched off here as well, so that every translation unit
+// that includes this header (the Poisson x pass of the fused star gather
+// among them) computes the same indices as the particle kernels.
 __device__ __forceinline__ void cic_axis(double x, double dx, int n, int& i0, int& i1, double& w0,
                                          double& w1, bool& ok) {
+#pragma clang fp contract(off)
   double fi = floor((x - 0.5 * dx) / dx);
   double fip1 = fi + 1.0;
   w0 = ((fip1 + 0.5) * dx - x) / dx;
@@ -94,10 +98,22 @@ __device__ __forceinline__ void ld3(const T* __restrict__ A3, int idx, double o[
   o[2] = q[2];
 }
 
+// sum over the corners k of w_k a_k, in the corner order
+__device__ __forceinline__ void interp_sum(const Cic& c, const double (&a)[8][3], double acc[3]) {
+  acc[0] = acc[1] = acc[2] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int sx = kCorner[k][0], sy = kCorner[k][1], sz = kCorner[k][2];
+    const double w = c.wx[sx] * c.wy[sy] * c.wz[sz];
+    acc[0] += w * a[k][0];
+    acc[1] += w * a[k][1];
+    acc[2] += w * a[k][2];
+  }
+}
+
 template <class T>
 __device__ __forceinline__ void interp_one(const CicGeom& g, const Cic& c, const T* A3,
                                            double acc[3]) {
-  acc[0] = acc[1] = acc[2] = 0.0;
   if (!c.ok) {
     acc[0] = acc[1] = acc[2] = __builtin_nan("");
     return;
@@ -109,15 +125,47 @@ __device__ __forceinline__ void interp_one(const CicGeom& g, const Cic& c, const
     const int sx = kCorner[k][0], sy = kCorner[k][1], sz = kCorner[k][2];
     ld3(A3, (c.ix[sx] * g.ny + c.iy[sy]) * g.nz + c.iz[sz], a[k]);
   }
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int sx = kCorner[k][0], sy = kCorner[k][1], sz = kCorner[k][2];
-    const double w = c.wx[sx] * c.wy[sy] * c.wz[sz];
-    acc[0] += w * a[k][0];
-    acc[1] += w * a[k][1];
-    acc[2] += w * a[k][2];
-  }
+  interp_sum(c, a, acc);
 }
+
+// interp_one from the corner values a fused gather stored for the star:
+// cvp[3 k + e] is component e at corner k, the grid value interp_one would
+// load, so the result is the same to the bit.  96 (FP32) or 192 (FP64)
+// contiguous bytes per star, 16-byte aligned.
+template <class T>
+__device__ __forceinline__ void interp_cv(const Cic& c, const T* __restrict__ cvp, double acc[3]) {
+  if (!c.ok) {
+    acc[0] = acc[1] = acc[2] = __builtin_nan("");
+    return;
+  }
+  double a[8][3];
+  if constexpr (sizeof(T) == 4) {
+    const float4* q = reinterpret_cast<const float4*>(cvp);
+    float f[24];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const float4 t = q[i];
+      f[4 * i] = t.x;
+      f[4 * i + 1] = t.y;
+      f[4 * i + 2] = t.z;
+      f[4 * i + 3] = t.w;
+    }
+#pragma unroll
+    for (int i = 0; i < 24; ++i) a[i / 3][i % 3] = f[i];
+  } else {
+    const double2* q = reinterpret_cast<const double2*>(cvp);
+#pragma unroll
+    for (int i = 0; i < 12; ++i) {
+      const double2 t = q[i];
+      a[(2 * i) / 3][(2 * i) % 3] = t.x;
+      a[(2 * i + 1) / 3][(2 * i + 1) % 3] = t.y;
+    }
+  }
+  interp_sum(c, a, acc);
+}
+
+// reals of corner values per star
+constexpr int kCvReals = 24;
 
 // floored modulo with numpy semantics (quantum-stars.py:312 jnp.mod(pos, L)).
 // The remainder fmod(x, L) is exact; for 0 < L and -L < x < 2L it is x, or

@@ -92,13 +92,27 @@ struct GasSignalFold {
   int nblk;
 };
 
+// Fused star gather of a Poisson x gradient pass: workgroup t writes, for
+// every star registered in column tile t (offset/sorted of the step's
+// deposit over make_column_tiles(d, poisson_x_gather_columns(...))), the
+// acceleration (a_x, a_y, a_z) of each of the star's CIC corners that lie in
+// its columns into cv[(8 p + k) * 3 + c] (k: the corner order of the
+// gather), locating them by the registrations' corner descriptors (desc, of
+// the deposit's scatter pass).  A3 is then not written.
+struct StarGather {
+  const int* offset;
+  const int* sorted;
+  const unsigned long long* desc;  // corner descriptors of the registrations
+  void* cv;
+};
+
 // Writes V and, with grad, the cell-interleaved grid A3 = (a_x, a_y, a_z)
 // and folds max |a| into diag[kDiagAMax]; sig: the CFL signal fold above.
 template <class T>
 hipError_t launch_poisson_x(const Dims& d, const SpecTables<T>& tb, int grad, const void* H,
                             const void* Uy, const void* Uzz, void* V, void* A3,
                             const void* addend, double* diag, hipStream_t s,
-                            GasSignalFold* sig = nullptr);
+                            GasSignalFold* sig = nullptr, const StarGather* gather = nullptr);
 
 // Largest number of workgroups launch_poisson_x uses on grid d (either
 // precision, either sweep form): the size of a GasSignalFold's part array.
@@ -127,7 +141,8 @@ hipError_t launch_star_step(const Dims& d, double* pos, double* vel, long long n
 // vel += interp(A3)*h; diag[dt_s] = min |dx/vel| (diag[dt_s] must hold +inf)
 template <class T>
 hipError_t launch_star_kick(const Dims& d, const double* pos, double* vel, long long np,
-                            const void* A3, double h, double* diag, hipStream_t s);
+                            const void* A3, double h, double* diag, hipStream_t s,
+                            const void* cv = nullptr);
 
 // ---- adjoints of the particle operations ----------------------------------
 // Adjoint of vel += h*interp(A3, pos), preceded (dt_xv != 0) by that of
@@ -212,22 +227,49 @@ struct TileGeom {
   int ntx, nty, ntz;  // tiles per axis
   int ntiles;
 };
+// Tile t's registration counter is count[t * kCountStride]: one 128-byte
+// line per counter.  Counters that share a line share its atomic unit, and
+// the counting atomics of a step (one to eight per star) then queue behind
+// each other (32 counters per line made the counting pass of 2^16 stars on
+// 512 tiles take 46 us instead of 16).
+constexpr int kCountStride = 32;
 struct StarBins {
-  int* count;   // ntiles, zero between steps
+  int* count;   // ntiles counters, kCountStride ints apart, zero between steps
   int* offset;  // ntiles + 1
   int* sorted;  // 8 np registrations at most
   int* rank;    // 8 np: slot of each registration within its tile
   int ntiles;
+  TileGeom tg;  // the tiles the registrations refer to
+  void* cv;     // corner values of the fused gather, 24 reals per star (or null)
+  unsigned long long* desc;  // column tiles: corner descriptor per registration
 };
 TileGeom make_tiles(const Dims& d);
+// Column tiles of the fused star gather: a tile is the set of x pencils that
+// one workgroup of the Poisson x gradient pass transforms, all of x by cols
+// consecutive (y, z) columns (q = y nz + z), so tile t is that pass's
+// workgroup t and a step's deposit registrations are also its gather lists.
+constexpr int kTileColMaxCells = 8192;
+TileGeom make_column_tiles(const Dims& d, int cols);
+// Columns per workgroup of the Poisson x gradient pass on grid d (c64: the
+// single-precision build; wide: the wide sweep form), or 0 when the fused
+// gather does not serve that pass (column tiles above kTileColMaxCells).
+int poisson_x_gather_columns(const Dims& d, int c64, int wide);
+// Bytes of the registration part of a star scratch (either tile kind), and
+// of the whole scratch with the corner-value area for reals of real_bytes.
 size_t star_scratch_bytes(const Dims& d, long long np);
-StarBins carve_star_scratch(const Dims& d, long long np, void* scratch);
+size_t star_scratch_total_bytes(const Dims& d, long long np, int real_bytes);
+// cols = 0: the deposit's own tiles; else the column tiles of cols columns.
+// real_bytes = 0: no corner-value area (a taped copy of the registrations).
+StarBins carve_star_scratch(const Dims& d, long long np, void* scratch, int real_bytes = 0,
+                            int cols = 0);
 // Kick/drift/wrap as launch_star_step (move = 1) or nothing (move = 0), and
-// registration counting of the (new) positions.
+// registration counting of the (new) positions in the tiles of sb.  The
+// kicks interpolate A3, or with cv non-null the star's eight corner values
+// that the fused gather left there (the same numbers, so the same result).
 template <class T>
 hipError_t launch_star_step_binned(const Dims& d, double* pos, double* vel, long long np,
                                    const void* A3, double h_owed, double h1, double dt, int move,
-                                   const StarBins& sb, hipStream_t s);
+                                   const StarBins& sb, hipStream_t s, const void* cv = nullptr);
 // Scan, scatter and per-tile deposit: overwrites every cell of rho.  det:
 // order-independent fixed-point accumulation (bitwise reproducible).
 template <class T>

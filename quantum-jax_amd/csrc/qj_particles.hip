@@ -117,13 +117,17 @@ struct k_star_kick {
   static __device__ __forceinline__ void run(CicGeom g, const double* __restrict__ pos,
                                              double* __restrict__ vel, long long np,
                                              const T* __restrict__ A3, double h,
-                                             double* __restrict__ diag) {
+                                             double* __restrict__ diag,
+                                             const T* __restrict__ cv) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double dts = __builtin_inf();
     if (p < np) {
       Cic c = cic(g, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]);
       double a[3];
-      interp_one(g, c, A3, a);
+      if (cv)
+        interp_cv(c, cv + kCvReals * p, a);
+      else
+        interp_one(g, c, A3, a);
   #pragma unroll
       for (int d = 0; d < 3; ++d) {
         double v = vel[3 * p + d] + a[d] * h;
@@ -201,10 +205,30 @@ __device__ __forceinline__ void register_tiles(const TileGeom& tg, const Cic& c,
   const unsigned m = stencil_tiles(tg, c, ids);
 #pragma unroll
   for (int k = 0; k < 8; ++k)
-    if (m >> k & 1u) r[k] = atomicAdd(&count[ids[k]], 1);
+    if (m >> k & 1u) r[k] = atomicAdd(&count[ids[k] * kCountStride], 1);
 #pragma unroll
   for (int k = 0; k < 8; ++k)
     if (m >> k & 1u) rank[8 * p + k] = r[k];
+}
+
+// Corner descriptor of a star for column tile t (all of x by 2^(ly + lz)
+// consecutive (y, z) columns from q = t 2^(ly + lz), q = y nz + z): bits
+// 0-8 and 9-17 hold the star's two cell x indices, and the four 9-bit
+// fields from bit 18 hold, for the (y, z) corner pairs (sy, sz) in the
+// order 2 sy + sz, 1 + the column's index in the tile, or 0 when the column
+// is outside it.  The Poisson x pass reads these instead of the position.
+__device__ __forceinline__ unsigned long long corner_desc(const CicGeom& g, const TileGeom& tg,
+                                                         const Cic& c, int t) {
+  const int w = tg.ly + tg.lz;
+  const int q_lo = t << w;
+  unsigned long long d = (unsigned long long)c.ix[0] | ((unsigned long long)c.ix[1] << 9);
+#pragma unroll
+  for (int yz = 0; yz < 4; ++yz) {
+    const int qc = c.iy[yz >> 1] * g.nz + c.iz[yz & 1] - q_lo;
+    const unsigned f = ((unsigned)qc < (1u << w)) ? (unsigned)qc + 1u : 0u;
+    d |= (unsigned long long)f << (18 + 9 * yz);
+  }
+  return d;
 }
 
 // Adjoint of vel += h * interp(A3, pos) (and, first, of the drift
@@ -301,7 +325,7 @@ struct k_star_step_bin {
                                              double* __restrict__ vel, long long np,
                                              const T* __restrict__ A3, double h_owed, double h1,
                                              double dt, int* __restrict__ count,
-                                             int* __restrict__ rank) {
+                                             int* __restrict__ rank, const T* __restrict__ cv) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     double x[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
@@ -310,7 +334,10 @@ struct k_star_step_bin {
       if (h_owed != 0.0 || h1 != 0.0) {
         Cic c = cic(g, x[0], x[1], x[2]);
         double a[3];
-        interp_one(g, c, A3, a);
+        if (cv)
+          interp_cv(c, cv + kCvReals * p, a);
+        else
+          interp_one(g, c, A3, a);
         if (h_owed != 0.0) {
   #pragma unroll
           for (int d = 0; d < 3; ++d) v[d] = v[d] + a[d] * h_owed;  // quantum-stars.py:321
@@ -353,11 +380,11 @@ struct k_bin_scan {
     int local = 0;
     if (held_all) {
 #pragma unroll
-      for (int r = 0; r < kHeld; ++r) held[r] = (b + r < e) ? count[b + r] : 0;
+      for (int r = 0; r < kHeld; ++r) held[r] = (b + r < e) ? count[(b + r) * kCountStride] : 0;
 #pragma unroll
       for (int r = 0; r < kHeld; ++r) local += held[r];
     } else {
-      for (int i = b; i < e; ++i) local += count[i];
+      for (int i = b; i < e; ++i) local += count[i * kCountStride];
     }
     // inclusive scan of the per-thread sums within the wave, then across waves
     int incl = local;
@@ -376,15 +403,15 @@ struct k_bin_scan {
         if (b + r < e) {
           offset[b + r] = run;
           run += held[r];
-          count[b + r] = 0;
+          count[(b + r) * kCountStride] = 0;
         }
       }
     } else {
       for (int i = b; i < e; ++i) {
-        const int c = count[i];
+        const int c = count[i * kCountStride];
         offset[i] = run;
         run += c;
-        count[i] = 0;
+        count[i * kCountStride] = 0;
       }
     }
     if (tid == 1023) offset[ntiles] = base + incl;
@@ -392,13 +419,16 @@ struct k_bin_scan {
 };
 
 // Registrations into their tile's slots (the ranks of the counting pass).
+// With desc (column tiles of the fused star gather), each registration also
+// gets the star's corner descriptor relative to its tile (corner_desc).
 struct k_bin_scatter {
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(CicGeom g, TileGeom tg,
                                              const double* __restrict__ pos, long long np,
                                              const int* __restrict__ rank,
                                              const int* __restrict__ offset,
-                                             int* __restrict__ sorted) {
+                                             int* __restrict__ sorted,
+                                             unsigned long long* __restrict__ desc) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     Cic c = cic(g, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]);
@@ -411,6 +441,11 @@ struct k_bin_scatter {
 #pragma unroll
     for (int k = 0; k < 8; ++k)
       if (m >> k & 1u) sorted[o[k]] = (int)p;
+    if (desc) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (m >> k & 1u) desc[o[k]] = corner_desc(g, tg, c, ids[k]);
+    }
   }
 };
 
@@ -762,6 +797,19 @@ inline int pblocks(long long np) { return (int)((np + 255) / 256); }
     else { constexpr int VOL_ = kTileMaxCells; __VA_ARGS__; }                \
   } while (0)
 
+// QJ_TILE_VOL with the column tiles' larger volume as well (deposit kernels
+// whose LDS holds one real or one 64-bit word per cell).
+#define QJ_TILE_VOL_COL(tg, ...)                                             \
+  do {                                                                       \
+    const int vcol_ = 1 << ((tg).lx + (tg).ly + (tg).lz);                    \
+    if (vcol_ > kTileMaxCells) {                                             \
+      constexpr int VOL_ = kTileColMaxCells;                                 \
+      __VA_ARGS__;                                                           \
+    } else {                                                                 \
+      QJ_TILE_VOL(tg, __VA_ARGS__);                                          \
+    }                                                                        \
+  } while (0)
+
 }  // namespace
 
 template <class T>
@@ -800,9 +848,11 @@ hipError_t launch_star_step(const Dims& d, double* pos, double* vel, long long n
 
 template <class T>
 hipError_t launch_star_kick(const Dims& d, const double* pos, double* vel, long long np,
-                            const void* A3, double h, double* diag, hipStream_t s) {
+                            const void* A3, double h, double* diag, hipStream_t s,
+                            const void* cv) {
   if (np <= 0) return hipSuccess;
-  launch<k_star_kick<T>>(pblocks(np), 256, s, make_geom(d), pos, vel, np, (const T*)A3, h, diag);
+  launch<k_star_kick<T>>(pblocks(np), 256, s, make_geom(d), pos, vel, np, (const T*)A3, h, diag,
+                         (const T*)cv);
   return hipGetLastError();
 }
 
@@ -847,37 +897,84 @@ TileGeom make_tiles(const Dims& d) {
   return t;
 }
 
-size_t star_scratch_bytes(const Dims& d, long long np) {
-  const TileGeom t = make_tiles(d);
-  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  return al((size_t)t.ntiles * 4) + al((size_t)(t.ntiles + 1) * 4) + 2 * al((size_t)np * 8 * 4);
+TileGeom make_column_tiles(const Dims& d, int cols) {
+  auto lg = [](int v) {
+    int r = 0;
+    while ((1 << r) < v) ++r;
+    return r;
+  };
+  TileGeom t;
+  const int ncols = d.ny * d.nz;
+  if (cols > ncols) cols = ncols;
+  t.lx = lg(d.nx);
+  if (cols <= d.nz) {
+    t.ly = 0;
+    t.lz = lg(cols);
+  } else {
+    t.lz = lg(d.nz);
+    t.ly = lg(cols / d.nz);
+  }
+  t.ntx = 1;
+  t.nty = d.ny >> t.ly;
+  t.ntz = d.nz >> t.lz;
+  t.ntiles = t.nty * t.ntz;
+  return t;
 }
 
-StarBins carve_star_scratch(const Dims& d, long long np, void* scratch) {
-  const TileGeom t = make_tiles(d);
-  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+namespace {
+// count and offset entries for either tile kind (column tiles have at most
+// as many tiles as the smallest Poisson x workgroup covers columns: 16)
+int max_tiles(const Dims& d) {
+  const int a = make_tiles(d).ntiles, b = make_column_tiles(d, 16).ntiles;
+  return a > b ? a : b;
+}
+size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+}  // namespace
+
+size_t star_scratch_bytes(const Dims& d, long long np) {
+  const int nt = max_tiles(d);
+  return al256((size_t)nt * kCountStride * 4) + al256((size_t)(nt + 1) * 4) +
+         2 * al256((size_t)np * 8 * 4);
+}
+
+size_t star_scratch_total_bytes(const Dims& d, long long np, int real_bytes) {
+  return star_scratch_bytes(d, np) + al256((size_t)np * kCvReals * real_bytes) +
+         al256((size_t)np * 8 * sizeof(unsigned long long));
+}
+
+StarBins carve_star_scratch(const Dims& d, long long np, void* scratch, int real_bytes,
+                            int cols) {
+  const int nt = max_tiles(d);
   StarBins sb;
   char* c = (char*)scratch;
+  const size_t cb = al256((size_t)nt * kCountStride * 4);
   sb.count = (int*)c;
-  sb.offset = (int*)(c + al((size_t)t.ntiles * 4));
-  sb.sorted = (int*)(c + al((size_t)t.ntiles * 4) + al((size_t)(t.ntiles + 1) * 4));
-  sb.rank = (int*)((char*)sb.sorted + al((size_t)np * 8 * 4));
-  sb.ntiles = t.ntiles;
+  sb.offset = (int*)(c + cb);
+  sb.sorted = (int*)(c + cb + al256((size_t)(nt + 1) * 4));
+  sb.rank = (int*)((char*)sb.sorted + al256((size_t)np * 8 * 4));
+  sb.cv = real_bytes ? (void*)(c + star_scratch_bytes(d, np)) : nullptr;
+  sb.desc = (real_bytes && cols)
+                ? (unsigned long long*)((char*)sb.cv + al256((size_t)np * kCvReals * real_bytes))
+                : nullptr;
+  sb.tg = cols ? make_column_tiles(d, cols) : make_tiles(d);
+  sb.ntiles = sb.tg.ntiles;
   return sb;
 }
 
 template <class T>
 hipError_t launch_star_step_binned(const Dims& d, double* pos, double* vel, long long np,
                                    const void* A3, double h_owed, double h1, double dt, int move,
-                                   const StarBins& sb, hipStream_t s) {
+                                   const StarBins& sb, hipStream_t s, const void* cv) {
   if (np <= 0) return hipSuccess;
-  const TileGeom tg = make_tiles(d);
+  const TileGeom tg = sb.tg;
   if (move)
     launch<k_star_step_bin<T, true>>(pblocks(np), 256, s, make_geom(d), tg, pos, vel, np,
-                                     (const T*)A3, h_owed, h1, dt, sb.count, sb.rank);
+                                     (const T*)A3, h_owed, h1, dt, sb.count, sb.rank,
+                                     (const T*)cv);
   else
     launch<k_star_step_bin<T, false>>(pblocks(np), 256, s, make_geom(d), tg, pos, vel, np,
-                                      (const T*)A3, h_owed, h1, dt, sb.count, sb.rank);
+                                      (const T*)A3, h_owed, h1, dt, sb.count, sb.rank,
+                                      (const T*)nullptr);
   return hipGetLastError();
 }
 
@@ -885,19 +982,20 @@ template <class T>
 hipError_t launch_tile_deposit(const Dims& d, const double* pos, const double* mass,
                                double m_scalar, long long np, const StarBins& sb, void* rho,
                                hipStream_t s, int det) {
-  const TileGeom tg = make_tiles(d);
+  const TileGeom tg = sb.tg;
   launch<k_bin_scan>(1, 1024, s, sb.count, sb.offset, tg.ntiles);
   if (np > 0)
     launch<k_bin_scatter>(pblocks(np), 256, s, make_geom(d), tg, pos, np, (const int*)sb.rank,
-                          (const int*)sb.offset, sb.sorted);
+                          (const int*)sb.offset, sb.sorted, sb.desc);
   if (det)
-    QJ_TILE_VOL(tg, launch<k_tile_deposit_det<T, VOL_>>(tg.ntiles, 256, s, make_geom(d), tg, pos,
+    QJ_TILE_VOL_COL(tg, launch<k_tile_deposit_det<T, VOL_>>(tg.ntiles, 256, s, make_geom(d), tg,
+                                                            pos, mass, m_scalar,
+                                                            (const int*)sb.offset,
+                                                            (const int*)sb.sorted, (T*)rho));
+  else
+    QJ_TILE_VOL_COL(tg, launch<k_tile_deposit<T, VOL_>>(tg.ntiles, 256, s, make_geom(d), tg, pos,
                                                         mass, m_scalar, (const int*)sb.offset,
                                                         (const int*)sb.sorted, (T*)rho));
-  else
-    QJ_TILE_VOL(tg, launch<k_tile_deposit<T, VOL_>>(tg.ntiles, 256, s, make_geom(d), tg, pos, mass,
-                                                    m_scalar, (const int*)sb.offset,
-                                                    (const int*)sb.sorted, (T*)rho));
   return hipGetLastError();
 }
 
@@ -932,7 +1030,8 @@ hipError_t launch_interp_adj_binned(const Dims& d, const double* pos, long long 
                                     const void* A3, double h, double* vbar, double dt_xv,
                                     double* xbar, void* abar3, const StarBins& sb, bool taped,
                                     hipStream_t s) {
-  const TileGeom tg = make_tiles(d);
+  const TileGeom tg = sb.tg;
+  if (tg.lx + tg.ly + tg.lz > 12) return hipErrorInvalidValue;  // the deposit's own tiles only
   if (np > 0)
     launch<k_interp_adj<T, true>>(pblocks(np), 256, s, make_geom(d), pos, np, (const T*)A3, h,
                                   vbar, dt_xv, xbar, (T*)nullptr, tg,
@@ -941,7 +1040,8 @@ hipError_t launch_interp_adj_binned(const Dims& d, const double* pos, long long 
     launch<k_bin_scan>(1, 1024, s, sb.count, sb.offset, tg.ntiles);
     if (np > 0)
       launch<k_bin_scatter>(pblocks(np), 256, s, make_geom(d), tg, pos, np,
-                            (const int*)sb.rank, (const int*)sb.offset, sb.sorted);
+                            (const int*)sb.rank, (const int*)sb.offset, sb.sorted,
+                            (unsigned long long*)nullptr);
   }
   QJ_TILE_VOL(tg, launch<k_tile_deposit_vec<T, VOL_>>(tg.ntiles, 256, s, make_geom(d), tg, pos,
                                                       (const double*)vbar, h,
@@ -961,14 +1061,14 @@ hipError_t launch_interp_adj_binned(const Dims& d, const double* pos, long long 
                                           double, double, double, const double*, double, void*,   \
                                           hipStream_t);                                            \
   template hipError_t launch_star_kick<T>(const Dims&, const double*, double*, long long,          \
-                                          const void*, double, double*, hipStream_t);            \
+                                          const void*, double, double*, hipStream_t, const void*);\
   template hipError_t launch_interp_adj<T>(const Dims&, const double*, long long, const void*,     \
                                            double, double*, double, double*, void*, hipStream_t); \
   template hipError_t launch_deposit_adj<T>(const Dims&, const double*, const double*, double,     \
                                             long long, const void*, double*, double*, hipStream_t); \
   template hipError_t launch_star_step_binned<T>(const Dims&, double*, double*, long long,          \
                                                  const void*, double, double, double, int,         \
-                                                 const StarBins&, hipStream_t);                    \
+                                                 const StarBins&, hipStream_t, const void*);       \
   template hipError_t launch_tile_deposit<T>(const Dims&, const double*, const double*, double,    \
                                              long long, const StarBins&, void*, hipStream_t, int);\
   template hipError_t launch_interp_adj_binned<T>(const Dims&, const double*, long long,           \

@@ -13,6 +13,7 @@
 //            y (FFT) -> z (FFT, -4 pi G/k^2, IFFT; dV/dz spectrum) ->
 //            y (IFFT; dV/dy spectrum) -> x (Hermitian-completed IFFT to
 //            V, a_x, a_y, a_z and max |a|).
+#include "qj_cic.h"
 #include "qj_fft.h"
 #include "qj_kernels.h"
 
@@ -749,19 +750,34 @@ struct SigArgs {
   double* part;
 };
 
-template <class T, int N, bool GRAD, bool WIDE = true, bool SIG = false>
+// CVG (stars steps): instead of writing A3, the pass hands each star
+// registered in its column tile the accelerations at its CIC corners (the
+// StarGather contract): a star needs 8 of the grid's cells, 1 in 64 at one
+// star per 64 cells, and the star step then reads 96 contiguous bytes per
+// star instead of gathering 128-byte lines of A3 at random.
+template <class T>
+struct GatherArgs {
+  const int* offset;
+  const int* sorted;
+  const unsigned long long* desc;
+  T* cv;
+};
+
+template <class T, int N, bool GRAD, bool WIDE = true, bool SIG = false, bool CVG = false>
 struct k_poisson_x {
   using Cfg = StridedCfg<T, N, WIDE>;
   static constexpr int kThreads = Cfg::TH;
   // two 512-thread workgroups per CU (LDS) need four waves per SIMD
   static constexpr int kMinWaves = (Cfg::kWide && GRAD) ? 4 : 0;
+  static_assert(!(SIG && CVG) && (GRAD || !CVG), "the gather serves the stars gradient pass");
   static __device__ __forceinline__ void run(const cpx<T>* __restrict__ H,
                                              const cpx<T>* __restrict__ Uy,
                                              const cpx<T>* __restrict__ Uzz, PassGeom g,
                                              const cpx<T>* __restrict__ tw, double kfx,
                                              T* __restrict__ V, T* __restrict__ A3,
                                              const T* __restrict__ addend,
-                                             double* __restrict__ diag, SigArgs<T> sg) {
+                                             double* __restrict__ diag, SigArgs<T> sg,
+                                             GatherArgs<T> ga) {
     using S = PencilShape<N, Cfg::TH>;
     constexpr int VW = Cfg::VW;
     constexpr int E = S::E;
@@ -845,10 +861,12 @@ struct k_poisson_x {
   #pragma unroll
           for (int w = 0; w < VW; ++w) {
             const T a0 = -u[w][m].r, a1 = -u[w][m].i, a2 = v[w][m].i;
-            T* o = A3 + kAccComp * (idx + w);
-            o[0] = a0;
-            o[1] = a1;
-            o[2] = a2;
+            if constexpr (!CVG) {
+              T* o = A3 + kAccComp * (idx + w);
+              o[0] = a0;
+              o[1] = a1;
+              o[2] = a2;
+            }
             const T m0 = fabs(a0), m1 = fabs(a1), m2 = fabs(a2);
             amax = (m0 > amax || m0 != m0) ? m0 : amax;
             amax = (m1 > amax || m1 != m1) ? m1 : amax;
@@ -875,6 +893,63 @@ struct k_poisson_x {
   #pragma unroll
         for (int w = 1; w < kThreads / 64; ++w) am = (wmax[w] > am || wmax[w] != wmax[w]) ? wmax[w] : am;
         atomic_max_nonneg(&diag[kDiagAMax], am);
+      }
+      if constexpr (CVG) {
+        // The transform exchange buffer is free now: the workgroup's
+        // accelerations are staged through it half a pencil (x < N/2, then
+        // x >= N/2) at a time, cell (x, column c) at 3 (x' C V + c), and each
+        // thread serves whole registrations of the tile.
+        constexpr int CV = S::C * VW;  // columns of the workgroup
+        constexpr int HX = N / 2;
+        // short pencils (one thread per pencil) have no exchange buffer
+        constexpr bool kOwn = HX * CV * 3 > 2 * S::kLdsStrided * VW;
+        __shared__ T own_buf[kOwn ? HX * CV * 3 : 1];
+        T* la = kOwn ? own_buf : reinterpret_cast<T*>(lds);
+        const int b0 = ga.offset[blockIdx.x], b1 = ga.offset[blockIdx.x + 1];
+        // the thread's first registration is loaded before the staging
+        const int i0 = b0 + (int)threadIdx.x;
+        int p0 = 0;
+        unsigned long long d0 = 0;
+        if (i0 < b1) {
+          p0 = ga.sorted[i0];
+          d0 = ga.desc[i0];
+        }
+  #pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          __syncthreads();  // the last readers of the buffer are done
+          if (act) {
+  #pragma unroll
+            for (int mm = 0; mm < E / 2; ++mm) {
+              const int m = h * (E / 2) + mm;
+              const int xl = j + TT * mm;  // x - h HX
+  #pragma unroll
+              for (int w = 0; w < VW; ++w) {
+                T* o = la + 3 * (xl * CV + col * VW + w);
+                o[0] = -u[w][m].r;
+                o[1] = -u[w][m].i;
+                o[2] = v[w][m].i;
+              }
+            }
+          }
+          __syncthreads();
+          for (int i = i0; i < b1; i += kThreads) {
+            const int p = (i == i0) ? p0 : ga.sorted[i];
+            const unsigned long long d = (i == i0) ? d0 : ga.desc[i];
+            T* out = ga.cv + kCvReals * (long long)p;
+  #pragma unroll
+            for (int k = 0; k < 8; ++k) {
+              const int sx = kCorner[k][0], sy = kCorner[k][1], sz = kCorner[k][2];
+              const int xl = (int)((d >> (9 * sx)) & 511u) - h * HX;
+              const int f = (int)((d >> (18 + 9 * (2 * sy + sz))) & 511u);  // column + 1
+              if ((unsigned)xl < (unsigned)HX && f != 0) {
+                const T* q = la + 3 * (xl * CV + f - 1);
+                out[3 * k] = q[0];
+                out[3 * k + 1] = q[1];
+                out[3 * k + 2] = q[2];
+              }
+            }
+          }
+        }
       }
       if constexpr (SIG) {
   #pragma unroll
@@ -1245,6 +1320,31 @@ int poisson_x_blocks(const Dims& d) {
   return a > b ? a : b;
 }
 
+namespace {
+template <class T>
+int gather_columns_t(const Dims& d, bool wide) {
+  int cv = 0;
+  auto take = [&](int c, int vw) { cv = c * vw; };
+  switch (d.nx) {
+#define QJ_GC(n)                                                                         \
+  case n:                                                                                \
+    if (wide) take(StridedCfg<T, n, true>::C, StridedCfg<T, n, true>::VW);               \
+    else take(StridedCfg<T, n, false>::C, StridedCfg<T, n, false>::VW);                  \
+    break;
+    QJ_GC(8) QJ_GC(16) QJ_GC(32) QJ_GC(64) QJ_GC(128) QJ_GC(256) QJ_GC(512)
+#undef QJ_GC
+    default: return 0;
+  }
+  const long long ncols = (long long)d.ny * d.nz;
+  const long long vol = (long long)d.nx * (cv < ncols ? cv : ncols);
+  return vol <= kTileColMaxCells ? cv : 0;
+}
+}  // namespace
+
+int poisson_x_gather_columns(const Dims& d, int c64, int wide) {
+  return c64 ? gather_columns_t<float>(d, wide != 0) : gather_columns_t<double>(d, wide != 0);
+}
+
 int source_x_blocks(const Dims& d) {
   // partials are sized for the larger of the two precisions' grids
   const int a = source_x_blocks_t<float>(d), b = source_x_blocks_t<double>(d);
@@ -1335,15 +1435,21 @@ template <class T>
 hipError_t launch_poisson_x(const Dims& d, const SpecTables<T>& tb, int grad, const void* H,
                             const void* Uy, const void* Uzz, void* V, void* A3,
                             const void* addend, double* diag, hipStream_t s,
-                            GasSignalFold* sig) {
+                            GasSignalFold* sig, const StarGather* gather) {
   PassGeom g = geom_x(d);
   const cpx<T>* tw = (const cpx<T>*)tb.tw_x;
   const double kfx = 2.0 * M_PI / d.Lx;
   if (sig && (!grad || !sig->v[0] || !sig->v[1] || !sig->v[2] || !sig->owed || !sig->part))
     return hipErrorInvalidValue;
+  if (gather && (!grad || sig || !gather->offset || !gather->sorted || !gather->desc ||
+                 !gather->cv ||
+                 poisson_x_gather_columns(d, sizeof(T) == 4, tb.wide) == 0))
+    return hipErrorInvalidValue;
   SigArgs<T> sa{};
   if (sig) sa = SigArgs<T>{(const T*)sig->v[0], (const T*)sig->v[1], (const T*)sig->v[2],
                            sig->owed, sig->cs, sig->part};
+  GatherArgs<T> ga{};
+  if (gather) ga = GatherArgs<T>{gather->offset, gather->sorted, gather->desc, (T*)gather->cv};
   QJ_DISPATCH_N(d.nx, {
     using Cfg = StridedCfg<T, N_, true>;
     using Nar = StridedCfg<T, N_, false>;
@@ -1356,23 +1462,31 @@ hipError_t launch_poisson_x(const Dims& d, const SpecTables<T>& tb, int grad, co
     if (narrow && sig)
       launch<k_poisson_x<T, N_, true, false, true>>(Nar::blocks(g.ncols), Nar::TH, s,
           (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A3,
-          (const T*)addend, diag, sa);
+          (const T*)addend, diag, sa, ga);
+    else if (narrow && gather)
+      launch<k_poisson_x<T, N_, true, false, false, true>>(Nar::blocks(g.ncols), Nar::TH, s,
+          (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A3,
+          (const T*)addend, diag, sa, ga);
     else if (narrow)
       launch<k_poisson_x<T, N_, true, false>>(Nar::blocks(g.ncols), Nar::TH, s,
           (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A3,
-          (const T*)addend, diag, sa);
+          (const T*)addend, diag, sa, ga);
     else if (grad && sig)
       launch<k_poisson_x<T, N_, true, true, true>>(nb, Cfg::TH, s,
           (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A3,
-          (const T*)addend, diag, sa);
+          (const T*)addend, diag, sa, ga);
+    else if (grad && gather)
+      launch<k_poisson_x<T, N_, true, true, false, true>>(nb, Cfg::TH, s,
+          (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A3,
+          (const T*)addend, diag, sa, ga);
     else if (grad)
       launch<k_poisson_x<T, N_, true>>(nb, Cfg::TH, s,
           (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A3,
-          (const T*)addend, diag, sa);
+          (const T*)addend, diag, sa, ga);
     else
       launch<k_poisson_x<T, N_, false>>(nb, Cfg::TH, s,
           (const cpx<T>*)H, (const cpx<T>*)Uy, (const cpx<T>*)Uzz, g, tw, kfx, (T*)V, (T*)A3,
-          (const T*)addend, diag, sa);
+          (const T*)addend, diag, sa, ga);
   });
   return hipGetLastError();
 }
@@ -1494,7 +1608,8 @@ hipError_t launch_adj_poisson(const Dims& d, const SpecTables<T>& tb, const void
                                           void*, hipStream_t);                                  \
   template hipError_t launch_poisson_x<T>(const Dims&, const SpecTables<T>&, int, const void*,  \
                                           const void*, const void*, void*, void*, const void*,  \
-                                          double*, hipStream_t, GasSignalFold*);
+                                          double*, hipStream_t, GasSignalFold*,                 \
+                                          const StarGather*);
 
 #define QJ_INST_ADJ(T)                                                                          \
   template hipError_t launch_kick_adj<T>(void*, const void*, const void*, const void*, double,   \

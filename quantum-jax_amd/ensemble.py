@@ -142,7 +142,10 @@ class Ensemble:
     def _snapshot(self):
         snap = []
         for s in self.sims:
-            t = {k: getattr(s, k) for k in ("_psi", "pos", "vel", "rho_s", "grid", "diag", "brho")}
+            # star_scratch: after stars steps with the fused star gather the
+            # stars' corner accelerations live there instead of in grid
+            t = {k: getattr(s, k) for k in ("_psi", "pos", "vel", "rho_s", "grid", "diag", "brho",
+                                            "star_scratch")}
             t = {k: (v.clone() if v is not None else None) for k, v in t.items()}
             if s.gas is not None:
                 t["gas"] = [g.clone() for g in s.gas]
@@ -180,6 +183,7 @@ class Ensemble:
     # walking workgroups are a tile schedule of every group.
     KNOBS = (("fuse_x_source", _lib.OPT_FUSE_X_SOURCE, lambda key: True, (0, 1)),
              ("binned_deposit", _lib.OPT_BINNED_DEPOSIT, lambda key: key[2] == "stars", (0, 1)),
+             ("star_gather", _lib.OPT_STAR_GATHER, lambda key: key[2] == "stars", (1, 0)),
              ("wide_sweeps", _lib.OPT_WIDE_SWEEPS,
               lambda key: key[1] == "complex64" and 256 in key[0][:2], (0, 1)),
              ("z_sweep_blocks", _lib.OPT_Z_SWEEP_BLOCKS, lambda key: True, (0, 256, 1024, 2048)))

@@ -198,7 +198,16 @@ class KDKSimulation:
     @property
     def a_grid(self) -> torch.Tensor:
         """The reference's a_grid of the current state (stars/gas), (nx,ny,nz,3)
-        stacked on the last axis."""
+        stacked on the last axis.
+
+        After stars steps with the fused star gather (_lib.OPT_STAR_GATHER,
+        the default) the stepper keeps only the accelerations at the stars'
+        corners, so the grid is computed here from the state's density
+        (|psi|^2 is the same before and after the owed kick) into a new tensor."""
+        if self._st.grid_valid == 2:
+            from .operators import acceleration_grid
+            rho = self._psi.abs().pow(2).to(self.plan.real_dtype) + self.rho_s
+            return acceleration_grid(rho, self.plan, self.phys)[1]
         return self.grid[self.plan.ncell:].view(self.plan.shape + (_lib.ACC_COMPONENTS,))
 
     def a_max(self) -> float:

@@ -47,8 +47,12 @@ def make(qj, kind, g, side):
 def snapshot(sim):
     # the unfinalized buffers: finalizing would apply the owed kick on the
     # host's side of the capture, which the captured steps do not know of
-    # (plain steps write V only; the acceleration part of their grid is unused)
-    out = [sim._psi.clone(), (sim.grid if (sim.np or sim.gas is not None) else sim.V).clone()]
+    # (plain steps write V only; the acceleration part of their grid is unused,
+    # and after stars steps with the fused star gather a_grid is computed from
+    # the state)
+    out = [sim._psi.clone(), sim.V.clone()]
+    if sim.np or sim.gas is not None:
+        out.append(sim.a_grid.clone())
     if sim.np:
         out += [sim.pos.clone(), sim.vel.clone(), sim.rho_s.clone()]
     if sim.gas is not None:

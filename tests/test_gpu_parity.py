@@ -508,8 +508,9 @@ def test_sweep_shape_options_agree(qj, cuda, kind):
         sim.step(3)
         st = sim.state()
         # plain steps write V only; star steps also the acceleration grid
-        out[opts] = [st[k].clone() for k in ("psi", "pos", "vel") if k in st] + [
-            sim.grid.clone() if kind == "stars" else sim.V.clone()]
+        out[opts] = [st[k].clone() for k in ("psi", "pos", "vel") if k in st] + [sim.V.clone()]
+        if kind == "stars":
+            out[opts].append(sim.a_grid.clone())
         del sim, plan
     base = out[(1, 0)]
     for opts, res in out.items():
@@ -556,7 +557,7 @@ def test_side_stream_option_agrees(qj, cuda, kind, n):
                 sim.step_cfl(4, cfl=0.4)
         st = sim.state()
         out[side] = [v.clone() for k, v in sorted(st.items()) if isinstance(v, torch.Tensor)]
-        out[side].append(sim.grid.clone())
+        out[side] += [sim.V.clone(), sim.a_grid.clone()]
         if kind == "gas_cfl":
             out[side].append(torch.tensor(sim.dts))
         del sim, plan

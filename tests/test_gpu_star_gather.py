@@ -1,0 +1,143 @@
+"""The fused star gather (QJ_OPT_STAR_GATHER): the Poisson x pass hands each
+star its eight corner accelerations instead of writing the acceleration
+grid.  The stars then kick with the same corner values, so stars steps with
+the option on agree with the FP64 oracle and with the option off to FP
+rounding (the two Poisson x instantiations may round a transform's last bit
+differently); the grid-valid state moves between the two forms in either
+direction; a_grid of a gathered state is computed from the state.
+
+quantum-stars.py:250-288 (get_acceleration: grid, then the CIC gather) and
+:295-330 (the step whose kicks use it).
+"""
+
+import importlib
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+C64, C128 = torch.complex64, torch.complex128
+TOLS = {C64: 2e-5, C128: 1e-11}
+
+
+def rel(a, b):
+    a = a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
+    b = b.detach().cpu().numpy() if isinstance(b, torch.Tensor) else np.asarray(b)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def dev(x, dtype):
+    return torch.as_tensor(np.ascontiguousarray(x)).to("cuda", dtype)
+
+
+def _lib():
+    return importlib.import_module("quantum-jax_amd._lib")
+
+
+# 8^3 and 16^3: one thread per x pencil (the gather's own LDS buffer); 64 x
+# 16 x 32: column tiles spanning several y rows; 256^3 c64: the wide Poisson
+# x form with 8192-cell column tiles; 512 x 16 x 16: 512-point pencils
+GRIDS = [O.Grid(8, 8, 8), O.Grid(16, 16, 16), O.Grid(64, 16, 32, 1.0, 0.5, 2.0),
+         O.Grid(32, 32, 32), O.Grid(512, 16, 16, 2.0, 1.0, 1.0)]
+
+
+def _sim(qj, g, dtype, n_p, gather, seed=3, det=0, wide=1):
+    plan = qj.Plan(g.nx, g.ny, g.nz, L=(g.Lx, g.Ly, g.Lz), dtype=dtype)
+    plan.set_option(_lib().OPT_STAR_GATHER, gather)
+    plan.set_option(_lib().OPT_DETERMINISTIC_DEPOSIT, det)
+    plan.set_option(_lib().OPT_WIDE_SWEEPS, wide)
+    psi = O.filtered_gaussian_psi(g, seed, sigma_k=6 * np.pi)
+    pos, vel, m_s = O.star_ic(g, n_p, seed + 1000, sigma_v=0.2)
+    sim = qj.KDKSimulation(plan, qj.Physics(rho_bar=1.1, four_pi_G=20.0), 1e-3, dev(psi, dtype),
+                           dev(pos, torch.float64), dev(vel, torch.float64), mass=m_s)
+    return sim, (psi, pos, vel, m_s)
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+@pytest.mark.parametrize("g", GRIDS, ids=lambda g: f"{g.nx}x{g.ny}x{g.nz}")
+def test_gathered_steps_match_oracle_and_grid_steps(qj, cuda, dtype, g):
+    n_p = 3000
+    res = {}
+    for gather in (1, 0):
+        sim, ic = _sim(qj, g, dtype, n_p, gather)
+        sim.step(3)
+        assert sim._st.grid_valid == (2 if gather else 1)
+        st = sim.state()
+        res[gather] = (st["psi"].clone(), st["pos"].clone(), st["vel"].clone(), st["a_max"],
+                       st["dt_s"])
+    psi, pos, vel, m_s = ic
+    s = (psi, pos, vel, 0.0, 0.0, np.inf)
+    for _ in range(3):
+        s = O.compute_step_stars(*s, 1e-3, m_s, g, O.Physics(rho_bar=1.1, four_pi_G=20.0))
+    for k, ref in ((0, s[0]), (1, s[1]), (2, s[2])):
+        assert rel(res[1][k], ref) < TOLS[dtype], k
+        assert rel(res[1][k], res[0][k]) < TOLS[dtype] / 10, k
+    assert abs(res[1][3] - s[4]) < 1e-5 * s[4]
+    assert abs(res[1][4] - s[5]) < 1e-5 * s[5]
+
+
+@pytest.mark.parametrize("wide", [1, 0])
+def test_gathered_steps_at_256(qj, cuda, wide):
+    # 8192-cell column tiles, 2048 Poisson x workgroups; the deterministic
+    # deposit, so both runs see the same densities up to the transforms'
+    # rounding
+    g = O.Grid(256, 256, 256)
+    res = {}
+    for gather in (1, 0):
+        sim, _ = _sim(qj, g, C64, 1 << 16, gather, det=1, wide=wide)
+        sim.step(2)
+        st = sim.state()
+        res[gather] = (st["psi"], st["pos"], st["vel"], sim.rho_s)
+    for a, b in zip(res[1], res[0]):
+        assert rel(a, b) < 1e-6
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+def test_option_switches_between_steps(qj, cuda, dtype):
+    # on -> off -> on: the state kept only corner values (grid_valid 2), then
+    # a step reads them and writes the grid again (1), then gathers again
+    g = O.Grid(32, 16, 64, 1.0, 0.5, 2.0)
+    sims = []
+    for sched in ((1, 0, 1), (0, 0, 0)):
+        sim, _ = _sim(qj, g, dtype, 2000, sched[0])
+        for gth in sched:
+            sim.plan.set_option(_lib().OPT_STAR_GATHER, gth)
+            sim.step(2)
+            assert sim._st.grid_valid == (2 if gth else 1)
+        sims.append(sim)
+    a, b = (s.state() for s in sims)
+    for k in ("psi", "pos", "vel"):
+        assert rel(a[k], b[k]) < TOLS[dtype] / 10, k
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+def test_a_grid_of_gathered_state(qj, cuda, dtype):
+    g = O.Grid(32, 32, 32)
+    out = []
+    for gather in (1, 0):
+        sim, _ = _sim(qj, g, dtype, 1500, gather)
+        sim.step(2)
+        out.append((sim.V.clone(), sim.a_grid.clone()))
+    assert rel(out[0][0], out[1][0]) < TOLS[dtype] / 10
+    assert rel(out[0][1], out[1][1]) < TOLS[dtype] / 10
+
+
+def test_atomic_deposit_after_gathered_steps(qj, cuda):
+    # the float-atomic star step reads the grid: a gathered state has its grid
+    # computed again before such a step
+    g = O.Grid(32, 32, 32)
+    res = []
+    for sched in ((1, 0), (0, 0)):
+        sim, _ = _sim(qj, g, C128, 1500, sched[0])
+        sim.step(2)
+        sim.plan.set_option(_lib().OPT_BINNED_DEPOSIT, 0)
+        sim.plan.set_option(_lib().OPT_STAR_GATHER, sched[1])
+        sim.step(2)
+        assert sim._st.grid_valid == 1
+        res.append(sim.state())
+    for k in ("psi", "pos", "vel"):
+        assert rel(res[0][k], res[1][k]) < 1e-12, k
