@@ -66,15 +66,21 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def stage_bytes(name, nx, ny, nz, c, r, n_p, stars):
+def stage_bytes(name, nx, ny, nz, c, r, n_p, stars, gather=True):
     """Algorithmic HBM bytes of one launch of a stage (read + write once).
 
     Stars use the tile-binned deposit: it overwrites every cell of rho_s
     (N r), so the source pass only reads rho_s; its other bytes are the
-    star positions and registrations (about 64 B per star)."""
+    star positions and registrations (about 64 B per star on the deposit's
+    own tiles).  With the fused star gather (the default for stars steps)
+    the Poisson x pass writes V and, instead of the acceleration grid, each
+    star's eight corner accelerations (24 r B), reading its two column-tile
+    registrations (index and corner descriptor, 12 B each); the deposit's
+    scatter writes those descriptors (about 112 B per star in all)."""
     N = nx * ny * nz
     Nh = (nx // 2 + 1) * ny * nz
     src = r
+    fused = stars and gather and n_p > 0
     table = {
         "drift_z_kick": N * (2 * c + r),
         "drift_z": 2 * N * c, "drift_y": 2 * N * c, "drift_x": 2 * N * c,
@@ -83,8 +89,9 @@ def stage_bytes(name, nx, ny, nz, c, r, n_p, stars):
         "fft_y_half": 2 * Nh * c,
         "poisson_z": 3 * Nh * c,
         "poisson_y": 5 * Nh * c,
-        "poisson_x": 3 * Nh * c + 4 * N * r,      # V plane + (a_x, a_y, a_z) cells
-        "deposit": N * r + n_p * 64,
+        # V plane + (a_x, a_y, a_z) cells, or V + the stars' corner values
+        "poisson_x": 3 * Nh * c + (N * r + n_p * (24 * r + 24) if fused else 4 * N * r),
+        "deposit": N * r + n_p * (112 if fused else 64),
         "star_step": n_p * (96 + 8 * 3 * r + 8),
         "star_kick": n_p * (72 + 8 * 3 * r),
         "kick": N * (2 * c + r),
@@ -355,7 +362,8 @@ def main():
     for k in range(nst):
         if cnt_arr[k]:
             avg = ms_arr[k] / cnt_arr[k]
-            b = stage_bytes(names[k], n, n, n, c_b, r_b, n_p, True)
+            b = stage_bytes(names[k], n, n, n, c_b, r_b, n_p, True,
+                            bool(plan.get_option(lib_mod.OPT_STAR_GATHER)))
             stages[names[k]] = {
                 "launches_per_step": round(cnt_arr[k] / tsteps, 3),
                 "avg_ms": round(avg, 5),
