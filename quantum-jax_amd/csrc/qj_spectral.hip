@@ -795,6 +795,19 @@ struct k_poisson_x {
       else
         load_herm_rows<N, VW, E>(x, a, base, se, j, TT, act);
     };
+    // the fused star gather: the thread's first registration of the tile is
+    // loaded before the spectra, so its latency hides behind the transforms
+    int gb0 = 0, gb1 = 0, gp0 = 0;
+    unsigned long long gd0 = 0;
+    if constexpr (CVG) {
+      gb0 = ga.offset[blockIdx.x];
+      gb1 = ga.offset[blockIdx.x + 1];
+      const int i0 = gb0 + (int)threadIdx.x;
+      if (i0 < gb1) {
+        gp0 = ga.sorted[i0];
+        gd0 = ga.desc[i0];
+      }
+    }
     cpx<T> v[VW][E];
     load_herm(v, H);
     if constexpr (!GRAD) {
@@ -905,15 +918,9 @@ struct k_poisson_x {
         constexpr bool kOwn = HX * CV * 3 > 2 * S::kLdsStrided * VW;
         __shared__ T own_buf[kOwn ? HX * CV * 3 : 1];
         T* la = kOwn ? own_buf : reinterpret_cast<T*>(lds);
-        const int b0 = ga.offset[blockIdx.x], b1 = ga.offset[blockIdx.x + 1];
-        // the thread's first registration is loaded before the staging
-        const int i0 = b0 + (int)threadIdx.x;
-        int p0 = 0;
-        unsigned long long d0 = 0;
-        if (i0 < b1) {
-          p0 = ga.sorted[i0];
-          d0 = ga.desc[i0];
-        }
+        const int b1 = gb1, i0 = gb0 + (int)threadIdx.x;
+        const int p0 = gp0;
+        const unsigned long long d0 = gd0;
   #pragma unroll
         for (int h = 0; h < 2; ++h) {
           __syncthreads();  // the last readers of the buffer are done
