@@ -169,6 +169,9 @@ struct XState {
 template <class T, int FLUX>
 struct k_hydro_stream {
   static constexpr int kThreads = HTile<T>::threads;
+  // FP32: 29.8 KB of LDS fits five 192-thread workgroups per CU, which four
+  // waves per SIMD (at most 128 registers) let run
+  static constexpr int kMinWaves = sizeof(T) == 4 ? 4 : 0;
   static __device__ __forceinline__ void run(HGeom<T> g, int TY, int TZ, int XC,
                                              const T* __restrict__ rho, const T* __restrict__ vx,
                                              const T* __restrict__ vy, const T* __restrict__ vz,
@@ -182,8 +185,8 @@ struct k_hydro_stream {
     }
     __shared__ T raw[4][4][HTile<T>::RY * HTile<T>::RZ];  // [slot][var][cell]
     __shared__ T st[12][HTile<T>::SY * HTile<T>::SZ];  // [p0..3, sy0..3, sz0..3][cell]
-    // +y and +z face fluxes of a plane, double-buffered: [buf][y, z][var][cell]
-    __shared__ T fyz[2][2][4][HTile<T>::SY * HTile<T>::SZ];
+    // +y and +z face fluxes of a plane: [y, z][var][cell]
+    __shared__ T fyz[2][4][HTile<T>::SY * HTile<T>::SZ];
     const int tid = threadIdx.x;
     const int nthr = TY * TZ;
     const int RZ = TZ + 4, RY = TY + 4, SZ = TZ + 2, SY = TY + 2;
@@ -324,42 +327,46 @@ struct k_hydro_stream {
       }
     };
     // Each y and z face of a plane evaluated once: a tile cell takes its +y
-    // and +z faces (kept in Fy, Fz and stored), the ring row below the tile
-    // its +y faces and the ring column left of it its +z faces (the tile's
-    // -y and -z faces), all stored to fyz[x & 1] for the update after the
-    // plane's next barrier.  The states of plane x stay in st until the
-    // states of plane x + 1 overwrite them after that barrier.
-    auto faces = [&](int x, T Fy[4], T Fz[4]) {
-      const int b = x & 1;
+    // and +z faces (kept in Fy, Fz), the ring row below the tile its +y faces
+    // (in Fy) and the ring column left of it its +z faces (in Fz), the
+    // tile's -y and -z faces.  faces() only computes them; store_faces()
+    // puts them into fyz after the plane's first barrier, which every update
+    // of the plane before (the last readers of fyz) has passed, so a single
+    // buffer serves (and the workgroup's LDS fits five workgroups per CU).
+    // oy, oz: the cells of fyz they go to (-1: none).
+    auto faces = [&](T Fy[4], T Fz[4], int& oy, int& oz) {
       const int c = tid;
+      oy = oz = -1;
       if (c < nthr) {
         const int o = (ty + 1) * SZ + (tz + 1);
         face(1, o, Fy);
         face(2, o, Fz);
-  #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          fyz[b][0][k][o] = Fy[k];
-          fyz[b][1][k][o] = Fz[k];
-        }
+        oy = oz = o;
         return;
       }
       const int r = c - nthr;
       if (r < SZ) {  // ring row yy = -1: its +y face is the tile's -y face
         const int zz = r - 1;
         if (zz < 0 || zz >= TZ) return;
-        T F[4];
-        face(1, zz + 1, F);
-  #pragma unroll
-        for (int k = 0; k < 4; ++k) fyz[b][0][k][zz + 1] = F[k];
+        face(1, zz + 1, Fy);
+        oy = zz + 1;
         return;
       }
       const int q = r - 2 * SZ;
       if (q < 0 || (q & 1) || q / 2 >= TY) return;
       const int o = (q / 2 + 1) * SZ;  // ring column zz = -1 of row q / 2
-      T F[4];
-      face(2, o, F);
+      face(2, o, Fz);
+      oz = o;
+    };
+    auto store_faces = [&](const T Fy[4], const T Fz[4], int oy, int oz) {
+      if (oy >= 0) {
   #pragma unroll
-      for (int k = 0; k < 4; ++k) fyz[b][1][k][o] = F[k];
+        for (int k = 0; k < 4; ++k) fyz[0][k][oy] = Fy[k];
+      }
+      if (oz >= 0) {
+  #pragma unroll
+        for (int k = 0; k < 4; ++k) fyz[1][k][oz] = Fz[k];
+      }
     };
 
     XState<T> xm, xc0, xp;
@@ -401,16 +408,17 @@ struct k_hydro_stream {
     auto plane = [&](int x, const XState<T>& cur, XState<T>& nxt, const T* Fm, T* Fp) {
       // the y and z face fluxes of plane x from its states
       T Fy[4], Fz[4];
-      faces(x, Fy, Fz);
+      int oy, oz;
+      faces(Fy, Fz, oy, oz);
       // plane x+2 (fetched during the previous plane) into its slot, which
       // planes x-1 .. x+1 in use leave free; then fetch plane x+3
       put(x + 2, pre);
       if (x + 3 <= x0 + XC + 1) fetch(x + 3, pre);
       __syncthreads();
+      store_faces(Fy, Fz, oy, oz);
       states(x + 1, nxt);
       __syncthreads();
       if (own) {
-        const int b = x & 1;
         const int ro = (ty + 2) * RZ + (tz + 2);
         const T r0 = raw[x & 3][0][ro];
         // quantum-fluid.py:338 get_conserved on the (kicked) state of plane x
@@ -434,12 +442,12 @@ struct k_hydro_stream {
   #pragma unroll
         for (int k = 0; k < 4; ++k) {
           U[k] += -fac * Fy[k];
-          U[k] += fac * fyz[b][0][k][ty * SZ + (tz + 1)];
+          U[k] += fac * fyz[0][k][ty * SZ + (tz + 1)];
         }
   #pragma unroll
         for (int k = 0; k < 4; ++k) {
           U[k] += -fac * Fz[k];
-          U[k] += fac * fyz[b][1][k][(ty + 1) * SZ + tz];
+          U[k] += fac * fyz[1][k][(ty + 1) * SZ + tz];
         }
         const int i = gidx(x, y0 + ty, z0 + tz);
         const T iM = rcp(U[0]);
