@@ -187,9 +187,13 @@ __device__ __forceinline__ double nan_max(double a, double b) { return (b > a ||
 // FMAs, the norm in FP64.  Every kernel that evaluates it calls this, so
 // they all give the same bits.
 template <class T>
-__device__ __forceinline__ double gas_signal(T u, T v, T w, T ax, T ay, T az, T h, double cs) {
+__device__ __forceinline__ double gas_signal_sq(T u, T v, T w, T ax, T ay, T az, T h) {
   const double du = fmad(ax, h, u), dv = fmad(ay, h, v), dw = fmad(az, h, w);
-  return cs + sqrt(__builtin_fma(dw, dw, __builtin_fma(dv, dv, du * du)));
+  return __builtin_fma(dw, dw, __builtin_fma(dv, dv, du * du));
+}
+template <class T>
+__device__ __forceinline__ double gas_signal(T u, T v, T w, T ax, T ay, T az, T h, double cs) {
+  return cs + sqrt(gas_signal_sq<T>(u, v, w, ax, ay, az, h));
 }
 template <class T>
 __device__ __forceinline__ double gas_signal(T u, T v, T w, double cs) {

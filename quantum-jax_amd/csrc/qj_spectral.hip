@@ -846,50 +846,57 @@ struct k_poisson_x {
       }
       pencil_fft_strided<N, +1, T, VW, false, Cfg::TH>(u, j, col, lds, tw);
       pencil_fft_strided<N, +1, T, VW, false, Cfg::TH>(v, j, col, lds, tw);
-      // gas velocity of the pencil's cells (loaded after the transforms:
-      // live through them it would cost the pass its second workgroup per CU)
-      T gv[SIG ? 3 : 1][SIG ? E : 1][VW];
-      if constexpr (SIG) {
-        if (act) {
-  #pragma unroll
-          for (int m = 0; m < E; ++m) {
-            const int idx = base + (j + TT * m) * se;
-            ldr<VW>(sg.vx, idx, gv[0][m]);
-            ldr<VW>(sg.vy, idx, gv[1][m]);
-            ldr<VW>(sg.vz, idx, gv[2][m]);
-          }
-        }
-      }
+      // Gas velocity of the pencil's cells, loaded after the transforms (live
+      // through them it would cost the pass its second workgroup per CU) in
+      // groups of four rows.  The signal is reduced as the largest squared
+      // speed |v + h a|^2: cs + sqrt(.) is monotonic, so cs + sqrt(max) is
+      // the max of gas_signal to the bit, with one square root per thread.
+      constexpr int MG = SIG ? 4 : 1;
       T amax = (T)0;
-      double smax = 0.0;
+      double smax2 = 0.0;
       const T h = SIG ? (T)*sg.owed : (T)0;
       if (act) {
   #pragma unroll
-        for (int m = 0; m < E; ++m) {
-          const int idx = base + (j + TT * m) * se;
-          T t[VW];
+        for (int m0 = 0; m0 < E; m0 += MG) {
+          T gv[SIG ? 3 : 1][MG][VW];
+          if constexpr (SIG) {
   #pragma unroll
-          for (int w = 0; w < VW; ++w) t[w] = v[w][m].r;
-          str<VW>(V, idx, t);
-  #pragma unroll
-          for (int w = 0; w < VW; ++w) {
-            const T a0 = -u[w][m].r, a1 = -u[w][m].i, a2 = v[w][m].i;
-            if constexpr (!CVG) {
-              T* o = A3 + kAccComp * (idx + w);
-              o[0] = a0;
-              o[1] = a1;
-              o[2] = a2;
+            for (int mm = 0; mm < MG; ++mm) {
+              const int idx = base + (j + TT * (m0 + mm)) * se;
+              ldr<VW>(sg.vx, idx, gv[0][mm]);
+              ldr<VW>(sg.vy, idx, gv[1][mm]);
+              ldr<VW>(sg.vz, idx, gv[2][mm]);
             }
-            const T m0 = fabs(a0), m1 = fabs(a1), m2 = fabs(a2);
-            amax = (m0 > amax || m0 != m0) ? m0 : amax;
-            amax = (m1 > amax || m1 != m1) ? m1 : amax;
-            amax = (m2 > amax || m2 != m2) ? m2 : amax;
-            if constexpr (SIG)
-              smax = nan_max(smax, gas_signal<T>(gv[0][m][w], gv[1][m][w], gv[2][m][w], a0, a1,
-                                                 a2, h, sg.cs));
+          }
+  #pragma unroll
+          for (int mm = 0; mm < MG; ++mm) {
+            const int m = m0 + mm;
+            const int idx = base + (j + TT * m) * se;
+            T t[VW];
+  #pragma unroll
+            for (int w = 0; w < VW; ++w) t[w] = v[w][m].r;
+            str<VW>(V, idx, t);
+  #pragma unroll
+            for (int w = 0; w < VW; ++w) {
+              const T a0 = -u[w][m].r, a1 = -u[w][m].i, a2 = v[w][m].i;
+              if constexpr (!CVG) {
+                T* o = A3 + kAccComp * (idx + w);
+                o[0] = a0;
+                o[1] = a1;
+                o[2] = a2;
+              }
+              const T f0 = fabs(a0), f1 = fabs(a1), f2 = fabs(a2);
+              amax = (f0 > amax || f0 != f0) ? f0 : amax;
+              amax = (f1 > amax || f1 != f1) ? f1 : amax;
+              amax = (f2 > amax || f2 != f2) ? f2 : amax;
+              if constexpr (SIG)
+                smax2 = nan_max(smax2, gas_signal_sq<T>(gv[0][mm][w], gv[1][mm][w],
+                                                        gv[2][mm][w], a0, a1, a2, h));
+            }
           }
         }
       }
+      double smax = SIG ? (double)sg.cs + sqrt(smax2) : 0.0;
       // block max (NaN-propagating), one atomic per block: device-scope
       // atomics on one address serialise, so one per wave would stretch the
       // kernel's tail
