@@ -167,9 +167,9 @@ def ensemble_leg(qj, dev, phys, nsteps=50):
     t0 = time.perf_counter()
     tuned = ens.autotune(reps=3)
     t_tune = time.perf_counter() - t0
-    log(f"ensemble autotune: {tuned['mode']} fuse_x_source {tuned['fuse_x_source']} "
-        f"binned_deposit {tuned.get('binned_deposit')} wide_sweeps {tuned.get('wide_sweeps')} "
-        f"z_sweep_blocks {tuned.get('z_sweep_blocks')} ({t_tune:.1f} s)")
+    knob_names = [k[0] for k in qj.Ensemble.KNOBS]
+    log(f"ensemble autotune: {tuned['mode']} " +
+        " ".join(f"{k} {tuned.get(k)}" for k in knob_names) + f" ({t_tune:.1f} s)")
     ens.step(1)
     torch.cuda.synchronize()
     a.record()
@@ -233,10 +233,8 @@ def ensemble_leg(qj, dev, phys, nsteps=50):
         "value": cells / (ms * 1e-3), "unit": "cell-updates/s", "ms_per_step": ms,
         "mode_ms_per_step": modes,
         "speedup_vs_eager": modes["eager"] / ms,
-        "autotuned": {"mode": tuned["mode"], "fuse_x_source": tuned["fuse_x_source"],
-                      "binned_deposit": tuned.get("binned_deposit"),
-                      "wide_sweeps": tuned.get("wide_sweeps"),
-                      "z_sweep_blocks": tuned.get("z_sweep_blocks"), "tuning_s": t_tune},
+        "autotuned": dict({"mode": tuned["mode"]}, **{k: tuned.get(k) for k in knob_names},
+                          tuning_s=t_tune),
         "launch_groups": groups,
         "adjoint": adj,
     }

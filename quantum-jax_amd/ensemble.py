@@ -186,7 +186,9 @@ class Ensemble:
              ("star_gather", _lib.OPT_STAR_GATHER, lambda key: key[2] == "stars", (1, 0)),
              ("wide_sweeps", _lib.OPT_WIDE_SWEEPS,
               lambda key: key[1] == "complex64" and 256 in key[0][:2], (0, 1)),
-             ("z_sweep_blocks", _lib.OPT_Z_SWEEP_BLOCKS, lambda key: True, (0, 256, 1024, 2048)))
+             # 65535: one pencil group per workgroup (no walking)
+             ("z_sweep_blocks", _lib.OPT_Z_SWEEP_BLOCKS, lambda key: True,
+              (0, 256, 1024, 2048, 65535)))
 
     def autotune(self, reps: int = 3, modes=("batched", "graph", "graph_fork"),
                  cache_path: str | None = None, verbose: bool = False) -> dict:
