@@ -210,10 +210,12 @@ struct StridedVW {
 // half the registers per lane.  A sweep that holds several spectra per
 // pencil then runs four waves per SIMD instead of two (the Poisson x pass:
 // 225 -> 123 VGPRs, 136 -> 121 us at 256^3 in isolation).
-template <class T, int N, bool WIDE>
+// WTH: workgroup size of the wide form, 512 threads (32 pencils, two rows
+// of a pencil per wave) or 256 (16 pencils, four rows per wave).
+template <class T, int N, bool WIDE, int WTH = 512>
 struct StridedCfg {
   static constexpr bool kWide = WIDE && sizeof(T) == 4 && N == 256;
-  static constexpr int TH = kWide ? 512 : 256;
+  static constexpr int TH = kWide ? WTH : 256;
   static constexpr int VW = kWide ? 1 : StridedVW<T, N>::value;
   static constexpr int C = PencilShape<N, TH>::C;
   static int blocks(int ncols) { return (ncols + C * VW - 1) / (C * VW); }

@@ -614,11 +614,12 @@ struct k_poisson_z {
 // ---------------------------------------------------------------------------
 template <class T, int N, bool GRAD, bool WIDE = true>
 struct k_poisson_y {
-  // the wide form (one pencil per lane, 512 threads) for the gradient pass,
-  // which holds two spectra per pencil
-  using Cfg = StridedCfg<T, N, GRAD && WIDE>;
+  // the wide form (one pencil per lane) for the gradient pass, which holds
+  // two spectra per pencil, in 256-thread workgroups (16 pencils): 63.8 ->
+  // 59.4 us at 256^3 against 512 threads (the Poisson x pass keeps 512)
+  using Cfg = StridedCfg<T, N, GRAD && WIDE, 256>;
   static constexpr int kThreads = Cfg::TH;
-  // two 512-thread blocks per CU need four waves per SIMD (128 VGPRs)
+  // four 256-thread blocks per CU need four waves per SIMD (128 VGPRs)
   static constexpr int kMinWaves = Cfg::kWide ? 4 : 0;
   static __device__ __forceinline__ void run(cpx<T>* __restrict__ H, cpx<T>* __restrict__ Uz,
                                              cpx<T>* __restrict__ Uy, PassGeom g,
@@ -682,26 +683,38 @@ __device__ __forceinline__ void load_herm_rows(cpx<T> (&v)[VW][E], const cpx<T>*
   }
 }
 
-// Wide form (512 threads, 32 columns, sixteen lanes per 256-point pencil):
-// the lanes of a wave hold two rows j of the stride-16 decomposition that
-// are Hermitian partners, j and 16 - j (wave 0: rows 0 and 8, which are
-// their own partners), each over the same 32 columns.  Element m of row j is
-// bin k = j + 16 m, whose mirror N - k is element 15 - m of row 16 - j
-// (row 0: element 16 - m of itself), so the rows load only the bins k <= N/2
-// and take the others from the partner half of the wave (lane xor 32).
+// Wide form (sixteen lanes per 256-point pencil, one pencil per lane): the
+// lanes of a wave hold rows j of the stride-16 decomposition in Hermitian
+// partner pairs, j and 16 - j (rows 0 and 8 are their own partners), each
+// over the workgroup's columns.  Element m of row j is bin k = j + 16 m,
+// whose mirror N - k is element 15 - m of row 16 - j (row 0: element 16 - m
+// of itself), so the rows load only the bins k <= N/2 and take the others
+// from the partner row.  512 threads, 32 columns: a wave holds one pair,
+// row w in lanes 0-31 and its partner in lanes 32-63 (wave 0: rows 0 and 8);
+// 256 threads, 16 columns: a wave holds two pairs (a, 16 - a) with
+// a = 2 w + lane / 32, partner rows 16 lanes apart (a = 0: rows 0 and 8).
 template <class S>
 __device__ __forceinline__ void paired_row(int& j, int& col) {
-  static_assert(S::kThreads == 512 && S::C == 32 && S::TT == 16, "wide 256-point layout only");
+  static_assert(S::TT == 16 && ((S::kThreads == 512 && S::C == 32) ||
+                                (S::kThreads == 256 && S::C == 16)),
+                "wide 256-point layout only");
   const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
-  col = lane & 31;
-  j = (lane < 32) ? w : (w == 0 ? 8 : 16 - w);
+  if constexpr (S::C == 32) {
+    col = lane & 31;
+    j = (lane < 32) ? w : (w == 0 ? 8 : 16 - w);
+  } else {
+    col = lane & 15;
+    const int a = 2 * w + (lane >> 5);
+    j = (lane & 16) ? (a == 0 ? 8 : 16 - a) : a;
+  }
 }
 
-template <int N, int E>
+template <int N, int E, int C>
 __device__ __forceinline__ void load_herm_paired(cpx<float> (&v)[1][E],
                                                  const cpx<float>* __restrict__ a, int base,
                                                  int se, int j, bool act) {
   static_assert(N == 256 && E == 16, "the paired rows need 256-point pencils of 16 lanes");
+  constexpr int PD = C;  // lane distance to the partner row (32 or 16)
   // own bins: m = 0..7 (k = j + 16 m <= 127), and k = 128 for row 0
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
@@ -718,7 +731,7 @@ __device__ __forceinline__ void load_herm_paired(cpx<float> (&v)[1][E],
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
     const cpx<float> own = v[0][e];
-    const cpx<float> p = {__shfl_xor(own.r, 32, 64), __shfl_xor(own.i, 32, 64)};
+    const cpx<float> p = {__shfl_xor(own.r, PD, 64), __shfl_xor(own.i, PD, 64)};
     if (j == 8)
       v[0][15 - e] = {own.r, -own.i};
     else if (j != 0)
@@ -791,7 +804,7 @@ struct k_poisson_x {
     const int se = (int)g.s_elem;
     auto load_herm = [&](cpx<T>(&x)[VW][E], const cpx<T>* __restrict__ a) {
       if constexpr (Cfg::kWide)
-        load_herm_paired<N, E>(x, a, base, se, j, act);
+        load_herm_paired<N, E, S::C>(x, a, base, se, j, act);
       else
         load_herm_rows<N, VW, E>(x, a, base, se, j, TT, act);
     };
@@ -1436,8 +1449,8 @@ hipError_t launch_poisson_y(const Dims& d, const SpecTables<T>& tb, int grad, vo
       launch<k_poisson_y<T, N_, true, false>>(nblocks_s<T, N_>(g.ncols), 256, s,
           (cpx<T>*)H, (cpx<T>*)Uz, (cpx<T>*)Uy, g, tw, kfy);
     else if (grad)
-      launch<k_poisson_y<T, N_, true>>(StridedCfg<T, N_, true>::blocks(g.ncols),
-          StridedCfg<T, N_, true>::TH, s, (cpx<T>*)H, (cpx<T>*)Uz, (cpx<T>*)Uy, g, tw, kfy);
+      launch<k_poisson_y<T, N_, true>>(k_poisson_y<T, N_, true>::Cfg::blocks(g.ncols),
+          k_poisson_y<T, N_, true>::Cfg::TH, s, (cpx<T>*)H, (cpx<T>*)Uz, (cpx<T>*)Uy, g, tw, kfy);
     else
       launch<k_poisson_y<T, N_, false>>(nblocks_s<T, N_>(g.ncols), 256, s,
           (cpx<T>*)H, (cpx<T>*)Uz, (cpx<T>*)Uy, g, tw, kfy);
