@@ -211,21 +211,22 @@ __device__ __forceinline__ void register_tiles(const TileGeom& tg, const Cic& c,
     if (m >> k & 1u) rank[8 * p + k] = r[k];
 }
 
-// Corner descriptor of a star for column tile t (all of x by 2^(ly + lz)
-// consecutive (y, z) columns from q = t 2^(ly + lz), q = y nz + z): bits
-// 0-8 and 9-17 hold the star's two cell x indices, and the four 9-bit
-// fields from bit 18 hold, for the (y, z) corner pairs (sy, sz) in the
-// order 2 sy + sz, 1 + the column's index in the tile, or 0 when the column
-// is outside it.  The Poisson x pass reads these instead of the position.
+// Corner descriptor of a star for column tile t (all of x by 2^ly y rows by
+// 2^lz z columns): bits 0-8 and 9-17 hold the star's two cell x indices,
+// and the four 9-bit fields from bit 18 hold, for the (y, z) corner pairs
+// (sy, sz) in the order 2 sy + sz, 1 + the column's index in the tile
+// (dy 2^lz + dz), or 0 when the column is outside it.  The Poisson x pass
+// reads these instead of the position.
 __device__ __forceinline__ unsigned long long corner_desc(const CicGeom& g, const TileGeom& tg,
                                                          const Cic& c, int t) {
-  const int w = tg.ly + tg.lz;
-  const int q_lo = t << w;
+  (void)g;
+  const int y0 = (t / tg.ntz) << tg.ly, z0 = (t % tg.ntz) << tg.lz;
   unsigned long long d = (unsigned long long)c.ix[0] | ((unsigned long long)c.ix[1] << 9);
 #pragma unroll
   for (int yz = 0; yz < 4; ++yz) {
-    const int qc = c.iy[yz >> 1] * g.nz + c.iz[yz & 1] - q_lo;
-    const unsigned f = ((unsigned)qc < (1u << w)) ? (unsigned)qc + 1u : 0u;
+    const int dy = c.iy[yz >> 1] - y0, dz = c.iz[yz & 1] - z0;
+    const bool in = (unsigned)dy < (1u << tg.ly) && (unsigned)dz < (1u << tg.lz);
+    const unsigned f = in ? (unsigned)((dy << tg.lz) | dz) + 1u : 0u;
     d |= (unsigned long long)f << (18 + 9 * yz);
   }
   return d;
@@ -903,17 +904,17 @@ TileGeom make_column_tiles(const Dims& d, int cols) {
     while ((1 << r) < v) ++r;
     return r;
   };
+  // cols columns as R y rows by Z z columns, Z = min(cols / 2, nz): a star's
+  // y pair then lies in one tile for every other y, which makes 1.5 tiles
+  // per star on average where a one-row tile makes 2
+  int Z = cols / 2 < d.nz ? cols / 2 : d.nz;
+  if (Z < 1) Z = 1;
+  int R = cols / Z;
+  if (R > d.ny) R = d.ny;
   TileGeom t;
-  const int ncols = d.ny * d.nz;
-  if (cols > ncols) cols = ncols;
   t.lx = lg(d.nx);
-  if (cols <= d.nz) {
-    t.ly = 0;
-    t.lz = lg(cols);
-  } else {
-    t.lz = lg(d.nz);
-    t.ly = lg(cols / d.nz);
-  }
+  t.ly = lg(R);
+  t.lz = lg(Z);
   t.ntx = 1;
   t.nty = d.ny >> t.ly;
   t.ntz = d.nz >> t.lz;

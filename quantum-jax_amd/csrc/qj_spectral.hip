@@ -774,6 +774,8 @@ struct GatherArgs {
   const int* sorted;
   const unsigned long long* desc;
   T* cv;
+  int lz;   // log2 of the z columns of a column tile (its y rows: C VW / 2^lz)
+  int lnz;  // log2 nz
 };
 
 template <class T, int N, bool GRAD, bool WIDE = true, bool SIG = false, bool CVG = false>
@@ -798,8 +800,19 @@ struct k_poisson_x {
     __shared__ cpx<T> lds[S::kLdsStrided * VW];
     int col = threadIdx.x % S::C, j = threadIdx.x / S::C;
     if constexpr (Cfg::kWide) paired_row<S>(j, col);
-    const int q0 = (blockIdx.x * S::C + col) * VW;
-    const bool act = q0 < g.ncols;
+    int q0 = (blockIdx.x * S::C + col) * VW;
+    bool act = q0 < g.ncols;
+    if constexpr (CVG) {
+      // the workgroup's columns form its column tile: C VW columns as rows
+      // of 2^lz z columns (a lane's VW columns adjacent in z), tile t at
+      // (t / (nz >> lz), t % (nz >> lz)) in tile units
+      const int nbz = 1 << (ga.lnz - ga.lz);
+      const int c0 = col * VW;
+      const int y = (blockIdx.x / nbz) * ((S::C * VW) >> ga.lz) + (c0 >> ga.lz);
+      const int z = ((blockIdx.x % nbz) << ga.lz) + (c0 & ((1 << ga.lz) - 1));
+      act = (y << ga.lnz) < g.ncols;
+      q0 = (y << ga.lnz) + z;
+    }
     const int base = act ? (int)pencil_base(g, q0) : 0;
     const int se = (int)g.s_elem;
     auto load_herm = [&](cpx<T>(&x)[VW][E], const cpx<T>* __restrict__ a) {
@@ -1476,7 +1489,12 @@ hipError_t launch_poisson_x(const Dims& d, const SpecTables<T>& tb, int grad, co
   if (sig) sa = SigArgs<T>{(const T*)sig->v[0], (const T*)sig->v[1], (const T*)sig->v[2],
                            sig->owed, sig->cs, sig->part};
   GatherArgs<T> ga{};
-  if (gather) ga = GatherArgs<T>{gather->offset, gather->sorted, gather->desc, (T*)gather->cv};
+  if (gather) {
+    const TileGeom ct = make_column_tiles(d, poisson_x_gather_columns(d, sizeof(T) == 4, tb.wide));
+    int lnz = 0;
+    while ((1 << lnz) < d.nz) ++lnz;
+    ga = GatherArgs<T>{gather->offset, gather->sorted, gather->desc, (T*)gather->cv, ct.lz, lnz};
+  }
   QJ_DISPATCH_N(d.nx, {
     using Cfg = StridedCfg<T, N_, true>;
     using Nar = StridedCfg<T, N_, false>;
