@@ -355,7 +355,13 @@ int qj_deposit_adjoint(qj_plan* plan, const double* pos, const double* mass, dou
  * (grid after V) is then not written (qj_state.grid_valid 2); 0 = the grid
  * is written and the stars gather from it. */
 #define QJ_OPT_STAR_GATHER 6
-#define QJ_OPT_COUNT 7
+/* QJ_OPT_GATHER_ROWS: y rows of the fused gather's column tiles (0 =
+ * automatic, 1 or 2).  Two rows cut the stars' registrations by a quarter;
+ * on grids where they leave the Poisson x pass half-line rows they pay only
+ * with many stars per cell (automatic: at least one star per 32 cells).
+ * Same results to rounding. */
+#define QJ_OPT_GATHER_ROWS 7
+#define QJ_OPT_COUNT 8
 int qj_plan_set_option(qj_plan* plan, int option, int value);
 int qj_plan_get_option(const qj_plan* plan, int option, int* value);
 

@@ -155,7 +155,7 @@ struct qj_plan {
   int nblk_sig = 0;
   // schedule options (qj_plan_set_option): fused x, binned, deterministic,
   // wide Poisson sweeps, z-sweep blocks, side stream
-  int opt[QJ_OPT_COUNT] = {1, 1, 0, 1, 0, 1, 1};
+  int opt[QJ_OPT_COUNT] = {1, 1, 0, 1, 0, 1, 1, 0};
   // second stream (default priority) for the star and gas work of a step,
   // with the fork and join events
   hipStream_t side = nullptr;
@@ -407,7 +407,9 @@ int qj_plan_set_option(qj_plan* p, int option, int value) {
   if (option < 0 || option >= QJ_OPT_COUNT) return fail(QJ_EARG, "unknown option");
   if (option == QJ_OPT_Z_SWEEP_BLOCKS && (value < 0 || value > 65535))
     return fail(QJ_EARG, "QJ_OPT_Z_SWEEP_BLOCKS must be in [0, 65535]");
-  if (option != QJ_OPT_Z_SWEEP_BLOCKS && value != 0 && value != 1)
+  if (option == QJ_OPT_GATHER_ROWS && (value < 0 || value > 2))
+    return fail(QJ_EARG, "QJ_OPT_GATHER_ROWS takes 0, 1 or 2");
+  if (option != QJ_OPT_Z_SWEEP_BLOCKS && option != QJ_OPT_GATHER_ROWS && value != 0 && value != 1)
     return fail(QJ_EARG, "this option takes 0 or 1");
   p->opt[option] = value;
   return QJ_OK;
@@ -646,6 +648,23 @@ int gather_cols(const qj_plan* p, const qj_state* st, bool tape) {
   return poisson_x_gather_columns(p->d, sizeof(T) == 4, p->opt[QJ_OPT_WIDE_SWEEPS]);
 }
 
+// y rows of the column tiles (QJ_OPT_GATHER_ROWS, 0 = automatic): two rows
+// cut the stars' registrations by a quarter, but below 2 x 128-byte rows of
+// columns they cut the Poisson x pass's V rows to half cache lines (+5 us at
+// 256^3).  Measured A/B: 128^3 + 2^16 stars (one per 32 cells) 127.3 ->
+// 123.9 us/step with two rows; 256^3 + 2^18 (one per 64 cells) 482.6 ->
+// 487.0 us/step, and the 48-box ensemble with 2^14 stars per box 5.08 ->
+// 5.33 ms/step.  Automatic: two rows where they keep whole lines or with at
+// least one star per 32 cells.
+template <class T>
+int gather_rows(const qj_plan* p, const qj_state* st, int cols) {
+  const int o = p->opt[QJ_OPT_GATHER_ROWS];
+  if (o == 1 || o == 2) return o;
+  const int full = 128 / (int)sizeof(T);  // reals per 128-byte row
+  if (cols / 2 >= full) return 2;
+  return (st->np * 32 >= p->ncell()) ? 2 : 1;
+}
+
 // V and A3 of the current state (star deposit, source pass, Poisson chain);
 // cols > 0: the fused star gather (V and the stars' corner values).
 template <class T>
@@ -660,8 +679,9 @@ int initial_grid_t(qj_plan* p, qj_state* st, const qj_params* pr, const Work& w,
   StarGather ga{};
   if (stars && binned) {
     // tile-binned deposit of the current positions (overwrites rho_s)
-    StarBins sb = carve_star_scratch(p->d, st->np, st->star_scratch, sizeof(T), cols);
-    if (cols) ga = StarGather{sb.offset, sb.sorted, sb.desc, sb.cv};
+    StarBins sb = carve_star_scratch(p->d, st->np, st->star_scratch, sizeof(T), cols,
+                                     cols ? gather_rows<T>(p, st, cols) : 1);
+    if (cols) ga = StarGather{sb.offset, sb.sorted, sb.desc, sb.cv, sb.tg.lz};
     QJ_CHECK(fill<int>(sb.count, (long long)sb.ntiles * kCountStride, 0, s), "bin reset");
     QJ_STAGE(kStageDeposit,
              launch_star_step_binned<T>(p->d, st->pos, st->vel, st->np, nullptr, 0.0, 0.0, 0.0, 0,
@@ -746,7 +766,9 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
     // gather); the kicks read the corner values when the previous Poisson
     // x pass left them instead of the grid
     StarBins sb{};
-    if (stars && binned) sb = carve_star_scratch(p->d, st->np, st->star_scratch, sizeof(T), cols);
+    if (stars && binned)
+      sb = carve_star_scratch(p->d, st->np, st->star_scratch, sizeof(T), cols,
+                              cols ? gather_rows<T>(p, st, cols) : 1);
     const void* cv_in = (st->grid_valid == 2) ? sb.cv : nullptr;
     // per-step kick and drift times (a schedule, e.g. the comoving factors of
     // quantum-cosmo.py:170-175), else dt for both
@@ -857,7 +879,7 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
       sig->cap = p->nblk_sig;
     }
     StarGather ga{};
-    if (cols) ga = StarGather{sb.offset, sb.sorted, sb.desc, sb.cv};
+    if (cols) ga = StarGather{sb.offset, sb.sorted, sb.desc, sb.cv, sb.tg.lz};
     if (int rc = poisson_tail<T>(p, w, grad, pr->four_pi_G, pr->rho_bar, pr->dc_mode, Vn, A3n,
                                  nullptr, st->diag, s, 0.0, sig, cols ? &ga : nullptr))
       return rc;

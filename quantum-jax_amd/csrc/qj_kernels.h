@@ -104,6 +104,7 @@ struct StarGather {
   const int* sorted;
   const unsigned long long* desc;  // corner descriptors of the registrations
   void* cv;
+  int lz;  // log2 of the column tiles' z extent (their TileGeom::lz)
 };
 
 // Writes V and, with grad, the cell-interleaved grid A3 = (a_x, a_y, a_z)
@@ -249,7 +250,9 @@ TileGeom make_tiles(const Dims& d);
 // consecutive (y, z) columns (q = y nz + z), so tile t is that pass's
 // workgroup t and a step's deposit registrations are also its gather lists.
 constexpr int kTileColMaxCells = 8192;
-TileGeom make_column_tiles(const Dims& d, int cols);
+// rows: the tile's cols columns as rows y rows of cols / rows z columns
+// (at most nz; rows 2 halves the registrations of a star's y pair).
+TileGeom make_column_tiles(const Dims& d, int cols, int rows);
 // Columns per workgroup of the Poisson x gradient pass on grid d (c64: the
 // single-precision build; wide: the wide sweep form), or 0 when the fused
 // gather does not serve that pass (column tiles above kTileColMaxCells).
@@ -261,7 +264,7 @@ size_t star_scratch_total_bytes(const Dims& d, long long np, int real_bytes);
 // cols = 0: the deposit's own tiles; else the column tiles of cols columns.
 // real_bytes = 0: no corner-value area (a taped copy of the registrations).
 StarBins carve_star_scratch(const Dims& d, long long np, void* scratch, int real_bytes = 0,
-                            int cols = 0);
+                            int cols = 0, int rows = 1);
 // Kick/drift/wrap as launch_star_step (move = 1) or nothing (move = 0), and
 // registration counting of the (new) positions in the tiles of sb.  The
 // kicks interpolate A3, or with cv non-null the star's eight corner values

@@ -898,16 +898,17 @@ TileGeom make_tiles(const Dims& d) {
   return t;
 }
 
-TileGeom make_column_tiles(const Dims& d, int cols) {
+TileGeom make_column_tiles(const Dims& d, int cols, int rows) {
   auto lg = [](int v) {
     int r = 0;
     while ((1 << r) < v) ++r;
     return r;
   };
-  // cols columns as R y rows by Z z columns, Z = min(cols / 2, nz): a star's
-  // y pair then lies in one tile for every other y, which makes 1.5 tiles
-  // per star on average where a one-row tile makes 2
-  int Z = cols / 2 < d.nz ? cols / 2 : d.nz;
+  // cols columns as R y rows by Z z columns, Z = min(cols / rows, nz); with
+  // two rows a star's y pair lies in one tile for every other y, 1.5 tiles
+  // per star on average where one-row tiles make 2
+  if (rows < 1) rows = 1;
+  int Z = cols / rows < d.nz ? cols / rows : d.nz;
   if (Z < 1) Z = 1;
   int R = cols / Z;
   if (R > d.ny) R = d.ny;
@@ -926,7 +927,7 @@ namespace {
 // count and offset entries for either tile kind (column tiles have at most
 // as many tiles as the smallest Poisson x workgroup covers columns: 16)
 int max_tiles(const Dims& d) {
-  const int a = make_tiles(d).ntiles, b = make_column_tiles(d, 16).ntiles;
+  const int a = make_tiles(d).ntiles, b = make_column_tiles(d, 16, 1).ntiles;
   return a > b ? a : b;
 }
 size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -944,7 +945,7 @@ size_t star_scratch_total_bytes(const Dims& d, long long np, int real_bytes) {
 }
 
 StarBins carve_star_scratch(const Dims& d, long long np, void* scratch, int real_bytes,
-                            int cols) {
+                            int cols, int rows) {
   const int nt = max_tiles(d);
   StarBins sb;
   char* c = (char*)scratch;
@@ -957,7 +958,7 @@ StarBins carve_star_scratch(const Dims& d, long long np, void* scratch, int real
   sb.desc = (real_bytes && cols)
                 ? (unsigned long long*)((char*)sb.cv + al256((size_t)np * kCvReals * real_bytes))
                 : nullptr;
-  sb.tg = cols ? make_column_tiles(d, cols) : make_tiles(d);
+  sb.tg = cols ? make_column_tiles(d, cols, rows) : make_tiles(d);
   sb.ntiles = sb.tg.ntiles;
   return sb;
 }

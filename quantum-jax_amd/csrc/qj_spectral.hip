@@ -1490,10 +1490,11 @@ hipError_t launch_poisson_x(const Dims& d, const SpecTables<T>& tb, int grad, co
                            sig->owed, sig->cs, sig->part};
   GatherArgs<T> ga{};
   if (gather) {
-    const TileGeom ct = make_column_tiles(d, poisson_x_gather_columns(d, sizeof(T) == 4, tb.wide));
     int lnz = 0;
     while ((1 << lnz) < d.nz) ++lnz;
-    ga = GatherArgs<T>{gather->offset, gather->sorted, gather->desc, (T*)gather->cv, ct.lz, lnz};
+    if (gather->lz < 0 || gather->lz > lnz) return hipErrorInvalidValue;
+    ga = GatherArgs<T>{gather->offset, gather->sorted, gather->desc, (T*)gather->cv, gather->lz,
+                       lnz};
   }
   QJ_DISPATCH_N(d.nx, {
     using Cfg = StridedCfg<T, N_, true>;

@@ -141,3 +141,22 @@ def test_atomic_deposit_after_gathered_steps(qj, cuda):
         res.append(sim.state())
     for k in ("psi", "pos", "vel"):
         assert rel(res[0][k], res[1][k]) < 1e-12, k
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+@pytest.mark.parametrize("g", [O.Grid(32, 32, 32), O.Grid(64, 16, 32, 1.0, 0.5, 2.0),
+                               O.Grid(256, 32, 64)], ids=lambda g: f"{g.nx}x{g.ny}x{g.nz}")
+def test_column_tile_rows_agree(qj, cuda, dtype, g):
+    # QJ_OPT_GATHER_ROWS: one- and two-row column tiles (and the automatic
+    # choice) against the grid path
+    res = {}
+    for gather, rows in ((0, 0), (1, 0), (1, 1), (1, 2)):
+        sim, _ = _sim(qj, g, dtype, 4000, gather)
+        sim.plan.set_option(_lib().OPT_GATHER_ROWS, rows)
+        sim.step(3)
+        st = sim.state()
+        res[(gather, rows)] = (st["psi"].clone(), st["pos"].clone(), st["vel"].clone())
+    base = res[(0, 0)]
+    for key, r in res.items():
+        for a, b in zip(r, base):
+            assert rel(a, b) < TOLS[dtype] / 10, key
