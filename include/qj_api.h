@@ -361,7 +361,14 @@ int qj_deposit_adjoint(qj_plan* plan, const double* pos, const double* mass, dou
  * with many stars per cell (automatic: at least one star per 32 cells).
  * Same results to rounding. */
 #define QJ_OPT_GATHER_ROWS 7
-#define QJ_OPT_COUNT 8
+/* QJ_OPT_FUSED_DEPOSIT (default 1): where the fused gather's column tiles
+ * serve a stars step and the deposit is not the deterministic one, the
+ * Poisson source pass deposits the stars of its column tile itself (in LDS,
+ * added to |psi|^2 and written to rho_s); 0 = the separate per-tile deposit
+ * kernel writes rho_s and the source pass reads it.  Same results to
+ * rounding (float atomics add in either order). */
+#define QJ_OPT_FUSED_DEPOSIT 8
+#define QJ_OPT_COUNT 9
 int qj_plan_set_option(qj_plan* plan, int option, int value);
 int qj_plan_get_option(const qj_plan* plan, int option, int* value);
 

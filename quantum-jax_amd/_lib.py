@@ -29,7 +29,8 @@ OPT_Z_SWEEP_BLOCKS = 4
 OPT_SIDE_STREAM = 5
 OPT_STAR_GATHER = 6
 OPT_GATHER_ROWS = 7
-OPT_COUNT = 8
+OPT_FUSED_DEPOSIT = 8
+OPT_COUNT = 9
 ENS_EAGER, ENS_BATCHED, ENS_GRAPH, ENS_GRAPH_FORK = 0, 1, 2, 3
 # bytes of the static workspace of qj_kdk_steps_noalloc (one per GPU)
 NOALLOC_SCRATCH_BYTES = 40 * 1024 * 1024

@@ -155,7 +155,7 @@ struct qj_plan {
   int nblk_sig = 0;
   // schedule options (qj_plan_set_option): fused x, binned, deterministic,
   // wide Poisson sweeps, z-sweep blocks, side stream
-  int opt[QJ_OPT_COUNT] = {1, 1, 0, 1, 0, 1, 1, 0};
+  int opt[QJ_OPT_COUNT] = {1, 1, 0, 1, 0, 1, 1, 0, 1};
   // second stream (default priority) for the star and gas work of a step,
   // with the fork and join events
   hipStream_t side = nullptr;
@@ -753,6 +753,11 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
   const bool binned = st->star_scratch && p->opt[QJ_OPT_BINNED_DEPOSIT];
   const int clear = 0;  // both deposits leave rho_s as the source of this state
   const int cols = gather_cols<T>(p, st, grid_out != nullptr);
+  // the star deposit fused into the source pass: on the gather's column
+  // tiles, which are also the source pass's workgroup columns
+  const bool fdep = cols && p->opt[QJ_OPT_FUSED_DEPOSIT] &&
+                    !p->opt[QJ_OPT_DETERMINISTIC_DEPOSIT] &&
+                    cols == source_x_columns(p->d, sizeof(T) == 4);
   // the corner values of the fused gather are served from the binned
   // deposit's scratch; the float-atomic star step reads the grid, so a state
   // that kept only corner values has its grid computed again
@@ -802,7 +807,8 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
                  "star step");
         QJ_STAGE(kStageDeposit,
                  launch_tile_deposit<T>(p->d, st->pos, st->mass, st->m_scalar, st->np, sb,
-                                        st->rho_s, s, p->opt[QJ_OPT_DETERMINISTIC_DEPOSIT]),
+                                        fdep ? nullptr : st->rho_s, s,
+                                        p->opt[QJ_OPT_DETERMINISTIC_DEPOSIT]),
                  "deposit");
       } else if (stars) {
         // the float-atomic deposit accumulates into a cleared plane
@@ -853,12 +859,17 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
     if (int rc = side_work(fork ? p->side : s)) return rc;
     if (int rc = drift_zy(s)) return rc;
     if (fork) QJ_CHECK(hipEventRecord(p->ev_join, p->side), "join");
-    src = stars ? st->rho_s : (gas ? live_gas(0) : nullptr);
+    src = stars ? (fdep ? nullptr : st->rho_s) : (gas ? live_gas(0) : nullptr);
+    StarDeposit sd{};
+    if (fdep)
+      sd = StarDeposit{sb.offset, sb.sorted, sb.desc, sb.wts, st->rho_s, sb.tg.ly, sb.tg.lz,
+                       sb.tg.ntiles};
     if (fork) QJ_CHECK(hipStreamWaitEvent(s, p->ev_join, 0), "join");
     if (p->opt[QJ_OPT_FUSE_X_SOURCE]) {
       QJ_STAGE(kStageDriftXSource,
                launch_source_x<T>(p->d, tb, 1, st->psi, src, clear, brho, pr->b_coeff,
-                                  pr->rho_bar, w.H, p->partials, st->diag, s, psi_out),
+                                  pr->rho_bar, w.H, p->partials, st->diag, s, psi_out,
+                                  fdep ? &sd : nullptr),
                "drift x + source");
       if (psi_out) st->psi = psi_out;
     } else {
@@ -866,7 +877,8 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
                launch_drift_axis<T>(p->d, tb, 0, st->psi, nullptr, nullptr, nullptr, 0.0, 0, s), "drift x");
       QJ_STAGE(kStageSourceX,
                launch_source_x<T>(p->d, tb, 0, st->psi, src, clear, brho, pr->b_coeff,
-                                  pr->rho_bar, w.H, p->partials, st->diag, s),
+                                  pr->rho_bar, w.H, p->partials, st->diag, s, nullptr,
+                                  fdep ? &sd : nullptr),
                "source x");
     }
     void* Vn = grid_out ? grid_out : V;

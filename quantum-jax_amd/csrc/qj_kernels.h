@@ -55,12 +55,32 @@ hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, v
                              int kick, hipStream_t s, int conj_d = 0,
                              const double* dslot = nullptr, void* psi_out = nullptr);
 
+// The star deposit fused into the source pass: the column tiles (of
+// make_column_tiles with the pass's workgroup columns, source_x_columns)
+// hold the registrations of the stars (a scanned and scattered StarBins with
+// the corner descriptors and weights); the pass adds their density to
+// |psi|^2 and writes it to rho, every cell, as launch_tile_deposit would.
+struct StarDeposit {
+  const int* offset;
+  const int* sorted;
+  const unsigned long long* desc;  // StarBins::desc
+  const void* wts;                 // StarBins::wts
+  void* rho;
+  int ly, lz;  // the column tiles' TileGeom::ly and ::lz
+  int ntiles;
+};
+
 template <class T>
 // psi_out (with drift): the drifted psi goes to psi_out instead of psi.
+// dep: the fused star deposit (src must then be NULL).
 hipError_t launch_source_x(const Dims& d, const SpecTables<T>& tb, int drift, void* psi,
                            void* src, int clear_src, void* brho, double b_coeff, double rho_bar,
                            void* H, double* partials, double* diag, hipStream_t s,
-                           void* psi_out = nullptr);
+                           void* psi_out = nullptr, const StarDeposit* dep = nullptr);
+
+// Columns per workgroup of the source pass on grid d in the precision of
+// the c64 flag: the column-tile width of its fused deposit.
+int source_x_columns(const Dims& d, int c64);
 
 template <class T>
 hipError_t launch_fft_y_half(const Dims& d, const SpecTables<T>& tb, void* H, hipStream_t s);
@@ -243,6 +263,10 @@ struct StarBins {
   TileGeom tg;  // the tiles the registrations refer to
   void* cv;     // corner values of the fused gather, 24 reals per star (or null)
   unsigned long long* desc;  // column tiles: corner descriptor per registration
+  // column tiles: each star's eight corner weights m w / vol (8 reals per
+  // star, corner (sx, sy, sz) at 4 sy + 2 sz + sx), written by the scatter
+  // for the deposit fused into the source pass
+  void* wts;
 };
 TileGeom make_tiles(const Dims& d);
 // Column tiles of the fused star gather: a tile is the set of x pencils that
@@ -275,6 +299,9 @@ hipError_t launch_star_step_binned(const Dims& d, double* pos, double* vel, long
                                    const StarBins& sb, hipStream_t s, const void* cv = nullptr);
 // Scan, scatter and per-tile deposit: overwrites every cell of rho.  det:
 // order-independent fixed-point accumulation (bitwise reproducible).
+// rho NULL: the scan and scatter only, the scatter also writing the corner
+// weights sb.wts (the deposit is then fused into the source pass,
+// StarDeposit).
 template <class T>
 hipError_t launch_tile_deposit(const Dims& d, const double* pos, const double* mass,
                                double m_scalar, long long np, const StarBins& sb, void* rho,

@@ -421,7 +421,12 @@ struct k_bin_scan {
 
 // Registrations into their tile's slots (the ranks of the counting pass).
 // With desc (column tiles of the fused star gather), each registration also
-// gets the star's corner descriptor relative to its tile (corner_desc).
+// gets the star's corner descriptor relative to its tile (corner_desc); with
+// wts, the star's eight corner weights m w / vol as the per-tile deposit
+// computes them (the same products in the same order), so the deposit fused
+// into the source pass reads 8 reals per registration instead of computing
+// the CIC weights of the star again.
+template <class T>
 struct k_bin_scatter {
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(CicGeom g, TileGeom tg,
@@ -429,7 +434,9 @@ struct k_bin_scatter {
                                              const int* __restrict__ rank,
                                              const int* __restrict__ offset,
                                              int* __restrict__ sorted,
-                                             unsigned long long* __restrict__ desc) {
+                                             unsigned long long* __restrict__ desc,
+                                             T* __restrict__ wts, const double* __restrict__ mass,
+                                             double m_scalar) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     Cic c = cic(g, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]);
@@ -446,6 +453,17 @@ struct k_bin_scatter {
 #pragma unroll
       for (int k = 0; k < 8; ++k)
         if (m >> k & 1u) desc[o[k]] = corner_desc(g, tg, c, ids[k]);
+    }
+    if (wts) {
+      const double fac = (mass ? mass[p] : m_scalar) / g.vol;
+      T w[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int sx = kCorner[k][0], sy = kCorner[k][1], sz = kCorner[k][2];
+        w[4 * sy + 2 * sz + sx] = (T)(c.wx[sx] * c.wy[sy] * c.wz[sz] * fac);
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e) wts[8 * p + e] = w[e];
     }
   }
 };
@@ -941,7 +959,7 @@ size_t star_scratch_bytes(const Dims& d, long long np) {
 
 size_t star_scratch_total_bytes(const Dims& d, long long np, int real_bytes) {
   return star_scratch_bytes(d, np) + al256((size_t)np * kCvReals * real_bytes) +
-         al256((size_t)np * 8 * sizeof(unsigned long long));
+         al256((size_t)np * 8 * sizeof(unsigned long long)) + al256((size_t)np * 8 * real_bytes);
 }
 
 StarBins carve_star_scratch(const Dims& d, long long np, void* scratch, int real_bytes,
@@ -958,6 +976,8 @@ StarBins carve_star_scratch(const Dims& d, long long np, void* scratch, int real
   sb.desc = (real_bytes && cols)
                 ? (unsigned long long*)((char*)sb.cv + al256((size_t)np * kCvReals * real_bytes))
                 : nullptr;
+  sb.wts = sb.desc ? (void*)((char*)sb.desc + al256((size_t)np * 8 * sizeof(unsigned long long)))
+                   : nullptr;
   sb.tg = cols ? make_column_tiles(d, cols, rows) : make_tiles(d);
   sb.ntiles = sb.tg.ntiles;
   return sb;
@@ -986,9 +1006,13 @@ hipError_t launch_tile_deposit(const Dims& d, const double* pos, const double* m
                                hipStream_t s, int det) {
   const TileGeom tg = sb.tg;
   launch<k_bin_scan>(1, 1024, s, sb.count, sb.offset, tg.ntiles);
+  // the fused deposit reads the corner weights of the registrations
+  if (!rho && (!sb.desc || !sb.wts || tg.ntx != 1)) return hipErrorInvalidValue;
   if (np > 0)
-    launch<k_bin_scatter>(pblocks(np), 256, s, make_geom(d), tg, pos, np, (const int*)sb.rank,
-                          (const int*)sb.offset, sb.sorted, sb.desc);
+    launch<k_bin_scatter<T>>(pblocks(np), 256, s, make_geom(d), tg, pos, np, (const int*)sb.rank,
+                             (const int*)sb.offset, sb.sorted, sb.desc,
+                             rho ? (T*)nullptr : (T*)sb.wts, mass, m_scalar);
+  if (!rho) return hipGetLastError();
   if (det)
     QJ_TILE_VOL_COL(tg, launch<k_tile_deposit_det<T, VOL_>>(tg.ntiles, 256, s, make_geom(d), tg,
                                                             pos, mass, m_scalar,
@@ -1041,9 +1065,10 @@ hipError_t launch_interp_adj_binned(const Dims& d, const double* pos, long long 
   if (!taped) {  // else offset and sorted hold the forward's registrations of pos
     launch<k_bin_scan>(1, 1024, s, sb.count, sb.offset, tg.ntiles);
     if (np > 0)
-      launch<k_bin_scatter>(pblocks(np), 256, s, make_geom(d), tg, pos, np,
-                            (const int*)sb.rank, (const int*)sb.offset, sb.sorted,
-                            (unsigned long long*)nullptr);
+      launch<k_bin_scatter<T>>(pblocks(np), 256, s, make_geom(d), tg, pos, np,
+                               (const int*)sb.rank, (const int*)sb.offset, sb.sorted,
+                               (unsigned long long*)nullptr, (T*)nullptr,
+                               (const double*)nullptr, 0.0);
   }
   QJ_TILE_VOL(tg, launch<k_tile_deposit_vec<T, VOL_>>(tg.ntiles, 256, s, make_geom(d), tg, pos,
                                                       (const double*)vbar, h,

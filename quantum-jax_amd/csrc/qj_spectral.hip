@@ -373,8 +373,45 @@ __device__ double block_compensated_sum(const double* __restrict__ x, int n) {
 // Poisson source pass along x (optionally fused with the x drift sweep):
 //   rho = |psi|^2 + src; FP64 mass; forward x-FFT of (rho - rho_bar);
 //   store kx = 0..nx/2 into H.
+// DEP (stars steps with the fused deposit): the workgroup's columns are a
+// column tile of the star registrations (the layout of the Poisson x
+// gather); the pass adds the CIC corner weights of the tile's stars
+// (quantum-stars.py:226-247, m w / vol, which the scatter wrote per star)
+// at the corners inside the tile into LDS, adds the tile to |psi|^2 and
+// writes it to rho_s, so the star density is not written by one kernel and
+// read back by the next, and the deposit needs no launch.
 // ---------------------------------------------------------------------------
-template <class T, int N, bool DRIFT>
+template <class T>
+struct DepositArgs {
+  const int* offset;
+  const int* sorted;
+  const unsigned long long* desc;  // corner descriptors of the registrations
+  const T* wts;                    // 8 corner weights per star
+  T* rho;                          // rho_s, the tile's star density
+  int lz;                          // log2 of the tile's z columns
+  int lnz;                         // log2 nz
+};
+
+// the eight corner weights of star i
+template <class T>
+__device__ __forceinline__ void load_wts8(const T* __restrict__ w, long long i, T (&o)[8]) {
+  if constexpr (sizeof(T) == 4) {
+    const float4* q = reinterpret_cast<const float4*>(w + 8 * i);
+    const float4 a = q[0], b = q[1];
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+    o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
+  } else {
+    const double2* q = reinterpret_cast<const double2*>(w + 8 * i);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const double2 a = q[e];
+      o[2 * e] = a.x;
+      o[2 * e + 1] = a.y;
+    }
+  }
+}
+
+template <class T, int N, bool DRIFT, bool DEP = false>
 struct k_source_x {
   static constexpr int kThreads = 256;
   static __device__ __forceinline__ void run(cpx<T>* __restrict__ psi, T* __restrict__ src,
@@ -384,7 +421,7 @@ struct k_source_x {
                                              const cpx<T>* __restrict__ tw, cpx<T>* __restrict__ H,
                                              double* __restrict__ partials,
                                              double* __restrict__ diag,
-                                             cpx<T>* __restrict__ pout) {
+                                             cpx<T>* __restrict__ pout, DepositArgs<T> dp) {
     // one exchange buffer that the lane's pencils take turns with (half the
     // LDS of QJ_STRIDED_SETUP): the pass holds 162 registers, so the LDS
     // decides between two and three waves per SIMD
@@ -392,10 +429,38 @@ struct k_source_x {
     constexpr int VW = StridedVW<T, N>::value;
     constexpr int E = S::E;
     constexpr int TT = S::TT;
+    constexpr int CV = S::C * VW;  // columns of the workgroup
     __shared__ cpx<T> lds[S::kLdsStrided];
     const int col = threadIdx.x % S::C, j = threadIdx.x / S::C;
-    const int q0 = (blockIdx.x * S::C + col) * VW;
-    const bool act = q0 < g.ncols;
+    int q0 = (blockIdx.x * S::C + col) * VW;
+    bool act = q0 < g.ncols;
+    // the fused deposit: the thread's first registration of the tile is
+    // loaded before psi and its star's weights after psi's rows, so both
+    // latencies hide behind the drift transforms
+    int db0 = 0, db1 = 0, dp0 = 0;
+    unsigned long long dd = 0;
+    T dw[8];
+    if constexpr (DEP) {
+      // column tile t = blockIdx.x at (t / (nz >> lz), t % (nz >> lz)) in
+      // tile units, its CV columns as rows of 2^lz z columns (a lane's VW
+      // columns adjacent in z)
+      const int nbz = 1 << (dp.lnz - dp.lz);
+      const int c0 = col * VW;
+      const int y = (blockIdx.x / nbz) * (CV >> dp.lz) + (c0 >> dp.lz);
+      const int z = ((blockIdx.x % nbz) << dp.lz) + (c0 & ((1 << dp.lz) - 1));
+      act = (y << dp.lnz) < g.ncols;
+      q0 = (y << dp.lnz) + z;
+      db0 = dp.offset[blockIdx.x];
+      db1 = dp.offset[blockIdx.x + 1];
+      // loaded by every thread (from the tile's first slot when the thread
+      // has none, slot 0 of an empty tile): no branch around the loads.  An
+      // empty tile's slot 0 may be stale, so its star is taken to be star 0.
+      const int i0 = db0 + (int)threadIdx.x;
+      const int ic = i0 < db1 ? i0 : (db1 > db0 ? db0 : 0);
+      const int p0 = dp.sorted[ic];
+      dp0 = db1 > db0 ? p0 : 0;
+      dd = dp.desc[ic];
+    }
     const int base = act ? (int)pencil_base(g, q0) : 0;
     const int se = (int)g.s_elem;
     cpx<T> v[VW][E];
@@ -408,7 +473,7 @@ struct k_source_x {
     // behind the drift transforms (LDS, not registers, limits the blocks per
     // CU): 120 -> 88 us at 256^3 c64
     T sv[VW][E];
-    if (src && act) {
+    if (!DEP && src && act) {
   #pragma unroll
       for (int m = 0; m < E; ++m) {
         T t[VW];
@@ -419,6 +484,7 @@ struct k_source_x {
     }
     if (psi) {
       load_rows<VW, E>(v, psi, base, se, j, TT, act);
+      if constexpr (DEP) load_wts8(dp.wts, dp0, dw);
       if constexpr (DRIFT) {
         pencil_fft_strided<N, -1, T, VW, true>(v, j, col, lds, tw);
   #pragma unroll
@@ -437,13 +503,64 @@ struct k_source_x {
         for (int w = 0; w < VW; ++w)
           rho[w][m] = fmad(v[w][m].r, v[w][m].r, v[w][m].i * v[w][m].i);
     }
+    if constexpr (DEP) {
+      // The exchange buffer is free between the drift and the source
+      // transform: the tile's density is summed there (cell (x, column c) at
+      // x CV + c, float atomics as in k_tile_deposit), added to the thread's
+      // cells and written to rho_s.  Short pencils (one thread per pencil)
+      // have no exchange buffer and take one of their own.
+      constexpr bool kOwn = N * CV * sizeof(T) > sizeof(lds);
+      __shared__ T own_tile[kOwn ? N * CV : 1];
+      T* acc = kOwn ? own_tile : reinterpret_cast<T*>(lds);
+      __syncthreads();  // the drift's last exchange reads are done
+      for (int l = threadIdx.x; l < N * CV; l += kThreads) acc[l] = (T)0;
+      __syncthreads();
+      const int i0 = db0 + (int)threadIdx.x;
+      for (int i = i0; i < db1; i += kThreads) {
+        unsigned long long d = dd;
+        T w[8];
+  #pragma unroll
+        for (int e = 0; e < 8; ++e) w[e] = dw[e];
+        if (i != i0) {
+          d = dp.desc[i];
+          load_wts8(dp.wts, dp.sorted[i], w);
+        }
+        // the descriptor: x of the two x corners (9 bits each), then per
+        // (y, z) corner pair yz = 2 sy + sz its tile column + 1, or 0 outside
+        const int x0 = (int)(d & 511u) * CV, x1 = (int)((d >> 9) & 511u) * CV;
+  #pragma unroll
+        for (int yz = 0; yz < 4; ++yz) {
+          const int f = (int)((d >> (18 + 9 * yz)) & 511u);
+          if (f != 0) {
+            atomicAdd(&acc[x0 + f - 1], w[2 * yz]);
+            atomicAdd(&acc[x1 + f - 1], w[2 * yz + 1]);
+          }
+        }
+      }
+      __syncthreads();
+      if (act) {
+  #pragma unroll
+        for (int m = 0; m < E; ++m) {
+          const int xr = j + TT * m;
+          T t[VW];
+  #pragma unroll
+          for (int w = 0; w < VW; ++w) {
+            t[w] = acc[xr * CV + col * VW + w];
+            rho[w][m] += t[w];
+          }
+          str<VW>(dp.rho, base + xr * se, t);
+        }
+      }
+      // the source transform's first exchange waits at a barrier before it
+      // writes the buffer
+    }
     double local = 0.0;
     bool bad = false;
   #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int idx = base + (j + TT * m) * se;
       if (act) {
-        if (src) {
+        if (!DEP && src) {
   #pragma unroll
           for (int w = 0; w < VW; ++w) rho[w][m] += sv[w][m];
           if (clear_src) {
@@ -1385,6 +1502,23 @@ int poisson_x_gather_columns(const Dims& d, int c64, int wide) {
   return c64 ? gather_columns_t<float>(d, wide != 0) : gather_columns_t<double>(d, wide != 0);
 }
 
+namespace {
+template <class T>
+int source_x_columns_t(const Dims& d) {
+  switch (d.nx) {
+#define QJ_SC(n) \
+  case n: return PencilShape<n>::C * StridedVW<T, n>::value;
+    QJ_SC(8) QJ_SC(16) QJ_SC(32) QJ_SC(64) QJ_SC(128) QJ_SC(256) QJ_SC(512)
+#undef QJ_SC
+    default: return 0;
+  }
+}
+}  // namespace
+
+int source_x_columns(const Dims& d, int c64) {
+  return c64 ? source_x_columns_t<float>(d) : source_x_columns_t<double>(d);
+}
+
 int source_x_blocks(const Dims& d) {
   // partials are sized for the larger of the two precisions' grids
   const int a = source_x_blocks_t<float>(d), b = source_x_blocks_t<double>(d);
@@ -1396,20 +1530,45 @@ template <class T>
 hipError_t launch_source_x(const Dims& d, const SpecTables<T>& tb, int drift, void* psi,
                            void* src, int clear_src, void* brho, double b_coeff, double rho_bar,
                            void* H, double* partials, double* diag, hipStream_t s,
-                           void* psi_out) {
+                           void* psi_out, const StarDeposit* dep) {
   if (psi_out && !drift) return hipErrorInvalidValue;  // only the drift writes psi
   PassGeom g = geom_x(d);
   const cpx<T>* D = (const cpx<T>*)tb.drift_x;
   const cpx<T>* tw = (const cpx<T>*)tb.tw_x;
+  DepositArgs<T> da{};
+  if (dep) {
+    // the tiles must be the workgroups' columns: cols columns as rows of
+    // 2^lz z columns, one tile per workgroup
+    const int cols = source_x_columns(d, sizeof(T) == 4);
+    const long long ncols = (long long)d.ny * d.nz;
+    int lnz = 0;
+    while ((1 << lnz) < d.nz) ++lnz;
+    if (src || !psi || !dep->offset || !dep->sorted || !dep->desc || !dep->wts || !dep->rho ||
+        (long long)d.nx * (cols < ncols ? cols : ncols) > kTileColMaxCells || dep->lz < 0 ||
+        dep->lz > lnz ||
+        (1 << dep->lz) > cols || dep->ntiles != source_x_blocks_t<T>(d) ||
+        ((1 << (dep->ly + dep->lz)) != cols && dep->ntiles != 1))
+      return hipErrorInvalidValue;
+    da = DepositArgs<T>{dep->offset, dep->sorted, dep->desc, (const T*)dep->wts, (T*)dep->rho,
+                        dep->lz, lnz};
+  }
   QJ_DISPATCH_N(d.nx, {
-    if (drift)
+    if (drift && dep)
+      launch<k_source_x<T, N_, true, true>>(nblocks_s<T, N_>(g.ncols), 256, s,
+          (cpx<T>*)psi, (T*)src, clear_src, (T*)brho, b_coeff, rho_bar, g, D, tw, (cpx<T>*)H,
+          partials, diag, (cpx<T>*)psi_out, da);
+    else if (dep)
+      launch<k_source_x<T, N_, false, true>>(nblocks_s<T, N_>(g.ncols), 256, s,
+          (cpx<T>*)psi, (T*)src, clear_src, (T*)brho, b_coeff, rho_bar, g, D, tw, (cpx<T>*)H,
+          partials, diag, (cpx<T>*)psi_out, da);
+    else if (drift)
       launch<k_source_x<T, N_, true>>(nblocks_s<T, N_>(g.ncols), 256, s,
           (cpx<T>*)psi, (T*)src, clear_src, (T*)brho, b_coeff, rho_bar, g, D, tw, (cpx<T>*)H,
-          partials, diag, (cpx<T>*)psi_out);
+          partials, diag, (cpx<T>*)psi_out, da);
     else
       launch<k_source_x<T, N_, false>>(nblocks_s<T, N_>(g.ncols), 256, s,
           (cpx<T>*)psi, (T*)src, clear_src, (T*)brho, b_coeff, rho_bar, g, D, tw, (cpx<T>*)H,
-          partials, diag, (cpx<T>*)psi_out);
+          partials, diag, (cpx<T>*)psi_out, da);
   });
   return hipGetLastError();
 }
@@ -1644,7 +1803,7 @@ hipError_t launch_adj_poisson(const Dims& d, const SpecTables<T>& tb, const void
                                            hipStream_t, int, const double*, void*);             \
   template hipError_t launch_source_x<T>(const Dims&, const SpecTables<T>&, int, void*, void*,  \
                                          int, void*, double, double, void*, double*, double*,   \
-                                         hipStream_t, void*);                                   \
+                                         hipStream_t, void*, const StarDeposit*);               \
   template hipError_t launch_fft_y_half<T>(const Dims&, const SpecTables<T>&, void*,            \
                                            hipStream_t);                                        \
   template hipError_t launch_poisson_z<T>(const Dims&, const SpecTables<T>&, int, void*, void*, \

@@ -160,3 +160,52 @@ def test_column_tile_rows_agree(qj, cuda, dtype, g):
     for key, r in res.items():
         for a, b in zip(r, base):
             assert rel(a, b) < TOLS[dtype] / 10, key
+
+
+# QJ_OPT_FUSED_DEPOSIT: the source pass deposits its column tile's stars
+# (64 x 16 x 32: tiles of two y rows; 8^3: one tile smaller than the
+# workgroup's columns; 512 x 16 x 16: 512-point pencils)
+@pytest.mark.parametrize("dtype", [C64, C128])
+@pytest.mark.parametrize("g", GRIDS + [O.Grid(128, 64, 32, 1.0, 0.5, 0.25)],
+                         ids=lambda g: f"{g.nx}x{g.ny}x{g.nz}")
+@pytest.mark.parametrize("fuse_x", [1, 0])
+def test_fused_deposit_matches_tile_deposit(qj, cuda, dtype, g, fuse_x):
+    L = _lib()
+    res = {}
+    for fd in (1, 0):
+        sim, ic = _sim(qj, g, dtype, 3000, 1)
+        sim.plan.set_option(L.OPT_FUSED_DEPOSIT, fd)
+        sim.plan.set_option(L.OPT_FUSE_X_SOURCE, fuse_x)
+        sim.step(3)
+        st = sim.state()
+        res[fd] = (st["psi"].clone(), st["pos"].clone(), st["vel"].clone(), sim.rho_s.clone())
+    for k in range(4):
+        assert rel(res[1][k], res[0][k]) < TOLS[dtype] / 10, k
+    # rho_s is the CIC deposit of the positions after the step (the oracle's
+    # bin_stars at those positions)
+    m_s = ic[3]
+    ref = O.bin_stars(res[1][1].cpu().numpy(), m_s, g)
+    assert rel(res[1][3], ref) < (1e-6 if dtype == C64 else 1e-12)
+
+
+def test_fused_deposit_star_masses(qj, cuda):
+    # per-star masses (the mass array the adjoint differentiates)
+    g = O.Grid(32, 32, 32)
+    L = _lib()
+    psi = O.filtered_gaussian_psi(g, 3, sigma_k=6 * np.pi)
+    pos, vel, m_s = O.star_ic(g, 2500, 1003, sigma_v=0.2)
+    m = m_s * (0.5 + np.random.default_rng(5).random(2500))
+    out = []
+    for fd in (1, 0):
+        plan = qj.Plan(32, dtype=C128)
+        plan.set_option(L.OPT_FUSED_DEPOSIT, fd)
+        sim = qj.KDKSimulation(plan, qj.Physics(rho_bar=1.1, four_pi_G=20.0), 1e-3,
+                               dev(psi, C128), dev(pos, torch.float64), dev(vel, torch.float64),
+                               mass=dev(m, torch.float64))
+        sim.step(3)
+        st = sim.state()
+        out.append((st["psi"].clone(), st["pos"].clone(), sim.rho_s.clone()))
+    for a, b in zip(*out):
+        assert rel(a, b) < 1e-12
+    ref = O.bin_particles(out[0][1].cpu().numpy(), m, g)
+    assert rel(out[0][2], ref) < 1e-12
