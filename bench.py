@@ -66,7 +66,7 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def stage_bytes(name, nx, ny, nz, c, r, n_p, stars, gather=True):
+def stage_bytes(name, nx, ny, nz, c, r, n_p, stars, gather=True, fdep=True):
     """Algorithmic HBM bytes of one launch of a stage (read + write once).
 
     Stars use the tile-binned deposit: it overwrites every cell of rho_s
@@ -76,22 +76,28 @@ def stage_bytes(name, nx, ny, nz, c, r, n_p, stars, gather=True):
     the Poisson x pass writes V and, instead of the acceleration grid, each
     star's eight corner accelerations (24 r B), reading its two column-tile
     registrations (index and corner descriptor, 12 B each); the deposit's
-    scatter writes those descriptors (about 112 B per star in all)."""
+    scatter writes those descriptors (about 112 B per star in all).  With
+    the fused deposit as well (the default) the source pass writes rho_s
+    (N r) and reads the two registrations and the star's eight corner
+    weights (24 + 8 r B per star), and the deposit stage is the scan and
+    the scatter (positions, ranks, registrations, descriptors and weights:
+    56 + 8 r B per star)."""
     N = nx * ny * nz
     Nh = (nx // 2 + 1) * ny * nz
     src = r
     fused = stars and gather and n_p > 0
+    fd = fused and fdep
     table = {
         "drift_z_kick": N * (2 * c + r),
         "drift_z": 2 * N * c, "drift_y": 2 * N * c, "drift_x": 2 * N * c,
-        "drift_x_source": 2 * N * c + N * src + Nh * c,
-        "source_x": N * c + N * src + Nh * c,
+        "drift_x_source": 2 * N * c + N * src + Nh * c + (n_p * (24 + 8 * r) if fd else 0),
+        "source_x": N * c + N * src + Nh * c + (n_p * (24 + 8 * r) if fd else 0),
         "fft_y_half": 2 * Nh * c,
         "poisson_z": 3 * Nh * c,
         "poisson_y": 5 * Nh * c,
         # V plane + (a_x, a_y, a_z) cells, or V + the stars' corner values
         "poisson_x": 3 * Nh * c + (N * r + n_p * (24 * r + 24) if fused else 4 * N * r),
-        "deposit": N * r + n_p * (112 if fused else 64),
+        "deposit": n_p * (56 + 8 * r) if fd else N * r + n_p * (112 if fused else 64),
         "star_step": n_p * (96 + 8 * 3 * r + 8),
         "star_kick": n_p * (72 + 8 * 3 * r),
         "kick": N * (2 * c + r),
@@ -361,7 +367,9 @@ def main():
         if cnt_arr[k]:
             avg = ms_arr[k] / cnt_arr[k]
             b = stage_bytes(names[k], n, n, n, c_b, r_b, n_p, True,
-                            bool(plan.get_option(lib_mod.OPT_STAR_GATHER)))
+                            bool(plan.get_option(lib_mod.OPT_STAR_GATHER)),
+                            bool(plan.get_option(lib_mod.OPT_FUSED_DEPOSIT)) and
+                            not plan.get_option(lib_mod.OPT_DETERMINISTIC_DEPOSIT))
             stages[names[k]] = {
                 "launches_per_step": round(cnt_arr[k] / tsteps, 3),
                 "avg_ms": round(avg, 5),
