@@ -151,10 +151,23 @@ struct HTile {
   static constexpr int threads = (SY * SZ + 63) / 64 * 64;  // tile cells + ring, whole waves
 };
 
+// Element i of a field, addressed by a 32-bit byte offset from the array's
+// base (every field is under 4 GiB): the loads and stores then take the
+// scalar-base form, one shared offset for all the arrays of a cell, instead
+// of 64-bit address arithmetic per array.
+template <class T>
+__device__ __forceinline__ const T& at(const T* __restrict__ p, unsigned i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + i * (unsigned)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ T& at(T* __restrict__ p, unsigned i) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + i * (unsigned)sizeof(T));
+}
+
 // (a_x, a_y, a_z) of cell i of the acceleration grid
 template <class T>
-__device__ __forceinline__ void ld_acc(const T* __restrict__ A3, int i, T a[3]) {
-  const T* q = A3 + kAccComp * i;
+__device__ __forceinline__ void ld_acc(const T* __restrict__ A3, unsigned i, T a[3]) {
+  const T* q = &at(A3, kAccComp * i);
   a[0] = q[0];
   a[1] = q[1];
   a[2] = q[2];
@@ -199,9 +212,14 @@ struct k_hydro_stream {
     // slopes are kept in cell units (times dx): hdx = dt / (2 dx) scales the
     // predictor, and the faces are p -+ slope / 2
     const T hdx = (T)0.5 * g.dt / g.dx, eps_dx = (T)1.0e-12 * g.dx;
+    const T c2 = g.cs * g.cs;
 
-    auto gidx = [&](int x, int y, int z) {
-      return (((x & (g.nx - 1)) * g.ny + (y & (g.ny - 1))) * g.nz) + (z & (g.nz - 1));
+    // unsigned cell indices (every grid has under 2^32 / 16 cells): a 32-bit
+    // offset from the array's base, so the loads and stores take the
+    // scalar-base form instead of 64-bit address arithmetic per array
+    auto gidx = [&](int x, int y, int z) -> unsigned {
+      return (((unsigned)(x & (g.nx - 1)) * (unsigned)g.ny + (unsigned)(y & (g.ny - 1))) *
+              (unsigned)g.nz) + (unsigned)(z & (g.nz - 1));
     };
     // A plane of the raw region (tile + 2-cell halo, at most 2 cells per
     // thread) is fetched into registers one iteration before it is written to
@@ -219,11 +237,11 @@ struct k_hydro_stream {
         const int c = tid + k * (int)blockDim.x;
         if (c >= RY * RZ) break;
         const int yy = c / RZ - 2, zz = c % RZ - 2;
-        const int i = gidx(x, y0 + yy, z0 + zz);
-        f.q[k][0] = rho[i];
-        f.q[k][1] = vx[i];
-        f.q[k][2] = vy[i];
-        f.q[k][3] = vz[i];
+        const unsigned i = gidx(x, y0 + yy, z0 + zz);
+        f.q[k][0] = at(rho, i);
+        f.q[k][1] = at(vx, i);
+        f.q[k][2] = at(vy, i);
+        f.q[k][3] = at(vz, i);
         if (kick != (T)0) ld_acc(A3, i, f.a[k]);
       }
     };
@@ -263,13 +281,15 @@ struct k_hydro_stream {
         s[2][k] = limit<T>(v0[k], raw[sc][k][o - 1], raw[sc][k][o + 1], eps_dx);
       }
       const T r = v0[0], u = v0[1], v = v0[2], w = v0[3];
-      const T ir = rcp(r);
-      // quantum-fluid.py:295-311
+      // quantum-fluid.py:295-311; the pressure terms (1/rho) rho_d cs cs as
+      // (cs^2 / rho) rho_d: one product per component instead of three (the
+      // FP32 rounding of that term moves by at most an ulp)
+      const T irc2 = rcp(r) * c2;
       p[0] = r - hdx * (u * s[0][0] + r * s[0][1] + v * s[1][0] + r * s[1][2] + w * s[2][0] +
                         r * s[2][3]);
-      p[1] = u - hdx * (u * s[0][1] + v * s[1][1] + w * s[2][1] + ir * s[0][0] * g.cs * g.cs);
-      p[2] = v - hdx * (u * s[0][2] + v * s[1][2] + w * s[2][2] + ir * s[1][0] * g.cs * g.cs);
-      p[3] = w - hdx * (u * s[0][3] + v * s[1][3] + w * s[2][3] + ir * s[2][0] * g.cs * g.cs);
+      p[1] = u - hdx * (u * s[0][1] + v * s[1][1] + w * s[2][1] + irc2 * s[0][0]);
+      p[2] = v - hdx * (u * s[0][2] + v * s[1][2] + w * s[2][2] + irc2 * s[1][0]);
+      p[3] = w - hdx * (u * s[0][3] + v * s[1][3] + w * s[2][3] + irc2 * s[2][0]);
     };
     // states of plane x for the tile and its ring into st; own column
     // (p, s_x) returned in xs
@@ -449,12 +469,12 @@ struct k_hydro_stream {
           U[k] += -fac * Fz[k];
           U[k] += fac * fyz[1][k][(ty + 1) * SZ + tz];
         }
-        const int i = gidx(x, y0 + ty, z0 + tz);
+        const unsigned i = gidx(x, y0 + ty, z0 + tz);
         const T iM = rcp(U[0]);
-        rho_o[i] = U[0] * ivol;  // quantum-fluid.py:191-197 (M / vol)
-        vx_o[i] = U[1] * iM;
-        vy_o[i] = U[2] * iM;
-        vz_o[i] = U[3] * iM;
+        at(rho_o, i) = U[0] * ivol;  // quantum-fluid.py:191-197 (M / vol)
+        at(vx_o, i) = U[1] * iM;
+        at(vy_o, i) = U[2] * iM;
+        at(vz_o, i) = U[3] * iM;
       }
     };
     // XC is even (a power of two of at least 8, or 32)
