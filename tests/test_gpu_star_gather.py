@@ -209,3 +209,46 @@ def test_fused_deposit_star_masses(qj, cuda):
         assert rel(a, b) < 1e-12
     ref = O.bin_particles(out[0][1].cpu().numpy(), m, g)
     assert rel(out[0][2], ref) < 1e-12
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+@pytest.mark.parametrize("rows", [1, 2])
+def test_fused_deposit_on_tile_edges(qj, cuda, dtype, rows):
+    # stars whose CIC clouds straddle column-tile edges in y and z, wrap
+    # around the periodic faces, or sit on cell centres (a zero-weight
+    # corner), each in cells of its own: the fused and the separate deposit
+    # then add one value per cell and agree bitwise, and both equal the
+    # oracle's deposit.  No gravity and no velocity, so the stars stay put.
+    L = _lib()
+    n = 64
+    g = O.Grid(n, n, n)
+    dx = g.Lx / n
+    c = lambda i: (i + 0.5) * dx  # cell centre
+    pos = np.array([
+        [c(3), 2 * dx, c(9)],            # y face of cells 1 | 2 (a two-row tile edge)
+        [c(20), c(40), 32 * dx],         # z face of cells 31 | 32
+        [c(30), 16 * dx, 48 * dx],       # both
+        [c(40), 0.1 * dx, c(20)],        # y wraps (cells 63 | 0)
+        [c(50), c(10), n * dx - 0.25 * dx],  # z wraps
+        [0.2 * dx, c(50), c(50)],        # x wraps
+        [c(12), c(25), c(5)],            # on a cell centre
+        [10 * dx, 34 * dx, 16 * dx],     # faces on all three axes
+        [c(60), 62.7 * dx, 63.9 * dx],   # near the far corner
+        [c(5), 53.5 * dx, 30.5 * dx],
+    ])
+    m = np.linspace(0.5, 1.5, len(pos)) * 1e-4
+    psi = O.filtered_gaussian_psi(g, 8, sigma_k=6 * np.pi)
+    out = []
+    for fd in (1, 0):
+        plan = qj.Plan(n, dtype=dtype)
+        plan.set_option(L.OPT_FUSED_DEPOSIT, fd)
+        plan.set_option(L.OPT_GATHER_ROWS, rows)
+        sim = qj.KDKSimulation(plan, qj.Physics(rho_bar=1.0, four_pi_G=0.0), 1e-3, dev(psi, dtype),
+                               dev(pos, torch.float64), dev(np.zeros_like(pos), torch.float64),
+                               mass=dev(m, torch.float64))
+        sim.step(2)
+        assert torch.equal(sim.pos.cpu(), torch.as_tensor(pos))
+        out.append(sim.rho_s.clone())
+    assert torch.equal(out[0], out[1])
+    ref = O.bin_particles(pos, m, g)
+    assert rel(out[0], ref) < (1e-7 if dtype == C64 else 1e-15)
