@@ -191,6 +191,10 @@ typedef struct {
    * (V = get_potential(rho) + external_potential(), quantum-tidal.py:175-202);
    * psi-only states */
   void* v_ext;
+  /* host-side bookkeeping, initialise to 0: 1 when rho_s holds the star
+   * density of pos; stars steps with the fused deposit that does not write
+   * it (QJ_OPT_FUSED_DEPOSIT 2) leave 0, and qj_star_density computes it */
+  int rho_valid;
 } qj_state;
 
 typedef struct {
@@ -257,6 +261,16 @@ int qj_kdk_steps_cfl(qj_plan* plan, qj_state* state, const qj_params* prm, int n
 int qj_bh_accretion(qj_plan* plan, void* psi, const double* pos, const double* vel, double* m_bh,
                     long long n_bh, double G, double c, double sigma, double m_per_hbar,
                     double dt, void* stream);
+
+/* The star density rho_s of the state's current positions (the deposit of
+ * quantum-stars.py:212-247, bin_stars), for a state whose rho_valid is 0:
+ * the stars are registered again in the deposit's own tiles and added with
+ * the deterministic (fixed-point, order-independent) tile deposit when the
+ * state has a star scratch, else with float atomics.  Sets rho_valid.  The
+ * registrations of the last step in star_scratch are overwritten (the next
+ * step registers its own); the fused gather's corner values are kept. */
+int qj_star_density(qj_plan* plan, qj_state* state, void* stream);
+
 /* Bytes of qj_state.star_scratch for np stars: per-tile counters and
  * offsets and up to 8 registrations per star (bin_stars of
  * quantum-stars.py:212-247, restructured as owner-computes by tile). */
@@ -361,10 +375,12 @@ int qj_deposit_adjoint(qj_plan* plan, const double* pos, const double* mass, dou
  * with many stars per cell (automatic: at least one star per 32 cells).
  * Same results to rounding. */
 #define QJ_OPT_GATHER_ROWS 7
-/* QJ_OPT_FUSED_DEPOSIT (default 1): where the fused gather's column tiles
+/* QJ_OPT_FUSED_DEPOSIT (default 2): where the fused gather's column tiles
  * serve a stars step and the deposit is not the deterministic one, the
  * Poisson source pass deposits the stars of its column tile itself (in LDS,
- * added to |psi|^2 and written to rho_s); 0 = the separate per-tile deposit
+ * added to |psi|^2); 2 = rho_s is then not written (qj_state.rho_valid 0,
+ * qj_star_density computes it when a caller needs it), 1 = the pass also
+ * writes the tile's density to rho_s, 0 = the separate per-tile deposit
  * kernel writes rho_s and the source pass reads it.  Same results to
  * rounding (float atomics add in either order). */
 #define QJ_OPT_FUSED_DEPOSIT 8

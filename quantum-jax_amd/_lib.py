@@ -49,7 +49,7 @@ class QjState(C.Structure):
         ("pos", vp), ("vel", vp), ("mass", vp), ("m_scalar", f64), ("np", i64),
         ("grid", vp), ("brho", vp), ("diag", vp),
         ("grid_valid", i32), ("gas_swapped", i32), ("pending", f64), ("pending_star", f64),
-        ("star_scratch", vp), ("v_ext", vp),
+        ("star_scratch", vp), ("v_ext", vp), ("rho_valid", i32),
     ]
 
 
@@ -96,6 +96,7 @@ SIGNATURES = {
     "qj_kdk_steps_cfl": (i32, [vp, vp, vp, i32, f64, f64, dp, vp, vp]),
     "qj_kdk_steps": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), i32, vp, vp]),
     "qj_kdk_finalize": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), vp]),
+    "qj_star_density": (i32, [vp, C.POINTER(QjState), vp]),
     "qj_kdk_steps_noalloc": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), i32, vp]),
     "qj_kdk_steps_sched": (i32, [vp, C.POINTER(QjState), C.POINTER(QjParams), i32, dp, dp, vp, vp]),
     "qj_adjoint_scratch_size": (C.c_size_t, [vp]),

@@ -155,7 +155,7 @@ struct qj_plan {
   int nblk_sig = 0;
   // schedule options (qj_plan_set_option): fused x, binned, deterministic,
   // wide Poisson sweeps, z-sweep blocks, side stream
-  int opt[QJ_OPT_COUNT] = {1, 1, 0, 1, 0, 1, 1, 0, 1};
+  int opt[QJ_OPT_COUNT] = {1, 1, 0, 1, 0, 1, 1, 0, 2};
   // second stream (default priority) for the star and gas work of a step,
   // with the fork and join events
   hipStream_t side = nullptr;
@@ -407,9 +407,10 @@ int qj_plan_set_option(qj_plan* p, int option, int value) {
   if (option < 0 || option >= QJ_OPT_COUNT) return fail(QJ_EARG, "unknown option");
   if (option == QJ_OPT_Z_SWEEP_BLOCKS && (value < 0 || value > 65535))
     return fail(QJ_EARG, "QJ_OPT_Z_SWEEP_BLOCKS must be in [0, 65535]");
-  if (option == QJ_OPT_GATHER_ROWS && (value < 0 || value > 2))
-    return fail(QJ_EARG, "QJ_OPT_GATHER_ROWS takes 0, 1 or 2");
-  if (option != QJ_OPT_Z_SWEEP_BLOCKS && option != QJ_OPT_GATHER_ROWS && value != 0 && value != 1)
+  if ((option == QJ_OPT_GATHER_ROWS || option == QJ_OPT_FUSED_DEPOSIT) && (value < 0 || value > 2))
+    return fail(QJ_EARG, "QJ_OPT_GATHER_ROWS and QJ_OPT_FUSED_DEPOSIT take 0, 1 or 2");
+  if (option != QJ_OPT_Z_SWEEP_BLOCKS && option != QJ_OPT_GATHER_ROWS &&
+      option != QJ_OPT_FUSED_DEPOSIT && value != 0 && value != 1)
     return fail(QJ_EARG, "this option takes 0 or 1");
   p->opt[option] = value;
   return QJ_OK;
@@ -699,6 +700,7 @@ int initial_grid_t(qj_plan* p, qj_state* st, const qj_params* pr, const Work& w,
              "deposit");
   }
   const int clear = 0;  // both deposits leave rho_s as the source of this state
+  if (stars) st->rho_valid = 1;
   void* src = stars ? st->rho_s
                     : (gas ? (st->gas_swapped ? st->gas_tmp[0] : st->gas[0]) : nullptr);
   QJ_STAGE(kStageSourceX,
@@ -710,6 +712,34 @@ int initial_grid_t(qj_plan* p, qj_state* st, const qj_params* pr, const Work& w,
                                ga.cv ? &ga : nullptr))
     return rc;
   st->grid_valid = ga.cv ? 2 : 1;
+  return QJ_OK;
+}
+
+// The star density of the current positions into rho_s (qj_star_density):
+// the stars registered in the deposit's own tiles and added with the
+// fixed-point tile deposit (the same bits on every call), or with float
+// atomics without a star scratch.  Only the registration part of the
+// scratch is used, so the fused gather's corner values after it survive.
+template <class T>
+int star_density_t(qj_plan* p, qj_state* st, hipStream_t s) {
+  if (st->np > 0) {
+    if (!st->pos || !st->rho_s) return fail(QJ_EARG, "qj_star_density: stars need pos and rho_s");
+    if (st->star_scratch && p->opt[QJ_OPT_BINNED_DEPOSIT]) {
+      StarBins sb = carve_star_scratch(p->d, st->np, st->star_scratch);
+      QJ_CHECK(fill<int>(sb.count, (long long)sb.ntiles * kCountStride, 0, s), "bin reset");
+      QJ_CHECK(launch_star_step_binned<T>(p->d, st->pos, st->vel, st->np, nullptr, 0.0, 0.0, 0.0,
+                                          0, sb, s),
+               "bin");
+      QJ_CHECK(launch_tile_deposit<T>(p->d, st->pos, st->mass, st->m_scalar, st->np, sb,
+                                      st->rho_s, s, 1),
+               "deposit");
+    } else {
+      QJ_CHECK(fill<T>(st->rho_s, p->ncell(), (T)0, s), "rho_s clear");
+      QJ_CHECK(launch_deposit<T>(p->d, st->pos, st->mass, st->m_scalar, st->np, st->rho_s, s),
+               "deposit");
+    }
+  }
+  st->rho_valid = 1;
   return QJ_OK;
 }
 
@@ -755,8 +785,8 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
   const int cols = gather_cols<T>(p, st, grid_out != nullptr);
   // the star deposit fused into the source pass: on the gather's column
   // tiles, which are also the source pass's workgroup columns
-  const bool fdep = cols && p->opt[QJ_OPT_FUSED_DEPOSIT] &&
-                    !p->opt[QJ_OPT_DETERMINISTIC_DEPOSIT] &&
+  const int fd_opt = p->opt[QJ_OPT_FUSED_DEPOSIT];
+  const bool fdep = cols && fd_opt && !p->opt[QJ_OPT_DETERMINISTIC_DEPOSIT] &&
                     cols == source_x_columns(p->d, sizeof(T) == 4);
   // the corner values of the fused gather are served from the binned
   // deposit's scratch; the float-atomic star step reads the grid, so a state
@@ -862,8 +892,8 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
     src = stars ? (fdep ? nullptr : st->rho_s) : (gas ? live_gas(0) : nullptr);
     StarDeposit sd{};
     if (fdep)
-      sd = StarDeposit{sb.offset, sb.sorted, sb.desc, sb.wts, st->rho_s, sb.tg.ly, sb.tg.lz,
-                       sb.tg.ntiles};
+      sd = StarDeposit{sb.offset, sb.sorted, sb.desc, sb.wts, fd_opt == 1 ? st->rho_s : nullptr,
+                       sb.tg.ly, sb.tg.lz, sb.tg.ntiles};
     if (fork) QJ_CHECK(hipStreamWaitEvent(s, p->ev_join, 0), "join");
     if (p->opt[QJ_OPT_FUSE_X_SOURCE]) {
       QJ_STAGE(kStageDriftXSource,
@@ -897,6 +927,8 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
       return rc;
     if (grid_out) st->grid = grid_out;
     st->grid_valid = cols ? 2 : 1;
+    // the fused deposit without its rho_s write leaves the density uncomputed
+    if (stars) st->rho_valid = (fdep && fd_opt == 2) ? 0 : 1;
     // the closing half kicks (psi, gas and stars) are owed to the next step
     if (!dslot) st->pending = 0.5 * dk;
     if (stars) st->pending_star = 0.5 * dt;
@@ -1330,6 +1362,13 @@ int qj_deposit_adjoint(qj_plan* p, const double* pos, const double* mass, double
       QJ_PRECISION(p, launch_deposit_adj<float>(p->d, pos, mass, m_scalar, np, rhobar, m_bar, x_bar, s),
                    launch_deposit_adj<double>(p->d, pos, mass, m_scalar, np, rhobar, m_bar, x_bar, s));
   return e == hipSuccess ? QJ_OK : hipfail(e, "qj_deposit_adjoint");
+}
+
+int qj_star_density(qj_plan* p, qj_state* st, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!st) return fail(QJ_EARG, "qj_star_density: null state");
+  hipStream_t s = (hipStream_t)stream;
+  return p->dtype == QJ_C64 ? star_density_t<float>(p, st, s) : star_density_t<double>(p, st, s);
 }
 
 int qj_kdk_finalize(qj_plan* p, qj_state* st, const qj_params* pr, void* stream) {

@@ -59,13 +59,14 @@ hipError_t launch_drift_axis(const Dims& d, const SpecTables<T>& tb, int axis, v
 // make_column_tiles with the pass's workgroup columns, source_x_columns)
 // hold the registrations of the stars (a scanned and scattered StarBins with
 // the corner descriptors and weights); the pass adds their density to
-// |psi|^2 and writes it to rho, every cell, as launch_tile_deposit would.
+// |psi|^2 and, with rho non-null, writes it to rho, every cell, as
+// launch_tile_deposit would.
 struct StarDeposit {
   const int* offset;
   const int* sorted;
   const unsigned long long* desc;  // StarBins::desc
   const void* wts;                 // StarBins::wts
-  void* rho;
+  void* rho;                       // the density is written here, or nowhere (null)
   int ly, lz;  // the column tiles' TileGeom::ly and ::lz
   int ntiles;
 };

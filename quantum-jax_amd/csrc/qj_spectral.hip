@@ -387,7 +387,7 @@ struct DepositArgs {
   const int* sorted;
   const unsigned long long* desc;  // corner descriptors of the registrations
   const T* wts;                    // 8 corner weights per star
-  T* rho;                          // rho_s, the tile's star density
+  T* rho;                          // rho_s, the tile's star density (or null)
   int lz;                          // log2 of the tile's z columns
   int lnz;                         // log2 nz
 };
@@ -548,7 +548,7 @@ struct k_source_x {
             t[w] = acc[xr * CV + col * VW + w];
             rho[w][m] += t[w];
           }
-          str<VW>(dp.rho, base + xr * se, t);
+          if (dp.rho) str<VW>(dp.rho, base + xr * se, t);
         }
       }
       // the source transform's first exchange waits at a barrier before it
@@ -1543,7 +1543,7 @@ hipError_t launch_source_x(const Dims& d, const SpecTables<T>& tb, int drift, vo
     const long long ncols = (long long)d.ny * d.nz;
     int lnz = 0;
     while ((1 << lnz) < d.nz) ++lnz;
-    if (src || !psi || !dep->offset || !dep->sorted || !dep->desc || !dep->wts || !dep->rho ||
+    if (src || !psi || !dep->offset || !dep->sorted || !dep->desc || !dep->wts ||
         (long long)d.nx * (cols < ncols ? cols : ncols) > kTileColMaxCells || dep->lz < 0 ||
         dep->lz > lnz ||
         (1 << dep->lz) > cols || dep->ntiles != source_x_blocks_t<T>(d) ||

@@ -144,7 +144,7 @@ class Ensemble:
         for s in self.sims:
             # star_scratch: after stars steps with the fused star gather the
             # stars' corner accelerations live there instead of in grid
-            t = {k: getattr(s, k) for k in ("_psi", "pos", "vel", "rho_s", "grid", "diag", "brho",
+            t = {k: getattr(s, k) for k in ("_psi", "pos", "vel", "_rho_s", "grid", "diag", "brho",
                                             "star_scratch")}
             t = {k: (v.clone() if v is not None else None) for k, v in t.items()}
             if s.gas is not None:
@@ -185,7 +185,7 @@ class Ensemble:
              ("binned_deposit", _lib.OPT_BINNED_DEPOSIT, lambda key: key[2] == "stars", (0, 1)),
              ("star_gather", _lib.OPT_STAR_GATHER, lambda key: key[2] == "stars", (1, 0)),
              ("gather_rows", _lib.OPT_GATHER_ROWS, lambda key: key[2] == "stars", (0, 1, 2)),
-             ("fused_deposit", _lib.OPT_FUSED_DEPOSIT, lambda key: key[2] == "stars", (1, 0)),
+             ("fused_deposit", _lib.OPT_FUSED_DEPOSIT, lambda key: key[2] == "stars", (2, 0)),
              ("wide_sweeps", _lib.OPT_WIDE_SWEEPS,
               lambda key: key[1] == "complex64" and 256 in key[0][:2], (0, 1)),
              # 65535: one pencil group per workgroup (no walking)

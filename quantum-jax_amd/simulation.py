@@ -45,7 +45,7 @@ class KDKSimulation:
         self.pos = self.vel = None
         self.mass_t = None
         self.m_scalar = 0.0
-        self.rho_s = None
+        self._rho_s = None
         self.star_scratch = None
         if self.np:
             self.pos = plan.particles(pos, "pos").clone()
@@ -54,7 +54,7 @@ class KDKSimulation:
                 self.mass_t = plan.particles(mass, "mass", cols=0).clone()
             else:
                 self.m_scalar = float(mass)
-            self.rho_s = torch.zeros(plan.shape, dtype=rdt, device=dev)
+            self._rho_s = torch.zeros(plan.shape, dtype=rdt, device=dev)
             # tile-binned deposit scratch (counters must start at zero)
             nb = int(_lib.lib().qj_star_scratch_size(plan.handle, self.np))
             self.star_scratch = torch.zeros(nb, dtype=torch.uint8, device=dev)
@@ -69,7 +69,7 @@ class KDKSimulation:
         self.brho = torch.empty(plan.shape, dtype=rdt, device=dev) if phys.b_coeff else None
         st = _lib.QjState()
         st.psi = self._psi.data_ptr()
-        st.rho_s = ptr(self.rho_s).value
+        st.rho_s = ptr(self._rho_s).value
         if self.gas is not None:
             for k in range(4):
                 st.gas[k] = self.gas[k].data_ptr()
@@ -87,6 +87,7 @@ class KDKSimulation:
         st.pending = 0.0
         st.pending_star = 0.0
         st.star_scratch = ptr(self.star_scratch).value
+        st.rho_valid = 0
         self.v_ext = plan.field(v_ext, "v_ext").clone() if v_ext is not None else None
         st.v_ext = ptr(self.v_ext).value
         self._st = st
@@ -194,6 +195,21 @@ class KDKSimulation:
     def V(self) -> torch.Tensor:
         """Potential of the current state (computed on the first step)."""
         return self.grid[: self.plan.ncell].view(self.plan.shape)
+
+    @property
+    def rho_s(self) -> torch.Tensor | None:
+        """Star density of the current positions (bin_stars of
+        quantum-stars.py:212-247); None without stars.
+
+        Steps with the fused star deposit (_lib.OPT_FUSED_DEPOSIT 2, the
+        default) add it to the Poisson source without writing it, so it is
+        computed here when read (qj_star_density), into the state's buffer."""
+        if self._rho_s is not None and not self._st.rho_valid:
+            with torch.cuda.device(self.plan.device):
+                _lib.check(_lib.lib().qj_star_density(self.plan.handle, C.byref(self._st),
+                                                      self.plan.stream()),
+                           "qj_star_density")
+        return self._rho_s
 
     @property
     def a_grid(self) -> torch.Tensor:

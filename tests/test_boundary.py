@@ -49,8 +49,30 @@ def test_state_struct_layout_matches_header():
     body = text[text.index("typedef struct {\n  void* psi;"):text.index("} qj_state;")]
     fields = re.findall(r"\b([a-z_]+)(?:\[\d\])?;", re.sub(r"/\*.*?\*/", "", body, flags=re.S))
     assert fields == [f[0] for f in _lib.QjState._fields_]
-    # 2 + 4 + 4 + 6 pointer-sized members, two ints, two doubles
-    assert ctypes.sizeof(_lib.QjState) == 184
+    # 2 + 4 + 4 + 6 pointer-sized members, two ints, two doubles, one int
+    # (padded to 8)
+    assert ctypes.sizeof(_lib.QjState) == 192
+    # and the C compiler's layout of the header agrees, member by member
+    import shutil
+    import subprocess
+    import tempfile
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        return
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "layout.c")
+        with open(src, "w") as fh:
+            fh.write('#include "qj_api.h"\n#include <stdio.h>\n#include <stddef.h>\n'
+                     "int main(void) {\n"
+                     '  printf("%zu\\n", sizeof(qj_state));\n' +
+                     "".join(f'  printf("%zu\\n", offsetof(qj_state, {f}));\n' for f in fields) +
+                     "  return 0;\n}\n")
+        exe = os.path.join(d, "layout")
+        subprocess.run([cc, "-I", os.path.dirname(HEADER), src, "-o", exe], check=True)
+        got = [int(x) for x in subprocess.run([exe], capture_output=True, text=True,
+                                              check=True).stdout.split()]
+    assert got[0] == ctypes.sizeof(_lib.QjState)
+    assert got[1:] == [getattr(_lib.QjState, f).offset for f in fields]
 
 
 def test_params_struct_layout_matches_header():

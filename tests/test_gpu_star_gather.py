@@ -162,9 +162,11 @@ def test_column_tile_rows_agree(qj, cuda, dtype, g):
             assert rel(a, b) < TOLS[dtype] / 10, key
 
 
-# QJ_OPT_FUSED_DEPOSIT: the source pass deposits its column tile's stars
-# (64 x 16 x 32: tiles of two y rows; 8^3: one tile smaller than the
-# workgroup's columns; 512 x 16 x 16: 512-point pencils)
+# QJ_OPT_FUSED_DEPOSIT: the source pass deposits its column tile's stars,
+# writing rho_s from its LDS tile (1) or leaving it to qj_star_density (2,
+# the default), against the separate deposit kernel (0).  64 x 16 x 32:
+# tiles of two y rows; 8^3: one tile smaller than the workgroup's columns;
+# 512 x 16 x 16: 512-point pencils
 @pytest.mark.parametrize("dtype", [C64, C128])
 @pytest.mark.parametrize("g", GRIDS + [O.Grid(128, 64, 32, 1.0, 0.5, 0.25)],
                          ids=lambda g: f"{g.nx}x{g.ny}x{g.nz}")
@@ -172,15 +174,18 @@ def test_column_tile_rows_agree(qj, cuda, dtype, g):
 def test_fused_deposit_matches_tile_deposit(qj, cuda, dtype, g, fuse_x):
     L = _lib()
     res = {}
-    for fd in (1, 0):
+    for fd in (2, 1, 0):
         sim, ic = _sim(qj, g, dtype, 3000, 1)
         sim.plan.set_option(L.OPT_FUSED_DEPOSIT, fd)
         sim.plan.set_option(L.OPT_FUSE_X_SOURCE, fuse_x)
         sim.step(3)
+        assert sim._st.rho_valid == (0 if fd == 2 else 1)
         st = sim.state()
         res[fd] = (st["psi"].clone(), st["pos"].clone(), st["vel"].clone(), sim.rho_s.clone())
-    for k in range(4):
-        assert rel(res[1][k], res[0][k]) < TOLS[dtype] / 10, k
+        assert sim._st.rho_valid == 1
+    for fd in (2, 1):
+        for k in range(4):
+            assert rel(res[fd][k], res[0][k]) < TOLS[dtype] / 10, (fd, k)
     # rho_s is the CIC deposit of the positions after the step (the oracle's
     # bin_stars at those positions)
     m_s = ic[3]
@@ -196,7 +201,7 @@ def test_fused_deposit_star_masses(qj, cuda):
     pos, vel, m_s = O.star_ic(g, 2500, 1003, sigma_v=0.2)
     m = m_s * (0.5 + np.random.default_rng(5).random(2500))
     out = []
-    for fd in (1, 0):
+    for fd in (1, 0, 2):
         plan = qj.Plan(32, dtype=C128)
         plan.set_option(L.OPT_FUSED_DEPOSIT, fd)
         sim = qj.KDKSimulation(plan, qj.Physics(rho_bar=1.1, four_pi_G=20.0), 1e-3,
@@ -205,8 +210,9 @@ def test_fused_deposit_star_masses(qj, cuda):
         sim.step(3)
         st = sim.state()
         out.append((st["psi"].clone(), st["pos"].clone(), sim.rho_s.clone()))
-    for a, b in zip(*out):
-        assert rel(a, b) < 1e-12
+    for o in out[1:]:
+        for a, b in zip(out[0], o):
+            assert rel(a, b) < 1e-12
     ref = O.bin_particles(out[0][1].cpu().numpy(), m, g)
     assert rel(out[0][2], ref) < 1e-12
 
@@ -218,7 +224,8 @@ def test_fused_deposit_on_tile_edges(qj, cuda, dtype, rows):
     # around the periodic faces, or sit on cell centres (a zero-weight
     # corner), each in cells of its own: the fused and the separate deposit
     # then add one value per cell and agree bitwise, and both equal the
-    # oracle's deposit.  No gravity and no velocity, so the stars stay put.
+    # oracle's deposit (and so does the density qj_star_density computes for
+    # the default form).  No gravity and no velocity, so the stars stay put.
     L = _lib()
     n = 64
     g = O.Grid(n, n, n)
@@ -239,7 +246,7 @@ def test_fused_deposit_on_tile_edges(qj, cuda, dtype, rows):
     m = np.linspace(0.5, 1.5, len(pos)) * 1e-4
     psi = O.filtered_gaussian_psi(g, 8, sigma_k=6 * np.pi)
     out = []
-    for fd in (1, 0):
+    for fd in (1, 0, 2):
         plan = qj.Plan(n, dtype=dtype)
         plan.set_option(L.OPT_FUSED_DEPOSIT, fd)
         plan.set_option(L.OPT_GATHER_ROWS, rows)
@@ -251,4 +258,6 @@ def test_fused_deposit_on_tile_edges(qj, cuda, dtype, rows):
         out.append(sim.rho_s.clone())
     assert torch.equal(out[0], out[1])
     ref = O.bin_particles(pos, m, g)
-    assert rel(out[0], ref) < (1e-7 if dtype == C64 else 1e-15)
+    # (2: qj_star_density, the fixed-point deposit, rounds each value twice)
+    for o in out:
+        assert rel(o, ref) < (1e-7 if dtype == C64 else 1e-15)

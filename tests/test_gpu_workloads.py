@@ -109,6 +109,9 @@ def test_stars_256_cubed_binned_and_atomic_deposits_agree(qj, cuda, dtype):
     out = []
     for binned in (True, False):
         plan = qj.Plan(256, dtype=dtype)
+        # the binned run writes rho_s from the source pass's LDS tiles (the
+        # fused deposit's density itself, not qj_star_density's)
+        plan.set_option(qj._lib.OPT_FUSED_DEPOSIT, 1)
         sim = qj.KDKSimulation(plan, qj.Physics(rho_bar=1.1), DT_STARS, dev(o["psi0"], dtype),
                                dev(o["pos0"], torch.float64), dev(o["vel0"], torch.float64),
                                mass=o["m_s"])
