@@ -66,7 +66,7 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def stage_bytes(name, nx, ny, nz, c, r, n_p, stars, gather=True, fdep=True):
+def stage_bytes(name, nx, ny, nz, c, r, n_p, stars, gather=True, fdep=True, rho_write=False):
     """Algorithmic HBM bytes of one launch of a stage (read + write once).
 
     Stars use the tile-binned deposit: it overwrites every cell of rho_s
@@ -77,16 +77,18 @@ def stage_bytes(name, nx, ny, nz, c, r, n_p, stars, gather=True, fdep=True):
     star's eight corner accelerations (24 r B), reading its two column-tile
     registrations (index and corner descriptor, 12 B each); the deposit's
     scatter writes those descriptors (about 112 B per star in all).  With
-    the fused deposit as well (the default) the source pass writes rho_s
-    (N r) and reads the two registrations and the star's eight corner
-    weights (24 + 8 r B per star), and the deposit stage is the scan and
-    the scatter (positions, ranks, registrations, descriptors and weights:
-    56 + 8 r B per star)."""
+    the fused deposit as well (the default) the source pass reads the two
+    registrations and the star's eight corner weights (24 + 8 r B per star)
+    and writes rho_s (N r) only with rho_write (QJ_OPT_FUSED_DEPOSIT 1), and
+    the deposit stage is the scan and the scatter (positions, ranks,
+    registrations, descriptors and weights: 56 + 8 r B per star)."""
     N = nx * ny * nz
     Nh = (nx // 2 + 1) * ny * nz
     src = r
     fused = stars and gather and n_p > 0
     fd = fused and fdep
+    if fd and not rho_write:
+        src = 0  # rho_s neither read nor written
     table = {
         "drift_z_kick": N * (2 * c + r),
         "drift_z": 2 * N * c, "drift_y": 2 * N * c, "drift_x": 2 * N * c,
@@ -314,20 +316,25 @@ def main():
     value = N * args.steps / (ms_total * 1e-3)
     log(f"{ms_step:.4f} ms/step, {value:.4e} cell-updates/s (host wall {wall:.3f}s)")
 
-    # ---- the other x schedule (fused x drift + Poisson source vs split) ---
+    # ---- the other x schedules (fused x drift + Poisson source, split, and
+    # the automatic choice between them) ---------------------------------
     fuse0 = plan.get_option(lib_mod.OPT_FUSE_X_SOURCE)
-    plan.set_option(lib_mod.OPT_FUSE_X_SOURCE, 1 - fuse0)
-    sim.step(3)
-    torch.cuda.synchronize()
-    ev0.record()
-    sim.step(args.steps)
-    sim.finalize()
-    ev1.record()
-    torch.cuda.synchronize()
-    ms_alt = ev0.elapsed_time(ev1) / args.steps
-    log(f"x schedule fuse={1 - fuse0}: {ms_alt:.4f} ms/step (fuse={fuse0}: {ms_step:.4f})")
+    schedules = {f"fuse_x_source={fuse0}": ms_step}
+    for alt in (0, 1, 2):
+        if alt == fuse0:
+            continue
+        plan.set_option(lib_mod.OPT_FUSE_X_SOURCE, alt)
+        sim.step(3)
+        torch.cuda.synchronize()
+        ev0.record()
+        sim.step(args.steps)
+        sim.finalize()
+        ev1.record()
+        torch.cuda.synchronize()
+        ms_alt = ev0.elapsed_time(ev1) / args.steps
+        log(f"x schedule fuse={alt}: {ms_alt:.4f} ms/step (fuse={fuse0}: {ms_step:.4f})")
+        schedules[f"fuse_x_source={alt}"] = ms_alt
     plan.set_option(lib_mod.OPT_FUSE_X_SOURCE, fuse0)
-    schedules = {f"fuse_x_source={fuse0}": ms_step, f"fuse_x_source={1 - fuse0}": ms_alt}
 
     # ---- star kernels on the caller's stream instead of the side stream --
     side0 = plan.get_option(lib_mod.OPT_SIDE_STREAM)
@@ -369,7 +376,8 @@ def main():
             b = stage_bytes(names[k], n, n, n, c_b, r_b, n_p, True,
                             bool(plan.get_option(lib_mod.OPT_STAR_GATHER)),
                             bool(plan.get_option(lib_mod.OPT_FUSED_DEPOSIT)) and
-                            not plan.get_option(lib_mod.OPT_DETERMINISTIC_DEPOSIT))
+                            not plan.get_option(lib_mod.OPT_DETERMINISTIC_DEPOSIT),
+                            plan.get_option(lib_mod.OPT_FUSED_DEPOSIT) == 1)
             stages[names[k]] = {
                 "launches_per_step": round(cnt_arr[k] / tsteps, 3),
                 "avg_ms": round(avg, 5),

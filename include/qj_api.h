@@ -332,7 +332,10 @@ int qj_deposit_adjoint(qj_plan* plan, const double* pos, const double* mass, dou
  * compiler picks the fusion) -------------------------------------------
  * QJ_OPT_FUSE_X_SOURCE: 1 = the x drift sweep and the Poisson source pass
  * run as one kernel (fewer HBM bytes, more work per workgroup); 0 = two
- * kernels.  Results are identical; the autotuner picks per grid. */
+ * kernels; 2 (default) = one kernel, except in stars steps whose star work
+ * runs on the side stream with the fused deposit on x pencils of 256 points
+ * or more, where the separate x drift runs beside the star work too.
+ * Results are identical; the autotuner picks per grid. */
 #define QJ_OPT_FUSE_X_SOURCE 0
 /* QJ_OPT_BINNED_DEPOSIT: with qj_state.star_scratch given, 1 = tile-binned
  * star deposit (default), 0 = float-atomic deposit (cheaper for sparse

@@ -155,7 +155,7 @@ struct qj_plan {
   int nblk_sig = 0;
   // schedule options (qj_plan_set_option): fused x, binned, deterministic,
   // wide Poisson sweeps, z-sweep blocks, side stream
-  int opt[QJ_OPT_COUNT] = {1, 1, 0, 1, 0, 1, 1, 0, 2};
+  int opt[QJ_OPT_COUNT] = {2, 1, 0, 1, 0, 1, 1, 0, 2};
   // second stream (default priority) for the star and gas work of a step,
   // with the fork and join events
   hipStream_t side = nullptr;
@@ -407,10 +407,10 @@ int qj_plan_set_option(qj_plan* p, int option, int value) {
   if (option < 0 || option >= QJ_OPT_COUNT) return fail(QJ_EARG, "unknown option");
   if (option == QJ_OPT_Z_SWEEP_BLOCKS && (value < 0 || value > 65535))
     return fail(QJ_EARG, "QJ_OPT_Z_SWEEP_BLOCKS must be in [0, 65535]");
-  if ((option == QJ_OPT_GATHER_ROWS || option == QJ_OPT_FUSED_DEPOSIT) && (value < 0 || value > 2))
-    return fail(QJ_EARG, "QJ_OPT_GATHER_ROWS and QJ_OPT_FUSED_DEPOSIT take 0, 1 or 2");
-  if (option != QJ_OPT_Z_SWEEP_BLOCKS && option != QJ_OPT_GATHER_ROWS &&
-      option != QJ_OPT_FUSED_DEPOSIT && value != 0 && value != 1)
+  const bool three = option == QJ_OPT_GATHER_ROWS || option == QJ_OPT_FUSED_DEPOSIT ||
+                     option == QJ_OPT_FUSE_X_SOURCE;
+  if (three && (value < 0 || value > 2)) return fail(QJ_EARG, "this option takes 0, 1 or 2");
+  if (option != QJ_OPT_Z_SWEEP_BLOCKS && !three && value != 0 && value != 1)
     return fail(QJ_EARG, "this option takes 0 or 1");
   p->opt[option] = value;
   return QJ_OK;
@@ -788,6 +788,17 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
   const int fd_opt = p->opt[QJ_OPT_FUSED_DEPOSIT];
   const bool fdep = cols && fd_opt && !p->opt[QJ_OPT_DETERMINISTIC_DEPOSIT] &&
                     cols == source_x_columns(p->d, sizeof(T) == 4);
+  // the x schedule; the automatic one (2) splits the x drift from the source
+  // pass where the star work runs on the side stream with the fused deposit
+  // and the x pencils have 256 points or more: the split x drift then runs
+  // beside the star work as well and the streams contend less (256^3 +
+  // 2^18 stars 472.9 -> 469.1 us/step, 2^16 stars 423.2 -> 419.8; at 128^3
+  // + 2^16 the fused pass is ahead, 115.8 against 121.6)
+  const bool forks = stars && p->opt[QJ_OPT_SIDE_STREAM] && p->side && !g_recorder;
+  const int fx_opt = p->opt[QJ_OPT_FUSE_X_SOURCE];
+  const bool fuse_x = fx_opt == 1 || (fx_opt == 2 && !(forks && fdep && p->d.nx >= 256));
+  if (psi_out && !fuse_x)
+    return fail(QJ_EARG, "kdk_steps_t: psi_out needs the fused x-source schedule");
   // the corner values of the fused gather are served from the binned
   // deposit's scratch; the float-atomic star step reads the grid, so a state
   // that kept only corner values has its grid computed again
@@ -826,7 +837,7 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
     // workgroups fill every CU, so the sweeps beside it only waited for them
     // (the 256^3 gas step measured the same either way).  Recording (the
     // ensemble) keeps one stream.
-    const bool fork = stars && p->opt[QJ_OPT_SIDE_STREAM] && p->side && !g_recorder;
+    const bool fork = forks;
     auto side_work = [&](hipStream_t s) -> int {
       // stars: owed closing kick of the previous step and this step's opening
       // half kick (same grid, same positions), drift, deposit at the new positions
@@ -895,7 +906,7 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
       sd = StarDeposit{sb.offset, sb.sorted, sb.desc, sb.wts, fd_opt == 1 ? st->rho_s : nullptr,
                        sb.tg.ly, sb.tg.lz, sb.tg.ntiles};
     if (fork) QJ_CHECK(hipStreamWaitEvent(s, p->ev_join, 0), "join");
-    if (p->opt[QJ_OPT_FUSE_X_SOURCE]) {
+    if (fuse_x) {
       QJ_STAGE(kStageDriftXSource,
                launch_source_x<T>(p->d, tb, 1, st->psi, src, clear, brho, pr->b_coeff,
                                   pr->rho_bar, w.H, p->partials, st->diag, s, psi_out,

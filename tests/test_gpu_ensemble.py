@@ -153,7 +153,7 @@ def test_autotune_leaves_state_and_results_unchanged(qj, cuda):
         s.step(2)
     res = ens.autotune(reps=2)
     assert res["mode"] in ("batched", "graph", "graph_fork")
-    assert set(res["fuse_x_source"].values()) <= {0, 1}
+    assert set(res["fuse_x_source"].values()) <= {0, 1, 2}
     compare(specs, mem, ref, 2)
     # the members continue under the tuned schedule options, the references
     # under the same ones (the x schedules agree bitwise in complex64 only)
