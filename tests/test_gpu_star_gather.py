@@ -261,3 +261,29 @@ def test_fused_deposit_on_tile_edges(qj, cuda, dtype, rows):
     # (2: qj_star_density, the fixed-point deposit, rounds each value twice)
     for o in out:
         assert rel(o, ref) < (1e-7 if dtype == C64 else 1e-15)
+
+
+@pytest.mark.parametrize("dtype", [C64, C128])
+def test_reading_rho_s_between_steps_changes_nothing(qj, cuda, dtype):
+    # qj_star_density (the density of a state whose fused-deposit steps did
+    # not write it) registers the stars again in the star scratch; the
+    # gather's corner values there survive it, and the next steps register
+    # their own, so a run that reads rho_s between its steps is the run that
+    # does not (to the rounding by which any two runs differ: the deposit's
+    # float atomics add in the order the counting atomics gave)
+    g = O.Grid(64, 32, 32, 1.0, 0.5, 0.5)
+    runs = []
+    for read in (True, False):
+        sim, _ = _sim(qj, g, dtype, 5000, 1)
+        for _ in range(3):
+            sim.step(2)
+            assert sim._st.grid_valid == 2
+            if read:
+                assert sim._st.rho_valid == 0
+                r = sim.rho_s
+                assert sim._st.rho_valid == 1
+                assert torch.isfinite(r).all()
+        st = sim.state()
+        runs.append((st["psi"].clone(), st["pos"].clone(), st["vel"].clone()))
+    for a, b in zip(*runs):
+        assert rel(a, b) < TOLS[dtype] / 10
