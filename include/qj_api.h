@@ -353,9 +353,9 @@ int qj_deposit_adjoint(qj_plan* plan, const double* pos, const double* mass, dou
  * workgroups; 0 = two pencils per lane in 256 threads.  Same results. */
 #define QJ_OPT_WIDE_SWEEPS 3
 /* QJ_OPT_Z_SWEEP_BLOCKS: workgroups of the z drift sweeps, which walk the
- * pencil groups (0 = default: at most 512, or one group per workgroup when
- * the side stream runs star work beside the sweep; otherwise at most this
- * many).  Same results. */
+ * pencil groups (0 = default: at most 512, or at most 1024 when the side
+ * stream runs star work beside the sweep; otherwise at most this many).
+ * Same results. */
 #define QJ_OPT_Z_SWEEP_BLOCKS 4
 /* QJ_OPT_SIDE_STREAM (default 1): the stepper runs each step's star kick,
  * drift and deposit on a second stream of the plan, beside the psi drift

@@ -873,13 +873,14 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
       return QJ_OK;
     };
     // psi: kick folded into the z drift sweep, then y and x (+ Poisson source)
-    // With the star work beside it, the z sweep runs one pencil group per
-    // workgroup instead of walking (two resident workgroups per CU for the
-    // whole sweep): the star kernels then take CUs between its workgroups
-    // (256^3 + 2^18 stars: 498 -> 487 us/step; alone the two forms take the
-    // same time)
+    // With the star work beside it, the z sweep runs 1024 walking workgroups
+    // (four pencil groups each at 256^3) instead of 512: the star kernels
+    // then take CUs between its workgroups.  256^3 + 2^18 stars with the
+    // fused deposit, us/step: one group per workgroup 469.3, 512 walking
+    // 467-475, 768 469.5, 1024 464.0, 1536 462.5, 2048 474.8; with 2^16
+    // stars 1024 and one group per workgroup take the same time
     SpecTables<T> tbz = tb;
-    if (fork && tbz.zblocks == 0) tbz.zblocks = 1 << 30;
+    if (fork && tbz.zblocks == 0) tbz.zblocks = 1024;
     auto drift_zy = [&](hipStream_t s) -> int {
       void* mid = (psi_out && psi_mid != st->psi) ? psi_mid : nullptr;  // z sweep out of place
       QJ_STAGE(kStageDriftZKick,
