@@ -153,8 +153,9 @@ struct qj_plan {
   // per-workgroup CFL signal maxima of the folded Poisson x pass
   double* sig_part = nullptr;
   int nblk_sig = 0;
-  // schedule options (qj_plan_set_option): fused x, binned, deterministic,
-  // wide Poisson sweeps, z-sweep blocks, side stream
+  // schedule options (qj_plan_set_option): x schedule (automatic), binned,
+  // deterministic, wide Poisson sweeps, z-sweep blocks, side stream, star
+  // gather, gather rows (automatic), fused deposit (without the rho_s write)
   int opt[QJ_OPT_COUNT] = {2, 1, 0, 1, 0, 1, 1, 0, 2};
   // second stream (default priority) for the star and gas work of a step,
   // with the fork and join events

@@ -1005,9 +1005,9 @@ hipError_t launch_tile_deposit(const Dims& d, const double* pos, const double* m
                                double m_scalar, long long np, const StarBins& sb, void* rho,
                                hipStream_t s, int det) {
   const TileGeom tg = sb.tg;
-  launch<k_bin_scan>(1, 1024, s, sb.count, sb.offset, tg.ntiles);
   // the fused deposit reads the corner weights of the registrations
   if (!rho && (!sb.desc || !sb.wts || tg.ntx != 1)) return hipErrorInvalidValue;
+  launch<k_bin_scan>(1, 1024, s, sb.count, sb.offset, tg.ntiles);
   if (np > 0)
     launch<k_bin_scatter<T>>(pblocks(np), 256, s, make_geom(d), tg, pos, np, (const int*)sb.rank,
                              (const int*)sb.offset, sb.sorted, sb.desc,

@@ -9,8 +9,10 @@ launch per stage (blockIdx.z = member), and the step is replayed from a
 hipGraph.  Results are bitwise identical to stepping each member alone.
 
 ``autotune`` times the schedule candidates on the live members (execution
-mode, and per shape group the x-sweep schedule and, with stars, the
-tile-binned or float-atomic deposit) and keeps the fastest;
+mode, and per shape group the x-sweep schedule, the Poisson sweep form,
+the z sweeps' workgroups and, with stars, the deposit form, the fused star
+gather, its column tiles' rows and the fused deposit) and keeps the
+fastest;
 member states are saved before and restored after, so tuning does not
 advance the simulation.
 """
@@ -198,15 +200,15 @@ class Ensemble:
 
         Coordinate search: the mode is chosen with every group on its current
         options, then each group's options are set in turn to each of their
-        candidate values and kept when the ensemble step gets faster: the
-        x-sweep schedule, for star groups the tile-binned vs float-atomic
-        deposit, for complex64 256-point groups the wide or narrow Poisson
-        sweep form, and the number of walking workgroups of the z sweeps.
-        Results are cached per ensemble signature in this process (and in
-        cache_path, a JSON file, when given).  Returns {"mode",
-        "fuse_x_source": {group: value}, "binned_deposit": {...},
-        "wide_sweeps": {...}, "z_sweep_blocks": {...}, "ms_per_step",
-        "trials"}.
+        candidate values and kept when the ensemble step gets faster (the
+        options of KNOBS: the x-sweep schedule; for star groups the
+        tile-binned vs float-atomic deposit, the fused star gather, its
+        column tiles' rows and the fused deposit; for complex64 256-point
+        groups the wide or narrow Poisson sweep form; the number of walking
+        workgroups of the z sweeps).  Results are cached per ensemble
+        signature in this process (and in cache_path, a JSON file, when
+        given).  Returns {"mode", one {group: value} dict per knob name
+        ("fuse_x_source", "binned_deposit", ...), "ms_per_step", "trials"}.
         """
         sig = self.signature()
         cache = dict(_TUNED)
