@@ -668,30 +668,41 @@ int gather_rows(const qj_plan* p, const qj_state* st, int cols) {
 }
 
 // V and A3 of the current state (star deposit, source pass, Poisson chain);
-// cols > 0: the fused star gather (V and the stars' corner values).
+// cols > 0: the fused star gather (V and the stars' corner values);
+// fdep_mode > 0 (with cols): the deposit fused into the source pass on the
+// one-row registration, as the steps run it (2: rho_s not written).
 template <class T>
 int initial_grid_t(qj_plan* p, qj_state* st, const qj_params* pr, const Work& w, hipStream_t s,
-                   int cols = 0) {
+                   int cols = 0, int fdep_mode = 0) {
   SpecTables<T> tb = tables<T>(p);
   const bool stars = st->np > 0;
   const bool gas = st->gas[0] != nullptr;
   void* A3 = (stars || gas) ? plane<T>(st->grid, p, 1) : nullptr;
   void* brho = (pr->b_coeff != 0.0) ? st->brho : nullptr;
   const bool binned = st->star_scratch && p->opt[QJ_OPT_BINNED_DEPOSIT];
+  const bool fused = stars && binned && cols && fdep_mode;
   StarGather ga{};
+  StarDeposit sd{};
   if (stars && binned) {
-    // tile-binned deposit of the current positions (overwrites rho_s)
+    // tile-binned deposit of the current positions (overwrites rho_s), or
+    // with the fused deposit the registrations only
     StarBins sb = carve_star_scratch(p->d, st->np, st->star_scratch, sizeof(T), cols,
                                      cols ? gather_rows<T>(p, st, cols) : 1);
-    if (cols) ga = StarGather{sb.offset, sb.sorted, sb.desc, sb.cv, sb.tg.lz};
+    sb.home = fused ? 1 : 0;
+    if (cols) ga = StarGather{sb.offset, sb.sorted, sb.desc, sb.cv, sb.tg.lz, sb.home};
+    if (fused)
+      sd = StarDeposit{sb.offset, sb.sorted, sb.desc, sb.wts,
+                       fdep_mode == 1 ? st->rho_s : nullptr, sb.tg.ly, sb.tg.lz, sb.tg.ntiles,
+                       sb.home};
     QJ_CHECK(fill<int>(sb.count, (long long)sb.ntiles * kCountStride, 0, s), "bin reset");
     QJ_STAGE(kStageDeposit,
              launch_star_step_binned<T>(p->d, st->pos, st->vel, st->np, nullptr, 0.0, 0.0, 0.0, 0,
                                         sb, s),
              "bin");
     QJ_STAGE(kStageDeposit,
-             launch_tile_deposit<T>(p->d, st->pos, st->mass, st->m_scalar, st->np, sb, st->rho_s,
-                                    s, p->opt[QJ_OPT_DETERMINISTIC_DEPOSIT]),
+             launch_tile_deposit<T>(p->d, st->pos, st->mass, st->m_scalar, st->np, sb,
+                                    fused ? nullptr : st->rho_s, s,
+                                    p->opt[QJ_OPT_DETERMINISTIC_DEPOSIT]),
              "deposit");
   } else if (stars) {
     // the float-atomic deposit accumulates into a cleared plane
@@ -701,12 +712,13 @@ int initial_grid_t(qj_plan* p, qj_state* st, const qj_params* pr, const Work& w,
              "deposit");
   }
   const int clear = 0;  // both deposits leave rho_s as the source of this state
-  if (stars) st->rho_valid = 1;
-  void* src = stars ? st->rho_s
+  if (stars) st->rho_valid = (fused && fdep_mode == 2) ? 0 : 1;
+  void* src = stars ? (fused ? nullptr : st->rho_s)
                     : (gas ? (st->gas_swapped ? st->gas_tmp[0] : st->gas[0]) : nullptr);
   QJ_STAGE(kStageSourceX,
            launch_source_x<T>(p->d, tb, 0, st->psi, src, clear, brho, pr->b_coeff,
-                              pr->rho_bar, w.H, p->partials, st->diag, s),
+                              pr->rho_bar, w.H, p->partials, st->diag, s, nullptr,
+                              fused ? &sd : nullptr),
            "source_x");
   if (int rc = poisson_tail<T>(p, w, (stars || gas) ? 1 : 0, pr->four_pi_G, pr->rho_bar,
                                pr->dc_mode, st->grid, A3, nullptr, st->diag, s, 0.0, nullptr,
@@ -807,15 +819,18 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
 
   for (int step = 0; step < nsteps; ++step) {
     if (!st->grid_valid) {
-      if (int rc = initial_grid_t<T>(p, st, pr, w, s, cols)) return rc;
+      if (int rc = initial_grid_t<T>(p, st, pr, w, s, cols, fdep ? fd_opt : 0)) return rc;
     }
     // the star registrations of this step (column tiles with the fused
     // gather); the kicks read the corner values when the previous Poisson
     // x pass left them instead of the grid
     StarBins sb{};
-    if (stars && binned)
+    if (stars && binned) {
       sb = carve_star_scratch(p->d, st->np, st->star_scratch, sizeof(T), cols,
                               cols ? gather_rows<T>(p, st, cols) : 1);
+      // the fused deposit registers each star in its lower y row only
+      sb.home = fdep ? 1 : 0;
+    }
     const void* cv_in = (st->grid_valid == 2) ? sb.cv : nullptr;
     // per-step kick and drift times (a schedule, e.g. the comoving factors of
     // quantum-cosmo.py:170-175), else dt for both
@@ -906,7 +921,7 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
     StarDeposit sd{};
     if (fdep)
       sd = StarDeposit{sb.offset, sb.sorted, sb.desc, sb.wts, fd_opt == 1 ? st->rho_s : nullptr,
-                       sb.tg.ly, sb.tg.lz, sb.tg.ntiles};
+                       sb.tg.ly, sb.tg.lz, sb.tg.ntiles, sb.home};
     if (fork) QJ_CHECK(hipStreamWaitEvent(s, p->ev_join, 0), "join");
     if (fuse_x) {
       QJ_STAGE(kStageDriftXSource,
@@ -934,7 +949,7 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
       sig->cap = p->nblk_sig;
     }
     StarGather ga{};
-    if (cols) ga = StarGather{sb.offset, sb.sorted, sb.desc, sb.cv, sb.tg.lz};
+    if (cols) ga = StarGather{sb.offset, sb.sorted, sb.desc, sb.cv, sb.tg.lz, sb.home};
     if (int rc = poisson_tail<T>(p, w, grad, pr->four_pi_G, pr->rho_bar, pr->dc_mode, Vn, A3n,
                                  nullptr, st->diag, s, 0.0, sig, cols ? &ga : nullptr))
       return rc;

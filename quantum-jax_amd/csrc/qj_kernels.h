@@ -69,6 +69,7 @@ struct StarDeposit {
   void* rho;                       // the density is written here, or nowhere (null)
   int ly, lz;  // the column tiles' TileGeom::ly and ::lz
   int ntiles;
+  int home;    // StarBins::home
 };
 
 template <class T>
@@ -125,7 +126,8 @@ struct StarGather {
   const int* sorted;
   const unsigned long long* desc;  // corner descriptors of the registrations
   void* cv;
-  int lz;  // log2 of the column tiles' z extent (their TileGeom::lz)
+  int lz;    // log2 of the column tiles' z extent (their TileGeom::lz)
+  int home;  // StarBins::home: lists of the lower y row, absolute descriptors
 };
 
 // Writes V and, with grad, the cell-interleaved grid A3 = (a_x, a_y, a_z)
@@ -268,6 +270,12 @@ struct StarBins {
   // star, corner (sx, sy, sz) at 4 sy + 2 sz + sx), written by the scatter
   // for the deposit fused into the source pass
   void* wts;
+  // column tiles, set by the caller: 1 = each star registers only in the
+  // tiles of its lower y row (one per z tile of its z pair), with absolute
+  // corner descriptors (home_desc); a tile's consumers read its own list and
+  // the list of the tile row below (the fused gather and deposit).  0 =
+  // every tile its stencil touches, descriptors relative to the tile.
+  int home;
 };
 TileGeom make_tiles(const Dims& d);
 // Column tiles of the fused star gather: a tile is the set of x pencils that

@@ -197,6 +197,26 @@ __device__ __forceinline__ unsigned stencil_tiles(const TileGeom& t, const Cic& 
   return mask;
 }
 
+// Column tiles in the one-row registration (StarBins::home): the tiles of
+// the star's lower y row, one per z tile its z pair touches (slot 0: the z
+// tile of iz0; slot 1: that of iz1 when it differs).
+__device__ __forceinline__ unsigned home_tiles(const TileGeom& t, const Cic& c, int ids[2]) {
+  const int ty = c.iy[0] >> t.ly;
+  const int z0 = c.iz[0] >> t.lz, z1 = c.iz[1] >> t.lz;
+  ids[0] = ty * t.ntz + z0;
+  ids[1] = ty * t.ntz + z1;
+  return z1 != z0 ? 3u : 1u;
+}
+
+// Absolute corner descriptor of the one-row registration: the star's two
+// cell indices per axis, 9 bits each (x0, x1, y0, y1, z0, z1 from bit 0);
+// a consumer tests each (y, z) corner pair against its own tile.
+__device__ __forceinline__ unsigned long long home_desc(const Cic& c) {
+  return (unsigned long long)c.ix[0] | ((unsigned long long)c.ix[1] << 9) |
+         ((unsigned long long)c.iy[0] << 18) | ((unsigned long long)c.iy[1] << 27) |
+         ((unsigned long long)c.iz[0] << 36) | ((unsigned long long)c.iz[1] << 45);
+}
+
 // Registers the star p in the tiles of its stencil: one counting atomic per
 // used slot, all issued before the returned ranks are stored.
 __device__ __forceinline__ void register_tiles(const TileGeom& tg, const Cic& c, long long p,
@@ -326,7 +346,8 @@ struct k_star_step_bin {
                                              double* __restrict__ vel, long long np,
                                              const T* __restrict__ A3, double h_owed, double h1,
                                              double dt, int* __restrict__ count,
-                                             int* __restrict__ rank, const T* __restrict__ cv) {
+                                             int* __restrict__ rank, const T* __restrict__ cv,
+                                             int home) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     double x[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
@@ -360,7 +381,18 @@ struct k_star_step_bin {
     }
     Cic c = cic(g, x[0], x[1], x[2]);
     if (!c.ok) return;  // non-finite positions deposit nothing
-    register_tiles(tg, c, p, count, rank);
+    if (home) {
+      int ids[2], r[2];
+      const unsigned m = home_tiles(tg, c, ids);
+  #pragma unroll
+      for (int k = 0; k < 2; ++k)
+        if (m >> k & 1u) r[k] = atomicAdd(&count[ids[k] * kCountStride], 1);
+  #pragma unroll
+      for (int k = 0; k < 2; ++k)
+        if (m >> k & 1u) rank[8 * p + k] = r[k];
+    } else {
+      register_tiles(tg, c, p, count, rank);
+    }
   }
 };
 
@@ -436,23 +468,38 @@ struct k_bin_scatter {
                                              int* __restrict__ sorted,
                                              unsigned long long* __restrict__ desc,
                                              T* __restrict__ wts, const double* __restrict__ mass,
-                                             double m_scalar) {
+                                             double m_scalar, int home) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     Cic c = cic(g, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]);
     if (!c.ok) return;
-    int ids[8], o[8];
-    const unsigned m = stencil_tiles(tg, c, ids);
+    if (home) {
+      int ids[2], o[2];
+      const unsigned m = home_tiles(tg, c, ids);
+      const unsigned long long d = home_desc(c);
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if (m >> k & 1u) o[k] = offset[ids[k]] + rank[8 * p + k];
+      for (int k = 0; k < 2; ++k)
+        if (m >> k & 1u) o[k] = offset[ids[k]] + rank[8 * p + k];
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if (m >> k & 1u) sorted[o[k]] = (int)p;
-    if (desc) {
+      for (int k = 0; k < 2; ++k)
+        if (m >> k & 1u) {
+          sorted[o[k]] = (int)p;
+          desc[o[k]] = d;
+        }
+    } else {
+      int ids[8], o[8];
+      const unsigned m = stencil_tiles(tg, c, ids);
 #pragma unroll
       for (int k = 0; k < 8; ++k)
-        if (m >> k & 1u) desc[o[k]] = corner_desc(g, tg, c, ids[k]);
+        if (m >> k & 1u) o[k] = offset[ids[k]] + rank[8 * p + k];
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (m >> k & 1u) sorted[o[k]] = (int)p;
+      if (desc) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (m >> k & 1u) desc[o[k]] = corner_desc(g, tg, c, ids[k]);
+      }
     }
     if (wts) {
       const double fac = (mass ? mass[p] : m_scalar) / g.vol;
@@ -980,6 +1027,7 @@ StarBins carve_star_scratch(const Dims& d, long long np, void* scratch, int real
                    : nullptr;
   sb.tg = cols ? make_column_tiles(d, cols, rows) : make_tiles(d);
   sb.ntiles = sb.tg.ntiles;
+  sb.home = 0;
   return sb;
 }
 
@@ -989,14 +1037,15 @@ hipError_t launch_star_step_binned(const Dims& d, double* pos, double* vel, long
                                    const StarBins& sb, hipStream_t s, const void* cv) {
   if (np <= 0) return hipSuccess;
   const TileGeom tg = sb.tg;
+  if (sb.home && (tg.ntx != 1 || !sb.desc)) return hipErrorInvalidValue;  // column tiles only
   if (move)
     launch<k_star_step_bin<T, true>>(pblocks(np), 256, s, make_geom(d), tg, pos, vel, np,
                                      (const T*)A3, h_owed, h1, dt, sb.count, sb.rank,
-                                     (const T*)cv);
+                                     (const T*)cv, sb.home);
   else
     launch<k_star_step_bin<T, false>>(pblocks(np), 256, s, make_geom(d), tg, pos, vel, np,
                                       (const T*)A3, h_owed, h1, dt, sb.count, sb.rank,
-                                      (const T*)nullptr);
+                                      (const T*)nullptr, sb.home);
   return hipGetLastError();
 }
 
@@ -1005,13 +1054,16 @@ hipError_t launch_tile_deposit(const Dims& d, const double* pos, const double* m
                                double m_scalar, long long np, const StarBins& sb, void* rho,
                                hipStream_t s, int det) {
   const TileGeom tg = sb.tg;
-  // the fused deposit reads the corner weights of the registrations
+  // the fused deposit reads the corner weights of the registrations; the
+  // per-tile deposit kernels read lists of every tile a star touches, not
+  // the one-row lists
   if (!rho && (!sb.desc || !sb.wts || tg.ntx != 1)) return hipErrorInvalidValue;
+  if (sb.home && rho) return hipErrorInvalidValue;
   launch<k_bin_scan>(1, 1024, s, sb.count, sb.offset, tg.ntiles);
   if (np > 0)
     launch<k_bin_scatter<T>>(pblocks(np), 256, s, make_geom(d), tg, pos, np, (const int*)sb.rank,
                              (const int*)sb.offset, sb.sorted, sb.desc,
-                             rho ? (T*)nullptr : (T*)sb.wts, mass, m_scalar);
+                             rho ? (T*)nullptr : (T*)sb.wts, mass, m_scalar, sb.home);
   if (!rho) return hipGetLastError();
   if (det)
     QJ_TILE_VOL_COL(tg, launch<k_tile_deposit_det<T, VOL_>>(tg.ntiles, 256, s, make_geom(d), tg,
@@ -1068,7 +1120,7 @@ hipError_t launch_interp_adj_binned(const Dims& d, const double* pos, long long 
       launch<k_bin_scatter<T>>(pblocks(np), 256, s, make_geom(d), tg, pos, np,
                                (const int*)sb.rank, (const int*)sb.offset, sb.sorted,
                                (unsigned long long*)nullptr, (T*)nullptr,
-                               (const double*)nullptr, 0.0);
+                               (const double*)nullptr, 0.0, 0);
   }
   QJ_TILE_VOL(tg, launch<k_tile_deposit_vec<T, VOL_>>(tg.ntiles, 256, s, make_geom(d), tg, pos,
                                                       (const double*)vbar, h,

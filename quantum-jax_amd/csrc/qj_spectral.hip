@@ -381,6 +381,45 @@ __device__ double block_compensated_sum(const double* __restrict__ x, int n) {
 // writes it to rho_s, so the star density is not written by one kernel and
 // read back by the next, and the deposit needs no launch.
 // ---------------------------------------------------------------------------
+// The registration lists a column tile's workgroup serves: its own, and with
+// the one-row registration (each star registered only in the tiles of its
+// lower y row) the list of the tile row below, whose stars' upper y corners
+// lie in this tile.  Entry v of the two is slot slot(v).
+struct TileLists {
+  int a0, n0, a1, n;
+  __device__ __forceinline__ int slot(int v) const { return v < n0 ? a0 + v : a1 + (v - n0); }
+};
+__device__ __forceinline__ TileLists tile_lists(const int* __restrict__ offset, int home,
+                                                int lnz, int lz) {
+  TileLists L;
+  L.a0 = offset[blockIdx.x];
+  L.n0 = offset[blockIdx.x + 1] - L.a0;
+  L.a1 = 0;
+  L.n = L.n0;
+  if (home) {
+    const int nbz = 1 << (lnz - lz);
+    const int nty = (int)gridDim.x / nbz;  // one workgroup per tile
+    if (nty > 1) {
+      const int tb = (((int)blockIdx.x / nbz + nty - 1) % nty) * nbz + (int)blockIdx.x % nbz;
+      L.a1 = offset[tb];
+      L.n += offset[tb + 1] - L.a1;
+    }
+  }
+  return L;
+}
+
+// Column + 1 of the star's (y, z) corner pair yz = 2 sy + sz in this
+// workgroup's tile, or 0 outside it: from a tile-relative descriptor
+// (corner_desc) or, home, an absolute one (home_desc) tested against the
+// tile at rows y0.. (R of them) and columns z0.. (2^lz).
+__device__ __forceinline__ int corner_column(unsigned long long d, int yz, int home, int y0,
+                                             int z0, int R, int lz) {
+  if (!home) return (int)((d >> (18 + 9 * yz)) & 511u);
+  const int dy = (int)((d >> (18 + 9 * (yz >> 1))) & 511u) - y0;
+  const int dz = (int)((d >> (36 + 9 * (yz & 1))) & 511u) - z0;
+  return ((unsigned)dy < (unsigned)R && (unsigned)dz < (1u << lz)) ? ((dy << lz) | dz) + 1 : 0;
+}
+
 template <class T>
 struct DepositArgs {
   const int* offset;
@@ -390,6 +429,7 @@ struct DepositArgs {
   T* rho;                          // rho_s, the tile's star density (or null)
   int lz;                          // log2 of the tile's z columns
   int lnz;                         // log2 nz
+  int home;                        // one-row registration (TileLists)
 };
 
 // the eight corner weights of star i
@@ -437,7 +477,8 @@ struct k_source_x {
     // the fused deposit: the thread's first registration of the tile is
     // loaded before psi and its star's weights after psi's rows, so both
     // latencies hide behind the drift transforms
-    int db0 = 0, db1 = 0, dp0 = 0;
+    TileLists dl{};
+    int dp0 = 0;
     unsigned long long dd = 0;
     T dw[8];
     if constexpr (DEP) {
@@ -450,15 +491,14 @@ struct k_source_x {
       const int z = ((blockIdx.x % nbz) << dp.lz) + (c0 & ((1 << dp.lz) - 1));
       act = (y << dp.lnz) < g.ncols;
       q0 = (y << dp.lnz) + z;
-      db0 = dp.offset[blockIdx.x];
-      db1 = dp.offset[blockIdx.x + 1];
-      // loaded by every thread (from the tile's first slot when the thread
-      // has none, slot 0 of an empty tile): no branch around the loads.  An
-      // empty tile's slot 0 may be stale, so its star is taken to be star 0.
-      const int i0 = db0 + (int)threadIdx.x;
-      const int ic = i0 < db1 ? i0 : (db1 > db0 ? db0 : 0);
+      dl = tile_lists(dp.offset, dp.home, dp.lnz, dp.lz);
+      // loaded by every thread (the lists' first slot when the thread has no
+      // entry, slot 0 when they are empty): no branch around the loads.  An
+      // empty list's slot 0 may be stale, so its star is taken to be star 0.
+      const int v0 = threadIdx.x;
+      const int ic = v0 < dl.n ? dl.slot(v0) : (dl.n > 0 ? dl.slot(0) : 0);
       const int p0 = dp.sorted[ic];
-      dp0 = db1 > db0 ? p0 : 0;
+      dp0 = dl.n > 0 ? p0 : 0;
       dd = dp.desc[ic];
     }
     const int base = act ? (int)pencil_base(g, q0) : 0;
@@ -515,13 +555,17 @@ struct k_source_x {
       __syncthreads();  // the drift's last exchange reads are done
       for (int l = threadIdx.x; l < N * CV; l += kThreads) acc[l] = (T)0;
       __syncthreads();
-      const int i0 = db0 + (int)threadIdx.x;
-      for (int i = i0; i < db1; i += kThreads) {
+      const int v0 = threadIdx.x;
+      const int nbzt = 1 << (dp.lnz - dp.lz);
+      const int Rt = CV >> dp.lz;
+      const int y0t = ((int)blockIdx.x / nbzt) * Rt, z0t = ((int)blockIdx.x % nbzt) << dp.lz;
+      for (int v = v0; v < dl.n; v += kThreads) {
         unsigned long long d = dd;
         T w[8];
   #pragma unroll
         for (int e = 0; e < 8; ++e) w[e] = dw[e];
-        if (i != i0) {
+        if (v != v0) {
+          const int i = dl.slot(v);
           d = dp.desc[i];
           load_wts8(dp.wts, dp.sorted[i], w);
         }
@@ -530,7 +574,7 @@ struct k_source_x {
         const int x0 = (int)(d & 511u) * CV, x1 = (int)((d >> 9) & 511u) * CV;
   #pragma unroll
         for (int yz = 0; yz < 4; ++yz) {
-          const int f = (int)((d >> (18 + 9 * yz)) & 511u);
+          const int f = corner_column(d, yz, dp.home, y0t, z0t, Rt, dp.lz);
           if (f != 0) {
             atomicAdd(&acc[x0 + f - 1], w[2 * yz]);
             atomicAdd(&acc[x1 + f - 1], w[2 * yz + 1]);
@@ -891,9 +935,11 @@ struct GatherArgs {
   const int* sorted;
   const unsigned long long* desc;
   T* cv;
-  int lz;   // log2 of the z columns of a column tile (its y rows: C VW / 2^lz)
-  int lnz;  // log2 nz
+  int lz;    // log2 of the z columns of a column tile (its y rows: C VW / 2^lz)
+  int lnz;   // log2 nz
+  int home;  // one-row registration: own and lower-row lists, absolute descriptors
 };
+
 
 template <class T, int N, bool GRAD, bool WIDE = true, bool SIG = false, bool CVG = false>
 struct k_poisson_x {
@@ -940,13 +986,13 @@ struct k_poisson_x {
     };
     // the fused star gather: the thread's first registration of the tile is
     // loaded before the spectra, so its latency hides behind the transforms
-    int gb0 = 0, gb1 = 0, gp0 = 0;
+    TileLists gl{};
+    int gp0 = 0;
     unsigned long long gd0 = 0;
     if constexpr (CVG) {
-      gb0 = ga.offset[blockIdx.x];
-      gb1 = ga.offset[blockIdx.x + 1];
-      const int i0 = gb0 + (int)threadIdx.x;
-      if (i0 < gb1) {
+      gl = tile_lists(ga.offset, ga.home, ga.lnz, ga.lz);
+      if ((int)threadIdx.x < gl.n) {
+        const int i0 = gl.slot(threadIdx.x);
         gp0 = ga.sorted[i0];
         gd0 = ga.desc[i0];
       }
@@ -1068,9 +1114,14 @@ struct k_poisson_x {
         constexpr bool kOwn = HX * CV * 3 > 2 * S::kLdsStrided * VW;
         __shared__ T own_buf[kOwn ? HX * CV * 3 : 1];
         T* la = kOwn ? own_buf : reinterpret_cast<T*>(lds);
-        const int b1 = gb1, i0 = gb0 + (int)threadIdx.x;
+        const int v0 = threadIdx.x;
         const int p0 = gp0;
         const unsigned long long d0 = gd0;
+        // the tile (rows y0.. of R, columns z0.. of 2^lz) for the absolute
+        // descriptors of the one-row registration
+        const int nbzt = 1 << (ga.lnz - ga.lz);
+        const int Rt = CV >> ga.lz;
+        const int y0t = ((int)blockIdx.x / nbzt) * Rt, z0t = ((int)blockIdx.x % nbzt) << ga.lz;
   #pragma unroll
         for (int h = 0; h < 2; ++h) {
           __syncthreads();  // the last readers of the buffer are done
@@ -1089,15 +1140,16 @@ struct k_poisson_x {
             }
           }
           __syncthreads();
-          for (int i = i0; i < b1; i += kThreads) {
-            const int p = (i == i0) ? p0 : ga.sorted[i];
-            const unsigned long long d = (i == i0) ? d0 : ga.desc[i];
+          for (int v = v0; v < gl.n; v += kThreads) {
+            const int i = gl.slot(v);
+            const int p = (v == v0) ? p0 : ga.sorted[i];
+            const unsigned long long d = (v == v0) ? d0 : ga.desc[i];
             T* out = ga.cv + kCvReals * (long long)p;
   #pragma unroll
             for (int k = 0; k < 8; ++k) {
               const int sx = kCorner[k][0], sy = kCorner[k][1], sz = kCorner[k][2];
               const int xl = (int)((d >> (9 * sx)) & 511u) - h * HX;
-              const int f = (int)((d >> (18 + 9 * (2 * sy + sz))) & 511u);  // column + 1
+              const int f = corner_column(d, 2 * sy + sz, ga.home, y0t, z0t, Rt, ga.lz);
               if ((unsigned)xl < (unsigned)HX && f != 0) {
                 const T* q = la + 3 * (xl * CV + f - 1);
                 out[3 * k] = q[0];
@@ -1550,7 +1602,7 @@ hipError_t launch_source_x(const Dims& d, const SpecTables<T>& tb, int drift, vo
         ((1 << (dep->ly + dep->lz)) != cols && dep->ntiles != 1))
       return hipErrorInvalidValue;
     da = DepositArgs<T>{dep->offset, dep->sorted, dep->desc, (const T*)dep->wts, (T*)dep->rho,
-                        dep->lz, lnz};
+                        dep->lz, lnz, dep->home};
   }
   QJ_DISPATCH_N(d.nx, {
     if (drift && dep)
@@ -1653,7 +1705,7 @@ hipError_t launch_poisson_x(const Dims& d, const SpecTables<T>& tb, int grad, co
     while ((1 << lnz) < d.nz) ++lnz;
     if (gather->lz < 0 || gather->lz > lnz) return hipErrorInvalidValue;
     ga = GatherArgs<T>{gather->offset, gather->sorted, gather->desc, (T*)gather->cv, gather->lz,
-                       lnz};
+                       lnz, gather->home};
   }
   QJ_DISPATCH_N(d.nx, {
     using Cfg = StridedCfg<T, N_, true>;
