@@ -77,11 +77,14 @@ def stage_bytes(name, nx, ny, nz, c, r, n_p, stars, gather=True, fdep=True, rho_
     star's eight corner accelerations (24 r B), reading its two column-tile
     registrations (index and corner descriptor, 12 B each); the deposit's
     scatter writes those descriptors (about 112 B per star in all).  With
-    the fused deposit as well (the default) the source pass reads the two
-    registrations and the star's eight corner weights (24 + 8 r B per star)
-    and writes rho_s (N r) only with rho_write (QJ_OPT_FUSED_DEPOSIT 1), and
-    the deposit stage is the scan and the scatter (positions, ranks,
-    registrations, descriptors and weights: 56 + 8 r B per star)."""
+    the fused deposit as well (the default) each star has one registration
+    in the tile of its lower y row (two where its z pair crosses a tile
+    edge), which the tile and the tile above both read: the source pass
+    reads it twice and the star's eight corner weights (24 + 8 r B per
+    star), and the Poisson x pass reads it twice as well; the source pass
+    writes rho_s (N r) only with rho_write (QJ_OPT_FUSED_DEPOSIT 1), and the
+    deposit stage is the scan and the scatter (position, rank, registration,
+    descriptor and weights: 40 + 8 r B per star)."""
     N = nx * ny * nz
     Nh = (nx // 2 + 1) * ny * nz
     src = r
@@ -99,7 +102,7 @@ def stage_bytes(name, nx, ny, nz, c, r, n_p, stars, gather=True, fdep=True, rho_
         "poisson_y": 5 * Nh * c,
         # V plane + (a_x, a_y, a_z) cells, or V + the stars' corner values
         "poisson_x": 3 * Nh * c + (N * r + n_p * (24 * r + 24) if fused else 4 * N * r),
-        "deposit": n_p * (56 + 8 * r) if fd else N * r + n_p * (112 if fused else 64),
+        "deposit": n_p * (40 + 8 * r) if fd else N * r + n_p * (112 if fused else 64),
         "star_step": n_p * (96 + 8 * 3 * r + 8),
         "star_kick": n_p * (72 + 8 * 3 * r),
         "kick": N * (2 * c + r),

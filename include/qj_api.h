@@ -384,8 +384,10 @@ int qj_deposit_adjoint(qj_plan* plan, const double* pos, const double* mass, dou
  * added to |psi|^2); 2 = rho_s is then not written (qj_state.rho_valid 0,
  * qj_star_density computes it when a caller needs it), 1 = the pass also
  * writes the tile's density to rho_s, 0 = the separate per-tile deposit
- * kernel writes rho_s and the source pass reads it.  Same results to
- * rounding (float atomics add in either order). */
+ * kernel writes rho_s and the source pass reads it.  With 1 and 2 each star
+ * registers only in the column tile of its lower y row, and the source
+ * pass and the gather also read the list of the tile row below.  Same
+ * results to rounding (float atomics add in either order). */
 #define QJ_OPT_FUSED_DEPOSIT 8
 #define QJ_OPT_COUNT 9
 int qj_plan_set_option(qj_plan* plan, int option, int value);
