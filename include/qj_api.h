@@ -354,7 +354,8 @@ int qj_deposit_adjoint(qj_plan* plan, const double* pos, const double* mass, dou
 #define QJ_OPT_WIDE_SWEEPS 3
 /* QJ_OPT_Z_SWEEP_BLOCKS: workgroups of the z drift sweeps, which walk the
  * pencil groups (0 = default: at most 512, or at most 1024 when the side
- * stream runs star work beside the sweep; otherwise at most this many).
+ * stream runs star work beside the sweep on a grid of fewer than 256^3
+ * cells; otherwise at most this many).
  * Same results. */
 #define QJ_OPT_Z_SWEEP_BLOCKS 4
 /* QJ_OPT_SIDE_STREAM (default 1): the stepper runs each step's star kick,
@@ -375,7 +376,8 @@ int qj_deposit_adjoint(qj_plan* plan, const double* pos, const double* mass, dou
 /* QJ_OPT_GATHER_ROWS: y rows of the fused gather's column tiles (0 =
  * automatic, 1 or 2).  Two rows cut the stars' registrations by a quarter;
  * on grids where they leave the Poisson x pass half-line rows they pay only
- * with many stars per cell (automatic: at least one star per 32 cells).
+ * with many stars per cell (automatic: at least one star per 32 cells, and
+ * not with the fused deposit, whose stars register in one row anyway).
  * Same results to rounding. */
 #define QJ_OPT_GATHER_ROWS 7
 /* QJ_OPT_FUSED_DEPOSIT (default 2): where the fused gather's column tiles

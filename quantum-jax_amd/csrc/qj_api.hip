@@ -657,13 +657,18 @@ int gather_cols(const qj_plan* p, const qj_state* st, bool tape) {
 // 123.9 us/step with two rows; 256^3 + 2^18 (one per 64 cells) 482.6 ->
 // 487.0 us/step, and the 48-box ensemble with 2^14 stars per box 5.08 ->
 // 5.33 ms/step.  Automatic: two rows where they keep whole lines or with at
-// least one star per 32 cells.
+// least one star per 32 cells.  With the fused deposit (fdep) a star
+// registers in one row whatever the tile height, so two rows save nothing
+// there and the automatic choice keeps one unless two keep whole lines
+// (128^3 + 2^16 stars 117.1 -> 116.1 us/step with one row, 64^3 + 2^14
+// 86.1 -> 85.1).
 template <class T>
-int gather_rows(const qj_plan* p, const qj_state* st, int cols) {
+int gather_rows(const qj_plan* p, const qj_state* st, int cols, bool fdep) {
   const int o = p->opt[QJ_OPT_GATHER_ROWS];
   if (o == 1 || o == 2) return o;
   const int full = 128 / (int)sizeof(T);  // reals per 128-byte row
   if (cols / 2 >= full) return 2;
+  if (fdep) return 1;
   return (st->np * 32 >= p->ncell()) ? 2 : 1;
 }
 
@@ -687,7 +692,7 @@ int initial_grid_t(qj_plan* p, qj_state* st, const qj_params* pr, const Work& w,
     // tile-binned deposit of the current positions (overwrites rho_s), or
     // with the fused deposit the registrations only
     StarBins sb = carve_star_scratch(p->d, st->np, st->star_scratch, sizeof(T), cols,
-                                     cols ? gather_rows<T>(p, st, cols) : 1);
+                                     cols ? gather_rows<T>(p, st, cols, fused) : 1);
     sb.home = fused ? 1 : 0;
     if (cols) ga = StarGather{sb.offset, sb.sorted, sb.desc, sb.cv, sb.tg.lz, sb.home};
     if (fused)
@@ -827,7 +832,7 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
     StarBins sb{};
     if (stars && binned) {
       sb = carve_star_scratch(p->d, st->np, st->star_scratch, sizeof(T), cols,
-                              cols ? gather_rows<T>(p, st, cols) : 1);
+                              cols ? gather_rows<T>(p, st, cols, fdep) : 1);
       // the fused deposit registers each star in its lower y row only
       sb.home = fdep ? 1 : 0;
     }
@@ -894,9 +899,12 @@ int kdk_steps_t(qj_plan* p, qj_state* st, const qj_params* pr, int nsteps, void*
     // then take CUs between its workgroups.  256^3 + 2^18 stars with the
     // fused deposit, us/step: one group per workgroup 469.3, 512 walking
     // 467-475, 768 469.5, 1024 464.0, 1536 462.5, 2048 474.8; with 2^16
-    // stars 1024 and one group per workgroup take the same time
+    // stars 1024 and one group per workgroup take the same time.  Since the
+    // one-row registration the star work is smaller and 512 are ahead again
+    // at 256^3 (454.7 against 461.2 us/step, interleaved runs), while 1024
+    // stay ahead below (128^3 + 2^16: 115.5 against 116.1)
     SpecTables<T> tbz = tb;
-    if (fork && tbz.zblocks == 0) tbz.zblocks = 1024;
+    if (fork && tbz.zblocks == 0 && p->ncell() < (1LL << 24)) tbz.zblocks = 1024;
     auto drift_zy = [&](hipStream_t s) -> int {
       void* mid = (psi_out && psi_mid != st->psi) ? psi_mid : nullptr;  // z sweep out of place
       QJ_STAGE(kStageDriftZKick,
